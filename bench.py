@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark: plain FP64 CG (TeaLeaf) at 4096^2 on one MI355X.
+
+Workload (one "step" = one TeaLeaf timestep, i.e. one full CG solve): the
+4096 x 4096 deck with K = 1/density, the tea.in rectangles plus 16 seeded
+random rectangles, eps 1e-15 absolute, max_iters 10000, dt 0.004.  W warm-up
+timesteps run first; the initial state is then restored and K timesteps are
+timed (device-synchronised on both sides) by bin/tealeaf itself.
+
+Printed (one JSON line): effective HBM throughput of the timed solve,
+  value = 88 B x N^2 x (CG iterations executed) / t_solve,
+88 B being the algorithmic FP64 traffic per cell per iteration (11 words),
+plus time-to-solution, iterations/s, cell-updates/s, the roofline of the
+dominant kernel (cg_fused_w: 8 words = 64 B per cell per launch, timed with
+HIP events on the library's stream) and a CPU baseline (the OpenMP CPU
+restatement of the reference kernels, unfused 13-word dataflow, timed on a
+bounded sample of the same problem on this host).
+"""
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = ("time-to-solution @4096², 1e-10 rel. residual: learned FCG vs plain CG; "
+          "HBM GB/s")
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_ITER = 88      # fused FP64 iteration, algorithmic
+BYTES_FUSED_W = 64            # cg_fused_w: p_old, r, u, kx, ky read; p, u, w written
+BYTES_FUSED_R = 24            # cg_fused_r: r, w read; r written
+PROFILE_TRAFFIC = os.path.join(REPO, "profiles", "cg_fused_w_4096_traffic.json")
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed timesteps")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed timesteps")
+    ap.add_argument("--cells", type=int, default=4096, help="grid is cells x cells")
+    ap.add_argument("--max-iters", type=int, default=10000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU-baseline sample length (0 disables it)")
+    ap.add_argument("--unfused", action="store_true",
+                    help="one launch per reference kernel (13 words/iteration)")
+    return ap.parse_args()
+
+
+def cpu_baseline(cells, seconds):
+    """OpenMP CPU restatement of the reference kernels, unfused dataflow
+    (calc_w, calc_ur, calc_p, U/P halos), on the same deck; times as many CG
+    iterations of the first timestep as fit in `seconds`."""
+    from exploringsycl_amd.deck import config_deck
+    from tests.oracle_ctypes import Oracle, OracleSolver
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    o = Oracle()
+    deck = config_deck(2, x_cells=cells)
+    s = OracleSolver(o, cells, cells, deck.states, coefficient=2)
+    x, y, h, f = s.x, s.y, s.h, s.f
+    rro = o.cg_init(x, y, h, 2, s.rx, s.ry, f)
+    for n in ("u", "p"):
+        o.halo_field(x, y, h, 1, f[n])
+    o.copy_u(x, y, h, f)
+    iters = 0
+    t0 = time.perf_counter()
+    while True:
+        pw = o.cg_calc_w(x, y, h, f)
+        alpha = rro / pw
+        rrn = o.cg_calc_ur(x, y, h, alpha, f)
+        o.cg_calc_p(x, y, h, rrn / rro, f)
+        for n in ("u", "p"):
+            o.halo_field(x, y, h, 1, f[n])
+        rro = rrn
+        iters += 1
+        el = time.perf_counter() - t0
+        if el >= seconds and iters >= 3:
+            break
+    gbs = BYTES_PER_CELL_ITER * cells * cells * iters / el / 1e9
+    return {
+        "value": round(gbs, 3), "unit": "GB/s",
+        "cores": threads, "kind": "port",
+        "sample": (f"first {iters} CG iterations of timestep 1 of the same "
+                   f"{cells}^2 deck, {el:.1f} s, OpenMP FP64 oracle "
+                   f"(unfused reference dataflow), same 88 B/cell/iteration "
+                   f"convention; {iters / el:.2f} iterations/s"),
+        "iterations_per_s": iters / el,
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build drives one GPU; multi-GPU runs are not part "
+              "of it (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from exploringsycl_amd.deck import config_deck
+    from exploringsycl_amd.native import TEALEAF_BIN
+    from exploringsycl_amd.runner import run_deck
+
+    n = args.cells
+    deck = config_deck(2, x_cells=n, end_step=args.steps)
+    deck.max_iters = args.max_iters
+    with tempfile.TemporaryDirectory(prefix="tea_bench_") as wd:
+        stats_path = os.path.join(wd, "kernel_stats.json")
+        env = {"TEA_HIP_STATS": stats_path, "TEA_HIP_FUSED": "0" if args.unfused else "1"}
+        t_wall = time.perf_counter()
+        rec, _ = run_deck(deck, binary=TEALEAF_BIN, workdir=wd, env=env,
+                          extra_args=["--warmup-steps", str(args.warmup)],
+                          timeout=3000)
+        t_wall = time.perf_counter() - t_wall
+        with open(stats_path) as fp:
+            stats = json.load(fp)
+
+    iters = sum(s["executed"] for s in rec["steps"])
+    t = rec["timed_seconds"]
+    value = BYTES_PER_CELL_ITER * n * n * iters / t / 1e9
+
+    kname, kbytes = (("cg_calc_w", 32) if args.unfused else ("cg_fused_w", BYTES_FUSED_W))
+    k = stats[kname]
+    avg_ms = k["total_ms"] / k["launches"]
+    achieved = kbytes * n * n / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC) and not args.unfused and n == 4096:
+        with open(PROFILE_TRAFFIC) as fp:
+            traffic = json.load(fp).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes_per_launch": kbytes * n * n,
+                "avg_launch_ms": round(avg_ms, 5)}
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded splitmix64 rectangles, no dataset)",
+        "config": {
+            "workload": (f"plain FP64 CG, {n}x{n} cells, {args.steps} timesteps, "
+                         "K=1/density, tea.in + 16 seeded rectangles, eps 1e-15 "
+                         f"absolute, max_iters {args.max_iters}"),
+            "cells": n, "timesteps": args.steps, "seed": 20263202,
+            "path": "unfused (13 words/iter)" if args.unfused else "fused (11 words/iter)",
+        },
+        "time_to_solution_s": round(t, 4),
+        "cg_iterations": iters,
+        "cg_iterations_per_s": round(iters / t, 2),
+        "cell_updates_per_s": n * n * iters / t,
+        "us_per_iteration": round(1e6 * t / max(iters, 1), 2),
+        "iterations_per_step": [s["executed"] for s in rec["steps"]],
+        "final_temp": rec["summaries"][-1]["temp"],
+        "kernels_ms": {kk: round(v["total_ms"], 3) for kk, v in stats.items()},
+        "roofline": roofline,
+        "wall_s_including_setup": round(t_wall, 2),
+    }
+    if args.cpu_seconds > 0:
+        line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

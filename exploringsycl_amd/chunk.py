@@ -1,0 +1,167 @@
+"""One TeaLeaf chunk on the GPU, driven through libteahip.so.
+
+Method names follow the reference's kernel_interface.h entry points
+(TeaLeaf/kernel_interface.h:13-71) without the ``run_`` prefix; scalar
+results are returned instead of accumulated into out-parameters.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import native
+from .native import FIELD_ID, HALO_SLOTS, check
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class DeviceChunk:
+    """Device state of one chunk of ``x`` by ``y`` cells, halo included."""
+
+    def __init__(self, x, y, halo_depth=2, device=0, fused=True):
+        self.lib = native.load()
+        self.x, self.y, self.halo_depth = x, y, halo_depth
+        h = ctypes.c_void_p()
+        check(self.lib.tea_hip_chunk_create(device, x, y, halo_depth,
+                                            ctypes.byref(h)), "chunk_create")
+        self.handle = h
+        if not fused:
+            self.set_fused(False)
+
+    def close(self):
+        if self.handle:
+            check(self.lib.tea_hip_chunk_destroy(self.handle), "chunk_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- fields ------------------------------------------------------------
+    def field_info(self, name):
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        check(self.lib.tea_hip_field(self.handle, FIELD_ID[name],
+                                     ctypes.byref(p), ctypes.byref(n)), "field")
+        return p.value, n.value
+
+    def read(self, name):
+        _, n = self.field_info(name)
+        out = np.empty(n, dtype=np.float64)
+        check(self.lib.tea_hip_field_read(self.handle, FIELD_ID[name],
+                                          _ptr(out), n), f"read {name}")
+        if n == self.x * self.y:
+            return out.reshape(self.y, self.x)
+        return out
+
+    def write(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        _, n = self.field_info(name)
+        if a.size > n:
+            raise ValueError(f"{name}: {a.size} values for a field of {n}")
+        check(self.lib.tea_hip_field_write(self.handle, FIELD_ID[name],
+                                           _ptr(a), a.size), f"write {name}")
+
+    # ---- kernel_interface entry points ---------------------------------------
+    def set_chunk_data(self, x_min, y_min, dx, dy):
+        check(self.lib.tea_hip_set_chunk_data(self.handle, x_min, y_min, dx, dy),
+              "set_chunk_data")
+
+    def set_chunk_state(self, states):
+        """``states``: list of dicts with density, energy and (for states
+        after the first) x_min, y_min, x_max, y_max, radius, geometry."""
+        n = len(states)
+        cols = {k: np.array([float(s.get(k, 0.0)) for s in states])
+                for k in ("density", "energy", "x_min", "y_min", "x_max",
+                          "y_max", "radius")}
+        geom = np.array([native.GEOMETRY[s.get("geometry", "rectangle")]
+                         for s in states], dtype=np.int32)
+        check(self.lib.tea_hip_set_chunk_state(
+            self.handle, n, *[_ptr(cols[k]) for k in
+                              ("density", "energy", "x_min", "y_min", "x_max",
+                               "y_max", "radius")],
+            geom.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+            "set_chunk_state")
+
+    def store_energy(self):
+        check(self.lib.tea_hip_store_energy(self.handle), "store_energy")
+
+    def local_halos(self, depth, fields, neighbours=(-1, -1, -1, -1)):
+        nb = (ctypes.c_int * 4)(*neighbours)
+        fl = (ctypes.c_int * 6)(*[1 if s in fields else 0 for s in HALO_SLOTS])
+        check(self.lib.tea_hip_local_halos(self.handle, depth, nb, fl),
+              "local_halos")
+
+    def field_summary(self):
+        v = [ctypes.c_double() for _ in range(4)]
+        check(self.lib.tea_hip_field_summary(self.handle,
+                                             *[ctypes.byref(a) for a in v]),
+              "field_summary")
+        return tuple(a.value for a in v)
+
+    def cg_init(self, coefficient, rx, ry):
+        v = ctypes.c_double()
+        check(self.lib.tea_hip_cg_init(self.handle, coefficient, rx, ry,
+                                       ctypes.byref(v)), "cg_init")
+        return v.value
+
+    def cg_calc_w(self):
+        v = ctypes.c_double()
+        check(self.lib.tea_hip_cg_calc_w(self.handle, ctypes.byref(v)), "cg_calc_w")
+        return v.value
+
+    def cg_calc_ur(self, alpha):
+        v = ctypes.c_double()
+        check(self.lib.tea_hip_cg_calc_ur(self.handle, alpha, ctypes.byref(v)),
+              "cg_calc_ur")
+        return v.value
+
+    def cg_calc_p(self, beta):
+        check(self.lib.tea_hip_cg_calc_p(self.handle, beta), "cg_calc_p")
+
+    def copy_u(self):
+        check(self.lib.tea_hip_copy_u(self.handle), "copy_u")
+
+    def calculate_residual(self):
+        check(self.lib.tea_hip_calculate_residual(self.handle), "calculate_residual")
+
+    def calculate_2norm(self, name):
+        p, _ = self.field_info(name)
+        v = ctypes.c_double()
+        check(self.lib.tea_hip_calculate_2norm(self.handle, ctypes.c_void_p(p),
+                                               ctypes.byref(v)), "calculate_2norm")
+        return v.value
+
+    def finalise(self):
+        check(self.lib.tea_hip_finalise(self.handle), "finalise")
+
+    # ---- mode and timing -------------------------------------------------------
+    def set_fused(self, on):
+        check(self.lib.tea_hip_set_fused(self.handle, 1 if on else 0), "set_fused")
+
+    def flush(self):
+        check(self.lib.tea_hip_flush(self.handle), "flush")
+
+    def timing(self, on=True):
+        check(self.lib.tea_hip_timing_enable(self.handle, 1 if on else 0), "timing")
+
+    def kernel_stats(self):
+        out = {}
+        for k, name in enumerate(native.kernel_names()):
+            n = ctypes.c_long()
+            ms = ctypes.c_double()
+            check(self.lib.tea_hip_kernel_stats(self.handle, k, ctypes.byref(n),
+                                                ctypes.byref(ms)), "kernel_stats")
+            if n.value:
+                out[name] = (n.value, ms.value)
+        return out

@@ -1,0 +1,494 @@
+// libteahip.so: the CG solve (TeaLeaf/c_kernels/sycl/cg.cpp) and the shared
+// solver kernels (TeaLeaf/c_kernels/sycl/solver_methods.cpp).
+//
+// Every FP64 expression below is evaluated in the reference's operand order
+// and the library is compiled with -ffp-contract=off, so each product and sum
+// rounds on its own; together with the reference's reduction tree
+// (span_reduce + final_reduce_kernel) the results are bitwise those of the
+// FP64 CPU restatement.
+
+#include <hip/hip_runtime.h>
+
+#include "tea_hip_impl.h"
+
+namespace tea {
+namespace {
+
+// The 5-point operator of TeaLeaf/shared.h:59-63 at cell i, given the
+// operand at the cell (c) and at its left/right/bottom/top neighbours.
+__device__ inline double smvp(double c, double l, double r, double b, double t,
+                              const double* __restrict__ kx,
+                              const double* __restrict__ ky, int i, int x) {
+  const double kx0 = kx[i], kx1 = kx[i + 1];
+  const double ky0 = ky[i], ky1 = ky[i + x];
+  const double diag = (1.0 + (kx1 + kx0)) + (ky1 + ky0);
+  return (diag * c - (kx1 * r + kx0 * l)) - (ky1 * t + ky0 * b);
+}
+
+// ---- cg_init (cg.cpp:7-134) ----------------------------------------------
+
+struct InitUBody {
+  double* __restrict__ p;
+  double* __restrict__ r;
+  double* __restrict__ u;
+  double* __restrict__ w;
+  const double* __restrict__ density;
+  const double* __restrict__ energy;
+  int coefficient, x, y;
+  __device__ void operator()(int i, int kk, int jj) const {
+    p[i] = 0.0;
+    r[i] = 0.0;
+    u[i] = energy[i] * density[i];
+    if (jj > 0 && jj < y - 1 && kk > 0 && kk < x - 1)
+      w[i] = coefficient == TEA_CONDUCTIVITY ? density[i] : 1.0 / density[i];
+  }
+};
+
+struct InitKBody {
+  const double* __restrict__ w;
+  double* __restrict__ kx;
+  double* __restrict__ ky;
+  double rx, ry;
+  int x, y, h;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (jj >= h && jj < y - 1 && kk >= h && kk < x - 1) {
+      kx[i] = rx * (w[i - 1] + w[i]) / (2.0 * w[i - 1] * w[i]);
+      ky[i] = ry * (w[i - x] + w[i]) / (2.0 * w[i - x] * w[i]);
+    }
+  }
+};
+
+struct InitOthersBody {
+  const double* __restrict__ kx;
+  const double* __restrict__ ky;
+  double* __restrict__ p;
+  double* __restrict__ r;
+  const double* __restrict__ u;
+  double* __restrict__ w;
+  int x;
+  __device__ void operator()(int i, int, int, double (&out)[1]) const {
+    const double s = smvp(u[i], u[i - 1], u[i + 1], u[i - x], u[i + x], kx, ky,
+                          i, x);
+    w[i] = s;
+    const double rr = u[i] - s;
+    r[i] = rr;
+    p[i] = rr;
+    out[0] = rr * rr;
+  }
+};
+
+// ---- one launch per reference kernel -------------------------------------
+
+struct CalcWBody {  // cg.cpp:137-195
+  const double* __restrict__ p;
+  const double* __restrict__ kx;
+  const double* __restrict__ ky;
+  double* __restrict__ w;
+  int x;
+  __device__ void operator()(int i, int, int, double (&out)[1]) const {
+    const double pc = p[i];
+    const double s = smvp(pc, p[i - 1], p[i + 1], p[i - x], p[i + x], kx, ky,
+                          i, x);
+    w[i] = s;
+    out[0] = s * pc;
+  }
+};
+
+struct CalcURBody {  // cg.cpp:198-254
+  double* __restrict__ u;
+  double* __restrict__ r;
+  const double* __restrict__ p;
+  const double* __restrict__ w;
+  double alpha;
+  __device__ void operator()(int i, int, int, double (&out)[1]) const {
+    u[i] = u[i] + alpha * p[i];
+    const double rn = r[i] - alpha * w[i];
+    r[i] = rn;
+    out[0] = rn * rn;
+  }
+};
+
+struct CalcPBody {  // cg.cpp:257-281
+  double* __restrict__ p;
+  const double* __restrict__ r;
+  double beta;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (interior(g, kk, jj)) p[i] = beta * p[i] + r[i];
+  }
+};
+
+// ---- fused iteration ------------------------------------------------------
+//
+// Kernel 1 (fused_w), in one pass over the grid:
+//   u += alpha_prev * p_old                  (deferred from cg_calc_ur)
+//   p_new = beta * p_old + r                 (deferred cg_calc_p)
+//   p_new, u on the depth-1 halo ring         (deferred run_local_halos)
+//   w = A p_new,  pw = sum p_new . w          (cg_calc_w)
+// reads p_old, r, u, kx, ky and writes p_new, u, w: 8 words per cell.
+// p_new at a neighbour is recomputed from p_old and r there; a neighbour on
+// the halo ring is the reflection of the cell itself, so its p_new is pc.
+// Kernel 2 (fused_r): r -= alpha w, rrn = r.r (3 words per cell).
+// Together 11 words = 88 bytes per cell per iteration against the
+// reference's 13 (calc_w 4 + calc_ur 6 + calc_p 3).
+
+struct FusedWBody {
+  const double* __restrict__ p_old;
+  double* __restrict__ p_new;
+  const double* __restrict__ r;
+  double* __restrict__ u;
+  double* __restrict__ w;
+  const double* __restrict__ kx;
+  const double* __restrict__ ky;
+  double alpha, beta;
+  int apply_u, apply_p, ring;
+  int x, xl, xr, yb, yt;  // interior bounds: xl <= kk <= xr, yb <= jj <= yt
+
+  __device__ inline double pn(int i) const { return beta * p_old[i] + r[i]; }
+
+  __device__ void operator()(int i, int kk, int jj, double (&out)[1]) const {
+    double pc, pl, pr, pb, pt;
+    const double po = p_old[i];
+    if (apply_p) {
+      pc = beta * po + r[i];
+      pl = kk == xl ? pc : pn(i - 1);
+      pr = kk == xr ? pc : pn(i + 1);
+      pb = jj == yb ? pc : pn(i - x);
+      pt = jj == yt ? pc : pn(i + x);
+    } else {
+      pc = po;
+      pl = p_old[i - 1];
+      pr = p_old[i + 1];
+      pb = p_old[i - x];
+      pt = p_old[i + x];
+    }
+    const double s = smvp(pc, pl, pr, pb, pt, kx, ky, i, x);
+    w[i] = s;
+    out[0] = s * pc;
+
+    if (apply_u) {
+      const double un = u[i] + alpha * po;
+      u[i] = un;
+      if (ring) write_ring(u, i, kk, jj, un);
+    }
+    if (apply_p) {
+      p_new[i] = pc;
+      if (ring)
+        write_ring(p_new, i, kk, jj, pc);
+      else
+        copy_ring(i, kk, jj);
+    }
+  }
+
+  // The reflective depth-1 halo of a boundary-adjacent cell: the face
+  // neighbours and, at the four interior corners, the diagonal one (which
+  // the reference's top/bottom pass fills from the left/right halo column).
+  __device__ inline void write_ring(double* f, int i, int kk, int jj,
+                                    double v) const {
+    const bool L = kk == xl, R = kk == xr, B = jj == yb, T = jj == yt;
+    if (L) f[i - 1] = v;
+    if (R) f[i + 1] = v;
+    if (B) f[i - x] = v;
+    if (T) f[i + x] = v;
+    if (L && B) f[i - 1 - x] = v;
+    if (R && B) f[i + 1 - x] = v;
+    if (L && T) f[i - 1 + x] = v;
+    if (R && T) f[i + 1 + x] = v;
+  }
+  // p updated without a halo refresh keeps its old halo values.
+  __device__ inline void copy_ring(int i, int kk, int jj) const {
+    const bool L = kk == xl, R = kk == xr, B = jj == yb, T = jj == yt;
+    if (L) p_new[i - 1] = p_old[i - 1];
+    if (R) p_new[i + 1] = p_old[i + 1];
+    if (B) p_new[i - x] = p_old[i - x];
+    if (T) p_new[i + x] = p_old[i + x];
+    if (L && B) p_new[i - 1 - x] = p_old[i - 1 - x];
+    if (R && B) p_new[i + 1 - x] = p_old[i + 1 - x];
+    if (L && T) p_new[i - 1 + x] = p_old[i - 1 + x];
+    if (R && T) p_new[i + 1 + x] = p_old[i + 1 + x];
+  }
+};
+
+struct FusedRBody {
+  double* __restrict__ r;
+  const double* __restrict__ w;
+  double alpha;
+  __device__ void operator()(int i, int, int, double (&out)[1]) const {
+    const double rn = r[i] - alpha * w[i];
+    r[i] = rn;
+    out[0] = rn * rn;
+  }
+};
+
+struct ApplyPendingBody {
+  double* __restrict__ u;
+  double* __restrict__ p;
+  const double* __restrict__ r;
+  double alpha, beta;
+  int apply_u, apply_p;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (!interior(g, kk, jj)) return;
+    const double po = p[i];
+    if (apply_u) u[i] = u[i] + alpha * po;
+    if (apply_p) p[i] = beta * po + r[i];
+  }
+};
+
+// ---- shared solver kernels (solver_methods.cpp) ---------------------------
+
+struct CopyUBody {
+  const double* __restrict__ u;
+  double* __restrict__ u0;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (interior(g, kk, jj)) u0[i] = u[i];
+  }
+};
+
+struct ResidualBody {
+  const double* __restrict__ u;
+  const double* __restrict__ u0;
+  double* __restrict__ r;
+  const double* __restrict__ kx;
+  const double* __restrict__ ky;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (!interior(g, kk, jj)) return;
+    const int x = g.x;
+    r[i] = u0[i] - smvp(u[i], u[i - 1], u[i + 1], u[i - x], u[i + x], kx, ky,
+                        i, x);
+  }
+};
+
+struct NormBody {
+  const double* __restrict__ b;
+  __device__ void operator()(int i, int, int, double (&out)[1]) const {
+    const double v = b[i];
+    out[0] = v * v;
+  }
+};
+
+struct FinaliseBody {
+  const double* __restrict__ u;
+  const double* __restrict__ density;
+  double* __restrict__ energy;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (interior(g, kk, jj)) energy[i] = u[i] / density[i];
+  }
+};
+
+template <class Body>
+int launch_each(tea_hip_chunk* c, int kid, const Body& body) {
+  {
+    Timed t(c, kid);
+    hipLaunchKernelGGL((each_kernel<Body>), flat_grid(c->geo.n), dim3(kBlock),
+                       0, c->stream, c->geo, body);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
+template <class Body>
+int launch_span1(tea_hip_chunk* c, int kid, const Body& body, double* out) {
+  {
+    Timed t(c, kid);
+    hipLaunchKernelGGL((span_kernel<1, Body>), span_grid(c), dim3(kBlock), 0,
+                       c->stream, c->geo, body, c->partials, c->nspan,
+                       c->ngroup == 1 ? 1 : 0);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  return finish_reduction(c, 1, out);
+}
+
+// Makes the second p buffer's cells outside the interior and the halo ring
+// equal to p's (the fused kernel writes only the interior and the ring).
+int sync_p_alt(tea_hip_chunk* c) {
+  if (c->p_alt_synced) return TEA_HIP_OK;
+  TEA_HIP_TRY(hipMemcpyAsync(c->p_alt, c->field[TEA_FIELD_P],
+                             (size_t)c->geo.n * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+  c->p_alt_synced = true;
+  return TEA_HIP_OK;
+}
+
+}  // namespace
+
+int apply_pending(tea_hip_chunk* c) {
+  if (!c->pend_u && !c->pend_p && !c->pend_halo) return TEA_HIP_OK;
+  double** f = c->field;
+  if (c->pend_u || c->pend_p) {
+    ApplyPendingBody body{f[TEA_FIELD_U], f[TEA_FIELD_P], f[TEA_FIELD_R],
+                          c->pend_alpha, c->pend_beta, c->pend_u ? 1 : 0,
+                          c->pend_p ? 1 : 0, c->geo};
+    TEA_RETURN_IF(launch_each(c, K_CG_APPLY_PENDING, body));
+    if (c->pend_p) c->p_alt_synced = false;
+  }
+  const bool halo = c->pend_halo;
+  c->pend_u = c->pend_p = c->pend_halo = false;
+  if (halo) {
+    static const int ext[4] = {-1, -1, -1, -1};
+    int fields[TEA_HALO_NUM] = {0};
+    fields[TEA_HALO_U] = fields[TEA_HALO_P] = 1;
+    TEA_RETURN_IF(launch_halos(c, 1, ext, fields));
+    c->p_alt_synced = false;
+  }
+  return TEA_HIP_OK;
+}
+
+}  // namespace tea
+
+using namespace tea;
+
+#define TEA_ENTER(c)                                      \
+  TEA_CHECK_ARG((c) != nullptr, "null chunk handle");     \
+  TEA_HIP_TRY(hipSetDevice((c)->device))
+
+extern "C" {
+
+int tea_hip_cg_init(tea_hip_chunk* c, int coefficient, double rx, double ry,
+                    double* rro) {
+  TEA_ENTER(c);
+  TEA_CHECK_ARG(rro != nullptr, "null rro");
+  TEA_CHECK_ARG(coefficient == TEA_CONDUCTIVITY ||
+                    coefficient == TEA_RECIP_CONDUCTIVITY,
+                "unknown conductivity model");
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  const Geo g = c->geo;
+  TEA_RETURN_IF(launch_each(
+      c, K_CG_INIT_U,
+      InitUBody{f[TEA_FIELD_P], f[TEA_FIELD_R], f[TEA_FIELD_U], f[TEA_FIELD_W],
+                f[TEA_FIELD_DENSITY], f[TEA_FIELD_ENERGY], coefficient, g.x,
+                g.y}));
+  TEA_RETURN_IF(launch_each(c, K_CG_INIT_K,
+                            InitKBody{f[TEA_FIELD_W], f[TEA_FIELD_KX],
+                                      f[TEA_FIELD_KY], rx, ry, g.x, g.y, g.h}));
+  c->p_alt_synced = false;
+  return launch_span1(c, K_CG_INIT_OTHERS,
+                      InitOthersBody{f[TEA_FIELD_KX], f[TEA_FIELD_KY],
+                                     f[TEA_FIELD_P], f[TEA_FIELD_R],
+                                     f[TEA_FIELD_U], f[TEA_FIELD_W], g.x},
+                      rro);
+}
+
+int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
+  TEA_ENTER(c);
+  TEA_CHECK_ARG(pw != nullptr, "null pw");
+  double** f = c->field;
+  const Geo g = c->geo;
+  if (!c->fused) {
+    TEA_RETURN_IF(apply_pending(c));
+    return launch_span1(c, K_CG_CALC_W,
+                        CalcWBody{f[TEA_FIELD_P], f[TEA_FIELD_KX],
+                                  f[TEA_FIELD_KY], f[TEA_FIELD_W], g.x},
+                        pw);
+  }
+  const bool apply_p = c->pend_p;
+  if (apply_p) TEA_RETURN_IF(sync_p_alt(c));
+  FusedWBody body{f[TEA_FIELD_P],
+                  c->p_alt,
+                  f[TEA_FIELD_R],
+                  f[TEA_FIELD_U],
+                  f[TEA_FIELD_W],
+                  f[TEA_FIELD_KX],
+                  f[TEA_FIELD_KY],
+                  c->pend_alpha,
+                  c->pend_beta,
+                  c->pend_u ? 1 : 0,
+                  apply_p ? 1 : 0,
+                  c->pend_halo ? 1 : 0,
+                  g.x,
+                  g.h,
+                  g.x - g.h - 1,
+                  g.h,
+                  g.y - g.h - 1};
+  c->pend_u = c->pend_p = c->pend_halo = false;
+  {
+    Timed t(c, K_CG_FUSED_W);
+    hipLaunchKernelGGL((span_kernel<1, FusedWBody>), span_grid(c),
+                       dim3(kBlock), 0, c->stream, g, body, c->partials,
+                       c->nspan, c->ngroup == 1 ? 1 : 0);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  if (apply_p) {
+    double* t = f[TEA_FIELD_P];
+    f[TEA_FIELD_P] = c->p_alt;
+    c->p_alt = t;
+  }
+  return finish_reduction(c, 1, pw);
+}
+
+int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
+  TEA_ENTER(c);
+  TEA_CHECK_ARG(rrn != nullptr, "null rrn");
+  double** f = c->field;
+  TEA_RETURN_IF(apply_pending(c));
+  if (!c->fused)
+    return launch_span1(c, K_CG_CALC_UR,
+                        CalcURBody{f[TEA_FIELD_U], f[TEA_FIELD_R],
+                                   f[TEA_FIELD_P], f[TEA_FIELD_W], alpha},
+                        rrn);
+  TEA_RETURN_IF(launch_span1(
+      c, K_CG_FUSED_R, FusedRBody{f[TEA_FIELD_R], f[TEA_FIELD_W], alpha}, rrn));
+  c->pend_u = true;
+  c->pend_alpha = alpha;
+  return TEA_HIP_OK;
+}
+
+int tea_hip_cg_calc_p(tea_hip_chunk* c, double beta) {
+  TEA_ENTER(c);
+  double** f = c->field;
+  if (!c->fused || c->pend_p || c->pend_halo) TEA_RETURN_IF(apply_pending(c));
+  if (!c->fused) {
+    c->p_alt_synced = false;
+    return launch_each(c, K_CG_CALC_P,
+                       CalcPBody{f[TEA_FIELD_P], f[TEA_FIELD_R], beta, c->geo});
+  }
+  c->pend_p = true;
+  c->pend_beta = beta;
+  return TEA_HIP_OK;
+}
+
+int tea_hip_copy_u(tea_hip_chunk* c) {
+  TEA_ENTER(c);
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  return launch_each(c, K_COPY_U,
+                     CopyUBody{f[TEA_FIELD_U], f[TEA_FIELD_U0], c->geo});
+}
+
+int tea_hip_calculate_residual(tea_hip_chunk* c) {
+  TEA_ENTER(c);
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  return launch_each(c, K_CALC_RESIDUAL,
+                     ResidualBody{f[TEA_FIELD_U], f[TEA_FIELD_U0],
+                                  f[TEA_FIELD_R], f[TEA_FIELD_KX],
+                                  f[TEA_FIELD_KY], c->geo});
+}
+
+int tea_hip_calculate_2norm(tea_hip_chunk* c, const double* buffer,
+                            double* norm) {
+  TEA_ENTER(c);
+  TEA_CHECK_ARG(norm != nullptr, "null norm");
+  TEA_RETURN_IF(apply_pending(c));
+  bool known = false;
+  for (int fid = 0; fid < TEA_NUM_FIELDS; ++fid)
+    known |= buffer == c->field[fid] &&
+             c->field_len[fid] >= (size_t)c->geo.n;
+  TEA_CHECK_ARG(known, "buffer is not an x*y field of this chunk");
+  return launch_span1(c, K_CALC_2NORM, NormBody{buffer}, norm);
+}
+
+int tea_hip_finalise(tea_hip_chunk* c) {
+  TEA_ENTER(c);
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  return launch_each(c, K_FINALISE,
+                     FinaliseBody{f[TEA_FIELD_U], f[TEA_FIELD_DENSITY],
+                                  f[TEA_FIELD_ENERGY], c->geo});
+}
+
+}  // extern "C"

@@ -1,0 +1,343 @@
+// libteahip.so: chunk handle, field access, timing and the final pass of
+// every reduction.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "tea_hip_impl.h"
+
+namespace tea {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+const char* const kKernelNames[K_NUM] = {
+    "set_chunk_data", "set_chunk_state", "store_energy", "local_halos",
+    "field_summary",  "cg_init_u",       "cg_init_k",    "cg_init_others",
+    "cg_calc_w",      "cg_calc_ur",      "cg_calc_p",    "cg_fused_w",
+    "cg_fused_r",     "cg_apply_pending", "copy_u",      "calculate_residual",
+    "calculate_2norm", "finalise",       "final_reduce"};
+
+Timed::Timed(tea_hip_chunk* c_, int kid_) : c(c_), kid(kid_) {
+  if (!c->timing) return;
+  if (c->event_pool.size() < 2) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      c->event_pool.push_back(e);
+    }
+  }
+  start = c->event_pool.back();
+  c->event_pool.pop_back();
+  stop = c->event_pool.back();
+  c->event_pool.pop_back();
+  (void)hipEventRecord(start, c->stream);
+}
+
+Timed::~Timed() {
+  if (!start) return;
+  (void)hipEventRecord(stop, c->stream);
+  c->inflight.push_back({kid, start, stop});
+}
+
+int resolve_timing(tea_hip_chunk* c) {
+  for (auto& p : c->inflight) {
+    float ms = 0.0f;
+    TEA_HIP_TRY(hipEventElapsedTime(&ms, p.start, p.stop));
+    c->launches[p.kid] += 1;
+    c->total_ms[p.kid] += ms;
+    c->event_pool.push_back(p.start);
+    c->event_pool.push_back(p.stop);
+  }
+  c->inflight.clear();
+  return TEA_HIP_OK;
+}
+
+int sync_chunk(tea_hip_chunk* c) {
+  TEA_HIP_TRY(hipStreamSynchronize(c->stream));
+  return resolve_timing(c);
+}
+
+// Final passes of the reference's reduceArray over the level-2 partials of
+// one launch: nr arrays of nspan values each.  One 1024-thread block; each
+// pass cuts the array into zero-padded groups of 64 and sums each group with
+// the 64-lane tree, exactly as TeaLeaf/c_kernels/sycl/sycl_shared.hpp:43-75
+// does with one work-group per group.  Results go straight to pinned host
+// memory.
+constexpr int kFinalThreads = 1024;
+constexpr int kFinalWaves = kFinalThreads / kWave;
+constexpr int kMaxLevel3 = 4096;  // supports nspan <= 64 * 4096
+
+__global__ __launch_bounds__(kFinalThreads) void final_reduce_kernel(
+    const double* __restrict__ partials, int nspan, int nr,
+    double* __restrict__ out) {
+  __shared__ double buf[2][kMaxLevel3];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  for (int q = 0; q < nr; ++q) {
+    const double* src = partials + (size_t)q * nspan;
+    double result;
+    if (nspan == 1) {
+      result = src[0];
+    } else {
+      int len = nspan;
+      int ng = (len + kWave - 1) / kWave;
+      for (int g = wv; g < ng; g += kFinalWaves) {
+        const int i = g * kWave + lane;
+        const double t = wave_tree(i < len ? src[i] : 0.0);
+        if (lane == 0) buf[0][g] = t;
+      }
+      __syncthreads();
+      int cur = 0;
+      len = ng;
+      while (len > 1) {
+        ng = (len + kWave - 1) / kWave;
+        for (int g = wv; g < ng; g += kFinalWaves) {
+          const int i = g * kWave + lane;
+          const double t = wave_tree(i < len ? buf[cur][i] : 0.0);
+          if (lane == 0) buf[cur ^ 1][g] = t;
+        }
+        __syncthreads();
+        cur ^= 1;
+        len = ng;
+      }
+      result = buf[cur][0];
+      __syncthreads();  // buf is reused by the next q
+    }
+    if (threadIdx.x == 0) out[q] = result;
+  }
+}
+
+int finish_reduction(tea_hip_chunk* c, int nr, double* out) {
+  {
+    Timed t(c, K_FINAL_REDUCE);
+    hipLaunchKernelGGL(final_reduce_kernel, dim3(1), dim3(kFinalThreads), 0,
+                       c->stream, c->partials, c->nspan, nr,
+                       c->host_result_dev);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  TEA_RETURN_IF(sync_chunk(c));
+  for (int q = 0; q < nr; ++q) out[q] = c->host_result[q];
+  return TEA_HIP_OK;
+}
+
+}  // namespace tea
+
+using namespace tea;
+
+#define TEA_CHECK_CHUNK(c) TEA_CHECK_ARG((c) != nullptr, "null chunk handle")
+
+extern "C" {
+
+const char* tea_hip_last_error(void) { return g_last_error.c_str(); }
+
+int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
+                         tea_hip_chunk** out) {
+  TEA_CHECK_ARG(out != nullptr, "null output handle");
+  *out = nullptr;
+  TEA_CHECK_ARG(halo_depth >= 1, "halo_depth must be >= 1");
+  TEA_CHECK_ARG(x > 2 * halo_depth && y > 2 * halo_depth,
+                "x and y must exceed 2*halo_depth");
+  const long n = (long)x * (long)y;
+  TEA_CHECK_ARG(n <= (long)kSpan * kWave * kMaxLevel3 && n < (1L << 31) - kSpan,
+                "chunk too large for the reduction workspace");
+  int ndev = 0;
+  TEA_HIP_TRY(hipGetDeviceCount(&ndev));
+  TEA_CHECK_ARG(device >= 0 && device < ndev, "no such HIP device");
+  TEA_HIP_TRY(hipSetDevice(device));
+
+  tea_hip_chunk* c = new tea_hip_chunk();
+  c->device = device;
+  c->geo.x = x;
+  c->geo.y = y;
+  c->geo.h = halo_depth;
+  c->geo.n = (int)n;
+  c->geo.dx = FastDiv((uint32_t)x);
+  c->ngroup = (int)((n + kWave - 1) / kWave);
+  c->nspan = (int)((n + kSpan - 1) / kSpan);
+
+  auto fail = [&](int code) {
+    tea_hip_chunk_destroy(c);
+    return code;
+  };
+
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreateWithFlags failed");
+    return fail(TEA_HIP_EHIP);
+  }
+
+  const size_t xy = (size_t)n;
+  const size_t X = (size_t)x, Y = (size_t)y;
+  for (int f = 0; f < TEA_NUM_FIELDS; ++f) c->field_len[f] = xy;
+  c->field_len[TEA_FIELD_X_AREA] = (X + 1) * Y;
+  c->field_len[TEA_FIELD_Y_AREA] = X * (Y + 1);
+  c->field_len[TEA_FIELD_CELL_X] = X;
+  c->field_len[TEA_FIELD_CELL_Y] = Y;
+  c->field_len[TEA_FIELD_CELL_DX] = X;
+  c->field_len[TEA_FIELD_CELL_DY] = Y;
+  c->field_len[TEA_FIELD_VERTEX_DX] = X + 1;
+  c->field_len[TEA_FIELD_VERTEX_DY] = Y + 1;
+  c->field_len[TEA_FIELD_VERTEX_X] = X + 1;
+  c->field_len[TEA_FIELD_VERTEX_Y] = Y + 1;
+  c->field_len[TEA_FIELD_COMMS_BUFFER] = std::max(X, Y) * (size_t)halo_depth;
+
+  for (int f = 0; f < TEA_NUM_FIELDS; ++f) {
+    const size_t bytes = c->field_len[f] * sizeof(double);
+    if (hipMalloc(&c->field[f], bytes) != hipSuccess) {
+      c->field[f] = nullptr;
+      set_error("hipMalloc of a field failed");
+      return fail(TEA_HIP_ENOMEM);
+    }
+    if (hipMemsetAsync(c->field[f], 0, bytes, c->stream) != hipSuccess) {
+      set_error("hipMemsetAsync failed");
+      return fail(TEA_HIP_EHIP);
+    }
+  }
+  if (hipMalloc(&c->p_alt, xy * sizeof(double)) != hipSuccess) {
+    c->p_alt = nullptr;
+    set_error("hipMalloc of the second p buffer failed");
+    return fail(TEA_HIP_ENOMEM);
+  }
+  if (hipMalloc(&c->partials, 4 * (size_t)c->nspan * sizeof(double)) !=
+      hipSuccess) {
+    c->partials = nullptr;
+    set_error("hipMalloc of the reduction workspace failed");
+    return fail(TEA_HIP_ENOMEM);
+  }
+  if (hipHostMalloc(&c->host_result, 64 * sizeof(double),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    c->host_result = nullptr;
+    set_error("hipHostMalloc of the result slots failed");
+    return fail(TEA_HIP_ENOMEM);
+  }
+  std::memset(c->host_result, 0, 64 * sizeof(double));
+  if (hipHostGetDevicePointer((void**)&c->host_result_dev, c->host_result, 0) !=
+      hipSuccess) {
+    set_error("hipHostGetDevicePointer failed");
+    return fail(TEA_HIP_EHIP);
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    set_error("hipStreamSynchronize failed");
+    return fail(TEA_HIP_EHIP);
+  }
+  *out = c;
+  return TEA_HIP_OK;
+}
+
+int tea_hip_chunk_destroy(tea_hip_chunk* c) {
+  if (!c) return TEA_HIP_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& p : c->inflight) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  for (int f = 0; f < TEA_NUM_FIELDS; ++f)
+    if (c->field[f]) (void)hipFree(c->field[f]);
+  if (c->p_alt) (void)hipFree(c->p_alt);
+  if (c->partials) (void)hipFree(c->partials);
+  if (c->host_result) (void)hipHostFree(c->host_result);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return TEA_HIP_OK;
+}
+
+int tea_hip_field(tea_hip_chunk* c, int field, double** dev_ptr,
+                  size_t* count) {
+  TEA_CHECK_CHUNK(c);
+  TEA_CHECK_ARG(field >= 0 && field < TEA_NUM_FIELDS, "bad field id");
+  if (dev_ptr) *dev_ptr = c->field[field];
+  if (count) *count = c->field_len[field];
+  return TEA_HIP_OK;
+}
+
+int tea_hip_field_write(tea_hip_chunk* c, int field, const double* host,
+                        size_t count) {
+  TEA_CHECK_CHUNK(c);
+  TEA_CHECK_ARG(field >= 0 && field < TEA_NUM_FIELDS, "bad field id");
+  TEA_CHECK_ARG(count <= c->field_len[field], "count exceeds field length");
+  TEA_CHECK_ARG(host != nullptr || count == 0, "null host buffer");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_RETURN_IF(apply_pending(c));
+  TEA_HIP_TRY(hipMemcpyAsync(c->field[field], host, count * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  if (field == TEA_FIELD_P) c->p_alt_synced = false;
+  return sync_chunk(c);
+}
+
+int tea_hip_field_read(tea_hip_chunk* c, int field, double* host,
+                       size_t count) {
+  TEA_CHECK_CHUNK(c);
+  TEA_CHECK_ARG(field >= 0 && field < TEA_NUM_FIELDS, "bad field id");
+  TEA_CHECK_ARG(count <= c->field_len[field], "count exceeds field length");
+  TEA_CHECK_ARG(host != nullptr || count == 0, "null host buffer");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_RETURN_IF(apply_pending(c));
+  TEA_HIP_TRY(hipMemcpyAsync(host, c->field[field], count * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  return sync_chunk(c);
+}
+
+int tea_hip_synchronize(tea_hip_chunk* c) {
+  TEA_CHECK_CHUNK(c);
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  return sync_chunk(c);
+}
+
+int tea_hip_set_fused(tea_hip_chunk* c, int fused) {
+  TEA_CHECK_CHUNK(c);
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_RETURN_IF(apply_pending(c));
+  c->fused = fused ? 1 : 0;
+  return TEA_HIP_OK;
+}
+
+int tea_hip_flush(tea_hip_chunk* c) {
+  TEA_CHECK_CHUNK(c);
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_RETURN_IF(apply_pending(c));
+  return sync_chunk(c);
+}
+
+int tea_hip_timing_enable(tea_hip_chunk* c, int enable) {
+  TEA_CHECK_CHUNK(c);
+  TEA_RETURN_IF(sync_chunk(c));
+  c->timing = enable != 0;
+  return TEA_HIP_OK;
+}
+
+int tea_hip_kernel_stats(tea_hip_chunk* c, int kernel_id, long* launches,
+                         double* total_ms) {
+  TEA_CHECK_CHUNK(c);
+  TEA_CHECK_ARG(kernel_id >= 0 && kernel_id < K_NUM, "bad kernel id");
+  TEA_RETURN_IF(sync_chunk(c));
+  if (launches) *launches = c->launches[kernel_id];
+  if (total_ms) *total_ms = c->total_ms[kernel_id];
+  return TEA_HIP_OK;
+}
+
+int tea_hip_timing_reset(tea_hip_chunk* c) {
+  TEA_CHECK_CHUNK(c);
+  TEA_RETURN_IF(sync_chunk(c));
+  for (int k = 0; k < K_NUM; ++k) {
+    c->launches[k] = 0;
+    c->total_ms[k] = 0.0;
+  }
+  return TEA_HIP_OK;
+}
+
+int tea_hip_num_kernels(void) { return K_NUM; }
+
+const char* tea_hip_kernel_name(int kernel_id) {
+  if (kernel_id < 0 || kernel_id >= K_NUM) return "";
+  return kKernelNames[kernel_id];
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+// The 24 kernel_interface.h entry points of the HIP backend
+// (TeaLeaf/kernel_interface.h:13-71; the SYCL backend's counterpart is
+// TeaLeaf/c_kernels/sycl/kernel_interface.cpp).  Each one unpacks
+// Chunk/Settings and forwards to the C ABI of libteahip.so (tea_hip.h).
+//
+// This file compiles against either host: this project's
+// (exploringsycl_amd/host) or the reference's own TeaLeaf/*.h, with
+// csrc/backend first on the include path for chunk_extension.h.  The symbols
+// are C++ (the reference host is compiled as C++).
+//
+// Error behaviour follows the reference: a failed call ends the run through
+// the host's die() with the library's message.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "kernel_interface.h"
+#include "tea_hip.h"
+
+namespace {
+
+void check(int status, const char* what, int line) {
+  if (status != TEA_HIP_OK)
+    die(line, __FILE__, "%s failed: %s\n", what, tea_hip_last_error());
+}
+
+#define HIP_CALL(expr) check((expr), #expr, __LINE__)
+
+tea_hip_chunk* H(Chunk* chunk) { return chunk->ext->hip; }
+
+double* field_ptr(tea_hip_chunk* h, int field) {
+  double* p = nullptr;
+  HIP_CALL(tea_hip_field(h, field, &p, nullptr));
+  return p;
+}
+
+// p is double-buffered by the fused CG iteration: republish its address
+// after every call that can swap it.
+void refresh_p(Chunk* chunk) { chunk->p = field_ptr(H(chunk), TEA_FIELD_P); }
+
+// Live handles, for the timing hook below.
+std::vector<tea_hip_chunk*> g_handles;
+
+int device_from_env() {
+  const char* s = getenv("TEA_HIP_DEVICE");
+  return s ? atoi(s) : 0;
+}
+
+}  // namespace
+
+void run_kernel_initialise(Chunk* chunk, Settings* settings) {
+  tea_hip_chunk* h = nullptr;
+  HIP_CALL(tea_hip_chunk_create(device_from_env(), chunk->x, chunk->y,
+                                settings->halo_depth, &h));
+  chunk->ext->hip = h;
+  g_handles.push_back(h);
+  const char* fused = getenv("TEA_HIP_FUSED");
+  if (fused && atoi(fused) == 0) HIP_CALL(tea_hip_set_fused(h, 0));
+
+  chunk->density0 = field_ptr(h, TEA_FIELD_DENSITY0);
+  chunk->density = field_ptr(h, TEA_FIELD_DENSITY);
+  chunk->energy0 = field_ptr(h, TEA_FIELD_ENERGY0);
+  chunk->energy = field_ptr(h, TEA_FIELD_ENERGY);
+  chunk->u = field_ptr(h, TEA_FIELD_U);
+  chunk->u0 = field_ptr(h, TEA_FIELD_U0);
+  chunk->p = field_ptr(h, TEA_FIELD_P);
+  chunk->r = field_ptr(h, TEA_FIELD_R);
+  chunk->mi = field_ptr(h, TEA_FIELD_MI);
+  chunk->w = field_ptr(h, TEA_FIELD_W);
+  chunk->kx = field_ptr(h, TEA_FIELD_KX);
+  chunk->ky = field_ptr(h, TEA_FIELD_KY);
+  chunk->sd = field_ptr(h, TEA_FIELD_SD);
+  chunk->cell_x = field_ptr(h, TEA_FIELD_CELL_X);
+  chunk->cell_y = field_ptr(h, TEA_FIELD_CELL_Y);
+  chunk->cell_dx = field_ptr(h, TEA_FIELD_CELL_DX);
+  chunk->cell_dy = field_ptr(h, TEA_FIELD_CELL_DY);
+  chunk->vertex_dx = field_ptr(h, TEA_FIELD_VERTEX_DX);
+  chunk->vertex_dy = field_ptr(h, TEA_FIELD_VERTEX_DY);
+  chunk->vertex_x = field_ptr(h, TEA_FIELD_VERTEX_X);
+  chunk->vertex_y = field_ptr(h, TEA_FIELD_VERTEX_Y);
+  chunk->volume = field_ptr(h, TEA_FIELD_VOLUME);
+  chunk->x_area = field_ptr(h, TEA_FIELD_X_AREA);
+  chunk->y_area = field_ptr(h, TEA_FIELD_Y_AREA);
+  chunk->ext->comms_buffer = field_ptr(h, TEA_FIELD_COMMS_BUFFER);
+
+  // Host-side coefficient logs of length max_iters
+  // (TeaLeaf/c_kernels/sycl/kernel_initialise.cpp:76-79).
+  const int n = settings->max_iters > 0 ? settings->max_iters : 1;
+  chunk->cg_alphas = new double[n]();
+  chunk->cg_betas = new double[n]();
+  chunk->cheby_alphas = new double[n]();
+  chunk->cheby_betas = new double[n]();
+}
+
+void run_kernel_finalise(Chunk* chunk, Settings* settings) {
+  (void)settings;
+  for (size_t i = 0; i < g_handles.size(); ++i)
+    if (g_handles[i] == H(chunk)) g_handles.erase(g_handles.begin() + i);
+  HIP_CALL(tea_hip_chunk_destroy(H(chunk)));
+  chunk->ext->hip = nullptr;
+  delete[] chunk->cg_alphas;
+  delete[] chunk->cg_betas;
+  delete[] chunk->cheby_alphas;
+  delete[] chunk->cheby_betas;
+  chunk->cg_alphas = chunk->cg_betas = nullptr;
+  chunk->cheby_alphas = chunk->cheby_betas = nullptr;
+}
+
+void run_set_chunk_data(Chunk* chunk, Settings* settings) {
+  const double x_min = settings->grid_x_min + settings->dx * (double)chunk->left;
+  const double y_min =
+      settings->grid_y_min + settings->dy * (double)chunk->bottom;
+  HIP_CALL(tea_hip_set_chunk_data(H(chunk), x_min, y_min, settings->dx,
+                                  settings->dy));
+}
+
+void run_set_chunk_state(Chunk* chunk, Settings* settings, State* states) {
+  const int n = settings->num_states;
+  if (n < 1) die(__LINE__, __FILE__, "no states defined\n");
+  double* buf = new double[7 * (size_t)n];
+  int* geometry = new int[n];
+  double *density = buf, *energy = buf + n, *x_min = buf + 2 * n,
+         *y_min = buf + 3 * n, *x_max = buf + 4 * n, *y_max = buf + 5 * n,
+         *radius = buf + 6 * n;
+  for (int s = 0; s < n; ++s) {
+    density[s] = states[s].density;
+    energy[s] = states[s].energy;
+    x_min[s] = states[s].x_min;
+    y_min[s] = states[s].y_min;
+    x_max[s] = states[s].x_max;
+    y_max[s] = states[s].y_max;
+    radius[s] = states[s].radius;
+    geometry[s] = states[s].geometry == RECTANGULAR ? TEA_GEOMETRY_RECTANGULAR
+                  : states[s].geometry == CIRCULAR  ? TEA_GEOMETRY_CIRCULAR
+                                                    : TEA_GEOMETRY_POINT;
+  }
+  HIP_CALL(tea_hip_set_chunk_state(H(chunk), n, density, energy, x_min, y_min,
+                                   x_max, y_max, radius, geometry));
+  delete[] buf;
+  delete[] geometry;
+}
+
+void run_local_halos(Chunk* chunk, Settings* settings, int depth) {
+  int fields[TEA_HALO_NUM];
+  for (int s = 0; s < TEA_HALO_NUM; ++s)
+    fields[s] = settings->fields_to_exchange[s] ? 1 : 0;
+  HIP_CALL(tea_hip_local_halos(H(chunk), depth, chunk->neighbours, fields));
+  refresh_p(chunk);
+}
+
+void run_pack_or_unpack(Chunk* chunk, Settings* settings, int depth, int face,
+                        bool pack, FieldBufferType field, double* buffer) {
+  (void)chunk; (void)settings; (void)depth; (void)face; (void)pack;
+  (void)field; (void)buffer;
+  die(__LINE__, __FILE__,
+      "run_pack_or_unpack: remote halo exchange is not part of this "
+      "single-GPU backend\n");
+}
+
+void run_store_energy(Chunk* chunk, Settings* settings) {
+  (void)settings;
+  HIP_CALL(tea_hip_store_energy(H(chunk)));
+}
+
+void run_field_summary(Chunk* chunk, Settings* settings, double* vol,
+                       double* mass, double* ie, double* temp) {
+  (void)settings;
+  double v, m, e, t;
+  HIP_CALL(tea_hip_field_summary(H(chunk), &v, &m, &e, &t));
+  refresh_p(chunk);
+  *vol += v;
+  *mass += m;
+  *ie += e;
+  *temp += t;
+}
+
+void run_cg_init(Chunk* chunk, Settings* settings, double rx, double ry,
+                 double* rro) {
+  double v;
+  HIP_CALL(tea_hip_cg_init(H(chunk), settings->coefficient, rx, ry, &v));
+  refresh_p(chunk);
+  *rro += v;
+}
+
+void run_cg_calc_w(Chunk* chunk, Settings* settings, double* pw) {
+  (void)settings;
+  double v;
+  HIP_CALL(tea_hip_cg_calc_w(H(chunk), &v));
+  refresh_p(chunk);
+  *pw += v;
+}
+
+void run_cg_calc_ur(Chunk* chunk, Settings* settings, double alpha,
+                    double* rrn) {
+  (void)settings;
+  double v;
+  HIP_CALL(tea_hip_cg_calc_ur(H(chunk), alpha, &v));
+  refresh_p(chunk);
+  *rrn += v;
+}
+
+void run_cg_calc_p(Chunk* chunk, Settings* settings, double beta) {
+  (void)settings;
+  HIP_CALL(tea_hip_cg_calc_p(H(chunk), beta));
+  refresh_p(chunk);
+}
+
+// Chebyshev, Jacobi and PPCG are other solvers of the reference; their
+// symbols exist so that its host links, and end the run if called.
+void run_cheby_init(Chunk* chunk, Settings* settings) {
+  (void)chunk; (void)settings;
+  die(__LINE__, __FILE__, "the Chebyshev solver is not part of this backend\n");
+}
+void run_cheby_iterate(Chunk* chunk, Settings* settings, double alpha,
+                       double beta) {
+  (void)chunk; (void)settings; (void)alpha; (void)beta;
+  die(__LINE__, __FILE__, "the Chebyshev solver is not part of this backend\n");
+}
+void run_jacobi_init(Chunk* chunk, Settings* settings, double rx, double ry) {
+  (void)chunk; (void)settings; (void)rx; (void)ry;
+  die(__LINE__, __FILE__, "the Jacobi solver is not part of this backend\n");
+}
+void run_jacobi_iterate(Chunk* chunk, Settings* settings, double* error) {
+  (void)chunk; (void)settings; (void)error;
+  die(__LINE__, __FILE__, "the Jacobi solver is not part of this backend\n");
+}
+void run_ppcg_init(Chunk* chunk, Settings* settings) {
+  (void)chunk; (void)settings;
+  die(__LINE__, __FILE__, "the PPCG solver is not part of this backend\n");
+}
+void run_ppcg_inner_iteration(Chunk* chunk, Settings* settings, double alpha,
+                              double beta) {
+  (void)chunk; (void)settings; (void)alpha; (void)beta;
+  die(__LINE__, __FILE__, "the PPCG solver is not part of this backend\n");
+}
+
+void run_copy_u(Chunk* chunk, Settings* settings) {
+  (void)settings;
+  HIP_CALL(tea_hip_copy_u(H(chunk)));
+  refresh_p(chunk);
+}
+
+void run_calculate_residual(Chunk* chunk, Settings* settings) {
+  (void)settings;
+  HIP_CALL(tea_hip_calculate_residual(H(chunk)));
+  refresh_p(chunk);
+}
+
+void run_calculate_2norm(Chunk* chunk, Settings* settings,
+                         FieldBufferType buffer, double* norm) {
+  (void)settings;
+  double v;
+  HIP_CALL(tea_hip_calculate_2norm(H(chunk), buffer, &v));
+  refresh_p(chunk);
+  *norm += v;
+}
+
+void run_finalise(Chunk* chunk, Settings* settings) {
+  (void)settings;
+  HIP_CALL(tea_hip_finalise(H(chunk)));
+  refresh_p(chunk);
+}
+
+// Optional host hook (declared weak by the host): with TEA_HIP_STATS=<file>
+// set, "timed_begin" switches on per-kernel event timing and clears the
+// counters, and "timed_end" waits for the device and writes, per kernel, the
+// launch count and summed device milliseconds of the timed region as JSON.
+// "timed_end" always waits for the device so the host's clock covers all
+// queued work.
+extern "C" void tea_backend_mark(const char* event) {
+  const char* path = getenv("TEA_HIP_STATS");
+  if (strcmp(event, "timed_begin") == 0) {
+    for (tea_hip_chunk* h : g_handles) {
+      HIP_CALL(tea_hip_synchronize(h));
+      if (path) {
+        HIP_CALL(tea_hip_timing_enable(h, 1));
+        HIP_CALL(tea_hip_timing_reset(h));
+      }
+    }
+    return;
+  }
+  if (strcmp(event, "timed_end") != 0) return;
+  for (tea_hip_chunk* h : g_handles) HIP_CALL(tea_hip_synchronize(h));
+  if (!path) return;
+  FILE* fp = fopen(path, "w");
+  if (!fp) die(__LINE__, __FILE__, "cannot open %s\n", path);
+  fprintf(fp, "{");
+  const int nk = tea_hip_num_kernels();
+  bool first = true;
+  for (int k = 0; k < nk; ++k) {
+    long launches = 0;
+    double ms = 0.0;
+    for (tea_hip_chunk* h : g_handles) {
+      long l = 0;
+      double t = 0.0;
+      HIP_CALL(tea_hip_kernel_stats(h, k, &l, &t));
+      launches += l;
+      ms += t;
+    }
+    if (!launches) continue;
+    fprintf(fp, "%s\n  \"%s\": {\"launches\": %ld, \"total_ms\": %.6f}",
+            first ? "" : ",", tea_hip_kernel_name(k), launches, ms);
+    first = false;
+  }
+  fprintf(fp, "\n}\n");
+  fclose(fp);
+}

@@ -1,0 +1,311 @@
+// Internal declarations of libteahip.so: the chunk handle, launch helpers
+// and the device-side reduction machinery shared by the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tea_hip.h"
+
+namespace tea {
+
+// ---------------------------------------------------------------------------
+// Errors
+
+void set_error(const std::string& msg);
+
+#define TEA_HIP_TRY(call)                                                     \
+  do {                                                                        \
+    hipError_t err_ = (call);                                                 \
+    if (err_ != hipSuccess) {                                                 \
+      ::tea::set_error(std::string(#call) + " failed: " +                     \
+                       hipGetErrorString(err_) + " (" + __FILE__ + ":" +      \
+                       std::to_string(__LINE__) + ")");                       \
+      return TEA_HIP_EHIP;                                                    \
+    }                                                                         \
+  } while (0)
+
+#define TEA_CHECK_ARG(cond, msg)                                              \
+  do {                                                                        \
+    if (!(cond)) {                                                            \
+      ::tea::set_error(std::string(__func__) + ": " + (msg));                 \
+      return TEA_HIP_EINVAL;                                                  \
+    }                                                                         \
+  } while (0)
+
+#define TEA_RETURN_IF(status)                                                 \
+  do {                                                                        \
+    int st_ = (status);                                                       \
+    if (st_ != TEA_HIP_OK) return st_;                                        \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Geometry of a chunk and fast division by its row length.
+
+// Division by a runtime constant d (1 <= d < 2^31) for n < 2^31 with one
+// multiply-high: q = (umulhi(n, m) + n) >> s, s = ceil(log2 d),
+// m = floor(2^32 (2^s - d) / d) + 1 (Granlund & Montgomery 1994, sec. 4).
+struct FastDiv {
+  uint32_t d = 1, m = 0, s = 0;
+  FastDiv() = default;
+  explicit FastDiv(uint32_t divisor) : d(divisor) {
+    s = 0;
+    while ((uint64_t(1) << s) < d) ++s;
+    m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1);
+  }
+  __host__ __device__ inline uint32_t div(uint32_t n) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t t = __umulhi(n, m);
+#else
+    const uint32_t t = uint32_t((uint64_t(n) * m) >> 32);
+#endif
+    return uint32_t((uint64_t(t) + n) >> s);
+  }
+};
+
+struct Geo {
+  int x, y, h;
+  int n;      // x*y
+  FastDiv dx; // division by x
+};
+
+__device__ inline bool interior(const Geo& g, int kk, int jj) {
+  return kk >= g.h && kk < g.x - g.h && jj >= g.h && jj < g.y - g.h;
+}
+
+// ---------------------------------------------------------------------------
+// Launch shape.  A "span" is 4096 consecutive linear cells = 64 groups of 64:
+// exactly one level-2 partial of the reference's reduction tree.  A span is
+// processed by one 256-thread block (4 wave64s), 16 cells per thread, with
+// cell index base + k*256 + threadIdx.x, so wave w's k-th cell set is group
+// 4k+w of the span.
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / kWave;
+constexpr int kItems = 16;
+constexpr int kSpan = kBlock * kItems;  // 4096
+constexpr int kNumXcd = 8;              // MI355X: 8 XCDs, one L2 each
+
+// Blocks are dispatched round-robin over the XCDs (block b -> XCD b % 8).
+// Renumber them so each XCD walks one contiguous range of spans: the rows a
+// stencil reads above and below were then fetched by the same XCD, into the
+// same L2, moments earlier.
+__device__ inline int xcd_span(int b, int nb) {
+  const int xcd = b % kNumXcd, idx = b / kNumXcd;
+  const int per = nb / kNumXcd, rem = nb % kNumXcd;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+// Pairwise tree of 64 lanes (lane l <- lane l^m for m = 1..32).  After the
+// step with mask m every lane of an aligned block of 2m lanes holds the same
+// value, the tree sum of that block, so the result equals the reference's
+// "a[2s*l] += a[2s*l+s]" tree bit for bit (IEEE addition commutes exactly).
+__device__ inline double wave_tree(double v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v = v + __shfl_xor(v, m);
+  return v;
+}
+
+// Level-1 trees of the 16 groups a thread's wave holds (value k of lane l is
+// cell l of group k), with the values transposed across lanes as the tree
+// goes so each butterfly step moves half as many values as the last: after
+// steps m = 1, 2, 4, 8 lane l holds one group, k = bitreverse4(l & 15), and
+// steps 16 and 32 finish it.  Each step still adds the two aligned halves of
+// a block, so every sum is the reference tree's.  Returns, in lanes 0..15,
+// the level-1 partial of group bitreverse4(lane).
+__device__ inline double wave_tree16(double (&v)[kItems]) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int step = 0; step < 4; ++step) {
+    const int m = 1 << step;
+    const int half = kItems >> (step + 1);
+    const bool upper = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      const double send = upper ? v[i] : v[half + i];
+      const double keep = upper ? v[half + i] : v[i];
+      v[i] = keep + __shfl_xor(send, m);
+    }
+  }
+  double s = v[0];
+  s = s + __shfl_xor(s, 16);
+  s = s + __shfl_xor(s, 32);
+  return s;
+}
+
+__device__ inline int bitrev4(int l) {
+  return ((l & 1) << 3) | ((l & 2) << 1) | ((l & 4) >> 1) | ((l & 8) >> 3);
+}
+
+// Reduces the 16 values per thread of one span (NR independent sums) to the
+// span's level-2 partials and stores them at partials[q*nspan + span].
+// `single_group` (the whole field is one group of 64) stores the level-1
+// value instead, as the reference's reduceArray then runs no pass at all.
+template <int NR>
+__device__ inline void span_reduce(double (&v)[NR][kItems], double* partials,
+                                   int nspan, int span, bool single_group) {
+  __shared__ double lds[NR][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const double t = wave_tree16(v[q]);
+    if (lane < kItems) lds[q][bitrev4(lane) * kWaves + wv] = t;
+  }
+  __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const double level1 = lds[q][lane];
+      const double t = single_group ? lds[q][0] : wave_tree(level1);
+      if (lane == 0) partials[q * nspan + span] = t;
+    }
+  }
+}
+
+// One block per span: each thread evaluates `body` on its 16 cells (only
+// interior cells; everything else contributes +0.0, as in the reference) and
+// the block stores the span's NR level-2 partials.
+template <int NR, class Body>
+__global__ __launch_bounds__(kBlock) void span_kernel(Geo g, Body body,
+                                                      double* partials,
+                                                      int nspan,
+                                                      int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  double v[NR][kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int e = base + k * kBlock + threadIdx.x;
+    double out[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) out[q] = 0.0;
+    if (e < g.n) {
+      const int jj = (int)g.dx.div((uint32_t)e);
+      const int kk = e - jj * g.x;
+      if (interior(g, kk, jj)) body(e, kk, jj, out);
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) v[q][k] = out[q];
+  }
+  span_reduce<NR>(v, partials, nspan, span, single_group != 0);
+}
+
+// One thread per cell of the padded x*y range.
+template <class Body>
+__global__ __launch_bounds__(kBlock) void each_kernel(Geo g, Body body) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= g.n) return;
+  const int jj = (int)g.dx.div((uint32_t)e);
+  const int kk = e - jj * g.x;
+  body(e, kk, jj);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel identifiers for timing.
+
+enum KernelId {
+  K_SET_CHUNK_DATA = 0,
+  K_SET_CHUNK_STATE,
+  K_STORE_ENERGY,
+  K_LOCAL_HALOS,
+  K_FIELD_SUMMARY,
+  K_CG_INIT_U,
+  K_CG_INIT_K,
+  K_CG_INIT_OTHERS,
+  K_CG_CALC_W,
+  K_CG_CALC_UR,
+  K_CG_CALC_P,
+  K_CG_FUSED_W,
+  K_CG_FUSED_R,
+  K_CG_APPLY_PENDING,
+  K_COPY_U,
+  K_CALC_RESIDUAL,
+  K_CALC_2NORM,
+  K_FINALISE,
+  K_FINAL_REDUCE,
+  K_NUM
+};
+
+extern const char* const kKernelNames[K_NUM];
+
+}  // namespace tea
+
+// ---------------------------------------------------------------------------
+// The chunk handle.
+
+struct tea_hip_chunk {
+  int device = 0;
+  tea::Geo geo{};
+  int nspan = 0;   // spans covering x*y (level-2 partial count)
+  int ngroup = 0;  // groups of 64 covering x*y (level-1 partial count)
+  hipStream_t stream = nullptr;
+
+  double* field[TEA_NUM_FIELDS] = {};
+  size_t field_len[TEA_NUM_FIELDS] = {};
+
+  // Second p buffer for the fused iteration (p is read as a stencil while
+  // the new p is written).  p_alt_synced: its non-interior cells equal p's.
+  double* p_alt = nullptr;
+  bool p_alt_synced = false;
+
+  // Reduction workspace: 4 x nspan level-2 partials, and pinned host memory
+  // that the final-reduce kernel writes its results into.
+  double* partials = nullptr;
+  double* host_result = nullptr;      // host view
+  double* host_result_dev = nullptr;  // device view of the same memory
+
+  // Deferred CG work (fused mode).
+  int fused = 1;
+  bool pend_u = false, pend_p = false, pend_halo = false;
+  double pend_alpha = 0.0, pend_beta = 0.0;
+
+  // Timing.
+  bool timing = false;
+  struct Pending {
+    int kid;
+    hipEvent_t start, stop;
+  };
+  std::vector<Pending> inflight;
+  std::vector<hipEvent_t> event_pool;
+  long launches[tea::K_NUM] = {};
+  double total_ms[tea::K_NUM] = {};
+};
+
+namespace tea {
+
+// Records start/stop events around one kernel launch when timing is on.
+struct Timed {
+  tea_hip_chunk* c;
+  int kid;
+  hipEvent_t start = nullptr, stop = nullptr;
+  Timed(tea_hip_chunk* c_, int kid_);
+  ~Timed();
+};
+
+// Resolves the recorded events (after the stream has been synchronised).
+int resolve_timing(tea_hip_chunk* c);
+
+// Synchronises the stream and resolves timing.
+int sync_chunk(tea_hip_chunk* c);
+
+// Launches the final pass of a reduction: reduces nr arrays of nspan level-2
+// partials each with the reference tree and writes the nr results to
+// host_result[0..nr), then waits and copies them to `out`.
+int finish_reduction(tea_hip_chunk* c, int nr, double* out);
+
+// Applies any deferred CG work (fused mode).
+int apply_pending(tea_hip_chunk* c);
+
+// One reflective-halo launch (all flagged fields and external faces).
+int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
+                 const int fields[6]);
+
+inline dim3 span_grid(const tea_hip_chunk* c) { return dim3(c->nspan); }
+inline dim3 flat_grid(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+}  // namespace tea

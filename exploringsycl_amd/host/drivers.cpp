@@ -1,0 +1,175 @@
+// Solver drivers of the CG path (control flow of TeaLeaf/drivers/
+// cg_driver.c:7-124, halo_update_driver.c:6-25, solve_finished_driver.c:7-43,
+// field_summary_driver.c:8-91 and the one-call init drivers).  Every driver
+// loops over the rank's chunks and calls kernel_interface.h.
+#include <string.h>
+
+#include "comms.h"
+#include "drivers.h"
+#include "kernel_interface.h"
+#include "tea_host.h"
+
+#define FOR_CHUNKS(settings, cc) \
+  for (int cc = 0; cc < (settings)->num_chunks_per_rank; ++cc)
+
+void kernel_initialise_driver(Chunk* chunks, Settings* settings) {
+  FOR_CHUNKS(settings, cc) run_kernel_initialise(&chunks[cc], settings);
+}
+
+void kernel_finalise_driver(Chunk* chunks, Settings* settings) {
+  FOR_CHUNKS(settings, cc) run_kernel_finalise(&chunks[cc], settings);
+}
+
+void set_chunk_data_driver(Chunk* chunks, Settings* settings) {
+  FOR_CHUNKS(settings, cc) run_set_chunk_data(&chunks[cc], settings);
+}
+
+void set_chunk_state_driver(Chunk* chunks, Settings* settings, State* states) {
+  FOR_CHUNKS(settings, cc) run_set_chunk_state(&chunks[cc], settings, states);
+}
+
+void store_energy_driver(Chunk* chunks, Settings* settings) {
+  FOR_CHUNKS(settings, cc) run_store_energy(&chunks[cc], settings);
+}
+
+// One process per GPU owns one chunk whose faces are all external, so there
+// is nothing to exchange with other ranks.
+void remote_halo_driver(Chunk* chunks, Settings* settings, int depth) {
+  (void)chunks; (void)settings; (void)depth;
+}
+
+void halo_update_driver(Chunk* chunks, Settings* settings, int depth) {
+  if (!is_fields_to_exchange(settings)) return;
+  remote_halo_driver(chunks, settings, depth);
+  FOR_CHUNKS(settings, cc) run_local_halos(&chunks[cc], settings, depth);
+}
+
+// ---- CG -------------------------------------------------------------------
+
+void cg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+               double* error) {
+  double rro = 0.0;
+  cg_init_driver(chunks, settings, rx, ry, &rro);
+  int tt;
+  for (tt = 0; tt < settings->max_iters; ++tt) {
+    cg_main_step_driver(chunks, settings, tt, &rro, error);
+    halo_update_driver(chunks, settings, 1);
+    if (std::sqrt(std::fabs(*error)) < settings->eps) break;
+  }
+  print_and_log(settings, "CG: \t\t\t%d iterations\n", tt);
+  if (!g_run.steps.empty()) {
+    g_run.steps.back().iterations = tt;
+    g_run.steps.back().executed = tt < settings->max_iters ? tt + 1 : tt;
+  }
+}
+
+void cg_init_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+                    double* rro) {
+  *rro = 0.0;
+  FOR_CHUNKS(settings, cc) run_cg_init(&chunks[cc], settings, rx, ry, rro);
+  reset_fields_to_exchange(settings);
+  settings->fields_to_exchange[FIELD_U] = true;
+  settings->fields_to_exchange[FIELD_P] = true;
+  halo_update_driver(chunks, settings, 1);
+  sum_over_ranks(settings, rro);
+  FOR_CHUNKS(settings, cc) run_copy_u(&chunks[cc], settings);
+}
+
+void cg_main_step_driver(Chunk* chunks, Settings* settings, int tt,
+                         double* rro, double* error) {
+  double pw = 0.0;
+  FOR_CHUNKS(settings, cc) run_cg_calc_w(&chunks[cc], settings, &pw);
+  sum_over_ranks(settings, &pw);
+
+  const double alpha = *rro / pw;
+  double rrn = 0.0;
+  FOR_CHUNKS(settings, cc) {
+    chunks[cc].cg_alphas[tt] = alpha;
+    run_cg_calc_ur(&chunks[cc], settings, alpha, &rrn);
+  }
+  sum_over_ranks(settings, &rrn);
+
+  const double beta = rrn / *rro;
+  FOR_CHUNKS(settings, cc) {
+    chunks[cc].cg_betas[tt] = beta;
+    run_cg_calc_p(&chunks[cc], settings, beta);
+  }
+  *error = rrn;
+  *rro = rrn;
+}
+
+// ---- end of a solve ---------------------------------------------------------
+
+void solve_finished_driver(Chunk* chunks, Settings* settings) {
+  double exact_error = 0.0;
+  if (settings->check_result) {
+    FOR_CHUNKS(settings, cc) {
+      run_calculate_residual(&chunks[cc], settings);
+      run_calculate_2norm(&chunks[cc], settings, chunks[cc].r, &exact_error);
+    }
+    sum_over_ranks(settings, &exact_error);
+  }
+  FOR_CHUNKS(settings, cc) run_finalise(&chunks[cc], settings);
+  settings->fields_to_exchange[FIELD_ENERGY1] = true;
+  halo_update_driver(chunks, settings, 1);
+}
+
+// ---- field summary and the known-answer check -------------------------------
+
+// Looks up the expected temperature for (x_cells, y_cells, end_step) in the
+// tea.problems table; returns false (and leaves *value alone) if the file or
+// the row is missing.
+static bool get_checking_value(Settings* settings, double* value) {
+  FILE* fp = fopen(settings->test_problem_filename, "r");
+  if (!fp) {
+    print_and_log(settings, "\nWARNING: Could not open the test problem file.\n");
+    return false;
+  }
+  char line[512];
+  while (fgets(line, sizeof(line), fp)) {
+    int x = 0, y = 0, steps = 0;
+    double v = 0.0;
+    if (sscanf(line, "%d %d %d %lf", &x, &y, &steps, &v) != 4) continue;
+    if (x == settings->grid_x_cells && y == settings->grid_y_cells &&
+        steps == settings->end_step) {
+      *value = v;
+      fclose(fp);
+      return true;
+    }
+  }
+  fclose(fp);
+  print_and_log(settings,
+                "\nWARNING: Problem was not found in the test problems file.\n");
+  return false;
+}
+
+void field_summary_driver(Chunk* chunks, Settings* settings,
+                          bool is_solve_finished) {
+  double vol = 0.0, ie = 0.0, temp = 0.0, mass = 0.0;
+  FOR_CHUNKS(settings, cc)
+    run_field_summary(&chunks[cc], settings, &vol, &mass, &ie, &temp);
+  sum_over_ranks(settings, &vol);
+  sum_over_ranks(settings, &mass);
+  sum_over_ranks(settings, &ie);
+  sum_over_ranks(settings, &temp);
+
+  const int step = (int)g_run.steps.size();
+  g_run.summaries.push_back({step, is_solve_finished, vol, mass, ie, temp});
+  print_and_log(settings,
+                "Field summary (step %d): vol %.15e mass %.15e ie %.15e "
+                "temp %.15e\n",
+                step, vol, mass, ie, temp);
+
+  if (settings->rank == MASTER && settings->check_result && is_solve_finished) {
+    print_and_log(settings, "\nChecking results...\n");
+    double expected = 1.0;
+    g_run.check_found = get_checking_value(settings, &expected);
+    g_run.check_value = expected;
+    print_and_log(settings, "Expected %.15e\n", expected);
+    print_and_log(settings, "Actual   %.15e\n", temp);
+    const double qa_diff = std::fabs(100.0 * (temp / expected) - 100.0);
+    g_run.check_passed = qa_diff < 0.001;
+    print_and_log(settings, g_run.check_passed ? "This run PASSED" : "This run FAILED");
+    print_and_log(settings, " (Difference is within %.8lf%%)\n", qa_diff);
+  }
+}

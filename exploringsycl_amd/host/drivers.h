@@ -1,0 +1,31 @@
+/* Host-side solver drivers: each loops over the rank's chunks and calls the
+ * kernel_interface entry points (TeaLeaf/drivers/drivers.h:5-72 for the CG
+ * path).  The Chebyshev, PPCG and Jacobi solvers of the reference are not
+ * part of this build. */
+#pragma once
+#ifndef TEA_DRIVERS_H
+#define TEA_DRIVERS_H
+
+#include "chunk.h"
+
+void set_chunk_data_driver(Chunk* chunk, Settings* settings);
+void set_chunk_state_driver(Chunk* chunk, Settings* settings, State* states);
+void kernel_initialise_driver(Chunk* chunks, Settings* settings);
+void kernel_finalise_driver(Chunk* chunks, Settings* settings);
+
+void halo_update_driver(Chunk* chunks, Settings* settings, int depth);
+void remote_halo_driver(Chunk* chunks, Settings* settings, int depth);
+
+void cg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+               double* error);
+void cg_init_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+                    double* rro);
+void cg_main_step_driver(Chunk* chunks, Settings* settings, int tt,
+                         double* rro, double* error);
+
+void field_summary_driver(Chunk* chunks, Settings* settings,
+                          bool solve_finished);
+void store_energy_driver(Chunk* chunk, Settings* settings);
+void solve_finished_driver(Chunk* chunks, Settings* settings);
+
+#endif

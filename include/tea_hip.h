@@ -1,0 +1,219 @@
+/*
+ * tea_hip.h — C ABI of the MI355X (gfx950) TeaLeaf CG backend, libteahip.so.
+ *
+ * This is the thin layer under TeaLeaf's kernel_interface.h: every entry
+ * point here replaces one run_* function of the reference's plugin boundary
+ * (TeaLeaf/kernel_interface.h:13-71, implemented for SYCL in
+ * TeaLeaf/c_kernels/sycl/kernel_interface.cpp) with plain pointers, sizes and
+ * an opaque per-chunk handle instead of Chunk* / Settings*.  The C++ adapter
+ * exploringsycl_amd/csrc/kernel_interface_hip.cpp maps the 24 run_* symbols
+ * onto these calls, so the reference's own host code links against it
+ * unchanged.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative TEA_HIP_E* code on
+ *    failure; tea_hip_last_error() then describes the failure.  No entry point
+ *    exits the process (the adapter turns errors into the host's die()).
+ *  - Fields are row-major FP64 device arrays, index = kk + jj*x, with x and y
+ *    including the halo (x = x_cells + 2*halo_depth), owned by the handle.
+ *  - Scalar results (rro, pw, rrn, norm, field-summary sums) are ASSIGNED to
+ *    the out-parameter; callers that accumulate over chunks add them up.
+ *    They are computed with the reference's reduction tree
+ *    (TeaLeaf/c_kernels/sycl/sycl_shared.hpp:35-79) and are bitwise equal to
+ *    the FP64 CPU restatement of the reference kernels.
+ *  - Calls on one handle are serialised on one HIP stream; calls that return
+ *    a scalar synchronise with it.  A handle must not be used from two host
+ *    threads at once.
+ */
+#ifndef TEA_HIP_H
+#define TEA_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tea_hip_chunk tea_hip_chunk;
+
+/* Error codes. */
+#define TEA_HIP_OK 0
+#define TEA_HIP_EINVAL -1  /* bad argument (size, field id, depth, ...) */
+#define TEA_HIP_EHIP -2    /* a HIP runtime call failed */
+#define TEA_HIP_ENOMEM -3  /* device or pinned allocation failed */
+
+/* Field identifiers, in the order of the FieldBufferType members of
+ * TeaLeaf/chunk.h:39-66, plus the backend's comms buffer
+ * (TeaLeaf/c_kernels/sycl/chunk_extension.h:9). */
+enum {
+  TEA_FIELD_DENSITY0 = 0, TEA_FIELD_DENSITY, TEA_FIELD_ENERGY0,
+  TEA_FIELD_ENERGY, TEA_FIELD_U, TEA_FIELD_U0, TEA_FIELD_P, TEA_FIELD_R,
+  TEA_FIELD_MI, TEA_FIELD_W, TEA_FIELD_KX, TEA_FIELD_KY, TEA_FIELD_SD,
+  TEA_FIELD_CELL_X, TEA_FIELD_CELL_Y, TEA_FIELD_CELL_DX, TEA_FIELD_CELL_DY,
+  TEA_FIELD_VERTEX_DX, TEA_FIELD_VERTEX_DY, TEA_FIELD_VERTEX_X,
+  TEA_FIELD_VERTEX_Y, TEA_FIELD_VOLUME, TEA_FIELD_X_AREA, TEA_FIELD_Y_AREA,
+  TEA_FIELD_COMMS_BUFFER, TEA_NUM_FIELDS
+};
+
+/* Halo-exchange field slots, as indexed by Settings::fields_to_exchange
+ * (TeaLeaf/shared.h:39-44). */
+enum {
+  TEA_HALO_DENSITY = 0, TEA_HALO_ENERGY0 = 1, TEA_HALO_ENERGY1 = 2,
+  TEA_HALO_U = 3, TEA_HALO_P = 4, TEA_HALO_SD = 5, TEA_HALO_NUM = 6
+};
+
+/* Faces, as indexed by Chunk::neighbours; a negative neighbour is an
+ * external (reflective) face (TeaLeaf/shared.h:30-35). */
+enum { TEA_FACE_LEFT = 0, TEA_FACE_RIGHT = 1, TEA_FACE_BOTTOM = 2,
+       TEA_FACE_TOP = 3 };
+
+/* Conductivity models (TeaLeaf/shared.h:46-47). */
+enum { TEA_CONDUCTIVITY = 1, TEA_RECIP_CONDUCTIVITY = 2 };
+
+/* State geometries (TeaLeaf/settings.h:127-132). */
+enum { TEA_GEOMETRY_RECTANGULAR = 0, TEA_GEOMETRY_CIRCULAR = 1,
+       TEA_GEOMETRY_POINT = 2 };
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Creates the device state of one chunk of x*y cells (halo included) on
+ * device `device`: all 25 fields, zero-filled, the reduction workspace and
+ * the stream.  Replaces run_kernel_initialise
+ * (TeaLeaf/kernel_interface.h:18; kernel_initialise.cpp:31-80). */
+int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
+                         tea_hip_chunk** out);
+
+/* Frees everything the handle owns.  Replaces run_kernel_finalise
+ * (TeaLeaf/kernel_interface.h:20; kernel_initialise.cpp:83-126). */
+int tea_hip_chunk_destroy(tea_hip_chunk* chunk);
+
+/* Device address and element count of a field.  The address of
+ * TEA_FIELD_P can change after CG calls (the fused iteration double-buffers
+ * p); ask again after each call instead of caching it. */
+int tea_hip_field(tea_hip_chunk* chunk, int field, double** dev_ptr,
+                  size_t* count);
+
+/* Blocking copies between host memory and a field (count elements from the
+ * start of the field).  Pending deferred CG work is applied first. */
+int tea_hip_field_write(tea_hip_chunk* chunk, int field, const double* host,
+                        size_t count);
+int tea_hip_field_read(tea_hip_chunk* chunk, int field, double* host,
+                       size_t count);
+
+/* Waits for all work queued on the handle's stream. */
+int tea_hip_synchronize(tea_hip_chunk* chunk);
+
+/* Text of the last error on the calling thread ("" if none). */
+const char* tea_hip_last_error(void);
+
+/* ---- initialisation kernels -------------------------------------------- */
+
+/* Mesh: vertices, cell centres, volume and face areas.  x_min/y_min are the
+ * chunk's lower-left coordinates.  Replaces run_set_chunk_data
+ * (TeaLeaf/kernel_interface.h:14; set_chunk_data.cpp:8-64). */
+int tea_hip_set_chunk_data(tea_hip_chunk* chunk, double x_min, double y_min,
+                           double dx, double dy);
+
+/* Applies state 0 to every cell, then states 1..num_states-1 in order, and
+ * sets u = energy0*density.  Arrays are indexed by state; bounds are the
+ * already shrunk ones the host parser produces.  Replaces
+ * run_set_chunk_state (TeaLeaf/kernel_interface.h:16;
+ * set_chunk_state.cpp:8-92). */
+int tea_hip_set_chunk_state(tea_hip_chunk* chunk, int num_states,
+                            const double* density, const double* energy,
+                            const double* x_min, const double* y_min,
+                            const double* x_max, const double* y_max,
+                            const double* radius, const int* geometry);
+
+/* energy := energy0 on every cell.  Replaces run_store_energy
+ * (TeaLeaf/kernel_interface.h:27; store_energy.cpp:6-24). */
+int tea_hip_store_energy(tea_hip_chunk* chunk);
+
+/* ---- solver-wide kernels ----------------------------------------------- */
+
+/* Reflective halo update of depth `depth` (1 <= depth <= halo_depth) for
+ * every field whose fields_to_exchange slot is non-zero, on every face whose
+ * neighbour is negative; one launch covers all fields and faces and gives the
+ * same result as the reference's left, right, top, bottom order.  Replaces
+ * run_local_halos (TeaLeaf/kernel_interface.h:23;
+ * kernel_interface.cpp:100-127, local_halos.cpp:7-102). */
+int tea_hip_local_halos(tea_hip_chunk* chunk, int depth,
+                        const int neighbours[4],
+                        const int fields_to_exchange[6]);
+
+/* Interior sums vol = sum V, mass = sum V*rho, ie = sum V*rho*energy0,
+ * temp = sum V*rho*u.  Replaces run_field_summary
+ * (TeaLeaf/kernel_interface.h:29-31; field_summary.cpp:6-151). */
+int tea_hip_field_summary(tea_hip_chunk* chunk, double* vol, double* mass,
+                          double* ie, double* temp);
+
+/* ---- CG solver kernels -------------------------------------------------- */
+
+/* p = r = 0, u = energy*density, conductivity w, face coefficients kx, ky,
+ * then w = A u, r = u - w, p = r and *rro = r.p.  Replaces run_cg_init
+ * (TeaLeaf/kernel_interface.h:34-36; cg.cpp:7-134). */
+int tea_hip_cg_init(tea_hip_chunk* chunk, int coefficient, double rx,
+                    double ry, double* rro);
+
+/* w = A p on the interior and *pw = p.w.  Replaces run_cg_calc_w
+ * (TeaLeaf/kernel_interface.h:37-38; cg.cpp:137-195). */
+int tea_hip_cg_calc_w(tea_hip_chunk* chunk, double* pw);
+
+/* u += alpha p, r -= alpha w, *rrn = r.r.  Replaces run_cg_calc_ur
+ * (TeaLeaf/kernel_interface.h:39-40; cg.cpp:198-254). */
+int tea_hip_cg_calc_ur(tea_hip_chunk* chunk, double alpha, double* rrn);
+
+/* p = beta p + r.  Replaces run_cg_calc_p (TeaLeaf/kernel_interface.h:41-42;
+ * cg.cpp:257-281). */
+int tea_hip_cg_calc_p(tea_hip_chunk* chunk, double beta);
+
+/* ---- shared solver kernels --------------------------------------------- */
+
+/* u0 = u on the interior.  Replaces run_copy_u
+ * (TeaLeaf/kernel_interface.h:63-64; solver_methods.cpp:7-31). */
+int tea_hip_copy_u(tea_hip_chunk* chunk);
+
+/* r = u0 - A u on the interior.  Replaces run_calculate_residual
+ * (TeaLeaf/kernel_interface.h:65-66; solver_methods.cpp:34-65). */
+int tea_hip_calculate_residual(tea_hip_chunk* chunk);
+
+/* *norm = sum over the interior of b^2, where b is one of the handle's
+ * fields given by its device address (as returned by tea_hip_field).
+ * Replaces run_calculate_2norm (TeaLeaf/kernel_interface.h:67-68;
+ * solver_methods.cpp:68-117). */
+int tea_hip_calculate_2norm(tea_hip_chunk* chunk, const double* buffer,
+                            double* norm);
+
+/* energy = u / density on the interior.  Replaces run_finalise
+ * (TeaLeaf/kernel_interface.h:69-70; solver_methods.cpp:120-145). */
+int tea_hip_finalise(tea_hip_chunk* chunk);
+
+/* ---- execution mode and measurement ------------------------------------ */
+
+/* Mode 1 (default): CG calls are fused.  tea_hip_cg_calc_ur updates r and
+ * returns r.r at once but defers u += alpha p; tea_hip_cg_calc_p and the
+ * depth-1 U/P halo update that follows it are deferred too, and the next
+ * tea_hip_cg_calc_w applies all of them in the same pass that computes
+ * w = A p (88 bytes per cell per iteration instead of 104).  Every other
+ * entry point applies pending work first, so results are bitwise those of
+ * mode 0.  Mode 0: one launch per reference kernel. */
+int tea_hip_set_fused(tea_hip_chunk* chunk, int fused);
+
+/* Applies any deferred CG work now. */
+int tea_hip_flush(tea_hip_chunk* chunk);
+
+/* Kernel timing with HIP events on the handle's stream (off by default). */
+int tea_hip_timing_enable(tea_hip_chunk* chunk, int enable);
+/* Launch count and summed device time of kernel `kernel_id`
+ * (0 <= kernel_id < tea_hip_num_kernels()). */
+int tea_hip_kernel_stats(tea_hip_chunk* chunk, int kernel_id,
+                         long* launches, double* total_ms);
+int tea_hip_timing_reset(tea_hip_chunk* chunk);
+int tea_hip_num_kernels(void);
+const char* tea_hip_kernel_name(int kernel_id);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,176 @@
+// kernel_interface.h entry points over the CPU oracle (tea_oracle.c).
+// TEST INFRASTRUCTURE ONLY: linked into oracle/build/tealeaf_oracle (this
+// project's host) and oracle/_ref/tealeaf_refhost_oracle (the reference's
+// own host), never into the product.
+//
+// Mirrors TeaLeaf/c_kernels/sycl/kernel_interface.cpp: the same kernels in
+// the same order for each entry point; scalar results are added into the
+// driver's accumulators.
+#include <stdlib.h>
+#include <string.h>
+
+#include "kernel_interface.h"
+#include "tea_oracle.h"
+
+static double* alloc_field(size_t n) {
+  double* p = (double*)calloc(n ? n : 1, sizeof(double));
+  if (!p) die(__LINE__, __FILE__, "Error allocating buffer\n");
+  return p;
+}
+
+void run_kernel_initialise(Chunk* c, Settings* s) {
+  const size_t x = c->x, y = c->y, xy = x * y;
+  c->density0 = alloc_field(xy);
+  c->density = alloc_field(xy);
+  c->energy0 = alloc_field(xy);
+  c->energy = alloc_field(xy);
+  c->u = alloc_field(xy);
+  c->u0 = alloc_field(xy);
+  c->p = alloc_field(xy);
+  c->r = alloc_field(xy);
+  c->mi = alloc_field(xy);
+  c->w = alloc_field(xy);
+  c->kx = alloc_field(xy);
+  c->ky = alloc_field(xy);
+  c->sd = alloc_field(xy);
+  c->volume = alloc_field(xy);
+  c->x_area = alloc_field((x + 1) * y);
+  c->y_area = alloc_field(x * (y + 1));
+  c->cell_x = alloc_field(x);
+  c->cell_y = alloc_field(y);
+  c->cell_dx = alloc_field(x);
+  c->cell_dy = alloc_field(y);
+  c->vertex_dx = alloc_field(x + 1);
+  c->vertex_dy = alloc_field(y + 1);
+  c->vertex_x = alloc_field(x + 1);
+  c->vertex_y = alloc_field(y + 1);
+  c->ext->comms_buffer = alloc_field((x > y ? x : y) * s->halo_depth);
+  const int n = s->max_iters > 0 ? s->max_iters : 1;
+  c->cg_alphas = new double[n]();
+  c->cg_betas = new double[n]();
+  c->cheby_alphas = new double[n]();
+  c->cheby_betas = new double[n]();
+}
+
+void run_kernel_finalise(Chunk* c, Settings* s) {
+  (void)s;
+  double* f[] = {c->density0, c->density, c->energy0, c->energy, c->u,
+                 c->u0, c->p, c->r, c->mi, c->w, c->kx, c->ky, c->sd,
+                 c->volume, c->x_area, c->y_area, c->cell_x, c->cell_y,
+                 c->cell_dx, c->cell_dy, c->vertex_dx, c->vertex_dy,
+                 c->vertex_x, c->vertex_y, c->ext->comms_buffer};
+  for (double* p : f) free(p);
+  delete[] c->cg_alphas;
+  delete[] c->cg_betas;
+  delete[] c->cheby_alphas;
+  delete[] c->cheby_betas;
+}
+
+void run_set_chunk_data(Chunk* c, Settings* s) {
+  const double x_min = s->grid_x_min + s->dx * (double)c->left;
+  const double y_min = s->grid_y_min + s->dy * (double)c->bottom;
+  orc_set_chunk_data(c->x, c->y, s->halo_depth, x_min, y_min, s->dx, s->dy,
+                     c->vertex_x, c->vertex_y, c->cell_x, c->cell_y, c->volume,
+                     c->x_area, c->y_area);
+}
+
+void run_set_chunk_state(Chunk* c, Settings* s, State* states) {
+  orc_set_chunk_initial_state(c->x, c->y, states[0].energy, states[0].density,
+                              c->energy0, c->density);
+  for (int ii = 1; ii < s->num_states; ++ii) {
+    const State& st = states[ii];
+    const int geom = st.geometry == RECTANGULAR ? ORC_RECTANGULAR
+                     : st.geometry == CIRCULAR  ? ORC_CIRCULAR
+                                                : ORC_POINT;
+    orc_set_chunk_state(c->x, c->y, geom, st.x_min, st.y_min, st.x_max,
+                        st.y_max, st.radius, st.density, st.energy,
+                        c->energy0, c->density, c->u, c->cell_x, c->cell_y,
+                        c->vertex_x, c->vertex_y);
+  }
+}
+
+void run_local_halos(Chunk* c, Settings* s, int depth) {
+  // Field order of kernel_interface.cpp:120-125.
+  const int order[6] = {FIELD_DENSITY, FIELD_P, FIELD_ENERGY0, FIELD_ENERGY1,
+                        FIELD_U, FIELD_SD};
+  for (int o = 0; o < 6; ++o) {
+    const int f = order[o];
+    if (!s->fields_to_exchange[f]) continue;
+    double* b = f == FIELD_DENSITY   ? c->density
+                : f == FIELD_P       ? c->p
+                : f == FIELD_ENERGY0 ? c->energy0
+                : f == FIELD_ENERGY1 ? c->energy
+                : f == FIELD_U       ? c->u
+                                     : c->sd;
+    orc_halo_field(c->x, c->y, s->halo_depth, depth, c->neighbours, b);
+  }
+}
+
+void run_pack_or_unpack(Chunk* c, Settings* s, int depth, int face, bool pack,
+                        FieldBufferType field, double* buffer) {
+  (void)c; (void)s; (void)depth; (void)face; (void)pack; (void)field;
+  (void)buffer;
+  die(__LINE__, __FILE__, "run_pack_or_unpack: not part of the oracle\n");
+}
+
+void run_store_energy(Chunk* c, Settings* s) {
+  (void)s;
+  orc_store_energy(c->x, c->y, c->energy, c->energy0);
+}
+
+void run_field_summary(Chunk* c, Settings* s, double* vol, double* mass,
+                       double* ie, double* temp) {
+  double out[4];
+  orc_field_summary(c->x, c->y, s->halo_depth, c->volume, c->density,
+                    c->energy0, c->u, out);
+  *vol += out[0];
+  *mass += out[1];
+  *ie += out[2];
+  *temp += out[3];
+}
+
+void run_cg_init(Chunk* c, Settings* s, double rx, double ry, double* rro) {
+  *rro += orc_cg_init(c->x, c->y, s->halo_depth, s->coefficient, rx, ry, c->p,
+                      c->r, c->u, c->w, c->kx, c->ky, c->density, c->energy);
+}
+
+void run_cg_calc_w(Chunk* c, Settings* s, double* pw) {
+  *pw += orc_cg_calc_w(c->x, c->y, s->halo_depth, c->w, c->p, c->kx, c->ky);
+}
+
+void run_cg_calc_ur(Chunk* c, Settings* s, double alpha, double* rrn) {
+  *rrn += orc_cg_calc_ur(c->x, c->y, s->halo_depth, alpha, c->u, c->r, c->p,
+                         c->w);
+}
+
+void run_cg_calc_p(Chunk* c, Settings* s, double beta) {
+  orc_cg_calc_p(c->x, c->y, s->halo_depth, beta, c->p, c->r);
+}
+
+static void not_built(const char* what) {
+  die(__LINE__, __FILE__, "%s is not part of the oracle\n", what);
+}
+void run_cheby_init(Chunk*, Settings*) { not_built("Chebyshev"); }
+void run_cheby_iterate(Chunk*, Settings*, double, double) { not_built("Chebyshev"); }
+void run_jacobi_init(Chunk*, Settings*, double, double) { not_built("Jacobi"); }
+void run_jacobi_iterate(Chunk*, Settings*, double*) { not_built("Jacobi"); }
+void run_ppcg_init(Chunk*, Settings*) { not_built("PPCG"); }
+void run_ppcg_inner_iteration(Chunk*, Settings*, double, double) { not_built("PPCG"); }
+
+void run_copy_u(Chunk* c, Settings* s) {
+  orc_copy_u(c->x, c->y, s->halo_depth, c->u, c->u0);
+}
+
+void run_calculate_residual(Chunk* c, Settings* s) {
+  orc_calculate_residual(c->x, c->y, s->halo_depth, c->u, c->u0, c->r, c->kx,
+                         c->ky);
+}
+
+void run_calculate_2norm(Chunk* c, Settings* s, FieldBufferType buffer,
+                         double* norm) {
+  *norm += orc_calculate_2norm(c->x, c->y, s->halo_depth, buffer);
+}
+
+void run_finalise(Chunk* c, Settings* s) {
+  orc_finalise(c->x, c->y, s->halo_depth, c->u, c->density, c->energy);
+}

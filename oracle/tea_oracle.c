@@ -1,0 +1,366 @@
+/*
+ * CPU oracle for the TeaLeaf CG path: restatement of the reference SYCL
+ * backend's FP64 arithmetic.  TEST INFRASTRUCTURE ONLY (see tea_oracle.h).
+ *
+ * Reference files (relative to the reference repository):
+ *   TeaLeaf/c_kernels/sycl/cg.cpp              cg_init_u/_k/_others, calc_w/ur/p
+ *   TeaLeaf/c_kernels/sycl/solver_methods.cpp  copy_u, residual, 2norm, finalise
+ *   TeaLeaf/c_kernels/sycl/local_halos.cpp     update_left/right/top/bottom
+ *   TeaLeaf/c_kernels/sycl/field_summary.cpp   field_summary_func (+_reduction)
+ *   TeaLeaf/c_kernels/sycl/set_chunk_data.cpp  mesh
+ *   TeaLeaf/c_kernels/sycl/set_chunk_state.cpp states
+ *   TeaLeaf/c_kernels/sycl/store_energy.cpp    energy := energy0
+ *   TeaLeaf/c_kernels/sycl/sycl_shared.hpp:35-79  reduceArray tree
+ *   TeaLeaf/shared.h:59-63                     SMVP macro
+ *
+ * How each function is pinned (checked by tests/test_oracle_*.py):
+ *   - whole path: the known answers of TeaLeaf/tea.problems (temperature at
+ *     the last step, 1e-5 relative as the reference host checks, and ~1e-10
+ *     against the values published to 15 digits), and the Fortran field
+ *     summaries of TeaLeaf/Benchmarks/tea_bm_1.out;
+ *   - the reference's own host driver (TeaLeaf/ and TeaLeaf/drivers/ sources) compiled
+ *     against this oracle reproduces the known answers (oracle/Makefile, _ref);
+ *   - orc_smvp / orc_cg_init: symmetry x'Ay = y'Ax, row sums of one, and a
+ *     dense numpy solve on a 10x10 grid against the CG result;
+ *   - halos: exact mirror identities;
+ *   - field summary: closed-form volume/mass/ie/temp for grids on which the
+ *     tea.in rectangles fall on cell edges;
+ *   - reduction tree: inputs whose pairwise-tree sum differs from the
+ *     sequential sum by a value worked out by hand, and exact integer sums.
+ * No function is left unpinned.
+ */
+#include "tea_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define GROUP 64
+
+static inline int is_interior(long kk, long jj, int x, int y, int h) {
+  return kk >= h && kk < x - h && jj >= h && jj < y - h;
+}
+
+/* Pairwise tree of sycl_shared.hpp:59-64 / cg.cpp:177-183. */
+double orc_tree64(double* a) {
+  for (int s = 1; s < GROUP; s *= 2)
+    for (int i = 0; i + s < GROUP; i += 2 * s) a[i] = a[i] + a[i + s];
+  return a[0];
+}
+
+/* reduceArray (sycl_shared.hpp:35-79): repeated 64-wide trees over the
+ * partials, zero padded, until one value is left.  `v` is consumed. */
+static double reduce_partials(double* v, size_t len) {
+  while (len != 1) {
+    size_t groups = (len + GROUP - 1) / GROUP;
+    for (size_t g = 0; g < groups; ++g) {
+      double a[GROUP];
+      for (int l = 0; l < GROUP; ++l) {
+        size_t i = g * GROUP + l;
+        a[l] = i < len ? v[i] : 0.0;
+      }
+      v[g] = orc_tree64(a); /* g <= g*64: groups are done in order */
+    }
+    len = groups;
+  }
+  return v[0];
+}
+
+double orc_tree_sum(const double* v, size_t n) {
+  if (n == 0) return 0.0;
+  size_t groups = (n + GROUP - 1) / GROUP;
+  double* part = (double*)malloc(groups * sizeof(double));
+  for (size_t g = 0; g < groups; ++g) {
+    double a[GROUP];
+    for (int l = 0; l < GROUP; ++l) {
+      size_t i = g * GROUP + l;
+      a[l] = i < n ? v[i] : 0.0;
+    }
+    part[g] = orc_tree64(a);
+  }
+  double r = reduce_partials(part, groups);
+  free(part);
+  return r;
+}
+
+/* shared.h:59-63, evaluated in the macro's order with every product and sum
+ * rounded separately. */
+double orc_smvp(const double* a, const double* kx, const double* ky, long i,
+                long x) {
+  const double t1 = kx[i + 1] + kx[i];
+  const double t2 = ky[i + x] + ky[i];
+  const double diag = (1.0 + t1) + t2;
+  const double c = diag * a[i];
+  const double bx = kx[i + 1] * a[i + 1] + kx[i] * a[i - 1];
+  const double by = ky[i + x] * a[i + x] + ky[i] * a[i - x];
+  return (c - bx) - by;
+}
+
+/* Walk the padded linear range group by group: BODY sets `val` for interior
+ * cell (kk, jj) at linear index i; the group's tree goes to part[g]. */
+#define GROUP_REDUCE(x, y, h, part, BODY)                                     \
+  do {                                                                        \
+    const long n_ = (long)(x) * (long)(y);                                    \
+    const long groups_ = (n_ + GROUP - 1) / GROUP;                            \
+    _Pragma("omp parallel for schedule(static)")                              \
+    for (long g = 0; g < groups_; ++g) {                                      \
+      double a_[GROUP];                                                       \
+      long i0_ = g * GROUP;                                                   \
+      long jj = i0_ / (x);                                                    \
+      long kk = i0_ - jj * (x);                                               \
+      for (int l = 0; l < GROUP; ++l) {                                       \
+        const long i = i0_ + l;                                               \
+        double val = 0.0;                                                     \
+        if (i < n_ && is_interior(kk, jj, (x), (y), (h))) { BODY }            \
+        a_[l] = val;                                                          \
+        if (++kk == (x)) { kk = 0; ++jj; }                                    \
+      }                                                                       \
+      (part)[g] = orc_tree64(a_);                                             \
+    }                                                                         \
+  } while (0)
+
+static double* alloc_partials(int x, int y) {
+  long n = (long)x * y;
+  return (double*)malloc(((n + GROUP - 1) / GROUP) * sizeof(double));
+}
+
+static double finish(double* part, int x, int y) {
+  long n = (long)x * y;
+  double r = reduce_partials(part, (size_t)((n + GROUP - 1) / GROUP));
+  free(part);
+  return r;
+}
+
+/* set_chunk_data.cpp:8-64 */
+void orc_set_chunk_data(int x, int y, int h, double x_min, double y_min,
+                        double dx, double dy, double* vertex_x,
+                        double* vertex_y, double* cell_x, double* cell_y,
+                        double* volume, double* x_area, double* y_area) {
+  for (long i = 0; i < x + 1; ++i)
+    vertex_x[i] = x_min + dx * ((double)i - (double)h);
+  for (long i = 0; i < y + 1; ++i)
+    vertex_y[i] = y_min + dy * ((double)i - (double)h);
+  for (long i = 0; i < x; ++i) cell_x[i] = 0.5 * (vertex_x[i] + vertex_x[i + 1]);
+  for (long i = 0; i < y; ++i) cell_y[i] = 0.5 * (vertex_y[i] + vertex_y[i + 1]);
+  const long n = (long)x * y;
+  const double vol = dx * dy;
+  for (long i = 0; i < n; ++i) {
+    volume[i] = vol;
+    x_area[i] = dy;
+    y_area[i] = dx;
+  }
+}
+
+/* set_chunk_state.cpp:8-27 */
+void orc_set_chunk_initial_state(int x, int y, double energy, double density,
+                                 double* energy0, double* density_f) {
+  const long n = (long)x * y;
+  for (long i = 0; i < n; ++i) {
+    energy0[i] = energy;
+    density_f[i] = density;
+  }
+}
+
+/* set_chunk_state.cpp:30-92 */
+void orc_set_chunk_state(int x, int y, int geometry, double sx_min,
+                         double sy_min, double sx_max, double sy_max,
+                         double radius, double s_density, double s_energy,
+                         double* energy0, double* density, double* u,
+                         const double* cell_x, const double* cell_y,
+                         const double* vertex_x, const double* vertex_y) {
+  for (long jj = 0; jj < y; ++jj) {
+    for (long kk = 0; kk < x; ++kk) {
+      const long i = kk + jj * x;
+      int apply = 0;
+      if (geometry == ORC_RECTANGULAR) {
+        apply = vertex_x[kk + 1] >= sx_min && vertex_x[kk] < sx_max &&
+                vertex_y[jj + 1] >= sy_min && vertex_y[jj] < sy_max;
+      } else if (geometry == ORC_CIRCULAR) {
+        const double ddx = cell_x[kk] - sx_min;
+        const double ddy = cell_y[jj] - sy_min;
+        apply = sqrt(ddx * ddx + ddy * ddy) <= radius;
+      } else if (geometry == ORC_POINT) {
+        apply = vertex_x[kk] == sx_min && vertex_y[jj] == sy_min;
+      }
+      if (apply) {
+        energy0[i] = s_energy;
+        density[i] = s_density;
+      }
+      if (kk > 0 && kk < x - 1 && jj > 0 && jj < y - 1)
+        u[i] = energy0[i] * density[i];
+    }
+  }
+}
+
+/* store_energy.cpp:6-24 */
+void orc_store_energy(int x, int y, double* energy, const double* energy0) {
+  memcpy(energy, energy0, sizeof(double) * (size_t)x * (size_t)y);
+}
+
+/* local_halos.cpp:7-102, applied in kernel_interface.cpp:100-127's face
+ * order.  Written with (row, column) indices rather than the reference's
+ * flattened ones; the copies are the same. */
+void orc_halo_field(int x, int y, int h, int depth, const int neighbours[4],
+                    double* b) {
+  if (neighbours[ORC_LEFT] < 0)
+    for (long jj = 0; jj < y; ++jj)
+      for (int k = 0; k < depth; ++k)
+        b[jj * x + (h - 1 - k)] = b[jj * x + (h + k)];
+  if (neighbours[ORC_RIGHT] < 0)
+    for (long jj = 0; jj < y; ++jj)
+      for (int k = 0; k < depth; ++k)
+        b[jj * x + (x - h + k)] = b[jj * x + (x - h - 1 - k)];
+  if (neighbours[ORC_TOP] < 0)
+    for (int k = 0; k < depth; ++k)
+      for (long kk = 0; kk < x; ++kk)
+        b[(long)(y - h + k) * x + kk] = b[(long)(y - h - 1 - k) * x + kk];
+  if (neighbours[ORC_BOTTOM] < 0)
+    for (int k = 0; k < depth; ++k)
+      for (long kk = 0; kk < x; ++kk)
+        b[(long)(h - 1 - k) * x + kk] = b[(long)(h + k) * x + kk];
+}
+
+/* field_summary.cpp:6-151: four trees over the same index range. */
+void orc_field_summary(int x, int y, int h, const double* volume,
+                       const double* density, const double* energy0,
+                       const double* u, double out[4]) {
+  const long n = (long)x * y;
+  const long groups = (n + GROUP - 1) / GROUP;
+  double* part[4];
+  for (int q = 0; q < 4; ++q) part[q] = (double*)malloc(groups * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (long g = 0; g < groups; ++g) {
+    double a[4][GROUP];
+    long i0 = g * GROUP;
+    long jj = i0 / x, kk = i0 - jj * x;
+    for (int l = 0; l < GROUP; ++l) {
+      const long i = i0 + l;
+      a[0][l] = a[1][l] = a[2][l] = a[3][l] = 0.0;
+      if (i < n && is_interior(kk, jj, x, y, h)) {
+        const double cell_vol = volume[i];
+        const double cell_mass = cell_vol * density[i];
+        a[0][l] = cell_vol;
+        a[1][l] = cell_mass;
+        a[2][l] = cell_mass * energy0[i];
+        a[3][l] = cell_mass * u[i];
+      }
+      if (++kk == x) { kk = 0; ++jj; }
+    }
+    for (int q = 0; q < 4; ++q) part[q][g] = orc_tree64(a[q]);
+  }
+  for (int q = 0; q < 4; ++q) {
+    out[q] = reduce_partials(part[q], (size_t)groups);
+    free(part[q]);
+  }
+}
+
+/* cg.cpp:7-134 (cg_init_u, cg_init_k, cg_init_others). */
+double orc_cg_init(int x, int y, int h, int coefficient, double rx, double ry,
+                   double* p, double* r, double* u, double* w, double* kx,
+                   double* ky, const double* density, const double* energy) {
+  const long n = (long)x * y;
+  /* cg_init_u (cg.cpp:7-37) */
+#pragma omp parallel for schedule(static)
+  for (long jj = 0; jj < y; ++jj) {
+    for (long kk = 0; kk < x; ++kk) {
+      const long i = kk + jj * x;
+      p[i] = 0.0;
+      r[i] = 0.0;
+      u[i] = energy[i] * density[i];
+      if (jj > 0 && jj < y - 1 && kk > 0 && kk < x - 1)
+        w[i] = coefficient == ORC_CONDUCTIVITY ? density[i] : 1.0 / density[i];
+    }
+  }
+  /* cg_init_k (cg.cpp:40-69) */
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - 1; ++jj) {
+    for (long kk = h; kk < x - 1; ++kk) {
+      const long i = kk + jj * x;
+      kx[i] = rx * (w[i - 1] + w[i]) / (2.0 * w[i - 1] * w[i]);
+      ky[i] = ry * (w[i - x] + w[i]) / (2.0 * w[i - x] * w[i]);
+    }
+  }
+  (void)n;
+  /* cg_init_others (cg.cpp:72-134) */
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, {
+    const double s = orc_smvp(u, kx, ky, i, x);
+    w[i] = s;
+    r[i] = u[i] - w[i];
+    p[i] = r[i];
+    val = r[i] * p[i];
+  });
+  return finish(part, x, y);
+}
+
+/* cg.cpp:137-195 */
+double orc_cg_calc_w(int x, int y, int h, double* w, const double* p,
+                     const double* kx, const double* ky) {
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, {
+    const double s = orc_smvp(p, kx, ky, i, x);
+    w[i] = s;
+    val = w[i] * p[i];
+  });
+  return finish(part, x, y);
+}
+
+/* cg.cpp:198-254 */
+double orc_cg_calc_ur(int x, int y, int h, double alpha, double* u, double* r,
+                      const double* p, const double* w) {
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, {
+    u[i] = u[i] + alpha * p[i];
+    r[i] = r[i] - alpha * w[i];
+    val = r[i] * r[i];
+  });
+  return finish(part, x, y);
+}
+
+/* cg.cpp:257-281 */
+void orc_cg_calc_p(int x, int y, int h, double beta, double* p,
+                   const double* r) {
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) {
+      const long i = kk + jj * x;
+      p[i] = beta * p[i] + r[i];
+    }
+}
+
+/* solver_methods.cpp:7-31 */
+void orc_copy_u(int x, int y, int h, const double* u, double* u0) {
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) u0[kk + jj * x] = u[kk + jj * x];
+}
+
+/* solver_methods.cpp:34-65 */
+void orc_calculate_residual(int x, int y, int h, const double* u,
+                            const double* u0, double* r, const double* kx,
+                            const double* ky) {
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) {
+      const long i = kk + jj * x;
+      r[i] = u0[i] - orc_smvp(u, kx, ky, i, x);
+    }
+}
+
+/* solver_methods.cpp:68-117 */
+double orc_calculate_2norm(int x, int y, int h, const double* b) {
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, { val = b[i] * b[i]; });
+  return finish(part, x, y);
+}
+
+/* solver_methods.cpp:120-145 */
+void orc_finalise(int x, int y, int h, const double* u, const double* density,
+                  double* energy) {
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) {
+      const long i = kk + jj * x;
+      energy[i] = u[i] / density[i];
+    }
+}

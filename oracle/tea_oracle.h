@@ -1,0 +1,107 @@
+/*
+ * CPU oracle for the TeaLeaf conjugate-gradient path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the
+ * kernel_interface adapter or the host driver) may link or call this code;
+ * only tests/, the smoke check and the CPU-baseline leg of bench.py use it,
+ * and only as the checker or the timed CPU reference.
+ *
+ * Every function restates, in plain C on host arrays, the FP64 arithmetic of
+ * one SYCL kernel of the reference backend (paths relative to the reference
+ * repository, cited per function in tea_oracle.c).  Arrays are row-major with
+ * index = kk + jj*x, x and y include the halo (x = x_cells + 2*halo_depth).
+ *
+ * Reductions follow the reference's tree exactly (sycl_shared.hpp:35-79 and
+ * the first-level trees inside cg.cpp / solver_methods.cpp /
+ * field_summary.cpp): the padded linear index range [0, 64*ceil(x*y/64)) is
+ * cut into groups of 64, every non-interior slot contributes +0.0, each group
+ * is summed by the pairwise tree "for s=1,2,..,32: a[2s*l] += a[2s*l+s]", and
+ * the partials are summed again by the same tree, zero-padded to 64, until one
+ * value is left.  The build compiles this file with -ffp-contract=off so each
+ * multiply and add rounds on its own, as written.
+ *
+ * Scalar results are RETURNED (assigned); the kernel_interface adapters add
+ * them into the driver's accumulators.
+ *
+ * Parity status: see the header of tea_oracle.c.
+ */
+#ifndef TEA_ORACLE_H
+#define TEA_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_CONDUCTIVITY = 1, ORC_RECIP_CONDUCTIVITY = 2 };
+enum { ORC_RECTANGULAR = 0, ORC_CIRCULAR = 1, ORC_POINT = 2 };
+enum { ORC_LEFT = 0, ORC_RIGHT = 1, ORC_BOTTOM = 2, ORC_TOP = 3 };
+enum {
+  ORC_FIELD_DENSITY = 0, ORC_FIELD_ENERGY0 = 1, ORC_FIELD_ENERGY1 = 2,
+  ORC_FIELD_U = 3, ORC_FIELD_P = 4, ORC_FIELD_SD = 5, ORC_NUM_FIELDS = 6
+};
+
+/* The reference's reduction tree over an arbitrary array (zero padded). */
+double orc_tree_sum(const double* v, size_t n);
+/* One 64-wide pairwise tree, in place; returns a[0]. */
+double orc_tree64(double* a);
+
+/* Mesh and state setup (set_chunk_data.cpp, set_chunk_state.cpp,
+ * store_energy.cpp). */
+void orc_set_chunk_data(int x, int y, int h, double x_min, double y_min,
+                        double dx, double dy, double* vertex_x,
+                        double* vertex_y, double* cell_x, double* cell_y,
+                        double* volume, double* x_area, double* y_area);
+void orc_set_chunk_initial_state(int x, int y, double energy, double density,
+                                 double* energy0, double* density_f);
+void orc_set_chunk_state(int x, int y, int geometry, double sx_min,
+                         double sy_min, double sx_max, double sy_max,
+                         double radius, double s_density, double s_energy,
+                         double* energy0, double* density, double* u,
+                         const double* cell_x, const double* cell_y,
+                         const double* vertex_x, const double* vertex_y);
+void orc_store_energy(int x, int y, double* energy, const double* energy0);
+
+/* Reflective halo copies for one buffer, faces in the reference order
+ * left, right, top, bottom; a face is updated only if neighbours[face] < 0
+ * (EXTERNAL_FACE). */
+void orc_halo_field(int x, int y, int h, int depth, const int neighbours[4],
+                    double* b);
+
+/* field_summary.cpp: out = {vol, mass, ie, temp}. */
+void orc_field_summary(int x, int y, int h, const double* volume,
+                       const double* density, const double* energy0,
+                       const double* u, double out[4]);
+
+/* cg.cpp: returns rro. */
+double orc_cg_init(int x, int y, int h, int coefficient, double rx, double ry,
+                   double* p, double* r, double* u, double* w, double* kx,
+                   double* ky, const double* density, const double* energy);
+/* cg.cpp: returns pw. */
+double orc_cg_calc_w(int x, int y, int h, double* w, const double* p,
+                     const double* kx, const double* ky);
+/* cg.cpp: returns rrn. */
+double orc_cg_calc_ur(int x, int y, int h, double alpha, double* u, double* r,
+                      const double* p, const double* w);
+void orc_cg_calc_p(int x, int y, int h, double beta, double* p,
+                   const double* r);
+
+/* solver_methods.cpp */
+void orc_copy_u(int x, int y, int h, const double* u, double* u0);
+void orc_calculate_residual(int x, int y, int h, const double* u,
+                            const double* u0, double* r, const double* kx,
+                            const double* ky);
+double orc_calculate_2norm(int x, int y, int h, const double* b);
+void orc_finalise(int x, int y, int h, const double* u, const double* density,
+                  double* energy);
+
+/* The 5-point operator applied at one cell (shared.h:59-63). */
+double orc_smvp(const double* a, const double* kx, const double* ky, long i,
+                long x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,74 @@
+"""The whole program on the GPU: bin/tealeaf (this host + the HIP backend)
+against the same host on the CPU oracle (bitwise field summaries and equal
+per-step iteration counts), and against TeaLeaf's known answers."""
+
+import os
+import subprocess
+
+import pytest
+
+from exploringsycl_amd.deck import Deck, config_deck, problem_seed
+from exploringsycl_amd.runner import run_deck
+from tests.oracle_ctypes import ORACLE_TEALEAF, REFHOST_HIP
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(gpu, cpu):
+    assert [s["iterations"] for s in gpu["steps"]] == [s["iterations"] for s in cpu["steps"]]
+    assert len(gpu["summaries"]) == len(cpu["summaries"])
+    for a, b in zip(gpu["summaries"], cpu["summaries"]):
+        for k in ("vol", "mass", "ie", "temp"):
+            assert a[k] == b[k], (k, a, b)
+
+
+DECKS = {
+    "bm1_10x10x10": Deck(10, 10, end_step=10),
+    "tea_64x64x1": Deck(64, 64, end_step=1),
+    "bm2_250x250x10": Deck(250, 250, end_step=10),
+    "config1_512x512x10": config_deck(1),
+    "recip_200x300_rects": Deck.with_random_rectangles(
+        200, 300, 16, problem_seed(2), end_step=3, coefficient="inverse"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DECKS))
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_matches_oracle_host(name, fused):
+    deck = DECKS[name]
+    gpu, _ = run_deck(deck, env={"TEA_HIP_FUSED": fused})
+    cpu, _ = run_deck(deck, binary=ORACLE_TEALEAF, env={"OMP_NUM_THREADS": "16"})
+    _compare(gpu, cpu)
+
+
+@pytest.mark.parametrize("cells,steps", [(10, 10), (64, 1), (250, 10), (500, 10),
+                                         (512, 20), (1000, 10), (1024, 20),
+                                         (4096, 1)])
+def test_known_answers(cells, steps):
+    """TeaLeaf/tea.problems rows: the host's own 1e-5 check must pass, and the
+    temperature must agree with the published value to 1e-9 relative."""
+    rec, _ = run_deck(Deck(cells, cells, end_step=steps), timeout=900)
+    assert rec["check_found"] and rec["check_passed"]
+    temp = rec["summaries"][-1]["temp"]
+    assert abs(temp / rec["check_value"] - 1.0) < 1e-9, (temp, rec["check_value"])
+
+
+@pytest.mark.skipif(not os.path.exists(REFHOST_HIP),
+                    reason="reference host build needs the reference tree")
+def test_reference_host_drives_hip_backend(tmp_path):
+    """The reference's own host code (compiled unchanged) linked to the HIP
+    backend through kernel_interface.h: same iterations and temperature as
+    this project's host."""
+    deck = DECKS["bm2_250x250x10"]
+    ours, _ = run_deck(deck)
+    deck.write(tmp_path / "tea.in")
+    from exploringsycl_amd.runner import TEA_PROBLEMS
+    import shutil
+    shutil.copy(TEA_PROBLEMS, tmp_path / "tea.problems")
+    p = subprocess.run([REFHOST_HIP], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    iters = [int(l.split()[1]) for l in p.stdout.splitlines() if l.startswith("CG:")]
+    assert iters == [s["iterations"] for s in ours["steps"]]
+    actual = [l for l in p.stdout.splitlines() if l.startswith("Actual")]
+    assert float(actual[-1].split()[1]) == float("%.15e" % ours["summaries"][-1]["temp"])

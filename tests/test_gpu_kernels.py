@@ -1,0 +1,253 @@
+"""Per-kernel parity of libteahip.so against the CPU oracle: every field and
+every returned scalar must be bitwise equal (FP64, same operation order,
+same reduction tree).  Runs through the C ABI (DeviceChunk)."""
+
+import numpy as np
+import pytest
+
+from exploringsycl_amd.native import HALO_SLOTS
+from tests.oracle_ctypes import new_fields
+
+pytestmark = pytest.mark.gpu
+
+# (x_cells, y_cells): one group, one span, ragged rows, several spans, and
+# enough spans for three final-reduce passes.
+SIZES = [(1, 1), (3, 5), (10, 10), (60, 3), (250, 250), (601, 599), (1030, 300)]
+BIG = (4096, 4096)
+H = 2
+
+
+def _chunk(x, y):
+    from exploringsycl_amd.chunk import DeviceChunk
+    return DeviceChunk(x, y, H)
+
+
+def _random_fields(x, y, seed):
+    rng = np.random.default_rng(seed)
+    f = new_fields(x, y, H)
+    for k in ("density", "energy0", "energy", "u", "u0", "p", "r", "w", "sd",
+              "volume"):
+        f[k][:] = rng.standard_normal(f[k].size)
+    f["density"][:] = rng.uniform(0.1, 100.0, f["density"].size)
+    f["volume"][:] = rng.uniform(0.5, 1.5, f["volume"].size)
+    f["kx"][:] = rng.uniform(0.0, 50.0, f["kx"].size)
+    f["ky"][:] = rng.uniform(0.0, 50.0, f["ky"].size)
+    return f
+
+
+def _upload(c, f, names):
+    for n in names:
+        c.write(n, f[n])
+
+
+def _assert_fields(c, f, names):
+    for n in names:
+        got = c.read(n).ravel()
+        want = f[n]
+        assert got.shape == want.shape, n
+        bad = np.flatnonzero((got.view(np.int64) != want.view(np.int64))
+                             & ~(np.isnan(got) & np.isnan(want)))
+        assert bad.size == 0, (f"field {n}: {bad.size} cells differ, first at "
+                               f"{bad[0]}: {got[bad[0]]!r} vs {want[bad[0]]!r}")
+
+
+def _same(a, b):
+    """Bitwise equal, or both NaN (NaN payloads differ between CPU and GPU)."""
+    a, b = np.float64(a), np.float64(b)
+    return a.view(np.int64) == b.view(np.int64) or (np.isnan(a) and np.isnan(b))
+
+
+@pytest.mark.parametrize("cells", SIZES)
+def test_set_chunk_data_and_state(oracle, cells):
+    xc, yc = cells
+    x, y = xc + 2 * H, yc + 2 * H
+    dx, dy = 10.0 / xc, 10.0 / yc
+    f = new_fields(x, y, H)
+    states = [dict(density=100.0, energy=1e-4),
+              dict(density=0.1, energy=25.0, x_min=0.0 + dx / 100, x_max=1.0 - dx / 100,
+                   y_min=1.0 + dy / 100, y_max=2.0 - dy / 100),
+              dict(density=3.0, energy=7.0, x_min=2.5, x_max=7.25, y_min=0.3,
+                   y_max=9.1),
+              dict(density=0.5, energy=2.0, x_min=5.0, y_min=5.0, x_max=0.0,
+                   y_max=0.0, radius=2.2, geometry="circular")]
+    with _chunk(x, y) as c:
+        c.set_chunk_data(0.0, 0.0, dx, dy)
+        c.set_chunk_state(states)
+        c.store_energy()
+        oracle.set_chunk_data(x, y, H, 0.0, 0.0, dx, dy, f)
+        oracle.set_chunk_state(x, y, states, f)
+        f["energy"][:] = f["energy0"]
+        _assert_fields(c, f, ["vertex_x", "vertex_y", "cell_x", "cell_y", "volume",
+                              "x_area", "y_area", "energy0", "density", "u", "energy"])
+
+
+@pytest.mark.parametrize("cells", SIZES)
+@pytest.mark.parametrize("depth", [1, 2])
+def test_local_halos(oracle, cells, depth):
+    xc, yc = cells
+    x, y = xc + 2 * H, yc + 2 * H
+    if min(xc, yc) < depth:
+        pytest.skip("interior smaller than the halo depth")
+    f = _random_fields(x, y, 7)
+    names = {"density": "density", "energy0": "energy0", "energy1": "energy",
+             "u": "u", "p": "p", "sd": "sd"}
+    flagged = ["density", "p", "energy1", "sd"]
+    with _chunk(x, y) as c:
+        _upload(c, f, names.values())
+        c.local_halos(depth, flagged)
+        for slot in flagged:
+            oracle.halo_field(x, y, H, depth, f[names[slot]])
+        _assert_fields(c, f, names.values())
+
+
+def test_local_halos_partial_faces(oracle):
+    x, y = 37 + 4, 29 + 4
+    f = _random_fields(x, y, 8)
+    for nb in [(-1, 3, -1, 5), (2, -1, 4, -1), (1, 1, -1, -1)]:
+        with _chunk(x, y) as c:
+            _upload(c, f, ["u"])
+            c.local_halos(2, ["u"], neighbours=nb)
+            g = f["u"].copy()
+            oracle.halo_field(x, y, H, 2, g, neighbours=nb)
+            got = c.read("u").ravel()
+            assert np.array_equal(got.view(np.int64), g.view(np.int64)), nb
+
+
+@pytest.mark.parametrize("cells", SIZES)
+def test_field_summary(oracle, cells):
+    x, y = cells[0] + 2 * H, cells[1] + 2 * H
+    f = _random_fields(x, y, 11)
+    with _chunk(x, y) as c:
+        _upload(c, f, ["volume", "density", "energy0", "u"])
+        got = c.field_summary()
+        want = oracle.field_summary(x, y, H, f)
+        assert all(_same(a, b) for a, b in zip(got, want)), (got, want)
+
+
+@pytest.mark.parametrize("cells", SIZES)
+@pytest.mark.parametrize("coef", [1, 2])
+def test_cg_init(oracle, cells, coef):
+    x, y = cells[0] + 2 * H, cells[1] + 2 * H
+    f = _random_fields(x, y, 13)
+    names = ["p", "r", "u", "w", "kx", "ky", "density", "energy"]
+    with _chunk(x, y) as c:
+        _upload(c, f, names)
+        got = c.cg_init(coef, 0.7, 1.3)
+        want = oracle.cg_init(x, y, H, coef, 0.7, 1.3, f)
+        assert _same(got, want), (got, want)
+        _assert_fields(c, f, names)
+
+
+@pytest.mark.parametrize("cells", SIZES + [BIG])
+def test_cg_calc_w_ur_p_unfused(oracle, cells):
+    x, y = cells[0] + 2 * H, cells[1] + 2 * H
+    f = _random_fields(x, y, 17)
+    names = ["p", "r", "u", "w", "kx", "ky"]
+    with _chunk(x, y) as c:
+        c.set_fused(False)
+        _upload(c, f, names)
+        pw = c.cg_calc_w()
+        assert _same(pw, oracle.cg_calc_w(x, y, H, f))
+        rrn = c.cg_calc_ur(0.37)
+        assert _same(rrn, oracle.cg_calc_ur(x, y, H, 0.37, f))
+        c.cg_calc_p(0.81)
+        oracle.cg_calc_p(x, y, H, 0.81, f)
+        _assert_fields(c, f, names)
+
+
+@pytest.mark.parametrize("cells", SIZES)
+def test_shared_solver_kernels(oracle, cells):
+    x, y = cells[0] + 2 * H, cells[1] + 2 * H
+    f = _random_fields(x, y, 19)
+    names = ["u", "u0", "r", "kx", "ky", "density", "energy"]
+    with _chunk(x, y) as c:
+        _upload(c, f, names)
+        c.calculate_residual()
+        oracle.calculate_residual(x, y, H, f)
+        assert _same(c.calculate_2norm("r"), oracle.calculate_2norm(x, y, H, f["r"]))
+        c.copy_u()
+        oracle.copy_u(x, y, H, f)
+        c.finalise()
+        oracle.finalise(x, y, H, f)
+        _assert_fields(c, f, names)
+
+
+def _cg_sequence(c, oracle, x, y, f, iters, coef=2):
+    """The driver's call sequence (TeaLeaf/drivers/cg_driver.c:31-124) on the
+    device and on the oracle; scalars must agree bitwise at every step."""
+    rro = c.cg_init(coef, 0.9, 1.1)
+    rro_o = oracle.cg_init(x, y, H, coef, 0.9, 1.1, f)
+    assert _same(rro, rro_o)
+    c.local_halos(1, ["u", "p"])
+    for n in ("u", "p"):
+        oracle.halo_field(x, y, H, 1, f[n])
+    c.copy_u()
+    oracle.copy_u(x, y, H, f)
+    for it in range(iters):
+        pw = c.cg_calc_w()
+        pw_o = oracle.cg_calc_w(x, y, H, f)
+        assert _same(pw, pw_o), (it, pw, pw_o)
+        with np.errstate(all="ignore"):  # tiny grids converge to rr == 0
+            alpha = float(np.float64(rro) / np.float64(pw))
+        rrn = c.cg_calc_ur(alpha)
+        rrn_o = oracle.cg_calc_ur(x, y, H, alpha, f)
+        assert _same(rrn, rrn_o), (it, rrn, rrn_o)
+        with np.errstate(all="ignore"):
+            beta = float(np.float64(rrn) / np.float64(rro))
+        c.cg_calc_p(beta)
+        oracle.cg_calc_p(x, y, H, beta, f)
+        c.local_halos(1, ["u", "p"])
+        for n in ("u", "p"):
+            oracle.halo_field(x, y, H, 1, f[n])
+        rro = rrn
+    return rro
+
+
+@pytest.mark.parametrize("cells", [(1, 1), (10, 10), (60, 3), (250, 250), (601, 599)])
+@pytest.mark.parametrize("fused", [True, False])
+def test_cg_iterations_match_oracle(oracle, cells, fused):
+    x, y = cells[0] + 2 * H, cells[1] + 2 * H
+    f = _random_fields(x, y, 23)
+    f["density"][:] = np.random.default_rng(5).uniform(0.1, 100.0, x * y)
+    f["energy"][:] = np.random.default_rng(6).uniform(0.1, 25.0, x * y)
+    for n in ("density", "energy"):
+        oracle.halo_field(x, y, H, 2, f[n])
+    names = ["p", "r", "u", "u0", "w", "kx", "ky", "density", "energy"]
+    with _chunk(x, y) as c:
+        c.set_fused(fused)
+        _upload(c, f, names)
+        _cg_sequence(c, oracle, x, y, f, iters=25)
+        # Reading applies any deferred work; every cell must match.
+        _assert_fields(c, f, names)
+        c.calculate_residual()
+        oracle.calculate_residual(x, y, H, f)
+        _assert_fields(c, f, ["r"])
+
+
+def test_fused_large_grid_matches_oracle(oracle):
+    xc = yc = 1024
+    x, y = xc + 2 * H, yc + 2 * H
+    f = _random_fields(x, y, 29)
+    f["density"][:] = np.random.default_rng(1).uniform(0.1, 100.0, x * y)
+    f["energy"][:] = np.random.default_rng(2).uniform(0.1, 25.0, x * y)
+    names = ["p", "r", "u", "u0", "w", "kx", "ky", "density", "energy"]
+    with _chunk(x, y) as c:
+        _upload(c, f, names)
+        _cg_sequence(c, oracle, x, y, f, iters=10)
+        _assert_fields(c, f, names)
+
+
+def test_two_norm_rejects_foreign_pointer():
+    from exploringsycl_amd.native import TeaHipError
+    import ctypes
+    with _chunk(14, 14) as c:
+        v = ctypes.c_double()
+        st = c.lib.tea_hip_calculate_2norm(c.handle, ctypes.c_void_p(16), ctypes.byref(v))
+        assert st != 0
+        assert b"not an x*y field" in c.lib.tea_hip_last_error()
+        with pytest.raises(TeaHipError):
+            c.local_halos(3, ["u"])  # deeper than the halo
+
+
+def test_halo_slot_names():
+    assert HALO_SLOTS == ["density", "energy0", "energy1", "u", "p", "sd"]
