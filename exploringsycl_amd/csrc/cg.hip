@@ -4,7 +4,8 @@
 // Every FP64 expression below is evaluated in the reference's operand order
 // and the library is compiled with -ffp-contract=off, so each product and sum
 // rounds on its own; together with the reference's reduction tree
-// (span_reduce + final_reduce_kernel) the results are bitwise those of the
+// (span_reduce on the device, the last passes on the host) the results are
+// bitwise those of the
 // FP64 CPU restatement.
 
 #include <hip/hip_runtime.h>
@@ -132,93 +133,163 @@ struct CalcPBody {  // cg.cpp:257-281
 // Together 11 words = 88 bytes per cell per iteration against the
 // reference's 13 (calc_w 4 + calc_ur 6 + calc_p 3).
 
-struct FusedWBody {
-  const double* __restrict__ p_old;
-  double* __restrict__ p_new;
-  const double* __restrict__ r;
-  double* __restrict__ u;
-  double* __restrict__ w;
-  const double* __restrict__ kx;
-  const double* __restrict__ ky;
+struct FusedWArgs {
+  const double* p_old;
+  double* p_new;
+  const double* r;
+  double* u;
+  double* w;
+  const double* kx;
+  const double* ky;
   double alpha, beta;
-  int apply_u, apply_p, ring;
-  int x, xl, xr, yb, yt;  // interior bounds: xl <= kk <= xr, yb <= jj <= yt
-
-  __device__ inline double pn(int i) const { return beta * p_old[i] + r[i]; }
-
-  __device__ void operator()(int i, int kk, int jj, double (&out)[1]) const {
-    double pc, pl, pr, pb, pt;
-    const double po = p_old[i];
-    if (apply_p) {
-      pc = beta * po + r[i];
-      pl = kk == xl ? pc : pn(i - 1);
-      pr = kk == xr ? pc : pn(i + 1);
-      pb = jj == yb ? pc : pn(i - x);
-      pt = jj == yt ? pc : pn(i + x);
-    } else {
-      pc = po;
-      pl = p_old[i - 1];
-      pr = p_old[i + 1];
-      pb = p_old[i - x];
-      pt = p_old[i + x];
-    }
-    const double s = smvp(pc, pl, pr, pb, pt, kx, ky, i, x);
-    w[i] = s;
-    out[0] = s * pc;
-
-    if (apply_u) {
-      const double un = u[i] + alpha * po;
-      u[i] = un;
-      if (ring) write_ring(u, i, kk, jj, un);
-    }
-    if (apply_p) {
-      p_new[i] = pc;
-      if (ring)
-        write_ring(p_new, i, kk, jj, pc);
-      else
-        copy_ring(i, kk, jj);
-    }
-  }
-
-  // The reflective depth-1 halo of a boundary-adjacent cell: the face
-  // neighbours and, at the four interior corners, the diagonal one (which
-  // the reference's top/bottom pass fills from the left/right halo column).
-  __device__ inline void write_ring(double* f, int i, int kk, int jj,
-                                    double v) const {
-    const bool L = kk == xl, R = kk == xr, B = jj == yb, T = jj == yt;
-    if (L) f[i - 1] = v;
-    if (R) f[i + 1] = v;
-    if (B) f[i - x] = v;
-    if (T) f[i + x] = v;
-    if (L && B) f[i - 1 - x] = v;
-    if (R && B) f[i + 1 - x] = v;
-    if (L && T) f[i - 1 + x] = v;
-    if (R && T) f[i + 1 + x] = v;
-  }
-  // p updated without a halo refresh keeps its old halo values.
-  __device__ inline void copy_ring(int i, int kk, int jj) const {
-    const bool L = kk == xl, R = kk == xr, B = jj == yb, T = jj == yt;
-    if (L) p_new[i - 1] = p_old[i - 1];
-    if (R) p_new[i + 1] = p_old[i + 1];
-    if (B) p_new[i - x] = p_old[i - x];
-    if (T) p_new[i + x] = p_old[i + x];
-    if (L && B) p_new[i - 1 - x] = p_old[i - 1 - x];
-    if (R && B) p_new[i + 1 - x] = p_old[i + 1 - x];
-    if (L && T) p_new[i - 1 + x] = p_old[i - 1 + x];
-    if (R && T) p_new[i + 1 + x] = p_old[i + 1 + x];
-  }
+  int apply_u, ring;
 };
 
-struct FusedRBody {
-  double* __restrict__ r;
-  const double* __restrict__ w;
-  double alpha;
-  __device__ void operator()(int i, int, int, double (&out)[1]) const {
-    const double rn = r[i] - alpha * w[i];
-    r[i] = rn;
-    out[0] = rn * rn;
+// Cells of one thread are visited in batches of BATCH: all loads of a batch
+// are issued before any of its arithmetic and stores, so each wave keeps
+// BATCH cells' worth of reads in flight (loads cannot move past the stores
+// of the previous batch, as the compiler cannot prove the arrays distinct).
+// BATCH trades reads in flight per wave against waves per SIMD.
+template <bool APPLY_P, int kBatch>
+__global__ __launch_bounds__(kBlock) void fused_w_kernel(Geo g, FusedWArgs a,
+                                                         double* partials,
+                                                         int nspan,
+                                                         int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  const int x = g.x;
+  const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
+  const double* __restrict__ P = a.p_old;
+  const double* __restrict__ R = a.r;
+  const double* __restrict__ KX = a.kx;
+  const double* __restrict__ KY = a.ky;
+  double prod[1][kItems];
+
+#pragma unroll
+  for (int b0 = 0; b0 < kItems; b0 += kBatch) {
+    int ci[kBatch], ck[kBatch], cj[kBatch];
+    bool in[kBatch];
+    double po[kBatch], pl[kBatch], pr[kBatch], pb[kBatch], pt[kBatch];
+    double rc[kBatch], rl[kBatch], rr[kBatch], rb[kBatch], rt[kBatch];
+    double uc[kBatch], kx0[kBatch], kx1[kBatch], ky0[kBatch], ky1[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int e = base + (b0 + j) * kBlock + threadIdx.x;
+      const int jj = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+      ci[j] = e;
+      cj[j] = jj;
+      ck[j] = e - jj * x;
+      in[j] = e < g.n && interior(g, ck[j], jj);
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int i = in[j] ? ci[j] : x + 1;  // any interior-safe index
+      po[j] = P[i];
+      pl[j] = P[i - 1];
+      pr[j] = P[i + 1];
+      pb[j] = P[i - x];
+      pt[j] = P[i + x];
+      if (APPLY_P) {
+        rc[j] = R[i];
+        rl[j] = R[i - 1];
+        rr[j] = R[i + 1];
+        rb[j] = R[i - x];
+        rt[j] = R[i + x];
+      }
+      kx0[j] = KX[i];
+      kx1[j] = KX[i + 1];
+      ky0[j] = KY[i];
+      ky1[j] = KY[i + x];
+      uc[j] = a.apply_u ? a.u[i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int i = ci[j], kk = ck[j], jj = cj[j];
+      double pc, l, rgt, bot, top;
+      if (APPLY_P) {
+        // p_new = beta p_old + r, here and at the four neighbours; a
+        // neighbour on the reflective halo ring is this cell's mirror image.
+        pc = a.beta * po[j] + rc[j];
+        l = kk == xl ? pc : a.beta * pl[j] + rl[j];
+        rgt = kk == xr ? pc : a.beta * pr[j] + rr[j];
+        bot = jj == yb ? pc : a.beta * pb[j] + rb[j];
+        top = jj == yt ? pc : a.beta * pt[j] + rt[j];
+      } else {
+        pc = po[j];
+        l = pl[j];
+        rgt = pr[j];
+        bot = pb[j];
+        top = pt[j];
+      }
+      const double diag = (1.0 + (kx1[j] + kx0[j])) + (ky1[j] + ky0[j]);
+      const double s = (diag * pc - (kx1[j] * rgt + kx0[j] * l)) -
+                       (ky1[j] * top + ky0[j] * bot);
+      prod[0][b0 + j] = in[j] ? s * pc : 0.0;
+      if (!in[j]) continue;
+      a.w[i] = s;
+      const bool L = kk == xl, Rr = kk == xr, B = jj == yb, T = jj == yt;
+      const bool edge = L | Rr | B | T;
+      if (a.apply_u) {
+        const double un = uc[j] + a.alpha * po[j];
+        a.u[i] = un;
+        if (a.ring && edge) {
+          if (L) a.u[i - 1] = un;
+          if (Rr) a.u[i + 1] = un;
+          if (B) a.u[i - x] = un;
+          if (T) a.u[i + x] = un;
+          if (L && B) a.u[i - 1 - x] = un;
+          if (Rr && B) a.u[i + 1 - x] = un;
+          if (L && T) a.u[i - 1 + x] = un;
+          if (Rr && T) a.u[i + 1 + x] = un;
+        }
+      }
+      if (APPLY_P) {
+        double* __restrict__ Q = a.p_new;
+        Q[i] = pc;
+        if (edge) {
+          // With the halo refresh folded in, the ring gets the new values;
+          // without it, p keeps its old halo.
+          if (L) Q[i - 1] = a.ring ? pc : pl[j];
+          if (Rr) Q[i + 1] = a.ring ? pc : pr[j];
+          if (B) Q[i - x] = a.ring ? pc : pb[j];
+          if (T) Q[i + x] = a.ring ? pc : pt[j];
+          if (L && B) Q[i - 1 - x] = a.ring ? pc : P[i - 1 - x];
+          if (Rr && B) Q[i + 1 - x] = a.ring ? pc : P[i + 1 - x];
+          if (L && T) Q[i - 1 + x] = a.ring ? pc : P[i - 1 + x];
+          if (Rr && T) Q[i + 1 + x] = a.ring ? pc : P[i + 1 + x];
+        }
+      }
+    }
   }
-};
+  span_reduce<1>(prod, partials, nspan, span, single_group != 0);
+}
+
+// r -= alpha w and r.r: all 16 cells' loads first, then the updates.
+__global__ __launch_bounds__(kBlock) void fused_r_kernel(
+    Geo g, double* __restrict__ r, const double* __restrict__ w, double alpha,
+    double* partials, int nspan, int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  double rv[kItems], wv[kItems];
+  bool in[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int e = base + k * kBlock + threadIdx.x;
+    const int jj = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+    in[k] = e < g.n && interior(g, e - jj * g.x, jj);
+    const int i = in[k] ? e : 0;
+    rv[k] = r[i];
+    wv[k] = w[i];
+  }
+  double prod[1][kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const double rn = rv[k] - alpha * wv[k];
+    prod[0][k] = in[k] ? rn * rn : 0.0;
+    if (in[k]) r[base + k * kBlock + threadIdx.x] = rn;
+  }
+  span_reduce<1>(prod, partials, nspan, span, single_group != 0);
+}
 
 struct ApplyPendingBody {
   double* __restrict__ u;
@@ -387,29 +458,22 @@ int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
   }
   const bool apply_p = c->pend_p;
   if (apply_p) TEA_RETURN_IF(sync_p_alt(c));
-  FusedWBody body{f[TEA_FIELD_P],
-                  c->p_alt,
-                  f[TEA_FIELD_R],
-                  f[TEA_FIELD_U],
-                  f[TEA_FIELD_W],
-                  f[TEA_FIELD_KX],
-                  f[TEA_FIELD_KY],
-                  c->pend_alpha,
-                  c->pend_beta,
-                  c->pend_u ? 1 : 0,
-                  apply_p ? 1 : 0,
-                  c->pend_halo ? 1 : 0,
-                  g.x,
-                  g.h,
-                  g.x - g.h - 1,
-                  g.h,
-                  g.y - g.h - 1};
+  FusedWArgs args{f[TEA_FIELD_P], c->p_alt,        f[TEA_FIELD_R],
+                  f[TEA_FIELD_U], f[TEA_FIELD_W],  f[TEA_FIELD_KX],
+                  f[TEA_FIELD_KY], c->pend_alpha,  c->pend_beta,
+                  c->pend_u ? 1 : 0, c->pend_halo ? 1 : 0};
   c->pend_u = c->pend_p = c->pend_halo = false;
   {
     Timed t(c, K_CG_FUSED_W);
-    hipLaunchKernelGGL((span_kernel<1, FusedWBody>), span_grid(c),
-                       dim3(kBlock), 0, c->stream, g, body, c->partials,
-                       c->nspan, c->ngroup == 1 ? 1 : 0);
+    const int sg = c->ngroup == 1 ? 1 : 0;
+#define TEA_FUSED_W(AP, B)                                                   \
+  hipLaunchKernelGGL((fused_w_kernel<AP, B>), span_grid(c), dim3(kBlock), 0, \
+                     c->stream, g, args, c->partials, c->nspan, sg)
+    if (!apply_p) TEA_FUSED_W(false, 4);
+    else if (c->batch == 2) TEA_FUSED_W(true, 2);
+    else if (c->batch == 8) TEA_FUSED_W(true, 8);
+    else TEA_FUSED_W(true, 4);
+#undef TEA_FUSED_W
   }
   TEA_HIP_TRY(hipGetLastError());
   if (apply_p) {
@@ -430,8 +494,14 @@ int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
                         CalcURBody{f[TEA_FIELD_U], f[TEA_FIELD_R],
                                    f[TEA_FIELD_P], f[TEA_FIELD_W], alpha},
                         rrn);
-  TEA_RETURN_IF(launch_span1(
-      c, K_CG_FUSED_R, FusedRBody{f[TEA_FIELD_R], f[TEA_FIELD_W], alpha}, rrn));
+  {
+    Timed t(c, K_CG_FUSED_R);
+    hipLaunchKernelGGL(fused_r_kernel, span_grid(c), dim3(kBlock), 0, c->stream,
+                       c->geo, f[TEA_FIELD_R], f[TEA_FIELD_W], alpha,
+                       c->partials, c->nspan, c->ngroup == 1 ? 1 : 0);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  TEA_RETURN_IF(finish_reduction(c, 1, rrn));
   c->pend_u = true;
   c->pend_alpha = alpha;
   return TEA_HIP_OK;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -22,7 +23,7 @@ const char* const kKernelNames[K_NUM] = {
     "field_summary",  "cg_init_u",       "cg_init_k",    "cg_init_others",
     "cg_calc_w",      "cg_calc_ur",      "cg_calc_p",    "cg_fused_w",
     "cg_fused_r",     "cg_apply_pending", "copy_u",      "calculate_residual",
-    "calculate_2norm", "finalise",       "final_reduce"};
+    "calculate_2norm", "finalise"};
 
 Timed::Timed(tea_hip_chunk* c_, int kid_) : c(c_), kid(kid_) {
   if (!c->timing) return;
@@ -64,66 +65,36 @@ int sync_chunk(tea_hip_chunk* c) {
   return resolve_timing(c);
 }
 
-// Final passes of the reference's reduceArray over the level-2 partials of
-// one launch: nr arrays of nspan values each.  One 1024-thread block; each
-// pass cuts the array into zero-padded groups of 64 and sums each group with
-// the 64-lane tree, exactly as TeaLeaf/c_kernels/sycl/sycl_shared.hpp:43-75
-// does with one work-group per group.  Results go straight to pinned host
-// memory.
-constexpr int kFinalThreads = 1024;
-constexpr int kFinalWaves = kFinalThreads / kWave;
-constexpr int kMaxLevel3 = 4096;  // supports nspan <= 64 * 4096
-
-__global__ __launch_bounds__(kFinalThreads) void final_reduce_kernel(
-    const double* __restrict__ partials, int nspan, int nr,
-    double* __restrict__ out) {
-  __shared__ double buf[2][kMaxLevel3];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
-  for (int q = 0; q < nr; ++q) {
-    const double* src = partials + (size_t)q * nspan;
-    double result;
-    if (nspan == 1) {
-      result = src[0];
-    } else {
-      int len = nspan;
-      int ng = (len + kWave - 1) / kWave;
-      for (int g = wv; g < ng; g += kFinalWaves) {
-        const int i = g * kWave + lane;
-        const double t = wave_tree(i < len ? src[i] : 0.0);
-        if (lane == 0) buf[0][g] = t;
+// Final passes of the reference's reduceArray (TeaLeaf/c_kernels/sycl/
+// sycl_shared.hpp:43-75) over one launch's level-2 partials, on the host:
+// the kernels write one partial per 4096-cell span straight into pinned host
+// memory, so after the stream is synchronised the remaining passes (e.g.
+// 4105 -> 65 -> 2 -> 1 at 4096^2) are a few thousand host additions, with
+// no extra launch.  Each pass cuts the array into zero-padded groups of 64
+// and sums each group with the pairwise tree.
+static double host_tree_passes(double* v, size_t len) {
+  while (len != 1) {
+    const size_t groups = (len + kWave - 1) / kWave;
+    for (size_t g = 0; g < groups; ++g) {
+      double a[kWave];
+      for (int l = 0; l < kWave; ++l) {
+        const size_t i = g * kWave + l;
+        a[l] = i < len ? v[i] : 0.0;
       }
-      __syncthreads();
-      int cur = 0;
-      len = ng;
-      while (len > 1) {
-        ng = (len + kWave - 1) / kWave;
-        for (int g = wv; g < ng; g += kFinalWaves) {
-          const int i = g * kWave + lane;
-          const double t = wave_tree(i < len ? buf[cur][i] : 0.0);
-          if (lane == 0) buf[cur ^ 1][g] = t;
-        }
-        __syncthreads();
-        cur ^= 1;
-        len = ng;
-      }
-      result = buf[cur][0];
-      __syncthreads();  // buf is reused by the next q
+      for (int st = 1; st < kWave; st *= 2)
+        for (int i = 0; i + st < kWave; i += 2 * st) a[i] = a[i] + a[i + st];
+      v[g] = a[0];  // g <= 64g: earlier groups never read it again
     }
-    if (threadIdx.x == 0) out[q] = result;
+    len = groups;
   }
+  return v[0];
 }
 
 int finish_reduction(tea_hip_chunk* c, int nr, double* out) {
-  {
-    Timed t(c, K_FINAL_REDUCE);
-    hipLaunchKernelGGL(final_reduce_kernel, dim3(1), dim3(kFinalThreads), 0,
-                       c->stream, c->partials, c->nspan, nr,
-                       c->host_result_dev);
-  }
-  TEA_HIP_TRY(hipGetLastError());
   TEA_RETURN_IF(sync_chunk(c));
-  for (int q = 0; q < nr; ++q) out[q] = c->host_result[q];
+  for (int q = 0; q < nr; ++q)
+    out[q] = host_tree_passes(c->host_partials + (size_t)q * c->nspan,
+                              (size_t)c->nspan);
   return TEA_HIP_OK;
 }
 
@@ -145,8 +116,7 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
   TEA_CHECK_ARG(x > 2 * halo_depth && y > 2 * halo_depth,
                 "x and y must exceed 2*halo_depth");
   const long n = (long)x * (long)y;
-  TEA_CHECK_ARG(n <= (long)kSpan * kWave * kMaxLevel3 && n < (1L << 31) - kSpan,
-                "chunk too large for the reduction workspace");
+  TEA_CHECK_ARG(n < (1L << 31) - kSpan, "chunk too large (x*y >= 2^31)");
   int ndev = 0;
   TEA_HIP_TRY(hipGetDeviceCount(&ndev));
   TEA_CHECK_ARG(device >= 0 && device < ndev, "no such HIP device");
@@ -161,6 +131,7 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
   c->geo.dx = FastDiv((uint32_t)x);
   c->ngroup = (int)((n + kWave - 1) / kWave);
   c->nspan = (int)((n + kSpan - 1) / kSpan);
+  if (const char* b = getenv("TEA_HIP_BATCH")) c->batch = atoi(b);
 
   auto fail = [&](int code) {
     tea_hip_chunk_destroy(c);
@@ -204,20 +175,15 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
     set_error("hipMalloc of the second p buffer failed");
     return fail(TEA_HIP_ENOMEM);
   }
-  if (hipMalloc(&c->partials, 4 * (size_t)c->nspan * sizeof(double)) !=
-      hipSuccess) {
-    c->partials = nullptr;
-    set_error("hipMalloc of the reduction workspace failed");
-    return fail(TEA_HIP_ENOMEM);
-  }
-  if (hipHostMalloc(&c->host_result, 64 * sizeof(double),
+  const size_t pbytes = 4 * (size_t)c->nspan * sizeof(double);
+  if (hipHostMalloc(&c->host_partials, pbytes,
                     hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
-    c->host_result = nullptr;
-    set_error("hipHostMalloc of the result slots failed");
+    c->host_partials = nullptr;
+    set_error("hipHostMalloc of the reduction partials failed");
     return fail(TEA_HIP_ENOMEM);
   }
-  std::memset(c->host_result, 0, 64 * sizeof(double));
-  if (hipHostGetDevicePointer((void**)&c->host_result_dev, c->host_result, 0) !=
+  std::memset(c->host_partials, 0, pbytes);
+  if (hipHostGetDevicePointer((void**)&c->partials, c->host_partials, 0) !=
       hipSuccess) {
     set_error("hipHostGetDevicePointer failed");
     return fail(TEA_HIP_EHIP);
@@ -242,8 +208,7 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (int f = 0; f < TEA_NUM_FIELDS; ++f)
     if (c->field[f]) (void)hipFree(c->field[f]);
   if (c->p_alt) (void)hipFree(c->p_alt);
-  if (c->partials) (void)hipFree(c->partials);
-  if (c->host_result) (void)hipHostFree(c->host_result);
+  if (c->host_partials) (void)hipHostFree(c->host_partials);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return TEA_HIP_OK;

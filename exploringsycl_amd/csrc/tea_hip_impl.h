@@ -227,7 +227,6 @@ enum KernelId {
   K_CALC_RESIDUAL,
   K_CALC_2NORM,
   K_FINALISE,
-  K_FINAL_REDUCE,
   K_NUM
 };
 
@@ -253,14 +252,14 @@ struct tea_hip_chunk {
   double* p_alt = nullptr;
   bool p_alt_synced = false;
 
-  // Reduction workspace: 4 x nspan level-2 partials, and pinned host memory
-  // that the final-reduce kernel writes its results into.
-  double* partials = nullptr;
-  double* host_result = nullptr;      // host view
-  double* host_result_dev = nullptr;  // device view of the same memory
+  // Reduction workspace: up to 4 x nspan level-2 partials in pinned,
+  // coherent host memory; kernels write through the device view.
+  double* partials = nullptr;       // device view
+  double* host_partials = nullptr;  // host view of the same memory
 
   // Deferred CG work (fused mode).
   int fused = 1;
+  int batch = 4;  // cells per load batch in the fused stencil kernel
   bool pend_u = false, pend_p = false, pend_halo = false;
   double pend_alpha = 0.0, pend_beta = 0.0;
 
@@ -293,9 +292,9 @@ int resolve_timing(tea_hip_chunk* c);
 // Synchronises the stream and resolves timing.
 int sync_chunk(tea_hip_chunk* c);
 
-// Launches the final pass of a reduction: reduces nr arrays of nspan level-2
-// partials each with the reference tree and writes the nr results to
-// host_result[0..nr), then waits and copies them to `out`.
+// Waits for the stream, then finishes nr reductions on the host: the
+// reference tree's remaining passes over each array of nspan level-2
+// partials; results go to out[0..nr).
 int finish_reduction(tea_hip_chunk* c, int nr, double* out);
 
 // Applies any deferred CG work (fused mode).
