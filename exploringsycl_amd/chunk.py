@@ -145,6 +145,41 @@ class DeviceChunk:
     def finalise(self):
         check(self.lib.tea_hip_finalise(self.handle), "finalise")
 
+    # ---- mixed-precision CG (no reference counterpart) -------------------------
+    def mpcg_init(self, coefficient, rx, ry):
+        rr, bb = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.tea_hip_mpcg_init(self.handle, coefficient, rx, ry,
+                                         ctypes.byref(rr), ctypes.byref(bb)),
+              "mpcg_init")
+        return rr.value, bb.value
+
+    def mpcg_calc_w(self, alpha, apply_u, beta, apply_p):
+        v = ctypes.c_double()
+        check(self.lib.tea_hip_mpcg_calc_w(self.handle, alpha, int(apply_u), beta,
+                                           int(apply_p), ctypes.byref(v)),
+              "mpcg_calc_w")
+        return v.value
+
+    def mpcg_calc_r(self, alpha):
+        v = ctypes.c_double()
+        check(self.lib.tea_hip_mpcg_calc_r(self.handle, alpha, ctypes.byref(v)),
+              "mpcg_calc_r")
+        return v.value
+
+    def mpcg_refresh(self, alpha, apply_u):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.tea_hip_mpcg_refresh(self.handle, alpha, int(apply_u),
+                                            ctypes.byref(a), ctypes.byref(b)),
+              "mpcg_refresh")
+        return a.value, b.value
+
+    def read_mp(self, name):
+        out = np.empty(self.x * self.y, dtype=np.float32)
+        check(self.lib.tea_hip_mp_read(self.handle, native.MP_FIELDS[name],
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       out.size), f"read_mp {name}")
+        return out.reshape(self.y, self.x)
+
     # ---- mode and timing -------------------------------------------------------
     def set_fused(self, on):
         check(self.lib.tea_hip_set_fused(self.handle, 1 if on else 0), "set_fused")

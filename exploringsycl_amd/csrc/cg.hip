@@ -264,6 +264,245 @@ __global__ __launch_bounds__(kBlock) void fused_w_kernel(Geo g, FusedWArgs a,
   span_reduce<1>(prod, partials, nspan, span, single_group != 0);
 }
 
+// ---- two cells per lane --------------------------------------------------
+//
+// When x is even every cell pair (2m, 2m+1) lies in one row and starts on a
+// 16-byte boundary, so p, r, u, kx, ky and the rows above and below are read
+// with one 16-byte load per pair, and the left/right neighbours come from
+// the adjacent lanes (lane l-1 holds 2m-1, lane l+1 holds 2m+2); only lanes
+// 0 and 63 load theirs.  A wave covers 128 consecutive cells = two groups
+// of the reduction tree (lanes 0-31 and 32-63); a 256-thread block covers a
+// 4096-cell span in 8 slots of 512 cells.  The first tree level is the
+// in-lane pair sum, then the butterfly runs over each 32-lane half.
+
+constexpr int kSlots = kSpan / (2 * kBlock);  // 8
+
+__device__ inline double2 ld2(const double* p, int i) {
+  return *reinterpret_cast<const double2*>(p + i);
+}
+
+// Stores a cell pair, or the interior member(s) only.
+__device__ inline void st2(double* p, int i, bool in0, bool in1, double v0,
+                           double v1) {
+  if (in0 && in1) {
+    *reinterpret_cast<double2*>(p + i) = double2{v0, v1};
+  } else {
+    if (in0) p[i] = v0;
+    if (in1) p[i + 1] = v1;
+  }
+}
+
+__device__ inline int bitrev3(int l) {
+  return ((l & 1) << 2) | (l & 2) | ((l & 4) >> 2);
+}
+
+// Level-1 trees of the 8 slot pair-sums of a lane, transposed across lanes
+// as in wave_tree16 (steps 1, 2, 4), then steps 8 and 16 within each 32-lane
+// half; stores each group's partial in lds[group-in-span] and finishes the
+// span's level-2 tree in wave 0.
+__device__ inline void span_reduce_pairs(double (&v)[kSlots], double* partials,
+                                         int nspan, int span,
+                                         bool single_group) {
+  __shared__ double lds[kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int step = 0; step < 3; ++step) {
+    const int m = 1 << step;
+    const int half = kSlots >> (step + 1);
+    const bool upper = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      const double send = upper ? v[i] : v[half + i];
+      const double keep = upper ? v[half + i] : v[i];
+      v[i] = keep + __shfl_xor(send, m);
+    }
+  }
+  double t = v[0];
+  t = t + __shfl_xor(t, 8);
+  t = t + __shfl_xor(t, 16);
+  if ((lane & 31) < kSlots)
+    lds[bitrev3(lane & 7) * 8 + wv * 2 + (lane >> 5)] = t;
+  __syncthreads();
+  if (wv == 0) {
+    const double r = single_group ? lds[0] : wave_tree(lds[lane]);
+    if (lane == 0) partials[span] = r;
+  }
+}
+
+template <bool APPLY_P, int SB>
+__global__ __launch_bounds__(kBlock) void fused_w2_kernel(Geo g, FusedWArgs a,
+                                                          double* partials,
+                                                          int nspan,
+                                                          int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int x = g.x;
+  const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
+  const double* __restrict__ P = a.p_old;
+  const double* __restrict__ R = a.r;
+  const double beta = a.beta;
+  double psum[kSlots];
+
+#pragma unroll
+  for (int s0 = 0; s0 < kSlots; s0 += SB) {
+    int e0[SB], kk[SB], jj[SB];
+    bool in0[SB], in1[SB], any[SB];
+    double2 P0[SB], Pt[SB], Pb[SB], R0[SB], Rt[SB], Rb[SB], KX[SB], KY[SB],
+        KYt[SB], U[SB];
+#pragma unroll
+    for (int j = 0; j < SB; ++j) {
+      const int e = base + (s0 + j) * (2 * kBlock) + 2 * threadIdx.x;
+      const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+      e0[j] = e;
+      jj[j] = row;
+      kk[j] = e - row * x;
+      in0[j] = e < g.n && interior(g, kk[j], row);
+      in1[j] = e < g.n && interior(g, kk[j] + 1, row);
+      any[j] = in0[j] || in1[j];
+    }
+#pragma unroll
+    for (int j = 0; j < SB; ++j) {
+      // The pair's own p, r and kx are also the left/right neighbours the
+      // adjacent lanes take by shuffle, so they are read whenever the pair
+      // is in range; the rows above and below only for interior pairs.
+      // x + 2 is an even index whose +-x rows are in range.
+      const int ic = e0[j] < g.n ? e0[j] : x + 2;
+      const int ir = any[j] ? e0[j] : x + 2;
+      P0[j] = ld2(P, ic);
+      Pt[j] = ld2(P, ir + x);
+      Pb[j] = ld2(P, ir - x);
+      if (APPLY_P) {
+        R0[j] = ld2(R, ic);
+        Rt[j] = ld2(R, ir + x);
+        Rb[j] = ld2(R, ir - x);
+      }
+      KX[j] = ld2(a.kx, ic);
+      KY[j] = ld2(a.ky, ir);
+      KYt[j] = ld2(a.ky, ir + x);
+      if (a.apply_u) U[j] = ld2(a.u, ir);
+    }
+#pragma unroll
+    for (int j = 0; j < SB; ++j) {
+      const int i = e0[j];
+      // Neighbours across the pair boundary, from the adjacent lanes.
+      double pl = __shfl_up(P0[j].y, 1), pr = __shfl_down(P0[j].x, 1);
+      double rl = 0.0, rr = 0.0;
+      if (APPLY_P) {
+        rl = __shfl_up(R0[j].y, 1);
+        rr = __shfl_down(R0[j].x, 1);
+      }
+      double kxr = __shfl_down(KX[j].x, 1);
+      if (lane == 0 && any[j]) {
+        pl = P[i - 1];
+        if (APPLY_P) rl = R[i - 1];
+      }
+      if (lane == kWave - 1 && any[j]) {
+        pr = P[i + 2];
+        if (APPLY_P) rr = R[i + 2];
+        kxr = a.kx[i + 2];
+      }
+      const int ka = kk[j], kb = kk[j] + 1, row = jj[j];
+      double ca, cb, la, ra, lb, rb2, ba, ta, bb, tb;
+      if (APPLY_P) {
+        ca = beta * P0[j].x + R0[j].x;
+        cb = beta * P0[j].y + R0[j].y;
+        la = ka == xl ? ca : beta * pl + rl;
+        ra = ka == xr ? ca : cb;
+        lb = kb == xl ? cb : ca;
+        rb2 = kb == xr ? cb : beta * pr + rr;
+        ba = row == yb ? ca : beta * Pb[j].x + Rb[j].x;
+        ta = row == yt ? ca : beta * Pt[j].x + Rt[j].x;
+        bb = row == yb ? cb : beta * Pb[j].y + Rb[j].y;
+        tb = row == yt ? cb : beta * Pt[j].y + Rt[j].y;
+      } else {
+        ca = P0[j].x;
+        cb = P0[j].y;
+        la = pl;
+        ra = cb;
+        lb = ca;
+        rb2 = pr;
+        ba = Pb[j].x;
+        ta = Pt[j].x;
+        bb = Pb[j].y;
+        tb = Pt[j].y;
+      }
+      const double kxa0 = KX[j].x, kxa1 = KX[j].y, kxb0 = KX[j].y, kxb1 = kxr;
+      const double diag_a = (1.0 + (kxa1 + kxa0)) + (KYt[j].x + KY[j].x);
+      const double diag_b = (1.0 + (kxb1 + kxb0)) + (KYt[j].y + KY[j].y);
+      const double sa = (diag_a * ca - (kxa1 * ra + kxa0 * la)) -
+                        (KYt[j].x * ta + KY[j].x * ba);
+      const double sb = (diag_b * cb - (kxb1 * rb2 + kxb0 * lb)) -
+                        (KYt[j].y * tb + KY[j].y * bb);
+      psum[s0 + j] = (in0[j] ? sa * ca : 0.0) + (in1[j] ? sb * cb : 0.0);
+      if (!any[j]) continue;
+      st2(a.w, i, in0[j], in1[j], sa, sb);
+      double ua = 0.0, ub = 0.0;
+      if (a.apply_u) {
+        ua = U[j].x + a.alpha * P0[j].x;
+        ub = U[j].y + a.alpha * P0[j].y;
+        st2(a.u, i, in0[j], in1[j], ua, ub);
+      }
+      if (APPLY_P) st2(a.p_new, i, in0[j], in1[j], ca, cb);
+      // Halo ring of boundary-adjacent cells (rare; per cell).
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const bool inc = c == 0 ? in0[j] : in1[j];
+        const int ic = i + c, kc = ka + c;
+        const bool L = kc == xl, Rg = kc == xr, B = row == yb, T = row == yt;
+        if (!inc || !(L | Rg | B | T)) continue;
+        const double pv = c == 0 ? ca : cb;
+        const double uv = c == 0 ? ua : ub;
+        int ring[8];
+        int nr = 0;
+        if (L) ring[nr++] = ic - 1;
+        if (Rg) ring[nr++] = ic + 1;
+        if (B) ring[nr++] = ic - x;
+        if (T) ring[nr++] = ic + x;
+        if (L && B) ring[nr++] = ic - 1 - x;
+        if (Rg && B) ring[nr++] = ic + 1 - x;
+        if (L && T) ring[nr++] = ic - 1 + x;
+        if (Rg && T) ring[nr++] = ic + 1 + x;
+        for (int q = 0; q < nr; ++q) {
+          if (a.apply_u && a.ring) a.u[ring[q]] = uv;
+          if (APPLY_P) a.p_new[ring[q]] = a.ring ? pv : P[ring[q]];
+        }
+      }
+    }
+  }
+  span_reduce_pairs(psum, partials, nspan, span, single_group != 0);
+}
+
+__global__ __launch_bounds__(kBlock) void fused_r2_kernel(
+    Geo g, double* __restrict__ r, const double* __restrict__ w, double alpha,
+    double* partials, int nspan, int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  double2 rv[kSlots], wv[kSlots];
+  bool in0[kSlots], in1[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const int e = base + k * (2 * kBlock) + 2 * threadIdx.x;
+    const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+    const int kk = e - row * g.x;
+    in0[k] = e < g.n && interior(g, kk, row);
+    in1[k] = e < g.n && interior(g, kk + 1, row);
+    const int i = (in0[k] || in1[k]) ? e : 0;
+    rv[k] = ld2(r, i);
+    wv[k] = ld2(w, i);
+  }
+  double psum[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const double ra = rv[k].x - alpha * wv[k].x;
+    const double rb = rv[k].y - alpha * wv[k].y;
+    psum[k] = (in0[k] ? ra * ra : 0.0) + (in1[k] ? rb * rb : 0.0);
+    st2(r, base + k * (2 * kBlock) + 2 * threadIdx.x, in0[k], in1[k], ra, rb);
+  }
+  span_reduce_pairs(psum, partials, nspan, span, single_group != 0);
+}
+
 // r -= alpha w and r.r: all 16 cells' loads first, then the updates.
 __global__ __launch_bounds__(kBlock) void fused_r_kernel(
     Geo g, double* __restrict__ r, const double* __restrict__ w, double alpha,
@@ -469,11 +708,22 @@ int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
 #define TEA_FUSED_W(AP, B)                                                   \
   hipLaunchKernelGGL((fused_w_kernel<AP, B>), span_grid(c), dim3(kBlock), 0, \
                      c->stream, g, args, c->partials, c->nspan, sg)
-    if (!apply_p) TEA_FUSED_W(false, 4);
-    else if (c->batch == 2) TEA_FUSED_W(true, 2);
-    else if (c->batch == 8) TEA_FUSED_W(true, 8);
-    else TEA_FUSED_W(true, 4);
+#define TEA_FUSED_W2(AP, B)                                                   \
+  hipLaunchKernelGGL((fused_w2_kernel<AP, B>), span_grid(c), dim3(kBlock), 0, \
+                     c->stream, g, args, c->partials, c->nspan, sg)
+    if (c->pairs) {
+      if (!apply_p) TEA_FUSED_W2(false, 2);
+      else if (c->batch == 1) TEA_FUSED_W2(true, 1);
+      else if (c->batch == 4) TEA_FUSED_W2(true, 4);
+      else TEA_FUSED_W2(true, 2);
+    } else {
+      if (!apply_p) TEA_FUSED_W(false, 4);
+      else if (c->batch == 2) TEA_FUSED_W(true, 2);
+      else if (c->batch == 8) TEA_FUSED_W(true, 8);
+      else TEA_FUSED_W(true, 4);
+    }
 #undef TEA_FUSED_W
+#undef TEA_FUSED_W2
   }
   TEA_HIP_TRY(hipGetLastError());
   if (apply_p) {
@@ -496,9 +746,14 @@ int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
                         rrn);
   {
     Timed t(c, K_CG_FUSED_R);
-    hipLaunchKernelGGL(fused_r_kernel, span_grid(c), dim3(kBlock), 0, c->stream,
-                       c->geo, f[TEA_FIELD_R], f[TEA_FIELD_W], alpha,
-                       c->partials, c->nspan, c->ngroup == 1 ? 1 : 0);
+    if (c->pairs)
+      hipLaunchKernelGGL(fused_r2_kernel, span_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, f[TEA_FIELD_R], f[TEA_FIELD_W],
+                         alpha, c->partials, c->nspan, c->ngroup == 1 ? 1 : 0);
+    else
+      hipLaunchKernelGGL(fused_r_kernel, span_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, f[TEA_FIELD_R], f[TEA_FIELD_W],
+                         alpha, c->partials, c->nspan, c->ngroup == 1 ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
   TEA_RETURN_IF(finish_reduction(c, 1, rrn));

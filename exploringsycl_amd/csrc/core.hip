@@ -23,7 +23,8 @@ const char* const kKernelNames[K_NUM] = {
     "field_summary",  "cg_init_u",       "cg_init_k",    "cg_init_others",
     "cg_calc_w",      "cg_calc_ur",      "cg_calc_p",    "cg_fused_w",
     "cg_fused_r",     "cg_apply_pending", "copy_u",      "calculate_residual",
-    "calculate_2norm", "finalise"};
+    "calculate_2norm", "finalise",       "mpcg_convert", "mpcg_w",
+    "mpcg_r",         "mpcg_apply_u",    "mpcg_refresh"};
 
 Timed::Timed(tea_hip_chunk* c_, int kid_) : c(c_), kid(kid_) {
   if (!c->timing) return;
@@ -132,6 +133,8 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
   c->ngroup = (int)((n + kWave - 1) / kWave);
   c->nspan = (int)((n + kSpan - 1) / kSpan);
   if (const char* b = getenv("TEA_HIP_BATCH")) c->batch = atoi(b);
+  c->pairs = (x % 2 == 0) && y >= 3;
+  if (const char* pv = getenv("TEA_HIP_PAIRS")) c->pairs = c->pairs && atoi(pv) != 0;
 
   auto fail = [&](int code) {
     tea_hip_chunk_destroy(c);
@@ -208,6 +211,8 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (int f = 0; f < TEA_NUM_FIELDS; ++f)
     if (c->field[f]) (void)hipFree(c->field[f]);
   if (c->p_alt) (void)hipFree(c->p_alt);
+  for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky})
+    if (b) (void)hipFree(b);
   if (c->host_partials) (void)hipHostFree(c->host_partials);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

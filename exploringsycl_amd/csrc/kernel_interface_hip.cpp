@@ -19,6 +19,12 @@
 #include "kernel_interface.h"
 #include "tea_hip.h"
 
+// This project's solver extensions exist only with its own host headers.
+#if __has_include("kernel_interface_ext.h")
+#include "kernel_interface_ext.h"
+#define TEA_HAVE_EXT 1
+#endif
+
 namespace {
 
 void check(int status, const char* what, int line) {
@@ -262,6 +268,47 @@ void run_finalise(Chunk* chunk, Settings* settings) {
   HIP_CALL(tea_hip_finalise(H(chunk)));
   refresh_p(chunk);
 }
+
+#ifdef TEA_HAVE_EXT
+// ---- mixed-precision CG (kernel_interface_ext.h) ---------------------------
+
+void run_mpcg_init(Chunk* chunk, Settings* settings, double rx, double ry,
+                   double* rr, double* bb) {
+  for (int f = 0; f < NUM_FACES; ++f)
+    if (chunk->neighbours[f] != EXTERNAL_FACE)
+      die(__LINE__, __FILE__, "mixed-precision CG needs all faces external\n");
+  double a, b;
+  HIP_CALL(tea_hip_mpcg_init(H(chunk), settings->coefficient, rx, ry, &a, &b));
+  refresh_p(chunk);
+  *rr += a;
+  *bb += b;
+}
+
+void run_mpcg_calc_w(Chunk* chunk, Settings* settings, double alpha,
+                     int apply_u, double beta, int apply_p, double* pw) {
+  (void)settings;
+  double v;
+  HIP_CALL(tea_hip_mpcg_calc_w(H(chunk), alpha, apply_u, beta, apply_p, &v));
+  *pw += v;
+}
+
+void run_mpcg_calc_r(Chunk* chunk, Settings* settings, double alpha,
+                     double* rr) {
+  (void)settings;
+  double v;
+  HIP_CALL(tea_hip_mpcg_calc_r(H(chunk), alpha, &v));
+  *rr += v;
+}
+
+void run_mpcg_refresh(Chunk* chunk, Settings* settings, double alpha,
+                      int apply_u, double* rr, double* rr_true) {
+  (void)settings;
+  double a, b;
+  HIP_CALL(tea_hip_mpcg_refresh(H(chunk), alpha, apply_u, &a, &b));
+  *rr += a;
+  *rr_true += b;
+}
+#endif
 
 // Optional host hook (declared weak by the host): with TEA_HIP_STATS=<file>
 // set, "timed_begin" switches on per-kernel event timing and clears the

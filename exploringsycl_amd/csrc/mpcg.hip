@@ -1,0 +1,385 @@
+// libteahip.so: mixed-precision CG (no counterpart in the reference, whose
+// CG is FP64 throughout: TeaLeaf/c_kernels/sycl/cg.cpp).
+//
+// Storage: p, r, w and the face coefficients kx, ky in FP32; the solution u,
+// the right-hand side u0 and the FP64 kx, ky of the plain path stay FP64.
+// Arithmetic:
+//   - w = A p in FP32, in the operand order of TeaLeaf/shared.h:59-63;
+//   - p = beta p + r and r = r - alpha w in FP64, rounded once to FP32;
+//   - u += alpha p in FP64;
+//   - every dot product takes exact FP64 products of the FP32 values and
+//     sums them with the reference's reduction tree (as the FP64 path);
+//   - a true residual r = u0 - A u in FP64 (FP64 coefficients) replaces the
+//     recursive one every refresh interval and at convergence.
+// Bytes per cell per iteration: kernel 1 reads p, r (4+4), u (8), kx, ky
+// (4+4) and writes p, w (4+4), u (8) = 40; kernel 2 reads r, w and writes r
+// = 12; 52 in total against 88 for the FP64 iteration.
+//
+// Requires all four faces to be external (one chunk per GPU); the halo ring
+// of u and p is kept current by the kernels themselves, as in the fused FP64
+// iteration.
+
+#include <hip/hip_runtime.h>
+
+#include "tea_hip_impl.h"
+
+namespace tea {
+namespace {
+
+struct MpArgs {
+  const float* p_old;
+  float* p_new;
+  const float* r;
+  double* u;
+  float* w;
+  const float* kx;
+  const float* ky;
+  double alpha, beta;
+  int apply_u, apply_p;
+};
+
+template <int kBatch>
+__global__ __launch_bounds__(kBlock) void mp_w_kernel(Geo g, MpArgs a,
+                                                      double* partials,
+                                                      int nspan,
+                                                      int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  const int x = g.x;
+  const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
+  const float* __restrict__ P = a.p_old;
+  const float* __restrict__ R = a.r;
+  double prod[1][kItems];
+#pragma unroll
+  for (int b0 = 0; b0 < kItems; b0 += kBatch) {
+    int ci[kBatch], ck[kBatch], cj[kBatch];
+    bool in[kBatch];
+    float po[kBatch], pl[kBatch], pr[kBatch], pb[kBatch], pt[kBatch];
+    float rc[kBatch], rl[kBatch], rr[kBatch], rb[kBatch], rt[kBatch];
+    float kx0[kBatch], kx1[kBatch], ky0[kBatch], ky1[kBatch];
+    double uc[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int e = base + (b0 + j) * kBlock + threadIdx.x;
+      const int jj = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+      ci[j] = e;
+      cj[j] = jj;
+      ck[j] = e - jj * x;
+      in[j] = e < g.n && interior(g, ck[j], jj);
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int i = in[j] ? ci[j] : x + 1;
+      po[j] = P[i];
+      pl[j] = P[i - 1];
+      pr[j] = P[i + 1];
+      pb[j] = P[i - x];
+      pt[j] = P[i + x];
+      rc[j] = R[i];
+      rl[j] = R[i - 1];
+      rr[j] = R[i + 1];
+      rb[j] = R[i - x];
+      rt[j] = R[i + x];
+      kx0[j] = a.kx[i];
+      kx1[j] = a.kx[i + 1];
+      ky0[j] = a.ky[i];
+      ky1[j] = a.ky[i + x];
+      uc[j] = a.apply_u ? a.u[i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int i = ci[j], kk = ck[j], jj = cj[j];
+      float pc, l, rgt, bot, top;
+      if (a.apply_p) {
+        pc = (float)(a.beta * (double)po[j] + (double)rc[j]);
+        l = kk == xl ? pc : (float)(a.beta * (double)pl[j] + (double)rl[j]);
+        rgt = kk == xr ? pc : (float)(a.beta * (double)pr[j] + (double)rr[j]);
+        bot = jj == yb ? pc : (float)(a.beta * (double)pb[j] + (double)rb[j]);
+        top = jj == yt ? pc : (float)(a.beta * (double)pt[j] + (double)rt[j]);
+      } else {
+        pc = po[j];
+        l = pl[j];
+        rgt = pr[j];
+        bot = pb[j];
+        top = pt[j];
+      }
+      const float diag = (1.0f + (kx1[j] + kx0[j])) + (ky1[j] + ky0[j]);
+      const float s = (diag * pc - (kx1[j] * rgt + kx0[j] * l)) -
+                      (ky1[j] * top + ky0[j] * bot);
+      prod[0][b0 + j] = in[j] ? (double)s * (double)pc : 0.0;
+      if (!in[j]) continue;
+      a.w[i] = s;
+      const bool L = kk == xl, Rr = kk == xr, B = jj == yb, T = jj == yt;
+      if (a.apply_u) {
+        const double un = uc[j] + a.alpha * (double)po[j];
+        a.u[i] = un;
+        if (L) a.u[i - 1] = un;
+        if (Rr) a.u[i + 1] = un;
+        if (B) a.u[i - x] = un;
+        if (T) a.u[i + x] = un;
+        if (L && B) a.u[i - 1 - x] = un;
+        if (Rr && B) a.u[i + 1 - x] = un;
+        if (L && T) a.u[i - 1 + x] = un;
+        if (Rr && T) a.u[i + 1 + x] = un;
+      }
+      if (a.apply_p) {
+        float* __restrict__ Q = a.p_new;
+        Q[i] = pc;
+        if (L) Q[i - 1] = pc;
+        if (Rr) Q[i + 1] = pc;
+        if (B) Q[i - x] = pc;
+        if (T) Q[i + x] = pc;
+        if (L && B) Q[i - 1 - x] = pc;
+        if (Rr && B) Q[i + 1 - x] = pc;
+        if (L && T) Q[i - 1 + x] = pc;
+        if (Rr && T) Q[i + 1 + x] = pc;
+      }
+    }
+  }
+  span_reduce<1>(prod, partials, nspan, span, single_group != 0);
+}
+
+__global__ __launch_bounds__(kBlock) void mp_r_kernel(
+    Geo g, float* __restrict__ r, const float* __restrict__ w, double alpha,
+    double* partials, int nspan, int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  float rv[kItems], wv[kItems];
+  bool in[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int e = base + k * kBlock + threadIdx.x;
+    const int jj = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+    in[k] = e < g.n && interior(g, e - jj * g.x, jj);
+    const int i = in[k] ? e : 0;
+    rv[k] = r[i];
+    wv[k] = w[i];
+  }
+  double prod[1][kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const float rn = (float)((double)rv[k] - alpha * (double)wv[k]);
+    prod[0][k] = in[k] ? (double)rn * (double)rn : 0.0;
+    if (in[k]) r[base + k * kBlock + threadIdx.x] = rn;
+  }
+  span_reduce<1>(prod, partials, nspan, span, single_group != 0);
+}
+
+// FP32 mirrors after the FP64 initialisation: coefficients and p on every
+// cell (p's halo is already reflected), r on every cell; reduces
+// (r32.r32, u0.u0) over the interior.
+struct MpConvertBody {
+  const double* r;
+  const double* u0;
+  __device__ void operator()(int i, int, int, double (&out)[2]) const {
+    const float rv = (float)r[i];
+    out[0] = (double)rv * (double)rv;
+    out[1] = u0[i] * u0[i];
+  }
+};
+
+__global__ __launch_bounds__(kBlock) void mp_convert_all_kernel(
+    int n, const double* __restrict__ kx, const double* __restrict__ ky,
+    const double* __restrict__ p, const double* __restrict__ r,
+    float* __restrict__ kx32, float* __restrict__ ky32,
+    float* __restrict__ p32a, float* __restrict__ p32b,
+    float* __restrict__ r32) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  kx32[i] = (float)kx[i];
+  ky32[i] = (float)ky[i];
+  const float pv = (float)p[i];
+  p32a[i] = pv;
+  p32b[i] = pv;
+  r32[i] = (float)r[i];
+}
+
+// u += alpha p (FP64 u, FP32 p) on the interior and its reflection on the
+// depth-1 halo ring.
+struct MpApplyUBody {
+  double* u;
+  const float* p;
+  double alpha;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (!interior(g, kk, jj)) return;
+    const int x = g.x;
+    const double un = u[i] + alpha * (double)p[i];
+    u[i] = un;
+    const bool L = kk == g.h, R = kk == x - g.h - 1, B = jj == g.h,
+               T = jj == g.y - g.h - 1;
+    if (L) u[i - 1] = un;
+    if (R) u[i + 1] = un;
+    if (B) u[i - x] = un;
+    if (T) u[i + x] = un;
+    if (L && B) u[i - 1 - x] = un;
+    if (R && B) u[i + 1 - x] = un;
+    if (L && T) u[i - 1 + x] = un;
+    if (R && T) u[i + 1 + x] = un;
+  }
+};
+
+// r = u0 - A u in FP64 with the FP64 coefficients; stores r rounded to FP32
+// and reduces (r32.r32, r.r).
+struct MpTrueResidualBody {
+  const double* u;
+  const double* u0;
+  const double* kx;
+  const double* ky;
+  float* r32;
+  int x;
+  __device__ void operator()(int i, int, int, double (&out)[2]) const {
+    const double kx0 = kx[i], kx1 = kx[i + 1], ky0 = ky[i], ky1 = ky[i + x];
+    const double diag = (1.0 + (kx1 + kx0)) + (ky1 + ky0);
+    const double au = (diag * u[i] - (kx1 * u[i + 1] + kx0 * u[i - 1])) -
+                      (ky1 * u[i + x] + ky0 * u[i - x]);
+    const double rt = u0[i] - au;
+    const float rf = (float)rt;
+    r32[i] = rf;
+    out[0] = (double)rf * (double)rf;
+    out[1] = rt * rt;
+  }
+};
+
+int ensure_mp_buffers(tea_hip_chunk* c) {
+  if (c->mp_r) return TEA_HIP_OK;
+  const size_t bytes = (size_t)c->geo.n * sizeof(float);
+  float** bufs[] = {&c->mp_p[0], &c->mp_p[1], &c->mp_r, &c->mp_w, &c->mp_kx,
+                    &c->mp_ky};
+  for (float** b : bufs) {
+    if (hipMalloc(b, bytes) != hipSuccess) {
+      *b = nullptr;
+      set_error("hipMalloc of an FP32 CG buffer failed");
+      return TEA_HIP_ENOMEM;
+    }
+    TEA_HIP_TRY(hipMemsetAsync(*b, 0, bytes, c->stream));
+  }
+  return TEA_HIP_OK;
+}
+
+}  // namespace
+}  // namespace tea
+
+using namespace tea;
+
+extern "C" {
+
+int tea_hip_mpcg_init(tea_hip_chunk* c, int coefficient, double rx, double ry,
+                      double* rr, double* bb) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(rr && bb, "null outputs");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_RETURN_IF(ensure_mp_buffers(c));
+  // FP64 initialisation exactly as the plain path: kx, ky, u, w, r, p.
+  double rro64 = 0.0;
+  TEA_RETURN_IF(tea_hip_cg_init(c, coefficient, rx, ry, &rro64));
+  static const int ext[4] = {-1, -1, -1, -1};
+  int fields[TEA_HALO_NUM] = {0};
+  fields[TEA_HALO_U] = fields[TEA_HALO_P] = 1;
+  TEA_RETURN_IF(launch_halos(c, 1, ext, fields));
+  TEA_RETURN_IF(tea_hip_copy_u(c));
+  double** f = c->field;
+  const Geo g = c->geo;
+  {
+    Timed t(c, K_MP_CONVERT);
+    hipLaunchKernelGGL(mp_convert_all_kernel, flat_grid(g.n), dim3(kBlock), 0,
+                       c->stream, g.n, f[TEA_FIELD_KX], f[TEA_FIELD_KY],
+                       f[TEA_FIELD_P], f[TEA_FIELD_R], c->mp_kx, c->mp_ky,
+                       c->mp_p[0], c->mp_p[1], c->mp_r);
+    MpConvertBody body{f[TEA_FIELD_R], f[TEA_FIELD_U0]};
+    hipLaunchKernelGGL((span_kernel<2, MpConvertBody>), span_grid(c),
+                       dim3(kBlock), 0, c->stream, g, body, c->partials,
+                       c->nspan, c->ngroup == 1 ? 1 : 0);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  c->mp_cur = 0;
+  double out[2];
+  TEA_RETURN_IF(finish_reduction(c, 2, out));
+  *rr = out[0];
+  *bb = out[1];
+  return TEA_HIP_OK;
+}
+
+int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
+                        double beta, int apply_p, double* pw) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
+  TEA_CHECK_ARG(pw != nullptr, "null pw");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->mp_r,
+           c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky,
+           alpha, beta, apply_u ? 1 : 0, apply_p ? 1 : 0};
+  {
+    Timed t(c, K_MP_W);
+    hipLaunchKernelGGL((mp_w_kernel<4>), span_grid(c), dim3(kBlock), 0,
+                       c->stream, c->geo, a, c->partials, c->nspan,
+                       c->ngroup == 1 ? 1 : 0);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  if (apply_p) c->mp_cur ^= 1;
+  return finish_reduction(c, 1, pw);
+}
+
+int tea_hip_mpcg_calc_r(tea_hip_chunk* c, double alpha, double* rr) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
+  TEA_CHECK_ARG(rr != nullptr, "null rr");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  {
+    Timed t(c, K_MP_R);
+    hipLaunchKernelGGL(mp_r_kernel, span_grid(c), dim3(kBlock), 0, c->stream,
+                       c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
+                       c->ngroup == 1 ? 1 : 0);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  return finish_reduction(c, 1, rr);
+}
+
+int tea_hip_mpcg_refresh(tea_hip_chunk* c, double alpha, int apply_u,
+                         double* rr, double* rr_true) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
+  TEA_CHECK_ARG(rr && rr_true, "null outputs");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  double** f = c->field;
+  const Geo g = c->geo;
+  if (apply_u) {
+    Timed t(c, K_MP_APPLY_U);
+    MpApplyUBody body{f[TEA_FIELD_U], c->mp_p[c->mp_cur], alpha, g};
+    hipLaunchKernelGGL((each_kernel<MpApplyUBody>), flat_grid(g.n),
+                       dim3(kBlock), 0, c->stream, g, body);
+  }
+  {
+    Timed t(c, K_MP_REFRESH);
+    MpTrueResidualBody body{f[TEA_FIELD_U], f[TEA_FIELD_U0], f[TEA_FIELD_KX],
+                            f[TEA_FIELD_KY], c->mp_r, g.x};
+    hipLaunchKernelGGL((span_kernel<2, MpTrueResidualBody>), span_grid(c),
+                       dim3(kBlock), 0, c->stream, g, body, c->partials,
+                       c->nspan, c->ngroup == 1 ? 1 : 0);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  double out[2];
+  TEA_RETURN_IF(finish_reduction(c, 2, out));
+  *rr = out[0];
+  *rr_true = out[1];
+  return TEA_HIP_OK;
+}
+
+int tea_hip_mp_read(tea_hip_chunk* c, int which, float* host, size_t count) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
+  TEA_CHECK_ARG(count <= (size_t)c->geo.n, "count exceeds the field length");
+  const float* src = which == TEA_MP_P    ? c->mp_p[c->mp_cur]
+                     : which == TEA_MP_R  ? c->mp_r
+                     : which == TEA_MP_W  ? c->mp_w
+                     : which == TEA_MP_KX ? c->mp_kx
+                     : which == TEA_MP_KY ? c->mp_ky
+                                          : nullptr;
+  TEA_CHECK_ARG(src != nullptr, "bad FP32 field id");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_HIP_TRY(hipMemcpyAsync(host, src, count * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+  return sync_chunk(c);
+}
+
+}  // extern "C"

@@ -227,6 +227,11 @@ enum KernelId {
   K_CALC_RESIDUAL,
   K_CALC_2NORM,
   K_FINALISE,
+  K_MP_CONVERT,
+  K_MP_W,
+  K_MP_R,
+  K_MP_APPLY_U,
+  K_MP_REFRESH,
   K_NUM
 };
 
@@ -259,9 +264,19 @@ struct tea_hip_chunk {
 
   // Deferred CG work (fused mode).
   int fused = 1;
-  int batch = 4;  // cells per load batch in the fused stencil kernel
+  int batch = 0;   // load batch of the fused stencil kernel (0: default)
+  bool pairs = false;  // two-cells-per-lane kernels (x even)
   bool pend_u = false, pend_p = false, pend_halo = false;
   double pend_alpha = 0.0, pend_beta = 0.0;
+
+  // Mixed-precision CG: FP32 p (two buffers, mp_cur is current), r, w and
+  // face coefficients; allocated by the first tea_hip_mpcg_init.
+  float* mp_p[2] = {nullptr, nullptr};
+  int mp_cur = 0;
+  float* mp_r = nullptr;
+  float* mp_w = nullptr;
+  float* mp_kx = nullptr;
+  float* mp_ky = nullptr;
 
   // Timing.
   bool timing = false;

@@ -7,6 +7,7 @@
 #include "comms.h"
 #include "drivers.h"
 #include "kernel_interface.h"
+#include "kernel_interface_ext.h"
 #include "tea_host.h"
 
 #define FOR_CHUNKS(settings, cc) \
@@ -46,16 +47,30 @@ void halo_update_driver(Chunk* chunks, Settings* settings, int depth) {
 
 // ---- CG -------------------------------------------------------------------
 
+// With rel_eps set (this project's addition) the stop test is
+// |r| < rel_eps |b| with b = u0, instead of the reference's absolute
+// sqrt(|rr|) < eps (TeaLeaf/drivers/cg_driver.c:24).
 void cg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
                double* error) {
   double rro = 0.0;
   cg_init_driver(chunks, settings, rx, ry, &rro);
+  double bb = 0.0;
+  if (g_extra.rel_eps > 0.0) {
+    FOR_CHUNKS(settings, cc)
+      run_calculate_2norm(&chunks[cc], settings, chunks[cc].u0, &bb);
+    sum_over_ranks(settings, &bb);
+  }
+  const double tol2 = g_extra.rel_eps * g_extra.rel_eps * bb;
   int tt;
   for (tt = 0; tt < settings->max_iters; ++tt) {
     cg_main_step_driver(chunks, settings, tt, &rro, error);
     halo_update_driver(chunks, settings, 1);
-    if (std::sqrt(std::fabs(*error)) < settings->eps) break;
+    if (g_extra.rel_eps > 0.0 ? std::fabs(*error) < tol2
+                              : std::sqrt(std::fabs(*error)) < settings->eps)
+      break;
   }
+  if (bb > 0.0 && !g_run.steps.empty())
+    g_run.steps.back().rel_residual = std::sqrt(std::fabs(*error) / bb);
   print_and_log(settings, "CG: \t\t\t%d iterations\n", tt);
   if (!g_run.steps.empty()) {
     g_run.steps.back().iterations = tt;
@@ -171,5 +186,76 @@ void field_summary_driver(Chunk* chunks, Settings* settings,
     g_run.check_passed = qa_diff < 0.001;
     print_and_log(settings, g_run.check_passed ? "This run PASSED" : "This run FAILED");
     print_and_log(settings, " (Difference is within %.8lf%%)\n", qa_diff);
+  }
+}
+
+// ---- mixed-precision CG -------------------------------------------------------
+//
+// Per iteration: pw = p.w (applying the previous alpha to u and beta to p in
+// the same pass), alpha = rr/pw, rr' = r.r after r -= alpha w, beta = rr'/rr.
+// Every refresh_interval iterations, and whenever the recursive residual
+// meets the tolerance, the FP64 true residual replaces r (u brought up to
+// date first); the solve stops when the TRUE relative residual
+// |b - A u| / |b| < rel_eps.  CG keeps its search direction across a
+// refresh (reliable updates).
+void mpcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+                 double* error) {
+  if (settings->num_chunks_per_rank != 1)
+    die(__LINE__, __FILE__, "mixed-precision CG drives one chunk per GPU\n");
+  Chunk* c = &chunks[0];
+  const double rel = g_extra.rel_eps > 0.0 ? g_extra.rel_eps : 1e-10;
+  const int refresh = g_extra.refresh_interval > 0 ? g_extra.refresh_interval : 50;
+  double rr = 0.0, bb = 0.0;
+  run_mpcg_init(c, settings, rx, ry, &rr, &bb);
+  const double tol2 = rel * rel * bb;
+
+  double alpha = 0.0, beta = 0.0, rr_true = rr;
+  bool pend_u = false, pend_p = false, converged = false;
+  int it = 0, since = 0, refreshes = 0;
+  for (it = 0; it < settings->max_iters; ++it) {
+    double pw = 0.0;
+    run_mpcg_calc_w(c, settings, alpha, pend_u, beta, pend_p, &pw);
+    alpha = rr / pw;
+    c->cg_alphas[it] = alpha;
+    double rrn = 0.0;
+    run_mpcg_calc_r(c, settings, alpha, &rrn);
+    pend_u = true;
+    ++since;
+    if (since >= refresh || rrn < tol2) {
+      double rt = 0.0;
+      rrn = 0.0;
+      run_mpcg_refresh(c, settings, alpha, 1, &rrn, &rt);
+      pend_u = false;
+      since = 0;
+      ++refreshes;
+      rr_true = rt;
+      if (rt < tol2) {
+        converged = true;
+        ++it;
+        break;
+      }
+    }
+    beta = rrn / rr;
+    c->cg_betas[it] = beta;
+    pend_p = true;
+    rr = rrn;
+  }
+  if (!converged && pend_u) {
+    double rrn = 0.0, rt = 0.0;
+    run_mpcg_refresh(c, settings, alpha, 1, &rrn, &rt);
+    ++refreshes;
+    rr_true = rt;
+  }
+  *error = rr_true;
+  const int printed = converged ? it - 1 : it;
+  print_and_log(settings, "MPCG: \t\t\t%d iterations (%d FP64 refreshes), "
+                "relative residual %.3e\n", printed, refreshes,
+                std::sqrt(rr_true / bb));
+  if (!g_run.steps.empty()) {
+    StepRecord& s = g_run.steps.back();
+    s.iterations = printed;
+    s.executed = it;
+    s.refreshes = refreshes;
+    s.rel_residual = std::sqrt(rr_true / bb);
   }
 }

@@ -23,6 +23,11 @@ void cg_init_driver(Chunk* chunks, Settings* settings, double rx, double ry,
 void cg_main_step_driver(Chunk* chunks, Settings* settings, int tt,
                          double* rro, double* error);
 
+/* Mixed-precision CG (this project's addition): relative-residual stop
+ * (rel_eps, default 1e-10) confirmed by the FP64 true residual. */
+void mpcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+                 double* error);
+
 void field_summary_driver(Chunk* chunks, Settings* settings,
                           bool solve_finished);
 void store_energy_driver(Chunk* chunk, Settings* settings);

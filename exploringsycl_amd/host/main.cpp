@@ -18,6 +18,7 @@
 
 RunRecord g_run;
 HostOptions g_opts;
+SolverExtras g_extra;
 
 static double now_seconds() {
   struct timespec t;
@@ -140,14 +141,18 @@ void write_run_json(const char* path, Settings* settings) {
           settings->grid_x_cells, settings->grid_y_cells, settings->end_step);
   fprintf(fp, "  \"coefficient\": %d, \"eps\": %.17g, \"max_iters\": %d,\n",
           settings->coefficient, settings->eps, settings->max_iters);
+  fprintf(fp, "  \"solver\": \"%s\", \"rel_eps\": %.17g,\n", settings->solver_name,
+          g_extra.rel_eps);
   fprintf(fp, "  \"warmup_steps\": %d, \"timed_seconds\": %.9f,\n",
           g_run.warmup_steps, g_run.timed_seconds);
   fprintf(fp, "  \"steps\": [");
   for (size_t i = 0; i < g_run.steps.size(); ++i) {
     const StepRecord& s = g_run.steps[i];
     fprintf(fp, "%s\n    {\"step\": %d, \"iterations\": %d, \"executed\": %d, "
+                "\"refreshes\": %d, \"rel_residual\": %.17g, "
                 "\"error\": %.17g, \"seconds\": %.9f}",
-            i ? "," : "", s.step, s.iterations, s.executed, s.error, s.seconds);
+            i ? "," : "", s.step, s.iterations, s.executed, s.refreshes,
+            s.rel_residual, s.error, s.seconds);
   }
   fprintf(fp, "\n  ],\n  \"summaries\": [");
   for (size_t i = 0; i < g_run.summaries.size(); ++i) {

@@ -98,6 +98,9 @@ void read_settings(const std::vector<std::string>& lines, Settings* s) {
     if (get_double("eps", line, &s->eps)) continue;
     if (get_int("num_chunks_per_rank", line, &s->num_chunks_per_rank)) continue;
     if (get_int("halo_depth", line, &s->halo_depth)) continue;
+    // Keys of this project's solvers.
+    if (get_double("rel_eps", line, &g_extra.rel_eps)) continue;
+    if (get_int("refresh_interval", line, &g_extra.refresh_interval)) continue;
 
     // Switches.
     if (starts_with("check_result", line)) { s->check_result = true; continue; }
@@ -105,6 +108,21 @@ void read_settings(const std::vector<std::string>& lines, Settings* s) {
     if (starts_with("preconditioner_on", line)) { s->preconditioner = true; continue; }
     if (starts_with("use_fortran_kernels", line)) { s->kernel_language = FORTRAN; continue; }
     if (starts_with("use_c_kernels", line)) { s->kernel_language = C; continue; }
+    if (starts_with("use_jacobi_pcg", line)) {
+      s->solver = JACOBI_PCG_SOLVER;
+      strcpy(s->solver_name, "Jacobi-PCG");
+      continue;
+    }
+    if (starts_with("use_mpcg", line)) {
+      s->solver = MPCG_SOLVER;
+      strcpy(s->solver_name, "MPCG");
+      continue;
+    }
+    if (starts_with("use_fcg", line)) {
+      s->solver = FCG_SOLVER;
+      strcpy(s->solver_name, "FCG");
+      continue;
+    }
     if (starts_with("use_jacobi", line)) {
       s->solver = JACOBI_SOLVER;
       strcpy(s->solver_name, "Jacobi");

@@ -45,7 +45,17 @@
 #define DEF_RANK 0
 #define DEF_IS_OFFLOAD false
 
-typedef enum { JACOBI_SOLVER, CG_SOLVER, CHEBY_SOLVER, PPCG_SOLVER } Solver;
+/* The first four are the reference's; the rest are this project's
+ * additions (appended, so the reference's values are unchanged). */
+typedef enum {
+  JACOBI_SOLVER,
+  CG_SOLVER,
+  CHEBY_SOLVER,
+  PPCG_SOLVER,
+  MPCG_SOLVER,       /* mixed-precision CG */
+  FCG_SOLVER,        /* flexible CG with the learned preconditioner */
+  JACOBI_PCG_SOLVER  /* diagonally preconditioned CG */
+} Solver;
 
 typedef enum { C, FORTRAN } Kernel_Language;
 

@@ -15,6 +15,8 @@ struct StepRecord {
   int step;           /* 1-based timestep */
   int iterations;     /* as printed: "CG: N iterations" */
   int executed;       /* CG iterations actually run (printed + 1 on convergence) */
+  int refreshes;      /* FP64 true-residual refreshes (mixed precision) */
+  double rel_residual; /* final |b - A u| / |b| where the solver computes it */
   double error;       /* last rr of the solve */
   double seconds;     /* wall time of the whole timestep */
 };
@@ -45,6 +47,15 @@ struct HostOptions {
 };
 
 extern HostOptions g_opts;
+
+/* Solver settings beyond the reference's tea.in keys (parsed from tea.in;
+ * kept out of Settings so its layout stays the reference's). */
+struct SolverExtras {
+  double rel_eps = 0.0;      /* rel_eps=: stop when |r|/|b| < rel_eps (0: off) */
+  int refresh_interval = 50; /* refresh_interval=: FP64 true residual period */
+};
+
+extern SolverExtras g_extra;
 
 /* The states read from tea.in (kept for --warmup-steps resets). */
 extern State* g_states;

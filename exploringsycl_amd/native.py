@@ -26,6 +26,7 @@ FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 HALO_SLOTS = ["density", "energy0", "energy1", "u", "p", "sd"]
 CONDUCTIVITY, RECIP_CONDUCTIVITY = 1, 2
 GEOMETRY = {"rectangle": 0, "circular": 1, "point": 2}
+MP_FIELDS = {"p": 0, "r": 1, "w": 2, "kx": 3, "ky": 4}
 
 _lib = None
 
@@ -55,6 +56,11 @@ _SIGNATURES = {
     "tea_hip_calculate_residual": [_vp],
     "tea_hip_calculate_2norm": [_vp, _vp, _dp],
     "tea_hip_finalise": [_vp],
+    "tea_hip_mpcg_init": [_vp, _i, _d, _d, _dp, _dp],
+    "tea_hip_mpcg_calc_w": [_vp, _d, _i, _d, _i, _dp],
+    "tea_hip_mpcg_calc_r": [_vp, _d, _dp],
+    "tea_hip_mpcg_refresh": [_vp, _d, _i, _dp, _dp],
+    "tea_hip_mp_read": [_vp, _i, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t],
     "tea_hip_set_fused": [_vp, _i],
     "tea_hip_flush": [_vp],
     "tea_hip_timing_enable": [_vp, _i],

@@ -188,6 +188,41 @@ int tea_hip_calculate_2norm(tea_hip_chunk* chunk, const double* buffer,
  * (TeaLeaf/kernel_interface.h:69-70; solver_methods.cpp:120-145). */
 int tea_hip_finalise(tea_hip_chunk* chunk);
 
+/* ---- mixed-precision CG --------------------------------------------------
+ * No counterpart in the reference (its CG is FP64 throughout).  p, r, w and
+ * the face coefficients are stored in FP32; u, u0, the dot products, alpha
+ * and beta stay FP64, and tea_hip_mpcg_refresh replaces the recursive
+ * residual by the FP64 true residual u0 - A u.  One chunk per GPU: all four
+ * faces must be external.  A solve is: init; then per iteration calc_w
+ * (applying the previous alpha to u and beta to p) and calc_r; refresh every
+ * few iterations and at convergence.  Dot products use the same reduction
+ * tree as the FP64 path. */
+
+/* FP64 initialisation as tea_hip_cg_init, the depth-1 U/P halos and
+ * u0 = u (tea_hip_copy_u), then the FP32 copies; *rr = r.r of the FP32
+ * residual, *bb = u0.u0 (the right-hand side's squared norm). */
+int tea_hip_mpcg_init(tea_hip_chunk* chunk, int coefficient, double rx,
+                      double ry, double* rr, double* bb);
+
+/* If apply_u: u += alpha p_old (FP64).  If apply_p: p = beta p_old + r.
+ * Then w = A p (FP32) and *pw = p.w. */
+int tea_hip_mpcg_calc_w(tea_hip_chunk* chunk, double alpha, int apply_u,
+                        double beta, int apply_p, double* pw);
+
+/* r -= alpha w and *rr = r.r. */
+int tea_hip_mpcg_calc_r(tea_hip_chunk* chunk, double alpha, double* rr);
+
+/* If apply_u: u += alpha p.  Then r = u0 - A u in FP64 (stored as FP32);
+ * *rr = r.r of the stored FP32 residual, *rr_true = the same of the FP64
+ * residual.  u is then current (halo ring included). */
+int tea_hip_mpcg_refresh(tea_hip_chunk* chunk, double alpha, int apply_u,
+                         double* rr, double* rr_true);
+
+/* Copies one of the FP32 arrays (TEA_MP_*) to host memory. */
+enum { TEA_MP_P = 0, TEA_MP_R = 1, TEA_MP_W = 2, TEA_MP_KX = 3, TEA_MP_KY = 4 };
+int tea_hip_mp_read(tea_hip_chunk* chunk, int which, float* host,
+                    size_t count);
+
 /* ---- execution mode and measurement ------------------------------------ */
 
 /* Mode 1 (default): CG calls are fused.  tea_hip_cg_calc_ur updates r and

@@ -8,6 +8,14 @@ typedef double* FieldBufferType;
 
 typedef struct ChunkExtension {
   FieldBufferType comms_buffer;
+  /* Mixed-precision CG arrays (this project's addition): FP32 p (two
+   * buffers, mp_cur is current), r, w, kx, ky. */
+  float* mp_p[2];
+  int mp_cur;
+  float* mp_r;
+  float* mp_w;
+  float* mp_kx;
+  float* mp_ky;
 } ChunkExtension;
 
 #endif

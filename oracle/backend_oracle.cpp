@@ -12,6 +12,11 @@
 #include "kernel_interface.h"
 #include "tea_oracle.h"
 
+#if __has_include("kernel_interface_ext.h")
+#include "kernel_interface_ext.h"
+#define TEA_HAVE_EXT 1
+#endif
+
 static double* alloc_field(size_t n) {
   double* p = (double*)calloc(n ? n : 1, sizeof(double));
   if (!p) die(__LINE__, __FILE__, "Error allocating buffer\n");
@@ -45,6 +50,10 @@ void run_kernel_initialise(Chunk* c, Settings* s) {
   c->vertex_x = alloc_field(x + 1);
   c->vertex_y = alloc_field(y + 1);
   c->ext->comms_buffer = alloc_field((x > y ? x : y) * s->halo_depth);
+  // The reference host mallocs the extension without clearing it.
+  c->ext->mp_p[0] = c->ext->mp_p[1] = nullptr;
+  c->ext->mp_r = c->ext->mp_w = c->ext->mp_kx = c->ext->mp_ky = nullptr;
+  c->ext->mp_cur = 0;
   const int n = s->max_iters > 0 ? s->max_iters : 1;
   c->cg_alphas = new double[n]();
   c->cg_betas = new double[n]();
@@ -60,6 +69,9 @@ void run_kernel_finalise(Chunk* c, Settings* s) {
                  c->cell_dx, c->cell_dy, c->vertex_dx, c->vertex_dy,
                  c->vertex_x, c->vertex_y, c->ext->comms_buffer};
   for (double* p : f) free(p);
+  float* g[] = {c->ext->mp_p[0], c->ext->mp_p[1], c->ext->mp_r, c->ext->mp_w,
+                c->ext->mp_kx, c->ext->mp_ky};
+  for (float* p : g) free(p);
   delete[] c->cg_alphas;
   delete[] c->cg_betas;
   delete[] c->cheby_alphas;
@@ -174,3 +186,55 @@ void run_calculate_2norm(Chunk* c, Settings* s, FieldBufferType buffer,
 void run_finalise(Chunk* c, Settings* s) {
   orc_finalise(c->x, c->y, s->halo_depth, c->u, c->density, c->energy);
 }
+
+#ifdef TEA_HAVE_EXT
+// ---- mixed-precision CG (kernel_interface_ext.h) -----------------------------
+
+void run_mpcg_init(Chunk* c, Settings* s, double rx, double ry, double* rr,
+                   double* bb) {
+  ChunkExtension* e = c->ext;
+  const size_t n = (size_t)c->x * c->y;
+  if (!e->mp_r) {
+    float** bufs[] = {&e->mp_p[0], &e->mp_p[1], &e->mp_r, &e->mp_w, &e->mp_kx,
+                      &e->mp_ky};
+    for (float** b : bufs) *b = (float*)calloc(n, sizeof(float));
+  }
+  const int h = s->halo_depth;
+  (void)orc_cg_init(c->x, c->y, h, s->coefficient, rx, ry, c->p, c->r, c->u,
+                    c->w, c->kx, c->ky, c->density, c->energy);
+  orc_halo_field(c->x, c->y, h, 1, c->neighbours, c->u);
+  orc_halo_field(c->x, c->y, h, 1, c->neighbours, c->p);
+  orc_copy_u(c->x, c->y, h, c->u, c->u0);
+  double out[2];
+  orc_mpcg_convert(c->x, c->y, h, c->kx, c->ky, c->p, c->r, c->u0, e->mp_kx,
+                   e->mp_ky, e->mp_p[0], e->mp_r, out);
+  memcpy(e->mp_p[1], e->mp_p[0], n * sizeof(float));
+  e->mp_cur = 0;
+  *rr += out[0];
+  *bb += out[1];
+}
+
+void run_mpcg_calc_w(Chunk* c, Settings* s, double alpha, int apply_u,
+                     double beta, int apply_p, double* pw) {
+  ChunkExtension* e = c->ext;
+  *pw += orc_mpcg_calc_w(c->x, c->y, s->halo_depth, alpha, apply_u, beta,
+                         apply_p, e->mp_p[e->mp_cur], e->mp_p[e->mp_cur ^ 1],
+                         e->mp_r, c->u, e->mp_w, e->mp_kx, e->mp_ky);
+  if (apply_p) e->mp_cur ^= 1;
+}
+
+void run_mpcg_calc_r(Chunk* c, Settings* s, double alpha, double* rr) {
+  *rr += orc_mpcg_calc_r(c->x, c->y, s->halo_depth, alpha, c->ext->mp_r,
+                         c->ext->mp_w);
+}
+
+void run_mpcg_refresh(Chunk* c, Settings* s, double alpha, int apply_u,
+                      double* rr, double* rr_true) {
+  ChunkExtension* e = c->ext;
+  double out[2];
+  orc_mpcg_refresh(c->x, c->y, s->halo_depth, alpha, apply_u, c->u,
+                   e->mp_p[e->mp_cur], c->u0, c->kx, c->ky, e->mp_r, out);
+  *rr += out[0];
+  *rr_true += out[1];
+}
+#endif

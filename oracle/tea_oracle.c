@@ -364,3 +364,132 @@ void orc_finalise(int x, int y, int h, const double* u, const double* density,
       energy[i] = u[i] / density[i];
     }
 }
+
+/* ---- mixed-precision CG (no reference counterpart) ----------------------
+ * Restates exploringsycl_amd/csrc/mpcg.hip: w = A p in FP32 in the SMVP
+ * operand order; p = beta p + r and r = r - alpha w in FP64 rounded once to
+ * FP32; u += alpha p in FP64; dot products of exact FP64 products with the
+ * reference's tree.  Pinned by the FP64 path it reduces to (tests compare
+ * its converged solution with the FP64 CG's) and bitwise by the GPU tests. */
+
+static const int kAllExternal[4] = {-1, -1, -1, -1};
+
+static float smvp32(const float* a, const float* kx, const float* ky, long i,
+                    long x) {
+  const float t1 = kx[i + 1] + kx[i];
+  const float t2 = ky[i + x] + ky[i];
+  const float diag = (1.0f + t1) + t2;
+  const float c = diag * a[i];
+  const float bx = kx[i + 1] * a[i + 1] + kx[i] * a[i - 1];
+  const float by = ky[i + x] * a[i + x] + ky[i] * a[i - x];
+  return (c - bx) - by;
+}
+
+static void halo1_float(int x, int y, int h, float* b) {
+  /* orc_halo_field on a float array: copy through a double image. */
+  const long n = (long)x * y;
+  double* t = (double*)malloc(n * sizeof(double));
+  for (long i = 0; i < n; ++i) t[i] = b[i];
+  orc_halo_field(x, y, h, 1, kAllExternal, t);
+  for (long i = 0; i < n; ++i) b[i] = (float)t[i];
+  free(t);
+}
+
+void orc_mpcg_convert(int x, int y, int h, const double* kx, const double* ky,
+                      const double* p, const double* r, const double* u0,
+                      float* kx32, float* ky32, float* p32, float* r32,
+                      double out[2]) {
+  const long n = (long)x * y;
+  for (long i = 0; i < n; ++i) {
+    kx32[i] = (float)kx[i];
+    ky32[i] = (float)ky[i];
+    p32[i] = (float)p[i];
+    r32[i] = (float)r[i];
+  }
+  double* part0 = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part0, { val = (double)r32[i] * (double)r32[i]; });
+  out[0] = finish(part0, x, y);
+  double* part1 = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part1, { val = u0[i] * u0[i]; });
+  out[1] = finish(part1, x, y);
+}
+
+double orc_mpcg_calc_w(int x, int y, int h, double alpha, int apply_u,
+                       double beta, int apply_p, const float* p_old,
+                       float* p_new, const float* r, double* u, float* w,
+                       const float* kx, const float* ky) {
+  if (apply_u) {
+    for (long jj = h; jj < y - h; ++jj)
+      for (long kk = h; kk < x - h; ++kk) {
+        const long i = kk + jj * x;
+        u[i] = u[i] + alpha * (double)p_old[i];
+      }
+    orc_halo_field(x, y, h, 1, kAllExternal, u);
+  }
+  const float* p = p_old;
+  if (apply_p) {
+    for (long jj = h; jj < y - h; ++jj)
+      for (long kk = h; kk < x - h; ++kk) {
+        const long i = kk + jj * x;
+        p_new[i] = (float)(beta * (double)p_old[i] + (double)r[i]);
+      }
+    halo1_float(x, y, h, p_new);
+    p = p_new;
+  }
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, {
+    const float s = smvp32(p, kx, ky, i, x);
+    w[i] = s;
+    val = (double)s * (double)p[i];
+  });
+  return finish(part, x, y);
+}
+
+double orc_mpcg_calc_r(int x, int y, int h, double alpha, float* r,
+                       const float* w) {
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, {
+    const float rn = (float)((double)r[i] - alpha * (double)w[i]);
+    r[i] = rn;
+    val = (double)rn * (double)rn;
+  });
+  return finish(part, x, y);
+}
+
+void orc_mpcg_refresh(int x, int y, int h, double alpha, int apply_u,
+                      double* u, const float* p, const double* u0,
+                      const double* kx, const double* ky, float* r32,
+                      double out[2]) {
+  if (apply_u) {
+    for (long jj = h; jj < y - h; ++jj)
+      for (long kk = h; kk < x - h; ++kk) {
+        const long i = kk + jj * x;
+        u[i] = u[i] + alpha * (double)p[i];
+      }
+    orc_halo_field(x, y, h, 1, kAllExternal, u);
+  }
+  double* part0 = alloc_partials(x, y);
+  double* part1 = alloc_partials(x, y);
+  const long n_ = (long)x * y, groups_ = (n_ + GROUP - 1) / GROUP;
+#pragma omp parallel for schedule(static)
+  for (long g = 0; g < groups_; ++g) {
+    double a0[GROUP], a1[GROUP];
+    long i0 = g * GROUP, jj = i0 / x, kk = i0 - jj * x;
+    for (int l = 0; l < GROUP; ++l) {
+      const long i = i0 + l;
+      a0[l] = a1[l] = 0.0;
+      if (i < n_ && is_interior(kk, jj, x, y, h)) {
+        const double rt = u0[i] - orc_smvp(u, kx, ky, i, x);
+        const float rf = (float)rt;
+        r32[i] = rf;
+        a0[l] = (double)rf * (double)rf;
+        a1[l] = rt * rt;
+      }
+      if (++kk == x) { kk = 0; ++jj; }
+    }
+    part0[g] = orc_tree64(a0);
+    part1[g] = orc_tree64(a1);
+  }
+  out[0] = finish(part0, x, y);
+  out[1] = finish(part1, x, y);
+}

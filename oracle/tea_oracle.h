@@ -96,6 +96,25 @@ double orc_calculate_2norm(int x, int y, int h, const double* b);
 void orc_finalise(int x, int y, int h, const double* u, const double* density,
                   double* energy);
 
+/* Mixed-precision CG (no reference counterpart; restates the arithmetic of
+ * exploringsycl_amd/csrc/mpcg.hip): FP32 p, r, w, kx, ky; FP64 u, u0,
+ * dot products and scalars.  Halo updates of p and u use orc_halo_field
+ * (depth 1, all faces external). */
+void orc_mpcg_convert(int x, int y, int h, const double* kx, const double* ky,
+                      const double* p, const double* r, const double* u0,
+                      float* kx32, float* ky32, float* p32, float* r32,
+                      double out[2]);
+double orc_mpcg_calc_w(int x, int y, int h, double alpha, int apply_u,
+                       double beta, int apply_p, const float* p_old,
+                       float* p_new, const float* r, double* u, float* w,
+                       const float* kx, const float* ky);
+double orc_mpcg_calc_r(int x, int y, int h, double alpha, float* r,
+                       const float* w);
+void orc_mpcg_refresh(int x, int y, int h, double alpha, int apply_u,
+                      double* u, const float* p, const double* u0,
+                      const double* kx, const double* ky, float* r32,
+                      double out[2]);
+
 /* The 5-point operator applied at one cell (shared.h:59-63). */
 double orc_smvp(const double* a, const double* kx, const double* ky, long i,
                 long x);

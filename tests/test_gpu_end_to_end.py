@@ -29,14 +29,26 @@ DECKS = {
     "config1_512x512x10": config_deck(1),
     "recip_200x300_rects": Deck.with_random_rectangles(
         200, 300, 16, problem_seed(2), end_step=3, coefficient="inverse"),
+    "odd_99x77x2": Deck.with_random_rectangles(
+        99, 77, 8, problem_seed(2, 5), end_step=2, coefficient="inverse"),
+    "relstop_300x256x2": Deck.with_random_rectangles(
+        300, 256, 12, problem_seed(3, 1), density="log10",
+        density_range=(-1.0, 2.0), end_step=2, coefficient="inverse",
+        extra=["rel_eps=1e-10"]),
 }
+
+# (fused iteration, two-cells-per-lane kernels): the default, the one-cell
+# kernels, and one launch per reference kernel.
+MODES = {"fused_pairs": {"TEA_HIP_FUSED": "1"},
+         "fused_single": {"TEA_HIP_FUSED": "1", "TEA_HIP_PAIRS": "0"},
+         "unfused": {"TEA_HIP_FUSED": "0"}}
 
 
 @pytest.mark.parametrize("name", sorted(DECKS))
-@pytest.mark.parametrize("fused", ["1", "0"])
-def test_matches_oracle_host(name, fused):
+@pytest.mark.parametrize("mode", sorted(MODES))
+def test_matches_oracle_host(name, mode):
     deck = DECKS[name]
-    gpu, _ = run_deck(deck, env={"TEA_HIP_FUSED": fused})
+    gpu, _ = run_deck(deck, env=MODES[mode])
     cpu, _ = run_deck(deck, binary=ORACLE_TEALEAF, env={"OMP_NUM_THREADS": "16"})
     _compare(gpu, cpu)
 
@@ -72,3 +84,44 @@ def test_reference_host_drives_hip_backend(tmp_path):
     assert iters == [s["iterations"] for s in ours["steps"]]
     actual = [l for l in p.stdout.splitlines() if l.startswith("Actual")]
     assert float(actual[-1].split()[1]) == float("%.15e" % ours["summaries"][-1]["temp"])
+
+
+MP_DECKS = {
+    "mp_128x128x2": Deck.with_random_rectangles(
+        128, 128, 32, problem_seed(3), density="log10", density_range=(-1.0, 2.0),
+        end_step=2, coefficient="inverse", extra=["use_mpcg", "rel_eps=1e-10"]),
+    "mp_odd_121x97x1": Deck.with_random_rectangles(
+        121, 97, 16, problem_seed(3, 2), density="log10", density_range=(-1.0, 2.0),
+        end_step=1, coefficient="inverse",
+        extra=["use_mpcg", "rel_eps=1e-10", "refresh_interval=20"]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MP_DECKS))
+def test_mixed_precision_matches_oracle(name):
+    """Mixed-precision CG on the GPU against its CPU restatement: same
+    iteration and refresh counts, bitwise field summaries."""
+    deck = MP_DECKS[name]
+    gpu, _ = run_deck(deck)
+    cpu, _ = run_deck(deck, binary=ORACLE_TEALEAF, env={"OMP_NUM_THREADS": "16"})
+    assert gpu["solver"] == "MPCG"
+    assert [s["refreshes"] for s in gpu["steps"]] == [s["refreshes"] for s in cpu["steps"]]
+    _compare(gpu, cpu)
+    for s in gpu["steps"]:
+        assert s["rel_residual"] < 1e-10
+
+
+def test_mixed_precision_close_to_fp64():
+    """Same problem with FP64 CG and with mixed-precision CG, both stopped at
+    1e-10 relative residual: the field summaries agree to 1e-9 relative."""
+    base = dict(density="log10", density_range=(-1.0, 2.0), end_step=2,
+                coefficient="inverse")
+    fp64, _ = run_deck(Deck.with_random_rectangles(256, 256, 32, problem_seed(3, 3),
+                                                   extra=["rel_eps=1e-10"], **base))
+    mp, _ = run_deck(Deck.with_random_rectangles(256, 256, 32, problem_seed(3, 3),
+                                                 extra=["use_mpcg", "rel_eps=1e-10"],
+                                                 **base))
+    a, b = fp64["summaries"][-1], mp["summaries"][-1]
+    for k in ("vol", "mass", "ie"):
+        assert a[k] == b[k]
+    assert abs(a["temp"] / b["temp"] - 1) < 1e-9

@@ -251,3 +251,84 @@ def test_two_norm_rejects_foreign_pointer():
 
 def test_halo_slot_names():
     assert HALO_SLOTS == ["density", "energy0", "energy1", "u", "p", "sd"]
+
+
+def test_mixed_precision_kernels_match_oracle(oracle):
+    """The mixed-precision CG entry points against their CPU restatement,
+    call by call, on a ragged grid (bitwise FP32 arrays and FP64 scalars)."""
+    import ctypes
+    xc, yc = 123, 77
+    x, y = xc + 2 * H, yc + 2 * H
+    f = _random_fields(x, y, 31)
+    f["density"][:] = np.random.default_rng(3).uniform(0.1, 100.0, x * y)
+    f["energy"][:] = np.random.default_rng(4).uniform(0.1, 25.0, x * y)
+    for n in ("density", "energy"):
+        oracle.halo_field(x, y, H, 2, f[n])
+    names = ["p", "r", "u", "u0", "w", "kx", "ky", "density", "energy"]
+    L = oracle.lib
+    F = ctypes.POINTER(ctypes.c_float)
+    P = ctypes.POINTER(ctypes.c_double)
+    for fn, args in {
+            "orc_mpcg_convert": [ctypes.c_int] * 3 + [P] * 5 + [F] * 4 + [P],
+            "orc_mpcg_calc_w": [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int,
+                                                     ctypes.c_double, ctypes.c_int]
+                               + [F, F, F, P, F, F, F],
+            "orc_mpcg_calc_r": [ctypes.c_int] * 3 + [ctypes.c_double, F, F],
+            "orc_mpcg_refresh": [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int,
+                                                      P, F, P, P, P, F, P]}.items():
+        getattr(L, fn).argtypes = args
+    getattr(L, "orc_mpcg_calc_w").restype = ctypes.c_double
+    getattr(L, "orc_mpcg_calc_r").restype = ctypes.c_double
+    n = x * y
+    mp = {k: np.zeros(n, dtype=np.float32) for k in ("p0", "p1", "r", "w", "kx", "ky")}
+
+    def fp(a):
+        return a.ctypes.data_as(F)
+
+    def dp(a):
+        return a.ctypes.data_as(P)
+
+    with _chunk(x, y) as c:
+        _upload(c, f, names)
+        rr, bb = c.mpcg_init(2, 0.9, 1.1)
+        oracle.cg_init(x, y, H, 2, 0.9, 1.1, f)
+        for nm in ("u", "p"):
+            oracle.halo_field(x, y, H, 1, f[nm])
+        oracle.copy_u(x, y, H, f)
+        out = np.zeros(2)
+        L.orc_mpcg_convert(x, y, H, dp(f["kx"]), dp(f["ky"]), dp(f["p"]), dp(f["r"]),
+                           dp(f["u0"]), fp(mp["kx"]), fp(mp["ky"]), fp(mp["p0"]),
+                           fp(mp["r"]), dp(out))
+        mp["p1"][:] = mp["p0"]
+        assert _same(rr, out[0]) and _same(bb, out[1])
+        cur = 0
+        alpha = beta = 0.0
+        pu = pp = False
+        for it in range(30):
+            pw = c.mpcg_calc_w(alpha, pu, beta, pp)
+            po, pn = (mp["p0"], mp["p1"]) if cur == 0 else (mp["p1"], mp["p0"])
+            pw_o = L.orc_mpcg_calc_w(x, y, H, alpha, int(pu), beta, int(pp), fp(po),
+                                     fp(pn), fp(mp["r"]), dp(f["u"]), fp(mp["w"]),
+                                     fp(mp["kx"]), fp(mp["ky"]))
+            if pp:
+                cur ^= 1
+            assert _same(pw, pw_o), (it, pw, pw_o)
+            alpha = rr / pw
+            rrn = c.mpcg_calc_r(alpha)
+            assert _same(rrn, L.orc_mpcg_calc_r(x, y, H, alpha, fp(mp["r"]), fp(mp["w"])))
+            pu = True
+            if it % 7 == 6:
+                a, b = c.mpcg_refresh(alpha, True)
+                L.orc_mpcg_refresh(x, y, H, alpha, 1, dp(f["u"]),
+                                   fp(mp["p0"] if cur == 0 else mp["p1"]), dp(f["u0"]),
+                                   dp(f["kx"]), dp(f["ky"]), fp(mp["r"]), dp(out))
+                assert _same(a, out[0]) and _same(b, out[1])
+                rrn, pu = a, False
+            beta = rrn / rr
+            pp = True
+            rr = rrn
+        for nm, arr in (("r", mp["r"]), ("w", mp["w"]),
+                        ("p", mp["p0"] if cur == 0 else mp["p1"])):
+            got = c.read_mp(nm).ravel()
+            assert np.array_equal(got.view(np.int32), arr.view(np.int32)), nm
+        _assert_fields(c, f, ["u"])
