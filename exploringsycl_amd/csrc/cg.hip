@@ -281,6 +281,20 @@ __device__ inline double2 ld2(const double* p, int i) {
   return *reinterpret_cast<const double2*>(p + i);
 }
 
+// Streaming (nontemporal) access for data read or written once per pass;
+// stencil operands (p, r, kx, ky) use ld2 so the neighbouring rows can reuse
+// them from L2.
+__device__ inline double2 ld2_nt(const double* p, int i) {
+  const double* q = p + i;
+  return double2{__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+}
+
+__device__ inline void st2_nt(double* p, int i, bool in0, bool in1, double v0,
+                              double v1) {
+  if (in0) __builtin_nontemporal_store(v0, p + i);
+  if (in1) __builtin_nontemporal_store(v1, p + i + 1);
+}
+
 // Stores a cell pair, or the interior member(s) only.
 __device__ inline void st2(double* p, int i, bool in0, bool in1, double v0,
                            double v1) {
@@ -381,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void fused_w2_kernel(Geo g, FusedWArgs a,
       KX[j] = ld2(a.kx, ic);
       KY[j] = ld2(a.ky, ir);
       KYt[j] = ld2(a.ky, ir + x);
-      if (a.apply_u) U[j] = ld2(a.u, ir);
+      if (a.apply_u) U[j] = ld2_nt(a.u, ir);
     }
 #pragma unroll
     for (int j = 0; j < SB; ++j) {
@@ -437,14 +451,14 @@ __global__ __launch_bounds__(kBlock) void fused_w2_kernel(Geo g, FusedWArgs a,
                         (KYt[j].y * tb + KY[j].y * bb);
       psum[s0 + j] = (in0[j] ? sa * ca : 0.0) + (in1[j] ? sb * cb : 0.0);
       if (!any[j]) continue;
-      st2(a.w, i, in0[j], in1[j], sa, sb);
+      st2_nt(a.w, i, in0[j], in1[j], sa, sb);
       double ua = 0.0, ub = 0.0;
       if (a.apply_u) {
         ua = U[j].x + a.alpha * P0[j].x;
         ub = U[j].y + a.alpha * P0[j].y;
-        st2(a.u, i, in0[j], in1[j], ua, ub);
+        st2_nt(a.u, i, in0[j], in1[j], ua, ub);
       }
-      if (APPLY_P) st2(a.p_new, i, in0[j], in1[j], ca, cb);
+      if (APPLY_P) st2_nt(a.p_new, i, in0[j], in1[j], ca, cb);
       // Halo ring of boundary-adjacent cells (rare; per cell).
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -489,8 +503,8 @@ __global__ __launch_bounds__(kBlock) void fused_r2_kernel(
     in0[k] = e < g.n && interior(g, kk, row);
     in1[k] = e < g.n && interior(g, kk + 1, row);
     const int i = (in0[k] || in1[k]) ? e : 0;
-    rv[k] = ld2(r, i);
-    wv[k] = ld2(w, i);
+    rv[k] = ld2_nt(r, i);
+    wv[k] = ld2_nt(w, i);
   }
   double psum[kSlots];
 #pragma unroll
@@ -498,7 +512,7 @@ __global__ __launch_bounds__(kBlock) void fused_r2_kernel(
     const double ra = rv[k].x - alpha * wv[k].x;
     const double rb = rv[k].y - alpha * wv[k].y;
     psum[k] = (in0[k] ? ra * ra : 0.0) + (in1[k] ? rb * rb : 0.0);
-    st2(r, base + k * (2 * kBlock) + 2 * threadIdx.x, in0[k], in1[k], ra, rb);
+    st2_nt(r, base + k * (2 * kBlock) + 2 * threadIdx.x, in0[k], in1[k], ra, rb);
   }
   span_reduce_pairs(psum, partials, nspan, span, single_group != 0);
 }
