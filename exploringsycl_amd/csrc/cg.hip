@@ -284,25 +284,22 @@ __device__ inline double2 ld2(const double* p, int i) {
 // Streaming (nontemporal) access for data read or written once per pass;
 // stencil operands (p, r, kx, ky) use ld2 so the neighbouring rows can reuse
 // them from L2.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
 __device__ inline double2 ld2_nt(const double* p, int i) {
-  const double* q = p + i;
-  return double2{__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+  const dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(p + i));
+  return double2{v.x, v.y};
 }
 
+// Stores a cell pair with one 16-byte streaming store, or the interior
+// member alone.
 __device__ inline void st2_nt(double* p, int i, bool in0, bool in1, double v0,
                               double v1) {
-  if (in0) __builtin_nontemporal_store(v0, p + i);
-  if (in1) __builtin_nontemporal_store(v1, p + i + 1);
-}
-
-// Stores a cell pair, or the interior member(s) only.
-__device__ inline void st2(double* p, int i, bool in0, bool in1, double v0,
-                           double v1) {
   if (in0 && in1) {
-    *reinterpret_cast<double2*>(p + i) = double2{v0, v1};
+    __builtin_nontemporal_store(dvec2{v0, v1}, reinterpret_cast<dvec2*>(p + i));
   } else {
-    if (in0) p[i] = v0;
-    if (in1) p[i + 1] = v1;
+    if (in0) __builtin_nontemporal_store(v0, p + i);
+    if (in1) __builtin_nontemporal_store(v1, p + i + 1);
   }
 }
 
@@ -728,8 +725,8 @@ int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
     if (c->pairs) {
       if (!apply_p) TEA_FUSED_W2(false, 2);
       else if (c->batch == 1) TEA_FUSED_W2(true, 1);
-      else if (c->batch == 4) TEA_FUSED_W2(true, 4);
-      else TEA_FUSED_W2(true, 2);
+      else if (c->batch == 2) TEA_FUSED_W2(true, 2);
+      else TEA_FUSED_W2(true, 4);
     } else {
       if (!apply_p) TEA_FUSED_W(false, 4);
       else if (c->batch == 2) TEA_FUSED_W(true, 2);

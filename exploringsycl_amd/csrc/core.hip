@@ -24,7 +24,8 @@ const char* const kKernelNames[K_NUM] = {
     "cg_calc_w",      "cg_calc_ur",      "cg_calc_p",    "cg_fused_w",
     "cg_fused_r",     "cg_apply_pending", "copy_u",      "calculate_residual",
     "calculate_2norm", "finalise",       "mpcg_convert", "mpcg_w",
-    "mpcg_r",         "mpcg_apply_u",    "mpcg_refresh"};
+    "mpcg_r",         "mpcg_apply_u",    "mpcg_refresh",  "fcg_w",
+    "fcg_jacobi",     "fcg_restrict",    "fcg_net",       "fcg_prolong"};
 
 Timed::Timed(tea_hip_chunk* c_, int kid_) : c(c_), kid(kid_) {
   if (!c->timing) return;
@@ -211,8 +212,11 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (int f = 0; f < TEA_NUM_FIELDS; ++f)
     if (c->field[f]) (void)hipFree(c->field[f]);
   if (c->p_alt) (void)hipFree(c->p_alt);
-  for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky})
+  for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky,
+                   c->fcg_z, c->fcg_feat, c->fcg_rbar, c->fcg_scale})
     if (b) (void)hipFree(b);
+  if (c->fcg_part) (void)hipFree(c->fcg_part);
+  if (c->fcg_net) tea_nn_destroy(c->fcg_net);
   if (c->host_partials) (void)hipHostFree(c->host_partials);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -1,0 +1,315 @@
+// libteahip.so: flexible CG with the learned preconditioner, and
+// Jacobi-preconditioned CG (no counterpart in the reference).
+//
+// Storage and precision follow the mixed-precision CG (mpcg.hip): p, r, w,
+// z and the face coefficients in FP32; u, u0, dot products and scalars in
+// FP64; the FP64 true-residual refresh is tea_hip_mpcg_refresh.
+//
+// Preconditioners, z = M(r):
+//   Jacobi:  z = r / diag(A), diag = 1 + kx[i] + kx[i+1] + ky[i] + ky[i+x].
+//   Learned: z = r / diag(A) + s * P(N(R r / s, f_kx, f_ky)), where R is the
+//            c x c block mean onto the coarse grid, s the RMS of R r (both
+//            computed on the device), N the 6-layer network on the coarse
+//            grid, P the piecewise-constant prolongation, and f_kx, f_ky the
+//            coarse means of log kx, log ky, normalised with the constants
+//            stored with the network.
+// The network's output is a coarse-grid correction: with c = 4 it costs
+// 1/16 of a full-resolution pass and its 13-cell receptive field spans 52
+// fine cells.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "tea_hip_impl.h"
+
+namespace tea {
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// z = r / diag and z.w over the interior (span kernel body).
+struct JacobiBody {
+  const float* r;
+  const float* kx;
+  const float* ky;
+  const float* w;
+  float* z;
+  int x;
+  __device__ void operator()(int i, int, int, double (&out)[1]) const {
+    const double d = (1.0 + ((double)kx[i + 1] + (double)kx[i])) +
+                     ((double)ky[i + x] + (double)ky[i]);
+    const float zv = (float)((double)r[i] / d);
+    z[i] = zv;
+    out[0] = (double)zv * (double)w[i];
+  }
+};
+
+// Coarse block means of r and per-block partial sums of their squares.
+__global__ __launch_bounds__(256) void restrict_kernel(
+    Geo g, int c, int hc, int wc, const float* __restrict__ r,
+    float* __restrict__ rbar, double* __restrict__ part) {
+  __shared__ double s[256];
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  double v2 = 0.0;
+  if (idx < hc * wc) {
+    const int X = idx % wc, Y = idx / wc;
+    double sum = 0.0;
+    for (int b = 0; b < c; ++b) {
+      const int row = (g.h + Y * c + b) * g.x + g.h + X * c;
+      for (int a = 0; a < c; ++a) sum += (double)r[row + a];
+    }
+    const float m = (float)(sum / (double)(c * c));
+    rbar[idx] = m;
+    v2 = (double)m * (double)m;
+  }
+  s[threadIdx.x] = v2;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// s = RMS of the coarse residual (1 if it is zero).
+__global__ __launch_bounds__(256) void scale_kernel(const double* __restrict__ part,
+                                                    int nblk, long count,
+                                                    float* __restrict__ scale) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) v += part[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double rms = sqrt(s[0] / (double)count);
+    scale[0] = rms > 0.0 ? (float)rms : 1.0f;
+  }
+}
+
+// Network input: channel 0 = rbar / s, channels 1, 2 = features.
+__global__ __launch_bounds__(256) void pack_kernel(int hc, int wc,
+                                                   const float* __restrict__ rbar,
+                                                   const float* __restrict__ feat,
+                                                   const float* __restrict__ scale,
+                                                   bf16* __restrict__ in0) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= hc * wc) return;
+  const int X = idx % wc, Y = idx / wc;
+  const float inv = 1.0f / scale[0];
+  bf16x8 o = bf16x8{};
+  o[0] = (bf16)(rbar[idx] * inv);
+  o[1] = (bf16)feat[idx];
+  o[2] = (bf16)feat[hc * wc + idx];
+  *reinterpret_cast<bf16x8*>(in0 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * 8) = o;
+}
+
+// z = r / diag + s * net_out[coarse cell] and z.w.
+struct ProlongBody {
+  const float* r;
+  const float* kx;
+  const float* ky;
+  const float* w;
+  const float* net;
+  const float* scale;
+  float* z;
+  int x, h, c, wc;
+  __device__ void operator()(int i, int kk, int jj, double (&out)[1]) const {
+    const double d = (1.0 + ((double)kx[i + 1] + (double)kx[i])) +
+                     ((double)ky[i + x] + (double)ky[i]);
+    const int X = (kk - h) / c, Y = (jj - h) / c;
+    const float zv = (float)((double)r[i] / d + (double)scale[0] * (double)net[Y * wc + X]);
+    z[i] = zv;
+    out[0] = (double)zv * (double)w[i];
+  }
+};
+
+// Coarse features: normalised block means of log kx and log ky.
+__global__ __launch_bounds__(256) void feature_kernel(Geo g, int c, int hc, int wc,
+                                                      const float* __restrict__ kx,
+                                                      const float* __restrict__ ky,
+                                                      float m0, float s0, float m1,
+                                                      float s1,
+                                                      float* __restrict__ feat) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= hc * wc) return;
+  const int X = idx % wc, Y = idx / wc;
+  double lx = 0.0, ly = 0.0;
+  for (int b = 0; b < c; ++b) {
+    const int row = (g.h + Y * c + b) * g.x + g.h + X * c;
+    for (int a = 0; a < c; ++a) {
+      lx += log(fmax((double)kx[row + a], 1e-30));
+      ly += log(fmax((double)ky[row + a], 1e-30));
+    }
+  }
+  const double n = (double)(c * c);
+  feat[idx] = (float)((lx / n - m0) / s0);
+  feat[hc * wc + idx] = (float)((ly / n - m1) / s1);
+}
+
+int ensure_fcg_buffers(tea_hip_chunk* c) {
+  if (c->fcg_z) return TEA_HIP_OK;
+  if (hipMalloc(&c->fcg_z, (size_t)c->geo.n * sizeof(float)) != hipSuccess) {
+    c->fcg_z = nullptr;
+    set_error("hipMalloc of the FCG z buffer failed");
+    return TEA_HIP_ENOMEM;
+  }
+  TEA_HIP_TRY(hipMemsetAsync(c->fcg_z, 0, (size_t)c->geo.n * sizeof(float), c->stream));
+  return TEA_HIP_OK;
+}
+
+}  // namespace
+}  // namespace tea
+
+using namespace tea;
+
+extern "C" {
+
+int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
+                       const float* feature_norm, int coarsen) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_RETURN_IF(ensure_fcg_buffers(c));
+  if (!params) return TEA_HIP_OK;  // Jacobi only
+  TEA_CHECK_ARG(feature_norm != nullptr, "null feature normalisation");
+  TEA_CHECK_ARG(coarsen >= 1, "coarsening factor must be >= 1");
+  const int nx = c->geo.x - 2 * c->geo.h, ny = c->geo.y - 2 * c->geo.h;
+  TEA_CHECK_ARG(nx % coarsen == 0 && ny % coarsen == 0,
+                "the interior must be a multiple of the coarsening factor");
+  const int hc = ny / coarsen, wc = nx / coarsen;
+  TEA_CHECK_ARG(wc % 32 == 0, "coarse width must be a multiple of 32");
+  if (c->fcg_net) {
+    tea_nn_destroy(c->fcg_net);
+    c->fcg_net = nullptr;
+  }
+  TEA_RETURN_IF(tea_nn_create(c->device, 1, hc, wc, &c->fcg_net));
+  TEA_RETURN_IF(tea_nn_set_params(c->fcg_net, params, count));
+  c->fcg_c = coarsen;
+  c->fcg_hc = hc;
+  c->fcg_wc = wc;
+  for (int q = 0; q < 4; ++q) c->fcg_norm[q] = feature_norm[q];
+  TEA_CHECK_ARG(c->fcg_norm[1] > 0.0f && c->fcg_norm[3] > 0.0f, "feature scales must be positive");
+  const size_t cn = (size_t)hc * wc;
+  const size_t nblk = (cn + 255) / 256;
+  if (c->fcg_feat) (void)hipFree(c->fcg_feat);
+  if (c->fcg_rbar) (void)hipFree(c->fcg_rbar);
+  if (c->fcg_part) (void)hipFree(c->fcg_part);
+  if (c->fcg_scale) (void)hipFree(c->fcg_scale);
+  c->fcg_feat = c->fcg_rbar = c->fcg_scale = nullptr;
+  c->fcg_part = nullptr;
+  if (hipMalloc(&c->fcg_feat, 2 * cn * sizeof(float)) != hipSuccess ||
+      hipMalloc(&c->fcg_rbar, cn * sizeof(float)) != hipSuccess ||
+      hipMalloc(&c->fcg_part, nblk * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->fcg_scale, sizeof(float)) != hipSuccess) {
+    set_error("hipMalloc of the preconditioner's coarse buffers failed");
+    return TEA_HIP_ENOMEM;
+  }
+  return TEA_HIP_OK;
+}
+
+// Coarse features from the FP32 coefficients (after tea_hip_mpcg_init).
+static int fcg_features(tea_hip_chunk* c) {
+  if (!c->fcg_net) return TEA_HIP_OK;
+  const int cn = c->fcg_hc * c->fcg_wc;
+  hipLaunchKernelGGL(feature_kernel, dim3((cn + 255) / 256), dim3(256), 0, c->stream,
+                     c->geo, c->fcg_c, c->fcg_hc, c->fcg_wc, c->mp_kx, c->mp_ky,
+                     c->fcg_norm[0], c->fcg_norm[1], c->fcg_norm[2], c->fcg_norm[3],
+                     c->fcg_feat);
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
+int tea_hip_fcg_init(tea_hip_chunk* c, int coefficient, double rx, double ry,
+                     double* rr, double* bb) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(c->fcg_z != nullptr, "tea_hip_fcg_attach has not been called");
+  TEA_RETURN_IF(tea_hip_mpcg_init(c, coefficient, rx, ry, rr, bb));
+  return fcg_features(c);
+}
+
+int tea_hip_fcg_precondition(tea_hip_chunk* c, int use_net, double* zw) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(c->fcg_z != nullptr && c->mp_r != nullptr, "FCG not initialised");
+  TEA_CHECK_ARG(zw != nullptr, "null zw");
+  TEA_CHECK_ARG(!use_net || c->fcg_net != nullptr, "no network attached");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  const Geo g = c->geo;
+  const int sg = c->ngroup == 1 ? 1 : 0;
+  if (!use_net) {
+    Timed t(c, K_FCG_JACOBI);
+    JacobiBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, c->fcg_z, g.x};
+    hipLaunchKernelGGL((span_kernel<1, JacobiBody>), span_grid(c), dim3(kBlock), 0,
+                       c->stream, g, body, c->partials, c->nspan, sg);
+  } else {
+    const int cn = c->fcg_hc * c->fcg_wc;
+    const int nblk = (cn + 255) / 256;
+    {
+      Timed t(c, K_FCG_RESTRICT);
+      hipLaunchKernelGGL(restrict_kernel, dim3(nblk), dim3(256), 0, c->stream, g,
+                         c->fcg_c, c->fcg_hc, c->fcg_wc, c->mp_r, c->fcg_rbar,
+                         c->fcg_part);
+      hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(256), 0, c->stream, c->fcg_part,
+                         nblk, (long)cn, c->fcg_scale);
+      hipLaunchKernelGGL(pack_kernel, dim3(nblk), dim3(256), 0, c->stream, c->fcg_hc,
+                         c->fcg_wc, c->fcg_rbar, c->fcg_feat, c->fcg_scale,
+                         nn::input_tensor(c->fcg_net));
+    }
+    {
+      Timed t(c, K_FCG_NET);
+      TEA_RETURN_IF(nn::forward_on_stream(c->fcg_net, c->stream));
+    }
+    {
+      Timed t(c, K_FCG_PROLONG);
+      ProlongBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
+                       nn::output(c->fcg_net), c->fcg_scale, c->fcg_z,
+                       g.x, g.h, c->fcg_c, c->fcg_wc};
+      hipLaunchKernelGGL((span_kernel<1, ProlongBody>), span_grid(c), dim3(kBlock), 0,
+                         c->stream, g, body, c->partials, c->nspan, sg);
+    }
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  return finish_reduction(c, 1, zw);
+}
+
+int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
+                       double* pw, double* pr) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(c->fcg_z != nullptr && c->mp_r != nullptr, "FCG not initialised");
+  TEA_CHECK_ARG(pw && pr, "null outputs");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  // p = z + beta p_old (the mixed-precision kernel with z in place of r),
+  // w = A p, and the dots p.w and p.r.
+  MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->fcg_z,
+           c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky, c->mp_r,
+           alpha, beta, apply_u ? 1 : 0, 1};
+  {
+    Timed t(c, K_FCG_W);
+    hipLaunchKernelGGL((mp_w_kernel<4, true>), span_grid(c), dim3(kBlock), 0,
+                       c->stream, c->geo, a, c->partials, c->nspan,
+                       c->ngroup == 1 ? 1 : 0);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  c->mp_cur ^= 1;
+  double out[2];
+  TEA_RETURN_IF(finish_reduction(c, 2, out));
+  *pw = out[0];
+  *pr = out[1];
+  return TEA_HIP_OK;
+}
+
+int tea_hip_fcg_read_z(tea_hip_chunk* c, float* host, size_t count) {
+  TEA_CHECK_ARG(c != nullptr && c->fcg_z != nullptr, "FCG not initialised");
+  TEA_CHECK_ARG(count <= (size_t)c->geo.n, "count exceeds the field length");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_HIP_TRY(hipMemcpyAsync(host, c->fcg_z, count * sizeof(float), hipMemcpyDeviceToHost,
+                             c->stream));
+  return sync_chunk(c);
+}
+
+}  // extern "C"

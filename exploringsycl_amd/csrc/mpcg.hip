@@ -24,21 +24,10 @@
 #include "tea_hip_impl.h"
 
 namespace tea {
-namespace {
 
-struct MpArgs {
-  const float* p_old;
-  float* p_new;
-  const float* r;
-  double* u;
-  float* w;
-  const float* kx;
-  const float* ky;
-  double alpha, beta;
-  int apply_u, apply_p;
-};
-
-template <int kBatch>
+// p_new = beta p_old + r (r is the residual for CG, the preconditioned
+// residual z for flexible CG); with DOT2 also reduces p.rdot.
+template <int kBatch, bool DOT2>
 __global__ __launch_bounds__(kBlock) void mp_w_kernel(Geo g, MpArgs a,
                                                       double* partials,
                                                       int nspan,
@@ -49,7 +38,7 @@ __global__ __launch_bounds__(kBlock) void mp_w_kernel(Geo g, MpArgs a,
   const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
   const float* __restrict__ P = a.p_old;
   const float* __restrict__ R = a.r;
-  double prod[1][kItems];
+  double prod[DOT2 ? 2 : 1][kItems];
 #pragma unroll
   for (int b0 = 0; b0 < kItems; b0 += kBatch) {
     int ci[kBatch], ck[kBatch], cj[kBatch];
@@ -107,6 +96,7 @@ __global__ __launch_bounds__(kBlock) void mp_w_kernel(Geo g, MpArgs a,
       const float s = (diag * pc - (kx1[j] * rgt + kx0[j] * l)) -
                       (ky1[j] * top + ky0[j] * bot);
       prod[0][b0 + j] = in[j] ? (double)s * (double)pc : 0.0;
+      if (DOT2) prod[DOT2 ? 1 : 0][b0 + j] = in[j] ? (double)pc * (double)a.rdot[i] : 0.0;
       if (!in[j]) continue;
       a.w[i] = s;
       const bool L = kk == xl, Rr = kk == xr, B = jj == yb, T = jj == yt;
@@ -136,8 +126,10 @@ __global__ __launch_bounds__(kBlock) void mp_w_kernel(Geo g, MpArgs a,
       }
     }
   }
-  span_reduce<1>(prod, partials, nspan, span, single_group != 0);
+  span_reduce<DOT2 ? 2 : 1>(prod, partials, nspan, span, single_group != 0);
 }
+
+template __global__ void mp_w_kernel<4, true>(Geo, MpArgs, double*, int, int);
 
 __global__ __launch_bounds__(kBlock) void mp_r_kernel(
     Geo g, float* __restrict__ r, const float* __restrict__ w, double alpha,
@@ -164,6 +156,8 @@ __global__ __launch_bounds__(kBlock) void mp_r_kernel(
   }
   span_reduce<1>(prod, partials, nspan, span, single_group != 0);
 }
+
+namespace {
 
 // FP32 mirrors after the FP64 initialisation: coefficients and p on every
 // cell (p's halo is already reflected), r on every cell; reduces
@@ -307,11 +301,11 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
   TEA_CHECK_ARG(pw != nullptr, "null pw");
   TEA_HIP_TRY(hipSetDevice(c->device));
   MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->mp_r,
-           c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky,
+           c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky, nullptr,
            alpha, beta, apply_u ? 1 : 0, apply_p ? 1 : 0};
   {
     Timed t(c, K_MP_W);
-    hipLaunchKernelGGL((mp_w_kernel<4>), span_grid(c), dim3(kBlock), 0,
+    hipLaunchKernelGGL((mp_w_kernel<4, false>), span_grid(c), dim3(kBlock), 0,
                        c->stream, c->geo, a, c->partials, c->nspan,
                        c->ngroup == 1 ? 1 : 0);
   }

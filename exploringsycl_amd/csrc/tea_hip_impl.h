@@ -232,11 +232,52 @@ enum KernelId {
   K_MP_R,
   K_MP_APPLY_U,
   K_MP_REFRESH,
+  K_FCG_W,
+  K_FCG_JACOBI,
+  K_FCG_RESTRICT,
+  K_FCG_NET,
+  K_FCG_PROLONG,
   K_NUM
 };
 
 extern const char* const kKernelNames[K_NUM];
 
+}  // namespace tea
+
+namespace tea {
+
+// Arguments of the mixed-precision stencil kernel (mpcg.hip), shared with
+// flexible CG (fcg.hip).
+struct MpArgs {
+  const float* p_old;
+  float* p_new;
+  const float* r;     // added to beta p_old (the residual, or z for FCG)
+  double* u;
+  float* w;
+  const float* kx;
+  const float* ky;
+  const float* rdot;  // second dot product p.rdot (DOT2 only)
+  double alpha, beta;
+  int apply_u, apply_p;
+};
+
+template <int kBatch, bool DOT2>
+__global__ void mp_w_kernel(Geo g, MpArgs a, double* partials, int nspan,
+                            int single_group);
+__global__ void mp_r_kernel(Geo g, float* r, const float* w, double alpha,
+                            double* partials, int nspan, int single_group);
+
+}  // namespace tea
+
+struct tea_nn;
+
+namespace tea {
+namespace nn {
+typedef __bf16 bf16;
+int forward_on_stream(tea_nn* n, hipStream_t st);
+bf16* input_tensor(tea_nn* n);
+const float* output(tea_nn* n);
+}  // namespace nn
 }  // namespace tea
 
 // ---------------------------------------------------------------------------
@@ -277,6 +318,18 @@ struct tea_hip_chunk {
   float* mp_w = nullptr;
   float* mp_kx = nullptr;
   float* mp_ky = nullptr;
+
+  // Flexible CG: the preconditioned residual z (FP32), the network on the
+  // coarse grid (coarsening factor fcg_c), its input features and the
+  // device-side scale of the coarse residual.
+  float* fcg_z = nullptr;
+  tea_nn* fcg_net = nullptr;
+  int fcg_c = 0, fcg_hc = 0, fcg_wc = 0;
+  float fcg_norm[4] = {0.0f, 1.0f, 0.0f, 1.0f};
+  float* fcg_feat = nullptr;   // [2][hc][wc]
+  float* fcg_rbar = nullptr;   // [hc][wc]
+  double* fcg_part = nullptr;  // per-block sums of rbar^2
+  float* fcg_scale = nullptr;  // device scalar s
 
   // Timing.
   bool timing = false;

@@ -34,6 +34,7 @@ _d = ctypes.c_double
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
+_fp = ctypes.POINTER(ctypes.c_float)
 _i = ctypes.c_int
 
 _SIGNATURES = {
@@ -60,7 +61,12 @@ _SIGNATURES = {
     "tea_hip_mpcg_calc_w": [_vp, _d, _i, _d, _i, _dp],
     "tea_hip_mpcg_calc_r": [_vp, _d, _dp],
     "tea_hip_mpcg_refresh": [_vp, _d, _i, _dp, _dp],
-    "tea_hip_mp_read": [_vp, _i, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t],
+    "tea_hip_mp_read": [_vp, _i, _fp, ctypes.c_size_t],
+    "tea_hip_fcg_attach": [_vp, _fp, ctypes.c_size_t, _fp, _i],
+    "tea_hip_fcg_init": [_vp, _i, _d, _d, _dp, _dp],
+    "tea_hip_fcg_precondition": [_vp, _i, _dp],
+    "tea_hip_fcg_calc_w": [_vp, _d, _i, _d, _dp, _dp],
+    "tea_hip_fcg_read_z": [_vp, _fp, ctypes.c_size_t],
     "tea_hip_set_fused": [_vp, _i],
     "tea_hip_flush": [_vp],
     "tea_hip_timing_enable": [_vp, _i],

@@ -223,6 +223,70 @@ enum { TEA_MP_P = 0, TEA_MP_R = 1, TEA_MP_W = 2, TEA_MP_KX = 3, TEA_MP_KY = 4 };
 int tea_hip_mp_read(tea_hip_chunk* chunk, int which, float* host,
                     size_t count);
 
+/* ---- learned preconditioner network --------------------------------------
+ * No counterpart in the reference.  A 6-layer 3x3 convolutional network,
+ * 32 hidden channels, ReLU after the first five layers, inputs (residual,
+ * kx feature, ky feature), one output channel; 38,177 FP32 master
+ * parameters; BF16 weights and activations with FP32 accumulation on MFMA.
+ * Parameter order: W1 (32,3,3,3), b1, W2..W5 (32,32,3,3) with b2..b5,
+ * W6 (1,32,3,3), b6, each W indexed [out][in][ky][kx].  Inputs and outputs
+ * are host arrays: input [batch][height][width][3], output and target
+ * [batch][height][width].  width must be a multiple of 32; training needs
+ * width <= 128. */
+typedef struct tea_nn tea_nn;
+
+int tea_nn_num_params(void);
+int tea_nn_create(int device, int max_batch, int height, int width,
+                  tea_nn** out);
+int tea_nn_destroy(tea_nn* net);
+/* Sets the FP32 master parameters (and clears the Adam state). */
+int tea_nn_set_params(tea_nn* net, const float* params, size_t count);
+int tea_nn_get_params(tea_nn* net, float* params, size_t count);
+/* Network output for `batch` inputs. */
+int tea_nn_forward(tea_nn* net, int batch, const float* input, float* output);
+/* Mean squared error against `target` and its gradient with respect to the
+ * master parameters (grad may be NULL). */
+int tea_nn_loss_grad(tea_nn* net, int batch, const float* input,
+                     const float* target, float* loss, float* grad);
+/* tea_nn_loss_grad followed by one Adam step (beta 0.9/0.999, eps 1e-8). */
+int tea_nn_train_step(tea_nn* net, int batch, const float* input,
+                      const float* target, float lr, float* loss);
+/* Copies a training set of `count` samples to device memory. */
+int tea_nn_load_dataset(tea_nn* net, int count, const float* inputs,
+                        const float* targets);
+/* One Adam pass over the resident samples order[0..count) in batches of
+ * `batch`; *mean_loss is the mean batch loss. */
+int tea_nn_train_epoch(tea_nn* net, const int* order, int count, int batch,
+                       float lr, float* mean_loss);
+
+/* ---- flexible CG with the learned preconditioner ------------------------
+ * No counterpart in the reference.  FP32 p, r, w, z; FP64 u, dots, scalars;
+ * FP64 true-residual refresh with tea_hip_mpcg_refresh.  Preconditioners:
+ * Jacobi, z = r / diag(A); learned, z = r / diag(A) + s P(N(R r / s, f)),
+ * with R the c x c block mean, P its piecewise-constant transpose, s the RMS
+ * of R r and N the network on the coarse grid (input features f: coarse
+ * means of log kx and log ky, normalised by feature_norm = {mean_x, std_x,
+ * mean_y, std_y}).  An iteration: calc_w (p = z + beta p_old, w = A p,
+ * p.w and p.r, previous alpha applied to u), alpha = p.r / p.w,
+ * tea_hip_mpcg_calc_r, precondition (z = M(r), z.w), beta = -z.w / p.w. */
+
+/* Allocates the FCG buffers; with params != NULL also loads the network
+ * (tea_nn_num_params() values) for a coarsening factor `coarsen` (the
+ * interior must divide by it and the coarse width be a multiple of 32). */
+int tea_hip_fcg_attach(tea_hip_chunk* chunk, const float* params,
+                       size_t count, const float* feature_norm, int coarsen);
+/* tea_hip_mpcg_init plus the network's coarse input features. */
+int tea_hip_fcg_init(tea_hip_chunk* chunk, int coefficient, double rx,
+                     double ry, double* rr, double* bb);
+/* z = M(r) with the network (use_net != 0) or Jacobi; *zw = z.w. */
+int tea_hip_fcg_precondition(tea_hip_chunk* chunk, int use_net, double* zw);
+/* If apply_u: u += alpha p_old.  p = z + beta p_old; w = A p;
+ * *pw = p.w, *pr = p.r. */
+int tea_hip_fcg_calc_w(tea_hip_chunk* chunk, double alpha, int apply_u,
+                       double beta, double* pw, double* pr);
+/* Copies z to host memory. */
+int tea_hip_fcg_read_z(tea_hip_chunk* chunk, float* host, size_t count);
+
 /* ---- execution mode and measurement ------------------------------------ */
 
 /* Mode 1 (default): CG calls are fused.  tea_hip_cg_calc_ur updates r and

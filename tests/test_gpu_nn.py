@@ -1,0 +1,76 @@
+"""The network kernels (BF16 MFMA forward and backward) against the CPU
+oracle, and training sanity."""
+
+import numpy as np
+import pytest
+
+from oracle import nn_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(b, h, w):
+    from exploringsycl_amd.nn import Net
+    return Net(b, h, w)
+
+
+def _case(seed, b=3, h=16, w=64):
+    rng = np.random.default_rng(seed)
+    p = O.init_params(seed)
+    p[O.num_params() - 1] = 0.05
+    # non-zero biases to exercise them
+    k = 0
+    for o, i in O.SHAPES:
+        k += o * i * 9
+        p[k:k + o] = rng.uniform(-0.1, 0.1, o)
+        k += o
+    x = rng.standard_normal((b, h, w, 3)).astype(np.float32)
+    t = rng.standard_normal((b, h, w)).astype(np.float32)
+    return p, x, t
+
+
+@pytest.mark.parametrize("shape", [(3, 16, 64), (2, 7, 32), (1, 64, 128)])
+def test_forward_matches_oracle(shape):
+    p, x, t = _case(1, *shape)
+    net = _net(*shape)
+    net.set_params(p)
+    got = net.forward(x)
+    want, _ = O.forward(p, x, bf16_mode=True)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert err < 2e-3, err
+    assert np.allclose(net.get_params(), p)
+
+
+@pytest.mark.parametrize("shape", [(3, 16, 64), (2, 9, 32)])
+def test_gradients_match_oracle(shape):
+    p, x, t = _case(2, *shape)
+    net = _net(*shape)
+    net.set_params(p)
+    loss, g = net.loss_grad(x, t)
+    loss_o, g_o = O.loss_grad(p, x, t, bf16_mode=True)
+    assert abs(loss / loss_o - 1) < 2e-3
+    k = 0
+    for li, (o, i) in enumerate(O.SHAPES):
+        n = o * i * 9 + o
+        a, b = g[k:k + n], g_o[k:k + n]
+        err = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert err < 3e-2, (li, err)
+        k += n
+
+
+def test_training_reduces_loss():
+    """Learn a fixed linear 3x3 filter of channel 0 from random inputs."""
+    rng = np.random.default_rng(3)
+    b, h, w = 8, 16, 32
+    net = _net(b, h, w)
+    net.set_params(O.init_params(4))
+    x = rng.standard_normal((64, h, w, 3)).astype(np.float32)
+    k = np.array([[0.05, 0.1, 0.05], [0.1, 0.4, 0.1], [0.05, 0.1, 0.05]])
+    xp = np.pad(x[..., 0], ((0, 0), (1, 1), (1, 1)))
+    t = sum(k[i, j] * xp[:, i:i + h, j:j + w] for i in range(3) for j in range(3))
+    t = t.astype(np.float32)
+    net.load_dataset(x, t)
+    first = net.train_epoch(np.arange(64), b, 3e-3)
+    for _ in range(30):
+        last = net.train_epoch(rng.permutation(64), b, 3e-3)
+    assert last < 0.2 * first, (first, last)
