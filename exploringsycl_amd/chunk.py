@@ -180,6 +180,45 @@ class DeviceChunk:
                                        out.size), f"read_mp {name}")
         return out.reshape(self.y, self.x)
 
+    # ---- flexible CG (no reference counterpart) --------------------------------
+    def fcg_attach(self, params=None, norm=None, coarsen=0):
+        """Allocates the FCG state; with ``params`` also loads the network."""
+        fp = ctypes.POINTER(ctypes.c_float)
+        if params is None:
+            check(self.lib.tea_hip_fcg_attach(self.handle, None, 0, None, 0), "fcg_attach")
+            return
+        self._fcg_params = np.ascontiguousarray(params, dtype=np.float32)
+        self._fcg_norm = np.ascontiguousarray(norm, dtype=np.float32)
+        check(self.lib.tea_hip_fcg_attach(self.handle, self._fcg_params.ctypes.data_as(fp),
+                                          self._fcg_params.size,
+                                          self._fcg_norm.ctypes.data_as(fp), int(coarsen)),
+              "fcg_attach")
+
+    def fcg_init(self, coefficient, rx, ry):
+        rr, bb = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.tea_hip_fcg_init(self.handle, coefficient, rx, ry,
+                                        ctypes.byref(rr), ctypes.byref(bb)), "fcg_init")
+        return rr.value, bb.value
+
+    def fcg_precondition(self, use_net):
+        v = ctypes.c_double()
+        check(self.lib.tea_hip_fcg_precondition(self.handle, int(use_net), ctypes.byref(v)),
+              "fcg_precondition")
+        return v.value
+
+    def fcg_calc_w(self, alpha, apply_u, beta):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.tea_hip_fcg_calc_w(self.handle, alpha, int(apply_u), beta,
+                                          ctypes.byref(a), ctypes.byref(b)), "fcg_calc_w")
+        return a.value, b.value
+
+    def read_z(self):
+        out = np.empty(self.x * self.y, dtype=np.float32)
+        check(self.lib.tea_hip_fcg_read_z(self.handle,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                          out.size), "read_z")
+        return out.reshape(self.y, self.x)
+
     # ---- mode and timing -------------------------------------------------------
     def set_fused(self, on):
         check(self.lib.tea_hip_set_fused(self.handle, 1 if on else 0), "set_fused")

@@ -308,6 +308,41 @@ void run_mpcg_refresh(Chunk* chunk, Settings* settings, double alpha,
   *rr += a;
   *rr_true += b;
 }
+
+void run_fcg_attach(Chunk* chunk, Settings* settings, const float* params,
+                    size_t count, const float* norm, int coarsen) {
+  (void)settings;
+  HIP_CALL(tea_hip_fcg_attach(H(chunk), params, count, norm, coarsen));
+}
+
+void run_fcg_init(Chunk* chunk, Settings* settings, double rx, double ry,
+                  double* rr, double* bb) {
+  for (int f = 0; f < NUM_FACES; ++f)
+    if (chunk->neighbours[f] != EXTERNAL_FACE)
+      die(__LINE__, __FILE__, "flexible CG needs every face external\n");
+  double a = 0.0, b = 0.0;
+  HIP_CALL(tea_hip_fcg_init(H(chunk), settings->coefficient, rx, ry, &a, &b));
+  refresh_p(chunk);
+  *rr += a;
+  *bb += b;
+}
+
+void run_fcg_precondition(Chunk* chunk, Settings* settings, int use_net,
+                          double* zw) {
+  (void)settings;
+  double v;
+  HIP_CALL(tea_hip_fcg_precondition(H(chunk), use_net, &v));
+  *zw += v;
+}
+
+void run_fcg_calc_w(Chunk* chunk, Settings* settings, double alpha,
+                    int apply_u, double beta, double* pw, double* pr) {
+  (void)settings;
+  double a, b;
+  HIP_CALL(tea_hip_fcg_calc_w(H(chunk), alpha, apply_u, beta, &a, &b));
+  *pw += a;
+  *pr += b;
+}
 #endif
 
 // Optional host hook (declared weak by the host): with TEA_HIP_STATS=<file>

@@ -77,8 +77,9 @@ class Deck:
 
     def __init__(self, x_cells, y_cells, end_step=10, coefficient="density",
                  eps=1e-15, max_iters=10000, states=None, dt=0.004,
-                 summary_frequency=10, extra=(), comment=None):
+                 summary_frequency=10, extra=(), comment=None, extent=10.0):
         self.x_cells, self.y_cells = x_cells, y_cells
+        self.extent = extent
         self.end_step = end_step
         self.coefficient = coefficient
         self.eps, self.max_iters = eps, max_iters
@@ -90,11 +91,20 @@ class Deck:
 
     @classmethod
     def with_random_rectangles(cls, x_cells, y_cells, count, seed, density="uniform",
-                               density_range=(0.1, 100.0), **kw):
-        states = list(TEA_IN_STATES) + seeded_rectangles(
-            count, seed, density=density, density_range=density_range)
+                               density_range=(0.1, 100.0), extent=10.0, **kw):
+        """The reference states plus ``count`` seeded rectangles; with
+        ``extent`` != 10 the domain is extent x extent and every rectangle
+        (those of tea.in included) is scaled with it."""
+        f = extent / 10.0
+        scaled = [dict(s) for s in TEA_IN_STATES]
+        for s in scaled[1:]:
+            for k in ("xmin", "xmax", "ymin", "ymax"):
+                s[k] = s[k] * f
+        states = scaled + seeded_rectangles(
+            count, seed, density=density, density_range=density_range,
+            domain=(0.0, extent))
         kw.setdefault("comment", f"seed {seed}: {count} splitmix64 rectangles")
-        return cls(x_cells, y_cells, states=states, **kw)
+        return cls(x_cells, y_cells, states=states, extent=extent, **kw)
 
     def render(self):
         lines = ["*tea"]
@@ -108,7 +118,7 @@ class Deck:
             lines.append(line)
         lines += [
             f"x_cells={self.x_cells}", f"y_cells={self.y_cells}",
-            "xmin=0.0", "ymin=0.0", "xmax=10.0", "ymax=10.0",
+            "xmin=0.0", "ymin=0.0", f"xmax={_num(self.extent)}", f"ymax={_num(self.extent)}",
             f"initial_timestep={_num(self.dt)}",
             f"end_step={self.end_step}",
             f"max_iters={self.max_iters}",

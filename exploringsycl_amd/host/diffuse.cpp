@@ -20,7 +20,7 @@ static void solve(Chunk* chunks, Settings* settings, int tt,
   print_and_log(settings, "\nTimestep %d\n", tt + 1);
   profiler_start_timer(settings->wallclock_profile);
   const double t0 = now_seconds();
-  g_run.steps.push_back({tt + 1, 0, 0, 0, 0.0, 0.0, 0.0});
+  g_run.steps.push_back({tt + 1, 0, 0, 0, -1, 0.0, 0.0, 0.0});
 
   double dt = settings->dt_init;
   for (int cc = 0; cc < settings->num_chunks_per_rank; ++cc)
@@ -42,6 +42,10 @@ static void solve(Chunk* chunks, Settings* settings, int tt,
       break;
     case MPCG_SOLVER:
       mpcg_driver(chunks, settings, rx, ry, &error);
+      break;
+    case FCG_SOLVER:
+    case JACOBI_PCG_SOLVER:
+      fcg_driver(chunks, settings, rx, ry, &error);
       break;
     default:
       die(__LINE__, __FILE__,

@@ -4,6 +4,8 @@
 // loops over the rank's chunks and calls kernel_interface.h.
 #include <string.h>
 
+#include <string>
+
 #include "comms.h"
 #include "drivers.h"
 #include "kernel_interface.h"
@@ -256,6 +258,118 @@ void mpcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
     s.iterations = printed;
     s.executed = it;
     s.refreshes = refreshes;
+    s.rel_residual = std::sqrt(rr_true / bb);
+  }
+}
+
+// ---- flexible CG ---------------------------------------------------------
+
+// FCG(1) (Notay 2000): with z = M(r), p = z + beta p_old, w = A p,
+// alpha = p.r / p.w, r -= alpha w, z' = M(r), beta = -z'.w / p.w.  With a
+// fixed SPD M this is preconditioned CG; the learned M varies with r, which
+// the explicit orthogonalisation against p_old tolerates.  The FP64 true
+// residual is refreshed as in mpcg_driver (refresh_interval, and at the
+// tolerance), and the solve stops when |b - A u| / |b| < rel_eps.  With the
+// network, every stall_window iterations |r| must have fallen by
+// stall_factor; otherwise the solve continues as Jacobi-PCG from a restart
+// (beta = 0).
+void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+                double* error) {
+  if (settings->num_chunks_per_rank != 1)
+    die(__LINE__, __FILE__, "flexible CG drives one chunk per GPU\n");
+  Chunk* c = &chunks[0];
+  const bool learned = settings->solver == FCG_SOLVER;
+  static Chunk* attached = NULL;
+  if (attached != c) {
+    if (learned) {
+      if (g_extra.fcg_weights.empty())
+        die(__LINE__, __FILE__, "use_fcg needs fcg_weights=<network file>\n");
+      NetWeights nw;
+      std::string why;
+      if (!read_net_weights(g_extra.fcg_weights.c_str(), &nw, &why))
+        die(__LINE__, __FILE__, "network file %s: %s\n",
+            g_extra.fcg_weights.c_str(), why.c_str());
+      run_fcg_attach(c, settings, nw.params.data(), nw.params.size(), nw.norm,
+                     nw.coarsen);
+    } else {
+      run_fcg_attach(c, settings, NULL, 0, NULL, 0);
+    }
+    attached = c;
+  }
+  const double rel = g_extra.rel_eps > 0.0 ? g_extra.rel_eps : 1e-10;
+  const int refresh = g_extra.refresh_interval > 0 ? g_extra.refresh_interval : 50;
+  const int window = g_extra.stall_window > 0 ? g_extra.stall_window : 20;
+  const double need = g_extra.stall_factor * g_extra.stall_factor;
+
+  double rr = 0.0, bb = 0.0;
+  run_fcg_init(c, settings, rx, ry, &rr, &bb);
+  const double tol2 = rel * rel * bb;
+
+  bool use_net = learned;
+  double zw = 0.0;
+  run_fcg_precondition(c, settings, use_net, &zw);
+
+  double alpha = 0.0, beta = 0.0, rr_true = rr, rr_mark = rr;
+  bool pend_u = false, converged = false;
+  int it = 0, since = 0, refreshes = 0, fallback_at = -1, mark = 0;
+  for (it = 0; it < settings->max_iters; ++it) {
+    double pw = 0.0, pr = 0.0;
+    run_fcg_calc_w(c, settings, alpha, pend_u, beta, &pw, &pr);
+    alpha = pr / pw;
+    c->cg_alphas[it] = alpha;
+    double rrn = 0.0;
+    run_mpcg_calc_r(c, settings, alpha, &rrn);
+    pend_u = true;
+    ++since;
+    if (since >= refresh || rrn < tol2) {
+      double rt = 0.0;
+      rrn = 0.0;
+      run_mpcg_refresh(c, settings, alpha, 1, &rrn, &rt);
+      pend_u = false;
+      since = 0;
+      ++refreshes;
+      rr_true = rt;
+      if (rt < tol2) {
+        converged = true;
+        ++it;
+        break;
+      }
+    }
+    bool restart = false;
+    if (use_net && it + 1 - mark >= window) {
+      if (rrn * need > rr_mark) {
+        use_net = false;
+        restart = true;
+        fallback_at = it + 1;
+        print_and_log(settings, "FCG: \t\t\tstagnation after %d iterations, "
+                      "continuing as Jacobi-PCG\n", it + 1);
+      }
+      rr_mark = rrn;
+      mark = it + 1;
+    }
+    zw = 0.0;
+    run_fcg_precondition(c, settings, use_net, &zw);
+    beta = restart ? 0.0 : -zw / pw;
+    c->cg_betas[it] = beta;
+    rr = rrn;
+  }
+  if (!converged && pend_u) {
+    double rrn = 0.0, rt = 0.0;
+    run_mpcg_refresh(c, settings, alpha, 1, &rrn, &rt);
+    ++refreshes;
+    rr_true = rt;
+  }
+  *error = rr_true;
+  const int printed = converged ? it - 1 : it;
+  print_and_log(settings, "%s: \t\t%d iterations (%d FP64 refreshes), "
+                "relative residual %.3e\n", settings->solver_name, printed,
+                refreshes, std::sqrt(rr_true / bb));
+  if (!g_run.steps.empty()) {
+    StepRecord& s = g_run.steps.back();
+    s.iterations = printed;
+    s.executed = it;
+    s.refreshes = refreshes;
+    s.fallback_at = fallback_at;
     s.rel_residual = std::sqrt(rr_true / bb);
   }
 }

@@ -28,6 +28,13 @@ void cg_main_step_driver(Chunk* chunks, Settings* settings, int tt,
 void mpcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
                  double* error);
 
+/* Flexible CG with the learned preconditioner (FCG_SOLVER, network file
+ * from fcg_weights=) or with Jacobi alone (JACOBI_PCG_SOLVER); the same
+ * stop as mpcg_driver.  A learned solve whose |r| falls by less than
+ * stall_factor over stall_window iterations restarts as Jacobi-PCG. */
+void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+                double* error);
+
 void field_summary_driver(Chunk* chunks, Settings* settings,
                           bool solve_finished);
 void store_energy_driver(Chunk* chunk, Settings* settings);

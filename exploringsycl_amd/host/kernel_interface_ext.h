@@ -21,4 +21,26 @@ void run_mpcg_calc_r(Chunk* chunk, Settings* settings, double alpha,
 void run_mpcg_refresh(Chunk* chunk, Settings* settings, double alpha,
                       int apply_u, double* rr, double* rr_true);
 
+/* Flexible CG, FCG(1), with the FP32 vectors and FP64 scalars of the
+ * mixed-precision CG.  z = M(r): Jacobi, or Jacobi plus the learned
+ * network's coarse-grid correction.
+ *   run_fcg_attach: allocates the FCG state; params != NULL loads the
+ *     network (count parameters, feature normalisation norm[4], coarsening
+ *     factor coarsen).
+ *   run_fcg_init: as run_mpcg_init, plus the network's coefficient features.
+ *   run_fcg_precondition: z = M(r) (network if use_net, else Jacobi); zw +=
+ *     z.w.
+ *   run_fcg_calc_w: u += alpha p_old if apply_u; p = z + beta p_old; w = A p;
+ *     pw += p.w, pr += p.r.
+ * The residual update and the FP64 refresh are run_mpcg_calc_r and
+ * run_mpcg_refresh. */
+void run_fcg_attach(Chunk* chunk, Settings* settings, const float* params,
+                    size_t count, const float* norm, int coarsen);
+void run_fcg_init(Chunk* chunk, Settings* settings, double rx, double ry,
+                  double* rr, double* bb);
+void run_fcg_precondition(Chunk* chunk, Settings* settings, int use_net,
+                          double* zw);
+void run_fcg_calc_w(Chunk* chunk, Settings* settings, double alpha,
+                    int apply_u, double beta, double* pw, double* pr);
+
 #endif

@@ -149,10 +149,10 @@ void write_run_json(const char* path, Settings* settings) {
   for (size_t i = 0; i < g_run.steps.size(); ++i) {
     const StepRecord& s = g_run.steps[i];
     fprintf(fp, "%s\n    {\"step\": %d, \"iterations\": %d, \"executed\": %d, "
-                "\"refreshes\": %d, \"rel_residual\": %.17g, "
+                "\"refreshes\": %d, \"fallback_at\": %d, \"rel_residual\": %.17g, "
                 "\"error\": %.17g, \"seconds\": %.9f}",
             i ? "," : "", s.step, s.iterations, s.executed, s.refreshes,
-            s.rel_residual, s.error, s.seconds);
+            s.fallback_at, s.rel_residual, s.error, s.seconds);
   }
   fprintf(fp, "\n  ],\n  \"summaries\": [");
   for (size_t i = 0; i < g_run.summaries.size(); ++i) {

@@ -16,6 +16,7 @@ struct StepRecord {
   int iterations;     /* as printed: "CG: N iterations" */
   int executed;       /* CG iterations actually run (printed + 1 on convergence) */
   int refreshes;      /* FP64 true-residual refreshes (mixed precision) */
+  int fallback_at;    /* FCG: iteration at which Jacobi took over (-1: never) */
   double rel_residual; /* final |b - A u| / |b| where the solver computes it */
   double error;       /* last rr of the solve */
   double seconds;     /* wall time of the whole timestep */
@@ -53,7 +54,21 @@ extern HostOptions g_opts;
 struct SolverExtras {
   double rel_eps = 0.0;      /* rel_eps=: stop when |r|/|b| < rel_eps (0: off) */
   int refresh_interval = 50; /* refresh_interval=: FP64 true residual period */
+  std::string fcg_weights;   /* fcg_weights=: the preconditioner network file */
+  int stall_window = 20;     /* stall_window=: FCG stagnation window (iterations) */
+  double stall_factor = 2.0; /* stall_factor=: least |r| reduction over the window */
 };
+
+/* The learned preconditioner's network file: one ASCII line
+ *   "tea-nn-weights 1 <count> <coarsen> <m0> <s0> <m1> <s1>\n"
+ * followed by <count> little-endian float32 parameters.  Returns false (and
+ * sets *why) if the file cannot be read or is malformed. */
+struct NetWeights {
+  std::vector<float> params;
+  float norm[4] = {0.f, 1.f, 0.f, 1.f};
+  int coarsen = 0;
+};
+bool read_net_weights(const char* path, NetWeights* out, std::string* why);
 
 extern SolverExtras g_extra;
 

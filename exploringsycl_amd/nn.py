@@ -7,6 +7,10 @@ import numpy as np
 from . import native
 from .native import check
 
+# (out, in) channels of the six 3x3 layers, in parameter order; each layer's
+# weights [out][in][3][3] are followed by its biases [out].
+LAYER_SHAPES = [(32, 3)] + [(32, 32)] * 4 + [(1, 32)]
+
 _F = ctypes.POINTER(ctypes.c_float)
 _I = ctypes.POINTER(ctypes.c_int)
 

@@ -21,6 +21,7 @@ def run_deck(deck, binary=TEALEAF_BIN, workdir=None, extra_args=(), env=None,
         raise TeaHipError(f"{binary} is not built")
     own = workdir is None
     workdir = workdir or tempfile.mkdtemp(prefix="tea_")
+    os.makedirs(workdir, exist_ok=True)
     try:
         deck.write(os.path.join(workdir, "tea.in"))
         if problems and os.path.exists(problems):

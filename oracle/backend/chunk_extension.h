@@ -16,6 +16,8 @@ typedef struct ChunkExtension {
   float* mp_w;
   float* mp_kx;
   float* mp_ky;
+  /* Flexible CG: the preconditioned residual z (Jacobi only here). */
+  float* fcg_z;
 } ChunkExtension;
 
 #endif

@@ -54,6 +54,7 @@ void run_kernel_initialise(Chunk* c, Settings* s) {
   c->ext->mp_p[0] = c->ext->mp_p[1] = nullptr;
   c->ext->mp_r = c->ext->mp_w = c->ext->mp_kx = c->ext->mp_ky = nullptr;
   c->ext->mp_cur = 0;
+  c->ext->fcg_z = nullptr;
   const int n = s->max_iters > 0 ? s->max_iters : 1;
   c->cg_alphas = new double[n]();
   c->cg_betas = new double[n]();
@@ -70,7 +71,7 @@ void run_kernel_finalise(Chunk* c, Settings* s) {
                  c->vertex_x, c->vertex_y, c->ext->comms_buffer};
   for (double* p : f) free(p);
   float* g[] = {c->ext->mp_p[0], c->ext->mp_p[1], c->ext->mp_r, c->ext->mp_w,
-                c->ext->mp_kx, c->ext->mp_ky};
+                c->ext->mp_kx, c->ext->mp_ky, c->ext->fcg_z};
   for (float* p : g) free(p);
   delete[] c->cg_alphas;
   delete[] c->cg_betas;
@@ -236,5 +237,41 @@ void run_mpcg_refresh(Chunk* c, Settings* s, double alpha, int apply_u,
                    e->mp_p[e->mp_cur], c->u0, c->kx, c->ky, e->mp_r, out);
   *rr += out[0];
   *rr_true += out[1];
+}
+
+// ---- flexible CG (Jacobi preconditioner only) ----------------------------
+
+void run_fcg_attach(Chunk* c, Settings* s, const float* params, size_t count,
+                    const float* norm, int coarsen) {
+  (void)s; (void)count; (void)norm; (void)coarsen;
+  if (params)
+    die(__LINE__, __FILE__, "the oracle backend has no network preconditioner\n");
+  if (!c->ext->fcg_z)
+    c->ext->fcg_z = (float*)calloc((size_t)c->x * c->y, sizeof(float));
+}
+
+void run_fcg_init(Chunk* c, Settings* s, double rx, double ry, double* rr,
+                  double* bb) {
+  run_mpcg_init(c, s, rx, ry, rr, bb);
+}
+
+void run_fcg_precondition(Chunk* c, Settings* s, int use_net, double* zw) {
+  if (use_net)
+    die(__LINE__, __FILE__, "the oracle backend has no network preconditioner\n");
+  ChunkExtension* e = c->ext;
+  *zw += orc_fcg_jacobi(c->x, c->y, s->halo_depth, e->mp_r, e->mp_kx, e->mp_ky,
+                        e->mp_w, e->fcg_z);
+}
+
+void run_fcg_calc_w(Chunk* c, Settings* s, double alpha, int apply_u,
+                    double beta, double* pw, double* pr) {
+  ChunkExtension* e = c->ext;
+  double out[2];
+  orc_fcg_calc_w(c->x, c->y, s->halo_depth, alpha, apply_u, beta,
+                 e->mp_p[e->mp_cur], e->mp_p[e->mp_cur ^ 1], e->fcg_z, e->mp_r,
+                 c->u, e->mp_w, e->mp_kx, e->mp_ky, out);
+  e->mp_cur ^= 1;
+  *pw += out[0];
+  *pr += out[1];
 }
 #endif

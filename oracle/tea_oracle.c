@@ -493,3 +493,32 @@ void orc_mpcg_refresh(int x, int y, int h, double alpha, int apply_u,
   out[0] = finish(part0, x, y);
   out[1] = finish(part1, x, y);
 }
+
+/* ---- flexible CG (no reference counterpart) ---------------------------
+ * Restates exploringsycl_amd/csrc/fcg.hip: the Jacobi preconditioner
+ * z = r / diag(A) with the diagonal formed in FP64 from the FP32
+ * coefficients, and the direction update p = z + beta p_old fused with
+ * w = A p and the two dots p.w, p.r. */
+double orc_fcg_jacobi(int x, int y, int h, const float* r, const float* kx,
+                      const float* ky, const float* w, float* z) {
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, {
+    const double d = (1.0 + ((double)kx[i + 1] + (double)kx[i])) +
+                     ((double)ky[i + x] + (double)ky[i]);
+    const float zv = (float)((double)r[i] / d);
+    z[i] = zv;
+    val = (double)zv * (double)w[i];
+  });
+  return finish(part, x, y);
+}
+
+void orc_fcg_calc_w(int x, int y, int h, double alpha, int apply_u,
+                    double beta, const float* p_old, float* p_new,
+                    const float* z, const float* r, double* u, float* w,
+                    const float* kx, const float* ky, double out[2]) {
+  out[0] = orc_mpcg_calc_w(x, y, h, alpha, apply_u, beta, 1, p_old, p_new, z,
+                           u, w, kx, ky);
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, { val = (double)p_new[i] * (double)r[i]; });
+  out[1] = finish(part, x, y);
+}

@@ -115,6 +115,17 @@ void orc_mpcg_refresh(int x, int y, int h, double alpha, int apply_u,
                       const double* kx, const double* ky, float* r32,
                       double out[2]);
 
+/* Flexible CG (no reference counterpart; restates the arithmetic of
+ * exploringsycl_amd/csrc/fcg.hip).  orc_fcg_jacobi: z = r / diag(A),
+ * returns z.w.  orc_fcg_calc_w: orc_mpcg_calc_w with p = z + beta p_old,
+ * out = {p.w, p.r}. */
+double orc_fcg_jacobi(int x, int y, int h, const float* r, const float* kx,
+                      const float* ky, const float* w, float* z);
+void orc_fcg_calc_w(int x, int y, int h, double alpha, int apply_u,
+                    double beta, const float* p_old, float* p_new,
+                    const float* z, const float* r, double* u, float* w,
+                    const float* kx, const float* ky, double out[2]);
+
 /* The 5-point operator applied at one cell (shared.h:59-63). */
 double orc_smvp(const double* a, const double* kx, const double* ky, long i,
                 long x);

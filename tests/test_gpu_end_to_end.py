@@ -125,3 +125,28 @@ def test_mixed_precision_close_to_fp64():
     for k in ("vol", "mass", "ie"):
         assert a[k] == b[k]
     assert abs(a["temp"] / b["temp"] - 1) < 1e-9
+
+
+JPCG_DECKS = {
+    "jpcg_128x128x2": Deck.with_random_rectangles(
+        128, 128, 24, problem_seed(5, 1), density="log10", density_range=(-2.0, 2.0),
+        end_step=2, coefficient="inverse", extra=["use_jacobi_pcg", "rel_eps=1e-10"]),
+    "jpcg_odd_121x97x1": Deck.with_random_rectangles(
+        121, 97, 16, problem_seed(5, 2), density="log10", density_range=(-2.0, 2.0),
+        end_step=1, coefficient="inverse",
+        extra=["use_jacobi_pcg", "rel_eps=1e-10", "refresh_interval=20"]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(JPCG_DECKS))
+def test_jacobi_pcg_matches_oracle(name):
+    """Flexible CG with the Jacobi preconditioner on the GPU against its CPU
+    restatement: same iteration and refresh counts, bitwise summaries."""
+    deck = JPCG_DECKS[name]
+    gpu, _ = run_deck(deck)
+    cpu, _ = run_deck(deck, binary=ORACLE_TEALEAF, env={"OMP_NUM_THREADS": "16"})
+    assert gpu["solver"] == "Jacobi-PCG"
+    assert [s["refreshes"] for s in gpu["steps"]] == [s["refreshes"] for s in cpu["steps"]]
+    _compare(gpu, cpu)
+    for s in gpu["steps"]:
+        assert s["rel_residual"] < 1e-10

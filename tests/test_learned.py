@@ -1,0 +1,104 @@
+"""Host side of the learned preconditioner: the training problems' fields and
+coefficients against the oracle's cg_init, the tile decoupling used to batch
+them, and the network file."""
+
+import numpy as np
+import pytest
+
+from exploringsycl_amd import learned
+from tests.oracle_ctypes import new_fields
+
+H = 2
+
+
+def test_tile_problems_are_seeded_and_in_range():
+    a = learned.tile_problem(7)
+    b = learned.tile_problem(7)
+    for u, v in zip(a[:2], b[:2]):
+        assert np.array_equal(u, v)
+    dens, ener, dx = a
+    assert dens.shape == (256, 256)
+    assert dx == 10.0 / learned.RESOLUTIONS[7 % 2]
+    rect = dens != 100.0
+    assert rect.any()
+    lo, hi = learned.DENSITY_RANGE
+    assert dens[rect].min() >= 10.0 ** lo and dens[rect].max() <= 10.0 ** hi
+    assert (ener[rect] >= 0.1).all() and (ener[rect] <= 25.0).all()
+    assert not np.array_equal(learned.tile_problem(8)[0], dens)
+
+
+def _oracle_init(oracle, dens, dx):
+    ny, nx = dens.shape
+    x, y = nx + 2 * H, ny + 2 * H
+    f = new_fields(x, y, H)
+    f["density"].reshape(y, x)[H:-H, H:-H] = dens
+    f["energy"].reshape(y, x)[H:-H, H:-H] = 1.0
+    for k in ("density", "energy"):
+        oracle.halo_field(x, y, H, 2, f[k])
+    rx = learned.DT / (dx * dx)
+    oracle.cg_init(x, y, H, 2, rx, rx, f)
+    return x, y, f
+
+
+def test_face_coefficients_match_oracle_cg_init(oracle):
+    dens, _, dx = learned.tile_problem(3)
+    kx, ky = learned.face_coefficients(dens, dx)
+    x, y, f = _oracle_init(oracle, dens, dx)
+    okx = f["kx"].reshape(y, x)[H:y - H, H:x - H + 1]
+    oky = f["ky"].reshape(y, x)[H:y - H + 1, H:x - H]
+    assert np.array_equal(kx, okx)
+    assert np.array_equal(ky, oky)
+
+
+def test_zero_faces_decouple_tiles(oracle):
+    """A 2 x 2 tiling with zero faces on every tile edge applies, tile by
+    tile, the operator of each tile alone with reflective boundaries."""
+    rng = np.random.default_rng(5)
+    n, g = 16, 2
+    dx = 10.0 / 4096
+    tiles = [10.0 ** rng.uniform(-2, 2, (n, n)) for _ in range(g * g)]
+    X = g * n + 2 * H
+    kx = np.zeros((X, X))
+    ky = np.zeros((X, X))
+    p = np.zeros((X, X))
+    want = np.zeros((g * n, g * n))
+    for t, dens in enumerate(tiles):
+        gy, gx = divmod(t, g)
+        fx, fy = learned.face_coefficients(dens, dx)
+        fx[:, [0, -1]] = 0.0
+        fy[[0, -1], :] = 0.0
+        kx[H + gy * n:H + (gy + 1) * n, H + gx * n:H + (gx + 1) * n + 1] = fx
+        ky[H + gy * n:H + (gy + 1) * n + 1, H + gx * n:H + (gx + 1) * n] = fy
+        v = rng.standard_normal((n, n))
+        p[H + gy * n:H + (gy + 1) * n, H + gx * n:H + (gx + 1) * n] = v
+        # The same tile alone, reflective halos, coefficients from cg_init.
+        x, y, f = _oracle_init(oracle, dens, dx)
+        f["p"].reshape(y, x)[H:-H, H:-H] = v
+        oracle.halo_field(x, y, H, 1, f["p"])
+        oracle.cg_calc_w(x, y, H, f)
+        want[gy * n:(gy + 1) * n, gx * n:(gx + 1) * n] = f["w"].reshape(y, x)[H:-H, H:-H]
+    f = dict(p=p.ravel().copy(), w=np.zeros(X * X), kx=kx.ravel(), ky=ky.ravel())
+    oracle.cg_calc_w(X, X, H, f)
+    got = f["w"].reshape(X, X)[H:-H, H:-H]
+    assert np.allclose(got, want, rtol=1e-12, atol=0)
+
+
+def test_network_file_round_trip(tmp_path):
+    params = np.random.default_rng(0).standard_normal(38177).astype(np.float32)
+    norm = np.array([1.5, 2.0, -0.5, 3.0], dtype=np.float32)
+    path = str(tmp_path / "net.bin")
+    learned.write_weights(path, params, norm)
+    p2, n2, c = learned.read_weights(path)
+    assert np.array_equal(p2, params) and np.array_equal(n2, norm) and c == learned.COARSEN
+    with open(path, "r+b") as f:
+        f.truncate(1000)
+    with pytest.raises(ValueError):
+        learned.read_weights(path)
+
+
+def test_initial_params_layout():
+    p = learned.initial_params(0)
+    assert p.size == 38177 and p.dtype == np.float32
+    # Biases are zero: the last parameter is b6, the 32 before W6 are b5.
+    assert p[-1] == 0.0
+    assert np.all(p[-1 - 288 - 32:-1 - 288] == 0.0)
