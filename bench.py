@@ -7,6 +7,13 @@ random rectangles, eps 1e-15 absolute, max_iters 10000, dt 0.004.  W warm-up
 timesteps run first; the initial state is then restored and K timesteps are
 timed (device-synchronised on both sides) by bin/tealeaf itself.
 
+A second part times the solve the metric names: one seeded 4096 x 4096
+field of 64 rectangles (log10 density uniform in [-2, 2], K = 1/density,
+seed 20269202, not among the training problems), one timestep solved
+to 1e-10 relative residual by plain FP64 CG, mixed-precision CG, Jacobi-PCG
+and flexible CG with the learned preconditioner; each after one warm-up
+step, with its iterations, time and final-field error against FP64 CG.
+
 Printed (one JSON line): effective HBM throughput of the timed solve,
   value = 88 B x N^2 x (CG iterations executed) / t_solve,
 88 B being the algorithmic FP64 traffic per cell per iteration (11 words),
@@ -48,6 +55,8 @@ def parse_args():
                     help="CPU-baseline sample length (0 disables it)")
     ap.add_argument("--unfused", action="store_true",
                     help="one launch per reference kernel (13 words/iteration)")
+    ap.add_argument("--tts", type=int, default=1,
+                    help="run the time-to-solution comparison (0 skips it)")
     return ap.parse_args()
 
 
@@ -91,6 +100,57 @@ def cpu_baseline(cells, seconds):
                    f"convention; {iters / el:.2f} iterations/s"),
         "iterations_per_s": iters / el,
     }
+
+
+def time_to_solution(n, timeout=1200):
+    """Plain FP64 CG, MPCG, Jacobi-PCG and learned FCG on one unseen field of
+    the 64-rectangle workload at n x n, stopped at 1e-10 relative residual."""
+    import numpy as np
+    import shutil
+    from exploringsycl_amd.deck import Deck, problem_seed
+    from exploringsycl_amd.learned import WEIGHTS_PATH
+    from exploringsycl_amd.native import TEALEAF_BIN
+    from exploringsycl_amd.runner import run_deck
+
+    seed = problem_seed(5, 3000)
+    solvers = [("plain_fp64_cg", []), ("mixed_precision_cg", ["use_mpcg"]),
+               ("jacobi_pcg", ["use_jacobi_pcg"])]
+    if os.path.exists(WEIGHTS_PATH):
+        solvers.append(("learned_fcg", ["use_fcg", "fcg_weights=net.bin"]))
+    out = {"workload": (f"{n}x{n} cells, 64 seeded rectangles (log10 density "
+                        f"U[-2,2], seed {seed}) + tea.in states, K=1/density, "
+                        "1 timestep, rel. residual 1e-10, after 1 warm-up step"),
+           "solvers": {}}
+    fields = {}
+    with tempfile.TemporaryDirectory(prefix="tea_tts_") as root:
+        for name, extra in solvers:
+            deck = Deck.with_random_rectangles(
+                n, n, 64, seed, density="log10", density_range=(-2.0, 2.0),
+                end_step=1, coefficient="inverse", extra=["rel_eps=1e-10"] + extra)
+            wd = os.path.join(root, name)
+            os.makedirs(wd)
+            if name == "learned_fcg":
+                shutil.copy(WEIGHTS_PATH, os.path.join(wd, "net.bin"))
+            rec, _ = run_deck(deck, binary=TEALEAF_BIN, workdir=wd, timeout=timeout,
+                              extra_args=["--warmup-steps", "1", "--field-out", "u.bin"])
+            st = rec["steps"][0]
+            fields[name] = np.fromfile(os.path.join(wd, "u.bin"))
+            out["solvers"][name] = {
+                "seconds": round(rec["timed_seconds"], 4),
+                "iterations": st["iterations"],
+                "rel_residual": st["rel_residual"],
+                "fallback_at": st["fallback_at"],
+                "temp": rec["summaries"][-1]["temp"],
+            }
+    ref = fields["plain_fp64_cg"]
+    tref = out["solvers"]["plain_fp64_cg"]
+    for name, rec in out["solvers"].items():
+        d = fields[name] - ref
+        rec["field_err_max_rel"] = float(np.abs(d).max() / np.abs(ref).max())
+        rec["field_err_l2_rel"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
+        rec["temp_rel_err"] = abs(rec["temp"] / tref["temp"] - 1.0)
+        rec["speedup_vs_plain_cg"] = round(tref["seconds"] / rec["seconds"], 3)
+    return out
 
 
 def main():
@@ -167,6 +227,12 @@ def main():
         "roofline": roofline,
         "wall_s_including_setup": round(t_wall, 2),
     }
+    if args.tts:
+        tts = time_to_solution(n)
+        line["time_to_solution"] = tts
+        sv = tts["solvers"]
+        line["learned_fcg_vs_plain_cg_speedup"] = (
+            sv["learned_fcg"]["speedup_vs_plain_cg"] if "learned_fcg" in sv else None)
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     print(json.dumps(line), flush=True)

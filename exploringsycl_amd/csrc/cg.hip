@@ -275,12 +275,6 @@ __global__ __launch_bounds__(kBlock) void fused_w_kernel(Geo g, FusedWArgs a,
 // 4096-cell span in 8 slots of 512 cells.  The first tree level is the
 // in-lane pair sum, then the butterfly runs over each 32-lane half.
 
-constexpr int kSlots = kSpan / (2 * kBlock);  // 8
-
-__device__ inline double2 ld2(const double* p, int i) {
-  return *reinterpret_cast<const double2*>(p + i);
-}
-
 // Streaming (nontemporal) access for data read or written once per pass;
 // stencil operands (p, r, kx, ky) use ld2 so the neighbouring rows can reuse
 // them from L2.
@@ -303,42 +297,11 @@ __device__ inline void st2_nt(double* p, int i, bool in0, bool in1, double v0,
   }
 }
 
-__device__ inline int bitrev3(int l) {
-  return ((l & 1) << 2) | (l & 2) | ((l & 4) >> 2);
-}
-
-// Level-1 trees of the 8 slot pair-sums of a lane, transposed across lanes
-// as in wave_tree16 (steps 1, 2, 4), then steps 8 and 16 within each 32-lane
-// half; stores each group's partial in lds[group-in-span] and finishes the
-// span's level-2 tree in wave 0.
 __device__ inline void span_reduce_pairs(double (&v)[kSlots], double* partials,
                                          int nspan, int span,
                                          bool single_group) {
-  __shared__ double lds[kWave];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
-#pragma unroll
-  for (int step = 0; step < 3; ++step) {
-    const int m = 1 << step;
-    const int half = kSlots >> (step + 1);
-    const bool upper = (lane & m) != 0;
-#pragma unroll
-    for (int i = 0; i < half; ++i) {
-      const double send = upper ? v[i] : v[half + i];
-      const double keep = upper ? v[half + i] : v[i];
-      v[i] = keep + __shfl_xor(send, m);
-    }
-  }
-  double t = v[0];
-  t = t + __shfl_xor(t, 8);
-  t = t + __shfl_xor(t, 16);
-  if ((lane & 31) < kSlots)
-    lds[bitrev3(lane & 7) * 8 + wv * 2 + (lane >> 5)] = t;
-  __syncthreads();
-  if (wv == 0) {
-    const double r = single_group ? lds[0] : wave_tree(lds[lane]);
-    if (lane == 0) partials[span] = r;
-  }
+  double (&vv)[1][kSlots] = *reinterpret_cast<double (*)[1][kSlots]>(&v);
+  span_reduce_pairs_n<1>(vv, partials, nspan, span, single_group);
 }
 
 template <bool APPLY_P, int SB>

@@ -7,15 +7,15 @@
 //
 // Preconditioners, z = M(r):
 //   Jacobi:  z = r / diag(A), diag = 1 + kx[i] + kx[i+1] + ky[i] + ky[i+x].
-//   Learned: z = r / diag(A) + s * P(N(R r / s, f_kx, f_ky)), where R is the
-//            c x c block mean onto the coarse grid, s the RMS of R r (both
+//   Learned: z = r / diag(A) + P(s N(R r / s, f)), where R is the c x c
+//            block mean onto the coarse grid, s the RMS of R r (both
 //            computed on the device), N the 6-layer network on the coarse
-//            grid, P the piecewise-constant prolongation, and f_kx, f_ky the
-//            coarse means of log kx, log ky, normalised with the constants
-//            stored with the network.
-// The network's output is a coarse-grid correction: with c = 4 it costs
-// 1/16 of a full-resolution pass and its 13-cell receptive field spans 52
-// fine cells.
+//            grid, P the piecewise-constant prolongation and f the coarse
+//            operator's face features (feature_kernel).  N approximates the
+//            inverse of T = P^T A P / c^2, so that with N = T^-1 this is the
+//            two-level additive preconditioner D^-1 + P (P^T A P)^-1 P^T.
+// With c = 8 the network runs on 1/64 of the cells; its 13-cell receptive
+// field spans 104 fine cells.
 
 #include <hip/hip_runtime.h>
 
@@ -130,7 +130,58 @@ struct ProlongBody {
   }
 };
 
-// Coarse features: normalised block means of log kx and log ky.
+// z = r / diag (+ s * net[coarse cell] with NET) and z.w, two cells per
+// lane (x even): the arithmetic of JacobiBody / ProlongBody cell by cell.
+// The coarse cell of interior cell (kk, jj) is ((kk-h) >> shift,
+// (jj-h) >> shift).
+template <bool NET>
+__global__ __launch_bounds__(kBlock) void fcg_z2_kernel(
+    Geo g, const float* __restrict__ r, const float* __restrict__ kx,
+    const float* __restrict__ ky, const float* __restrict__ w,
+    const float* __restrict__ net, const float* __restrict__ scale,
+    float* __restrict__ z, int shift, int wc, double* partials, int nspan,
+    int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int x = g.x;
+  const double sc = NET ? (double)scale[0] : 0.0;
+  double psum[1][kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const int e = base + k * (2 * kBlock) + 2 * threadIdx.x;
+    const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+    const int kk = e - row * x;
+    const bool in0 = e < g.n && interior(g, kk, row);
+    const bool in1 = e < g.n && interior(g, kk + 1, row);
+    const bool any = in0 || in1;
+    const int ic = e < g.n ? e : x + 2;
+    const int ir = any ? e : x + 2;
+    const float2 R = ld2(r, ir), KX = ld2(kx, ic), KY = ld2(ky, ir),
+                 KYt = ld2(ky, ir + x), W = ld2(w, ir);
+    float kxr = __shfl_down(KX.x, 1);
+    if (lane == kWave - 1 && any) kxr = kx[e + 2];
+    const double da = (1.0 + ((double)KX.y + (double)KX.x)) + ((double)KYt.x + (double)KY.x);
+    const double db = (1.0 + ((double)kxr + (double)KX.y)) + ((double)KYt.y + (double)KY.y);
+    float za = 0.0f, zb = 0.0f;
+    if (NET) {
+      const int Y = (row - g.h) >> shift;
+      if (in0) za = (float)((double)R.x / da + sc * (double)net[Y * wc + ((kk - g.h) >> shift)]);
+      if (in1) zb = (float)((double)R.y / db + sc * (double)net[Y * wc + ((kk + 1 - g.h) >> shift)]);
+    } else {
+      za = (float)((double)R.x / da);
+      zb = (float)((double)R.y / db);
+    }
+    psum[0][k] = (in0 ? (double)za * (double)W.x : 0.0) + (in1 ? (double)zb * (double)W.y : 0.0);
+    if (any) st2(z, e, in0, in1, za, zb);
+  }
+  span_reduce_pairs_n<1>(psum, partials, nspan, span, single_group != 0);
+}
+
+// Coarse features: for each coarse cell, log(1 + k) of its left and bottom
+// faces in the coarse operator T = P^T A P / c^2, where k is the sum of the
+// c fine face coefficients along that face over c^2 (zero on the domain
+// edge, the reflective boundary), normalised by (m, s).
 __global__ __launch_bounds__(256) void feature_kernel(Geo g, int c, int hc, int wc,
                                                       const float* __restrict__ kx,
                                                       const float* __restrict__ ky,
@@ -140,17 +191,17 @@ __global__ __launch_bounds__(256) void feature_kernel(Geo g, int c, int hc, int 
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= hc * wc) return;
   const int X = idx % wc, Y = idx / wc;
-  double lx = 0.0, ly = 0.0;
+  const int corner = (g.h + Y * c) * g.x + g.h + X * c;
+  double sx = 0.0, sy = 0.0;
   for (int b = 0; b < c; ++b) {
-    const int row = (g.h + Y * c + b) * g.x + g.h + X * c;
-    for (int a = 0; a < c; ++a) {
-      lx += log(fmax((double)kx[row + a], 1e-30));
-      ly += log(fmax((double)ky[row + a], 1e-30));
-    }
+    sx += (double)kx[corner + b * g.x];
+    sy += (double)ky[corner + b];
   }
-  const double n = (double)(c * c);
-  feat[idx] = (float)((lx / n - m0) / s0);
-  feat[hc * wc + idx] = (float)((ly / n - m1) / s1);
+  const double cc = (double)(c * c);
+  const double kxc = X == 0 ? 0.0 : sx / cc;
+  const double kyc = Y == 0 ? 0.0 : sy / cc;
+  feat[idx] = (float)((log1p(kxc) - m0) / s0);
+  feat[hc * wc + idx] = (float)((log1p(kyc) - m1) / s1);
 }
 
 int ensure_fcg_buffers(tea_hip_chunk* c) {
@@ -178,7 +229,8 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
   TEA_RETURN_IF(ensure_fcg_buffers(c));
   if (!params) return TEA_HIP_OK;  // Jacobi only
   TEA_CHECK_ARG(feature_norm != nullptr, "null feature normalisation");
-  TEA_CHECK_ARG(coarsen >= 1, "coarsening factor must be >= 1");
+  TEA_CHECK_ARG(coarsen >= 1 && (coarsen & (coarsen - 1)) == 0,
+                "coarsening factor must be a power of two");
   const int nx = c->geo.x - 2 * c->geo.h, ny = c->geo.y - 2 * c->geo.h;
   TEA_CHECK_ARG(nx % coarsen == 0 && ny % coarsen == 0,
                 "the interior must be a multiple of the coarsening factor");
@@ -243,9 +295,15 @@ int tea_hip_fcg_precondition(tea_hip_chunk* c, int use_net, double* zw) {
   const int sg = c->ngroup == 1 ? 1 : 0;
   if (!use_net) {
     Timed t(c, K_FCG_JACOBI);
-    JacobiBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, c->fcg_z, g.x};
-    hipLaunchKernelGGL((span_kernel<1, JacobiBody>), span_grid(c), dim3(kBlock), 0,
-                       c->stream, g, body, c->partials, c->nspan, sg);
+    if (c->pairs) {
+      hipLaunchKernelGGL((fcg_z2_kernel<false>), span_grid(c), dim3(kBlock), 0,
+                         c->stream, g, c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, nullptr,
+                         nullptr, c->fcg_z, 0, 0, c->partials, c->nspan, sg);
+    } else {
+      JacobiBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, c->fcg_z, g.x};
+      hipLaunchKernelGGL((span_kernel<1, JacobiBody>), span_grid(c), dim3(kBlock), 0,
+                         c->stream, g, body, c->partials, c->nspan, sg);
+    }
   } else {
     const int cn = c->fcg_hc * c->fcg_wc;
     const int nblk = (cn + 255) / 256;
@@ -266,11 +324,20 @@ int tea_hip_fcg_precondition(tea_hip_chunk* c, int use_net, double* zw) {
     }
     {
       Timed t(c, K_FCG_PROLONG);
-      ProlongBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
-                       nn::output(c->fcg_net), c->fcg_scale, c->fcg_z,
-                       g.x, g.h, c->fcg_c, c->fcg_wc};
-      hipLaunchKernelGGL((span_kernel<1, ProlongBody>), span_grid(c), dim3(kBlock), 0,
-                         c->stream, g, body, c->partials, c->nspan, sg);
+      if (c->pairs) {
+        int shift = 0;
+        while ((1 << shift) < c->fcg_c) ++shift;
+        hipLaunchKernelGGL((fcg_z2_kernel<true>), span_grid(c), dim3(kBlock), 0,
+                           c->stream, g, c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
+                           nn::output(c->fcg_net), c->fcg_scale, c->fcg_z, shift,
+                           c->fcg_wc, c->partials, c->nspan, sg);
+      } else {
+        ProlongBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
+                         nn::output(c->fcg_net), c->fcg_scale, c->fcg_z,
+                         g.x, g.h, c->fcg_c, c->fcg_wc};
+        hipLaunchKernelGGL((span_kernel<1, ProlongBody>), span_grid(c), dim3(kBlock), 0,
+                           c->stream, g, body, c->partials, c->nspan, sg);
+      }
     }
   }
   TEA_HIP_TRY(hipGetLastError());
@@ -290,9 +357,14 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
            alpha, beta, apply_u ? 1 : 0, 1};
   {
     Timed t(c, K_FCG_W);
-    hipLaunchKernelGGL((mp_w_kernel<4, true>), span_grid(c), dim3(kBlock), 0,
-                       c->stream, c->geo, a, c->partials, c->nspan,
-                       c->ngroup == 1 ? 1 : 0);
+    if (c->pairs)
+      hipLaunchKernelGGL((mp_w2_kernel<true, 4>), span_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, a, c->partials, c->nspan,
+                         c->ngroup == 1 ? 1 : 0);
+    else
+      hipLaunchKernelGGL((mp_w_kernel<4, true>), span_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, a, c->partials, c->nspan,
+                         c->ngroup == 1 ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
   c->mp_cur ^= 1;

@@ -309,6 +309,12 @@ void run_mpcg_refresh(Chunk* chunk, Settings* settings, double alpha,
   *rr_true += b;
 }
 
+void run_read_u(Chunk* chunk, Settings* settings, double* host) {
+  (void)settings;
+  HIP_CALL(tea_hip_field_read(H(chunk), TEA_FIELD_U, host,
+                              (size_t)chunk->x * chunk->y));
+}
+
 void run_fcg_attach(Chunk* chunk, Settings* settings, const float* params,
                     size_t count, const float* norm, int coarsen) {
   (void)settings;

@@ -157,6 +157,190 @@ __global__ __launch_bounds__(kBlock) void mp_r_kernel(
   span_reduce<1>(prod, partials, nspan, span, single_group != 0);
 }
 
+// Two cells per lane (x even): the arithmetic of mp_w_kernel cell by cell,
+// with 8-byte loads of the FP32 operands and 16-byte loads of u.
+template <bool DOT2, int SB>
+__global__ __launch_bounds__(kBlock) void mp_w2_kernel(Geo g, MpArgs a,
+                                                       double* partials,
+                                                       int nspan,
+                                                       int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int x = g.x;
+  const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
+  const float* __restrict__ P = a.p_old;
+  const float* __restrict__ R = a.r;
+  const double beta = a.beta;
+  double psum[DOT2 ? 2 : 1][kSlots];
+
+#pragma unroll
+  for (int s0 = 0; s0 < kSlots; s0 += SB) {
+    int e0[SB], kk[SB], jj[SB];
+    bool in0[SB], in1[SB], any[SB];
+    float2 P0[SB], Pt[SB], Pb[SB], R0[SB], Rt[SB], Rb[SB], KX[SB], KY[SB], KYt[SB],
+        RD[SB];
+    double2 U[SB];
+#pragma unroll
+    for (int j = 0; j < SB; ++j) {
+      const int e = base + (s0 + j) * (2 * kBlock) + 2 * threadIdx.x;
+      const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+      e0[j] = e;
+      jj[j] = row;
+      kk[j] = e - row * x;
+      in0[j] = e < g.n && interior(g, kk[j], row);
+      in1[j] = e < g.n && interior(g, kk[j] + 1, row);
+      any[j] = in0[j] || in1[j];
+    }
+#pragma unroll
+    for (int j = 0; j < SB; ++j) {
+      // A pair's own p, r and kx are also its neighbours' left/right
+      // operands (by shuffle), so they are read whenever the pair is in
+      // range; the rows above and below only for interior pairs.  x + 2 is
+      // an even index whose +-x rows are in range.
+      const int ic = e0[j] < g.n ? e0[j] : x + 2;
+      const int ir = any[j] ? e0[j] : x + 2;
+      P0[j] = ld2(P, ic);
+      Pt[j] = ld2(P, ir + x);
+      Pb[j] = ld2(P, ir - x);
+      if (a.apply_p) {
+        R0[j] = ld2(R, ic);
+        Rt[j] = ld2(R, ir + x);
+        Rb[j] = ld2(R, ir - x);
+      }
+      KX[j] = ld2(a.kx, ic);
+      KY[j] = ld2(a.ky, ir);
+      KYt[j] = ld2(a.ky, ir + x);
+      if (a.apply_u) U[j] = ld2(a.u, ir);
+      if (DOT2) RD[j] = ld2(a.rdot, ir);
+    }
+#pragma unroll
+    for (int j = 0; j < SB; ++j) {
+      const int i = e0[j];
+      float pl = __shfl_up(P0[j].y, 1), pr = __shfl_down(P0[j].x, 1);
+      float rl = 0.0f, rr = 0.0f;
+      if (a.apply_p) {
+        rl = __shfl_up(R0[j].y, 1);
+        rr = __shfl_down(R0[j].x, 1);
+      }
+      float kxr = __shfl_down(KX[j].x, 1);
+      if (lane == 0 && any[j]) {
+        pl = P[i - 1];
+        if (a.apply_p) rl = R[i - 1];
+      }
+      if (lane == kWave - 1 && any[j]) {
+        pr = P[i + 2];
+        if (a.apply_p) rr = R[i + 2];
+        kxr = a.kx[i + 2];
+      }
+      const int ka = kk[j], kb = kk[j] + 1, row = jj[j];
+      float ca, cb, la, ra, lb, rb2, ba, ta, bb, tb;
+      if (a.apply_p) {
+        ca = (float)(beta * (double)P0[j].x + (double)R0[j].x);
+        cb = (float)(beta * (double)P0[j].y + (double)R0[j].y);
+        la = ka == xl ? ca : (float)(beta * (double)pl + (double)rl);
+        ra = ka == xr ? ca : cb;
+        lb = kb == xl ? cb : ca;
+        rb2 = kb == xr ? cb : (float)(beta * (double)pr + (double)rr);
+        ba = row == yb ? ca : (float)(beta * (double)Pb[j].x + (double)Rb[j].x);
+        ta = row == yt ? ca : (float)(beta * (double)Pt[j].x + (double)Rt[j].x);
+        bb = row == yb ? cb : (float)(beta * (double)Pb[j].y + (double)Rb[j].y);
+        tb = row == yt ? cb : (float)(beta * (double)Pt[j].y + (double)Rt[j].y);
+      } else {
+        ca = P0[j].x;
+        cb = P0[j].y;
+        la = pl;
+        ra = cb;
+        lb = ca;
+        rb2 = pr;
+        ba = Pb[j].x;
+        ta = Pt[j].x;
+        bb = Pb[j].y;
+        tb = Pt[j].y;
+      }
+      const float kxa0 = KX[j].x, kxa1 = KX[j].y, kxb0 = KX[j].y, kxb1 = kxr;
+      const float diag_a = (1.0f + (kxa1 + kxa0)) + (KYt[j].x + KY[j].x);
+      const float diag_b = (1.0f + (kxb1 + kxb0)) + (KYt[j].y + KY[j].y);
+      const float sa = (diag_a * ca - (kxa1 * ra + kxa0 * la)) -
+                       (KYt[j].x * ta + KY[j].x * ba);
+      const float sb = (diag_b * cb - (kxb1 * rb2 + kxb0 * lb)) -
+                       (KYt[j].y * tb + KY[j].y * bb);
+      psum[0][s0 + j] = (in0[j] ? (double)sa * (double)ca : 0.0) +
+                        (in1[j] ? (double)sb * (double)cb : 0.0);
+      if (DOT2)
+        psum[DOT2 ? 1 : 0][s0 + j] = (in0[j] ? (double)ca * (double)RD[j].x : 0.0) +
+                                     (in1[j] ? (double)cb * (double)RD[j].y : 0.0);
+      if (!any[j]) continue;
+      st2(a.w, i, in0[j], in1[j], sa, sb);
+      double ua = 0.0, ub = 0.0;
+      if (a.apply_u) {
+        ua = U[j].x + a.alpha * (double)P0[j].x;
+        ub = U[j].y + a.alpha * (double)P0[j].y;
+        st2(a.u, i, in0[j], in1[j], ua, ub);
+      }
+      if (a.apply_p) st2(a.p_new, i, in0[j], in1[j], ca, cb);
+      // Halo ring of boundary-adjacent cells (rare; per cell).
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const bool inc = c == 0 ? in0[j] : in1[j];
+        const int ic = i + c, kc = ka + c;
+        const bool L = kc == xl, Rg = kc == xr, B = row == yb, T = row == yt;
+        if (!inc || !(L | Rg | B | T)) continue;
+        const float pv = c == 0 ? ca : cb;
+        const double uv = c == 0 ? ua : ub;
+        int ring[8];
+        int nr = 0;
+        if (L) ring[nr++] = ic - 1;
+        if (Rg) ring[nr++] = ic + 1;
+        if (B) ring[nr++] = ic - x;
+        if (T) ring[nr++] = ic + x;
+        if (L && B) ring[nr++] = ic - 1 - x;
+        if (Rg && B) ring[nr++] = ic + 1 - x;
+        if (L && T) ring[nr++] = ic - 1 + x;
+        if (Rg && T) ring[nr++] = ic + 1 + x;
+        for (int q = 0; q < nr; ++q) {
+          if (a.apply_u) a.u[ring[q]] = uv;
+          if (a.apply_p) a.p_new[ring[q]] = pv;
+        }
+      }
+    }
+  }
+  span_reduce_pairs_n<DOT2 ? 2 : 1>(psum, partials, nspan, span, single_group != 0);
+}
+
+template __global__ void mp_w2_kernel<true, 4>(Geo, MpArgs, double*, int, int);
+
+// r -= alpha w and r.r, two cells per lane.
+__global__ __launch_bounds__(kBlock) void mp_r2_kernel(
+    Geo g, float* __restrict__ r, const float* __restrict__ w, double alpha,
+    double* partials, int nspan, int single_group) {
+  const int span = xcd_span(blockIdx.x, gridDim.x);
+  const int base = span * kSpan;
+  float2 rv[kSlots], wv[kSlots];
+  bool in0[kSlots], in1[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const int e = base + k * (2 * kBlock) + 2 * threadIdx.x;
+    const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+    const int kk = e - row * g.x;
+    in0[k] = e < g.n && interior(g, kk, row);
+    in1[k] = e < g.n && interior(g, kk + 1, row);
+    const int i = (in0[k] || in1[k]) ? e : 0;
+    rv[k] = ld2(r, i);
+    wv[k] = ld2(w, i);
+  }
+  double psum[1][kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const float ra = (float)((double)rv[k].x - alpha * (double)wv[k].x);
+    const float rb = (float)((double)rv[k].y - alpha * (double)wv[k].y);
+    psum[0][k] = (in0[k] ? (double)ra * (double)ra : 0.0) +
+                 (in1[k] ? (double)rb * (double)rb : 0.0);
+    st2(r, base + k * (2 * kBlock) + 2 * threadIdx.x, in0[k], in1[k], ra, rb);
+  }
+  span_reduce_pairs_n<1>(psum, partials, nspan, span, single_group != 0);
+}
+
 namespace {
 
 // FP32 mirrors after the FP64 initialisation: coefficients and p on every
@@ -305,9 +489,14 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
            alpha, beta, apply_u ? 1 : 0, apply_p ? 1 : 0};
   {
     Timed t(c, K_MP_W);
-    hipLaunchKernelGGL((mp_w_kernel<4, false>), span_grid(c), dim3(kBlock), 0,
-                       c->stream, c->geo, a, c->partials, c->nspan,
-                       c->ngroup == 1 ? 1 : 0);
+    if (c->pairs)
+      hipLaunchKernelGGL((mp_w2_kernel<false, 4>), span_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, a, c->partials, c->nspan,
+                         c->ngroup == 1 ? 1 : 0);
+    else
+      hipLaunchKernelGGL((mp_w_kernel<4, false>), span_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, a, c->partials, c->nspan,
+                         c->ngroup == 1 ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
   if (apply_p) c->mp_cur ^= 1;
@@ -321,9 +510,14 @@ int tea_hip_mpcg_calc_r(tea_hip_chunk* c, double alpha, double* rr) {
   TEA_HIP_TRY(hipSetDevice(c->device));
   {
     Timed t(c, K_MP_R);
-    hipLaunchKernelGGL(mp_r_kernel, span_grid(c), dim3(kBlock), 0, c->stream,
-                       c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
-                       c->ngroup == 1 ? 1 : 0);
+    if (c->pairs)
+      hipLaunchKernelGGL(mp_r2_kernel, span_grid(c), dim3(kBlock), 0, c->stream,
+                         c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
+                         c->ngroup == 1 ? 1 : 0);
+    else
+      hipLaunchKernelGGL(mp_r_kernel, span_grid(c), dim3(kBlock), 0, c->stream,
+                         c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
+                         c->ngroup == 1 ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
   return finish_reduction(c, 1, rr);

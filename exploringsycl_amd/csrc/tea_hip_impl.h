@@ -167,6 +167,84 @@ __device__ inline void span_reduce(double (&v)[NR][kItems], double* partials,
   }
 }
 
+
+// ---- two cells per lane ---------------------------------------------------
+//
+// Kernels over cell pairs (2m, 2m+1), m = lane index within a 512-cell slot:
+// with x even every pair starts on an aligned boundary, so each operand is
+// one 8-byte (FP32) or 16-byte (FP64) load, and the left/right neighbours
+// come from the adjacent lanes (lane l-1 holds 2m-1, lane l+1 holds 2m+2);
+// only lanes 0 and 63 load theirs.  A wave covers 128 consecutive cells =
+// two groups of the reduction tree (lanes 0-31 and 32-63); a 256-thread
+// block covers a 4096-cell span in 8 slots of 512 cells.  The first tree
+// level is the in-lane pair sum, then the butterfly runs over each 32-lane
+// half.
+
+constexpr int kSlots = kSpan / (2 * kBlock);  // 8
+
+__device__ inline double2 ld2(const double* p, int i) {
+  return *reinterpret_cast<const double2*>(p + i);
+}
+
+__device__ inline float2 ld2(const float* p, int i) {
+  return *reinterpret_cast<const float2*>(p + i);
+}
+
+// Stores a cell pair with one vector store, or the interior member alone.
+template <class T>
+__device__ inline void st2(T* p, int i, bool in0, bool in1, T v0, T v1) {
+  if (in0 && in1) {
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+    *reinterpret_cast<vec2*>(p + i) = vec2{v0, v1};
+  } else {
+    if (in0) p[i] = v0;
+    if (in1) p[i + 1] = v1;
+  }
+}
+
+__device__ inline int bitrev3(int l) {
+  return ((l & 1) << 2) | (l & 2) | ((l & 4) >> 2);
+}
+
+// Level-1 trees of the 8 slot pair-sums of a lane, transposed across lanes
+// as in wave_tree16 (steps 1, 2, 4), then steps 8 and 16 within each 32-lane
+// half; stores each group's partial in lds[q][group-in-span] and finishes
+// the span's level-2 trees in wave 0 (partials[q * nspan + span]).
+template <int NR>
+__device__ inline void span_reduce_pairs_n(double (&v)[NR][kSlots], double* partials,
+                                           int nspan, int span, bool single_group) {
+  __shared__ double lds[NR][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+#pragma unroll
+    for (int step = 0; step < 3; ++step) {
+      const int m = 1 << step;
+      const int half = kSlots >> (step + 1);
+      const bool upper = (lane & m) != 0;
+#pragma unroll
+      for (int i = 0; i < half; ++i) {
+        const double send = upper ? v[q][i] : v[q][half + i];
+        const double keep = upper ? v[q][half + i] : v[q][i];
+        v[q][i] = keep + __shfl_xor(send, m);
+      }
+    }
+    double t = v[q][0];
+    t = t + __shfl_xor(t, 8);
+    t = t + __shfl_xor(t, 16);
+    if ((lane & 31) < kSlots)
+      lds[q][bitrev3(lane & 7) * 8 + wv * 2 + (lane >> 5)] = t;
+  }
+  __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const double r = single_group ? lds[q][0] : wave_tree(lds[q][lane]);
+      if (lane == 0) partials[q * nspan + span] = r;
+    }
+  }
+}
 // One block per span: each thread evaluates `body` on its 16 cells (only
 // interior cells; everything else contributes +0.0, as in the reference) and
 // the block stores the span's NR level-2 partials.
@@ -266,6 +344,9 @@ __global__ void mp_w_kernel(Geo g, MpArgs a, double* partials, int nspan,
                             int single_group);
 __global__ void mp_r_kernel(Geo g, float* r, const float* w, double alpha,
                             double* partials, int nspan, int single_group);
+template <bool DOT2, int SB>
+__global__ void mp_w2_kernel(Geo g, MpArgs a, double* partials, int nspan,
+                             int single_group);
 
 }  // namespace tea
 

@@ -43,4 +43,7 @@ void run_fcg_precondition(Chunk* chunk, Settings* settings, int use_net,
 void run_fcg_calc_w(Chunk* chunk, Settings* settings, double alpha,
                     int apply_u, double beta, double* pw, double* pr);
 
+/* Copies the chunk's u (x*y values, halo included) to host memory. */
+void run_read_u(Chunk* chunk, Settings* settings, double* host);
+
 #endif

@@ -1,7 +1,7 @@
 // tealeaf: the host program.  Usage:
 //   tealeaf [--in tea.in] [--out tea.out] [--problems tea.problems]
 //           [-s|--solver cg] [-x CELLS] [-y CELLS] [--json FILE]
-//           [--warmup-steps W] [--quiet]
+//           [--warmup-steps W] [--quiet] [--field-out FILE]
 // Reads the deck, sets up the chunk through the kernel backend it is linked
 // with, runs end_step timesteps and checks the final temperature against the
 // tea.problems table (TeaLeaf/main.c:9-99 for the reference's flow).
@@ -10,10 +10,13 @@
 #include <string.h>
 #include <time.h>
 
+#include <vector>
+
 #include "application.h"
 #include "comms.h"
 #include "drivers.h"
 #include "kernel_interface.h"
+#include "kernel_interface_ext.h"
 #include "tea_host.h"
 
 RunRecord g_run;
@@ -35,6 +38,7 @@ static void usage(Settings* settings) {
   printf("\t--json FILE: write the run record (iterations, summaries, timing)\n");
   printf("\t--warmup-steps W: run W untimed steps first, then reset and time\n");
   printf("\t--quiet: write the log only\n");
+  printf("\t--field-out FILE: write the final interior u as raw float64 (row-major)\n");
   (void)settings;
 }
 
@@ -70,6 +74,8 @@ static void parse_args(Settings* settings, int argc, char** argv) {
       g_opts.warmup_steps = atoi(argv[++aa]);
     } else if (!strcmp(a, "--quiet")) {
       g_opts.quiet = true;
+    } else if (!strcmp(a, "--field-out") && has_val) {
+      g_opts.field_out = argv[++aa];
     } else if (!strcmp(a, "-h") || !strcmp(a, "-help") || !strcmp(a, "--help")) {
       usage(settings);
       exit(0);
@@ -79,6 +85,8 @@ static void parse_args(Settings* settings, int argc, char** argv) {
     }
   }
 }
+
+static void write_field(const char* path, Chunk* chunks, Settings* settings);
 
 // Puts the chunk back into its initial state: states, depth-2 halos and
 // energy := energy0 (the tail of initialise_application).
@@ -124,6 +132,7 @@ int main(int argc, char** argv) {
 
   if (settings->rank == MASTER) PRINT_PROFILING_RESULTS(settings->kernel_profile);
   if (!g_opts.json_path.empty()) write_run_json(g_opts.json_path.c_str(), settings);
+  if (!g_opts.field_out.empty()) write_field(g_opts.field_out.c_str(), chunks, settings);
 
   kernel_finalise_driver(chunks, settings);
   for (int cc = 0; cc < settings->num_chunks_per_rank; ++cc) finalise_chunk(&chunks[cc]);
@@ -132,6 +141,19 @@ int main(int argc, char** argv) {
   if (settings->tea_out_fp) fclose(settings->tea_out_fp);
   finalise_comms();
   return g_run.check_found && !g_run.check_passed ? 3 : EXIT_SUCCESS;
+}
+
+// The interior of chunk 0's u, row by row, as raw float64.
+static void write_field(const char* path, Chunk* chunks, Settings* settings) {
+  Chunk* c = &chunks[0];
+  std::vector<double> u((size_t)c->x * c->y);
+  run_read_u(c, settings, u.data());
+  FILE* fp = fopen(path, "wb");
+  if (!fp) die(__LINE__, __FILE__, "Could not open %s\n", path);
+  const int h = settings->halo_depth;
+  for (int jj = h; jj < c->y - h; ++jj)
+    fwrite(&u[(size_t)jj * c->x + h], sizeof(double), (size_t)(c->x - 2 * h), fp);
+  fclose(fp);
 }
 
 void write_run_json(const char* path, Settings* settings) {

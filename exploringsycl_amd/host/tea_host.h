@@ -45,6 +45,7 @@ struct HostOptions {
   std::string json_path;   /* --json FILE: write g_run as JSON at the end */
   int warmup_steps = 0;    /* --warmup-steps W: run W steps, reset, then time */
   bool quiet = false;      /* --quiet: log to tea.out only */
+  std::string field_out;   /* --field-out FILE: final interior u, raw float64 */
 };
 
 extern HostOptions g_opts;

@@ -241,6 +241,11 @@ void run_mpcg_refresh(Chunk* c, Settings* s, double alpha, int apply_u,
 
 // ---- flexible CG (Jacobi preconditioner only) ----------------------------
 
+void run_read_u(Chunk* c, Settings* s, double* host) {
+  (void)s;
+  memcpy(host, c->u, (size_t)c->x * c->y * sizeof(double));
+}
+
 void run_fcg_attach(Chunk* c, Settings* s, const float* params, size_t count,
                     const float* norm, int coarsen) {
   (void)s; (void)count; (void)norm; (void)coarsen;

@@ -39,7 +39,7 @@ def _bf(v):
     return O.bf16(np.asarray(v, dtype=np.float32)).astype(np.float64)
 
 
-@pytest.mark.parametrize("n", [128, 256])
+@pytest.mark.parametrize("n", [32 * learned.COARSEN, 64 * learned.COARSEN])
 def test_learned_preconditioner_matches_numpy(n):
     """z = r/diag + s P(N(R r/s, f)) and z.w against restrict + the network
     oracle + prolong in numpy on the device's own FP32 r, kx, ky, w."""
@@ -70,10 +70,10 @@ def test_learned_preconditioner_matches_numpy(n):
     rbar = ri.reshape(n // cc, cc, n // cc, cc).sum(axis=(1, 3)) / (cc * cc)
     rbar = rbar.astype(np.float32).astype(np.float64)
     s = np.float32(np.sqrt((rbar * rbar).sum() / rbar.size))
-    lx = np.log(np.maximum(kx[H:-H, H:-H], 1e-30))
-    ly = np.log(np.maximum(ky[H:-H, H:-H], 1e-30))
-    f0 = (learned.block_mean(lx) - norm[0]) / norm[1]
-    f1 = (learned.block_mean(ly) - norm[2]) / norm[3]
+    kxc, kyc = learned.coarse_faces(kx[H:H + n, H:H + n + 1], ky[H:H + n + 1, H:H + n], cc)
+    g0, g1 = learned.coarse_features(kxc, kyc)
+    f0 = (g0 - norm[0]) / norm[1]
+    f1 = (g1 - norm[2]) / norm[3]
     x = np.stack([np.float32(rbar) * (np.float32(1) / s), f0, f1], axis=-1)[None]
     out, _ = O.forward(params, x.astype(np.float32), bf16_mode=True)
     out = out[0].astype(np.float64)
@@ -88,26 +88,27 @@ def test_learned_preconditioner_matches_numpy(n):
 
 
 def test_training_pairs_satisfy_the_operator(oracle):
-    """Pairs from the tiled FP64 CG: A e_k = r_k on the tile alone, with A
-    the tile's own operator (reflective edges) from the oracle."""
-    data = learned.generate(0, 3, snapshots=2, rng_seed=1, keep_fine=True)
+    """Pairs from the tiled FP64 CG on the coarse operators: T e_k = r_k on
+    the tile alone, T applied by the oracle's stencil with the tile's coarse
+    face coefficients (zero on its edges)."""
+    data = learned.generate(0, 3, snapshots=2, rng_seed=1, keep_coarse=True)
     n = data["inputs"].shape[0]
-    assert n >= 3 and data["targets"].shape == (n, 64, 64)
+    cw = learned.TILE // learned.COARSEN
+    assert n >= 3 and data["targets"].shape == (n, cw, cw)
     assert np.isfinite(data["inputs"]).all() and np.isfinite(data["targets"]).all()
-    r, e, fx, fy = data["fine_r"], data["fine_e"], data["fine_kx"], data["fine_ky"]
-    T = r.shape[0]
-    X = T + 2 * H
+    r, e = data["coarse_r"], data["coarse_e"]
+    X = cw + 2 * H
     kx = np.zeros((X, X))
     ky = np.zeros((X, X))
-    kx[H:H + T, H:H + T + 1] = fx
-    ky[H:H + T + 1, H:H + T] = fy
+    kx[H:H + cw, H:H + cw + 1] = data["coarse_kx"]
+    ky[H:H + cw + 1, H:H + cw] = data["coarse_ky"]
     p = np.zeros((X, X))
     p[H:-H, H:-H] = e
     f = dict(p=p.ravel().copy(), w=np.zeros(X * X), kx=kx.ravel(), ky=ky.ravel())
     oracle.halo_field(X, X, H, 1, f["p"])
     oracle.cg_calc_w(X, X, H, f)
-    Ae = f["w"].reshape(X, X)[H:-H, H:-H]
-    assert np.linalg.norm(Ae - r) <= 1e-6 * np.linalg.norm(r)
+    Te = f["w"].reshape(X, X)[H:-H, H:-H]
+    assert np.linalg.norm(Te - r) <= 1e-6 * np.linalg.norm(r)
 
 
 def _fcg_deck(n, seed, extra):

@@ -102,3 +102,45 @@ def test_initial_params_layout():
     # Biases are zero: the last parameter is b6, the 32 before W6 are b5.
     assert p[-1] == 0.0
     assert np.all(p[-1 - 288 - 32:-1 - 288] == 0.0)
+
+
+def test_coarse_faces_are_the_galerkin_operator():
+    """coarse_faces: I + the 5-point operator with its face coefficients is
+    P^T A P / c^2 for the piecewise-constant P, A the tile's operator with
+    reflective edges (zero edge faces)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(2)
+    n, c = 32, 8
+    dens = 10.0 ** rng.uniform(-2, 2, (n, n))
+    kx, ky = learned.face_coefficients(dens, 10.0 / 4096)
+    kx = kx.astype(np.float32).astype(np.float64)
+    ky = ky.astype(np.float32).astype(np.float64)
+    kx[:, [0, -1]] = 0.0
+    ky[[0, -1], :] = 0.0
+
+    def five_point(kx, ky):
+        m = ky.shape[1]
+        N = kx.shape[0] * m
+        A = sp.lil_matrix((N, N))
+        for j in range(kx.shape[0]):
+            for k in range(m):
+                i = j * m + k
+                A[i, i] = 1 + kx[j, k] + kx[j, k + 1] + ky[j, k] + ky[j + 1, k]
+                if k > 0:
+                    A[i, i - 1] = -kx[j, k]
+                if k < m - 1:
+                    A[i, i + 1] = -kx[j, k + 1]
+                if j > 0:
+                    A[i, i - m] = -ky[j, k]
+                if j < kx.shape[0] - 1:
+                    A[i, i + m] = -ky[j + 1, k]
+        return A.tocsr()
+
+    A = five_point(kx, ky)
+    rows = np.arange(n * n)
+    cols = (rows // n // c) * (n // c) + (rows % n) // c
+    P = sp.csr_matrix((np.ones(n * n), (rows, cols)), shape=(n * n, (n // c) ** 2))
+    want = (P.T @ A @ P).toarray() / (c * c)
+    kxc, kyc = learned.coarse_faces(kx, ky, c)
+    got = five_point(kxc, kyc).toarray()
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-12)
