@@ -7,13 +7,19 @@
 //
 // Preconditioners, z = M(r):
 //   Jacobi:  z = r / diag(A), diag = 1 + kx[i] + kx[i+1] + ky[i] + ky[i+x].
-//   Learned: z = r / diag(A) + P(s N(R r / s, f)), where R is the c x c
-//            block mean onto the coarse grid, s the RMS of R r (both
-//            computed on the device), N the 6-layer network on the coarse
-//            grid, P the piecewise-constant prolongation and f the coarse
-//            operator's face features (feature_kernel).  N approximates the
-//            inverse of T = P^T A P / c^2, so that with N = T^-1 this is the
-//            two-level additive preconditioner D^-1 + P (P^T A P)^-1 P^T.
+//   Learned: z = r / diag(A) + g P y, y = D^-1/2 N(D^-1/2 R r / s, f),
+//            where R is the c x c block mean onto the coarse grid, T =
+//            P^T A P / c^2 the Galerkin coarse operator, D its diagonal, s
+//            the RMS of D^-1/2 R r, N the 6-layer network on the coarse
+//            grid, P the piecewise-constant prolongation, f the couplings
+//            of D^-1/2 T D^-1/2 (feature_kernel) and g = (y.Rr) / (y.T y)
+//            the energy-optimal factor of the coarse direction (all
+//            computed on the device).  N is trained to do what a few CG
+//            steps on T do, so that the coarse term is a local
+//            approximation of the two-level additive preconditioner
+//            D^-1 + P (P^T A P)^-1 P^T.  With g,
+//            z.r = r.D^-1 r + c^2 (y.Rr)^2 / (y.T y) > 0 for any y: the
+//            network cannot make the flexible CG step degenerate.
 // With c = 8 the network runs on 1/64 of the cells; its 13-cell receptive
 // field spans 104 fine cells.
 
@@ -47,10 +53,11 @@ struct JacobiBody {
   }
 };
 
-// Coarse block means of r and per-block partial sums of their squares.
+// Coarse block means of r and per-block partial sums of (D^-1/2 rbar)^2.
 __global__ __launch_bounds__(256) void restrict_kernel(
     Geo g, int c, int hc, int wc, const float* __restrict__ r,
-    float* __restrict__ rbar, double* __restrict__ part) {
+    const float* __restrict__ dis, float* __restrict__ rbar,
+    double* __restrict__ part) {
   __shared__ double s[256];
   const int idx = blockIdx.x * 256 + threadIdx.x;
   double v2 = 0.0;
@@ -63,7 +70,8 @@ __global__ __launch_bounds__(256) void restrict_kernel(
     }
     const float m = (float)(sum / (double)(c * c));
     rbar[idx] = m;
-    v2 = (double)m * (double)m;
+    const double ms = (double)m * (double)dis[idx];
+    v2 = ms * ms;
   }
   s[threadIdx.x] = v2;
   __syncthreads();
@@ -74,7 +82,7 @@ __global__ __launch_bounds__(256) void restrict_kernel(
   if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-// s = RMS of the coarse residual (1 if it is zero).
+// s = RMS of D^-1/2 rbar (1 if it is zero).
 __global__ __launch_bounds__(256) void scale_kernel(const double* __restrict__ part,
                                                     int nblk, long count,
                                                     float* __restrict__ scale) {
@@ -93,9 +101,10 @@ __global__ __launch_bounds__(256) void scale_kernel(const double* __restrict__ p
   }
 }
 
-// Network input: channel 0 = rbar / s, channels 1, 2 = features.
+// Network input: channel 0 = D^-1/2 rbar / s, channels 1, 2 = features.
 __global__ __launch_bounds__(256) void pack_kernel(int hc, int wc,
                                                    const float* __restrict__ rbar,
+                                                   const float* __restrict__ dis,
                                                    const float* __restrict__ feat,
                                                    const float* __restrict__ scale,
                                                    bf16* __restrict__ in0) {
@@ -104,7 +113,7 @@ __global__ __launch_bounds__(256) void pack_kernel(int hc, int wc,
   const int X = idx % wc, Y = idx / wc;
   const float inv = 1.0f / scale[0];
   bf16x8 o = bf16x8{};
-  o[0] = (bf16)(rbar[idx] * inv);
+  o[0] = (bf16)((rbar[idx] * dis[idx]) * inv);
   o[1] = (bf16)feat[idx];
   o[2] = (bf16)feat[hc * wc + idx];
   *reinterpret_cast<bf16x8*>(in0 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * 8) = o;
@@ -178,16 +187,14 @@ __global__ __launch_bounds__(kBlock) void fcg_z2_kernel(
   span_reduce_pairs_n<1>(psum, partials, nspan, span, single_group != 0);
 }
 
-// Coarse features: for each coarse cell, log(1 + k) of its left and bottom
-// faces in the coarse operator T = P^T A P / c^2, where k is the sum of the
-// c fine face coefficients along that face over c^2 (zero on the domain
-// edge, the reflective boundary), normalised by (m, s).
-__global__ __launch_bounds__(256) void feature_kernel(Geo g, int c, int hc, int wc,
-                                                      const float* __restrict__ kx,
-                                                      const float* __restrict__ ky,
-                                                      float m0, float s0, float m1,
-                                                      float s1,
-                                                      float* __restrict__ feat) {
+// The coarse operator T = P^T A P / c^2: for each coarse cell, the face
+// coefficients of its left and bottom faces (kc[0][..], kc[1][..]): the sum
+// of the c fine face coefficients along the face over c^2, zero on the
+// domain edge (the reflective boundary).
+__global__ __launch_bounds__(256) void coarse_coef_kernel(Geo g, int c, int hc, int wc,
+                                                          const float* __restrict__ kx,
+                                                          const float* __restrict__ ky,
+                                                          float* __restrict__ kc) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= hc * wc) return;
   const int X = idx % wc, Y = idx / wc;
@@ -198,10 +205,117 @@ __global__ __launch_bounds__(256) void feature_kernel(Geo g, int c, int hc, int 
     sy += (double)ky[corner + b];
   }
   const double cc = (double)(c * c);
-  const double kxc = X == 0 ? 0.0 : sx / cc;
-  const double kyc = Y == 0 ? 0.0 : sy / cc;
-  feat[idx] = (float)((log1p(kxc) - m0) / s0);
-  feat[hc * wc + idx] = (float)((log1p(kyc) - m1) / s1);
+  kc[idx] = X == 0 ? 0.0f : (float)(sx / cc);
+  kc[hc * wc + idx] = Y == 0 ? 0.0f : (float)(sy / cc);
+}
+
+// Diagonal of T at coarse cell (X, Y): 1 + its four face coefficients.
+__device__ inline double coarse_diag(const float* kc, int hc, int wc, int X, int Y) {
+  const int idx = Y * wc + X;
+  const float* kxc = kc;
+  const float* kyc = kc + hc * wc;
+  const double kr = X + 1 < wc ? (double)kxc[idx + 1] : 0.0;
+  const double kt = Y + 1 < hc ? (double)kyc[idx + wc] : 0.0;
+  return (1.0 + ((double)kxc[idx] + kr)) + ((double)kyc[idx] + kt);
+}
+
+// Network features: the couplings of each coarse cell to its left and
+// bottom neighbours in D^-1/2 T D^-1/2, k / sqrt(D_i D_j) (zero on the
+// domain edge), normalised by (m, s); also dis = D^-1/2 per coarse cell.
+__global__ __launch_bounds__(256) void feature_kernel(int hc, int wc,
+                                                      const float* __restrict__ kc,
+                                                      float m0, float s0, float m1,
+                                                      float s1,
+                                                      float* __restrict__ feat,
+                                                      float* __restrict__ dis) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= hc * wc) return;
+  const int X = idx % wc, Y = idx / wc;
+  const double di = 1.0 / sqrt(coarse_diag(kc, hc, wc, X, Y));
+  double left = 0.0, bottom = 0.0;
+  if (X > 0) left = (double)kc[idx] * di / sqrt(coarse_diag(kc, hc, wc, X - 1, Y));
+  if (Y > 0) bottom = (double)kc[hc * wc + idx] * di / sqrt(coarse_diag(kc, hc, wc, X, Y - 1));
+  feat[idx] = (float)((left - m0) / s0);
+  feat[hc * wc + idx] = (float)((bottom - m1) / s1);
+  dis[idx] = (float)di;
+}
+
+// The network's output in the scaled variables back to T's: y = D^-1/2 y.
+__global__ __launch_bounds__(256) void unscale_kernel(int cn, const float* __restrict__ dis,
+                                                      float* __restrict__ y) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx < cn) y[idx] *= dis[idx];
+}
+
+// Per-block partial sums over coarse cells of y.rbar and y.(T y), with T
+// the coarse operator (identity plus the 5-point operator of the face
+// coefficients kc, zero beyond the domain edge).
+__global__ __launch_bounds__(256) void coarse_dots_kernel(int hc, int wc,
+                                                          const float* __restrict__ y,
+                                                          const float* __restrict__ rbar,
+                                                          const float* __restrict__ kc,
+                                                          double* __restrict__ part) {
+  __shared__ double s0[256], s1[256];
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  double a = 0.0, b = 0.0;
+  if (idx < hc * wc) {
+    const int X = idx % wc, Y = idx / wc;
+    const float* kxc = kc;
+    const float* kyc = kc + hc * wc;
+    const double yc = y[idx];
+    const double kl = kxc[idx], kb = kyc[idx];
+    const double kr = X + 1 < wc ? (double)kxc[idx + 1] : 0.0;
+    const double kt = Y + 1 < hc ? (double)kyc[idx + wc] : 0.0;
+    double ty = (1.0 + kl + kr + kb + kt) * yc;
+    if (X > 0) ty -= kl * (double)y[idx - 1];
+    if (X + 1 < wc) ty -= kr * (double)y[idx + 1];
+    if (Y > 0) ty -= kb * (double)y[idx - wc];
+    if (Y + 1 < hc) ty -= kt * (double)y[idx + wc];
+    a = yc * (double)rbar[idx];
+    b = yc * ty;
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s0[threadIdx.x] += s0[threadIdx.x + st];
+      s1[threadIdx.x] += s1[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s0[0];
+    part[gridDim.x + blockIdx.x] = s1[0];
+  }
+}
+
+// The A-optimal factor of the coarse correction P y: (y.rbar) / (y.T y)
+// (the minimiser of the energy norm of e - g P y, since (P y)^T r =
+// c^2 y.rbar and (P y)^T A (P y) = c^2 y.T y); 0 if y.T y is not positive.
+// Written to scale[1].
+__global__ __launch_bounds__(256) void gamma_kernel(const double* __restrict__ part,
+                                                    int nblk, float* __restrict__ scale) {
+  __shared__ double s0[256], s1[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) {
+    a += part[i];
+    b += part[nblk + i];
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s0[threadIdx.x] += s0[threadIdx.x + st];
+      s1[threadIdx.x] += s1[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double g = s0[0] / s1[0];
+    scale[1] = (s1[0] > 0.0 && isfinite(g)) ? (float)g : 0.0f;
+  }
 }
 
 int ensure_fcg_buffers(tea_hip_chunk* c) {
@@ -219,6 +333,69 @@ int ensure_fcg_buffers(tea_hip_chunk* c) {
 }  // namespace tea
 
 using namespace tea;
+
+// z = M(r) and the partials of z.w into `partials` (not waited for).
+static int launch_precondition(tea_hip_chunk* c, int use_net, double* partials) {
+  const Geo g = c->geo;
+  const int sg = c->ngroup == 1 ? 1 : 0;
+  if (!use_net) {
+    Timed t(c, K_FCG_JACOBI);
+    if (c->pairs) {
+      hipLaunchKernelGGL((fcg_z2_kernel<false>), span_grid(c), dim3(kBlock), 0,
+                         c->stream, g, c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, nullptr,
+                         nullptr, c->fcg_z, 0, 0, partials, c->nspan, sg);
+    } else {
+      JacobiBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, c->fcg_z, g.x};
+      hipLaunchKernelGGL((span_kernel<1, JacobiBody>), span_grid(c), dim3(kBlock), 0,
+                         c->stream, g, body, partials, c->nspan, sg);
+    }
+  } else {
+    const int cn = c->fcg_hc * c->fcg_wc;
+    const int nblk = (cn + 255) / 256;
+    {
+      Timed t(c, K_FCG_RESTRICT);
+      hipLaunchKernelGGL(restrict_kernel, dim3(nblk), dim3(256), 0, c->stream, g,
+                         c->fcg_c, c->fcg_hc, c->fcg_wc, c->mp_r, c->fcg_dis,
+                         c->fcg_rbar, c->fcg_part);
+      hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(256), 0, c->stream, c->fcg_part,
+                         nblk, (long)cn, c->fcg_scale);
+      hipLaunchKernelGGL(pack_kernel, dim3(nblk), dim3(256), 0, c->stream, c->fcg_hc,
+                         c->fcg_wc, c->fcg_rbar, c->fcg_dis, c->fcg_feat, c->fcg_scale,
+                         nn::input_tensor(c->fcg_net));
+    }
+    {
+      Timed t(c, K_FCG_NET);
+      TEA_RETURN_IF(nn::forward_on_stream(c->fcg_net, c->stream));
+      hipLaunchKernelGGL(unscale_kernel, dim3(nblk), dim3(256), 0, c->stream, cn, c->fcg_dis,
+                         nn::output_mutable(c->fcg_net));
+      hipLaunchKernelGGL(coarse_dots_kernel, dim3(nblk), dim3(256), 0, c->stream,
+                         c->fcg_hc, c->fcg_wc, nn::output(c->fcg_net), c->fcg_rbar,
+                         c->fcg_kc, c->fcg_part);
+      hipLaunchKernelGGL(gamma_kernel, dim3(1), dim3(256), 0, c->stream, c->fcg_part,
+                         nblk, c->fcg_scale);
+    }
+    {
+      Timed t(c, K_FCG_PROLONG);
+      if (c->pairs) {
+        int shift = 0;
+        while ((1 << shift) < c->fcg_c) ++shift;
+        hipLaunchKernelGGL((fcg_z2_kernel<true>), span_grid(c), dim3(kBlock), 0,
+                           c->stream, g, c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
+                           nn::output(c->fcg_net), c->fcg_scale + 1, c->fcg_z, shift,
+                           c->fcg_wc, partials, c->nspan, sg);
+      } else {
+        ProlongBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
+                         nn::output(c->fcg_net), c->fcg_scale + 1, c->fcg_z,
+                         g.x, g.h, c->fcg_c, c->fcg_wc};
+        hipLaunchKernelGGL((span_kernel<1, ProlongBody>), span_grid(c), dim3(kBlock), 0,
+                           c->stream, g, body, partials, c->nspan, sg);
+      }
+    }
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
 
 extern "C" {
 
@@ -253,12 +430,16 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
   if (c->fcg_rbar) (void)hipFree(c->fcg_rbar);
   if (c->fcg_part) (void)hipFree(c->fcg_part);
   if (c->fcg_scale) (void)hipFree(c->fcg_scale);
-  c->fcg_feat = c->fcg_rbar = c->fcg_scale = nullptr;
+  if (c->fcg_kc) (void)hipFree(c->fcg_kc);
+  if (c->fcg_dis) (void)hipFree(c->fcg_dis);
+  c->fcg_feat = c->fcg_rbar = c->fcg_scale = c->fcg_kc = c->fcg_dis = nullptr;
   c->fcg_part = nullptr;
   if (hipMalloc(&c->fcg_feat, 2 * cn * sizeof(float)) != hipSuccess ||
+      hipMalloc(&c->fcg_kc, 2 * cn * sizeof(float)) != hipSuccess ||
+      hipMalloc(&c->fcg_dis, cn * sizeof(float)) != hipSuccess ||
       hipMalloc(&c->fcg_rbar, cn * sizeof(float)) != hipSuccess ||
-      hipMalloc(&c->fcg_part, nblk * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->fcg_scale, sizeof(float)) != hipSuccess) {
+      hipMalloc(&c->fcg_part, 2 * nblk * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->fcg_scale, 2 * sizeof(float)) != hipSuccess) {
     set_error("hipMalloc of the preconditioner's coarse buffers failed");
     return TEA_HIP_ENOMEM;
   }
@@ -269,10 +450,12 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
 static int fcg_features(tea_hip_chunk* c) {
   if (!c->fcg_net) return TEA_HIP_OK;
   const int cn = c->fcg_hc * c->fcg_wc;
-  hipLaunchKernelGGL(feature_kernel, dim3((cn + 255) / 256), dim3(256), 0, c->stream,
-                     c->geo, c->fcg_c, c->fcg_hc, c->fcg_wc, c->mp_kx, c->mp_ky,
-                     c->fcg_norm[0], c->fcg_norm[1], c->fcg_norm[2], c->fcg_norm[3],
-                     c->fcg_feat);
+  const dim3 grid((cn + 255) / 256);
+  hipLaunchKernelGGL(coarse_coef_kernel, grid, dim3(256), 0, c->stream, c->geo, c->fcg_c,
+                     c->fcg_hc, c->fcg_wc, c->mp_kx, c->mp_ky, c->fcg_kc);
+  hipLaunchKernelGGL(feature_kernel, grid, dim3(256), 0, c->stream, c->fcg_hc, c->fcg_wc,
+                     c->fcg_kc, c->fcg_norm[0], c->fcg_norm[1], c->fcg_norm[2],
+                     c->fcg_norm[3], c->fcg_feat, c->fcg_dis);
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
 }
@@ -291,57 +474,24 @@ int tea_hip_fcg_precondition(tea_hip_chunk* c, int use_net, double* zw) {
   TEA_CHECK_ARG(zw != nullptr, "null zw");
   TEA_CHECK_ARG(!use_net || c->fcg_net != nullptr, "no network attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  const Geo g = c->geo;
-  const int sg = c->ngroup == 1 ? 1 : 0;
-  if (!use_net) {
-    Timed t(c, K_FCG_JACOBI);
-    if (c->pairs) {
-      hipLaunchKernelGGL((fcg_z2_kernel<false>), span_grid(c), dim3(kBlock), 0,
-                         c->stream, g, c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, nullptr,
-                         nullptr, c->fcg_z, 0, 0, c->partials, c->nspan, sg);
-    } else {
-      JacobiBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, c->fcg_z, g.x};
-      hipLaunchKernelGGL((span_kernel<1, JacobiBody>), span_grid(c), dim3(kBlock), 0,
-                         c->stream, g, body, c->partials, c->nspan, sg);
-    }
-  } else {
-    const int cn = c->fcg_hc * c->fcg_wc;
-    const int nblk = (cn + 255) / 256;
-    {
-      Timed t(c, K_FCG_RESTRICT);
-      hipLaunchKernelGGL(restrict_kernel, dim3(nblk), dim3(256), 0, c->stream, g,
-                         c->fcg_c, c->fcg_hc, c->fcg_wc, c->mp_r, c->fcg_rbar,
-                         c->fcg_part);
-      hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(256), 0, c->stream, c->fcg_part,
-                         nblk, (long)cn, c->fcg_scale);
-      hipLaunchKernelGGL(pack_kernel, dim3(nblk), dim3(256), 0, c->stream, c->fcg_hc,
-                         c->fcg_wc, c->fcg_rbar, c->fcg_feat, c->fcg_scale,
-                         nn::input_tensor(c->fcg_net));
-    }
-    {
-      Timed t(c, K_FCG_NET);
-      TEA_RETURN_IF(nn::forward_on_stream(c->fcg_net, c->stream));
-    }
-    {
-      Timed t(c, K_FCG_PROLONG);
-      if (c->pairs) {
-        int shift = 0;
-        while ((1 << shift) < c->fcg_c) ++shift;
-        hipLaunchKernelGGL((fcg_z2_kernel<true>), span_grid(c), dim3(kBlock), 0,
-                           c->stream, g, c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
-                           nn::output(c->fcg_net), c->fcg_scale, c->fcg_z, shift,
-                           c->fcg_wc, c->partials, c->nspan, sg);
-      } else {
-        ProlongBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
-                         nn::output(c->fcg_net), c->fcg_scale, c->fcg_z,
-                         g.x, g.h, c->fcg_c, c->fcg_wc};
-        hipLaunchKernelGGL((span_kernel<1, ProlongBody>), span_grid(c), dim3(kBlock), 0,
-                           c->stream, g, body, c->partials, c->nspan, sg);
-      }
-    }
-  }
-  TEA_HIP_TRY(hipGetLastError());
+  TEA_RETURN_IF(launch_precondition(c, use_net, c->partials));
   return finish_reduction(c, 1, zw);
+}
+
+int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
+                                    double* rr, double* zw) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(c->fcg_z != nullptr && c->mp_r != nullptr, "FCG not initialised");
+  TEA_CHECK_ARG(rr != nullptr && zw != nullptr, "null outputs");
+  TEA_CHECK_ARG(!use_net || c->fcg_net != nullptr, "no network attached");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_RETURN_IF(launch_mp_r(c, alpha, c->partials));
+  TEA_RETURN_IF(launch_precondition(c, use_net, c->partials + c->nspan));
+  double out[2];
+  TEA_RETURN_IF(finish_reduction(c, 2, out));
+  *rr = out[0];
+  *zw = out[1];
+  return TEA_HIP_OK;
 }
 
 int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,

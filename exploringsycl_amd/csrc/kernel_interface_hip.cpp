@@ -341,6 +341,16 @@ void run_fcg_precondition(Chunk* chunk, Settings* settings, int use_net,
   *zw += v;
 }
 
+void run_fcg_calc_r_precondition(Chunk* chunk, Settings* settings,
+                                 double alpha, int use_net, double* rr,
+                                 double* zw) {
+  (void)settings;
+  double a, b;
+  HIP_CALL(tea_hip_fcg_calc_r_precondition(H(chunk), alpha, use_net, &a, &b));
+  *rr += a;
+  *zw += b;
+}
+
 void run_fcg_calc_w(Chunk* chunk, Settings* settings, double alpha,
                     int apply_u, double beta, double* pw, double* pr) {
   (void)settings;

@@ -436,6 +436,22 @@ int ensure_mp_buffers(tea_hip_chunk* c) {
 }
 
 }  // namespace
+
+// r -= alpha w and the partials of r.r into `partials` (not waited for).
+int launch_mp_r(tea_hip_chunk* c, double alpha, double* partials) {
+  Timed t(c, K_MP_R);
+  if (c->pairs)
+    hipLaunchKernelGGL(mp_r2_kernel, span_grid(c), dim3(kBlock), 0, c->stream, c->geo,
+                       c->mp_r, c->mp_w, alpha, partials, c->nspan,
+                       c->ngroup == 1 ? 1 : 0);
+  else
+    hipLaunchKernelGGL(mp_r_kernel, span_grid(c), dim3(kBlock), 0, c->stream, c->geo,
+                       c->mp_r, c->mp_w, alpha, partials, c->nspan,
+                       c->ngroup == 1 ? 1 : 0);
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
 }  // namespace tea
 
 using namespace tea;
@@ -508,18 +524,7 @@ int tea_hip_mpcg_calc_r(tea_hip_chunk* c, double alpha, double* rr) {
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
   TEA_CHECK_ARG(rr != nullptr, "null rr");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  {
-    Timed t(c, K_MP_R);
-    if (c->pairs)
-      hipLaunchKernelGGL(mp_r2_kernel, span_grid(c), dim3(kBlock), 0, c->stream,
-                         c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
-                         c->ngroup == 1 ? 1 : 0);
-    else
-      hipLaunchKernelGGL(mp_r_kernel, span_grid(c), dim3(kBlock), 0, c->stream,
-                         c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
-                         c->ngroup == 1 ? 1 : 0);
-  }
-  TEA_HIP_TRY(hipGetLastError());
+  TEA_RETURN_IF(launch_mp_r(c, alpha, c->partials));
   return finish_reduction(c, 1, rr);
 }
 

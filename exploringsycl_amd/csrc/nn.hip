@@ -74,15 +74,22 @@ constexpr int nsteps() {
 }
 
 // MODE 0: out = relu(D + bias).  MODE 1: out = D * (mask > 0).
-// Grid (H, B), 256 threads; each wave walks 32-pixel tiles of row y.
+// One 32-pixel tile per wave: wave g of the launch (4 per 256-thread block)
+// takes tile g of the B x H x (W/32) tiles, so that even one sample of a
+// wide grid gives thousands of waves.
 template <int CIN, int MODE>
 __global__ __launch_bounds__(256) void conv_mfma_kernel(
     const bf16* __restrict__ in, const bf16* __restrict__ wt,
     const float* __restrict__ bias, const bf16* __restrict__ mask,
-    bf16* __restrict__ out, int H, int W) {
+    bf16* __restrict__ out, int H, int W, int B) {
   constexpr int NS = nsteps<CIN>();
-  const int y = blockIdx.x, b = blockIdx.y;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tiles_row = W / 32;
+  const long gid = (long)blockIdx.x * 4 + wv;
+  if (gid >= (long)B * H * tiles_row) return;
+  const int tile = (int)(gid % tiles_row);
+  const int y = (int)((gid / tiles_row) % H);
+  const int b = (int)(gid / ((long)tiles_row * H));
   const int hf = lane >> 5, col = lane & 31;
   const int Wp = W + 2;
   const size_t plane = (size_t)(H + 2) * Wp;
@@ -102,7 +109,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
     for (int v = 0; v < 16; ++v) bco[v] = bias[8 * (v >> 2) + 4 * hf + (v & 3)];
   }
   const bf16* inb = in + (size_t)b * plane * CIN;
-  for (int tile = wv; tile < W / 32; tile += 4) {
+  {
     const int xo = tile * 32 + col;
     bf16x8 Bv[NS];
 #pragma unroll
@@ -514,7 +521,10 @@ int pack_params(tea_nn* n) {
   return TEA_HIP_OK;
 }
 
-dim3 rows_grid(const tea_nn* n, int batch) { return dim3(n->H, batch); }
+dim3 rows_grid(const tea_nn* n, int batch) {
+  const long tiles = (long)batch * n->H * (n->W / 32);
+  return dim3((unsigned)((tiles + 3) / 4));
+}
 unsigned flat_blocks(long count) { return (unsigned)((count + 255) / 256); }
 
 // Forward pass over `batch` samples whose packed input is in n->in0, on
@@ -522,11 +532,11 @@ unsigned flat_blocks(long count) { return (unsigned)((count + 255) / 256); }
 int forward_on(tea_nn* n, int batch, hipStream_t st) {
   const int H = n->H, W = n->W;
   hipLaunchKernelGGL((conv_mfma_kernel<kC0, 0>), rows_grid(n, batch), dim3(256), 0,
-                     st, n->in0, n->wf1, n->bias, nullptr, n->act[0], H, W);
+                     st, n->in0, n->wf1, n->bias, nullptr, n->act[0], H, W, batch);
   for (int l = 1; l < 5; ++l)
     hipLaunchKernelGGL((conv_mfma_kernel<kC, 0>), rows_grid(n, batch), dim3(256), 0,
                        st, n->act[l - 1], n->wfh + (size_t)(l - 1) * kWh,
-                       n->bias + l * kC, nullptr, n->act[l], H, W);
+                       n->bias + l * kC, nullptr, n->act[l], H, W, batch);
   hipLaunchKernelGGL(out_layer_kernel, dim3(flat_blocks((long)batch * H * W)), dim3(256),
                      0, st, n->act[4], n->w6, n->bias, n->out, batch, H, W);
   TEA_HIP_TRY(hipGetLastError());
@@ -584,7 +594,7 @@ int loss_and_grad(tea_nn* n, int batch, float* loss) {
     TEA_RETURN_IF(wgrad_layer(n, batch, l, n->del[l], n->act[l - 1]));
     hipLaunchKernelGGL((conv_mfma_kernel<kC, 1>), rows_grid(n, batch), dim3(256), 0,
                        n->stream, n->del[l], n->wbh + (size_t)(l - 1) * kWh, nullptr,
-                       n->act[l - 1], n->del[l - 1], H, W);
+                       n->act[l - 1], n->del[l - 1], H, W, batch);
   }
   TEA_RETURN_IF(wgrad_layer(n, batch, 0, n->del[0], n->in0));
   TEA_HIP_TRY(hipGetLastError());
@@ -648,6 +658,7 @@ namespace nn {
 int forward_on_stream(tea_nn* n, hipStream_t st) { return forward_on(n, 1, st); }
 bf16* input_tensor(tea_nn* n) { return n->in0; }
 const float* output(tea_nn* n) { return n->out; }
+float* output_mutable(tea_nn* n) { return n->out; }
 int sync(tea_nn* n) {
   TEA_HIP_TRY(hipStreamSynchronize(n->stream));
   return TEA_HIP_OK;

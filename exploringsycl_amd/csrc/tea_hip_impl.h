@@ -358,6 +358,7 @@ typedef __bf16 bf16;
 int forward_on_stream(tea_nn* n, hipStream_t st);
 bf16* input_tensor(tea_nn* n);
 const float* output(tea_nn* n);
+float* output_mutable(tea_nn* n);
 }  // namespace nn
 }  // namespace tea
 
@@ -409,8 +410,10 @@ struct tea_hip_chunk {
   float fcg_norm[4] = {0.0f, 1.0f, 0.0f, 1.0f};
   float* fcg_feat = nullptr;   // [2][hc][wc]
   float* fcg_rbar = nullptr;   // [hc][wc]
-  double* fcg_part = nullptr;  // per-block sums of rbar^2
-  float* fcg_scale = nullptr;  // device scalar s
+  float* fcg_kc = nullptr;     // [2][hc][wc] coarse left / bottom face coefficients
+  float* fcg_dis = nullptr;    // [hc][wc] D^-1/2 of the coarse operator
+  double* fcg_part = nullptr;  // per-block partial sums (2 * blocks)
+  float* fcg_scale = nullptr;  // {s, prolongation factor}
 
   // Timing.
   bool timing = false;
@@ -448,6 +451,9 @@ int finish_reduction(tea_hip_chunk* c, int nr, double* out);
 
 // Applies any deferred CG work (fused mode).
 int apply_pending(tea_hip_chunk* c);
+// Mixed-precision residual update r -= alpha w with the partials of r.r
+// written to `partials` (mpcg.hip); the caller finishes the reduction.
+int launch_mp_r(tea_hip_chunk* c, double alpha, double* partials);
 
 // One reflective-halo launch (all flagged fields and external faces).
 int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],

@@ -4,6 +4,7 @@
 // loops over the rank's chunks and calls kernel_interface.h.
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 
 #include "comms.h"
@@ -270,9 +271,10 @@ void mpcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
 // the explicit orthogonalisation against p_old tolerates.  The FP64 true
 // residual is refreshed as in mpcg_driver (refresh_interval, and at the
 // tolerance), and the solve stops when |b - A u| / |b| < rel_eps.  With the
-// network, every stall_window iterations |r| must have fallen by
-// stall_factor; otherwise the solve continues as Jacobi-PCG from a restart
-// (beta = 0).
+// network, the smallest |r| of every stall_window iterations must be
+// stall_factor below the smallest of the window before (the initial |r| for
+// the first); otherwise the solve continues as Jacobi-PCG from a restart
+// (beta = 0).  Window minima, because |r| of CG-type methods oscillates.
 void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
                 double* error) {
   if (settings->num_chunks_per_rank != 1)
@@ -309,7 +311,7 @@ void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
   double zw = 0.0;
   run_fcg_precondition(c, settings, use_net, &zw);
 
-  double alpha = 0.0, beta = 0.0, rr_true = rr, rr_mark = rr;
+  double alpha = 0.0, beta = 0.0, rr_true = rr, rr_mark = rr, rr_win = rr;
   bool pend_u = false, converged = false;
   int it = 0, since = 0, refreshes = 0, fallback_at = -1, mark = 0;
   for (it = 0; it < settings->max_iters; ++it) {
@@ -317,10 +319,14 @@ void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
     run_fcg_calc_w(c, settings, alpha, pend_u, beta, &pw, &pr);
     alpha = pr / pw;
     c->cg_alphas[it] = alpha;
+    // r -= alpha w and z = M(r) in one round trip; z is recomputed below
+    // in the rare iterations where r is replaced or M changes.
     double rrn = 0.0;
-    run_mpcg_calc_r(c, settings, alpha, &rrn);
+    zw = 0.0;
+    run_fcg_calc_r_precondition(c, settings, alpha, use_net, &rrn, &zw);
     pend_u = true;
     ++since;
+    bool stale_z = false;
     if (since >= refresh || rrn < tol2) {
       double rt = 0.0;
       rrn = 0.0;
@@ -329,6 +335,7 @@ void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
       since = 0;
       ++refreshes;
       rr_true = rt;
+      stale_z = true;
       if (rt < tol2) {
         converged = true;
         ++it;
@@ -336,19 +343,24 @@ void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
       }
     }
     bool restart = false;
+    rr_win = std::min(rr_win, rrn);
     if (use_net && it + 1 - mark >= window) {
-      if (rrn * need > rr_mark) {
+      if (rr_win * need > rr_mark) {
         use_net = false;
         restart = true;
+        stale_z = true;
         fallback_at = it + 1;
         print_and_log(settings, "FCG: \t\t\tstagnation after %d iterations, "
                       "continuing as Jacobi-PCG\n", it + 1);
       }
-      rr_mark = rrn;
+      rr_mark = rr_win;
+      rr_win = rrn;
       mark = it + 1;
     }
-    zw = 0.0;
-    run_fcg_precondition(c, settings, use_net, &zw);
+    if (stale_z) {
+      zw = 0.0;
+      run_fcg_precondition(c, settings, use_net, &zw);
+    }
     beta = restart ? 0.0 : -zw / pw;
     c->cg_betas[it] = beta;
     rr = rrn;

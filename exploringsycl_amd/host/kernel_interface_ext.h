@@ -32,6 +32,8 @@ void run_mpcg_refresh(Chunk* chunk, Settings* settings, double alpha,
  *     z.w.
  *   run_fcg_calc_w: u += alpha p_old if apply_u; p = z + beta p_old; w = A p;
  *     pw += p.w, pr += p.r.
+ *   run_fcg_calc_r_precondition: r -= alpha w, rr += r.r, then z = M(r),
+ *     zw += z.w, with one wait for both results.
  * The residual update and the FP64 refresh are run_mpcg_calc_r and
  * run_mpcg_refresh. */
 void run_fcg_attach(Chunk* chunk, Settings* settings, const float* params,
@@ -42,6 +44,9 @@ void run_fcg_precondition(Chunk* chunk, Settings* settings, int use_net,
                           double* zw);
 void run_fcg_calc_w(Chunk* chunk, Settings* settings, double alpha,
                     int apply_u, double beta, double* pw, double* pr);
+void run_fcg_calc_r_precondition(Chunk* chunk, Settings* settings,
+                                 double alpha, int use_net, double* rr,
+                                 double* zw);
 
 /* Copies the chunk's u (x*y values, halo included) to host memory. */
 void run_read_u(Chunk* chunk, Settings* settings, double* host);

@@ -65,6 +65,7 @@ _SIGNATURES = {
     "tea_hip_fcg_attach": [_vp, _fp, ctypes.c_size_t, _fp, _i],
     "tea_hip_fcg_init": [_vp, _i, _d, _d, _dp, _dp],
     "tea_hip_fcg_precondition": [_vp, _i, _dp],
+    "tea_hip_fcg_calc_r_precondition": [_vp, _d, _i, _dp, _dp],
     "tea_hip_fcg_calc_w": [_vp, _d, _i, _d, _dp, _dp],
     "tea_hip_fcg_read_z": [_vp, _fp, ctypes.c_size_t],
     "tea_hip_set_fused": [_vp, _i],

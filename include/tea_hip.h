@@ -280,6 +280,10 @@ int tea_hip_fcg_init(tea_hip_chunk* chunk, int coefficient, double rx,
                      double ry, double* rr, double* bb);
 /* z = M(r) with the network (use_net != 0) or Jacobi; *zw = z.w. */
 int tea_hip_fcg_precondition(tea_hip_chunk* chunk, int use_net, double* zw);
+/* tea_hip_mpcg_calc_r then tea_hip_fcg_precondition with one wait:
+ * r -= alpha w, *rr = r.r, z = M(r), *zw = z.w. */
+int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* chunk, double alpha,
+                                    int use_net, double* rr, double* zw);
 /* If apply_u: u += alpha p_old.  p = z + beta p_old; w = A p;
  * *pw = p.w, *pr = p.r. */
 int tea_hip_fcg_calc_w(tea_hip_chunk* chunk, double alpha, int apply_u,

@@ -268,6 +268,12 @@ void run_fcg_precondition(Chunk* c, Settings* s, int use_net, double* zw) {
                         e->mp_w, e->fcg_z);
 }
 
+void run_fcg_calc_r_precondition(Chunk* c, Settings* s, double alpha,
+                                 int use_net, double* rr, double* zw) {
+  run_mpcg_calc_r(c, s, alpha, rr);
+  run_fcg_precondition(c, s, use_net, zw);
+}
+
 void run_fcg_calc_w(Chunk* c, Settings* s, double alpha, int apply_u,
                     double beta, double* pw, double* pr) {
   ChunkExtension* e = c->ext;
