@@ -213,7 +213,7 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
     if (c->field[f]) (void)hipFree(c->field[f]);
   if (c->p_alt) (void)hipFree(c->p_alt);
   for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky,
-                   c->fcg_z, c->fcg_feat, c->fcg_rbar, c->fcg_scale, c->fcg_kc, c->fcg_dis})
+                   c->fcg_z, c->fcg_feat, c->fcg_rbar, c->fcg_scale, c->fcg_kc, c->fcg_dis, c->fcg_sig, c->fcg_tmp})
     if (b) (void)hipFree(b);
   if (c->fcg_part) (void)hipFree(c->fcg_part);
   if (c->fcg_net) tea_nn_destroy(c->fcg_net);

@@ -7,15 +7,18 @@
 //
 // Preconditioners, z = M(r):
 //   Jacobi:  z = r / diag(A), diag = 1 + kx[i] + kx[i+1] + ky[i] + ky[i+x].
-//   Learned: z = r / diag(A) + g P y, y = D^-1/2 N(D^-1/2 R r / s, f),
+//   Learned: z = r / diag(A) + g P y,
+//            y = D^-1/2 sigma N(D^-1/2 R r / sigma, f),
 //            where R is the c x c block mean onto the coarse grid, T =
-//            P^T A P / c^2 the Galerkin coarse operator, D its diagonal, s
-//            the RMS of D^-1/2 R r, N the 6-layer network on the coarse
+//            P^T A P / c^2 the Galerkin coarse operator, D its diagonal,
+//            sigma the RMS of D^-1/2 R r over the 13 x 13 window around each
+//            coarse cell (the receptive field: every region reaches the
+//            network at unit scale), N the 6-layer network on the coarse
 //            grid, P the piecewise-constant prolongation, f the couplings
 //            of D^-1/2 T D^-1/2 (feature_kernel) and g = (y.Rr) / (y.T y)
 //            the energy-optimal factor of the coarse direction (all
-//            computed on the device).  N is trained to do what a few CG
-//            steps on T do, so that the coarse term is a local
+//            computed on the device).  N is trained to do what 6 sweeps of
+//            TeaLeaf's Jacobi iteration on T do, so that the coarse term is a local
 //            approximation of the two-level additive preconditioner
 //            D^-1 + P (P^T A P)^-1 P^T.  With g,
 //            z.r = r.D^-1 r + c^2 (y.Rr)^2 / (y.T y) > 0 for any y: the
@@ -53,73 +56,122 @@ struct JacobiBody {
   }
 };
 
-// Coarse block means of r and per-block partial sums of (D^-1/2 rbar)^2.
-__global__ __launch_bounds__(256) void restrict_kernel(
-    Geo g, int c, int hc, int wc, const float* __restrict__ r,
-    const float* __restrict__ dis, float* __restrict__ rbar,
-    double* __restrict__ part) {
-  __shared__ double s[256];
+// Coarse block means of r, and per-block partial sums of (D^-1/2 rbar)^2.
+__global__ __launch_bounds__(256) void restrict_kernel(Geo g, int c, int hc, int wc,
+                                                       const float* __restrict__ r,
+                                                       const float* __restrict__ dis,
+                                                       float* __restrict__ rbar,
+                                                       double* __restrict__ part) {
+  __shared__ double sh[256];
   const int idx = blockIdx.x * 256 + threadIdx.x;
   double v2 = 0.0;
   if (idx < hc * wc) {
-    const int X = idx % wc, Y = idx / wc;
-    double sum = 0.0;
+  const int X = idx % wc, Y = idx / wc;
+  double sum = 0.0;
+  if (((g.x | g.h | c) & 1) == 0) {
+    // Every block row starts on an even index: 8-byte loads.
+    for (int b = 0; b < c; ++b) {
+      const int row = (g.h + Y * c + b) * g.x + g.h + X * c;
+      for (int a = 0; a < c; a += 2) {
+        const float2 v = ld2(r, row + a);
+        sum += (double)v.x + (double)v.y;
+      }
+    }
+  } else {
     for (int b = 0; b < c; ++b) {
       const int row = (g.h + Y * c + b) * g.x + g.h + X * c;
       for (int a = 0; a < c; ++a) sum += (double)r[row + a];
     }
-    const float m = (float)(sum / (double)(c * c));
-    rbar[idx] = m;
-    const double ms = (double)m * (double)dis[idx];
-    v2 = ms * ms;
   }
-  s[threadIdx.x] = v2;
+  const float m = (float)(sum / (double)(c * c));
+  rbar[idx] = m;
+  const double ms = (double)m * (double)dis[idx];
+  v2 = ms * ms;
+  }
+  sh[threadIdx.x] = v2;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// s = RMS of D^-1/2 rbar (1 if it is zero).
-__global__ __launch_bounds__(256) void scale_kernel(const double* __restrict__ part,
-                                                    int nblk, long count,
-                                                    float* __restrict__ scale) {
-  __shared__ double s[256];
+// scale[0] = RMS of D^-1/2 rbar over the coarse grid.
+__global__ __launch_bounds__(256) void rms_kernel(const double* __restrict__ part, int nblk,
+                                                  long count, float* __restrict__ scale) {
+  __shared__ double sh[256];
   double v = 0.0;
   for (int i = threadIdx.x; i < nblk; i += 256) v += part[i];
-  s[threadIdx.x] = v;
+  sh[threadIdx.x] = v;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const double rms = sqrt(s[0] / (double)count);
-    scale[0] = rms > 0.0 ? (float)rms : 1.0f;
-  }
+  if (threadIdx.x == 0) scale[0] = (float)sqrt(sh[0] / (double)count);
 }
 
-// Network input: channel 0 = D^-1/2 rbar / s, channels 1, 2 = features.
+// Local scale of the network input: sigma = RMS of D^-1/2 rbar over the
+// (2 kWin + 1)^2 window around each coarse cell (zero beyond the edge), in
+// two passes (rows into tmp, then columns), each summed in a fixed order,
+// and at least kFloor times the RMS over the grid (so that regions of
+// round-off do not reach the network magnified).
+constexpr int kWin = 6;         // the network's receptive-field radius
+constexpr float kFloor = 1e-3f;
+
+__global__ __launch_bounds__(256) void box_rows_kernel(int hc, int wc,
+                                                       const float* __restrict__ rbar,
+                                                       const float* __restrict__ dis,
+                                                       float* __restrict__ tmp) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= hc * wc) return;
+  const int X = idx % wc;
+  double s = 0.0;
+  for (int d = -kWin; d <= kWin; ++d) {
+    if (X + d < 0 || X + d >= wc) continue;
+    const double v = (double)rbar[idx + d] * (double)dis[idx + d];
+    s += v * v;
+  }
+  tmp[idx] = (float)s;
+}
+
+__global__ __launch_bounds__(256) void box_cols_kernel(int hc, int wc,
+                                                       const float* __restrict__ tmp,
+                                                       const float* __restrict__ scale,
+                                                       float* __restrict__ sig) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= hc * wc) return;
+  const int Y = idx / wc;
+  double s = 0.0;
+  for (int d = -kWin; d <= kWin; ++d) {
+    if (Y + d < 0 || Y + d >= hc) continue;
+    s += (double)tmp[idx + d * wc];
+  }
+  sig[idx] = fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))),
+                   kFloor * scale[0]);
+}
+
+// Network input: channel 0 = D^-1/2 rbar / sigma (0 where sigma is 0),
+// channels 1, 2 = features.
 __global__ __launch_bounds__(256) void pack_kernel(int hc, int wc,
                                                    const float* __restrict__ rbar,
                                                    const float* __restrict__ dis,
+                                                   const float* __restrict__ sig,
                                                    const float* __restrict__ feat,
-                                                   const float* __restrict__ scale,
                                                    bf16* __restrict__ in0) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= hc * wc) return;
   const int X = idx % wc, Y = idx / wc;
-  const float inv = 1.0f / scale[0];
+  const float sg = sig[idx];
   bf16x8 o = bf16x8{};
-  o[0] = (bf16)((rbar[idx] * dis[idx]) * inv);
+  o[0] = sg > 0.0f ? (bf16)((rbar[idx] * dis[idx]) / sg) : (bf16)0.0f;
   o[1] = (bf16)feat[idx];
   o[2] = (bf16)feat[hc * wc + idx];
   *reinterpret_cast<bf16x8*>(in0 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * 8) = o;
 }
 
-// z = r / diag + s * net_out[coarse cell] and z.w.
+// z = r / diag + g * y[coarse cell] and z.w (g = scale[0]).
 struct ProlongBody {
   const float* r;
   const float* kx;
@@ -139,7 +191,7 @@ struct ProlongBody {
   }
 };
 
-// z = r / diag (+ s * net[coarse cell] with NET) and z.w, two cells per
+// z = r / diag (+ g * y[coarse cell] with NET, g = scale[0]) and z.w, two cells per
 // lane (x even): the arithmetic of JacobiBody / ProlongBody cell by cell.
 // The coarse cell of interior cell (kk, jj) is ((kk-h) >> shift,
 // (jj-h) >> shift).
@@ -240,11 +292,12 @@ __global__ __launch_bounds__(256) void feature_kernel(int hc, int wc,
   dis[idx] = (float)di;
 }
 
-// The network's output in the scaled variables back to T's: y = D^-1/2 y.
+// The network's output back in T's variables: y = D^-1/2 sigma N.
 __global__ __launch_bounds__(256) void unscale_kernel(int cn, const float* __restrict__ dis,
+                                                      const float* __restrict__ sig,
                                                       float* __restrict__ y) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx < cn) y[idx] *= dis[idx];
+  if (idx < cn) y[idx] = y[idx] * (sig[idx] * dis[idx]);
 }
 
 // Per-block partial sums over coarse cells of y.rbar and y.(T y), with T
@@ -357,17 +410,21 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, double* partials) 
       hipLaunchKernelGGL(restrict_kernel, dim3(nblk), dim3(256), 0, c->stream, g,
                          c->fcg_c, c->fcg_hc, c->fcg_wc, c->mp_r, c->fcg_dis,
                          c->fcg_rbar, c->fcg_part);
-      hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(256), 0, c->stream, c->fcg_part,
-                         nblk, (long)cn, c->fcg_scale);
+      hipLaunchKernelGGL(rms_kernel, dim3(1), dim3(256), 0, c->stream, c->fcg_part, nblk,
+                         (long)cn, c->fcg_scale);
+      hipLaunchKernelGGL(box_rows_kernel, dim3(nblk), dim3(256), 0, c->stream,
+                         c->fcg_hc, c->fcg_wc, c->fcg_rbar, c->fcg_dis, c->fcg_tmp);
+      hipLaunchKernelGGL(box_cols_kernel, dim3(nblk), dim3(256), 0, c->stream,
+                         c->fcg_hc, c->fcg_wc, c->fcg_tmp, c->fcg_scale, c->fcg_sig);
       hipLaunchKernelGGL(pack_kernel, dim3(nblk), dim3(256), 0, c->stream, c->fcg_hc,
-                         c->fcg_wc, c->fcg_rbar, c->fcg_dis, c->fcg_feat, c->fcg_scale,
+                         c->fcg_wc, c->fcg_rbar, c->fcg_dis, c->fcg_sig, c->fcg_feat,
                          nn::input_tensor(c->fcg_net));
     }
     {
       Timed t(c, K_FCG_NET);
       TEA_RETURN_IF(nn::forward_on_stream(c->fcg_net, c->stream));
       hipLaunchKernelGGL(unscale_kernel, dim3(nblk), dim3(256), 0, c->stream, cn, c->fcg_dis,
-                         nn::output_mutable(c->fcg_net));
+                         c->fcg_sig, nn::output_mutable(c->fcg_net));
       hipLaunchKernelGGL(coarse_dots_kernel, dim3(nblk), dim3(256), 0, c->stream,
                          c->fcg_hc, c->fcg_wc, nn::output(c->fcg_net), c->fcg_rbar,
                          c->fcg_kc, c->fcg_part);
@@ -432,11 +489,16 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
   if (c->fcg_scale) (void)hipFree(c->fcg_scale);
   if (c->fcg_kc) (void)hipFree(c->fcg_kc);
   if (c->fcg_dis) (void)hipFree(c->fcg_dis);
+  if (c->fcg_sig) (void)hipFree(c->fcg_sig);
+  if (c->fcg_tmp) (void)hipFree(c->fcg_tmp);
   c->fcg_feat = c->fcg_rbar = c->fcg_scale = c->fcg_kc = c->fcg_dis = nullptr;
+  c->fcg_sig = c->fcg_tmp = nullptr;
   c->fcg_part = nullptr;
   if (hipMalloc(&c->fcg_feat, 2 * cn * sizeof(float)) != hipSuccess ||
       hipMalloc(&c->fcg_kc, 2 * cn * sizeof(float)) != hipSuccess ||
       hipMalloc(&c->fcg_dis, cn * sizeof(float)) != hipSuccess ||
+      hipMalloc(&c->fcg_sig, cn * sizeof(float)) != hipSuccess ||
+      hipMalloc(&c->fcg_tmp, cn * sizeof(float)) != hipSuccess ||
       hipMalloc(&c->fcg_rbar, cn * sizeof(float)) != hipSuccess ||
       hipMalloc(&c->fcg_part, 2 * nblk * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->fcg_scale, 2 * sizeof(float)) != hipSuccess) {

@@ -74,22 +74,17 @@ constexpr int nsteps() {
 }
 
 // MODE 0: out = relu(D + bias).  MODE 1: out = D * (mask > 0).
-// One 32-pixel tile per wave: wave g of the launch (4 per 256-thread block)
-// takes tile g of the B x H x (W/32) tiles, so that even one sample of a
-// wide grid gives thousands of waves.
+// Grid (H, B), 256 threads; each wave walks 32-pixel tiles of row y, so the
+// weights it holds in registers serve several tiles.
 template <int CIN, int MODE>
 __global__ __launch_bounds__(256) void conv_mfma_kernel(
     const bf16* __restrict__ in, const bf16* __restrict__ wt,
     const float* __restrict__ bias, const bf16* __restrict__ mask,
     bf16* __restrict__ out, int H, int W, int B) {
   constexpr int NS = nsteps<CIN>();
+  (void)B;
+  const int y = blockIdx.x, b = blockIdx.y;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int tiles_row = W / 32;
-  const long gid = (long)blockIdx.x * 4 + wv;
-  if (gid >= (long)B * H * tiles_row) return;
-  const int tile = (int)(gid % tiles_row);
-  const int y = (int)((gid / tiles_row) % H);
-  const int b = (int)(gid / ((long)tiles_row * H));
   const int hf = lane >> 5, col = lane & 31;
   const int Wp = W + 2;
   const size_t plane = (size_t)(H + 2) * Wp;
@@ -109,7 +104,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
     for (int v = 0; v < 16; ++v) bco[v] = bias[8 * (v >> 2) + 4 * hf + (v & 3)];
   }
   const bf16* inb = in + (size_t)b * plane * CIN;
-  {
+  for (int tile = wv; tile < W / 32; tile += 4) {
     const int xo = tile * 32 + col;
     bf16x8 Bv[NS];
 #pragma unroll
@@ -521,10 +516,7 @@ int pack_params(tea_nn* n) {
   return TEA_HIP_OK;
 }
 
-dim3 rows_grid(const tea_nn* n, int batch) {
-  const long tiles = (long)batch * n->H * (n->W / 32);
-  return dim3((unsigned)((tiles + 3) / 4));
-}
+dim3 rows_grid(const tea_nn* n, int batch) { return dim3(n->H, batch); }
 unsigned flat_blocks(long count) { return (unsigned)((count + 255) / 256); }
 
 // Forward pass over `batch` samples whose packed input is in n->in0, on

@@ -412,6 +412,8 @@ struct tea_hip_chunk {
   float* fcg_rbar = nullptr;   // [hc][wc]
   float* fcg_kc = nullptr;     // [2][hc][wc] coarse left / bottom face coefficients
   float* fcg_dis = nullptr;    // [hc][wc] D^-1/2 of the coarse operator
+  float* fcg_sig = nullptr;    // [hc][wc] local input scale of the network
+  float* fcg_tmp = nullptr;    // [hc][wc] row pass of the local scale
   double* fcg_part = nullptr;  // per-block partial sums (2 * blocks)
   float* fcg_scale = nullptr;  // {s, prolongation factor}
 

@@ -72,13 +72,14 @@ def test_learned_preconditioner_matches_numpy(n):
     kxc, kyc = learned.coarse_faces(kx[H:H + n, H:H + n + 1], ky[H:H + n + 1, H:H + n], cc)
     dis = learned.coarse_scaling(kxc, kyc).astype(np.float32)
     rs = (np.float32(rbar) * dis).astype(np.float64)
-    s = np.float32(np.sqrt((rs * rs).sum() / rs.size))
+    sig = np.maximum(learned.local_scale(rs), learned.FLOOR * np.sqrt(np.mean(rs * rs)))
+    sig = sig.astype(np.float32)
     g0, g1 = learned.coarse_features(kxc, kyc)
     f0 = (g0 - norm[0]) / norm[1]
     f1 = (g1 - norm[2]) / norm[3]
-    x = np.stack([(np.float32(rbar) * dis) * (np.float32(1) / s), f0, f1], axis=-1)[None]
+    x = np.stack([(np.float32(rbar) * dis) / sig, f0, f1], axis=-1)[None]
     out, _ = O.forward(params, x.astype(np.float32), bf16_mode=True)
-    y = (out[0] * dis).astype(np.float64)
+    y = (out[0] * (sig * dis)).astype(np.float64)
     # Energy-optimal factor of the coarse direction: (y.Rr) / (y.T y).
     kl = kxc[:, :-1].astype(np.float32).astype(np.float64)
     kr = kxc[:, 1:].astype(np.float32).astype(np.float64)
@@ -98,40 +99,32 @@ def test_learned_preconditioner_matches_numpy(n):
     assert abs(zw - want_zw) <= 1e-9 * np.abs(got * w[H:-H, H:-H]).sum()
 
 
-def test_training_targets_are_plain_cg_iterates(oracle):
-    """The target of each training input r is the plain FP64 CG iterate
-    after learned.STEPS iterations on the problem's own coarse operator
-    (a chunk of one tile, zero faces on its edges) from e = 0: the oracle's
-    CG from the same fields reproduces it bitwise."""
+def test_training_pairs():
+    """Training pairs from the generator on the GPU: each target is the
+    Jacobi sweeps of its input field on the problems' coarse operators
+    (zero faces between problems), and the first pair's input channels,
+    features and target are that field, those coefficients and those
+    sweeps in the scaled variables over the input's local scale."""
     data = learned.generate(0, 3, snapshots=2, rng_seed=1, keep_coarse=True)
     n = data["inputs"].shape[0]
     cw = learned.TILE // learned.COARSEN
     assert n >= 3 and data["targets"].shape == (n, cw, cw)
     assert np.isfinite(data["inputs"]).all() and np.isfinite(data["targets"]).all()
-    r, got = data["coarse_r"], data["coarse_e"]
-    X = cw + 2 * H
-    kx = np.zeros((X, X))
-    ky = np.zeros((X, X))
-    kx[H:H + cw, H:H + cw + 1] = data["coarse_kx"]
-    ky[H:H + cw + 1, H:H + cw] = data["coarse_ky"]
-    u0 = np.zeros((X, X))
-    u0[H:-H, H:-H] = r
-    f = dict(u=np.zeros(X * X), u0=u0.ravel(), r=np.zeros(X * X), p=np.zeros(X * X),
-             w=np.zeros(X * X), kx=kx.ravel(), ky=ky.ravel())
-    oracle.calculate_residual(X, X, H, f)
-    oracle.cg_calc_p(X, X, H, 0.0, f)
-    for name in ("u", "p"):
-        oracle.halo_field(X, X, H, 1, f[name])
-    rro = oracle.calculate_2norm(X, X, H, f["r"])
-    for _ in range(learned.STEPS):
-        pw = oracle.cg_calc_w(X, X, H, f)
-        rrn = oracle.cg_calc_ur(X, X, H, rro / pw, f)
-        oracle.cg_calc_p(X, X, H, rrn / rro, f)
-        for name in ("u", "p"):
-            oracle.halo_field(X, X, H, 1, f[name])
-        rro = rrn
-    want = f["u"].reshape(X, X)[H:-H, H:-H]
-    assert np.array_equal(got, want)
+    field, ek = data["coarse_field"], data["coarse_e_field"]
+    want = learned.jacobi_sweeps(field, data["chunk_kx"], data["chunk_ky"])
+    assert np.array_equal(ek, want)
+    # The first pair: tile (0, 0) of the first field.
+    ys = xs = slice(H, H + cw)
+    kxc = data["chunk_kx"][ys, H:H + cw + 1]
+    kyc = data["chunk_ky"][H:H + cw + 1, xs]
+    dis = learned.coarse_scaling(kxc, kyc)
+    rs = field[ys, xs] * dis
+    sig = np.maximum(learned.local_scale(rs), learned.FLOOR * np.sqrt(np.mean(rs * rs)))
+    assert np.allclose(data["inputs"][0, ..., 0], rs / sig, rtol=1e-6)
+    assert np.allclose(data["targets"][0], ek[ys, xs] / dis / sig, rtol=1e-6)
+    f0, f1 = learned.coarse_features(kxc, kyc)
+    assert np.allclose(data["inputs"][0, ..., 1], f0, rtol=1e-6)
+    assert np.allclose(data["inputs"][0, ..., 2], f1, rtol=1e-6)
 
 
 def _fcg_deck(n, seed, extra):
@@ -169,16 +162,20 @@ def test_fcg_with_untrained_network_converges(tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(learned.WEIGHTS_PATH),
                     reason="no trained network file")
-@pytest.mark.parametrize("n", [512, 1024])
-def test_trained_network_gains_over_jacobi(tmp_path, n):
-    """The trained network against the same flexible CG without it (Jacobi
-    only) on unseen fields: fewer iterations to the same 1e-10, and no
-    fallback."""
-    learned_rec = _run_fcg(tmp_path, n, 900 + n, learned.WEIGHTS_PATH, "fcg")
-    jacobi_rec = _run_fcg(tmp_path, n, 900 + n, None, "jpcg")
+@pytest.mark.parametrize("n,seed", [(512, 3000), (4096, 3000)])
+def test_trained_network_gains_over_jacobi(tmp_path, n, seed):
+    """What the trained network gains: flexible CG with it against the same
+    flexible CG without it (Jacobi-PCG) on an unseen field of the
+    64-rectangle workload (log10 density in [-2, 2]), both to 1e-10 relative
+    residual; the learned solve must need at most 2/3 of the iterations,
+    without falling back, and agree with the Jacobi solve's temperature."""
+    learned_rec = _run_fcg(tmp_path, n, seed, learned.WEIGHTS_PATH, "fcg")
+    jacobi_rec = _run_fcg(tmp_path, n, seed, None, "jpcg")
     a, b = learned_rec["steps"][0], jacobi_rec["steps"][0]
     print(f"{n}^2: learned FCG {a['iterations']} iterations, Jacobi-PCG "
           f"{b['iterations']}")
     assert a["rel_residual"] < 1e-10 and b["rel_residual"] < 1e-10
     assert a["fallback_at"] == -1
-    assert a["iterations"] < b["iterations"]
+    assert a["iterations"] <= (2 * b["iterations"]) // 3
+    ta, tb = learned_rec["summaries"][-1]["temp"], jacobi_rec["summaries"][-1]["temp"]
+    assert abs(ta / tb - 1) < 1e-8

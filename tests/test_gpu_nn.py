@@ -29,7 +29,7 @@ def _case(seed, b=3, h=16, w=64):
     return p, x, t
 
 
-@pytest.mark.parametrize("shape", [(3, 16, 64), (2, 7, 32), (1, 64, 128)])
+@pytest.mark.parametrize("shape", [(3, 16, 64), (2, 7, 32), (1, 64, 128), (1, 12, 512)])
 def test_forward_matches_oracle(shape):
     p, x, t = _case(1, *shape)
     net = _net(*shape)

@@ -144,3 +144,73 @@ def test_coarse_faces_are_the_galerkin_operator():
     kxc, kyc = learned.coarse_faces(kx, ky, c)
     got = five_point(kxc, kyc).toarray()
     assert np.allclose(got, want, rtol=1e-13, atol=1e-12)
+
+
+def _coarse_chunk(n=12, seed=3):
+    rng = np.random.default_rng(seed)
+    X = n + 4
+    kx = np.zeros((X, X))
+    ky = np.zeros((X, X))
+    kx[2:2 + n, 3:2 + n] = 10.0 ** rng.uniform(-1, 3, (n, n - 1))
+    ky[3:2 + n, 2:2 + n] = 10.0 ** rng.uniform(-1, 3, (n - 1, n))
+    return kx, ky
+
+
+def _dense(kx, ky, n):
+    A = np.zeros((n * n, n * n))
+    for j in range(n):
+        for k in range(n):
+            i = j * n + k
+            J, K = j + 2, k + 2
+            A[i, i] = 1 + kx[J, K] + kx[J, K + 1] + ky[J, K] + ky[J + 1, K]
+            if k > 0:
+                A[i, i - 1] = -kx[J, K]
+            if k < n - 1:
+                A[i, i + 1] = -kx[J, K + 1]
+            if j > 0:
+                A[i, i - n] = -ky[J, K]
+            if j < n - 1:
+                A[i, i + n] = -ky[J + 1, K]
+    return A
+
+
+def test_jacobi_sweeps_pinned():
+    """jacobi_sweeps: one sweep from zero is rhs / diag; many sweeps reach
+    the dense solve of (I + L) e = rhs; 6 sweeps are a symmetric positive
+    definite linear map whose reach is exactly 6 cells."""
+    n = 12
+    kx, ky = _coarse_chunk(n)
+    A = _dense(kx, ky, n)
+    rng = np.random.default_rng(4)
+    rhs = np.zeros((n + 4, n + 4))
+    rhs[2:-2, 2:-2] = rng.standard_normal((n, n))
+    one = learned.jacobi_sweeps(rhs, kx, ky, 1)[2:-2, 2:-2].ravel()
+    assert np.allclose(one, rhs[2:-2, 2:-2].ravel() / np.diag(A), rtol=1e-14, atol=0)
+    many = learned.jacobi_sweeps(rhs, kx, ky, 20000)[2:-2, 2:-2].ravel()
+    assert np.allclose(many, np.linalg.solve(A, rhs[2:-2, 2:-2].ravel()), rtol=1e-8, atol=1e-12)
+    # The 6-sweep map as a matrix, p(D^-1 T) D^-1 = D^-1/2 p(S) D^-1/2 with
+    # S = D^-1/2 T D^-1/2: symmetric, positive definite, zero beyond 6 cells.
+    M = np.zeros((n * n, n * n))
+    for q in range(n * n):
+        e = np.zeros((n + 4, n + 4))
+        e[2 + q // n, 2 + q % n] = 1.0
+        M[:, q] = learned.jacobi_sweeps(e, kx, ky, 6)[2:-2, 2:-2].ravel()
+    assert np.allclose(M, M.T, rtol=1e-10, atol=1e-14)
+    assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
+    jj, kk = np.divmod(np.arange(n * n), n)
+    far = (np.abs(jj[:, None] - jj[None, :]) + np.abs(kk[:, None] - kk[None, :])) > 6
+    assert np.all(M[far] == 0.0)
+
+
+def test_local_scale_is_the_window_rms():
+    """local_scale: the RMS over the 13 x 13 window with zero padding, as a
+    brute-force loop computes it."""
+    v = np.random.default_rng(6).standard_normal((20, 17))
+    got = learned.local_scale(v)
+    w = learned.WINDOW
+    want = np.zeros_like(v)
+    for j in range(v.shape[0]):
+        for k in range(v.shape[1]):
+            blk = v[max(0, j - w):j + w + 1, max(0, k - w):k + w + 1]
+            want[j, k] = np.sqrt((blk * blk).sum() / (2 * w + 1) ** 2)
+    assert np.allclose(got, want, rtol=1e-12, atol=0)
