@@ -145,6 +145,80 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
   }
 }
 
+// Forward layer for rows of 128 pixels or more: a block takes a 128-pixel
+// segment of row y, stages input rows y-1..y+1 over the segment plus its
+// 1-pixel ring in LDS once (130 pixels x CIN channels x 3 rows, in 16-byte
+// chunks; a pixel's chunks are padded to kPitch so that the 16 lanes of a
+// 16-byte LDS read, which take consecutive pixels, hit distinct banks),
+// and each wave computes one 32-pixel tile from LDS.  Same arithmetic and
+// output as conv_mfma_kernel<CIN, 0>.  Grid (W / 128, H, B).
+template <int CIN>
+__global__ __launch_bounds__(256) void conv_fwd_lds_kernel(
+    const bf16* __restrict__ in, const bf16* __restrict__ wt,
+    const float* __restrict__ bias, bf16* __restrict__ out, int H, int W) {
+  constexpr int NS = nsteps<CIN>();
+  constexpr int CPC = CIN / 8;                   // 16-byte chunks per pixel
+  constexpr int kPitch = CPC == 1 ? 1 : CPC + 1;  // padded chunks per pixel
+  constexpr int kPx = 130;
+  __shared__ bf16x8 sm[3][kPx * kPitch];
+  const int seg = blockIdx.x, y = blockIdx.y, b = blockIdx.z;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int hf = lane >> 5, col = lane & 31;
+  const int Wp = W + 2;
+  const size_t plane = (size_t)(H + 2) * Wp;
+  const bf16* inb = in + (size_t)b * plane * CIN;
+  const int x0 = seg * 128;  // padded column of the segment's left ring pixel
+  for (int q = threadIdx.x; q < 3 * kPx * CPC; q += 256) {
+    const int row = q / (kPx * CPC), rem = q - row * (kPx * CPC);
+    const int px = rem / CPC, ch = rem - px * CPC;
+    sm[row][px * kPitch + ch] = *reinterpret_cast<const bf16x8*>(
+        inb + ((size_t)(y + row) * Wp + (x0 + px)) * CIN + ch * 8);
+  }
+  bf16x8 A[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) {
+    int tap, ci;
+    kmap<CIN>(t, hf, tap, ci);
+    if (tap < kTaps)
+      A[t] = *reinterpret_cast<const bf16x8*>(wt + ((size_t)col * kTaps + tap) * CIN + ci);
+    else
+      A[t] = bf16x8{};
+  }
+  float bco[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) bco[v] = bias[8 * (v >> 2) + 4 * hf + (v & 3)];
+  __syncthreads();
+  const int xl = wv * 32 + col;  // pixel within the segment
+  bf16x8 Bv[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) {
+    int tap, ci;
+    kmap<CIN>(t, hf, tap, ci);
+    if (tap < kTaps) {
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      Bv[t] = sm[dy + 1][(xl + 1 + dx) * kPitch + ci / 8];
+    } else {
+      Bv[t] = bf16x8{};
+    }
+  }
+  f32x16 acc = {};
+#pragma unroll
+  for (int t = 0; t < NS; ++t)
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[t], Bv[t], acc, 0, 0, 0);
+  const size_t pix = (size_t)b * plane + (size_t)(y + 1) * Wp + (x0 + xl + 1);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int co0 = 8 * g + 4 * hf;
+    bf16x4 o;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float vv = acc[4 * g + q] + bco[4 * g + q];
+      o[q] = (bf16)(vv > 0.0f ? vv : 0.0f);
+    }
+    *reinterpret_cast<bf16x4*>(out + pix * kC + co0) = o;
+  }
+}
+
 // ---- output layer (32 -> 1), FP32 on the vector units ------------------------
 
 // out[b][y][x] = b6 + sum_{tap,ci} w6[tap][ci] * a5[b][y+1+dy][x+1+dx][ci].
@@ -523,12 +597,21 @@ unsigned flat_blocks(long count) { return (unsigned)((count + 255) / 256); }
 // stream `st` (the network's own, or a solver's).
 int forward_on(tea_nn* n, int batch, hipStream_t st) {
   const int H = n->H, W = n->W;
-  hipLaunchKernelGGL((conv_mfma_kernel<kC0, 0>), rows_grid(n, batch), dim3(256), 0,
-                     st, n->in0, n->wf1, n->bias, nullptr, n->act[0], H, W, batch);
-  for (int l = 1; l < 5; ++l)
-    hipLaunchKernelGGL((conv_mfma_kernel<kC, 0>), rows_grid(n, batch), dim3(256), 0,
-                       st, n->act[l - 1], n->wfh + (size_t)(l - 1) * kWh,
-                       n->bias + l * kC, nullptr, n->act[l], H, W, batch);
+  if (W % 128 == 0) {
+    const dim3 grid(W / 128, H, batch);
+    hipLaunchKernelGGL((conv_fwd_lds_kernel<kC0>), grid, dim3(256), 0, st, n->in0, n->wf1,
+                       n->bias, n->act[0], H, W);
+    for (int l = 1; l < 5; ++l)
+      hipLaunchKernelGGL((conv_fwd_lds_kernel<kC>), grid, dim3(256), 0, st, n->act[l - 1],
+                         n->wfh + (size_t)(l - 1) * kWh, n->bias + l * kC, n->act[l], H, W);
+  } else {
+    hipLaunchKernelGGL((conv_mfma_kernel<kC0, 0>), rows_grid(n, batch), dim3(256), 0,
+                       st, n->in0, n->wf1, n->bias, nullptr, n->act[0], H, W, batch);
+    for (int l = 1; l < 5; ++l)
+      hipLaunchKernelGGL((conv_mfma_kernel<kC, 0>), rows_grid(n, batch), dim3(256), 0,
+                         st, n->act[l - 1], n->wfh + (size_t)(l - 1) * kWh,
+                         n->bias + l * kC, nullptr, n->act[l], H, W, batch);
+  }
   hipLaunchKernelGGL(out_layer_kernel, dim3(flat_blocks((long)batch * H * W)), dim3(256),
                      0, st, n->act[4], n->w6, n->bias, n->out, batch, H, W);
   TEA_HIP_TRY(hipGetLastError());

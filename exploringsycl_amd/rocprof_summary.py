@@ -2,7 +2,8 @@
 
   python -m exploringsycl_amd.rocprof_summary stats DIR
       per-kernel launches, average and total device time from the
-      --kernel-trace --stats run (DIR/*_kernel_stats.csv).
+      --kernel-trace --stats run (DIR/*_kernel_stats.csv, summed over the
+      processes of the run).
 
   python -m exploringsycl_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR \
       --kernel REGEX --copy-kernel REGEX --copy-bytes B --algorithmic-bytes A
@@ -20,19 +21,25 @@ import re
 import sys
 
 
-def _one(pattern):
+def _all(pattern):
     files = sorted(glob.glob(pattern))
     if not files:
         raise SystemExit(f"no file matches {pattern}")
-    return files[0]
+    return files
 
 
 def kernel_stats(d):
-    rows = []
-    with open(_one(os.path.join(d, "*kernel_stats.csv"))) as f:
-        for r in csv.DictReader(f):
-            rows.append((r["Name"], int(r["Calls"]), float(r["AverageNs"]) / 1e3,
-                         float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
+    """(name, calls, average us, total ms, % of all kernel time), summed over
+    every process's stats file in d (one per process with -o name_%pid%)."""
+    calls, total = {}, {}
+    for path in _all(os.path.join(d, "*kernel_stats.csv")):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                calls[r["Name"]] = calls.get(r["Name"], 0) + int(r["Calls"])
+                total[r["Name"]] = total.get(r["Name"], 0.0) + float(r["TotalDurationNs"])
+    grand = sum(total.values()) or 1.0
+    rows = [(k, calls[k], total[k] / calls[k] / 1e3, total[k] / 1e6, 100.0 * total[k] / grand)
+            for k in calls]
     rows.sort(key=lambda t: -t[3])
     return rows
 
@@ -53,14 +60,15 @@ def short(name):
 def counter_means(d, counter):
     """{kernel name: mean counter value per dispatch}."""
     acc = {}
-    with open(_one(os.path.join(d, "*counter_collection.csv"))) as f:
-        for r in csv.DictReader(f):
-            if r["Counter_Name"] != counter:
-                continue
-            key = (r["Kernel_Name"], r["Dispatch_Id"])
-            acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
+    for path in _all(os.path.join(d, "*counter_collection.csv")):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                key = (path, r["Kernel_Name"], r["Dispatch_Id"])
+                acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
     per = {}
-    for (name, _), v in acc.items():
+    for (_, name, _), v in acc.items():
         per.setdefault(name, []).append(v)
     return {k: sum(v) / len(v) for k, v in per.items()}
 
