@@ -181,18 +181,25 @@ class DeviceChunk:
         return out.reshape(self.y, self.x)
 
     # ---- flexible CG (no reference counterpart) --------------------------------
-    def fcg_attach(self, params=None, norm=None, coarsen=0):
-        """Allocates the FCG state; with ``params`` also loads the network."""
+    def fcg_attach(self, params=None, norm=None, coarsen=0, levels=2):
+        """Allocates the FCG state; with ``params`` also loads the network on
+        up to ``levels`` coarse levels."""
         fp = ctypes.POINTER(ctypes.c_float)
         if params is None:
-            check(self.lib.tea_hip_fcg_attach(self.handle, None, 0, None, 0), "fcg_attach")
+            check(self.lib.tea_hip_fcg_attach(self.handle, None, 0, None, 0, 0), "fcg_attach")
             return
         self._fcg_params = np.ascontiguousarray(params, dtype=np.float32)
         self._fcg_norm = np.ascontiguousarray(norm, dtype=np.float32)
         check(self.lib.tea_hip_fcg_attach(self.handle, self._fcg_params.ctypes.data_as(fp),
                                           self._fcg_params.size,
-                                          self._fcg_norm.ctypes.data_as(fp), int(coarsen)),
+                                          self._fcg_norm.ctypes.data_as(fp), int(coarsen),
+                                          int(levels)),
               "fcg_attach")
+
+    def fcg_levels(self):
+        v = ctypes.c_int()
+        check(self.lib.tea_hip_fcg_levels(self.handle, ctypes.byref(v)), "fcg_levels")
+        return v.value
 
     def fcg_init(self, coefficient, rx, ry):
         rr, bb = ctypes.c_double(), ctypes.c_double()

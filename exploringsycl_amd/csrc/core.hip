@@ -212,11 +212,15 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (int f = 0; f < TEA_NUM_FIELDS; ++f)
     if (c->field[f]) (void)hipFree(c->field[f]);
   if (c->p_alt) (void)hipFree(c->p_alt);
-  for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky,
-                   c->fcg_z, c->fcg_feat, c->fcg_rbar, c->fcg_scale, c->fcg_kc, c->fcg_dis, c->fcg_sig, c->fcg_tmp})
+  for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky, c->fcg_z})
     if (b) (void)hipFree(b);
-  if (c->fcg_part) (void)hipFree(c->fcg_part);
-  if (c->fcg_net) tea_nn_destroy(c->fcg_net);
+  if (c->fcg_cross) (void)hipFree(c->fcg_cross);
+  for (FcgLevel& L : c->fcg_lv) {
+    for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale})
+      if (b) (void)hipFree(b);
+    if (L.part) (void)hipFree(L.part);
+    if (L.net) tea_nn_destroy(L.net);
+  }
   if (c->host_partials) (void)hipHostFree(c->host_partials);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -171,42 +171,51 @@ __global__ __launch_bounds__(256) void pack_kernel(int hc, int wc,
   *reinterpret_cast<bf16x8*>(in0 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * 8) = o;
 }
 
-// z = r / diag + g * y[coarse cell] and z.w (g = scale[0]).
+// One coarse term of the learned preconditioner at a fine cell: g[0] times
+// y at the coarse cell ((kk-h) >> shift, (jj-h) >> shift).
+struct CoarseTerm {
+  const float* y;
+  const float* g;
+  int shift, wc;
+  __device__ double at(int kk, int jj, int h) const {
+    return (double)g[0] * (double)y[((jj - h) >> shift) * wc + ((kk - h) >> shift)];
+  }
+};
+
+// z = r / diag + the NL coarse terms, and z.w (span kernel body).
+template <int NL>
 struct ProlongBody {
   const float* r;
   const float* kx;
   const float* ky;
   const float* w;
-  const float* net;
-  const float* scale;
+  CoarseTerm t0, t1;
   float* z;
-  int x, h, c, wc;
+  int x, h;
   __device__ void operator()(int i, int kk, int jj, double (&out)[1]) const {
     const double d = (1.0 + ((double)kx[i + 1] + (double)kx[i])) +
                      ((double)ky[i + x] + (double)ky[i]);
-    const int X = (kk - h) / c, Y = (jj - h) / c;
-    const float zv = (float)((double)r[i] / d + (double)scale[0] * (double)net[Y * wc + X]);
+    double v = (double)r[i] / d;
+    if (NL > 0) v = v + t0.at(kk, jj, h);
+    if (NL > 1) v = v + t1.at(kk, jj, h);
+    const float zv = (float)v;
     z[i] = zv;
     out[0] = (double)zv * (double)w[i];
   }
 };
 
-// z = r / diag (+ g * y[coarse cell] with NET, g = scale[0]) and z.w, two cells per
-// lane (x even): the arithmetic of JacobiBody / ProlongBody cell by cell.
-// The coarse cell of interior cell (kk, jj) is ((kk-h) >> shift,
-// (jj-h) >> shift).
-template <bool NET>
+// The same, two cells per lane (x even): z = r / diag (+ the NL coarse
+// terms) and z.w, the arithmetic of JacobiBody / ProlongBody cell by cell.
+template <int NL>
 __global__ __launch_bounds__(kBlock) void fcg_z2_kernel(
     Geo g, const float* __restrict__ r, const float* __restrict__ kx,
-    const float* __restrict__ ky, const float* __restrict__ w,
-    const float* __restrict__ net, const float* __restrict__ scale,
-    float* __restrict__ z, int shift, int wc, double* partials, int nspan,
+    const float* __restrict__ ky, const float* __restrict__ w, CoarseTerm t0,
+    CoarseTerm t1, float* __restrict__ z, double* partials, int nspan,
     int single_group) {
   const int span = xcd_span(blockIdx.x, gridDim.x);
   const int base = span * kSpan;
   const int lane = threadIdx.x & (kWave - 1);
   const int x = g.x;
-  const double sc = NET ? (double)scale[0] : 0.0;
   double psum[1][kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) {
@@ -225,18 +234,49 @@ __global__ __launch_bounds__(kBlock) void fcg_z2_kernel(
     const double da = (1.0 + ((double)KX.y + (double)KX.x)) + ((double)KYt.x + (double)KY.x);
     const double db = (1.0 + ((double)kxr + (double)KX.y)) + ((double)KYt.y + (double)KY.y);
     float za = 0.0f, zb = 0.0f;
-    if (NET) {
-      const int Y = (row - g.h) >> shift;
-      if (in0) za = (float)((double)R.x / da + sc * (double)net[Y * wc + ((kk - g.h) >> shift)]);
-      if (in1) zb = (float)((double)R.y / db + sc * (double)net[Y * wc + ((kk + 1 - g.h) >> shift)]);
-    } else {
-      za = (float)((double)R.x / da);
-      zb = (float)((double)R.y / db);
+    if (in0) {
+      double v = (double)R.x / da;
+      if (NL > 0) v = v + t0.at(kk, row, g.h);
+      if (NL > 1) v = v + t1.at(kk, row, g.h);
+      za = (float)v;
+    }
+    if (in1) {
+      double v = (double)R.y / db;
+      if (NL > 0) v = v + t0.at(kk + 1, row, g.h);
+      if (NL > 1) v = v + t1.at(kk + 1, row, g.h);
+      zb = (float)v;
     }
     psum[0][k] = (in0 ? (double)za * (double)W.x : 0.0) + (in1 ? (double)zb * (double)W.y : 0.0);
     if (any) st2(z, e, in0, in1, za, zb);
   }
   span_reduce_pairs_n<1>(psum, partials, nspan, span, single_group != 0);
+}
+
+// Block means of a coarser level from a finer one (factor f) and the
+// per-block partial sums of (D^-1/2 rbar)^2 of the coarser level.
+__global__ __launch_bounds__(256) void restrict_coarse_kernel(
+    int wf, int f, int hc, int wc, const float* __restrict__ fine,
+    const float* __restrict__ dis, float* __restrict__ rbar, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  double v2 = 0.0;
+  if (idx < hc * wc) {
+    const int X = idx % wc, Y = idx / wc;
+    double sum = 0.0;
+    for (int b = 0; b < f; ++b)
+      for (int a = 0; a < f; ++a) sum += (double)fine[(Y * f + b) * wf + X * f + a];
+    const float m = (float)(sum / (double)(f * f));
+    rbar[idx] = m;
+    const double ms = (double)m * (double)dis[idx];
+    v2 = ms * ms;
+  }
+  sh[threadIdx.x] = v2;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 // The coarse operator T = P^T A P / c^2: for each coarse cell, the face
@@ -371,6 +411,91 @@ __global__ __launch_bounds__(256) void gamma_kernel(const double* __restrict__ p
   }
 }
 
+// Partial sums over level-1 cells of y1.(T1 Q y2): Q y2 is the level-2
+// correction on the level-1 grid (constant over blocks of f x f), T1 the
+// level-1 coarse operator.  With P2 = P1 Q, (P1 y1)^T A (P2 y2) =
+// c1^2 y1.(T1 Q y2): the cross term of the two coarse directions.
+__global__ __launch_bounds__(256) void cross_dots_kernel(int hc, int wc, int f, int wc2,
+                                                         const float* __restrict__ y1,
+                                                         const float* __restrict__ y2,
+                                                         const float* __restrict__ kc,
+                                                         double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (idx < hc * wc) {
+    const int X = idx % wc, Y = idx / wc;
+    const float* kxc = kc;
+    const float* kyc = kc + hc * wc;
+    auto q = [&](int xx, int yy) { return (double)y2[(yy / f) * wc2 + xx / f]; };
+    const double qc = q(X, Y);
+    const double kl = kxc[idx], kb = kyc[idx];
+    const double kr = X + 1 < wc ? (double)kxc[idx + 1] : 0.0;
+    const double kt = Y + 1 < hc ? (double)kyc[idx + wc] : 0.0;
+    double t = (1.0 + kl + kr + kb + kt) * qc;
+    if (X > 0) t -= kl * q(X - 1, Y);
+    if (X + 1 < wc) t -= kr * q(X + 1, Y);
+    if (Y > 0) t -= kb * q(X, Y - 1);
+    if (Y + 1 < hc) t -= kt * q(X, Y + 1);
+    v = (double)y1[idx] * t;
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Joint factors of the two coarse directions: minimise the energy norm of
+// e - g1 P1 y1 - g2 P2 y2, i.e. solve [B11 B12; B12 B22] g = a with
+// a_l = c_l^2 y_l.rbar_l, B_ll = c_l^2 y_l.T_l y_l, B12 = c1^2 y1.T1 Q y2
+// (the Gram matrix of the two directions in the A inner product, SPD when
+// they are independent).  Falls back to the separate factors a_l / B_ll
+// when it is not safely positive definite.  Writes scale1[1], scale2[1].
+__global__ __launch_bounds__(256) void gamma2_kernel(const double* __restrict__ p1, int n1,
+                                                     const double* __restrict__ p2, int n2,
+                                                     const double* __restrict__ px, int nx,
+                                                     double c1sq, double c2sq,
+                                                     float* __restrict__ scale1,
+                                                     float* __restrict__ scale2) {
+  __shared__ double sh[5][256];
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < n1; i += 256) {
+    v[0] += p1[i];
+    v[1] += p1[n1 + i];
+  }
+  for (int i = threadIdx.x; i < n2; i += 256) {
+    v[2] += p2[i];
+    v[3] += p2[n2 + i];
+  }
+  for (int i = threadIdx.x; i < nx; i += 256) v[4] += px[i];
+  for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double a1 = c1sq * sh[0][0], b11 = c1sq * sh[1][0];
+    const double a2 = c2sq * sh[2][0], b22 = c2sq * sh[3][0];
+    const double b12 = c1sq * sh[4][0];
+    const double det = b11 * b22 - b12 * b12;
+    double g1, g2;
+    if (b11 > 0.0 && b22 > 0.0 && det > 1e-12 * b11 * b22) {
+      g1 = (a1 * b22 - a2 * b12) / det;
+      g2 = (a2 * b11 - a1 * b12) / det;
+    } else {
+      g1 = b11 > 0.0 ? a1 / b11 : 0.0;
+      g2 = 0.0;
+    }
+    scale1[1] = isfinite(g1) ? (float)g1 : 0.0f;
+    scale2[1] = isfinite(g2) ? (float)g2 : 0.0f;
+  }
+}
+
 int ensure_fcg_buffers(tea_hip_chunk* c) {
   if (c->fcg_z) return TEA_HIP_OK;
   if (hipMalloc(&c->fcg_z, (size_t)c->geo.n * sizeof(float)) != hipSuccess) {
@@ -387,137 +512,192 @@ int ensure_fcg_buffers(tea_hip_chunk* c) {
 
 using namespace tea;
 
-// z = M(r) and the partials of z.w into `partials` (not waited for).
-static int launch_precondition(tea_hip_chunk* c, int use_net, double* partials) {
-  const Geo g = c->geo;
-  const int sg = c->ngroup == 1 ? 1 : 0;
-  if (!use_net) {
-    Timed t(c, K_FCG_JACOBI);
-    if (c->pairs) {
-      hipLaunchKernelGGL((fcg_z2_kernel<false>), span_grid(c), dim3(kBlock), 0,
-                         c->stream, g, c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, nullptr,
-                         nullptr, c->fcg_z, 0, 0, partials, c->nspan, sg);
+// Coarse pipeline of one level: the coarse residual (from r for level 0,
+// from the level above otherwise), its local scale, the network, and the
+// factor g of the coarse direction.
+static int run_level(tea_hip_chunk* c, int l) {
+  FcgLevel& L = c->fcg_lv[l];
+  const int cn = L.hc * L.wc;
+  const int nblk = (cn + 255) / 256;
+  {
+    Timed t(c, K_FCG_RESTRICT);
+    if (l == 0) {
+      hipLaunchKernelGGL(restrict_kernel, dim3(nblk), dim3(256), 0, c->stream, c->geo,
+                         L.c, L.hc, L.wc, c->mp_r, L.dis, L.rbar, L.part);
     } else {
-      JacobiBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, c->fcg_z, g.x};
-      hipLaunchKernelGGL((span_kernel<1, JacobiBody>), span_grid(c), dim3(kBlock), 0,
-                         c->stream, g, body, partials, c->nspan, sg);
+      const FcgLevel& F = c->fcg_lv[l - 1];
+      hipLaunchKernelGGL(restrict_coarse_kernel, dim3(nblk), dim3(256), 0, c->stream,
+                         F.wc, L.c / F.c, L.hc, L.wc, F.rbar, L.dis, L.rbar, L.part);
     }
-  } else {
-    const int cn = c->fcg_hc * c->fcg_wc;
-    const int nblk = (cn + 255) / 256;
-    {
-      Timed t(c, K_FCG_RESTRICT);
-      hipLaunchKernelGGL(restrict_kernel, dim3(nblk), dim3(256), 0, c->stream, g,
-                         c->fcg_c, c->fcg_hc, c->fcg_wc, c->mp_r, c->fcg_dis,
-                         c->fcg_rbar, c->fcg_part);
-      hipLaunchKernelGGL(rms_kernel, dim3(1), dim3(256), 0, c->stream, c->fcg_part, nblk,
-                         (long)cn, c->fcg_scale);
-      hipLaunchKernelGGL(box_rows_kernel, dim3(nblk), dim3(256), 0, c->stream,
-                         c->fcg_hc, c->fcg_wc, c->fcg_rbar, c->fcg_dis, c->fcg_tmp);
-      hipLaunchKernelGGL(box_cols_kernel, dim3(nblk), dim3(256), 0, c->stream,
-                         c->fcg_hc, c->fcg_wc, c->fcg_tmp, c->fcg_scale, c->fcg_sig);
-      hipLaunchKernelGGL(pack_kernel, dim3(nblk), dim3(256), 0, c->stream, c->fcg_hc,
-                         c->fcg_wc, c->fcg_rbar, c->fcg_dis, c->fcg_sig, c->fcg_feat,
-                         nn::input_tensor(c->fcg_net));
-    }
-    {
-      Timed t(c, K_FCG_NET);
-      TEA_RETURN_IF(nn::forward_on_stream(c->fcg_net, c->stream));
-      hipLaunchKernelGGL(unscale_kernel, dim3(nblk), dim3(256), 0, c->stream, cn, c->fcg_dis,
-                         c->fcg_sig, nn::output_mutable(c->fcg_net));
-      hipLaunchKernelGGL(coarse_dots_kernel, dim3(nblk), dim3(256), 0, c->stream,
-                         c->fcg_hc, c->fcg_wc, nn::output(c->fcg_net), c->fcg_rbar,
-                         c->fcg_kc, c->fcg_part);
-      hipLaunchKernelGGL(gamma_kernel, dim3(1), dim3(256), 0, c->stream, c->fcg_part,
-                         nblk, c->fcg_scale);
-    }
-    {
-      Timed t(c, K_FCG_PROLONG);
-      if (c->pairs) {
-        int shift = 0;
-        while ((1 << shift) < c->fcg_c) ++shift;
-        hipLaunchKernelGGL((fcg_z2_kernel<true>), span_grid(c), dim3(kBlock), 0,
-                           c->stream, g, c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
-                           nn::output(c->fcg_net), c->fcg_scale + 1, c->fcg_z, shift,
-                           c->fcg_wc, partials, c->nspan, sg);
-      } else {
-        ProlongBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w,
-                         nn::output(c->fcg_net), c->fcg_scale + 1, c->fcg_z,
-                         g.x, g.h, c->fcg_c, c->fcg_wc};
-        hipLaunchKernelGGL((span_kernel<1, ProlongBody>), span_grid(c), dim3(kBlock), 0,
-                           c->stream, g, body, partials, c->nspan, sg);
-      }
-    }
+    hipLaunchKernelGGL(rms_kernel, dim3(1), dim3(256), 0, c->stream, L.part, nblk,
+                       (long)cn, L.scale);
+    hipLaunchKernelGGL(box_rows_kernel, dim3(nblk), dim3(256), 0, c->stream, L.hc, L.wc,
+                       L.rbar, L.dis, L.tmp);
+    hipLaunchKernelGGL(box_cols_kernel, dim3(nblk), dim3(256), 0, c->stream, L.hc, L.wc,
+                       L.tmp, L.scale, L.sig);
+    hipLaunchKernelGGL(pack_kernel, dim3(nblk), dim3(256), 0, c->stream, L.hc, L.wc,
+                       L.rbar, L.dis, L.sig, L.feat, nn::input_tensor(L.net));
+  }
+  {
+    Timed t(c, K_FCG_NET);
+    TEA_RETURN_IF(nn::forward_on_stream(L.net, c->stream));
+    hipLaunchKernelGGL(unscale_kernel, dim3(nblk), dim3(256), 0, c->stream, cn, L.dis,
+                       L.sig, nn::output_mutable(L.net));
+    hipLaunchKernelGGL(coarse_dots_kernel, dim3(nblk), dim3(256), 0, c->stream, L.hc,
+                       L.wc, nn::output(L.net), L.rbar, L.kc, L.part);
+    hipLaunchKernelGGL(gamma_kernel, dim3(1), dim3(256), 0, c->stream, L.part, nblk,
+                       L.scale);
   }
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
 }
 
+static CoarseTerm coarse_term(const tea_hip_chunk* c, int l) {
+  const FcgLevel& L = c->fcg_lv[l];
+  int shift = 0;
+  while ((1 << shift) < L.c) ++shift;
+  return CoarseTerm{nn::output(L.net), L.scale + 1, shift, L.wc};
+}
+
+// z = M(r) and the partials of z.w into `partials` (not waited for).
+static int launch_precondition(tea_hip_chunk* c, int use_net, double* partials) {
+  const Geo g = c->geo;
+  const int sg = c->ngroup == 1 ? 1 : 0;
+  const int nl = use_net ? c->fcg_levels : 0;
+  for (int l = 0; l < nl; ++l) TEA_RETURN_IF(run_level(c, l));
+  if (nl == 2) {
+    FcgLevel& L1 = c->fcg_lv[0];
+    FcgLevel& L2 = c->fcg_lv[1];
+    const int n1 = (L1.hc * L1.wc + 255) / 256, n2 = (L2.hc * L2.wc + 255) / 256;
+    Timed t(c, K_FCG_NET);
+    hipLaunchKernelGGL(cross_dots_kernel, dim3(n1), dim3(256), 0, c->stream, L1.hc, L1.wc,
+                       L2.c / L1.c, L2.wc, nn::output(L1.net), nn::output(L2.net), L1.kc,
+                       c->fcg_cross);
+    hipLaunchKernelGGL(gamma2_kernel, dim3(1), dim3(256), 0, c->stream, L1.part, n1,
+                       L2.part, n2, c->fcg_cross, n1, (double)L1.c * L1.c,
+                       (double)L2.c * L2.c, L1.scale, L2.scale);
+    TEA_HIP_TRY(hipGetLastError());
+  }
+  const CoarseTerm none{nullptr, nullptr, 0, 0};
+  const CoarseTerm t0 = nl > 0 ? coarse_term(c, 0) : none;
+  const CoarseTerm t1 = nl > 1 ? coarse_term(c, 1) : none;
+  Timed t(c, nl == 0 ? K_FCG_JACOBI : K_FCG_PROLONG);
+  if (c->pairs) {
+    if (nl == 0)
+      hipLaunchKernelGGL((fcg_z2_kernel<0>), span_grid(c), dim3(kBlock), 0, c->stream, g,
+                         c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, partials,
+                         c->nspan, sg);
+    else if (nl == 1)
+      hipLaunchKernelGGL((fcg_z2_kernel<1>), span_grid(c), dim3(kBlock), 0, c->stream, g,
+                         c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, partials,
+                         c->nspan, sg);
+    else
+      hipLaunchKernelGGL((fcg_z2_kernel<2>), span_grid(c), dim3(kBlock), 0, c->stream, g,
+                         c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, partials,
+                         c->nspan, sg);
+  } else if (nl == 0) {
+    JacobiBody body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, c->fcg_z, g.x};
+    hipLaunchKernelGGL((span_kernel<1, JacobiBody>), span_grid(c), dim3(kBlock), 0,
+                       c->stream, g, body, partials, c->nspan, sg);
+  } else if (nl == 1) {
+    ProlongBody<1> body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, g.x, g.h};
+    hipLaunchKernelGGL((span_kernel<1, ProlongBody<1>>), span_grid(c), dim3(kBlock), 0,
+                       c->stream, g, body, partials, c->nspan, sg);
+  } else {
+    ProlongBody<2> body{c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, g.x, g.h};
+    hipLaunchKernelGGL((span_kernel<1, ProlongBody<2>>), span_grid(c), dim3(kBlock), 0,
+                       c->stream, g, body, partials, c->nspan, sg);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
+static void free_level(FcgLevel& L) {
+  for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale})
+    if (b) (void)hipFree(b);
+  if (L.part) (void)hipFree(L.part);
+  if (L.net) tea_nn_destroy(L.net);
+  L = FcgLevel();
+}
 
 extern "C" {
 
 int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
-                       const float* feature_norm, int coarsen) {
+                       const float* feature_norm, int coarsen, int levels) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_RETURN_IF(ensure_fcg_buffers(c));
+  for (FcgLevel& L : c->fcg_lv) free_level(L);
+  if (c->fcg_cross) (void)hipFree(c->fcg_cross);
+  c->fcg_cross = nullptr;
+  c->fcg_levels = 0;
   if (!params) return TEA_HIP_OK;  // Jacobi only
   TEA_CHECK_ARG(feature_norm != nullptr, "null feature normalisation");
   TEA_CHECK_ARG(coarsen >= 1 && (coarsen & (coarsen - 1)) == 0,
                 "coarsening factor must be a power of two");
+  TEA_CHECK_ARG(levels >= 1 && levels <= kFcgMaxLevels, "levels must be 1 or 2");
+  for (int q = 0; q < 4; ++q) c->fcg_norm[q] = feature_norm[q];
+  TEA_CHECK_ARG(c->fcg_norm[1] > 0.0f && c->fcg_norm[3] > 0.0f,
+                "feature scales must be positive");
   const int nx = c->geo.x - 2 * c->geo.h, ny = c->geo.y - 2 * c->geo.h;
   TEA_CHECK_ARG(nx % coarsen == 0 && ny % coarsen == 0,
                 "the interior must be a multiple of the coarsening factor");
-  const int hc = ny / coarsen, wc = nx / coarsen;
-  TEA_CHECK_ARG(wc % 32 == 0, "coarse width must be a multiple of 32");
-  if (c->fcg_net) {
-    tea_nn_destroy(c->fcg_net);
-    c->fcg_net = nullptr;
+  TEA_CHECK_ARG((nx / coarsen) % 32 == 0, "coarse width must be a multiple of 32");
+  // Level l coarsens by coarsen * 4^l; a level is added while the grid
+  // divides and its width stays a multiple of 32.
+  for (int l = 0; l < levels; ++l) {
+    const int cl = coarsen << (2 * l);
+    if (nx % cl != 0 || ny % cl != 0 || (nx / cl) % 32 != 0) break;
+    FcgLevel& L = c->fcg_lv[l];
+    L.c = cl;
+    L.hc = ny / cl;
+    L.wc = nx / cl;
+    const size_t cn = (size_t)L.hc * L.wc;
+    const size_t nblk = (cn + 255) / 256;
+    TEA_RETURN_IF(tea_nn_create(c->device, 1, L.hc, L.wc, &L.net));
+    TEA_RETURN_IF(tea_nn_set_params(L.net, params, count));
+    if (hipMalloc(&L.feat, 2 * cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.kc, 2 * cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.dis, cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.sig, cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.tmp, cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.rbar, cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.part, 2 * nblk * sizeof(double)) != hipSuccess ||
+        hipMalloc(&L.scale, 2 * sizeof(float)) != hipSuccess) {
+      set_error("hipMalloc of the preconditioner's coarse buffers failed");
+      return TEA_HIP_ENOMEM;
+    }
+    c->fcg_levels = l + 1;
   }
-  TEA_RETURN_IF(tea_nn_create(c->device, 1, hc, wc, &c->fcg_net));
-  TEA_RETURN_IF(tea_nn_set_params(c->fcg_net, params, count));
-  c->fcg_c = coarsen;
-  c->fcg_hc = hc;
-  c->fcg_wc = wc;
-  for (int q = 0; q < 4; ++q) c->fcg_norm[q] = feature_norm[q];
-  TEA_CHECK_ARG(c->fcg_norm[1] > 0.0f && c->fcg_norm[3] > 0.0f, "feature scales must be positive");
-  const size_t cn = (size_t)hc * wc;
-  const size_t nblk = (cn + 255) / 256;
-  if (c->fcg_feat) (void)hipFree(c->fcg_feat);
-  if (c->fcg_rbar) (void)hipFree(c->fcg_rbar);
-  if (c->fcg_part) (void)hipFree(c->fcg_part);
-  if (c->fcg_scale) (void)hipFree(c->fcg_scale);
-  if (c->fcg_kc) (void)hipFree(c->fcg_kc);
-  if (c->fcg_dis) (void)hipFree(c->fcg_dis);
-  if (c->fcg_sig) (void)hipFree(c->fcg_sig);
-  if (c->fcg_tmp) (void)hipFree(c->fcg_tmp);
-  c->fcg_feat = c->fcg_rbar = c->fcg_scale = c->fcg_kc = c->fcg_dis = nullptr;
-  c->fcg_sig = c->fcg_tmp = nullptr;
-  c->fcg_part = nullptr;
-  if (hipMalloc(&c->fcg_feat, 2 * cn * sizeof(float)) != hipSuccess ||
-      hipMalloc(&c->fcg_kc, 2 * cn * sizeof(float)) != hipSuccess ||
-      hipMalloc(&c->fcg_dis, cn * sizeof(float)) != hipSuccess ||
-      hipMalloc(&c->fcg_sig, cn * sizeof(float)) != hipSuccess ||
-      hipMalloc(&c->fcg_tmp, cn * sizeof(float)) != hipSuccess ||
-      hipMalloc(&c->fcg_rbar, cn * sizeof(float)) != hipSuccess ||
-      hipMalloc(&c->fcg_part, 2 * nblk * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->fcg_scale, 2 * sizeof(float)) != hipSuccess) {
-    set_error("hipMalloc of the preconditioner's coarse buffers failed");
-    return TEA_HIP_ENOMEM;
+  if (c->fcg_levels == 2) {
+    const size_t n1 = ((size_t)c->fcg_lv[0].hc * c->fcg_lv[0].wc + 255) / 256;
+    if (hipMalloc(&c->fcg_cross, n1 * sizeof(double)) != hipSuccess) {
+      c->fcg_cross = nullptr;
+      set_error("hipMalloc of the preconditioner's cross partials failed");
+      return TEA_HIP_ENOMEM;
+    }
   }
   return TEA_HIP_OK;
 }
 
-// Coarse features from the FP32 coefficients (after tea_hip_mpcg_init).
+int tea_hip_fcg_levels(tea_hip_chunk* c, int* levels) {
+  TEA_CHECK_ARG(c != nullptr && levels != nullptr, "null argument");
+  *levels = c->fcg_levels;
+  return TEA_HIP_OK;
+}
+
+// Coarse operators and features of every level from the FP32 coefficients
+// (after tea_hip_mpcg_init).
 static int fcg_features(tea_hip_chunk* c) {
-  if (!c->fcg_net) return TEA_HIP_OK;
-  const int cn = c->fcg_hc * c->fcg_wc;
-  const dim3 grid((cn + 255) / 256);
-  hipLaunchKernelGGL(coarse_coef_kernel, grid, dim3(256), 0, c->stream, c->geo, c->fcg_c,
-                     c->fcg_hc, c->fcg_wc, c->mp_kx, c->mp_ky, c->fcg_kc);
-  hipLaunchKernelGGL(feature_kernel, grid, dim3(256), 0, c->stream, c->fcg_hc, c->fcg_wc,
-                     c->fcg_kc, c->fcg_norm[0], c->fcg_norm[1], c->fcg_norm[2],
-                     c->fcg_norm[3], c->fcg_feat, c->fcg_dis);
+  for (int l = 0; l < c->fcg_levels; ++l) {
+    FcgLevel& L = c->fcg_lv[l];
+    const dim3 grid(((unsigned)(L.hc * L.wc) + 255) / 256);
+    hipLaunchKernelGGL(coarse_coef_kernel, grid, dim3(256), 0, c->stream, c->geo, L.c,
+                       L.hc, L.wc, c->mp_kx, c->mp_ky, L.kc);
+    hipLaunchKernelGGL(feature_kernel, grid, dim3(256), 0, c->stream, L.hc, L.wc, L.kc,
+                       c->fcg_norm[0], c->fcg_norm[1], c->fcg_norm[2], c->fcg_norm[3],
+                       L.feat, L.dis);
+  }
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
 }
@@ -534,7 +714,7 @@ int tea_hip_fcg_precondition(tea_hip_chunk* c, int use_net, double* zw) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(c->fcg_z != nullptr && c->mp_r != nullptr, "FCG not initialised");
   TEA_CHECK_ARG(zw != nullptr, "null zw");
-  TEA_CHECK_ARG(!use_net || c->fcg_net != nullptr, "no network attached");
+  TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no network attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_RETURN_IF(launch_precondition(c, use_net, c->partials));
   return finish_reduction(c, 1, zw);
@@ -545,7 +725,7 @@ int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(c->fcg_z != nullptr && c->mp_r != nullptr, "FCG not initialised");
   TEA_CHECK_ARG(rr != nullptr && zw != nullptr, "null outputs");
-  TEA_CHECK_ARG(!use_net || c->fcg_net != nullptr, "no network attached");
+  TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no network attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_RETURN_IF(launch_mp_r(c, alpha, c->partials));
   TEA_RETURN_IF(launch_precondition(c, use_net, c->partials + c->nspan));

@@ -316,9 +316,9 @@ void run_read_u(Chunk* chunk, Settings* settings, double* host) {
 }
 
 void run_fcg_attach(Chunk* chunk, Settings* settings, const float* params,
-                    size_t count, const float* norm, int coarsen) {
+                    size_t count, const float* norm, int coarsen, int levels) {
   (void)settings;
-  HIP_CALL(tea_hip_fcg_attach(H(chunk), params, count, norm, coarsen));
+  HIP_CALL(tea_hip_fcg_attach(H(chunk), params, count, norm, coarsen, levels));
 }
 
 void run_fcg_init(Chunk* chunk, Settings* settings, double rx, double ry,

@@ -365,6 +365,21 @@ float* output_mutable(tea_nn* n);
 // ---------------------------------------------------------------------------
 // The chunk handle.
 
+// One coarse level of the learned preconditioner.
+struct FcgLevel {
+  int c = 0, hc = 0, wc = 0;  // coarsening factor, coarse rows and columns
+  tea_nn* net = nullptr;      // the network, sized hc x wc
+  float* feat = nullptr;      // [2][hc][wc] normalised features
+  float* rbar = nullptr;      // [hc][wc] block means of r
+  float* kc = nullptr;        // [2][hc][wc] coarse left / bottom face coefficients
+  float* dis = nullptr;       // [hc][wc] D^-1/2 of the coarse operator
+  float* sig = nullptr;       // [hc][wc] local input scale of the network
+  float* tmp = nullptr;       // [hc][wc] row pass of the local scale
+  double* part = nullptr;     // per-block partial sums (2 * blocks)
+  float* scale = nullptr;     // {RMS of D^-1/2 rbar, prolongation factor}
+};
+constexpr int kFcgMaxLevels = 2;
+
 struct tea_hip_chunk {
   int device = 0;
   tea::Geo geo{};
@@ -401,21 +416,14 @@ struct tea_hip_chunk {
   float* mp_kx = nullptr;
   float* mp_ky = nullptr;
 
-  // Flexible CG: the preconditioned residual z (FP32), the network on the
-  // coarse grid (coarsening factor fcg_c), its input features and the
-  // device-side scale of the coarse residual.
+  // Flexible CG: the preconditioned residual z (FP32) and, for the learned
+  // preconditioner, up to kFcgMaxLevels coarse levels (coarsening c, 4c),
+  // each with its network instance, coarse operator and work arrays.
   float* fcg_z = nullptr;
-  tea_nn* fcg_net = nullptr;
-  int fcg_c = 0, fcg_hc = 0, fcg_wc = 0;
   float fcg_norm[4] = {0.0f, 1.0f, 0.0f, 1.0f};
-  float* fcg_feat = nullptr;   // [2][hc][wc]
-  float* fcg_rbar = nullptr;   // [hc][wc]
-  float* fcg_kc = nullptr;     // [2][hc][wc] coarse left / bottom face coefficients
-  float* fcg_dis = nullptr;    // [hc][wc] D^-1/2 of the coarse operator
-  float* fcg_sig = nullptr;    // [hc][wc] local input scale of the network
-  float* fcg_tmp = nullptr;    // [hc][wc] row pass of the local scale
-  double* fcg_part = nullptr;  // per-block partial sums (2 * blocks)
-  float* fcg_scale = nullptr;  // {s, prolongation factor}
+  int fcg_levels = 0;
+  FcgLevel fcg_lv[kFcgMaxLevels];
+  double* fcg_cross = nullptr;  // partials of the two levels' cross term
 
   // Timing.
   bool timing = false;

@@ -292,9 +292,9 @@ void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
         die(__LINE__, __FILE__, "network file %s: %s\n",
             g_extra.fcg_weights.c_str(), why.c_str());
       run_fcg_attach(c, settings, nw.params.data(), nw.params.size(), nw.norm,
-                     nw.coarsen);
+                     nw.coarsen, g_extra.fcg_levels);
     } else {
-      run_fcg_attach(c, settings, NULL, 0, NULL, 0);
+      run_fcg_attach(c, settings, NULL, 0, NULL, 0, 0);
     }
     attached = c;
   }

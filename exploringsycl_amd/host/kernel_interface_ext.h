@@ -26,7 +26,7 @@ void run_mpcg_refresh(Chunk* chunk, Settings* settings, double alpha,
  * network's coarse-grid correction.
  *   run_fcg_attach: allocates the FCG state; params != NULL loads the
  *     network (count parameters, feature normalisation norm[4], coarsening
- *     factor coarsen).
+ *     factor coarsen) on up to `levels` coarse levels (coarsen * 4^l).
  *   run_fcg_init: as run_mpcg_init, plus the network's coefficient features.
  *   run_fcg_precondition: z = M(r) (network if use_net, else Jacobi); zw +=
  *     z.w.
@@ -37,7 +37,7 @@ void run_mpcg_refresh(Chunk* chunk, Settings* settings, double alpha,
  * The residual update and the FP64 refresh are run_mpcg_calc_r and
  * run_mpcg_refresh. */
 void run_fcg_attach(Chunk* chunk, Settings* settings, const float* params,
-                    size_t count, const float* norm, int coarsen);
+                    size_t count, const float* norm, int coarsen, int levels);
 void run_fcg_init(Chunk* chunk, Settings* settings, double rx, double ry,
                   double* rr, double* bb);
 void run_fcg_precondition(Chunk* chunk, Settings* settings, int use_net,

@@ -25,7 +25,10 @@ def run_deck(deck, binary=TEALEAF_BIN, workdir=None, extra_args=(), env=None,
     try:
         deck.write(os.path.join(workdir, "tea.in"))
         if problems and os.path.exists(problems):
-            shutil.copy(problems, os.path.join(workdir, "tea.problems"))
+            dst = os.path.join(workdir, "tea.problems")
+            if os.path.lexists(dst):
+                os.remove(dst)
+            shutil.copyfile(problems, dst)
         cmd = [binary, "--json", "run.json", "--quiet"] + list(extra_args)
         e = dict(os.environ)
         if env:

@@ -262,19 +262,27 @@ int tea_nn_train_epoch(tea_nn* net, const int* order, int count, int batch,
 /* ---- flexible CG with the learned preconditioner ------------------------
  * No counterpart in the reference.  FP32 p, r, w, z; FP64 u, dots, scalars;
  * FP64 true-residual refresh with tea_hip_mpcg_refresh.  Preconditioners:
- * Jacobi, z = r / diag(A); learned, z = r / diag(A) + s P(N(R r / s, f)),
- * with R the c x c block mean, P its piecewise-constant transpose, s the RMS
- * of R r and N the network on the coarse grid (input features f: coarse
- * means of log kx and log ky, normalised by feature_norm = {mean_x, std_x,
- * mean_y, std_y}).  An iteration: calc_w (p = z + beta p_old, w = A p,
+ * Jacobi, z = r / diag(A); learned, z = r / diag(A) + sum over levels of
+ * g P y, y = D^-1/2 sigma N(D^-1/2 R r / sigma, f), with R the block mean of
+ * the level, P its piecewise-constant prolongation, T = P^T A P / c^2 the
+ * Galerkin coarse operator, D its diagonal, sigma the local RMS of
+ * D^-1/2 R r (13 x 13 window, at least 1e-3 of its RMS over the grid), N
+ * the network, f the couplings of D^-1/2 T D^-1/2 to the left and bottom
+ * neighbours normalised by feature_norm = {mean_x, std_x, mean_y, std_y},
+ * and g = (y.Rr) / (y.T y).  An iteration: calc_w (p = z + beta p_old, w = A p,
  * p.w and p.r, previous alpha applied to u), alpha = p.r / p.w,
  * tea_hip_mpcg_calc_r, precondition (z = M(r), z.w), beta = -z.w / p.w. */
 
 /* Allocates the FCG buffers; with params != NULL also loads the network
- * (tea_nn_num_params() values) for a coarsening factor `coarsen` (the
- * interior must divide by it and the coarse width be a multiple of 32). */
+ * (tea_nn_num_params() values) on up to `levels` (1 or 2) coarse levels,
+ * level l coarsening by coarsen * 4^l: a level is used while the interior
+ * divides by its factor and its coarse width is a multiple of 32 (the first
+ * level must be).  coarsen must be a power of two. */
 int tea_hip_fcg_attach(tea_hip_chunk* chunk, const float* params,
-                       size_t count, const float* feature_norm, int coarsen);
+                       size_t count, const float* feature_norm, int coarsen,
+                       int levels);
+/* The number of coarse levels the network runs on (0: Jacobi only). */
+int tea_hip_fcg_levels(tea_hip_chunk* chunk, int* levels);
 /* tea_hip_mpcg_init plus the network's coarse input features. */
 int tea_hip_fcg_init(tea_hip_chunk* chunk, int coefficient, double rx,
                      double ry, double* rr, double* bb);

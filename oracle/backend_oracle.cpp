@@ -247,8 +247,8 @@ void run_read_u(Chunk* c, Settings* s, double* host) {
 }
 
 void run_fcg_attach(Chunk* c, Settings* s, const float* params, size_t count,
-                    const float* norm, int coarsen) {
-  (void)s; (void)count; (void)norm; (void)coarsen;
+                    const float* norm, int coarsen, int levels) {
+  (void)s; (void)count; (void)norm; (void)coarsen; (void)levels;
   if (params)
     die(__LINE__, __FILE__, "the oracle backend has no network preconditioner\n");
   if (!c->ext->fcg_z)

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 H = 2
 
 
-def _fcg_chunk(n, seed, params, norm):
+def _fcg_chunk(n, seed, params, norm, levels=2):
     from exploringsycl_amd.chunk import DeviceChunk
     rng = np.random.default_rng(seed)
     X = n + 2 * H
@@ -28,7 +28,7 @@ def _fcg_chunk(n, seed, params, norm):
     ener = rng.uniform(0.1, 25.0, (X, X))
     c.write("density", dens)
     c.write("energy", ener)
-    c.fcg_attach(params, norm, learned.COARSEN)
+    c.fcg_attach(params, norm, learned.COARSEN, levels)
     dx = 10.0 / 4096
     rx = learned.DT / (dx * dx)
     rr, bb = c.fcg_init(2, rx, rx)
@@ -39,10 +39,65 @@ def _bf(v):
     return O.bf16(np.asarray(v, dtype=np.float32)).astype(np.float64)
 
 
-@pytest.mark.parametrize("n", [32 * learned.COARSEN, 64 * learned.COARSEN])
-def test_learned_preconditioner_matches_numpy(n):
-    """z = r/diag + s P(N(R r/s, f)) and z.w against restrict + the network
-    oracle + prolong in numpy on the device's own FP32 r, kx, ky, w."""
+def _restated_levels(params, norm, r, kx, ky, n, levels):
+    """The learned preconditioner's coarse terms restated in numpy on the
+    device's own FP32 r, kx, ky: per level l (coarsening c = COARSEN * 4^l
+    while the coarse width stays a multiple of 32) the block means, local
+    scale, network oracle, unscaling and energy-optimal factor g; returns
+    [(c, g, y)]."""
+    out = []
+    prev = None
+    for l in range(levels):
+        cc = learned.COARSEN << (2 * l)
+        if n % cc or (n // cc) % 32:
+            break
+        m = n // cc
+        if prev is None:
+            ri = r[H:-H, H:-H]
+            rbar = ri.reshape(m, cc, m, cc).sum(axis=(1, 3)) / (cc * cc)
+        else:
+            rbar = prev.reshape(m, 4, m, 4).sum(axis=(1, 3)) / 16.0
+        rbar = rbar.astype(np.float32).astype(np.float64)
+        prev = rbar
+        kxc, kyc = learned.coarse_faces(kx[H:H + n, H:H + n + 1], ky[H:H + n + 1, H:H + n], cc)
+        dis = learned.coarse_scaling(kxc, kyc).astype(np.float32)
+        rs = (np.float32(rbar) * dis).astype(np.float64)
+        sig = np.maximum(learned.local_scale(rs), learned.FLOOR * np.sqrt(np.mean(rs * rs)))
+        sig = sig.astype(np.float32)
+        g0, g1 = learned.coarse_features(kxc, kyc)
+        f0 = (g0 - norm[0]) / norm[1]
+        f1 = (g1 - norm[2]) / norm[3]
+        x = np.stack([(np.float32(rbar) * dis) / sig, f0, f1], axis=-1)[None]
+        o, _ = O.forward(params, x.astype(np.float32), bf16_mode=True)
+        y = (o[0] * (sig * dis)).astype(np.float64)
+        kl = kxc[:, :-1].astype(np.float32).astype(np.float64)
+        kr = kxc[:, 1:].astype(np.float32).astype(np.float64)
+        kb = kyc[:-1, :].astype(np.float32).astype(np.float64)
+        kt = kyc[1:, :].astype(np.float32).astype(np.float64)
+        def T(v, kl=kl, kr=kr, kb=kb, kt=kt):
+            vp = np.pad(v, 1)
+            return ((1.0 + kl + kr + kb + kt) * v - kl * vp[1:-1, :-2] - kr * vp[1:-1, 2:]
+                    - kb * vp[:-2, 1:-1] - kt * vp[2:, 1:-1])
+        out.append(dict(c=cc, y=y, a=cc * cc * (y * rbar).sum(), b=cc * cc * (y * T(y)).sum(),
+                        T=T))
+    if len(out) == 2:
+        # Joint factors: the 2 x 2 Galerkin system of the two directions.
+        l1, l2 = out
+        qy2 = np.kron(l2["y"], np.ones((4, 4)))
+        b12 = l1["c"] ** 2 * (l1["y"] * l1["T"](qy2)).sum()
+        det = l1["b"] * l2["b"] - b12 * b12
+        g = np.linalg.solve([[l1["b"], b12], [b12, l2["b"]]], [l1["a"], l2["a"]])
+        assert det > 1e-12 * l1["b"] * l2["b"]
+        return [(l1["c"], g[0], l1["y"]), (l2["c"], g[1], l2["y"])]
+    return [(t["c"], t["a"] / t["b"], t["y"]) for t in out]
+
+
+@pytest.mark.parametrize("n,levels", [(32 * learned.COARSEN, 2), (64 * learned.COARSEN, 1),
+                                      (128 * learned.COARSEN, 2)])
+def test_learned_preconditioner_matches_numpy(n, levels):
+    """z = r/diag + sum over levels of g P y and z.w against the numpy
+    restatement with the network oracle, on the device's own FP32 r, kx,
+    ky, w (256^2 and 512^2 have one level, 1024^2 two)."""
     params = learned.initial_params(3)
     # Non-zero biases so every term is exercised.
     rng = np.random.default_rng(4)
@@ -52,8 +107,9 @@ def test_learned_preconditioner_matches_numpy(n):
         params[k:k + o] = rng.uniform(-0.05, 0.05, o)
         k += o
     norm = np.array([0.2, 0.1, 0.2, 0.1], dtype=np.float32)
-    c, _, _ = _fcg_chunk(n, n, params, norm)
+    c, _, _ = _fcg_chunk(n, n, params, norm, levels)
     try:
+        used = c.fcg_levels()
         # One direction, p = Jacobi(r) and w = A p, so that w is not zero.
         c.fcg_precondition(False)
         c.fcg_calc_w(0.0, False, 0.0)
@@ -65,35 +121,17 @@ def test_learned_preconditioner_matches_numpy(n):
         w = c.read_mp("w").astype(np.float64)
     finally:
         c.close()
-    cc = learned.COARSEN
+    terms = _restated_levels(params, norm, r, kx, ky, n, levels)
+    assert used == len(terms)
     ri = r[H:-H, H:-H]
-    rbar = ri.reshape(n // cc, cc, n // cc, cc).sum(axis=(1, 3)) / (cc * cc)
-    rbar = rbar.astype(np.float32).astype(np.float64)
-    kxc, kyc = learned.coarse_faces(kx[H:H + n, H:H + n + 1], ky[H:H + n + 1, H:H + n], cc)
-    dis = learned.coarse_scaling(kxc, kyc).astype(np.float32)
-    rs = (np.float32(rbar) * dis).astype(np.float64)
-    sig = np.maximum(learned.local_scale(rs), learned.FLOOR * np.sqrt(np.mean(rs * rs)))
-    sig = sig.astype(np.float32)
-    g0, g1 = learned.coarse_features(kxc, kyc)
-    f0 = (g0 - norm[0]) / norm[1]
-    f1 = (g1 - norm[2]) / norm[3]
-    x = np.stack([(np.float32(rbar) * dis) / sig, f0, f1], axis=-1)[None]
-    out, _ = O.forward(params, x.astype(np.float32), bf16_mode=True)
-    y = (out[0] * (sig * dis)).astype(np.float64)
-    # Energy-optimal factor of the coarse direction: (y.Rr) / (y.T y).
-    kl = kxc[:, :-1].astype(np.float32).astype(np.float64)
-    kr = kxc[:, 1:].astype(np.float32).astype(np.float64)
-    kb = kyc[:-1, :].astype(np.float32).astype(np.float64)
-    kt = kyc[1:, :].astype(np.float32).astype(np.float64)
-    yp = np.pad(y, 1)
-    ty = ((1.0 + kl + kr + kb + kt) * y - kl * yp[1:-1, :-2] - kr * yp[1:-1, 2:]
-          - kb * yp[:-2, 1:-1] - kt * yp[2:, 1:-1])
-    gamma = (y * rbar).sum() / (y * ty).sum()
     e = H + n  # one past the last interior index
     d = (1.0 + (kx[H:e, H + 1:e + 1] + kx[H:e, H:e])) + (ky[H + 1:e + 1, H:e] + ky[H:e, H:e])
-    want = ri / d + float(np.float32(gamma)) * np.kron(y, np.ones((cc, cc)))
+    want = ri / d
+    corr = 0.0
+    for cc, gamma, y in terms:
+        want = want + float(np.float32(gamma)) * np.kron(y, np.ones((cc, cc)))
+        corr += abs(gamma) * np.abs(y).max()
     got = z[H:-H, H:-H]
-    corr = abs(gamma) * np.abs(y).max()
     assert np.abs(got - want).max() <= 2e-2 * corr + 1e-6 * np.abs(want).max()
     want_zw = (got * w[H:-H, H:-H]).sum()
     assert abs(zw - want_zw) <= 1e-9 * np.abs(got * w[H:-H, H:-H]).sum()
