@@ -119,7 +119,7 @@ def time_to_solution(n, timeout=1200):
         solvers.append(("learned_fcg", ["use_fcg", "fcg_weights=net.bin"]))
     out = {"workload": (f"{n}x{n} cells, 64 seeded rectangles (log10 density "
                         f"U[-2,2], seed {seed}) + tea.in states, K=1/density, "
-                        "1 timestep, rel. residual 1e-10 (max 40000 iterations), "
+                        "1 timestep, rel. residual 1e-10 (max 12000 iterations), "
                         "after 1 warm-up step"),
            "solvers": {}}
     fields = {}
@@ -127,7 +127,7 @@ def time_to_solution(n, timeout=1200):
         for name, extra in solvers:
             deck = Deck.with_random_rectangles(
                 n, n, 64, seed, density="log10", density_range=(-2.0, 2.0),
-                end_step=1, coefficient="inverse", max_iters=40000,
+                end_step=1, coefficient="inverse", max_iters=12000,
                 extra=["rel_eps=1e-10"] + extra)
             wd = os.path.join(root, name)
             os.makedirs(wd)
