@@ -304,6 +304,149 @@ __device__ inline void span_reduce_pairs(double (&v)[kSlots], double* partials,
   span_reduce_pairs_n<1>(vv, partials, nspan, span, single_group);
 }
 
+// One cell pair of the fused iteration: its indices, its loads and its
+// arithmetic, shared by the span-per-block and pair-per-thread kernels so
+// both evaluate every cell identically.
+struct FusedPair {
+  int e0, kk, jj;
+  bool in0, in1, any;
+  double2 P0, Pt, Pb, R0, Rt, Rb, KX, KY, KYt, U;
+};
+
+__device__ inline void fused_pair_index(const Geo& g, int e, FusedPair& s) {
+  const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+  s.e0 = e;
+  s.jj = row;
+  s.kk = e - row * g.x;
+  s.in0 = e < g.n && interior(g, s.kk, row);
+  s.in1 = e < g.n && interior(g, s.kk + 1, row);
+  s.any = s.in0 || s.in1;
+}
+
+// The pair's own p, r and kx are also the left/right neighbours the adjacent
+// lanes take by shuffle, so they are read whenever the pair is in range; the
+// rows above and below only for interior pairs.  x + 2 is an even index whose
+// +-x rows are in range.
+template <bool APPLY_P>
+__device__ inline void fused_pair_load(const Geo& g, const FusedWArgs& a,
+                                       FusedPair& s) {
+  const int x = g.x;
+  const double* __restrict__ P = a.p_old;
+  const double* __restrict__ R = a.r;
+  const int ic = s.e0 < g.n ? s.e0 : x + 2;
+  const int ir = s.any ? s.e0 : x + 2;
+  s.P0 = ld2(P, ic);
+  s.Pt = ld2(P, ir + x);
+  s.Pb = ld2(P, ir - x);
+  if (APPLY_P) {
+    s.R0 = ld2(R, ic);
+    s.Rt = ld2(R, ir + x);
+    s.Rb = ld2(R, ir - x);
+  }
+  s.KX = ld2(a.kx, ic);
+  s.KY = ld2(a.ky, ir);
+  s.KYt = ld2(a.ky, ir + x);
+  if (a.apply_u) s.U = ld2_nt(a.u, ir);
+}
+
+// Computes and stores the pair (w, u, p_new and their halo ring) and returns
+// the in-lane pair sum of p_new . w (the first level of the reduction tree).
+template <bool APPLY_P>
+__device__ inline double fused_pair_compute(const Geo& g, const FusedWArgs& a,
+                                            const FusedPair& s, int lane) {
+  const int x = g.x;
+  const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
+  const double* __restrict__ P = a.p_old;
+  const double* __restrict__ R = a.r;
+  const double beta = a.beta;
+  const int i = s.e0;
+  // Neighbours across the pair boundary, from the adjacent lanes.
+  double pl = __shfl_up(s.P0.y, 1), pr = __shfl_down(s.P0.x, 1);
+  double rl = 0.0, rr = 0.0;
+  if (APPLY_P) {
+    rl = __shfl_up(s.R0.y, 1);
+    rr = __shfl_down(s.R0.x, 1);
+  }
+  double kxr = __shfl_down(s.KX.x, 1);
+  if (lane == 0 && s.any) {
+    pl = P[i - 1];
+    if (APPLY_P) rl = R[i - 1];
+  }
+  if (lane == kWave - 1 && s.any) {
+    pr = P[i + 2];
+    if (APPLY_P) rr = R[i + 2];
+    kxr = a.kx[i + 2];
+  }
+  const int ka = s.kk, kb = s.kk + 1, row = s.jj;
+  double ca, cb, la, ra, lb, rb2, ba, ta, bb, tb;
+  if (APPLY_P) {
+    ca = beta * s.P0.x + s.R0.x;
+    cb = beta * s.P0.y + s.R0.y;
+    la = ka == xl ? ca : beta * pl + rl;
+    ra = ka == xr ? ca : cb;
+    lb = kb == xl ? cb : ca;
+    rb2 = kb == xr ? cb : beta * pr + rr;
+    ba = row == yb ? ca : beta * s.Pb.x + s.Rb.x;
+    ta = row == yt ? ca : beta * s.Pt.x + s.Rt.x;
+    bb = row == yb ? cb : beta * s.Pb.y + s.Rb.y;
+    tb = row == yt ? cb : beta * s.Pt.y + s.Rt.y;
+  } else {
+    ca = s.P0.x;
+    cb = s.P0.y;
+    la = pl;
+    ra = cb;
+    lb = ca;
+    rb2 = pr;
+    ba = s.Pb.x;
+    ta = s.Pt.x;
+    bb = s.Pb.y;
+    tb = s.Pt.y;
+  }
+  const double kxa0 = s.KX.x, kxa1 = s.KX.y, kxb0 = s.KX.y, kxb1 = kxr;
+  const double diag_a = (1.0 + (kxa1 + kxa0)) + (s.KYt.x + s.KY.x);
+  const double diag_b = (1.0 + (kxb1 + kxb0)) + (s.KYt.y + s.KY.y);
+  const double sa = (diag_a * ca - (kxa1 * ra + kxa0 * la)) -
+                    (s.KYt.x * ta + s.KY.x * ba);
+  const double sb = (diag_b * cb - (kxb1 * rb2 + kxb0 * lb)) -
+                    (s.KYt.y * tb + s.KY.y * bb);
+  const double psum = (s.in0 ? sa * ca : 0.0) + (s.in1 ? sb * cb : 0.0);
+  if (!s.any) return psum;
+  st2_nt(a.w, i, s.in0, s.in1, sa, sb);
+  double ua = 0.0, ub = 0.0;
+  if (a.apply_u) {
+    ua = s.U.x + a.alpha * s.P0.x;
+    ub = s.U.y + a.alpha * s.P0.y;
+    st2_nt(a.u, i, s.in0, s.in1, ua, ub);
+  }
+  if (APPLY_P) st2_nt(a.p_new, i, s.in0, s.in1, ca, cb);
+  // Halo ring of boundary-adjacent cells (rare; per cell).
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const bool inc = c == 0 ? s.in0 : s.in1;
+    const int ic = i + c, kc = ka + c;
+    const bool L = kc == xl, Rg = kc == xr, B = row == yb, T = row == yt;
+    if (!inc || !(L | Rg | B | T)) continue;
+    const double pv = c == 0 ? ca : cb;
+    const double uv = c == 0 ? ua : ub;
+    int ring[8];
+    int nr = 0;
+    if (L) ring[nr++] = ic - 1;
+    if (Rg) ring[nr++] = ic + 1;
+    if (B) ring[nr++] = ic - x;
+    if (T) ring[nr++] = ic + x;
+    if (L && B) ring[nr++] = ic - 1 - x;
+    if (Rg && B) ring[nr++] = ic + 1 - x;
+    if (L && T) ring[nr++] = ic - 1 + x;
+    if (Rg && T) ring[nr++] = ic + 1 + x;
+    for (int q = 0; q < nr; ++q) {
+      if (a.apply_u && a.ring) a.u[ring[q]] = uv;
+      if (APPLY_P) a.p_new[ring[q]] = a.ring ? pv : P[ring[q]];
+    }
+  }
+  return psum;
+}
+
+// One span per block, 8 pairs per thread in batches of SB.
 template <bool APPLY_P, int SB>
 __global__ __launch_bounds__(kBlock) void fused_w2_kernel(Geo g, FusedWArgs a,
                                                           double* partials,
@@ -312,140 +455,37 @@ __global__ __launch_bounds__(kBlock) void fused_w2_kernel(Geo g, FusedWArgs a,
   const int span = xcd_span(blockIdx.x, gridDim.x);
   const int base = span * kSpan;
   const int lane = threadIdx.x & (kWave - 1);
-  const int x = g.x;
-  const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
-  const double* __restrict__ P = a.p_old;
-  const double* __restrict__ R = a.r;
-  const double beta = a.beta;
   double psum[kSlots];
-
 #pragma unroll
   for (int s0 = 0; s0 < kSlots; s0 += SB) {
-    int e0[SB], kk[SB], jj[SB];
-    bool in0[SB], in1[SB], any[SB];
-    double2 P0[SB], Pt[SB], Pb[SB], R0[SB], Rt[SB], Rb[SB], KX[SB], KY[SB],
-        KYt[SB], U[SB];
+    FusedPair s[SB];
 #pragma unroll
-    for (int j = 0; j < SB; ++j) {
-      const int e = base + (s0 + j) * (2 * kBlock) + 2 * threadIdx.x;
-      const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
-      e0[j] = e;
-      jj[j] = row;
-      kk[j] = e - row * x;
-      in0[j] = e < g.n && interior(g, kk[j], row);
-      in1[j] = e < g.n && interior(g, kk[j] + 1, row);
-      any[j] = in0[j] || in1[j];
-    }
+    for (int j = 0; j < SB; ++j)
+      fused_pair_index(g, base + (s0 + j) * (2 * kBlock) + 2 * threadIdx.x, s[j]);
 #pragma unroll
-    for (int j = 0; j < SB; ++j) {
-      // The pair's own p, r and kx are also the left/right neighbours the
-      // adjacent lanes take by shuffle, so they are read whenever the pair
-      // is in range; the rows above and below only for interior pairs.
-      // x + 2 is an even index whose +-x rows are in range.
-      const int ic = e0[j] < g.n ? e0[j] : x + 2;
-      const int ir = any[j] ? e0[j] : x + 2;
-      P0[j] = ld2(P, ic);
-      Pt[j] = ld2(P, ir + x);
-      Pb[j] = ld2(P, ir - x);
-      if (APPLY_P) {
-        R0[j] = ld2(R, ic);
-        Rt[j] = ld2(R, ir + x);
-        Rb[j] = ld2(R, ir - x);
-      }
-      KX[j] = ld2(a.kx, ic);
-      KY[j] = ld2(a.ky, ir);
-      KYt[j] = ld2(a.ky, ir + x);
-      if (a.apply_u) U[j] = ld2_nt(a.u, ir);
-    }
+    for (int j = 0; j < SB; ++j) fused_pair_load<APPLY_P>(g, a, s[j]);
 #pragma unroll
-    for (int j = 0; j < SB; ++j) {
-      const int i = e0[j];
-      // Neighbours across the pair boundary, from the adjacent lanes.
-      double pl = __shfl_up(P0[j].y, 1), pr = __shfl_down(P0[j].x, 1);
-      double rl = 0.0, rr = 0.0;
-      if (APPLY_P) {
-        rl = __shfl_up(R0[j].y, 1);
-        rr = __shfl_down(R0[j].x, 1);
-      }
-      double kxr = __shfl_down(KX[j].x, 1);
-      if (lane == 0 && any[j]) {
-        pl = P[i - 1];
-        if (APPLY_P) rl = R[i - 1];
-      }
-      if (lane == kWave - 1 && any[j]) {
-        pr = P[i + 2];
-        if (APPLY_P) rr = R[i + 2];
-        kxr = a.kx[i + 2];
-      }
-      const int ka = kk[j], kb = kk[j] + 1, row = jj[j];
-      double ca, cb, la, ra, lb, rb2, ba, ta, bb, tb;
-      if (APPLY_P) {
-        ca = beta * P0[j].x + R0[j].x;
-        cb = beta * P0[j].y + R0[j].y;
-        la = ka == xl ? ca : beta * pl + rl;
-        ra = ka == xr ? ca : cb;
-        lb = kb == xl ? cb : ca;
-        rb2 = kb == xr ? cb : beta * pr + rr;
-        ba = row == yb ? ca : beta * Pb[j].x + Rb[j].x;
-        ta = row == yt ? ca : beta * Pt[j].x + Rt[j].x;
-        bb = row == yb ? cb : beta * Pb[j].y + Rb[j].y;
-        tb = row == yt ? cb : beta * Pt[j].y + Rt[j].y;
-      } else {
-        ca = P0[j].x;
-        cb = P0[j].y;
-        la = pl;
-        ra = cb;
-        lb = ca;
-        rb2 = pr;
-        ba = Pb[j].x;
-        ta = Pt[j].x;
-        bb = Pb[j].y;
-        tb = Pt[j].y;
-      }
-      const double kxa0 = KX[j].x, kxa1 = KX[j].y, kxb0 = KX[j].y, kxb1 = kxr;
-      const double diag_a = (1.0 + (kxa1 + kxa0)) + (KYt[j].x + KY[j].x);
-      const double diag_b = (1.0 + (kxb1 + kxb0)) + (KYt[j].y + KY[j].y);
-      const double sa = (diag_a * ca - (kxa1 * ra + kxa0 * la)) -
-                        (KYt[j].x * ta + KY[j].x * ba);
-      const double sb = (diag_b * cb - (kxb1 * rb2 + kxb0 * lb)) -
-                        (KYt[j].y * tb + KY[j].y * bb);
-      psum[s0 + j] = (in0[j] ? sa * ca : 0.0) + (in1[j] ? sb * cb : 0.0);
-      if (!any[j]) continue;
-      st2_nt(a.w, i, in0[j], in1[j], sa, sb);
-      double ua = 0.0, ub = 0.0;
-      if (a.apply_u) {
-        ua = U[j].x + a.alpha * P0[j].x;
-        ub = U[j].y + a.alpha * P0[j].y;
-        st2_nt(a.u, i, in0[j], in1[j], ua, ub);
-      }
-      if (APPLY_P) st2_nt(a.p_new, i, in0[j], in1[j], ca, cb);
-      // Halo ring of boundary-adjacent cells (rare; per cell).
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const bool inc = c == 0 ? in0[j] : in1[j];
-        const int ic = i + c, kc = ka + c;
-        const bool L = kc == xl, Rg = kc == xr, B = row == yb, T = row == yt;
-        if (!inc || !(L | Rg | B | T)) continue;
-        const double pv = c == 0 ? ca : cb;
-        const double uv = c == 0 ? ua : ub;
-        int ring[8];
-        int nr = 0;
-        if (L) ring[nr++] = ic - 1;
-        if (Rg) ring[nr++] = ic + 1;
-        if (B) ring[nr++] = ic - x;
-        if (T) ring[nr++] = ic + x;
-        if (L && B) ring[nr++] = ic - 1 - x;
-        if (Rg && B) ring[nr++] = ic + 1 - x;
-        if (L && T) ring[nr++] = ic - 1 + x;
-        if (Rg && T) ring[nr++] = ic + 1 + x;
-        for (int q = 0; q < nr; ++q) {
-          if (a.apply_u && a.ring) a.u[ring[q]] = uv;
-          if (APPLY_P) a.p_new[ring[q]] = a.ring ? pv : P[ring[q]];
-        }
-      }
-    }
+    for (int j = 0; j < SB; ++j)
+      psum[s0 + j] = fused_pair_compute<APPLY_P>(g, a, s[j], lane);
   }
   span_reduce_pairs(psum, partials, nspan, span, single_group != 0);
+}
+
+// One pair per thread: a 256-thread block covers 512 cells (8 groups of the
+// reduction tree) and stores their 8-leaf subtree sum in sub[block];
+// span_finish_kernel then combines the 8 blocks of each span.  Small blocks
+// keep 4x more waves per SIMD in flight than the span-per-block kernel.
+template <bool APPLY_P>
+__global__ __launch_bounds__(kBlock) void fused_w1_kernel(Geo g, FusedWArgs a,
+                                                          double* sub,
+                                                          int nblk) {
+  const int blk = xcd_span(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x & (kWave - 1);
+  FusedPair s;
+  fused_pair_index(g, blk * kPairBlock + 2 * threadIdx.x, s);
+  fused_pair_load<APPLY_P>(g, a, s);
+  double v[1] = {fused_pair_compute<APPLY_P>(g, a, s, lane)};
+  block_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
 __global__ __launch_bounds__(kBlock) void fused_r2_kernel(
@@ -475,6 +515,26 @@ __global__ __launch_bounds__(kBlock) void fused_r2_kernel(
     st2_nt(r, base + k * (2 * kBlock) + 2 * threadIdx.x, in0[k], in1[k], ra, rb);
   }
   span_reduce_pairs(psum, partials, nspan, span, single_group != 0);
+}
+
+// fused_r2's arithmetic with one pair per thread; the block stores its
+// 8-leaf subtree of r.r (see fused_w1_kernel).
+__global__ __launch_bounds__(kBlock) void fused_r1_kernel(
+    Geo g, double* __restrict__ r, const double* __restrict__ w, double alpha,
+    double* sub, int nblk) {
+  const int blk = xcd_span(blockIdx.x, gridDim.x);
+  const int e = blk * kPairBlock + 2 * threadIdx.x;
+  const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+  const int kk = e - row * g.x;
+  const bool in0 = e < g.n && interior(g, kk, row);
+  const bool in1 = e < g.n && interior(g, kk + 1, row);
+  const int i = (in0 || in1) ? e : 0;
+  const double2 rv = ld2_nt(r, i), wv = ld2_nt(w, i);
+  const double ra = rv.x - alpha * wv.x;
+  const double rb = rv.y - alpha * wv.y;
+  double v[1] = {(in0 ? ra * ra : 0.0) + (in1 ? rb * rb : 0.0)};
+  st2_nt(r, e, in0, in1, ra, rb);
+  block_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
 // r -= alpha w and r.r: all 16 cells' loads first, then the updates.
@@ -676,7 +736,20 @@ int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
                   f[TEA_FIELD_KY], c->pend_alpha,  c->pend_beta,
                   c->pend_u ? 1 : 0, c->pend_halo ? 1 : 0};
   c->pend_u = c->pend_p = c->pend_halo = false;
-  {
+  int plen = c->nspan;
+  if (use_pair_blocks(c, 1)) {
+    {
+      Timed t(c, K_CG_FUSED_W);
+      if (apply_p)
+        hipLaunchKernelGGL(fused_w1_kernel<true>, pair_grid(c), dim3(kBlock),
+                           0, c->stream, g, args, c->sub, c->nblk);
+      else
+        hipLaunchKernelGGL(fused_w1_kernel<false>, pair_grid(c), dim3(kBlock),
+                           0, c->stream, g, args, c->sub, c->nblk);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    TEA_RETURN_IF(launch_span_finish(c, 1, &plen));
+  } else {
     Timed t(c, K_CG_FUSED_W);
     const int sg = c->ngroup == 1 ? 1 : 0;
 #define TEA_FUSED_W(AP, B)                                                   \
@@ -705,7 +778,7 @@ int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
     f[TEA_FIELD_P] = c->p_alt;
     c->p_alt = t;
   }
-  return finish_reduction(c, 1, pw);
+  return finish_reduction_n(c, 1, plen, pw);
 }
 
 int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
@@ -718,7 +791,17 @@ int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
                         CalcURBody{f[TEA_FIELD_U], f[TEA_FIELD_R],
                                    f[TEA_FIELD_P], f[TEA_FIELD_W], alpha},
                         rrn);
-  {
+  int rlen = c->nspan;
+  if (use_pair_blocks(c, 2)) {
+    {
+      Timed t(c, K_CG_FUSED_R);
+      hipLaunchKernelGGL(fused_r1_kernel, pair_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, f[TEA_FIELD_R], f[TEA_FIELD_W],
+                         alpha, c->sub, c->nblk);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    TEA_RETURN_IF(launch_span_finish(c, 1, &rlen));
+  } else {
     Timed t(c, K_CG_FUSED_R);
     if (c->pairs)
       hipLaunchKernelGGL(fused_r2_kernel, span_grid(c), dim3(kBlock), 0,
@@ -730,7 +813,7 @@ int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
                          alpha, c->partials, c->nspan, c->ngroup == 1 ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
-  TEA_RETURN_IF(finish_reduction(c, 1, rrn));
+  TEA_RETURN_IF(finish_reduction_n(c, 1, rlen, rrn));
   c->pend_u = true;
   c->pend_alpha = alpha;
   return TEA_HIP_OK;

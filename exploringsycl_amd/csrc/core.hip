@@ -25,10 +25,11 @@ const char* const kKernelNames[K_NUM] = {
     "cg_fused_r",     "cg_apply_pending", "copy_u",      "calculate_residual",
     "calculate_2norm", "finalise",       "mpcg_convert", "mpcg_w",
     "mpcg_r",         "mpcg_apply_u",    "mpcg_refresh",  "fcg_w",
-    "fcg_jacobi",     "fcg_restrict",    "fcg_net",       "fcg_prolong"};
+    "fcg_jacobi",     "fcg_restrict",    "fcg_net",       "fcg_prolong",
+    "span_finish"};
 
 Timed::Timed(tea_hip_chunk* c_, int kid_) : c(c_), kid(kid_) {
-  if (!c->timing) return;
+  if (!c->timing || !c->time_kernel[kid]) return;
   if (c->event_pool.size() < 2) {
     for (int i = 0; i < 64; ++i) {
       hipEvent_t e;
@@ -92,12 +93,63 @@ static double host_tree_passes(double* v, size_t len) {
   return v[0];
 }
 
-int finish_reduction(tea_hip_chunk* c, int nr, double* out) {
+// Level-2 partial of each span from its 8 block subtrees (zero for blocks
+// past the grid, as the reference pads its level-2 input with zeros): the
+// last three steps (s = 8, 16, 32) of the reference's level-2 tree.  Each
+// wave then holds 64 consecutive spans, one group of the next pass, and runs
+// that pass too (zero-padded past nspan), leaving ceil(nspan/64) partials;
+// with a single span there is no further pass and its partial is the sum.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void span_finish_kernel(
+    const double* __restrict__ sub, int nblk, double* partials, int nspan) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    double b[kSubPerSpan];
+#pragma unroll
+    for (int k = 0; k < kSubPerSpan; ++k) {
+      const int i = s * kSubPerSpan + k;
+      b[k] = (s < nspan && i < nblk) ? sub[q * nblk + i] : 0.0;
+    }
+    const double span_part =
+        ((b[0] + b[1]) + (b[2] + b[3])) + ((b[4] + b[5]) + (b[6] + b[7]));
+    if (nspan == 1) {
+      if (s == 0) partials[q * nspan] = span_part;
+      continue;
+    }
+    const double g3 = wave_tree(span_part);
+    if (lane == 0 && s < nspan) partials[q * nspan + (s / kWave)] = g3;
+  }
+}
+
+int launch_span_finish(tea_hip_chunk* c, int nr, int* len) {
+  TEA_CHECK_ARG(nr >= 1 && nr <= kSubMaxNr, "span_finish: bad reduction count");
+  const dim3 grid((unsigned)((c->nspan + kBlock - 1) / kBlock));
+  {
+    Timed t(c, K_SPAN_FINISH);
+    if (nr == 1)
+      hipLaunchKernelGGL(span_finish_kernel<1>, grid, dim3(kBlock), 0,
+                         c->stream, c->sub, c->nblk, c->partials, c->nspan);
+    else
+      hipLaunchKernelGGL(span_finish_kernel<2>, grid, dim3(kBlock), 0,
+                         c->stream, c->sub, c->nblk, c->partials, c->nspan);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  *len = c->nspan == 1 ? 1 : (c->nspan + kWave - 1) / kWave;
+  return TEA_HIP_OK;
+}
+
+int finish_reduction_n(tea_hip_chunk* c, int nr, int len, double* out) {
   TEA_RETURN_IF(sync_chunk(c));
   for (int q = 0; q < nr; ++q)
     out[q] = host_tree_passes(c->host_partials + (size_t)q * c->nspan,
-                              (size_t)c->nspan);
+                              (size_t)len);
   return TEA_HIP_OK;
+}
+
+int finish_reduction(tea_hip_chunk* c, int nr, double* out) {
+  return finish_reduction_n(c, nr, c->nspan, out);
 }
 
 }  // namespace tea
@@ -136,6 +188,18 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
   if (const char* b = getenv("TEA_HIP_BATCH")) c->batch = atoi(b);
   c->pairs = (x % 2 == 0) && y >= 3;
   if (const char* pv = getenv("TEA_HIP_PAIRS")) c->pairs = c->pairs && atoi(pv) != 0;
+  c->nblk = (int)((n + kPairBlock - 1) / kPairBlock);
+  if (const char* pb = getenv("TEA_HIP_PAIR_BLOCKS")) c->pair_blocks = atoi(pb);
+  for (int k = 0; k < K_NUM; ++k) c->time_kernel[k] = true;
+  if (const char* only = getenv("TEA_HIP_STATS_ONLY")) {
+    for (int k = 0; k < K_NUM; ++k) {
+      const std::string name = kKernelNames[k];
+      const std::string list = std::string(",") + only + ",";
+      c->time_kernel[k] = list.find("," + name + ",") != std::string::npos;
+    }
+  }
+  if (const char* sp = getenv("TEA_HIP_SPIN"))
+    if (atoi(sp) != 0) (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
 
   auto fail = [&](int code) {
     tea_hip_chunk_destroy(c);
@@ -179,6 +243,11 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
     set_error("hipMalloc of the second p buffer failed");
     return fail(TEA_HIP_ENOMEM);
   }
+  if (hipMalloc(&c->sub, (size_t)kSubMaxNr * c->nblk * sizeof(double)) != hipSuccess) {
+    c->sub = nullptr;
+    set_error("hipMalloc of the subtree sums failed");
+    return fail(TEA_HIP_ENOMEM);
+  }
   const size_t pbytes = 4 * (size_t)c->nspan * sizeof(double);
   if (hipHostMalloc(&c->host_partials, pbytes,
                     hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
@@ -212,6 +281,7 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (int f = 0; f < TEA_NUM_FIELDS; ++f)
     if (c->field[f]) (void)hipFree(c->field[f]);
   if (c->p_alt) (void)hipFree(c->p_alt);
+  if (c->sub) (void)hipFree(c->sub);
   for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky, c->fcg_z})
     if (b) (void)hipFree(b);
   if (c->fcg_cross) (void)hipFree(c->fcg_cross);

@@ -245,6 +245,44 @@ __device__ inline void span_reduce_pairs_n(double (&v)[NR][kSlots], double* part
     }
   }
 }
+// ---- one pair per thread ---------------------------------------------------
+//
+// A 256-thread block over 512 consecutive cells = 8 groups of the reduction
+// tree, one cell pair per lane.  The reference's level-2 tree over a span's
+// 64 level-1 partials (sycl_shared.hpp:59-64: a[2s*l] += a[2s*l + s] for
+// s = 1, 2, 4, ...) sums aligned runs of 8 leaves before it combines them, so
+// a block's 8-leaf subtree is an exact intermediate of that tree: blocks
+// store it in sub[q * nblk + block], and span_finish_kernel combines the 8
+// subtrees of each span into its level-2 partial.
+
+constexpr int kPairBlock = 2 * kBlock;              // 512 cells
+constexpr int kSubPerSpan = kSpan / kPairBlock;     // 8
+constexpr int kSubMaxNr = 2;                        // dot products per launch
+
+// From each lane's in-lane pair sum (tree step 1): steps 2..32 within each
+// 32-lane half (one group each), then the wave's two groups, then the
+// block's four waves, as the 8-leaf subtree of the reference's level-2 tree.
+template <int NR>
+__device__ inline void block_subtree_pairs(double (&v)[NR], double* sub,
+                                           int nblk, int blk) {
+  __shared__ double lds[NR][kWaves];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    double t = v[q];
+#pragma unroll
+    for (int m = 1; m < 32; m <<= 1) t = t + __shfl_xor(t, m);
+    const double upper = __shfl_down(t, 32);
+    if (lane == 0) lds[q][wv] = t + upper;
+  }
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    const int q = threadIdx.x;
+    sub[q * nblk + blk] = (lds[q][0] + lds[q][1]) + (lds[q][2] + lds[q][3]);
+  }
+}
+
 // One block per span: each thread evaluates `body` on its 16 cells (only
 // interior cells; everything else contributes +0.0, as in the reference) and
 // the block stores the span's NR level-2 partials.
@@ -315,6 +353,7 @@ enum KernelId {
   K_FCG_RESTRICT,
   K_FCG_NET,
   K_FCG_PROLONG,
+  K_SPAN_FINISH,
   K_NUM
 };
 
@@ -399,6 +438,11 @@ struct tea_hip_chunk {
   // coherent host memory; kernels write through the device view.
   double* partials = nullptr;       // device view
   double* host_partials = nullptr;  // host view of the same memory
+  // Subtree sums of the pair-per-thread kernels: kSubMaxNr x nblk, device.
+  int nblk = 0;                     // 512-cell blocks covering x*y
+  double* sub = nullptr;
+  int pair_blocks = 3;              // pair-per-thread kernels: bit 0 fused_w,
+                                    // bit 1 fused_r
 
   // Deferred CG work (fused mode).
   int fused = 1;
@@ -433,6 +477,7 @@ struct tea_hip_chunk {
   };
   std::vector<Pending> inflight;
   std::vector<hipEvent_t> event_pool;
+  bool time_kernel[tea::K_NUM];     // kernels timed (all, or TEA_HIP_STATS_ONLY)
   long launches[tea::K_NUM] = {};
   double total_ms[tea::K_NUM] = {};
 };
@@ -459,6 +504,22 @@ int sync_chunk(tea_hip_chunk* c);
 // partials; results go to out[0..nr).
 int finish_reduction(tea_hip_chunk* c, int nr, double* out);
 
+// Combines the 8 subtree sums of each span in c->sub (nr dot products) into
+// the span's level-2 partials and runs the next tree pass over them, leaving
+// *len partials per dot product in c->partials.
+int launch_span_finish(tea_hip_chunk* c, int nr, int* len);
+
+// finish_reduction over the first len partials of each array (len is what
+// launch_span_finish leaves).
+int finish_reduction_n(tea_hip_chunk* c, int nr, int len, double* out);
+
+// The pair-per-thread kernels apply: x even (pairs) and more than one group
+// (the reference then runs its level-2 pass; for a single group it stops at
+// level 1, which the span-per-block kernels reproduce).
+inline bool use_pair_blocks(const tea_hip_chunk* c, int kernel_bit) {
+  return c->pairs && (c->pair_blocks & kernel_bit) && c->ngroup > 1;
+}
+
 // Applies any deferred CG work (fused mode).
 int apply_pending(tea_hip_chunk* c);
 // Mixed-precision residual update r -= alpha w with the partials of r.r
@@ -470,6 +531,7 @@ int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
                  const int fields[6]);
 
 inline dim3 span_grid(const tea_hip_chunk* c) { return dim3(c->nspan); }
+inline dim3 pair_grid(const tea_hip_chunk* c) { return dim3(c->nblk); }
 inline dim3 flat_grid(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
 }  // namespace tea
