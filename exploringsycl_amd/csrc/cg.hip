@@ -311,6 +311,9 @@ struct FusedPair {
   int e0, kk, jj;
   bool in0, in1, any;
   double2 P0, Pt, Pb, R0, Rt, Rb, KX, KY, KYt, U;
+  // Wave-edge neighbours: lane 0 holds p and r at e0 - 1, lane 63 p, r and
+  // kx at e0 + 2 (the other lanes take theirs from adjacent lanes).
+  double PN, RN, KXN;
 };
 
 __device__ inline void fused_pair_index(const Geo& g, int e, FusedPair& s) {
@@ -347,6 +350,17 @@ __device__ inline void fused_pair_load(const Geo& g, const FusedWArgs& a,
   s.KY = ld2(a.ky, ir);
   s.KYt = ld2(a.ky, ir + x);
   if (a.apply_u) s.U = ld2_nt(a.u, ir);
+  // Issued with the pair's own loads so the wave waits once; ir - 1 and
+  // ir + 2 are in range (ir is an interior pair's index or x + 2).
+  const int lane = threadIdx.x & (kWave - 1);
+  if (lane == 0) {
+    s.PN = P[ir - 1];
+    if (APPLY_P) s.RN = R[ir - 1];
+  } else if (lane == kWave - 1) {
+    s.PN = P[ir + 2];
+    if (APPLY_P) s.RN = R[ir + 2];
+    s.KXN = a.kx[ir + 2];
+  }
 }
 
 // Computes and stores the pair (w, u, p_new and their halo ring) and returns
@@ -368,14 +382,14 @@ __device__ inline double fused_pair_compute(const Geo& g, const FusedWArgs& a,
     rr = __shfl_down(s.R0.x, 1);
   }
   double kxr = __shfl_down(s.KX.x, 1);
-  if (lane == 0 && s.any) {
-    pl = P[i - 1];
-    if (APPLY_P) rl = R[i - 1];
+  if (lane == 0) {
+    pl = s.PN;
+    if (APPLY_P) rl = s.RN;
   }
-  if (lane == kWave - 1 && s.any) {
-    pr = P[i + 2];
-    if (APPLY_P) rr = R[i + 2];
-    kxr = a.kx[i + 2];
+  if (lane == kWave - 1) {
+    pr = s.PN;
+    if (APPLY_P) rr = s.RN;
+    kxr = s.KXN;
   }
   const int ka = s.kk, kb = s.kk + 1, row = s.jj;
   double ca, cb, la, ra, lb, rb2, ba, ta, bb, tb;

@@ -148,6 +148,14 @@ int finish_reduction_n(tea_hip_chunk* c, int nr, int len, double* out) {
   return TEA_HIP_OK;
 }
 
+int finish_reduction_lens(tea_hip_chunk* c, int nr, const int* lens, double* out) {
+  TEA_RETURN_IF(sync_chunk(c));
+  for (int q = 0; q < nr; ++q)
+    out[q] = host_tree_passes(c->host_partials + (size_t)q * c->nspan,
+                              (size_t)lens[q]);
+  return TEA_HIP_OK;
+}
+
 int finish_reduction(tea_hip_chunk* c, int nr, double* out) {
   return finish_reduction_n(c, nr, c->nspan, out);
 }

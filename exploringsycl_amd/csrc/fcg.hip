@@ -727,10 +727,11 @@ int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
   TEA_CHECK_ARG(rr != nullptr && zw != nullptr, "null outputs");
   TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no network attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_RETURN_IF(launch_mp_r(c, alpha, c->partials));
+  int lens[2] = {c->nspan, c->nspan};
+  TEA_RETURN_IF(launch_mp_r(c, alpha, &lens[0]));
   TEA_RETURN_IF(launch_precondition(c, use_net, c->partials + c->nspan));
   double out[2];
-  TEA_RETURN_IF(finish_reduction(c, 2, out));
+  TEA_RETURN_IF(finish_reduction_lens(c, 2, lens, out));
   *rr = out[0];
   *zw = out[1];
   return TEA_HIP_OK;
@@ -747,7 +748,16 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
   MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->fcg_z,
            c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky, c->mp_r,
            alpha, beta, apply_u ? 1 : 0, 1};
-  {
+  int len = c->nspan;
+  if (use_pair_blocks(c, 1)) {
+    {
+      Timed t(c, K_FCG_W);
+      hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, a, c->sub, c->nblk);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    TEA_RETURN_IF(launch_span_finish(c, 2, &len));
+  } else {
     Timed t(c, K_FCG_W);
     if (c->pairs)
       hipLaunchKernelGGL((mp_w2_kernel<true, 4>), span_grid(c), dim3(kBlock), 0,
@@ -761,7 +771,7 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
   TEA_HIP_TRY(hipGetLastError());
   c->mp_cur ^= 1;
   double out[2];
-  TEA_RETURN_IF(finish_reduction(c, 2, out));
+  TEA_RETURN_IF(finish_reduction_n(c, 2, len, out));
   *pw = out[0];
   *pr = out[1];
   return TEA_HIP_OK;

@@ -158,7 +158,164 @@ __global__ __launch_bounds__(kBlock) void mp_r_kernel(
 }
 
 // Two cells per lane (x even): the arithmetic of mp_w_kernel cell by cell,
-// with 8-byte loads of the FP32 operands and 16-byte loads of u.
+// with 8-byte loads of the FP32 operands and 16-byte loads of u.  One pair's
+// indices, loads and arithmetic, shared by the span-per-block and
+// pair-per-thread kernels.
+struct MpPair {
+  int e0, kk, jj;
+  bool in0, in1, any;
+  float2 P0, Pt, Pb, R0, Rt, Rb, KX, KY, KYt, RD;
+  double2 U;
+  // Wave-edge neighbours: lane 0 holds p and r at e0 - 1, lane 63 p, r and
+  // kx at e0 + 2 (the other lanes take theirs from adjacent lanes).
+  float PN, RN, KXN;
+};
+
+__device__ inline void mp_pair_index(const Geo& g, int e, MpPair& s) {
+  const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+  s.e0 = e;
+  s.jj = row;
+  s.kk = e - row * g.x;
+  s.in0 = e < g.n && interior(g, s.kk, row);
+  s.in1 = e < g.n && interior(g, s.kk + 1, row);
+  s.any = s.in0 || s.in1;
+}
+
+// A pair's own p, r and kx are also its neighbours' left/right operands (by
+// shuffle), so they are read whenever the pair is in range; the rows above
+// and below only for interior pairs.  x + 2 is an even index whose +-x rows
+// are in range.  The wave-edge neighbours are issued with the rest, at
+// ir - 1 and ir + 2, which are in range too.
+template <bool DOT2>
+__device__ inline void mp_pair_load(const Geo& g, const MpArgs& a, MpPair& s) {
+  const int x = g.x;
+  const float* __restrict__ P = a.p_old;
+  const float* __restrict__ R = a.r;
+  const int ic = s.e0 < g.n ? s.e0 : x + 2;
+  const int ir = s.any ? s.e0 : x + 2;
+  s.P0 = ld2(P, ic);
+  s.Pt = ld2(P, ir + x);
+  s.Pb = ld2(P, ir - x);
+  if (a.apply_p) {
+    s.R0 = ld2(R, ic);
+    s.Rt = ld2(R, ir + x);
+    s.Rb = ld2(R, ir - x);
+  }
+  s.KX = ld2(a.kx, ic);
+  s.KY = ld2(a.ky, ir);
+  s.KYt = ld2(a.ky, ir + x);
+  if (a.apply_u) s.U = ld2(a.u, ir);
+  if (DOT2) s.RD = ld2(a.rdot, ir);
+  const int lane = threadIdx.x & (kWave - 1);
+  if (lane == 0) {
+    s.PN = P[ir - 1];
+    if (a.apply_p) s.RN = R[ir - 1];
+  } else if (lane == kWave - 1) {
+    s.PN = P[ir + 2];
+    if (a.apply_p) s.RN = R[ir + 2];
+    s.KXN = a.kx[ir + 2];
+  }
+}
+
+// Computes and stores the pair (w, u, p_new and their halo ring) and returns
+// the in-lane pair sums of p_new . w and, with DOT2, p_new . rdot.
+template <bool DOT2>
+__device__ inline void mp_pair_compute(const Geo& g, const MpArgs& a,
+                                       const MpPair& s, int lane,
+                                       double (&ps)[DOT2 ? 2 : 1]) {
+  const int x = g.x;
+  const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
+  const double beta = a.beta;
+  const int i = s.e0;
+  float pl = __shfl_up(s.P0.y, 1), pr = __shfl_down(s.P0.x, 1);
+  float rl = 0.0f, rr = 0.0f;
+  if (a.apply_p) {
+    rl = __shfl_up(s.R0.y, 1);
+    rr = __shfl_down(s.R0.x, 1);
+  }
+  float kxr = __shfl_down(s.KX.x, 1);
+  if (lane == 0) {
+    pl = s.PN;
+    if (a.apply_p) rl = s.RN;
+  }
+  if (lane == kWave - 1) {
+    pr = s.PN;
+    if (a.apply_p) rr = s.RN;
+    kxr = s.KXN;
+  }
+  const int ka = s.kk, kb = s.kk + 1, row = s.jj;
+  float ca, cb, la, ra, lb, rb2, ba, ta, bb, tb;
+  if (a.apply_p) {
+    ca = (float)(beta * (double)s.P0.x + (double)s.R0.x);
+    cb = (float)(beta * (double)s.P0.y + (double)s.R0.y);
+    la = ka == xl ? ca : (float)(beta * (double)pl + (double)rl);
+    ra = ka == xr ? ca : cb;
+    lb = kb == xl ? cb : ca;
+    rb2 = kb == xr ? cb : (float)(beta * (double)pr + (double)rr);
+    ba = row == yb ? ca : (float)(beta * (double)s.Pb.x + (double)s.Rb.x);
+    ta = row == yt ? ca : (float)(beta * (double)s.Pt.x + (double)s.Rt.x);
+    bb = row == yb ? cb : (float)(beta * (double)s.Pb.y + (double)s.Rb.y);
+    tb = row == yt ? cb : (float)(beta * (double)s.Pt.y + (double)s.Rt.y);
+  } else {
+    ca = s.P0.x;
+    cb = s.P0.y;
+    la = pl;
+    ra = cb;
+    lb = ca;
+    rb2 = pr;
+    ba = s.Pb.x;
+    ta = s.Pt.x;
+    bb = s.Pb.y;
+    tb = s.Pt.y;
+  }
+  const float kxa0 = s.KX.x, kxa1 = s.KX.y, kxb0 = s.KX.y, kxb1 = kxr;
+  const float diag_a = (1.0f + (kxa1 + kxa0)) + (s.KYt.x + s.KY.x);
+  const float diag_b = (1.0f + (kxb1 + kxb0)) + (s.KYt.y + s.KY.y);
+  const float sa = (diag_a * ca - (kxa1 * ra + kxa0 * la)) -
+                   (s.KYt.x * ta + s.KY.x * ba);
+  const float sb = (diag_b * cb - (kxb1 * rb2 + kxb0 * lb)) -
+                   (s.KYt.y * tb + s.KY.y * bb);
+  ps[0] = (s.in0 ? (double)sa * (double)ca : 0.0) +
+          (s.in1 ? (double)sb * (double)cb : 0.0);
+  if (DOT2)
+    ps[DOT2 ? 1 : 0] = (s.in0 ? (double)ca * (double)s.RD.x : 0.0) +
+                       (s.in1 ? (double)cb * (double)s.RD.y : 0.0);
+  if (!s.any) return;
+  st2(a.w, i, s.in0, s.in1, sa, sb);
+  double ua = 0.0, ub = 0.0;
+  if (a.apply_u) {
+    ua = s.U.x + a.alpha * (double)s.P0.x;
+    ub = s.U.y + a.alpha * (double)s.P0.y;
+    st2(a.u, i, s.in0, s.in1, ua, ub);
+  }
+  if (a.apply_p) st2(a.p_new, i, s.in0, s.in1, ca, cb);
+  // Halo ring of boundary-adjacent cells (rare; per cell).
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const bool inc = c == 0 ? s.in0 : s.in1;
+    const int ic = i + c, kc = ka + c;
+    const bool L = kc == xl, Rg = kc == xr, B = row == yb, T = row == yt;
+    if (!inc || !(L | Rg | B | T)) continue;
+    const float pv = c == 0 ? ca : cb;
+    const double uv = c == 0 ? ua : ub;
+    int ring[8];
+    int nr = 0;
+    if (L) ring[nr++] = ic - 1;
+    if (Rg) ring[nr++] = ic + 1;
+    if (B) ring[nr++] = ic - x;
+    if (T) ring[nr++] = ic + x;
+    if (L && B) ring[nr++] = ic - 1 - x;
+    if (Rg && B) ring[nr++] = ic + 1 - x;
+    if (L && T) ring[nr++] = ic - 1 + x;
+    if (Rg && T) ring[nr++] = ic + 1 + x;
+    for (int q = 0; q < nr; ++q) {
+      if (a.apply_u) a.u[ring[q]] = uv;
+      if (a.apply_p) a.p_new[ring[q]] = pv;
+    }
+  }
+}
+
+// One span per block, 8 pairs per thread in batches of SB.
 template <bool DOT2, int SB>
 __global__ __launch_bounds__(kBlock) void mp_w2_kernel(Geo g, MpArgs a,
                                                        double* partials,
@@ -167,148 +324,44 @@ __global__ __launch_bounds__(kBlock) void mp_w2_kernel(Geo g, MpArgs a,
   const int span = xcd_span(blockIdx.x, gridDim.x);
   const int base = span * kSpan;
   const int lane = threadIdx.x & (kWave - 1);
-  const int x = g.x;
-  const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
-  const float* __restrict__ P = a.p_old;
-  const float* __restrict__ R = a.r;
-  const double beta = a.beta;
   double psum[DOT2 ? 2 : 1][kSlots];
-
 #pragma unroll
   for (int s0 = 0; s0 < kSlots; s0 += SB) {
-    int e0[SB], kk[SB], jj[SB];
-    bool in0[SB], in1[SB], any[SB];
-    float2 P0[SB], Pt[SB], Pb[SB], R0[SB], Rt[SB], Rb[SB], KX[SB], KY[SB], KYt[SB],
-        RD[SB];
-    double2 U[SB];
+    MpPair s[SB];
+#pragma unroll
+    for (int j = 0; j < SB; ++j)
+      mp_pair_index(g, base + (s0 + j) * (2 * kBlock) + 2 * threadIdx.x, s[j]);
+#pragma unroll
+    for (int j = 0; j < SB; ++j) mp_pair_load<DOT2>(g, a, s[j]);
 #pragma unroll
     for (int j = 0; j < SB; ++j) {
-      const int e = base + (s0 + j) * (2 * kBlock) + 2 * threadIdx.x;
-      const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
-      e0[j] = e;
-      jj[j] = row;
-      kk[j] = e - row * x;
-      in0[j] = e < g.n && interior(g, kk[j], row);
-      in1[j] = e < g.n && interior(g, kk[j] + 1, row);
-      any[j] = in0[j] || in1[j];
-    }
-#pragma unroll
-    for (int j = 0; j < SB; ++j) {
-      // A pair's own p, r and kx are also its neighbours' left/right
-      // operands (by shuffle), so they are read whenever the pair is in
-      // range; the rows above and below only for interior pairs.  x + 2 is
-      // an even index whose +-x rows are in range.
-      const int ic = e0[j] < g.n ? e0[j] : x + 2;
-      const int ir = any[j] ? e0[j] : x + 2;
-      P0[j] = ld2(P, ic);
-      Pt[j] = ld2(P, ir + x);
-      Pb[j] = ld2(P, ir - x);
-      if (a.apply_p) {
-        R0[j] = ld2(R, ic);
-        Rt[j] = ld2(R, ir + x);
-        Rb[j] = ld2(R, ir - x);
-      }
-      KX[j] = ld2(a.kx, ic);
-      KY[j] = ld2(a.ky, ir);
-      KYt[j] = ld2(a.ky, ir + x);
-      if (a.apply_u) U[j] = ld2(a.u, ir);
-      if (DOT2) RD[j] = ld2(a.rdot, ir);
-    }
-#pragma unroll
-    for (int j = 0; j < SB; ++j) {
-      const int i = e0[j];
-      float pl = __shfl_up(P0[j].y, 1), pr = __shfl_down(P0[j].x, 1);
-      float rl = 0.0f, rr = 0.0f;
-      if (a.apply_p) {
-        rl = __shfl_up(R0[j].y, 1);
-        rr = __shfl_down(R0[j].x, 1);
-      }
-      float kxr = __shfl_down(KX[j].x, 1);
-      if (lane == 0 && any[j]) {
-        pl = P[i - 1];
-        if (a.apply_p) rl = R[i - 1];
-      }
-      if (lane == kWave - 1 && any[j]) {
-        pr = P[i + 2];
-        if (a.apply_p) rr = R[i + 2];
-        kxr = a.kx[i + 2];
-      }
-      const int ka = kk[j], kb = kk[j] + 1, row = jj[j];
-      float ca, cb, la, ra, lb, rb2, ba, ta, bb, tb;
-      if (a.apply_p) {
-        ca = (float)(beta * (double)P0[j].x + (double)R0[j].x);
-        cb = (float)(beta * (double)P0[j].y + (double)R0[j].y);
-        la = ka == xl ? ca : (float)(beta * (double)pl + (double)rl);
-        ra = ka == xr ? ca : cb;
-        lb = kb == xl ? cb : ca;
-        rb2 = kb == xr ? cb : (float)(beta * (double)pr + (double)rr);
-        ba = row == yb ? ca : (float)(beta * (double)Pb[j].x + (double)Rb[j].x);
-        ta = row == yt ? ca : (float)(beta * (double)Pt[j].x + (double)Rt[j].x);
-        bb = row == yb ? cb : (float)(beta * (double)Pb[j].y + (double)Rb[j].y);
-        tb = row == yt ? cb : (float)(beta * (double)Pt[j].y + (double)Rt[j].y);
-      } else {
-        ca = P0[j].x;
-        cb = P0[j].y;
-        la = pl;
-        ra = cb;
-        lb = ca;
-        rb2 = pr;
-        ba = Pb[j].x;
-        ta = Pt[j].x;
-        bb = Pb[j].y;
-        tb = Pt[j].y;
-      }
-      const float kxa0 = KX[j].x, kxa1 = KX[j].y, kxb0 = KX[j].y, kxb1 = kxr;
-      const float diag_a = (1.0f + (kxa1 + kxa0)) + (KYt[j].x + KY[j].x);
-      const float diag_b = (1.0f + (kxb1 + kxb0)) + (KYt[j].y + KY[j].y);
-      const float sa = (diag_a * ca - (kxa1 * ra + kxa0 * la)) -
-                       (KYt[j].x * ta + KY[j].x * ba);
-      const float sb = (diag_b * cb - (kxb1 * rb2 + kxb0 * lb)) -
-                       (KYt[j].y * tb + KY[j].y * bb);
-      psum[0][s0 + j] = (in0[j] ? (double)sa * (double)ca : 0.0) +
-                        (in1[j] ? (double)sb * (double)cb : 0.0);
-      if (DOT2)
-        psum[DOT2 ? 1 : 0][s0 + j] = (in0[j] ? (double)ca * (double)RD[j].x : 0.0) +
-                                     (in1[j] ? (double)cb * (double)RD[j].y : 0.0);
-      if (!any[j]) continue;
-      st2(a.w, i, in0[j], in1[j], sa, sb);
-      double ua = 0.0, ub = 0.0;
-      if (a.apply_u) {
-        ua = U[j].x + a.alpha * (double)P0[j].x;
-        ub = U[j].y + a.alpha * (double)P0[j].y;
-        st2(a.u, i, in0[j], in1[j], ua, ub);
-      }
-      if (a.apply_p) st2(a.p_new, i, in0[j], in1[j], ca, cb);
-      // Halo ring of boundary-adjacent cells (rare; per cell).
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const bool inc = c == 0 ? in0[j] : in1[j];
-        const int ic = i + c, kc = ka + c;
-        const bool L = kc == xl, Rg = kc == xr, B = row == yb, T = row == yt;
-        if (!inc || !(L | Rg | B | T)) continue;
-        const float pv = c == 0 ? ca : cb;
-        const double uv = c == 0 ? ua : ub;
-        int ring[8];
-        int nr = 0;
-        if (L) ring[nr++] = ic - 1;
-        if (Rg) ring[nr++] = ic + 1;
-        if (B) ring[nr++] = ic - x;
-        if (T) ring[nr++] = ic + x;
-        if (L && B) ring[nr++] = ic - 1 - x;
-        if (Rg && B) ring[nr++] = ic + 1 - x;
-        if (L && T) ring[nr++] = ic - 1 + x;
-        if (Rg && T) ring[nr++] = ic + 1 + x;
-        for (int q = 0; q < nr; ++q) {
-          if (a.apply_u) a.u[ring[q]] = uv;
-          if (a.apply_p) a.p_new[ring[q]] = pv;
-        }
-      }
+      double ps[DOT2 ? 2 : 1];
+      mp_pair_compute<DOT2>(g, a, s[j], lane, ps);
+      psum[0][s0 + j] = ps[0];
+      if (DOT2) psum[DOT2 ? 1 : 0][s0 + j] = ps[DOT2 ? 1 : 0];
     }
   }
   span_reduce_pairs_n<DOT2 ? 2 : 1>(psum, partials, nspan, span, single_group != 0);
 }
 
 template __global__ void mp_w2_kernel<true, 4>(Geo, MpArgs, double*, int, int);
+
+// One pair per thread: the block's 8-leaf subtrees go to sub (see
+// block_subtree_pairs) and span_finish_kernel completes the spans.
+template <bool DOT2>
+__global__ __launch_bounds__(kBlock) void mp_w1_kernel(Geo g, MpArgs a,
+                                                       double* sub, int nblk) {
+  const int blk = xcd_span(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x & (kWave - 1);
+  MpPair s;
+  mp_pair_index(g, blk * kPairBlock + 2 * threadIdx.x, s);
+  mp_pair_load<DOT2>(g, a, s);
+  double ps[DOT2 ? 2 : 1];
+  mp_pair_compute<DOT2>(g, a, s, lane, ps);
+  block_subtree_pairs<DOT2 ? 2 : 1>(ps, sub, nblk, blk);
+}
+
+template __global__ void mp_w1_kernel<true>(Geo, MpArgs, double*, int);
 
 // r -= alpha w and r.r, two cells per lane.
 __global__ __launch_bounds__(kBlock) void mp_r2_kernel(
@@ -339,6 +392,26 @@ __global__ __launch_bounds__(kBlock) void mp_r2_kernel(
     st2(r, base + k * (2 * kBlock) + 2 * threadIdx.x, in0[k], in1[k], ra, rb);
   }
   span_reduce_pairs_n<1>(psum, partials, nspan, span, single_group != 0);
+}
+
+// mp_r2_kernel's arithmetic with one pair per thread.
+__global__ __launch_bounds__(kBlock) void mp_r1_kernel(
+    Geo g, float* __restrict__ r, const float* __restrict__ w, double alpha,
+    double* sub, int nblk) {
+  const int blk = xcd_span(blockIdx.x, gridDim.x);
+  const int e = blk * kPairBlock + 2 * threadIdx.x;
+  const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+  const int kk = e - row * g.x;
+  const bool in0 = e < g.n && interior(g, kk, row);
+  const bool in1 = e < g.n && interior(g, kk + 1, row);
+  const int i = (in0 || in1) ? e : 0;
+  const float2 rv = ld2(r, i), wv = ld2(w, i);
+  const float ra = (float)((double)rv.x - alpha * (double)wv.x);
+  const float rb = (float)((double)rv.y - alpha * (double)wv.y);
+  double v[1] = {(in0 ? (double)ra * (double)ra : 0.0) +
+                 (in1 ? (double)rb * (double)rb : 0.0)};
+  st2(r, e, in0, in1, ra, rb);
+  block_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
 namespace {
@@ -437,16 +510,27 @@ int ensure_mp_buffers(tea_hip_chunk* c) {
 
 }  // namespace
 
-// r -= alpha w and the partials of r.r into `partials` (not waited for).
-int launch_mp_r(tea_hip_chunk* c, double alpha, double* partials) {
+// r -= alpha w and the partials of r.r into c->partials (not waited for);
+// *len is the number of partials left to finish.
+int launch_mp_r(tea_hip_chunk* c, double alpha, int* len) {
+  *len = c->nspan;
+  if (use_pair_blocks(c, 2)) {
+    {
+      Timed t(c, K_MP_R);
+      hipLaunchKernelGGL(mp_r1_kernel, pair_grid(c), dim3(kBlock), 0, c->stream,
+                         c->geo, c->mp_r, c->mp_w, alpha, c->sub, c->nblk);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    return launch_span_finish(c, 1, len);
+  }
   Timed t(c, K_MP_R);
   if (c->pairs)
     hipLaunchKernelGGL(mp_r2_kernel, span_grid(c), dim3(kBlock), 0, c->stream, c->geo,
-                       c->mp_r, c->mp_w, alpha, partials, c->nspan,
+                       c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
                        c->ngroup == 1 ? 1 : 0);
   else
     hipLaunchKernelGGL(mp_r_kernel, span_grid(c), dim3(kBlock), 0, c->stream, c->geo,
-                       c->mp_r, c->mp_w, alpha, partials, c->nspan,
+                       c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
                        c->ngroup == 1 ? 1 : 0);
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
@@ -503,7 +587,16 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
   MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->mp_r,
            c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky, nullptr,
            alpha, beta, apply_u ? 1 : 0, apply_p ? 1 : 0};
-  {
+  int len = c->nspan;
+  if (use_pair_blocks(c, 1)) {
+    {
+      Timed t(c, K_MP_W);
+      hipLaunchKernelGGL((mp_w1_kernel<false>), pair_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, a, c->sub, c->nblk);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    TEA_RETURN_IF(launch_span_finish(c, 1, &len));
+  } else {
     Timed t(c, K_MP_W);
     if (c->pairs)
       hipLaunchKernelGGL((mp_w2_kernel<false, 4>), span_grid(c), dim3(kBlock), 0,
@@ -516,7 +609,7 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
   }
   TEA_HIP_TRY(hipGetLastError());
   if (apply_p) c->mp_cur ^= 1;
-  return finish_reduction(c, 1, pw);
+  return finish_reduction_n(c, 1, len, pw);
 }
 
 int tea_hip_mpcg_calc_r(tea_hip_chunk* c, double alpha, double* rr) {
@@ -524,8 +617,9 @@ int tea_hip_mpcg_calc_r(tea_hip_chunk* c, double alpha, double* rr) {
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
   TEA_CHECK_ARG(rr != nullptr, "null rr");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_RETURN_IF(launch_mp_r(c, alpha, c->partials));
-  return finish_reduction(c, 1, rr);
+  int len;
+  TEA_RETURN_IF(launch_mp_r(c, alpha, &len));
+  return finish_reduction_n(c, 1, len, rr);
 }
 
 int tea_hip_mpcg_refresh(tea_hip_chunk* c, double alpha, int apply_u,

@@ -386,6 +386,8 @@ __global__ void mp_r_kernel(Geo g, float* r, const float* w, double alpha,
 template <bool DOT2, int SB>
 __global__ void mp_w2_kernel(Geo g, MpArgs a, double* partials, int nspan,
                              int single_group);
+template <bool DOT2>
+__global__ void mp_w1_kernel(Geo g, MpArgs a, double* sub, int nblk);
 
 }  // namespace tea
 
@@ -513,6 +515,9 @@ int launch_span_finish(tea_hip_chunk* c, int nr, int* len);
 // launch_span_finish leaves).
 int finish_reduction_n(tea_hip_chunk* c, int nr, int len, double* out);
 
+// The same with a length per array.
+int finish_reduction_lens(tea_hip_chunk* c, int nr, const int* lens, double* out);
+
 // The pair-per-thread kernels apply: x even (pairs) and more than one group
 // (the reference then runs its level-2 pass; for a single group it stops at
 // level 1, which the span-per-block kernels reproduce).
@@ -523,8 +528,8 @@ inline bool use_pair_blocks(const tea_hip_chunk* c, int kernel_bit) {
 // Applies any deferred CG work (fused mode).
 int apply_pending(tea_hip_chunk* c);
 // Mixed-precision residual update r -= alpha w with the partials of r.r
-// written to `partials` (mpcg.hip); the caller finishes the reduction.
-int launch_mp_r(tea_hip_chunk* c, double alpha, double* partials);
+// written to c->partials (mpcg.hip); *len is the number left to finish.
+int launch_mp_r(tea_hip_chunk* c, double alpha, int* len);
 
 // One reflective-halo launch (all flagged fields and external faces).
 int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
