@@ -493,7 +493,7 @@ template <bool APPLY_P>
 __global__ __launch_bounds__(kBlock) void fused_w1_kernel(Geo g, FusedWArgs a,
                                                           double* sub,
                                                           int nblk) {
-  const int blk = xcd_span(blockIdx.x, gridDim.x);
+  const int blk = blockIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
   FusedPair s;
   fused_pair_index(g, blk * kPairBlock + 2 * threadIdx.x, s);
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(kBlock) void fused_r2_kernel(
 __global__ __launch_bounds__(kBlock) void fused_r1_kernel(
     Geo g, double* __restrict__ r, const double* __restrict__ w, double alpha,
     double* sub, int nblk) {
-  const int blk = xcd_span(blockIdx.x, gridDim.x);
+  const int blk = blockIdx.x;
   const int e = blk * kPairBlock + 2 * threadIdx.x;
   const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
   const int kk = e - row * g.x;

@@ -351,7 +351,7 @@ template __global__ void mp_w2_kernel<true, 4>(Geo, MpArgs, double*, int, int);
 template <bool DOT2>
 __global__ __launch_bounds__(kBlock) void mp_w1_kernel(Geo g, MpArgs a,
                                                        double* sub, int nblk) {
-  const int blk = xcd_span(blockIdx.x, gridDim.x);
+  const int blk = blockIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
   MpPair s;
   mp_pair_index(g, blk * kPairBlock + 2 * threadIdx.x, s);
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void mp_r2_kernel(
 __global__ __launch_bounds__(kBlock) void mp_r1_kernel(
     Geo g, float* __restrict__ r, const float* __restrict__ w, double alpha,
     double* sub, int nblk) {
-  const int blk = xcd_span(blockIdx.x, gridDim.x);
+  const int blk = blockIdx.x;
   const int e = blk * kPairBlock + 2 * threadIdx.x;
   const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
   const int kk = e - row * g.x;

@@ -254,6 +254,12 @@ __device__ inline void span_reduce_pairs_n(double (&v)[NR][kSlots], double* part
 // a block's 8-leaf subtree is an exact intermediate of that tree: blocks
 // store it in sub[q * nblk + block], and span_finish_kernel combines the 8
 // subtrees of each span into its level-2 partial.
+//
+// These kernels keep the launch order of blocks (no xcd_span renumbering):
+// the hardware deals consecutive blocks round-robin to the 8 XCDs, so the
+// whole chip streams through memory as one front, and rows re-read above and
+// below a block were fetched moments earlier (measured at 4096^2: 8% faster
+// than one contiguous range per XCD for 512-cell blocks).
 
 constexpr int kPairBlock = 2 * kBlock;              // 512 cells
 constexpr int kSubPerSpan = kSpan / kPairBlock;     // 8
