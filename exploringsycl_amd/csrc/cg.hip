@@ -499,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void fused_w1_kernel(Geo g, FusedWArgs a,
   fused_pair_index(g, blk * kPairBlock + 2 * threadIdx.x, s);
   fused_pair_load<APPLY_P>(g, a, s);
   double v[1] = {fused_pair_compute<APPLY_P>(g, a, s, lane)};
-  block_subtree_pairs<1>(v, sub, nblk, blk);
+  wave_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
 __global__ __launch_bounds__(kBlock) void fused_r2_kernel(
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void fused_r1_kernel(
   const double rb = rv.y - alpha * wv.y;
   double v[1] = {(in0 ? ra * ra : 0.0) + (in1 ? rb * rb : 0.0)};
   st2_nt(r, e, in0, in1, ra, rb);
-  block_subtree_pairs<1>(v, sub, nblk, blk);
+  wave_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
 // r -= alpha w and r.r: all 16 cells' loads first, then the updates.

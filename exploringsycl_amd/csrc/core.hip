@@ -93,46 +93,61 @@ static double host_tree_passes(double* v, size_t len) {
   return v[0];
 }
 
-// Level-2 partial of each span from its 8 block subtrees (zero for blocks
-// past the grid, as the reference pads its level-2 input with zeros): the
-// last three steps (s = 8, 16, 32) of the reference's level-2 tree.  Each
-// wave then holds 64 consecutive spans, one group of the next pass, and runs
-// that pass too (zero-padded past nspan), leaving ceil(nspan/64) partials;
-// with a single span there is no further pass and its partial is the sum.
+// Level-2 partial of each span from the 32 wave sums of its 8 blocks (zero
+// for blocks past the grid, as the reference pads its level-2 input with
+// zeros), then the next tree pass.  A 512-thread block covers 64 spans, one
+// group of that pass: thread t takes 512-cell block 512 b + t and its four
+// contiguous wave sums (so the loads coalesce), forms (w0 + w1) + (w2 + w3),
+// lanes 8j..8j+7 then combine the span's 8 blocks pairwise (steps s = 8, 16,
+// 32 of the level-2 tree), and wave 0 runs the group's tree over the 64 span
+// partials (zero past nspan).  With a single span there is no further pass
+// and its partial is the sum.
+constexpr int kFinishBlock = kSubPerSpan * kWave;  // 512 threads = 64 spans
+
 template <int NR>
-__global__ __launch_bounds__(kBlock) void span_finish_kernel(
+__global__ __launch_bounds__(kFinishBlock) void span_finish_kernel(
     const double* __restrict__ sub, int nblk, double* partials, int nspan) {
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & (kWave - 1);
+  __shared__ double lds[NR][kWave];
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const int blk = blockIdx.x * kFinishBlock + t;
+  const int span = blk / kSubPerSpan;
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
-    double b[kSubPerSpan];
+    const double* __restrict__ sq = sub + (size_t)q * kWaves * nblk;
+    double w[kWaves];
 #pragma unroll
-    for (int k = 0; k < kSubPerSpan; ++k) {
-      const int i = s * kSubPerSpan + k;
-      b[k] = (s < nspan && i < nblk) ? sub[q * nblk + i] : 0.0;
-    }
-    const double span_part =
-        ((b[0] + b[1]) + (b[2] + b[3])) + ((b[4] + b[5]) + (b[6] + b[7]));
+    for (int k = 0; k < kWaves; ++k)
+      w[k] = (blk < nblk && span < nspan) ? sq[kWaves * blk + k] : 0.0;
+    double v = (w[0] + w[1]) + (w[2] + w[3]);
+    v = v + __shfl_xor(v, 1);
+    v = v + __shfl_xor(v, 2);
+    v = v + __shfl_xor(v, 4);
+    if ((t & (kSubPerSpan - 1)) == 0) lds[q][t / kSubPerSpan] = v;
+  }
+  __syncthreads();
+  if (t >= kWave) return;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
     if (nspan == 1) {
-      if (s == 0) partials[q * nspan] = span_part;
+      if (blockIdx.x == 0 && lane == 0) partials[q * nspan] = lds[q][0];
       continue;
     }
-    const double g3 = wave_tree(span_part);
-    if (lane == 0 && s < nspan) partials[q * nspan + (s / kWave)] = g3;
+    const double g3 = wave_tree(lds[q][lane]);
+    if (lane == 0) partials[q * nspan + blockIdx.x] = g3;
   }
 }
 
 int launch_span_finish(tea_hip_chunk* c, int nr, int* len) {
   TEA_CHECK_ARG(nr >= 1 && nr <= kSubMaxNr, "span_finish: bad reduction count");
-  const dim3 grid((unsigned)((c->nspan + kBlock - 1) / kBlock));
+  const dim3 grid((unsigned)((c->nspan + kWave - 1) / kWave));
   {
     Timed t(c, K_SPAN_FINISH);
     if (nr == 1)
-      hipLaunchKernelGGL(span_finish_kernel<1>, grid, dim3(kBlock), 0,
+      hipLaunchKernelGGL(span_finish_kernel<1>, grid, dim3(kFinishBlock), 0,
                          c->stream, c->sub, c->nblk, c->partials, c->nspan);
     else
-      hipLaunchKernelGGL(span_finish_kernel<2>, grid, dim3(kBlock), 0,
+      hipLaunchKernelGGL(span_finish_kernel<2>, grid, dim3(kFinishBlock), 0,
                          c->stream, c->sub, c->nblk, c->partials, c->nspan);
   }
   TEA_HIP_TRY(hipGetLastError());
@@ -251,7 +266,7 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
     set_error("hipMalloc of the second p buffer failed");
     return fail(TEA_HIP_ENOMEM);
   }
-  if (hipMalloc(&c->sub, (size_t)kSubMaxNr * c->nblk * sizeof(double)) != hipSuccess) {
+  if (hipMalloc(&c->sub, (size_t)kSubMaxNr * kWaves * c->nblk * sizeof(double)) != hipSuccess) {
     c->sub = nullptr;
     set_error("hipMalloc of the subtree sums failed");
     return fail(TEA_HIP_ENOMEM);

@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void mp_w1_kernel(Geo g, MpArgs a,
   mp_pair_load<DOT2>(g, a, s);
   double ps[DOT2 ? 2 : 1];
   mp_pair_compute<DOT2>(g, a, s, lane, ps);
-  block_subtree_pairs<DOT2 ? 2 : 1>(ps, sub, nblk, blk);
+  wave_subtree_pairs<DOT2 ? 2 : 1>(ps, sub, nblk, blk);
 }
 
 template __global__ void mp_w1_kernel<true>(Geo, MpArgs, double*, int);
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void mp_r1_kernel(
   double v[1] = {(in0 ? (double)ra * (double)ra : 0.0) +
                  (in1 ? (double)rb * (double)rb : 0.0)};
   st2(r, e, in0, in1, ra, rb);
-  block_subtree_pairs<1>(v, sub, nblk, blk);
+  wave_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
 namespace {

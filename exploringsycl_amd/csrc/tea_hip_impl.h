@@ -266,12 +266,14 @@ constexpr int kSubPerSpan = kSpan / kPairBlock;     // 8
 constexpr int kSubMaxNr = 2;                        // dot products per launch
 
 // From each lane's in-lane pair sum (tree step 1): steps 2..32 within each
-// 32-lane half (one group each), then the wave's two groups, then the
-// block's four waves, as the 8-leaf subtree of the reference's level-2 tree.
+// 32-lane half (one group each), then the wave's two groups; lane 0 stores
+// that sum, g(2w) + g(2w+1), in sub[q * 4 * nblk + 4 * blk + w].  The
+// block's four wave sums are combined by span_finish_kernel (tree steps 2
+// and 4 of the 8-leaf subtree), so the streaming kernels need no LDS and no
+// barrier.
 template <int NR>
-__device__ inline void block_subtree_pairs(double (&v)[NR], double* sub,
-                                           int nblk, int blk) {
-  __shared__ double lds[NR][kWaves];
+__device__ inline void wave_subtree_pairs(double (&v)[NR], double* sub,
+                                          int nblk, int blk) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
 #pragma unroll
@@ -280,12 +282,7 @@ __device__ inline void block_subtree_pairs(double (&v)[NR], double* sub,
 #pragma unroll
     for (int m = 1; m < 32; m <<= 1) t = t + __shfl_xor(t, m);
     const double upper = __shfl_down(t, 32);
-    if (lane == 0) lds[q][wv] = t + upper;
-  }
-  __syncthreads();
-  if (threadIdx.x < NR) {
-    const int q = threadIdx.x;
-    sub[q * nblk + blk] = (lds[q][0] + lds[q][1]) + (lds[q][2] + lds[q][3]);
+    if (lane == 0) sub[(size_t)q * kWaves * nblk + kWaves * blk + wv] = t + upper;
   }
 }
 
@@ -446,7 +443,7 @@ struct tea_hip_chunk {
   // coherent host memory; kernels write through the device view.
   double* partials = nullptr;       // device view
   double* host_partials = nullptr;  // host view of the same memory
-  // Subtree sums of the pair-per-thread kernels: kSubMaxNr x nblk, device.
+  // Wave sums of the pair-per-thread kernels: kSubMaxNr x 4 x nblk, device.
   int nblk = 0;                     // 512-cell blocks covering x*y
   double* sub = nullptr;
   int pair_blocks = 3;              // pair-per-thread kernels: bit 0 fused_w,
