@@ -154,24 +154,30 @@ class DeviceChunk:
         return rr.value, bb.value
 
     def mpcg_calc_w(self, alpha, apply_u, beta, apply_p):
-        v = ctypes.c_double()
+        """Returns (p.w, p.r)."""
+        pw, pr = ctypes.c_double(), ctypes.c_double()
         check(self.lib.tea_hip_mpcg_calc_w(self.handle, alpha, int(apply_u), beta,
-                                           int(apply_p), ctypes.byref(v)),
+                                           int(apply_p), ctypes.byref(pw),
+                                           ctypes.byref(pr)),
               "mpcg_calc_w")
-        return v.value
+        return pw.value, pr.value
 
     def mpcg_calc_r(self, alpha):
-        v = ctypes.c_double()
-        check(self.lib.tea_hip_mpcg_calc_r(self.handle, alpha, ctypes.byref(v)),
+        """Returns (r.r, r.w) of the updated r."""
+        rr, rw = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.tea_hip_mpcg_calc_r(self.handle, alpha, ctypes.byref(rr),
+                                           ctypes.byref(rw)),
               "mpcg_calc_r")
-        return v.value
+        return rr.value, rw.value
 
     def mpcg_refresh(self, alpha, apply_u):
-        a, b = ctypes.c_double(), ctypes.c_double()
+        """Returns (r.r of the FP32 residual, r.r of the FP64 one, r.w)."""
+        a, b, w = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         check(self.lib.tea_hip_mpcg_refresh(self.handle, alpha, int(apply_u),
-                                            ctypes.byref(a), ctypes.byref(b)),
+                                            ctypes.byref(a), ctypes.byref(b),
+                                            ctypes.byref(w)),
               "mpcg_refresh")
-        return a.value, b.value
+        return a.value, b.value, w.value
 
     def read_mp(self, name):
         out = np.empty(self.x * self.y, dtype=np.float32)

@@ -371,7 +371,6 @@ __device__ inline double fused_pair_compute(const Geo& g, const FusedWArgs& a,
   const int x = g.x;
   const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
   const double* __restrict__ P = a.p_old;
-  const double* __restrict__ R = a.r;
   const double beta = a.beta;
   const int i = s.e0;
   // Neighbours across the pair boundary, from the adjacent lanes.

@@ -235,13 +235,13 @@ __global__ __launch_bounds__(kBlock) void fcg_z2_kernel(
     const double db = (1.0 + ((double)kxr + (double)KX.y)) + ((double)KYt.y + (double)KY.y);
     float za = 0.0f, zb = 0.0f;
     if (in0) {
-      double v = (double)R.x / da;
+      double v = NL < 0 ? (double)R.x : (double)R.x / da;
       if (NL > 0) v = v + t0.at(kk, row, g.h);
       if (NL > 1) v = v + t1.at(kk, row, g.h);
       za = (float)v;
     }
     if (in1) {
-      double v = (double)R.y / db;
+      double v = NL < 0 ? (double)R.y : (double)R.y / db;
       if (NL > 0) v = v + t0.at(kk + 1, row, g.h);
       if (NL > 1) v = v + t1.at(kk + 1, row, g.h);
       zb = (float)v;
@@ -583,7 +583,11 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, double* partials) 
   const CoarseTerm t1 = nl > 1 ? coarse_term(c, 1) : none;
   Timed t(c, nl == 0 ? K_FCG_JACOBI : K_FCG_PROLONG);
   if (c->pairs) {
-    if (nl == 0)
+    if (nl == 0 && getenv("TEA_FCG_IDENTITY"))
+      hipLaunchKernelGGL((fcg_z2_kernel<-1>), span_grid(c), dim3(kBlock), 0, c->stream, g,
+                         c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, partials,
+                         c->nspan, sg);
+    else if (nl == 0)
       hipLaunchKernelGGL((fcg_z2_kernel<0>), span_grid(c), dim3(kBlock), 0, c->stream, g,
                          c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, partials,
                          c->nspan, sg);
@@ -728,7 +732,7 @@ int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
   TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no network attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
   int lens[2] = {c->nspan, c->nspan};
-  TEA_RETURN_IF(launch_mp_r(c, alpha, &lens[0]));
+  TEA_RETURN_IF(launch_mp_r(c, alpha, &lens[0], false));
   TEA_RETURN_IF(launch_precondition(c, use_net, c->partials + c->nspan));
   double out[2];
   TEA_RETURN_IF(finish_reduction_lens(c, 2, lens, out));

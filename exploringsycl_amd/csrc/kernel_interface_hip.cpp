@@ -285,28 +285,32 @@ void run_mpcg_init(Chunk* chunk, Settings* settings, double rx, double ry,
 }
 
 void run_mpcg_calc_w(Chunk* chunk, Settings* settings, double alpha,
-                     int apply_u, double beta, int apply_p, double* pw) {
+                     int apply_u, double beta, int apply_p, double* pw,
+                     double* pr) {
   (void)settings;
-  double v;
-  HIP_CALL(tea_hip_mpcg_calc_w(H(chunk), alpha, apply_u, beta, apply_p, &v));
-  *pw += v;
+  double a, b;
+  HIP_CALL(tea_hip_mpcg_calc_w(H(chunk), alpha, apply_u, beta, apply_p, &a, &b));
+  *pw += a;
+  *pr += b;
 }
 
 void run_mpcg_calc_r(Chunk* chunk, Settings* settings, double alpha,
-                     double* rr) {
+                     double* rr, double* rw) {
   (void)settings;
-  double v;
-  HIP_CALL(tea_hip_mpcg_calc_r(H(chunk), alpha, &v));
-  *rr += v;
+  double a, b;
+  HIP_CALL(tea_hip_mpcg_calc_r(H(chunk), alpha, &a, &b));
+  *rr += a;
+  *rw += b;
 }
 
 void run_mpcg_refresh(Chunk* chunk, Settings* settings, double alpha,
-                      int apply_u, double* rr, double* rr_true) {
+                      int apply_u, double* rr, double* rr_true, double* rw) {
   (void)settings;
-  double a, b;
-  HIP_CALL(tea_hip_mpcg_refresh(H(chunk), alpha, apply_u, &a, &b));
+  double a, b, w;
+  HIP_CALL(tea_hip_mpcg_refresh(H(chunk), alpha, apply_u, &a, &b, &w));
   *rr += a;
   *rr_true += b;
+  *rw += w;
 }
 
 void run_read_u(Chunk* chunk, Settings* settings, double* host) {

@@ -131,6 +131,8 @@ __global__ __launch_bounds__(kBlock) void mp_w_kernel(Geo g, MpArgs a,
 
 template __global__ void mp_w_kernel<4, true>(Geo, MpArgs, double*, int, int);
 
+// r -= alpha w and r.r (and, with RW, r.w of the updated r).
+template <bool RW>
 __global__ __launch_bounds__(kBlock) void mp_r_kernel(
     Geo g, float* __restrict__ r, const float* __restrict__ w, double alpha,
     double* partials, int nspan, int single_group) {
@@ -147,14 +149,15 @@ __global__ __launch_bounds__(kBlock) void mp_r_kernel(
     rv[k] = r[i];
     wv[k] = w[i];
   }
-  double prod[1][kItems];
+  double prod[RW ? 2 : 1][kItems];
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     const float rn = (float)((double)rv[k] - alpha * (double)wv[k]);
     prod[0][k] = in[k] ? (double)rn * (double)rn : 0.0;
+    if (RW) prod[RW ? 1 : 0][k] = in[k] ? (double)rn * (double)wv[k] : 0.0;
     if (in[k]) r[base + k * kBlock + threadIdx.x] = rn;
   }
-  span_reduce<1>(prod, partials, nspan, span, single_group != 0);
+  span_reduce<RW ? 2 : 1>(prod, partials, nspan, span, single_group != 0);
 }
 
 // Two cells per lane (x even): the arithmetic of mp_w_kernel cell by cell,
@@ -363,7 +366,8 @@ __global__ __launch_bounds__(kBlock) void mp_w1_kernel(Geo g, MpArgs a,
 
 template __global__ void mp_w1_kernel<true>(Geo, MpArgs, double*, int);
 
-// r -= alpha w and r.r, two cells per lane.
+// r -= alpha w and r.r (with RW also r.w), two cells per lane.
+template <bool RW>
 __global__ __launch_bounds__(kBlock) void mp_r2_kernel(
     Geo g, float* __restrict__ r, const float* __restrict__ w, double alpha,
     double* partials, int nspan, int single_group) {
@@ -382,19 +386,23 @@ __global__ __launch_bounds__(kBlock) void mp_r2_kernel(
     rv[k] = ld2(r, i);
     wv[k] = ld2(w, i);
   }
-  double psum[1][kSlots];
+  double psum[RW ? 2 : 1][kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) {
     const float ra = (float)((double)rv[k].x - alpha * (double)wv[k].x);
     const float rb = (float)((double)rv[k].y - alpha * (double)wv[k].y);
     psum[0][k] = (in0[k] ? (double)ra * (double)ra : 0.0) +
                  (in1[k] ? (double)rb * (double)rb : 0.0);
+    if (RW)
+      psum[RW ? 1 : 0][k] = (in0[k] ? (double)ra * (double)wv[k].x : 0.0) +
+                            (in1[k] ? (double)rb * (double)wv[k].y : 0.0);
     st2(r, base + k * (2 * kBlock) + 2 * threadIdx.x, in0[k], in1[k], ra, rb);
   }
-  span_reduce_pairs_n<1>(psum, partials, nspan, span, single_group != 0);
+  span_reduce_pairs_n<RW ? 2 : 1>(psum, partials, nspan, span, single_group != 0);
 }
 
 // mp_r2_kernel's arithmetic with one pair per thread.
+template <bool RW>
 __global__ __launch_bounds__(kBlock) void mp_r1_kernel(
     Geo g, float* __restrict__ r, const float* __restrict__ w, double alpha,
     double* sub, int nblk) {
@@ -408,10 +416,13 @@ __global__ __launch_bounds__(kBlock) void mp_r1_kernel(
   const float2 rv = ld2(r, i), wv = ld2(w, i);
   const float ra = (float)((double)rv.x - alpha * (double)wv.x);
   const float rb = (float)((double)rv.y - alpha * (double)wv.y);
-  double v[1] = {(in0 ? (double)ra * (double)ra : 0.0) +
-                 (in1 ? (double)rb * (double)rb : 0.0)};
+  double v[RW ? 2 : 1];
+  v[0] = (in0 ? (double)ra * (double)ra : 0.0) + (in1 ? (double)rb * (double)rb : 0.0);
+  if (RW)
+    v[RW ? 1 : 0] = (in0 ? (double)ra * (double)wv.x : 0.0) +
+                    (in1 ? (double)rb * (double)wv.y : 0.0);
   st2(r, e, in0, in1, ra, rb);
-  wave_subtree_pairs<1>(v, sub, nblk, blk);
+  wave_subtree_pairs<RW ? 2 : 1>(v, sub, nblk, blk);
 }
 
 namespace {
@@ -478,8 +489,9 @@ struct MpTrueResidualBody {
   const double* kx;
   const double* ky;
   float* r32;
+  const float* w;
   int x;
-  __device__ void operator()(int i, int, int, double (&out)[2]) const {
+  __device__ void operator()(int i, int, int, double (&out)[3]) const {
     const double kx0 = kx[i], kx1 = kx[i + 1], ky0 = ky[i], ky1 = ky[i + x];
     const double diag = (1.0 + (kx1 + kx0)) + (ky1 + ky0);
     const double au = (diag * u[i] - (kx1 * u[i + 1] + kx0 * u[i - 1])) -
@@ -489,6 +501,7 @@ struct MpTrueResidualBody {
     r32[i] = rf;
     out[0] = (double)rf * (double)rf;
     out[1] = rt * rt;
+    out[2] = (double)rf * (double)w[i];
   }
 };
 
@@ -510,28 +523,41 @@ int ensure_mp_buffers(tea_hip_chunk* c) {
 
 }  // namespace
 
-// r -= alpha w and the partials of r.r into c->partials (not waited for);
-// *len is the number of partials left to finish.
-int launch_mp_r(tea_hip_chunk* c, double alpha, int* len) {
+// r -= alpha w and the partials of r.r (and, with rw, of r.w in the second
+// array) into c->partials (not waited for); *len is the number of partials
+// left to finish in each array.
+int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw) {
   *len = c->nspan;
   if (use_pair_blocks(c, 2)) {
     {
       Timed t(c, K_MP_R);
-      hipLaunchKernelGGL(mp_r1_kernel, pair_grid(c), dim3(kBlock), 0, c->stream,
-                         c->geo, c->mp_r, c->mp_w, alpha, c->sub, c->nblk);
+      if (rw)
+        hipLaunchKernelGGL(mp_r1_kernel<true>, pair_grid(c), dim3(kBlock), 0, c->stream,
+                           c->geo, c->mp_r, c->mp_w, alpha, c->sub, c->nblk);
+      else
+        hipLaunchKernelGGL(mp_r1_kernel<false>, pair_grid(c), dim3(kBlock), 0, c->stream,
+                           c->geo, c->mp_r, c->mp_w, alpha, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
-    return launch_span_finish(c, 1, len);
+    return launch_span_finish(c, rw ? 2 : 1, len);
   }
   Timed t(c, K_MP_R);
-  if (c->pairs)
-    hipLaunchKernelGGL(mp_r2_kernel, span_grid(c), dim3(kBlock), 0, c->stream, c->geo,
-                       c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
-                       c->ngroup == 1 ? 1 : 0);
-  else
-    hipLaunchKernelGGL(mp_r_kernel, span_grid(c), dim3(kBlock), 0, c->stream, c->geo,
-                       c->mp_r, c->mp_w, alpha, c->partials, c->nspan,
-                       c->ngroup == 1 ? 1 : 0);
+  const int sg = c->ngroup == 1 ? 1 : 0;
+  if (c->pairs) {
+    if (rw)
+      hipLaunchKernelGGL(mp_r2_kernel<true>, span_grid(c), dim3(kBlock), 0, c->stream,
+                         c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan, sg);
+    else
+      hipLaunchKernelGGL(mp_r2_kernel<false>, span_grid(c), dim3(kBlock), 0, c->stream,
+                         c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan, sg);
+  } else {
+    if (rw)
+      hipLaunchKernelGGL(mp_r_kernel<true>, span_grid(c), dim3(kBlock), 0, c->stream,
+                         c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan, sg);
+    else
+      hipLaunchKernelGGL(mp_r_kernel<false>, span_grid(c), dim3(kBlock), 0, c->stream,
+                         c->geo, c->mp_r, c->mp_w, alpha, c->partials, c->nspan, sg);
+  }
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
 }
@@ -579,54 +605,64 @@ int tea_hip_mpcg_init(tea_hip_chunk* c, int coefficient, double rx, double ry,
 }
 
 int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
-                        double beta, int apply_p, double* pw) {
+                        double beta, int apply_p, double* pw, double* pr) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
-  TEA_CHECK_ARG(pw != nullptr, "null pw");
+  TEA_CHECK_ARG(pw != nullptr && pr != nullptr, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
+  // The two-dot kernel of flexible CG with z = r: p = r + beta p_old,
+  // w = A p, p.w and p.r.
   MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->mp_r,
-           c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky, nullptr,
+           c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky, c->mp_r,
            alpha, beta, apply_u ? 1 : 0, apply_p ? 1 : 0};
   int len = c->nspan;
   if (use_pair_blocks(c, 1)) {
     {
       Timed t(c, K_MP_W);
-      hipLaunchKernelGGL((mp_w1_kernel<false>), pair_grid(c), dim3(kBlock), 0,
+      hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
                          c->stream, c->geo, a, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
-    TEA_RETURN_IF(launch_span_finish(c, 1, &len));
+    TEA_RETURN_IF(launch_span_finish(c, 2, &len));
   } else {
     Timed t(c, K_MP_W);
     if (c->pairs)
-      hipLaunchKernelGGL((mp_w2_kernel<false, 4>), span_grid(c), dim3(kBlock), 0,
+      hipLaunchKernelGGL((mp_w2_kernel<true, 4>), span_grid(c), dim3(kBlock), 0,
                          c->stream, c->geo, a, c->partials, c->nspan,
                          c->ngroup == 1 ? 1 : 0);
     else
-      hipLaunchKernelGGL((mp_w_kernel<4, false>), span_grid(c), dim3(kBlock), 0,
+      hipLaunchKernelGGL((mp_w_kernel<4, true>), span_grid(c), dim3(kBlock), 0,
                          c->stream, c->geo, a, c->partials, c->nspan,
                          c->ngroup == 1 ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
   if (apply_p) c->mp_cur ^= 1;
-  return finish_reduction_n(c, 1, len, pw);
+  double out[2];
+  TEA_RETURN_IF(finish_reduction_n(c, 2, len, out));
+  *pw = out[0];
+  *pr = out[1];
+  return TEA_HIP_OK;
 }
 
-int tea_hip_mpcg_calc_r(tea_hip_chunk* c, double alpha, double* rr) {
+int tea_hip_mpcg_calc_r(tea_hip_chunk* c, double alpha, double* rr, double* rw) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
-  TEA_CHECK_ARG(rr != nullptr, "null rr");
+  TEA_CHECK_ARG(rr != nullptr && rw != nullptr, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
   int len;
-  TEA_RETURN_IF(launch_mp_r(c, alpha, &len));
-  return finish_reduction_n(c, 1, len, rr);
+  TEA_RETURN_IF(launch_mp_r(c, alpha, &len, true));
+  double out[2];
+  TEA_RETURN_IF(finish_reduction_n(c, 2, len, out));
+  *rr = out[0];
+  *rw = out[1];
+  return TEA_HIP_OK;
 }
 
 int tea_hip_mpcg_refresh(tea_hip_chunk* c, double alpha, int apply_u,
-                         double* rr, double* rr_true) {
+                         double* rr, double* rr_true, double* rw) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
-  TEA_CHECK_ARG(rr && rr_true, "null outputs");
+  TEA_CHECK_ARG(rr && rr_true && rw, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
   double** f = c->field;
   const Geo g = c->geo;
@@ -639,16 +675,17 @@ int tea_hip_mpcg_refresh(tea_hip_chunk* c, double alpha, int apply_u,
   {
     Timed t(c, K_MP_REFRESH);
     MpTrueResidualBody body{f[TEA_FIELD_U], f[TEA_FIELD_U0], f[TEA_FIELD_KX],
-                            f[TEA_FIELD_KY], c->mp_r, g.x};
-    hipLaunchKernelGGL((span_kernel<2, MpTrueResidualBody>), span_grid(c),
+                            f[TEA_FIELD_KY], c->mp_r, c->mp_w, g.x};
+    hipLaunchKernelGGL((span_kernel<3, MpTrueResidualBody>), span_grid(c),
                        dim3(kBlock), 0, c->stream, g, body, c->partials,
                        c->nspan, c->ngroup == 1 ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
-  double out[2];
-  TEA_RETURN_IF(finish_reduction(c, 2, out));
+  double out[3];
+  TEA_RETURN_IF(finish_reduction(c, 3, out));
   *rr = out[0];
   *rr_true = out[1];
+  *rw = out[2];
   return TEA_HIP_OK;
 }
 

@@ -384,6 +384,7 @@ struct MpArgs {
 template <int kBatch, bool DOT2>
 __global__ void mp_w_kernel(Geo g, MpArgs a, double* partials, int nspan,
                             int single_group);
+template <bool RW>
 __global__ void mp_r_kernel(Geo g, float* r, const float* w, double alpha,
                             double* partials, int nspan, int single_group);
 template <bool DOT2, int SB>
@@ -531,8 +532,9 @@ inline bool use_pair_blocks(const tea_hip_chunk* c, int kernel_bit) {
 // Applies any deferred CG work (fused mode).
 int apply_pending(tea_hip_chunk* c);
 // Mixed-precision residual update r -= alpha w with the partials of r.r
-// written to c->partials (mpcg.hip); *len is the number left to finish.
-int launch_mp_r(tea_hip_chunk* c, double alpha, int* len);
+// (and, with rw, of r.w as a second array) written to c->partials
+// (mpcg.hip); *len is the number left to finish in each.
+int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw);
 
 // One reflective-halo launch (all flagged fields and external faces).
 int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],

@@ -11,15 +11,17 @@
 
 /* Mixed-precision CG: FP32 p, r, w, kx, ky; FP64 u, dot products, alpha,
  * beta and a periodic FP64 true-residual refresh.  One chunk, all faces
- * external. */
+ * external.  calc_w returns p.w and p.r, calc_r and refresh r.r and r.w,
+ * for the flexible CG(1) recursion alpha = p.r / p.w, beta = -r.w / p.w. */
 void run_mpcg_init(Chunk* chunk, Settings* settings, double rx, double ry,
                    double* rr, double* bb);
 void run_mpcg_calc_w(Chunk* chunk, Settings* settings, double alpha,
-                     int apply_u, double beta, int apply_p, double* pw);
+                     int apply_u, double beta, int apply_p, double* pw,
+                     double* pr);
 void run_mpcg_calc_r(Chunk* chunk, Settings* settings, double alpha,
-                     double* rr);
+                     double* rr, double* rw);
 void run_mpcg_refresh(Chunk* chunk, Settings* settings, double alpha,
-                      int apply_u, double* rr, double* rr_true);
+                      int apply_u, double* rr, double* rr_true, double* rw);
 
 /* Flexible CG, FCG(1), with the FP32 vectors and FP64 scalars of the
  * mixed-precision CG.  z = M(r): Jacobi, or Jacobi plus the learned

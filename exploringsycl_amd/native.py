@@ -58,9 +58,9 @@ _SIGNATURES = {
     "tea_hip_calculate_2norm": [_vp, _vp, _dp],
     "tea_hip_finalise": [_vp],
     "tea_hip_mpcg_init": [_vp, _i, _d, _d, _dp, _dp],
-    "tea_hip_mpcg_calc_w": [_vp, _d, _i, _d, _i, _dp],
-    "tea_hip_mpcg_calc_r": [_vp, _d, _dp],
-    "tea_hip_mpcg_refresh": [_vp, _d, _i, _dp, _dp],
+    "tea_hip_mpcg_calc_w": [_vp, _d, _i, _d, _i, _dp, _dp],
+    "tea_hip_mpcg_calc_r": [_vp, _d, _dp, _dp],
+    "tea_hip_mpcg_refresh": [_vp, _d, _i, _dp, _dp, _dp],
     "tea_hip_mp_read": [_vp, _i, _fp, ctypes.c_size_t],
     "tea_hip_fcg_attach": [_vp, _fp, ctypes.c_size_t, _fp, _i, _i],
     "tea_hip_fcg_levels": [_vp, _ip],

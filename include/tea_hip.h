@@ -195,8 +195,12 @@ int tea_hip_finalise(tea_hip_chunk* chunk);
  * residual by the FP64 true residual u0 - A u.  One chunk per GPU: all four
  * faces must be external.  A solve is: init; then per iteration calc_w
  * (applying the previous alpha to u and beta to p) and calc_r; refresh every
- * few iterations and at convergence.  Dot products use the same reduction
- * tree as the FP64 path. */
+ * few iterations and at convergence.  The dots returned serve the flexible
+ * CG(1) recursion with the identity preconditioner (Notay 2000):
+ * alpha = p.r / p.w and beta = -r.w / p.w, which equal r.r / p.w and
+ * r'.r' / r.r in exact arithmetic but keep consecutive directions
+ * A-orthogonal when w = A p is formed in FP32.  Dot products use the same
+ * reduction tree as the FP64 path. */
 
 /* FP64 initialisation as tea_hip_cg_init, the depth-1 U/P halos and
  * u0 = u (tea_hip_copy_u), then the FP32 copies; *rr = r.r of the FP32
@@ -205,18 +209,20 @@ int tea_hip_mpcg_init(tea_hip_chunk* chunk, int coefficient, double rx,
                       double ry, double* rr, double* bb);
 
 /* If apply_u: u += alpha p_old (FP64).  If apply_p: p = beta p_old + r.
- * Then w = A p (FP32) and *pw = p.w. */
+ * Then w = A p (FP32), *pw = p.w and *pr = p.r. */
 int tea_hip_mpcg_calc_w(tea_hip_chunk* chunk, double alpha, int apply_u,
-                        double beta, int apply_p, double* pw);
+                        double beta, int apply_p, double* pw, double* pr);
 
-/* r -= alpha w and *rr = r.r. */
-int tea_hip_mpcg_calc_r(tea_hip_chunk* chunk, double alpha, double* rr);
+/* r -= alpha w, *rr = r.r and *rw = r.w (the updated r). */
+int tea_hip_mpcg_calc_r(tea_hip_chunk* chunk, double alpha, double* rr,
+                        double* rw);
 
 /* If apply_u: u += alpha p.  Then r = u0 - A u in FP64 (stored as FP32);
  * *rr = r.r of the stored FP32 residual, *rr_true = the same of the FP64
- * residual.  u is then current (halo ring included). */
+ * residual, *rw = r.w of the stored residual and the current w.  u is then
+ * current (halo ring included). */
 int tea_hip_mpcg_refresh(tea_hip_chunk* chunk, double alpha, int apply_u,
-                         double* rr, double* rr_true);
+                         double* rr, double* rr_true, double* rw);
 
 /* Copies one of the FP32 arrays (TEA_MP_*) to host memory. */
 enum { TEA_MP_P = 0, TEA_MP_R = 1, TEA_MP_W = 2, TEA_MP_KX = 3, TEA_MP_KY = 4 };

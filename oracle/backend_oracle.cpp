@@ -216,27 +216,32 @@ void run_mpcg_init(Chunk* c, Settings* s, double rx, double ry, double* rr,
 }
 
 void run_mpcg_calc_w(Chunk* c, Settings* s, double alpha, int apply_u,
-                     double beta, int apply_p, double* pw) {
+                     double beta, int apply_p, double* pw, double* pr) {
   ChunkExtension* e = c->ext;
   *pw += orc_mpcg_calc_w(c->x, c->y, s->halo_depth, alpha, apply_u, beta,
                          apply_p, e->mp_p[e->mp_cur], e->mp_p[e->mp_cur ^ 1],
                          e->mp_r, c->u, e->mp_w, e->mp_kx, e->mp_ky);
   if (apply_p) e->mp_cur ^= 1;
+  *pr += orc_dot32(c->x, c->y, s->halo_depth, e->mp_p[e->mp_cur], e->mp_r);
 }
 
-void run_mpcg_calc_r(Chunk* c, Settings* s, double alpha, double* rr) {
-  *rr += orc_mpcg_calc_r(c->x, c->y, s->halo_depth, alpha, c->ext->mp_r,
-                         c->ext->mp_w);
+void run_mpcg_calc_r(Chunk* c, Settings* s, double alpha, double* rr,
+                     double* rw) {
+  ChunkExtension* e = c->ext;
+  *rr += orc_mpcg_calc_r(c->x, c->y, s->halo_depth, alpha, e->mp_r, e->mp_w);
+  *rw += orc_dot32(c->x, c->y, s->halo_depth, e->mp_r, e->mp_w);
 }
 
 void run_mpcg_refresh(Chunk* c, Settings* s, double alpha, int apply_u,
-                      double* rr, double* rr_true) {
+                      double* rr, double* rr_true, double* rw) {
   ChunkExtension* e = c->ext;
-  double out[2];
+  double out[3];
   orc_mpcg_refresh(c->x, c->y, s->halo_depth, alpha, apply_u, c->u,
-                   e->mp_p[e->mp_cur], c->u0, c->kx, c->ky, e->mp_r, out);
+                   e->mp_p[e->mp_cur], c->u0, c->kx, c->ky, e->mp_r, e->mp_w,
+                   out);
   *rr += out[0];
   *rr_true += out[1];
+  *rw += out[2];
 }
 
 // ---- flexible CG (Jacobi preconditioner only) ----------------------------
@@ -270,7 +275,8 @@ void run_fcg_precondition(Chunk* c, Settings* s, int use_net, double* zw) {
 
 void run_fcg_calc_r_precondition(Chunk* c, Settings* s, double alpha,
                                  int use_net, double* rr, double* zw) {
-  run_mpcg_calc_r(c, s, alpha, rr);
+  *rr += orc_mpcg_calc_r(c->x, c->y, s->halo_depth, alpha, c->ext->mp_r,
+                         c->ext->mp_w);
   run_fcg_precondition(c, s, use_net, zw);
 }
 

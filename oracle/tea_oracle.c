@@ -456,10 +456,16 @@ double orc_mpcg_calc_r(int x, int y, int h, double alpha, float* r,
   return finish(part, x, y);
 }
 
+double orc_dot32(int x, int y, int h, const float* a, const float* b) {
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, { val = (double)a[i] * (double)b[i]; });
+  return finish(part, x, y);
+}
+
 void orc_mpcg_refresh(int x, int y, int h, double alpha, int apply_u,
                       double* u, const float* p, const double* u0,
                       const double* kx, const double* ky, float* r32,
-                      double out[2]) {
+                      const float* w, double out[3]) {
   if (apply_u) {
     for (long jj = h; jj < y - h; ++jj)
       for (long kk = h; kk < x - h; ++kk) {
@@ -470,28 +476,32 @@ void orc_mpcg_refresh(int x, int y, int h, double alpha, int apply_u,
   }
   double* part0 = alloc_partials(x, y);
   double* part1 = alloc_partials(x, y);
+  double* part2 = alloc_partials(x, y);
   const long n_ = (long)x * y, groups_ = (n_ + GROUP - 1) / GROUP;
 #pragma omp parallel for schedule(static)
   for (long g = 0; g < groups_; ++g) {
-    double a0[GROUP], a1[GROUP];
+    double a0[GROUP], a1[GROUP], a2[GROUP];
     long i0 = g * GROUP, jj = i0 / x, kk = i0 - jj * x;
     for (int l = 0; l < GROUP; ++l) {
       const long i = i0 + l;
-      a0[l] = a1[l] = 0.0;
+      a0[l] = a1[l] = a2[l] = 0.0;
       if (i < n_ && is_interior(kk, jj, x, y, h)) {
         const double rt = u0[i] - orc_smvp(u, kx, ky, i, x);
         const float rf = (float)rt;
         r32[i] = rf;
         a0[l] = (double)rf * (double)rf;
         a1[l] = rt * rt;
+        a2[l] = (double)rf * (double)w[i];
       }
       if (++kk == x) { kk = 0; ++jj; }
     }
     part0[g] = orc_tree64(a0);
     part1[g] = orc_tree64(a1);
+    part2[g] = orc_tree64(a2);
   }
   out[0] = finish(part0, x, y);
   out[1] = finish(part1, x, y);
+  out[2] = finish(part2, x, y);
 }
 
 /* ---- flexible CG (no reference counterpart) ---------------------------

@@ -113,7 +113,11 @@ double orc_mpcg_calc_r(int x, int y, int h, double alpha, float* r,
 void orc_mpcg_refresh(int x, int y, int h, double alpha, int apply_u,
                       double* u, const float* p, const double* u0,
                       const double* kx, const double* ky, float* r32,
-                      double out[2]);
+                      const float* w, double out[3]);
+/* Sum over the interior of a.b (FP64 products of FP32 values) with the
+ * reference's reduction tree: p.r and r.w of the mixed-precision CG's
+ * flexible recursion (alpha = p.r / p.w, beta = -r.w / p.w). */
+double orc_dot32(int x, int y, int h, const float* a, const float* b);
 
 /* Flexible CG (no reference counterpart; restates the arithmetic of
  * exploringsycl_amd/csrc/fcg.hip).  orc_fcg_jacobi: z = r / diag(A),

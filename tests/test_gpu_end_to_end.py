@@ -127,6 +127,26 @@ def test_mixed_precision_close_to_fp64():
     assert abs(a["temp"] / b["temp"] - 1) < 1e-9
 
 
+def test_mixed_precision_converges_on_stiff_field():
+    """The 4096^2 field of 64 rectangles with log10 density in [-2, 2] (the
+    time-to-solution workload): with the Fletcher-Reeves recursion
+    beta = rr'/rr the FP32 iteration reached 1.4e-10 and then diverged
+    (1.7e-9 after 12000 iterations); the flexible recursion reaches 1e-10
+    in about as many iterations as FP64 CG."""
+    base = dict(density="log10", density_range=(-2.0, 2.0), end_step=1,
+                coefficient="inverse", max_iters=12000)
+    seed = problem_seed(5, 3000)
+    fp64, _ = run_deck(Deck.with_random_rectangles(4096, 4096, 64, seed,
+                                                   extra=["rel_eps=1e-10"], **base))
+    mp, _ = run_deck(Deck.with_random_rectangles(4096, 4096, 64, seed,
+                                                 extra=["use_mpcg", "rel_eps=1e-10"],
+                                                 **base))
+    n64, nmp = fp64["steps"][0]["iterations"], mp["steps"][0]["iterations"]
+    assert mp["steps"][0]["rel_residual"] < 1e-10
+    assert nmp <= 1.05 * n64, (nmp, n64)
+    assert abs(fp64["summaries"][-1]["temp"] / mp["summaries"][-1]["temp"] - 1) < 1e-9
+
+
 JPCG_DECKS = {
     "jpcg_128x128x2": Deck.with_random_rectangles(
         128, 128, 24, problem_seed(5, 1), density="log10", density_range=(-2.0, 2.0),

@@ -275,10 +275,11 @@ def test_mixed_precision_kernels_match_oracle(oracle):
                                + [F, F, F, P, F, F, F],
             "orc_mpcg_calc_r": [ctypes.c_int] * 3 + [ctypes.c_double, F, F],
             "orc_mpcg_refresh": [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int,
-                                                      P, F, P, P, P, F, P]}.items():
+                                                      P, F, P, P, P, F, F, P],
+            "orc_dot32": [ctypes.c_int] * 3 + [F, F]}.items():
         getattr(L, fn).argtypes = args
-    getattr(L, "orc_mpcg_calc_w").restype = ctypes.c_double
-    getattr(L, "orc_mpcg_calc_r").restype = ctypes.c_double
+    for fn in ("orc_mpcg_calc_w", "orc_mpcg_calc_r", "orc_dot32"):
+        getattr(L, fn).restype = ctypes.c_double
     n = x * y
     mp = {k: np.zeros(n, dtype=np.float32) for k in ("p0", "p1", "r", "w", "kx", "ky")}
 
@@ -295,7 +296,7 @@ def test_mixed_precision_kernels_match_oracle(oracle):
         for nm in ("u", "p"):
             oracle.halo_field(x, y, H, 1, f[nm])
         oracle.copy_u(x, y, H, f)
-        out = np.zeros(2)
+        out = np.zeros(3)
         L.orc_mpcg_convert(x, y, H, dp(f["kx"]), dp(f["ky"]), dp(f["p"]), dp(f["r"]),
                            dp(f["u0"]), fp(mp["kx"]), fp(mp["ky"]), fp(mp["p0"]),
                            fp(mp["r"]), dp(out))
@@ -305,7 +306,9 @@ def test_mixed_precision_kernels_match_oracle(oracle):
         alpha = beta = 0.0
         pu = pp = False
         for it in range(30):
-            pw = c.mpcg_calc_w(alpha, pu, beta, pp)
+            # The flexible recursion of the mixed-precision driver:
+            # alpha = p.r / p.w, beta = -r.w / p.w.
+            pw, pr = c.mpcg_calc_w(alpha, pu, beta, pp)
             po, pn = (mp["p0"], mp["p1"]) if cur == 0 else (mp["p1"], mp["p0"])
             pw_o = L.orc_mpcg_calc_w(x, y, H, alpha, int(pu), beta, int(pp), fp(po),
                                      fp(pn), fp(mp["r"]), dp(f["u"]), fp(mp["w"]),
@@ -313,20 +316,22 @@ def test_mixed_precision_kernels_match_oracle(oracle):
             if pp:
                 cur ^= 1
             assert _same(pw, pw_o), (it, pw, pw_o)
-            alpha = rr / pw
-            rrn = c.mpcg_calc_r(alpha)
+            pcur = mp["p0"] if cur == 0 else mp["p1"]
+            assert _same(pr, L.orc_dot32(x, y, H, fp(pcur), fp(mp["r"]))), it
+            alpha = pr / pw
+            rrn, rw = c.mpcg_calc_r(alpha)
             assert _same(rrn, L.orc_mpcg_calc_r(x, y, H, alpha, fp(mp["r"]), fp(mp["w"])))
+            assert _same(rw, L.orc_dot32(x, y, H, fp(mp["r"]), fp(mp["w"]))), it
             pu = True
             if it % 7 == 6:
-                a, b = c.mpcg_refresh(alpha, True)
-                L.orc_mpcg_refresh(x, y, H, alpha, 1, dp(f["u"]),
-                                   fp(mp["p0"] if cur == 0 else mp["p1"]), dp(f["u0"]),
-                                   dp(f["kx"]), dp(f["ky"]), fp(mp["r"]), dp(out))
-                assert _same(a, out[0]) and _same(b, out[1])
-                rrn, pu = a, False
-            beta = rrn / rr
+                a, b, rw = c.mpcg_refresh(alpha, True)
+                L.orc_mpcg_refresh(x, y, H, alpha, 1, dp(f["u"]), fp(pcur), dp(f["u0"]),
+                                   dp(f["kx"]), dp(f["ky"]), fp(mp["r"]), fp(mp["w"]),
+                                   dp(out))
+                assert _same(a, out[0]) and _same(b, out[1]) and _same(rw, out[2])
+                pu = False
+            beta = -rw / pw
             pp = True
-            rr = rrn
         for nm, arr in (("r", mp["r"]), ("w", mp["w"]),
                         ("p", mp["p0"] if cur == 0 else mp["p1"])):
             got = c.read_mp(nm).ravel()
