@@ -28,7 +28,8 @@ const char* const kKernelNames[K_NUM] = {
     "fcg_jacobi",     "fcg_restrict",    "fcg_net",       "fcg_prolong",
     "span_finish"};
 
-Timed::Timed(tea_hip_chunk* c_, int kid_) : c(c_), kid(kid_) {
+Timed::Timed(tea_hip_chunk* c_, int kid_, hipStream_t s)
+    : c(c_), kid(kid_), stream(s ? s : c_->stream) {
   if (!c->timing || !c->time_kernel[kid]) return;
   if (c->event_pool.size() < 2) {
     for (int i = 0; i < 64; ++i) {
@@ -41,12 +42,12 @@ Timed::Timed(tea_hip_chunk* c_, int kid_) : c(c_), kid(kid_) {
   c->event_pool.pop_back();
   stop = c->event_pool.back();
   c->event_pool.pop_back();
-  (void)hipEventRecord(start, c->stream);
+  (void)hipEventRecord(start, stream);
 }
 
 Timed::~Timed() {
   if (!start) return;
-  (void)hipEventRecord(stop, c->stream);
+  (void)hipEventRecord(stop, stream);
   c->inflight.push_back({kid, start, stop});
 }
 
@@ -138,17 +139,19 @@ __global__ __launch_bounds__(kFinishBlock) void span_finish_kernel(
   }
 }
 
-int launch_span_finish(tea_hip_chunk* c, int nr, int* len) {
+int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot) {
   TEA_CHECK_ARG(nr >= 1 && nr <= kSubMaxNr, "span_finish: bad reduction count");
+  TEA_CHECK_ARG(slot >= 0 && slot + nr <= 4, "span_finish: bad partials slot");
   const dim3 grid((unsigned)((c->nspan + kWave - 1) / kWave));
+  double* dst = c->partials + (size_t)slot * c->nspan;
   {
     Timed t(c, K_SPAN_FINISH);
     if (nr == 1)
       hipLaunchKernelGGL(span_finish_kernel<1>, grid, dim3(kFinishBlock), 0,
-                         c->stream, c->sub, c->nblk, c->partials, c->nspan);
+                         c->stream, c->sub, c->nblk, dst, c->nspan);
     else
       hipLaunchKernelGGL(span_finish_kernel<2>, grid, dim3(kFinishBlock), 0,
-                         c->stream, c->sub, c->nblk, c->partials, c->nspan);
+                         c->stream, c->sub, c->nblk, dst, c->nspan);
   }
   TEA_HIP_TRY(hipGetLastError());
   *len = c->nspan == 1 ? 1 : (c->nspan + kWave - 1) / kWave;
@@ -308,6 +311,12 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky, c->fcg_z})
     if (b) (void)hipFree(b);
   if (c->fcg_cross) (void)hipFree(c->fcg_cross);
+  if (c->fcg_side) {
+    (void)hipStreamSynchronize(c->fcg_side);
+    (void)hipStreamDestroy(c->fcg_side);
+  }
+  if (c->fcg_fork) (void)hipEventDestroy(c->fcg_fork);
+  if (c->fcg_join) (void)hipEventDestroy(c->fcg_join);
   for (FcgLevel& L : c->fcg_lv) {
     for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale})
       if (b) (void)hipFree(b);

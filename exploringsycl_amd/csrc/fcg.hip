@@ -235,13 +235,13 @@ __global__ __launch_bounds__(kBlock) void fcg_z2_kernel(
     const double db = (1.0 + ((double)kxr + (double)KX.y)) + ((double)KYt.y + (double)KY.y);
     float za = 0.0f, zb = 0.0f;
     if (in0) {
-      double v = NL < 0 ? (double)R.x : (double)R.x / da;
+      double v = (double)R.x / da;
       if (NL > 0) v = v + t0.at(kk, row, g.h);
       if (NL > 1) v = v + t1.at(kk, row, g.h);
       za = (float)v;
     }
     if (in1) {
-      double v = NL < 0 ? (double)R.y : (double)R.y / db;
+      double v = (double)R.y / db;
       if (NL > 0) v = v + t0.at(kk + 1, row, g.h);
       if (NL > 1) v = v + t1.at(kk + 1, row, g.h);
       zb = (float)v;
@@ -250,6 +250,52 @@ __global__ __launch_bounds__(kBlock) void fcg_z2_kernel(
     if (any) st2(z, e, in0, in1, za, zb);
   }
   span_reduce_pairs_n<1>(psum, partials, nspan, span, single_group != 0);
+}
+
+// fcg_z2_kernel's arithmetic with one pair per thread; the waves store their
+// z.w sums for span_finish_kernel (see wave_subtree_pairs).  Lane 63 loads
+// the right neighbour's kx with the pair's own loads.
+template <int NL>
+__global__ __launch_bounds__(kBlock) void fcg_z1_kernel(
+    Geo g, const float* __restrict__ r, const float* __restrict__ kx,
+    const float* __restrict__ ky, const float* __restrict__ w, CoarseTerm t0,
+    CoarseTerm t1, float* __restrict__ z, double* sub, int nblk) {
+  const int blk = blockIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int x = g.x;
+  const int e = blk * kPairBlock + 2 * threadIdx.x;
+  const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+  const int kk = e - row * x;
+  const bool in0 = e < g.n && interior(g, kk, row);
+  const bool in1 = e < g.n && interior(g, kk + 1, row);
+  const bool any = in0 || in1;
+  const int ic = e < g.n ? e : x + 2;
+  const int ir = any ? e : x + 2;
+  const float2 R = ld2(r, ir), KX = ld2(kx, ic), KY = ld2(ky, ir),
+               KYt = ld2(ky, ir + x), W = ld2(w, ir);
+  float kxn = 0.0f;
+  if (lane == kWave - 1) kxn = kx[ir + 2];
+  float kxr = __shfl_down(KX.x, 1);
+  if (lane == kWave - 1) kxr = kxn;
+  const double da = (1.0 + ((double)KX.y + (double)KX.x)) + ((double)KYt.x + (double)KY.x);
+  const double db = (1.0 + ((double)kxr + (double)KX.y)) + ((double)KYt.y + (double)KY.y);
+  float za = 0.0f, zb = 0.0f;
+  if (in0) {
+    double v = (double)R.x / da;
+    if (NL > 0) v = v + t0.at(kk, row, g.h);
+    if (NL > 1) v = v + t1.at(kk, row, g.h);
+    za = (float)v;
+  }
+  if (in1) {
+    double v = (double)R.y / db;
+    if (NL > 0) v = v + t0.at(kk + 1, row, g.h);
+    if (NL > 1) v = v + t1.at(kk + 1, row, g.h);
+    zb = (float)v;
+  }
+  double v[1] = {(in0 ? (double)za * (double)W.x : 0.0) +
+                 (in1 ? (double)zb * (double)W.y : 0.0)};
+  if (any) st2(z, e, in0, in1, za, zb);
+  wave_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
 // Block means of a coarser level from a finer one (factor f) and the
@@ -515,38 +561,49 @@ using namespace tea;
 // Coarse pipeline of one level: the coarse residual (from r for level 0,
 // from the level above otherwise), its local scale, the network, and the
 // factor g of the coarse direction.
-static int run_level(tea_hip_chunk* c, int l) {
+// The restriction of level l (block means of r, or of the finer level's
+// means), on stream st.
+static int run_restrict(tea_hip_chunk* c, int l, hipStream_t st) {
+  FcgLevel& L = c->fcg_lv[l];
+  const int nblk = (L.hc * L.wc + 255) / 256;
+  Timed t(c, K_FCG_RESTRICT, st);
+  if (l == 0) {
+    hipLaunchKernelGGL(restrict_kernel, dim3(nblk), dim3(256), 0, st, c->geo, L.c, L.hc,
+                       L.wc, c->mp_r, L.dis, L.rbar, L.part);
+  } else {
+    const FcgLevel& F = c->fcg_lv[l - 1];
+    hipLaunchKernelGGL(restrict_coarse_kernel, dim3(nblk), dim3(256), 0, st, F.wc,
+                       L.c / F.c, L.hc, L.wc, F.rbar, L.dis, L.rbar, L.part);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
+// The rest of level l after its restriction, on stream st: the scales, the
+// network input, the network, the unscaling and the level's factor.
+static int run_level(tea_hip_chunk* c, int l, hipStream_t st) {
   FcgLevel& L = c->fcg_lv[l];
   const int cn = L.hc * L.wc;
   const int nblk = (cn + 255) / 256;
   {
-    Timed t(c, K_FCG_RESTRICT);
-    if (l == 0) {
-      hipLaunchKernelGGL(restrict_kernel, dim3(nblk), dim3(256), 0, c->stream, c->geo,
-                         L.c, L.hc, L.wc, c->mp_r, L.dis, L.rbar, L.part);
-    } else {
-      const FcgLevel& F = c->fcg_lv[l - 1];
-      hipLaunchKernelGGL(restrict_coarse_kernel, dim3(nblk), dim3(256), 0, c->stream,
-                         F.wc, L.c / F.c, L.hc, L.wc, F.rbar, L.dis, L.rbar, L.part);
-    }
-    hipLaunchKernelGGL(rms_kernel, dim3(1), dim3(256), 0, c->stream, L.part, nblk,
-                       (long)cn, L.scale);
-    hipLaunchKernelGGL(box_rows_kernel, dim3(nblk), dim3(256), 0, c->stream, L.hc, L.wc,
-                       L.rbar, L.dis, L.tmp);
-    hipLaunchKernelGGL(box_cols_kernel, dim3(nblk), dim3(256), 0, c->stream, L.hc, L.wc,
-                       L.tmp, L.scale, L.sig);
-    hipLaunchKernelGGL(pack_kernel, dim3(nblk), dim3(256), 0, c->stream, L.hc, L.wc,
-                       L.rbar, L.dis, L.sig, L.feat, nn::input_tensor(L.net));
+    Timed t(c, K_FCG_RESTRICT, st);
+    hipLaunchKernelGGL(rms_kernel, dim3(1), dim3(256), 0, st, L.part, nblk, (long)cn,
+                       L.scale);
+    hipLaunchKernelGGL(box_rows_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc, L.rbar,
+                       L.dis, L.tmp);
+    hipLaunchKernelGGL(box_cols_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc, L.tmp,
+                       L.scale, L.sig);
+    hipLaunchKernelGGL(pack_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc, L.rbar,
+                       L.dis, L.sig, L.feat, nn::input_tensor(L.net));
   }
   {
-    Timed t(c, K_FCG_NET);
-    TEA_RETURN_IF(nn::forward_on_stream(L.net, c->stream));
-    hipLaunchKernelGGL(unscale_kernel, dim3(nblk), dim3(256), 0, c->stream, cn, L.dis,
-                       L.sig, nn::output_mutable(L.net));
-    hipLaunchKernelGGL(coarse_dots_kernel, dim3(nblk), dim3(256), 0, c->stream, L.hc,
-                       L.wc, nn::output(L.net), L.rbar, L.kc, L.part);
-    hipLaunchKernelGGL(gamma_kernel, dim3(1), dim3(256), 0, c->stream, L.part, nblk,
-                       L.scale);
+    Timed t(c, K_FCG_NET, st);
+    TEA_RETURN_IF(nn::forward_on_stream(L.net, st));
+    hipLaunchKernelGGL(unscale_kernel, dim3(nblk), dim3(256), 0, st, cn, L.dis, L.sig,
+                       nn::output_mutable(L.net));
+    hipLaunchKernelGGL(coarse_dots_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc,
+                       nn::output(L.net), L.rbar, L.kc, L.part);
+    hipLaunchKernelGGL(gamma_kernel, dim3(1), dim3(256), 0, st, L.part, nblk, L.scale);
   }
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
@@ -559,12 +616,30 @@ static CoarseTerm coarse_term(const tea_hip_chunk* c, int l) {
   return CoarseTerm{nn::output(L.net), L.scale + 1, shift, L.wc};
 }
 
-// z = M(r) and the partials of z.w into `partials` (not waited for).
-static int launch_precondition(tea_hip_chunk* c, int use_net, double* partials) {
+// z = M(r) and the partials of z.w into array `slot` of c->partials (not
+// waited for); *len is the number of partials left to finish.
+static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len) {
   const Geo g = c->geo;
   const int sg = c->ngroup == 1 ? 1 : 0;
   const int nl = use_net ? c->fcg_levels : 0;
-  for (int l = 0; l < nl; ++l) TEA_RETURN_IF(run_level(c, l));
+  double* partials = c->partials + (size_t)slot * c->nspan;
+  *len = c->nspan;
+  if (nl == 1) {
+    TEA_RETURN_IF(run_restrict(c, 0, c->stream));
+    TEA_RETURN_IF(run_level(c, 0, c->stream));
+  } else if (nl == 2) {
+    // Level 2 (from level 1's block means) on the side stream while level 1
+    // runs on the chunk's stream; both are small grids, so their launch
+    // latencies overlap.
+    TEA_RETURN_IF(run_restrict(c, 0, c->stream));
+    TEA_HIP_TRY(hipEventRecord(c->fcg_fork, c->stream));
+    TEA_HIP_TRY(hipStreamWaitEvent(c->fcg_side, c->fcg_fork, 0));
+    TEA_RETURN_IF(run_restrict(c, 1, c->fcg_side));
+    TEA_RETURN_IF(run_level(c, 1, c->fcg_side));
+    TEA_RETURN_IF(run_level(c, 0, c->stream));
+    TEA_HIP_TRY(hipEventRecord(c->fcg_join, c->fcg_side));
+    TEA_HIP_TRY(hipStreamWaitEvent(c->stream, c->fcg_join, 0));
+  }
   if (nl == 2) {
     FcgLevel& L1 = c->fcg_lv[0];
     FcgLevel& L2 = c->fcg_lv[1];
@@ -581,13 +656,29 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, double* partials) 
   const CoarseTerm none{nullptr, nullptr, 0, 0};
   const CoarseTerm t0 = nl > 0 ? coarse_term(c, 0) : none;
   const CoarseTerm t1 = nl > 1 ? coarse_term(c, 1) : none;
-  Timed t(c, nl == 0 ? K_FCG_JACOBI : K_FCG_PROLONG);
+  const int kid = nl == 0 ? K_FCG_JACOBI : K_FCG_PROLONG;
+  if (use_pair_blocks(c, 2)) {
+    {
+      Timed t(c, kid);
+      if (nl == 0)
+        hipLaunchKernelGGL((fcg_z1_kernel<0>), pair_grid(c), dim3(kBlock), 0, c->stream, g,
+                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, c->sub,
+                           c->nblk);
+      else if (nl == 1)
+        hipLaunchKernelGGL((fcg_z1_kernel<1>), pair_grid(c), dim3(kBlock), 0, c->stream, g,
+                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, c->sub,
+                           c->nblk);
+      else
+        hipLaunchKernelGGL((fcg_z1_kernel<2>), pair_grid(c), dim3(kBlock), 0, c->stream, g,
+                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, c->sub,
+                           c->nblk);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    return launch_span_finish(c, 1, len, slot);
+  }
+  Timed t(c, kid);
   if (c->pairs) {
-    if (nl == 0 && getenv("TEA_FCG_IDENTITY"))
-      hipLaunchKernelGGL((fcg_z2_kernel<-1>), span_grid(c), dim3(kBlock), 0, c->stream, g,
-                         c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, partials,
-                         c->nspan, sg);
-    else if (nl == 0)
+    if (nl == 0)
       hipLaunchKernelGGL((fcg_z2_kernel<0>), span_grid(c), dim3(kBlock), 0, c->stream, g,
                          c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, partials,
                          c->nspan, sg);
@@ -680,6 +771,12 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
       set_error("hipMalloc of the preconditioner's cross partials failed");
       return TEA_HIP_ENOMEM;
     }
+    if (!c->fcg_side)
+      TEA_HIP_TRY(hipStreamCreateWithFlags(&c->fcg_side, hipStreamNonBlocking));
+    if (!c->fcg_fork)
+      TEA_HIP_TRY(hipEventCreateWithFlags(&c->fcg_fork, hipEventDisableTiming));
+    if (!c->fcg_join)
+      TEA_HIP_TRY(hipEventCreateWithFlags(&c->fcg_join, hipEventDisableTiming));
   }
   return TEA_HIP_OK;
 }
@@ -720,8 +817,9 @@ int tea_hip_fcg_precondition(tea_hip_chunk* c, int use_net, double* zw) {
   TEA_CHECK_ARG(zw != nullptr, "null zw");
   TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no network attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_RETURN_IF(launch_precondition(c, use_net, c->partials));
-  return finish_reduction(c, 1, zw);
+  int len;
+  TEA_RETURN_IF(launch_precondition(c, use_net, 0, &len));
+  return finish_reduction_n(c, 1, len, zw);
 }
 
 int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
@@ -733,7 +831,7 @@ int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
   TEA_HIP_TRY(hipSetDevice(c->device));
   int lens[2] = {c->nspan, c->nspan};
   TEA_RETURN_IF(launch_mp_r(c, alpha, &lens[0], false));
-  TEA_RETURN_IF(launch_precondition(c, use_net, c->partials + c->nspan));
+  TEA_RETURN_IF(launch_precondition(c, use_net, 1, &lens[1]));
   double out[2];
   TEA_RETURN_IF(finish_reduction_lens(c, 2, lens, out));
   *rr = out[0];

@@ -474,6 +474,11 @@ struct tea_hip_chunk {
   int fcg_levels = 0;
   FcgLevel fcg_lv[kFcgMaxLevels];
   double* fcg_cross = nullptr;  // partials of the two levels' cross term
+  // With two levels, the second level's chain runs on a side stream
+  // (default priority) between a fork after the first level's restriction
+  // and a join before the joint factors.
+  hipStream_t fcg_side = nullptr;
+  hipEvent_t fcg_fork = nullptr, fcg_join = nullptr;
 
   // Timing.
   bool timing = false;
@@ -494,8 +499,9 @@ namespace tea {
 struct Timed {
   tea_hip_chunk* c;
   int kid;
+  hipStream_t stream;  // the chunk's stream unless given
   hipEvent_t start = nullptr, stop = nullptr;
-  Timed(tea_hip_chunk* c_, int kid_);
+  Timed(tea_hip_chunk* c_, int kid_, hipStream_t s = nullptr);
   ~Timed();
 };
 
@@ -513,7 +519,8 @@ int finish_reduction(tea_hip_chunk* c, int nr, double* out);
 // Combines the 8 subtree sums of each span in c->sub (nr dot products) into
 // the span's level-2 partials and runs the next tree pass over them, leaving
 // *len partials per dot product in c->partials.
-int launch_span_finish(tea_hip_chunk* c, int nr, int* len);
+// The partials go to arrays slot .. slot + nr - 1 of c->partials.
+int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot = 0);
 
 // finish_reduction over the first len partials of each array (len is what
 // launch_span_finish leaves).
