@@ -37,6 +37,7 @@ constexpr int kC = 32;   // hidden channels
 constexpr int kC0 = 8;   // input channels in memory (3 used)
 constexpr int kIn = 3;   // real input channels
 constexpr int kTaps = 9;
+constexpr int kFwdRows = 4;  // output rows per block of the multi-row forward kernel
 
 // FP32 master parameters, in the order W1 (32,3,3,3), b1 (32), W2..W5
 // (32,32,3,3) with b2..b5 (32), W6 (1,32,3,3), b6 (1); W[co][ci][ky][kx].
@@ -216,6 +217,82 @@ __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(
       o[q] = (bf16)(vv > 0.0f ? vv : 0.0f);
     }
     *reinterpret_cast<bf16x4*>(out + pix * kC + co0) = o;
+  }
+}
+
+// conv_fwd_lds_kernel over R output rows per block: the block stages R + 2
+// input rows once (instead of 3 per output row) and each wave keeps its
+// weight fragments in registers across the R rows (instead of reloading
+// them per 32-pixel tile).  Each tile's MFMA sequence, bias and ReLU are
+// those of conv_fwd_lds_kernel, so the output is bitwise the same.
+// Grid (W / 128, H / R, B).
+template <int CIN, int R>
+__global__ __launch_bounds__(256) void conv_fwd_rows_kernel(
+    const bf16* __restrict__ in, const bf16* __restrict__ wt,
+    const float* __restrict__ bias, bf16* __restrict__ out, int H, int W) {
+  constexpr int NS = nsteps<CIN>();
+  constexpr int CPC = CIN / 8;
+  constexpr int kPitch = CPC == 1 ? 1 : CPC + 1;
+  constexpr int kPx = 130;
+  __shared__ bf16x8 sm[R + 2][kPx * kPitch];
+  const int seg = blockIdx.x, y0 = blockIdx.y * R, b = blockIdx.z;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int hf = lane >> 5, col = lane & 31;
+  const int Wp = W + 2;
+  const size_t plane = (size_t)(H + 2) * Wp;
+  const bf16* inb = in + (size_t)b * plane * CIN;
+  const int x0 = seg * 128;
+  for (int q = threadIdx.x; q < (R + 2) * kPx * CPC; q += 256) {
+    const int row = q / (kPx * CPC), rem = q - row * (kPx * CPC);
+    const int px = rem / CPC, ch = rem - px * CPC;
+    sm[row][px * kPitch + ch] = *reinterpret_cast<const bf16x8*>(
+        inb + ((size_t)(y0 + row) * Wp + (x0 + px)) * CIN + ch * 8);
+  }
+  bf16x8 A[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) {
+    int tap, ci;
+    kmap<CIN>(t, hf, tap, ci);
+    if (tap < kTaps)
+      A[t] = *reinterpret_cast<const bf16x8*>(wt + ((size_t)col * kTaps + tap) * CIN + ci);
+    else
+      A[t] = bf16x8{};
+  }
+  float bco[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) bco[v] = bias[8 * (v >> 2) + 4 * hf + (v & 3)];
+  __syncthreads();
+  const int xl = wv * 32 + col;
+#pragma unroll 1
+  for (int r = 0; r < R; ++r) {
+    bf16x8 Bv[NS];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+      int tap, ci;
+      kmap<CIN>(t, hf, tap, ci);
+      if (tap < kTaps) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        Bv[t] = sm[r + dy + 1][(xl + 1 + dx) * kPitch + ci / 8];
+      } else {
+        Bv[t] = bf16x8{};
+      }
+    }
+    f32x16 acc = {};
+#pragma unroll
+    for (int t = 0; t < NS; ++t)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[t], Bv[t], acc, 0, 0, 0);
+    const size_t pix = (size_t)b * plane + (size_t)(y0 + r + 1) * Wp + (x0 + xl + 1);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int co0 = 8 * g + 4 * hf;
+      bf16x4 o;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float vv = acc[4 * g + q] + bco[4 * g + q];
+        o[q] = (bf16)(vv > 0.0f ? vv : 0.0f);
+      }
+      *reinterpret_cast<bf16x4*>(out + pix * kC + co0) = o;
+    }
   }
 }
 
@@ -547,6 +624,7 @@ struct tea_nn {
   int device = 0;
   int max_batch = 0, H = 0, W = 0;
   hipStream_t stream = nullptr;
+  int fwd_rows = kFwdRows;  // rows per block of the forward kernel (0: one-row kernel)
   // parameters
   float *master = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
   long adam_t = 0;
@@ -597,7 +675,25 @@ unsigned flat_blocks(long count) { return (unsigned)((count + 255) / 256); }
 // stream `st` (the network's own, or a solver's).
 int forward_on(tea_nn* n, int batch, hipStream_t st) {
   const int H = n->H, W = n->W;
-  if (W % 128 == 0) {
+  const int R = n->fwd_rows;
+  if (W % 128 == 0 && (R == 2 || R == 4) && H % R == 0) {
+    const dim3 grid(W / 128, H / R, batch);
+    if (R == 4)
+      hipLaunchKernelGGL((conv_fwd_rows_kernel<kC0, 4>), grid, dim3(256), 0, st, n->in0,
+                         n->wf1, n->bias, n->act[0], H, W);
+    else
+      hipLaunchKernelGGL((conv_fwd_rows_kernel<kC0, 2>), grid, dim3(256), 0, st, n->in0,
+                         n->wf1, n->bias, n->act[0], H, W);
+    for (int l = 1; l < 5; ++l) {
+      const bf16* w = n->wfh + (size_t)(l - 1) * kWh;
+      if (R == 4)
+        hipLaunchKernelGGL((conv_fwd_rows_kernel<kC, 4>), grid, dim3(256), 0, st,
+                           n->act[l - 1], w, n->bias + l * kC, n->act[l], H, W);
+      else
+        hipLaunchKernelGGL((conv_fwd_rows_kernel<kC, 2>), grid, dim3(256), 0, st,
+                           n->act[l - 1], w, n->bias + l * kC, n->act[l], H, W);
+    }
+  } else if (W % 128 == 0) {
     const dim3 grid(W / 128, H, batch);
     hipLaunchKernelGGL((conv_fwd_lds_kernel<kC0>), grid, dim3(256), 0, st, n->in0, n->wf1,
                        n->bias, n->act[0], H, W);
@@ -756,6 +852,7 @@ int tea_nn_create(int device, int max_batch, int height, int width, tea_nn** out
   n->max_batch = max_batch;
   n->H = height;
   n->W = width;
+  if (const char* v = getenv("TEA_NN_FWD_ROWS")) n->fwd_rows = atoi(v);
   auto fail = [&](int code) {
     tea_nn_destroy(n);
     return code;

@@ -41,6 +41,23 @@ def test_forward_matches_oracle(shape):
     assert np.allclose(net.get_params(), p)
 
 
+@pytest.mark.parametrize("rows", [2, 4])
+@pytest.mark.parametrize("shape", [(2, 16, 128), (1, 12, 256), (1, 8, 512)])
+def test_forward_row_kernel_is_bitwise_the_one_row_kernel(shape, rows, monkeypatch):
+    """The multi-row forward kernel (R output rows per block, weights kept in
+    registers) against the one-row LDS kernel: same MFMA sequence per tile,
+    so the outputs must be identical bit for bit."""
+    p, x, _ = _case(4, *shape)
+    monkeypatch.setenv("TEA_NN_FWD_ROWS", "0")
+    one = _net(*shape)
+    monkeypatch.setenv("TEA_NN_FWD_ROWS", str(rows))
+    multi = _net(*shape)
+    one.set_params(p)
+    multi.set_params(p)
+    a, b = one.forward(x), multi.forward(x)
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+
+
 @pytest.mark.parametrize("shape", [(3, 16, 64), (2, 9, 32)])
 def test_gradients_match_oracle(shape):
     p, x, t = _case(2, *shape)
