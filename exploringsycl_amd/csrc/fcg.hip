@@ -97,21 +97,6 @@ __global__ __launch_bounds__(256) void restrict_kernel(Geo g, int c, int hc, int
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// scale[0] = RMS of D^-1/2 rbar over the coarse grid.
-__global__ __launch_bounds__(256) void rms_kernel(const double* __restrict__ part, int nblk,
-                                                  long count, float* __restrict__ scale) {
-  __shared__ double sh[256];
-  double v = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += 256) v += part[i];
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) scale[0] = (float)sqrt(sh[0] / (double)count);
-}
-
 // Local scale of the network input: sigma = RMS of D^-1/2 rbar over the
 // (2 kWin + 1)^2 window around each coarse cell (zero beyond the edge), in
 // two passes (rows into tmp, then columns), each summed in a fixed order,
@@ -120,10 +105,29 @@ __global__ __launch_bounds__(256) void rms_kernel(const double* __restrict__ par
 constexpr int kWin = 6;         // the network's receptive-field radius
 constexpr float kFloor = 1e-3f;
 
+// Row pass of the window sums; the grid's extra last block computes the
+// RMS scale[0] (of D^-1/2 rbar over the coarse grid, from the restriction's
+// per-block partials) at the same time, since the row pass does not need it.
 __global__ __launch_bounds__(256) void box_rows_kernel(int hc, int wc,
                                                        const float* __restrict__ rbar,
                                                        const float* __restrict__ dis,
-                                                       float* __restrict__ tmp) {
+                                                       float* __restrict__ tmp,
+                                                       const double* __restrict__ part,
+                                                       int npart,
+                                                       float* __restrict__ scale) {
+  if (blockIdx.x == gridDim.x - 1) {
+    __shared__ double sh[256];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < npart; i += 256) v += part[i];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) scale[0] = (float)sqrt(sh[0] / (double)((long)hc * wc));
+    return;
+  }
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= hc * wc) return;
   const int X = idx % wc;
@@ -136,34 +140,27 @@ __global__ __launch_bounds__(256) void box_rows_kernel(int hc, int wc,
   tmp[idx] = (float)s;
 }
 
+// Column pass: sigma (floored), then the cell's network input: channel 0 =
+// D^-1/2 rbar / sigma (0 where sigma is 0), channels 1, 2 = features.
 __global__ __launch_bounds__(256) void box_cols_kernel(int hc, int wc,
                                                        const float* __restrict__ tmp,
                                                        const float* __restrict__ scale,
-                                                       float* __restrict__ sig) {
+                                                       float* __restrict__ sig,
+                                                       const float* __restrict__ rbar,
+                                                       const float* __restrict__ dis,
+                                                       const float* __restrict__ feat,
+                                                       bf16* __restrict__ in0) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= hc * wc) return;
-  const int Y = idx / wc;
+  const int X = idx % wc, Y = idx / wc;
   double s = 0.0;
   for (int d = -kWin; d <= kWin; ++d) {
     if (Y + d < 0 || Y + d >= hc) continue;
     s += (double)tmp[idx + d * wc];
   }
-  sig[idx] = fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))),
-                   kFloor * scale[0]);
-}
-
-// Network input: channel 0 = D^-1/2 rbar / sigma (0 where sigma is 0),
-// channels 1, 2 = features.
-__global__ __launch_bounds__(256) void pack_kernel(int hc, int wc,
-                                                   const float* __restrict__ rbar,
-                                                   const float* __restrict__ dis,
-                                                   const float* __restrict__ sig,
-                                                   const float* __restrict__ feat,
-                                                   bf16* __restrict__ in0) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= hc * wc) return;
-  const int X = idx % wc, Y = idx / wc;
-  const float sg = sig[idx];
+  const float sg = fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))),
+                         kFloor * scale[0]);
+  sig[idx] = sg;
   bf16x8 o = bf16x8{};
   o[0] = sg > 0.0f ? (bf16)((rbar[idx] * dis[idx]) / sg) : (bf16)0.0f;
   o[1] = (bf16)feat[idx];
@@ -378,19 +375,16 @@ __global__ __launch_bounds__(256) void feature_kernel(int hc, int wc,
   dis[idx] = (float)di;
 }
 
-// The network's output back in T's variables: y = D^-1/2 sigma N.
-__global__ __launch_bounds__(256) void unscale_kernel(int cn, const float* __restrict__ dis,
-                                                      const float* __restrict__ sig,
-                                                      float* __restrict__ y) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx < cn) y[idx] = y[idx] * (sig[idx] * dis[idx]);
-}
-
-// Per-block partial sums over coarse cells of y.rbar and y.(T y), with T
-// the coarse operator (identity plus the 5-point operator of the face
-// coefficients kc, zero beyond the domain edge).
+// The network's output back in T's variables, y = D^-1/2 sigma N (into y,
+// not in place: neighbours read N), and the per-block partial sums over
+// coarse cells of y.rbar and y.(T y), with T the coarse operator (identity
+// plus the 5-point operator of the face coefficients kc, zero beyond the
+// domain edge).  Neighbours' y are formed by the same expression.
 __global__ __launch_bounds__(256) void coarse_dots_kernel(int hc, int wc,
-                                                          const float* __restrict__ y,
+                                                          const float* __restrict__ net,
+                                                          const float* __restrict__ dis,
+                                                          const float* __restrict__ sig,
+                                                          float* __restrict__ y,
                                                           const float* __restrict__ rbar,
                                                           const float* __restrict__ kc,
                                                           double* __restrict__ part) {
@@ -398,18 +392,21 @@ __global__ __launch_bounds__(256) void coarse_dots_kernel(int hc, int wc,
   const int idx = blockIdx.x * 256 + threadIdx.x;
   double a = 0.0, b = 0.0;
   if (idx < hc * wc) {
+    auto ys = [&](int j) { return net[j] * (sig[j] * dis[j]); };
     const int X = idx % wc, Y = idx / wc;
     const float* kxc = kc;
     const float* kyc = kc + hc * wc;
-    const double yc = y[idx];
+    const float yf = ys(idx);
+    y[idx] = yf;
+    const double yc = yf;
     const double kl = kxc[idx], kb = kyc[idx];
     const double kr = X + 1 < wc ? (double)kxc[idx + 1] : 0.0;
     const double kt = Y + 1 < hc ? (double)kyc[idx + wc] : 0.0;
     double ty = (1.0 + kl + kr + kb + kt) * yc;
-    if (X > 0) ty -= kl * (double)y[idx - 1];
-    if (X + 1 < wc) ty -= kr * (double)y[idx + 1];
-    if (Y > 0) ty -= kb * (double)y[idx - wc];
-    if (Y + 1 < hc) ty -= kt * (double)y[idx + wc];
+    if (X > 0) ty -= kl * (double)ys(idx - 1);
+    if (X + 1 < wc) ty -= kr * (double)ys(idx + 1);
+    if (Y > 0) ty -= kb * (double)ys(idx - wc);
+    if (Y + 1 < hc) ty -= kt * (double)ys(idx + wc);
     a = yc * (double)rbar[idx];
     b = yc * ty;
   }
@@ -579,31 +576,28 @@ static int run_restrict(tea_hip_chunk* c, int l, hipStream_t st) {
   return TEA_HIP_OK;
 }
 
-// The rest of level l after its restriction, on stream st: the scales, the
-// network input, the network, the unscaling and the level's factor.
-static int run_level(tea_hip_chunk* c, int l, hipStream_t st) {
+// The rest of level l after its restriction, on stream st: the scales and
+// the network input, the network, the unscaled output (into L.tmp) with its
+// dots, and -- unless the joint factors of two levels follow -- the level's
+// own factor.
+static int run_level(tea_hip_chunk* c, int l, hipStream_t st, bool own_factor) {
   FcgLevel& L = c->fcg_lv[l];
   const int cn = L.hc * L.wc;
   const int nblk = (cn + 255) / 256;
   {
     Timed t(c, K_FCG_RESTRICT, st);
-    hipLaunchKernelGGL(rms_kernel, dim3(1), dim3(256), 0, st, L.part, nblk, (long)cn,
-                       L.scale);
-    hipLaunchKernelGGL(box_rows_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc, L.rbar,
-                       L.dis, L.tmp);
+    hipLaunchKernelGGL(box_rows_kernel, dim3(nblk + 1), dim3(256), 0, st, L.hc, L.wc,
+                       L.rbar, L.dis, L.tmp, L.part, nblk, L.scale);
     hipLaunchKernelGGL(box_cols_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc, L.tmp,
-                       L.scale, L.sig);
-    hipLaunchKernelGGL(pack_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc, L.rbar,
-                       L.dis, L.sig, L.feat, nn::input_tensor(L.net));
+                       L.scale, L.sig, L.rbar, L.dis, L.feat, nn::input_tensor(L.net));
   }
   {
     Timed t(c, K_FCG_NET, st);
     TEA_RETURN_IF(nn::forward_on_stream(L.net, st));
-    hipLaunchKernelGGL(unscale_kernel, dim3(nblk), dim3(256), 0, st, cn, L.dis, L.sig,
-                       nn::output_mutable(L.net));
     hipLaunchKernelGGL(coarse_dots_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc,
-                       nn::output(L.net), L.rbar, L.kc, L.part);
-    hipLaunchKernelGGL(gamma_kernel, dim3(1), dim3(256), 0, st, L.part, nblk, L.scale);
+                       nn::output(L.net), L.dis, L.sig, L.tmp, L.rbar, L.kc, L.part);
+    if (own_factor)
+      hipLaunchKernelGGL(gamma_kernel, dim3(1), dim3(256), 0, st, L.part, nblk, L.scale);
   }
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
@@ -613,7 +607,7 @@ static CoarseTerm coarse_term(const tea_hip_chunk* c, int l) {
   const FcgLevel& L = c->fcg_lv[l];
   int shift = 0;
   while ((1 << shift) < L.c) ++shift;
-  return CoarseTerm{nn::output(L.net), L.scale + 1, shift, L.wc};
+  return CoarseTerm{L.tmp, L.scale + 1, shift, L.wc};
 }
 
 // z = M(r) and the partials of z.w into array `slot` of c->partials (not
@@ -626,7 +620,7 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
   *len = c->nspan;
   if (nl == 1) {
     TEA_RETURN_IF(run_restrict(c, 0, c->stream));
-    TEA_RETURN_IF(run_level(c, 0, c->stream));
+    TEA_RETURN_IF(run_level(c, 0, c->stream, true));
   } else if (nl == 2) {
     // Level 2 (from level 1's block means) on the side stream while level 1
     // runs on the chunk's stream; both are small grids, so their launch
@@ -635,8 +629,8 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
     TEA_HIP_TRY(hipEventRecord(c->fcg_fork, c->stream));
     TEA_HIP_TRY(hipStreamWaitEvent(c->fcg_side, c->fcg_fork, 0));
     TEA_RETURN_IF(run_restrict(c, 1, c->fcg_side));
-    TEA_RETURN_IF(run_level(c, 1, c->fcg_side));
-    TEA_RETURN_IF(run_level(c, 0, c->stream));
+    TEA_RETURN_IF(run_level(c, 1, c->fcg_side, false));
+    TEA_RETURN_IF(run_level(c, 0, c->stream, false));
     TEA_HIP_TRY(hipEventRecord(c->fcg_join, c->fcg_side));
     TEA_HIP_TRY(hipStreamWaitEvent(c->stream, c->fcg_join, 0));
   }
@@ -646,7 +640,7 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
     const int n1 = (L1.hc * L1.wc + 255) / 256, n2 = (L2.hc * L2.wc + 255) / 256;
     Timed t(c, K_FCG_NET);
     hipLaunchKernelGGL(cross_dots_kernel, dim3(n1), dim3(256), 0, c->stream, L1.hc, L1.wc,
-                       L2.c / L1.c, L2.wc, nn::output(L1.net), nn::output(L2.net), L1.kc,
+                       L2.c / L1.c, L2.wc, L1.tmp, L2.tmp, L1.kc,
                        c->fcg_cross);
     hipLaunchKernelGGL(gamma2_kernel, dim3(1), dim3(256), 0, c->stream, L1.part, n1,
                        L2.part, n2, c->fcg_cross, n1, (double)L1.c * L1.c,

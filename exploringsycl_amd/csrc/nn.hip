@@ -829,7 +829,6 @@ namespace nn {
 int forward_on_stream(tea_nn* n, hipStream_t st) { return forward_on(n, 1, st); }
 bf16* input_tensor(tea_nn* n) { return n->in0; }
 const float* output(tea_nn* n) { return n->out; }
-float* output_mutable(tea_nn* n) { return n->out; }
 int sync(tea_nn* n) {
   TEA_HIP_TRY(hipStreamSynchronize(n->stream));
   return TEA_HIP_OK;

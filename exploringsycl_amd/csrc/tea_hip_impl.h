@@ -403,7 +403,6 @@ typedef __bf16 bf16;
 int forward_on_stream(tea_nn* n, hipStream_t st);
 bf16* input_tensor(tea_nn* n);
 const float* output(tea_nn* n);
-float* output_mutable(tea_nn* n);
 }  // namespace nn
 }  // namespace tea
 

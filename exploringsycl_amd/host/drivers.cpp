@@ -307,7 +307,7 @@ void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
   }
   const double rel = g_extra.rel_eps > 0.0 ? g_extra.rel_eps : 1e-10;
   const int refresh = g_extra.refresh_interval > 0 ? g_extra.refresh_interval : 50;
-  const int window = g_extra.stall_window > 0 ? g_extra.stall_window : 200;
+  const int window = g_extra.stall_window > 0 ? g_extra.stall_window : 400;
   const double need = g_extra.stall_factor * g_extra.stall_factor;
 
   double rr = 0.0, bb = 0.0;

@@ -57,8 +57,8 @@ struct SolverExtras {
   int refresh_interval = 50; /* refresh_interval=: FP64 true residual period */
   std::string fcg_weights;   /* fcg_weights=: the preconditioner network file */
   int fcg_levels = 2;        /* fcg_levels=: coarse levels of the network (1 or 2) */
-  int stall_window = 200;    /* stall_window=: FCG stagnation window (iterations) */
-  double stall_factor = 2.0; /* stall_factor=: least |r| reduction over the window */
+  int stall_window = 400;    /* stall_window=: FCG stagnation window (iterations) */
+  double stall_factor = 4.0; /* stall_factor=: least |r| reduction over the window */
 };
 
 /* The learned preconditioner's network file: one ASCII line
