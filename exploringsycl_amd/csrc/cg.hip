@@ -352,14 +352,14 @@ __device__ inline void fused_pair_load(const Geo& g, const FusedWArgs& a,
   if (a.apply_u) s.U = ld2_nt(a.u, ir);
   // Issued with the pair's own loads so the wave waits once; ir - 1 and
   // ir + 2 are in range (ir is an interior pair's index or x + 2).
+  // Lanes 0 and 63 share each load instruction, at their own addresses
+  // (lane 0's kx value is not used).
   const int lane = threadIdx.x & (kWave - 1);
-  if (lane == 0) {
-    s.PN = P[ir - 1];
-    if (APPLY_P) s.RN = R[ir - 1];
-  } else if (lane == kWave - 1) {
-    s.PN = P[ir + 2];
-    if (APPLY_P) s.RN = R[ir + 2];
-    s.KXN = a.kx[ir + 2];
+  if (lane == 0 || lane == kWave - 1) {
+    const int ni = lane == 0 ? ir - 1 : ir + 2;
+    s.PN = P[ni];
+    if (APPLY_P) s.RN = R[ni];
+    s.KXN = a.kx[ni];
   }
 }
 

@@ -209,14 +209,14 @@ __device__ inline void mp_pair_load(const Geo& g, const MpArgs& a, MpPair& s) {
   s.KYt = ld2(a.ky, ir + x);
   if (a.apply_u) s.U = ld2(a.u, ir);
   if (DOT2) s.RD = ld2(a.rdot, ir);
+  // Lanes 0 and 63 share each load instruction, at their own addresses
+  // (lane 0's kx value is not used).
   const int lane = threadIdx.x & (kWave - 1);
-  if (lane == 0) {
-    s.PN = P[ir - 1];
-    if (a.apply_p) s.RN = R[ir - 1];
-  } else if (lane == kWave - 1) {
-    s.PN = P[ir + 2];
-    if (a.apply_p) s.RN = R[ir + 2];
-    s.KXN = a.kx[ir + 2];
+  if (lane == 0 || lane == kWave - 1) {
+    const int ni = lane == 0 ? ir - 1 : ir + 2;
+    s.PN = P[ni];
+    if (a.apply_p) s.RN = R[ni];
+    s.KXN = a.kx[ni];
   }
 }
 
