@@ -174,8 +174,11 @@ struct CoarseTerm {
   const float* y;
   const float* g;
   int shift, wc;
+  __device__ int index(int kk, int jj, int h) const {
+    return ((jj - h) >> shift) * wc + ((kk - h) >> shift);
+  }
   __device__ double at(int kk, int jj, int h) const {
-    return (double)g[0] * (double)y[((jj - h) >> shift) * wc + ((kk - h) >> shift)];
+    return (double)g[0] * (double)y[index(kk, jj, h)];
   }
 };
 
@@ -272,6 +275,19 @@ __global__ __launch_bounds__(kBlock) void fcg_z1_kernel(
                KYt = ld2(ky, ir + x), W = ld2(w, ir);
   float kxn = 0.0f;
   if (lane == kWave - 1) kxn = kx[ir + 2];
+  // The coarse values of both cells, issued with the pair's own loads (index
+  // 0 for cells outside the interior, whose value is not used).
+  float ya0 = 0.0f, yb0 = 0.0f, ya1 = 0.0f, yb1 = 0.0f, g0 = 0.0f, g1 = 0.0f;
+  if (NL > 0) {
+    g0 = t0.g[0];
+    ya0 = t0.y[in0 ? t0.index(kk, row, g.h) : 0];
+    yb0 = t0.y[in1 ? t0.index(kk + 1, row, g.h) : 0];
+  }
+  if (NL > 1) {
+    g1 = t1.g[0];
+    ya1 = t1.y[in0 ? t1.index(kk, row, g.h) : 0];
+    yb1 = t1.y[in1 ? t1.index(kk + 1, row, g.h) : 0];
+  }
   float kxr = __shfl_down(KX.x, 1);
   if (lane == kWave - 1) kxr = kxn;
   const double da = (1.0 + ((double)KX.y + (double)KX.x)) + ((double)KYt.x + (double)KY.x);
@@ -279,14 +295,14 @@ __global__ __launch_bounds__(kBlock) void fcg_z1_kernel(
   float za = 0.0f, zb = 0.0f;
   if (in0) {
     double v = (double)R.x / da;
-    if (NL > 0) v = v + t0.at(kk, row, g.h);
-    if (NL > 1) v = v + t1.at(kk, row, g.h);
+    if (NL > 0) v = v + (double)g0 * (double)ya0;
+    if (NL > 1) v = v + (double)g1 * (double)ya1;
     za = (float)v;
   }
   if (in1) {
     double v = (double)R.y / db;
-    if (NL > 0) v = v + t0.at(kk + 1, row, g.h);
-    if (NL > 1) v = v + t1.at(kk + 1, row, g.h);
+    if (NL > 0) v = v + (double)g0 * (double)yb0;
+    if (NL > 1) v = v + (double)g1 * (double)yb1;
     zb = (float)v;
   }
   double v[1] = {(in0 ? (double)za * (double)W.x : 0.0) +

@@ -675,8 +675,12 @@ unsigned flat_blocks(long count) { return (unsigned)((count + 255) / 256); }
 // stream `st` (the network's own, or a solver's).
 int forward_on(tea_nn* n, int batch, hipStream_t st) {
   const int H = n->H, W = n->W;
+  // Several rows per block only while the grid keeps at least 512 blocks:
+  // on small grids the one-row kernel's parallelism matters more than the
+  // saved weight and row loads.
   const int R = n->fwd_rows;
-  if (W % 128 == 0 && (R == 2 || R == 4) && H % R == 0) {
+  if (W % 128 == 0 && (R == 2 || R == 4) && H % R == 0 &&
+      (long)(W / 128) * (H / R) * batch >= 512) {
     const dim3 grid(W / 128, H / R, batch);
     if (R == 4)
       hipLaunchKernelGGL((conv_fwd_rows_kernel<kC0, 4>), grid, dim3(256), 0, st, n->in0,

@@ -42,7 +42,7 @@ def test_forward_matches_oracle(shape):
 
 
 @pytest.mark.parametrize("rows", [2, 4])
-@pytest.mark.parametrize("shape", [(2, 16, 128), (1, 12, 256), (1, 8, 512)])
+@pytest.mark.parametrize("shape", [(4, 512, 128), (1, 512, 512), (2, 256, 512)])
 def test_forward_row_kernel_is_bitwise_the_one_row_kernel(shape, rows, monkeypatch):
     """The multi-row forward kernel (R output rows per block, weights kept in
     registers) against the one-row LDS kernel: same MFMA sequence per tile,
