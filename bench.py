@@ -19,9 +19,10 @@ Printed (one JSON line): effective HBM throughput of the timed solve,
 88 B being the algorithmic FP64 traffic per cell per iteration (11 words),
 plus time-to-solution, iterations/s, cell-updates/s, the roofline of the
 dominant kernel (cg_fused_w: 8 words = 64 B per cell per launch, timed with
-HIP events on the library's stream) and a CPU baseline (the OpenMP CPU
-restatement of the reference kernels, unfused 13-word dataflow, timed on a
-bounded sample of the same problem on this host).
+HIP events on the library's stream; also set against a measured 16-byte
+device-copy rate) and a CPU baseline (the OpenMP CPU restatement of the
+reference kernels, unfused 13-word dataflow, timed on a bounded sample of the
+same problem on this host).
 """
 
 import argparse
@@ -57,7 +58,35 @@ def parse_args():
                     help="one launch per reference kernel (13 words/iteration)")
     ap.add_argument("--tts", type=int, default=1,
                     help="run the time-to-solution comparison (0 skips it)")
+    ap.add_argument("--tts-cells", type=int, default=0,
+                    help="grid of the time-to-solution comparison (default: --cells)")
+    ap.add_argument("--tts-max-iters", type=int, default=12000,
+                    help="iteration cap of the time-to-solution solves")
     return ap.parse_args()
+
+
+def copy_peak(gib=1.0, reps=20):
+    """Measured streaming ceiling: a 16-byte nontemporal device copy of
+    `gib` GiB (libteahip's tea_hip_copy_bandwidth), read + write bytes per
+    second, to set beside the 8 TB/s peak."""
+    import ctypes
+    from exploringsycl_amd import native
+    lib = native.load()
+    gbs = ctypes.c_double()
+    native.check(lib.tea_hip_copy_bandwidth(0, int(gib * (1 << 30)), reps,
+                                            ctypes.byref(gbs)), "copy_bandwidth")
+    return gbs.value
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(cells, seconds):
@@ -102,7 +131,7 @@ def cpu_baseline(cells, seconds):
     }
 
 
-def time_to_solution(n, timeout=1200):
+def time_to_solution(n, max_iters=12000, timeout=1200):
     """Plain FP64 CG, MPCG, Jacobi-PCG and learned FCG on one unseen field of
     the 64-rectangle workload at n x n, stopped at 1e-10 relative residual."""
     import numpy as np
@@ -119,7 +148,7 @@ def time_to_solution(n, timeout=1200):
         solvers.append(("learned_fcg", ["use_fcg", "fcg_weights=net.bin"]))
     out = {"workload": (f"{n}x{n} cells, 64 seeded rectangles (log10 density "
                         f"U[-2,2], seed {seed}) + tea.in states, K=1/density, "
-                        "1 timestep, rel. residual 1e-10 (max 12000 iterations), "
+                        f"1 timestep, rel. residual 1e-10 (max {max_iters} iterations), "
                         "after 1 warm-up step"),
            "solvers": {}}
     fields = {}
@@ -127,7 +156,7 @@ def time_to_solution(n, timeout=1200):
         for name, extra in solvers:
             deck = Deck.with_random_rectangles(
                 n, n, 64, seed, density="log10", density_range=(-2.0, 2.0),
-                end_step=1, coefficient="inverse", max_iters=12000,
+                end_step=1, coefficient="inverse", max_iters=max_iters,
                 extra=["rel_eps=1e-10"] + extra)
             wd = os.path.join(root, name)
             os.makedirs(wd)
@@ -233,13 +262,19 @@ def main():
         "wall_s_including_setup": round(t_wall, 2),
     }
     if args.tts:
-        tts = time_to_solution(n)
+        tts = time_to_solution(args.tts_cells or n, args.tts_max_iters)
         line["time_to_solution"] = tts
         sv = tts["solvers"]
         line["learned_fcg_vs_plain_cg_speedup"] = (
             sv["learned_fcg"]["speedup_vs_plain_cg"] if "learned_fcg" in sv else None)
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+        line["cpu_baseline"]["cpu_model"] = cpu_model()
+    peak = copy_peak()
+    roofline["copy_peak"] = {"value": round(peak, 1), "unit": "GB/s",
+                             "method": "16-byte nontemporal device copy of 1 GiB, "
+                                       "read + write bytes / time"}
+    roofline["frac_of_copy_peak"] = round(roofline["achieved"] / peak, 4)
     print(json.dumps(line), flush=True)
 
 

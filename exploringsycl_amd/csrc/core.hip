@@ -158,6 +158,18 @@ int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot) {
   return TEA_HIP_OK;
 }
 
+// dst = src in 16-byte pieces, one per thread, nontemporal both ways: the
+// streaming ceiling for tea_hip_copy_bandwidth.
+__global__ __launch_bounds__(kBlock) void stream_copy_kernel(const double* __restrict__ src,
+                                                             double* __restrict__ dst,
+                                                             size_t n2) {
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n2) return;
+  const dv2 v = __builtin_nontemporal_load(reinterpret_cast<const dv2*>(src) + i);
+  __builtin_nontemporal_store(v, reinterpret_cast<dv2*>(dst) + i);
+}
+
 int finish_reduction_n(tea_hip_chunk* c, int nr, int len, double* out) {
   TEA_RETURN_IF(sync_chunk(c));
   for (int q = 0; q < nr; ++q)
@@ -411,6 +423,55 @@ int tea_hip_timing_reset(tea_hip_chunk* c) {
     c->total_ms[k] = 0.0;
   }
   return TEA_HIP_OK;
+}
+
+int tea_hip_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
+  TEA_CHECK_ARG(gbs != nullptr, "null output");
+  TEA_CHECK_ARG(bytes >= 16 && bytes % 16 == 0, "bytes must be a positive multiple of 16");
+  TEA_CHECK_ARG(reps >= 1, "reps must be positive");
+  TEA_HIP_TRY(hipSetDevice(device));
+  const size_t n2 = bytes / 16;
+  double *src = nullptr, *dst = nullptr;
+  if (hipMalloc(&src, bytes) != hipSuccess) {
+    set_error("hipMalloc of the copy source failed");
+    return TEA_HIP_ENOMEM;
+  }
+  if (hipMalloc(&dst, bytes) != hipSuccess) {
+    (void)hipFree(src);
+    set_error("hipMalloc of the copy destination failed");
+    return TEA_HIP_ENOMEM;
+  }
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int status = TEA_HIP_OK;
+  float ms = 0.0f;
+  const dim3 grid((unsigned)((n2 + kBlock - 1) / kBlock));
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+      hipMemsetAsync(src, 0, bytes, st) != hipSuccess) {
+    set_error("copy bandwidth: stream, event or memset failed");
+    status = TEA_HIP_EHIP;
+  } else {
+    for (int k = 0; k < 2; ++k)  // warm-up
+      hipLaunchKernelGGL(stream_copy_kernel, grid, dim3(kBlock), 0, st, src, dst, n2);
+    (void)hipEventRecord(e0, st);
+    for (int k = 0; k < reps; ++k)
+      hipLaunchKernelGGL(stream_copy_kernel, grid, dim3(kBlock), 0, st, src, dst, n2);
+    (void)hipEventRecord(e1, st);
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.0f) {
+      set_error("copy bandwidth: launch or timing failed");
+      status = TEA_HIP_EHIP;
+    } else {
+      *gbs = 2.0 * (double)bytes / ((double)ms * 1e-3 / reps) / 1e9;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return status;
 }
 
 int tea_hip_num_kernels(void) { return K_NUM; }

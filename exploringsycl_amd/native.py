@@ -75,6 +75,7 @@ _SIGNATURES = {
     "tea_hip_kernel_stats": [_vp, _i, ctypes.POINTER(ctypes.c_long), _dp],
     "tea_hip_timing_reset": [_vp],
     "tea_hip_num_kernels": [],
+    "tea_hip_copy_bandwidth": [_i, ctypes.c_size_t, _i, _dp],
     "tea_hip_kernel_name": [_i],
     "tea_hip_last_error": [],
 }

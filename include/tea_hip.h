@@ -329,6 +329,13 @@ int tea_hip_timing_reset(tea_hip_chunk* chunk);
 int tea_hip_num_kernels(void);
 const char* tea_hip_kernel_name(int kernel_id);
 
+/* Measured streaming ceiling (no counterpart in the reference): copies
+ * `bytes` (a multiple of 16) between two device buffers `reps` times with
+ * 16-byte nontemporal loads and stores and sets *gbs = read + write bytes
+ * per second / 1e9 of the average copy, timed with HIP events.  Allocates
+ * and frees its buffers. */
+int tea_hip_copy_bandwidth(int device, size_t bytes, int reps, double* gbs);
+
 #ifdef __cplusplus
 }
 #endif

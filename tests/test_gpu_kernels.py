@@ -337,3 +337,16 @@ def test_mixed_precision_kernels_match_oracle(oracle):
             got = c.read_mp(nm).ravel()
             assert np.array_equal(got.view(np.int32), arr.view(np.int32)), nm
         _assert_fields(c, f, ["u"])
+
+
+def test_copy_bandwidth_is_a_plausible_rate():
+    """tea_hip_copy_bandwidth: a 256 MiB 16-byte device copy reports a rate
+    between 1 TB/s and the 8 TB/s peak; bad arguments are refused."""
+    import ctypes
+    from exploringsycl_amd import native
+    lib = native.load()
+    gbs = ctypes.c_double()
+    assert lib.tea_hip_copy_bandwidth(0, 256 << 20, 5, ctypes.byref(gbs)) == 0
+    assert 1000.0 < gbs.value < 8000.0, gbs.value
+    assert lib.tea_hip_copy_bandwidth(0, 17, 1, ctypes.byref(gbs)) != 0
+    assert lib.tea_hip_copy_bandwidth(0, 256, 1, None) != 0
