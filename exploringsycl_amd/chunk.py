@@ -145,6 +145,31 @@ class DeviceChunk:
     def finalise(self):
         check(self.lib.tea_hip_finalise(self.handle), "finalise")
 
+    # ---- Jacobi, Chebyshev, PPCG (kernel_interface.cpp:246-328) ---------------
+    def jacobi_init(self, coefficient, rx, ry):
+        check(self.lib.tea_hip_jacobi_init(self.handle, coefficient, rx, ry),
+              "jacobi_init")
+
+    def jacobi_iterate(self):
+        v = ctypes.c_double()
+        check(self.lib.tea_hip_jacobi_iterate(self.handle, ctypes.byref(v)),
+              "jacobi_iterate")
+        return v.value
+
+    def cheby_init(self, theta):
+        check(self.lib.tea_hip_cheby_init(self.handle, theta), "cheby_init")
+
+    def cheby_iterate(self, alpha, beta):
+        check(self.lib.tea_hip_cheby_iterate(self.handle, alpha, beta),
+              "cheby_iterate")
+
+    def ppcg_init(self, theta):
+        check(self.lib.tea_hip_ppcg_init(self.handle, theta), "ppcg_init")
+
+    def ppcg_inner_iteration(self, alpha, beta):
+        check(self.lib.tea_hip_ppcg_inner_iteration(self.handle, alpha, beta),
+              "ppcg_inner_iteration")
+
     # ---- mixed-precision CG (no reference counterpart) -------------------------
     def mpcg_init(self, coefficient, rx, ry):
         rr, bb = ctypes.c_double(), ctypes.c_double()

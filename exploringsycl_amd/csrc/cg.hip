@@ -275,28 +275,6 @@ __global__ __launch_bounds__(kBlock) void fused_w_kernel(Geo g, FusedWArgs a,
 // 4096-cell span in 8 slots of 512 cells.  The first tree level is the
 // in-lane pair sum, then the butterfly runs over each 32-lane half.
 
-// Streaming (nontemporal) access for data read or written once per pass;
-// stencil operands (p, r, kx, ky) use ld2 so the neighbouring rows can reuse
-// them from L2.
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-
-__device__ inline double2 ld2_nt(const double* p, int i) {
-  const dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(p + i));
-  return double2{v.x, v.y};
-}
-
-// Stores a cell pair with one 16-byte streaming store, or the interior
-// member alone.
-__device__ inline void st2_nt(double* p, int i, bool in0, bool in1, double v0,
-                              double v1) {
-  if (in0 && in1) {
-    __builtin_nontemporal_store(dvec2{v0, v1}, reinterpret_cast<dvec2*>(p + i));
-  } else {
-    if (in0) __builtin_nontemporal_store(v0, p + i);
-    if (in1) __builtin_nontemporal_store(v1, p + i + 1);
-  }
-}
-
 __device__ inline void span_reduce_pairs(double (&v)[kSlots], double* partials,
                                          int nspan, int span,
                                          bool single_group) {
@@ -697,10 +675,6 @@ int apply_pending(tea_hip_chunk* c) {
 }  // namespace tea
 
 using namespace tea;
-
-#define TEA_ENTER(c)                                      \
-  TEA_CHECK_ARG((c) != nullptr, "null chunk handle");     \
-  TEA_HIP_TRY(hipSetDevice((c)->device))
 
 extern "C" {
 

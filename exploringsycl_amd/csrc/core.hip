@@ -26,7 +26,10 @@ const char* const kKernelNames[K_NUM] = {
     "calculate_2norm", "finalise",       "mpcg_convert", "mpcg_w",
     "mpcg_r",         "mpcg_apply_u",    "mpcg_refresh",  "fcg_w",
     "fcg_jacobi",     "fcg_restrict",    "fcg_net",       "fcg_prolong",
-    "span_finish"};
+    "span_finish",    "jacobi_init",     "jacobi_copy_u", "jacobi_iterate",
+    "jacobi_fused",   "cheby_init",      "cheby_iterate", "cheby_calc_u",
+    "cheby_fused",    "ppcg_init",       "ppcg_calc_ur",  "ppcg_calc_sd",
+    "ppcg_fused"};
 
 Timed::Timed(tea_hip_chunk* c_, int kid_, hipStream_t s)
     : c(c_), kid(kid_), stream(s ? s : c_->stream) {
@@ -319,6 +322,7 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (int f = 0; f < TEA_NUM_FIELDS; ++f)
     if (c->field[f]) (void)hipFree(c->field[f]);
   if (c->p_alt) (void)hipFree(c->p_alt);
+  if (c->spare) (void)hipFree(c->spare);
   if (c->sub) (void)hipFree(c->sub);
   for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky, c->fcg_z})
     if (b) (void)hipFree(b);

@@ -9,7 +9,11 @@
 // are C++ (the reference host is compiled as C++).
 //
 // Error behaviour follows the reference: a failed call ends the run through
-// the host's die() with the library's message.
+// the host's die() with the library's message.  Scalar results are added to
+// or assigned to the caller's variable as the SYCL backend does: rro and pw
+// are added (cg.cpp:133,194); rrn, the field summary, the Jacobi error and
+// the 2-norm are assigned (cg.cpp:253, field_summary.cpp:147-150,
+// jacobi.cpp:116, solver_methods.cpp:116).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -42,9 +46,16 @@ double* field_ptr(tea_hip_chunk* h, int field) {
   return p;
 }
 
-// p is double-buffered by the fused CG iteration: republish its address
-// after every call that can swap it.
-void refresh_p(Chunk* chunk) { chunk->p = field_ptr(H(chunk), TEA_FIELD_P); }
+// p, u, r and sd are double-buffered by the fused iterations (CG, Jacobi,
+// Chebyshev, PPCG): republish their addresses after every call that can
+// swap them.
+void refresh_fields(Chunk* chunk) {
+  tea_hip_chunk* h = H(chunk);
+  chunk->p = field_ptr(h, TEA_FIELD_P);
+  chunk->u = field_ptr(h, TEA_FIELD_U);
+  chunk->r = field_ptr(h, TEA_FIELD_R);
+  chunk->sd = field_ptr(h, TEA_FIELD_SD);
+}
 
 // Live handles, for the timing hook below.
 std::vector<tea_hip_chunk*> g_handles;
@@ -153,7 +164,7 @@ void run_local_halos(Chunk* chunk, Settings* settings, int depth) {
   for (int s = 0; s < TEA_HALO_NUM; ++s)
     fields[s] = settings->fields_to_exchange[s] ? 1 : 0;
   HIP_CALL(tea_hip_local_halos(H(chunk), depth, chunk->neighbours, fields));
-  refresh_p(chunk);
+  refresh_fields(chunk);
 }
 
 void run_pack_or_unpack(Chunk* chunk, Settings* settings, int depth, int face,
@@ -173,20 +184,15 @@ void run_store_energy(Chunk* chunk, Settings* settings) {
 void run_field_summary(Chunk* chunk, Settings* settings, double* vol,
                        double* mass, double* ie, double* temp) {
   (void)settings;
-  double v, m, e, t;
-  HIP_CALL(tea_hip_field_summary(H(chunk), &v, &m, &e, &t));
-  refresh_p(chunk);
-  *vol += v;
-  *mass += m;
-  *ie += e;
-  *temp += t;
+  HIP_CALL(tea_hip_field_summary(H(chunk), vol, mass, ie, temp));
+  refresh_fields(chunk);
 }
 
 void run_cg_init(Chunk* chunk, Settings* settings, double rx, double ry,
                  double* rro) {
   double v;
   HIP_CALL(tea_hip_cg_init(H(chunk), settings->coefficient, rx, ry, &v));
-  refresh_p(chunk);
+  refresh_fields(chunk);
   *rro += v;
 }
 
@@ -194,79 +200,83 @@ void run_cg_calc_w(Chunk* chunk, Settings* settings, double* pw) {
   (void)settings;
   double v;
   HIP_CALL(tea_hip_cg_calc_w(H(chunk), &v));
-  refresh_p(chunk);
+  refresh_fields(chunk);
   *pw += v;
 }
 
 void run_cg_calc_ur(Chunk* chunk, Settings* settings, double alpha,
                     double* rrn) {
   (void)settings;
-  double v;
-  HIP_CALL(tea_hip_cg_calc_ur(H(chunk), alpha, &v));
-  refresh_p(chunk);
-  *rrn += v;
+  HIP_CALL(tea_hip_cg_calc_ur(H(chunk), alpha, rrn));
+  refresh_fields(chunk);
 }
 
 void run_cg_calc_p(Chunk* chunk, Settings* settings, double beta) {
   (void)settings;
   HIP_CALL(tea_hip_cg_calc_p(H(chunk), beta));
-  refresh_p(chunk);
+  refresh_fields(chunk);
 }
 
-// Chebyshev, Jacobi and PPCG are other solvers of the reference; their
-// symbols exist so that its host links, and end the run if called.
 void run_cheby_init(Chunk* chunk, Settings* settings) {
-  (void)chunk; (void)settings;
-  die(__LINE__, __FILE__, "the Chebyshev solver is not part of this backend\n");
+  (void)settings;
+  HIP_CALL(tea_hip_cheby_init(H(chunk), chunk->theta));
+  refresh_fields(chunk);
 }
+
 void run_cheby_iterate(Chunk* chunk, Settings* settings, double alpha,
                        double beta) {
-  (void)chunk; (void)settings; (void)alpha; (void)beta;
-  die(__LINE__, __FILE__, "the Chebyshev solver is not part of this backend\n");
+  (void)settings;
+  HIP_CALL(tea_hip_cheby_iterate(H(chunk), alpha, beta));
+  refresh_fields(chunk);
 }
+
 void run_jacobi_init(Chunk* chunk, Settings* settings, double rx, double ry) {
-  (void)chunk; (void)settings; (void)rx; (void)ry;
-  die(__LINE__, __FILE__, "the Jacobi solver is not part of this backend\n");
+  HIP_CALL(tea_hip_jacobi_init(H(chunk), settings->coefficient, rx, ry));
+  refresh_fields(chunk);
 }
+
 void run_jacobi_iterate(Chunk* chunk, Settings* settings, double* error) {
-  (void)chunk; (void)settings; (void)error;
-  die(__LINE__, __FILE__, "the Jacobi solver is not part of this backend\n");
+  (void)settings;
+  HIP_CALL(tea_hip_jacobi_iterate(H(chunk), error));
+  refresh_fields(chunk);
 }
+
 void run_ppcg_init(Chunk* chunk, Settings* settings) {
-  (void)chunk; (void)settings;
-  die(__LINE__, __FILE__, "the PPCG solver is not part of this backend\n");
+  (void)settings;
+  HIP_CALL(tea_hip_ppcg_init(H(chunk), chunk->theta));
+  refresh_fields(chunk);
 }
+
 void run_ppcg_inner_iteration(Chunk* chunk, Settings* settings, double alpha,
                               double beta) {
-  (void)chunk; (void)settings; (void)alpha; (void)beta;
-  die(__LINE__, __FILE__, "the PPCG solver is not part of this backend\n");
+  (void)settings;
+  HIP_CALL(tea_hip_ppcg_inner_iteration(H(chunk), alpha, beta));
+  refresh_fields(chunk);
 }
 
 void run_copy_u(Chunk* chunk, Settings* settings) {
   (void)settings;
   HIP_CALL(tea_hip_copy_u(H(chunk)));
-  refresh_p(chunk);
+  refresh_fields(chunk);
 }
 
 void run_calculate_residual(Chunk* chunk, Settings* settings) {
   (void)settings;
   HIP_CALL(tea_hip_calculate_residual(H(chunk)));
-  refresh_p(chunk);
+  refresh_fields(chunk);
 }
 
 void run_calculate_2norm(Chunk* chunk, Settings* settings,
                          FieldBufferType buffer, double* norm) {
   (void)settings;
-  double v;
-  HIP_CALL(tea_hip_calculate_2norm(H(chunk), buffer, &v));
-  refresh_p(chunk);
-  *norm += v;
+  HIP_CALL(tea_hip_calculate_2norm(H(chunk), buffer, norm));
+  refresh_fields(chunk);
 }
 
 void run_finalise(Chunk* chunk, Settings* settings) {
   (void)settings;
   HIP_CALL(tea_hip_finalise(H(chunk)));
-  refresh_p(chunk);
+  refresh_fields(chunk);
 }
 
 #ifdef TEA_HAVE_EXT
@@ -279,7 +289,7 @@ void run_mpcg_init(Chunk* chunk, Settings* settings, double rx, double ry,
       die(__LINE__, __FILE__, "mixed-precision CG needs all faces external\n");
   double a, b;
   HIP_CALL(tea_hip_mpcg_init(H(chunk), settings->coefficient, rx, ry, &a, &b));
-  refresh_p(chunk);
+  refresh_fields(chunk);
   *rr += a;
   *bb += b;
 }
@@ -332,7 +342,7 @@ void run_fcg_init(Chunk* chunk, Settings* settings, double rx, double ry,
       die(__LINE__, __FILE__, "flexible CG needs every face external\n");
   double a = 0.0, b = 0.0;
   HIP_CALL(tea_hip_fcg_init(H(chunk), settings->coefficient, rx, ry, &a, &b));
-  refresh_p(chunk);
+  refresh_fields(chunk);
   *rr += a;
   *bb += b;
 }

@@ -1,0 +1,497 @@
+// libteahip.so: the reference's Jacobi, Chebyshev and PPCG kernels
+// (TeaLeaf/c_kernels/sycl/jacobi.cpp, cheby.cpp, ppcg.cpp, composed as
+// TeaLeaf/c_kernels/sycl/kernel_interface.cpp:246-328 composes them).
+//
+// Each iteration entry point is one streaming pass in the default (fused)
+// mode, with one cell pair per lane as in the fused CG kernels:
+//   Jacobi     reads u (5-point), u0, kx, ky; writes the new u into r's
+//              buffer, the two fields' addresses are then exchanged, so the
+//              old u is "r = u" without a copy; reduces sum |u_new - u|.
+//              40 bytes per cell against the reference's 56 (copy 16 +
+//              sweep 40).
+//   Chebyshev  reads u (5-point), u0, kx, ky, p; writes w, r, p and the new
+//              u into a spare buffer exchanged with u (the stencil still
+//              reads the old u).  72 bytes per cell against 88.
+//   PPCG inner reads sd (5-point), kx, ky, r, u; writes r, u and the new sd
+//              into the spare buffer exchanged with sd.  64 bytes per cell
+//              against 80.
+// A double-buffered field's cells outside the interior are copied into the
+// new buffer in the same pass, so every field holds exactly what the
+// reference's in-place kernels leave.  Mode 0 (tea_hip_set_fused) and odd
+// row lengths run one launch per reference kernel instead.  Every FP64
+// expression keeps the reference's operand order; the library is compiled
+// with -ffp-contract=off.
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "tea_hip_impl.h"
+
+namespace tea {
+namespace {
+
+// The 5-point operator of TeaLeaf/shared.h:59-63 at one cell from its
+// operand values c (cell), l, r, b, t (left, right, bottom, top) and face
+// coefficients kx0 = kx[i], kx1 = kx[i+1], ky0 = ky[i], ky1 = ky[i+x].
+__device__ inline double op5(double c, double l, double r, double b, double t,
+                             double kx0, double kx1, double ky0, double ky1) {
+  const double diag = (1.0 + (kx1 + kx0)) + (ky1 + ky0);
+  return (diag * c - (kx1 * r + kx0 * l)) - (ky1 * t + ky0 * b);
+}
+
+// The Jacobi update of jacobi.cpp:88-91 from the same operands (r is the
+// previous iterate).
+__device__ inline double jacobi_cell(double u0, double l, double r, double b,
+                                     double t, double kx0, double kx1,
+                                     double ky0, double ky1) {
+  return ((u0 + (kx1 * r + kx0 * l)) + (ky1 * t + ky0 * b)) /
+         ((1.0 + (kx0 + kx1)) + (ky0 + ky1));
+}
+
+// ---- one launch per reference kernel --------------------------------------
+
+struct JacobiInitBody {  // jacobi.cpp:7-54
+  double* __restrict__ u;
+  double* __restrict__ u0;
+  const double* __restrict__ density;
+  const double* __restrict__ energy;
+  double* __restrict__ kx;
+  double* __restrict__ ky;
+  double rx, ry;
+  int coefficient, x, y, h;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (kk > 0 && kk < x - 1 && jj > 0 && jj < y - 1) {
+      const double v = energy[i] * density[i];
+      u0[i] = v;
+      u[i] = v;
+    }
+    if (jj >= h && jj < y - 1 && kk >= h && kk < x - 1) {
+      const bool cond = coefficient == TEA_CONDUCTIVITY;
+      const double dc = cond ? density[i] : 1.0 / density[i];
+      const double dl = cond ? density[i - 1] : 1.0 / density[i - 1];
+      const double dd = cond ? density[i - x] : 1.0 / density[i - x];
+      kx[i] = rx * (dl + dc) / (2.0 * dl * dc);
+      ky[i] = ry * (dd + dc) / (2.0 * dd * dc);
+    }
+  }
+};
+
+struct CopyAllBody {  // jacobi.cpp:120-137: r = u on every cell
+  const double* __restrict__ src;
+  double* __restrict__ dst;
+  __device__ void operator()(int i, int, int) const { dst[i] = src[i]; }
+};
+
+struct JacobiBody {  // jacobi.cpp:57-117
+  double* __restrict__ u;
+  const double* __restrict__ u0;
+  const double* __restrict__ r;
+  const double* __restrict__ kx;
+  const double* __restrict__ ky;
+  int x;
+  __device__ void operator()(int i, int, int, double (&out)[1]) const {
+    const double v = jacobi_cell(u0[i], r[i - 1], r[i + 1], r[i - x], r[i + x],
+                                 kx[i], kx[i + 1], ky[i], ky[i + x]);
+    u[i] = v;
+    out[0] = fabs(v - r[i]);
+  }
+};
+
+struct ChebyInitBody {  // cheby.cpp:7-43
+  double* __restrict__ p;
+  double* __restrict__ r;
+  const double* __restrict__ u;
+  const double* __restrict__ u0;
+  double* __restrict__ w;
+  const double* __restrict__ kx;
+  const double* __restrict__ ky;
+  double theta;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (!interior(g, kk, jj)) return;
+    const int x = g.x;
+    const double s = op5(u[i], u[i - 1], u[i + 1], u[i - x], u[i + x], kx[i],
+                         kx[i + 1], ky[i], ky[i + x]);
+    w[i] = s;
+    const double rv = u0[i] - s;
+    r[i] = rv;
+    p[i] = rv / theta;
+  }
+};
+
+struct ChebyIterateBody {  // cheby.cpp:73-110
+  double* __restrict__ p;
+  double* __restrict__ r;
+  const double* __restrict__ u;
+  const double* __restrict__ u0;
+  double* __restrict__ w;
+  const double* __restrict__ kx;
+  const double* __restrict__ ky;
+  double alpha, beta;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (!interior(g, kk, jj)) return;
+    const int x = g.x;
+    const double s = op5(u[i], u[i - 1], u[i + 1], u[i - x], u[i + x], kx[i],
+                         kx[i + 1], ky[i], ky[i + x]);
+    w[i] = s;
+    const double rv = u0[i] - s;
+    r[i] = rv;
+    p[i] = alpha * p[i] + beta * rv;
+  }
+};
+
+struct AddBody {  // cheby.cpp:46-70 (u += p), ppcg.cpp:58 (u += sd)
+  double* __restrict__ u;
+  const double* __restrict__ d;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (interior(g, kk, jj)) u[i] = u[i] + d[i];
+  }
+};
+
+struct DivideBody {  // ppcg.cpp:7-31: sd = r / theta
+  double* __restrict__ sd;
+  const double* __restrict__ r;
+  double theta;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (interior(g, kk, jj)) sd[i] = r[i] / theta;
+  }
+};
+
+struct PpcgCalcURBody {  // ppcg.cpp:34-66
+  const double* __restrict__ sd;
+  double* __restrict__ r;
+  double* __restrict__ u;
+  const double* __restrict__ kx;
+  const double* __restrict__ ky;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (!interior(g, kk, jj)) return;
+    const int x = g.x;
+    const double sc = sd[i];
+    r[i] = r[i] - op5(sc, sd[i - 1], sd[i + 1], sd[i - x], sd[i + x], kx[i],
+                      kx[i + 1], ky[i], ky[i + x]);
+    u[i] = u[i] + sc;
+  }
+};
+
+struct PpcgCalcSDBody {  // ppcg.cpp:69-94
+  double* __restrict__ sd;
+  const double* __restrict__ r;
+  double alpha, beta;
+  Geo g;
+  __device__ void operator()(int i, int kk, int jj) const {
+    if (interior(g, kk, jj)) sd[i] = alpha * sd[i] + beta * r[i];
+  }
+};
+
+// ---- one pair per lane --------------------------------------------------------
+//
+// Lane m of block b holds cells e = 512 b + 2m and e + 1 (x is even, so the
+// pair never straddles a row).  The stencil operand's left and right
+// neighbours come from the adjacent lanes; lanes 0 and 63 load theirs.
+
+struct PairAt {
+  int e;          // first cell
+  bool valid;     // e < x*y
+  bool in0, in1;  // interior cells
+  int ic;         // e if valid, else a safe even index (x + 2)
+  int ir;         // e if either cell is interior, else x + 2
+};
+
+__device__ inline PairAt pair_at(const Geo& g, int e) {
+  PairAt q;
+  q.e = e;
+  q.valid = e < g.n;
+  const int row = q.valid ? (int)g.dx.div((uint32_t)e) : 0;
+  const int kk = e - row * g.x;
+  q.in0 = q.valid && interior(g, kk, row);
+  q.in1 = q.valid && interior(g, kk + 1, row);
+  q.ic = q.valid ? e : g.x + 2;
+  q.ir = (q.in0 || q.in1) ? e : g.x + 2;
+  return q;
+}
+
+// A stencil operand at a pair: the pair itself, the rows above and below,
+// and the cells left of e and right of e + 1.  Must be called by every lane
+// of the wave (it shuffles).  Lane 0's and lane 63's extra loads are at
+// ir - 1 and ir + 2, in range for an interior pair and for x + 2.
+struct Operand {
+  double2 c, t, b;
+  double l, r;
+};
+
+__device__ inline Operand load_operand(const double* __restrict__ a,
+                                       const PairAt& q, int x, int lane) {
+  Operand o;
+  o.c = ld2(a, q.ic);
+  o.t = ld2(a, q.ir + x);
+  o.b = ld2(a, q.ir - x);
+  double edge = 0.0;
+  if (lane == 0 || lane == kWave - 1) edge = a[lane == 0 ? q.ir - 1 : q.ir + 2];
+  o.l = __shfl_up(o.c.y, 1);
+  o.r = __shfl_down(o.c.x, 1);
+  if (lane == 0) o.l = edge;
+  if (lane == kWave - 1) o.r = edge;
+  return o;
+}
+
+// Face coefficients of a pair: kx at e, e + 1 and e + 2, ky at e, e + 1
+// and one row up.
+struct Faces {
+  double2 kx, ky, kyt;
+  double kx2;
+};
+
+__device__ inline Faces load_faces(const double* __restrict__ kx,
+                                   const double* __restrict__ ky,
+                                   const PairAt& q, int x, int lane) {
+  Faces f;
+  f.kx = ld2(kx, q.ic);
+  f.ky = ld2(ky, q.ir);
+  f.kyt = ld2(ky, q.ir + x);
+  double edge = 0.0;
+  if (lane == kWave - 1) edge = kx[q.ir + 2];
+  f.kx2 = __shfl_down(f.kx.x, 1);
+  if (lane == kWave - 1) f.kx2 = edge;
+  return f;
+}
+
+__device__ inline double op5_lo(const Operand& o, const Faces& f) {
+  return op5(o.c.x, o.l, o.c.y, o.b.x, o.t.x, f.kx.x, f.kx.y, f.ky.x, f.kyt.x);
+}
+
+__device__ inline double op5_hi(const Operand& o, const Faces& f) {
+  return op5(o.c.y, o.c.x, o.r, o.b.y, o.t.y, f.kx.y, f.kx2, f.ky.y, f.kyt.y);
+}
+
+// Stores both cells of a valid pair with one 16-byte streaming store.
+__device__ inline void st2_all(double* p, const PairAt& q, double v0,
+                               double v1) {
+  if (q.valid) st2_nt(p, q.e, true, true, v0, v1);
+}
+
+__global__ __launch_bounds__(kBlock) void jacobi_pair_kernel(
+    Geo g, const double* __restrict__ u_old, double* __restrict__ u_new,
+    const double* __restrict__ u0, const double* __restrict__ kx,
+    const double* __restrict__ ky, double* sub, int nblk) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const PairAt q = pair_at(g, blockIdx.x * kPairBlock + 2 * threadIdx.x);
+  const Operand o = load_operand(u_old, q, g.x, lane);
+  const Faces f = load_faces(kx, ky, q, g.x, lane);
+  const double2 s = ld2_nt(u0, q.ir);
+  const double n0 = q.in0 ? jacobi_cell(s.x, o.l, o.c.y, o.b.x, o.t.x, f.kx.x,
+                                        f.kx.y, f.ky.x, f.kyt.x)
+                          : o.c.x;
+  const double n1 = q.in1 ? jacobi_cell(s.y, o.c.x, o.r, o.b.y, o.t.y, f.kx.y,
+                                        f.kx2, f.ky.y, f.kyt.y)
+                          : o.c.y;
+  st2_all(u_new, q, n0, n1);
+  double v[1] = {(q.in0 ? fabs(n0 - o.c.x) : 0.0) +
+                 (q.in1 ? fabs(n1 - o.c.y) : 0.0)};
+  wave_subtree_pairs<1>(v, sub, nblk, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void cheby_pair_kernel(
+    Geo g, const double* __restrict__ u_old, double* __restrict__ u_new,
+    const double* __restrict__ u0, double* __restrict__ p,
+    double* __restrict__ r, double* __restrict__ w,
+    const double* __restrict__ kx, const double* __restrict__ ky,
+    double alpha, double beta) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const PairAt q = pair_at(g, blockIdx.x * kPairBlock + 2 * threadIdx.x);
+  const Operand o = load_operand(u_old, q, g.x, lane);
+  const Faces f = load_faces(kx, ky, q, g.x, lane);
+  const double2 s0 = ld2_nt(u0, q.ir);
+  const double2 pv = ld2_nt(p, q.ir);
+  if (!q.valid) return;
+  const double w0 = op5_lo(o, f), w1 = op5_hi(o, f);
+  const double r0 = s0.x - w0, r1 = s0.y - w1;
+  const double p0 = alpha * pv.x + beta * r0, p1 = alpha * pv.y + beta * r1;
+  st2_nt(w, q.e, q.in0, q.in1, w0, w1);
+  st2_nt(r, q.e, q.in0, q.in1, r0, r1);
+  st2_nt(p, q.e, q.in0, q.in1, p0, p1);
+  st2_all(u_new, q, q.in0 ? o.c.x + p0 : o.c.x, q.in1 ? o.c.y + p1 : o.c.y);
+}
+
+__global__ __launch_bounds__(kBlock) void ppcg_pair_kernel(
+    Geo g, const double* __restrict__ sd_old, double* __restrict__ sd_new,
+    double* __restrict__ r, double* __restrict__ u,
+    const double* __restrict__ kx, const double* __restrict__ ky,
+    double alpha, double beta) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const PairAt q = pair_at(g, blockIdx.x * kPairBlock + 2 * threadIdx.x);
+  const Operand o = load_operand(sd_old, q, g.x, lane);
+  const Faces f = load_faces(kx, ky, q, g.x, lane);
+  const double2 rv = ld2_nt(r, q.ir);
+  const double2 uv = ld2_nt(u, q.ir);
+  if (!q.valid) return;
+  const double r0 = rv.x - op5_lo(o, f), r1 = rv.y - op5_hi(o, f);
+  st2_nt(r, q.e, q.in0, q.in1, r0, r1);
+  st2_nt(u, q.e, q.in0, q.in1, uv.x + o.c.x, uv.y + o.c.y);
+  st2_all(sd_new, q, q.in0 ? alpha * o.c.x + beta * r0 : o.c.x,
+          q.in1 ? alpha * o.c.y + beta * r1 : o.c.y);
+}
+
+template <class Body>
+int launch_each_cell(tea_hip_chunk* c, int kid, const Body& body) {
+  {
+    Timed t(c, kid);
+    hipLaunchKernelGGL((each_kernel<Body>), flat_grid(c->geo.n), dim3(kBlock),
+                       0, c->stream, c->geo, body);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
+// The fused Chebyshev and PPCG passes need the x*y spare buffer.
+int ensure_spare(tea_hip_chunk* c) {
+  if (c->spare) return TEA_HIP_OK;
+  if (hipMalloc(&c->spare, (size_t)c->geo.n * sizeof(double)) != hipSuccess) {
+    c->spare = nullptr;
+    set_error("hipMalloc of the spare field buffer failed");
+    return TEA_HIP_ENOMEM;
+  }
+  return TEA_HIP_OK;
+}
+
+// Fused passes run on even row lengths in the default mode.
+bool fused_pairs(const tea_hip_chunk* c) { return c->fused && c->pairs; }
+
+}  // namespace
+}  // namespace tea
+
+using namespace tea;
+
+extern "C" {
+
+int tea_hip_jacobi_init(tea_hip_chunk* c, int coefficient, double rx,
+                        double ry) {
+  TEA_ENTER(c);
+  TEA_CHECK_ARG(coefficient == TEA_CONDUCTIVITY ||
+                    coefficient == TEA_RECIP_CONDUCTIVITY,
+                "unknown coefficient");
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  const Geo g = c->geo;
+  return launch_each_cell(
+      c, K_JACOBI_INIT,
+      JacobiInitBody{f[TEA_FIELD_U], f[TEA_FIELD_U0], f[TEA_FIELD_DENSITY],
+                     f[TEA_FIELD_ENERGY], f[TEA_FIELD_KX], f[TEA_FIELD_KY], rx,
+                     ry, coefficient, g.x, g.y, g.h});
+}
+
+int tea_hip_jacobi_iterate(tea_hip_chunk* c, double* error) {
+  TEA_ENTER(c);
+  TEA_CHECK_ARG(error != nullptr, "null error");
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  const Geo g = c->geo;
+  if (!(c->fused && use_pair_blocks(c, 1))) {
+    TEA_RETURN_IF(launch_each_cell(c, K_JACOBI_COPY_U,
+                                   CopyAllBody{f[TEA_FIELD_U], f[TEA_FIELD_R]}));
+    {
+      Timed t(c, K_JACOBI_ITERATE);
+      hipLaunchKernelGGL(
+          (span_kernel<1, JacobiBody>), span_grid(c), dim3(kBlock), 0,
+          c->stream, g,
+          JacobiBody{f[TEA_FIELD_U], f[TEA_FIELD_U0], f[TEA_FIELD_R],
+                     f[TEA_FIELD_KX], f[TEA_FIELD_KY], g.x},
+          c->partials, c->nspan, c->ngroup == 1 ? 1 : 0);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    return finish_reduction(c, 1, error);
+  }
+  {
+    Timed t(c, K_JACOBI_FUSED);
+    hipLaunchKernelGGL(jacobi_pair_kernel, pair_grid(c), dim3(kBlock), 0,
+                       c->stream, g, f[TEA_FIELD_U], f[TEA_FIELD_R],
+                       f[TEA_FIELD_U0], f[TEA_FIELD_KX], f[TEA_FIELD_KY],
+                       c->sub, c->nblk);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  // The new u is in r's buffer; the old u becomes r (the reference's copy).
+  std::swap(f[TEA_FIELD_U], f[TEA_FIELD_R]);
+  int len = c->nspan;
+  TEA_RETURN_IF(launch_span_finish(c, 1, &len));
+  return finish_reduction_n(c, 1, len, error);
+}
+
+int tea_hip_cheby_init(tea_hip_chunk* c, double theta) {
+  TEA_ENTER(c);
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  return launch_each_cell(
+      c, K_CHEBY_INIT,
+      ChebyInitBody{f[TEA_FIELD_P], f[TEA_FIELD_R], f[TEA_FIELD_U],
+                    f[TEA_FIELD_U0], f[TEA_FIELD_W], f[TEA_FIELD_KX],
+                    f[TEA_FIELD_KY], theta, c->geo});
+}
+
+int tea_hip_cheby_iterate(tea_hip_chunk* c, double alpha, double beta) {
+  TEA_ENTER(c);
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  const Geo g = c->geo;
+  if (!fused_pairs(c)) {
+    TEA_RETURN_IF(launch_each_cell(
+        c, K_CHEBY_ITERATE,
+        ChebyIterateBody{f[TEA_FIELD_P], f[TEA_FIELD_R], f[TEA_FIELD_U],
+                         f[TEA_FIELD_U0], f[TEA_FIELD_W], f[TEA_FIELD_KX],
+                         f[TEA_FIELD_KY], alpha, beta, g}));
+    return launch_each_cell(c, K_CHEBY_CALC_U,
+                            AddBody{f[TEA_FIELD_U], f[TEA_FIELD_P], g});
+  }
+  TEA_RETURN_IF(ensure_spare(c));
+  {
+    Timed t(c, K_CHEBY_FUSED);
+    hipLaunchKernelGGL(cheby_pair_kernel, pair_grid(c), dim3(kBlock), 0,
+                       c->stream, g, f[TEA_FIELD_U], c->spare, f[TEA_FIELD_U0],
+                       f[TEA_FIELD_P], f[TEA_FIELD_R], f[TEA_FIELD_W],
+                       f[TEA_FIELD_KX], f[TEA_FIELD_KY], alpha, beta);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  std::swap(f[TEA_FIELD_U], c->spare);
+  return TEA_HIP_OK;
+}
+
+int tea_hip_ppcg_init(tea_hip_chunk* c, double theta) {
+  TEA_ENTER(c);
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  return launch_each_cell(
+      c, K_PPCG_INIT, DivideBody{f[TEA_FIELD_SD], f[TEA_FIELD_R], theta, c->geo});
+}
+
+int tea_hip_ppcg_inner_iteration(tea_hip_chunk* c, double alpha,
+                                 double beta) {
+  TEA_ENTER(c);
+  TEA_RETURN_IF(apply_pending(c));
+  double** f = c->field;
+  const Geo g = c->geo;
+  if (!fused_pairs(c)) {
+    TEA_RETURN_IF(launch_each_cell(
+        c, K_PPCG_CALC_UR,
+        PpcgCalcURBody{f[TEA_FIELD_SD], f[TEA_FIELD_R], f[TEA_FIELD_U],
+                       f[TEA_FIELD_KX], f[TEA_FIELD_KY], g}));
+    return launch_each_cell(
+        c, K_PPCG_CALC_SD,
+        PpcgCalcSDBody{f[TEA_FIELD_SD], f[TEA_FIELD_R], alpha, beta, g});
+  }
+  TEA_RETURN_IF(ensure_spare(c));
+  {
+    Timed t(c, K_PPCG_FUSED);
+    hipLaunchKernelGGL(ppcg_pair_kernel, pair_grid(c), dim3(kBlock), 0,
+                       c->stream, g, f[TEA_FIELD_SD], c->spare, f[TEA_FIELD_R],
+                       f[TEA_FIELD_U], f[TEA_FIELD_KX], f[TEA_FIELD_KY], alpha,
+                       beta);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  std::swap(f[TEA_FIELD_SD], c->spare);
+  return TEA_HIP_OK;
+}
+
+}  // extern "C"

@@ -245,6 +245,28 @@ __device__ inline void span_reduce_pairs_n(double (&v)[NR][kSlots], double* part
     }
   }
 }
+// Streaming (nontemporal) access for data read or written once per pass;
+// stencil operands (p, r, kx, ky) use ld2 so the neighbouring rows can reuse
+// them from L2.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+__device__ inline double2 ld2_nt(const double* p, int i) {
+  const dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(p + i));
+  return double2{v.x, v.y};
+}
+
+// Stores a cell pair with one 16-byte streaming store, or the interior
+// member alone.
+__device__ inline void st2_nt(double* p, int i, bool in0, bool in1, double v0,
+                              double v1) {
+  if (in0 && in1) {
+    __builtin_nontemporal_store(dvec2{v0, v1}, reinterpret_cast<dvec2*>(p + i));
+  } else {
+    if (in0) __builtin_nontemporal_store(v0, p + i);
+    if (in1) __builtin_nontemporal_store(v1, p + i + 1);
+  }
+}
+
 // ---- one pair per thread ---------------------------------------------------
 //
 // A 256-thread block over 512 consecutive cells = 8 groups of the reduction
@@ -357,6 +379,18 @@ enum KernelId {
   K_FCG_NET,
   K_FCG_PROLONG,
   K_SPAN_FINISH,
+  K_JACOBI_INIT,
+  K_JACOBI_COPY_U,
+  K_JACOBI_ITERATE,
+  K_JACOBI_FUSED,
+  K_CHEBY_INIT,
+  K_CHEBY_ITERATE,
+  K_CHEBY_CALC_U,
+  K_CHEBY_FUSED,
+  K_PPCG_INIT,
+  K_PPCG_CALC_UR,
+  K_PPCG_CALC_SD,
+  K_PPCG_FUSED,
   K_NUM
 };
 
@@ -448,6 +482,11 @@ struct tea_hip_chunk {
   double* sub = nullptr;
   int pair_blocks = 3;              // pair-per-thread kernels: bit 0 fused_w,
                                     // bit 1 fused_r
+
+  // Spare FP64 field buffer (x*y, allocated by the first Chebyshev or PPCG
+  // call): the fused Chebyshev and PPCG steps write the new u or sd into it
+  // and exchange it with the field.
+  double* spare = nullptr;
 
   // Deferred CG work (fused mode).
   int fused = 1;
@@ -545,6 +584,11 @@ int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw);
 // One reflective-halo launch (all flagged fields and external faces).
 int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
                  const int fields[6]);
+
+// Checks the handle and makes its device current (entry points).
+#define TEA_ENTER(c)                                      \
+  TEA_CHECK_ARG((c) != nullptr, "null chunk handle");     \
+  TEA_HIP_TRY(hipSetDevice((c)->device))
 
 inline dim3 span_grid(const tea_hip_chunk* c) { return dim3(c->nspan); }
 inline dim3 pair_grid(const tea_hip_chunk* c) { return dim3(c->nblk); }

@@ -37,6 +37,15 @@ static void solve(Chunk* chunks, Settings* settings, int tt,
 
   double error = 1e+10;
   switch (settings->solver) {
+    case JACOBI_SOLVER:
+      jacobi_driver(chunks, settings, rx, ry, &error);
+      break;
+    case CHEBY_SOLVER:
+      cheby_driver(chunks, settings, rx, ry, &error);
+      break;
+    case PPCG_SOLVER:
+      ppcg_driver(chunks, settings, rx, ry, &error);
+      break;
     case CG_SOLVER:
       cg_driver(chunks, settings, rx, ry, &error);
       break;

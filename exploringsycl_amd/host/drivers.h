@@ -1,7 +1,5 @@
 /* Host-side solver drivers: each loops over the rank's chunks and calls the
- * kernel_interface entry points (TeaLeaf/drivers/drivers.h:5-72 for the CG
- * path).  The Chebyshev, PPCG and Jacobi solvers of the reference are not
- * part of this build. */
+ * kernel_interface entry points (TeaLeaf/drivers/drivers.h:5-72). */
 #pragma once
 #ifndef TEA_DRIVERS_H
 #define TEA_DRIVERS_H
@@ -22,6 +20,16 @@ void cg_init_driver(Chunk* chunks, Settings* settings, double rx, double ry,
                     double* rro);
 void cg_main_step_driver(Chunk* chunks, Settings* settings, int tt,
                          double* rro, double* error);
+
+/* The reference's other solvers (drivers_jacobi_cheby_ppcg.cpp): Jacobi,
+ * Chebyshev after CG steps that estimate the eigenvalue bounds, and CG
+ * preconditioned by ppcg_inner_steps Chebyshev steps (PPCG). */
+void jacobi_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+                   double* error);
+void cheby_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+                  double* error);
+void ppcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
+                 double* error);
 
 /* Mixed-precision CG (this project's addition): relative-residual stop
  * (rel_eps, default 1e-10) confirmed by the FP64 true residual. */

@@ -172,9 +172,11 @@ void write_run_json(const char* path, Settings* settings) {
     const StepRecord& s = g_run.steps[i];
     fprintf(fp, "%s\n    {\"step\": %d, \"iterations\": %d, \"executed\": %d, "
                 "\"refreshes\": %d, \"fallback_at\": %d, \"rel_residual\": %.17g, "
-                "\"error\": %.17g, \"seconds\": %.9f}",
+                "\"error\": %.17g, \"seconds\": %.9f, \"solver_iterations\": %d, "
+                "\"estimated\": %d, \"eigmin\": %.17g, \"eigmax\": %.17g}",
             i ? "," : "", s.step, s.iterations, s.executed, s.refreshes,
-            s.fallback_at, s.rel_residual, s.error, s.seconds);
+            s.fallback_at, s.rel_residual, s.error, s.seconds,
+            s.solver_iterations, s.estimated, s.eigmin, s.eigmax);
   }
   fprintf(fp, "\n  ],\n  \"summaries\": [");
   for (size_t i = 0; i < g_run.summaries.size(); ++i) {

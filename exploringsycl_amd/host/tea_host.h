@@ -20,6 +20,11 @@ struct StepRecord {
   double rel_residual; /* final |b - A u| / |b| where the solver computes it */
   double error;       /* last rr of the solve */
   double seconds;     /* wall time of the whole timestep */
+  /* Chebyshev / PPCG: iterations after the switch from CG as printed, the
+   * printed Chebyshev estimate, and the eigenvalue bounds (-1 / 0: none). */
+  int solver_iterations = -1;
+  int estimated = -1;
+  double eigmin = 0.0, eigmax = 0.0;
 };
 
 struct SummaryRecord {

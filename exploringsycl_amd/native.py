@@ -57,6 +57,12 @@ _SIGNATURES = {
     "tea_hip_calculate_residual": [_vp],
     "tea_hip_calculate_2norm": [_vp, _vp, _dp],
     "tea_hip_finalise": [_vp],
+    "tea_hip_jacobi_init": [_vp, _i, _d, _d],
+    "tea_hip_jacobi_iterate": [_vp, _dp],
+    "tea_hip_cheby_init": [_vp, _d],
+    "tea_hip_cheby_iterate": [_vp, _d, _d],
+    "tea_hip_ppcg_init": [_vp, _d],
+    "tea_hip_ppcg_inner_iteration": [_vp, _d, _d],
     "tea_hip_mpcg_init": [_vp, _i, _d, _d, _dp, _dp],
     "tea_hip_mpcg_calc_w": [_vp, _d, _i, _d, _i, _dp, _dp],
     "tea_hip_mpcg_calc_r": [_vp, _d, _dp, _dp],

@@ -87,9 +87,11 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
  * (TeaLeaf/kernel_interface.h:20; kernel_initialise.cpp:83-126). */
 int tea_hip_chunk_destroy(tea_hip_chunk* chunk);
 
-/* Device address and element count of a field.  The address of
- * TEA_FIELD_P can change after CG calls (the fused iteration double-buffers
- * p); ask again after each call instead of caching it. */
+/* Device address and element count of a field.  The addresses of
+ * TEA_FIELD_P, TEA_FIELD_U, TEA_FIELD_R and TEA_FIELD_SD can change after
+ * solver calls (the fused CG, Jacobi, Chebyshev and PPCG iterations write
+ * the new p, u or sd into a second buffer and exchange the two); ask again
+ * after each call instead of caching them. */
 int tea_hip_field(tea_hip_chunk* chunk, int field, double** dev_ptr,
                   size_t* count);
 
@@ -187,6 +189,47 @@ int tea_hip_calculate_2norm(tea_hip_chunk* chunk, const double* buffer,
 /* energy = u / density on the interior.  Replaces run_finalise
  * (TeaLeaf/kernel_interface.h:69-70; solver_methods.cpp:120-145). */
 int tea_hip_finalise(tea_hip_chunk* chunk);
+
+/* ---- Jacobi, Chebyshev and PPCG solver kernels -------------------------- */
+
+/* u0 = u = energy*density one cell in from the edge of the padded grid, and
+ * the face coefficients kx, ky from the density (conductivity model
+ * `coefficient`).  Replaces run_jacobi_init (TeaLeaf/kernel_interface.h:
+ * 52-53; jacobi.cpp:7-54). */
+int tea_hip_jacobi_init(tea_hip_chunk* chunk, int coefficient, double rx,
+                        double ry);
+
+/* One Jacobi sweep: r = u on every cell, then on the interior
+ * u = (u0 + kx' r_right + kx r_left + ky' r_top + ky r_bottom) /
+ *     (1 + kx + kx' + ky + ky'),
+ * and *error = the sum over the interior of |u - r|.  The sweep writes the
+ * new u into r's buffer and exchanges the two fields' addresses.  Replaces
+ * run_jacobi_iterate (TeaLeaf/kernel_interface.h:54-55;
+ * kernel_interface.cpp:287-300, jacobi.cpp:57-137). */
+int tea_hip_jacobi_iterate(tea_hip_chunk* chunk, double* error);
+
+/* w = A u, r = u0 - w, p = r / theta on the interior.  Replaces
+ * run_cheby_init (TeaLeaf/kernel_interface.h:46-47; cheby.cpp:7-43). */
+int tea_hip_cheby_init(tea_hip_chunk* chunk, double theta);
+
+/* One Chebyshev step on the interior: w = A u, r = u0 - w,
+ * p = alpha p + beta r, u = u + p (one pass; the new u goes to a second
+ * buffer whose address replaces u's).  Replaces run_cheby_iterate
+ * (TeaLeaf/kernel_interface.h:48-49; kernel_interface.cpp:258-271,
+ * cheby.cpp:46-110). */
+int tea_hip_cheby_iterate(tea_hip_chunk* chunk, double alpha, double beta);
+
+/* sd = r / theta on the interior.  Replaces run_ppcg_init
+ * (TeaLeaf/kernel_interface.h:58-59; ppcg.cpp:7-31). */
+int tea_hip_ppcg_init(tea_hip_chunk* chunk, double theta);
+
+/* One inner Chebyshev step of PPCG on the interior: r = r - A sd,
+ * u = u + sd, sd = alpha sd + beta r (one pass; the new sd goes to a second
+ * buffer whose address replaces sd's).  Replaces run_ppcg_inner_iteration
+ * (TeaLeaf/kernel_interface.h:60-61; kernel_interface.cpp:314-328,
+ * ppcg.cpp:34-94). */
+int tea_hip_ppcg_inner_iteration(tea_hip_chunk* chunk, double alpha,
+                                 double beta);
 
 /* ---- mixed-precision CG --------------------------------------------------
  * No counterpart in the reference (its CG is FP64 throughout).  p, r, w and

@@ -4,8 +4,10 @@
 // own host), never into the product.
 //
 // Mirrors TeaLeaf/c_kernels/sycl/kernel_interface.cpp: the same kernels in
-// the same order for each entry point; scalar results are added into the
-// driver's accumulators.
+// the same order for each entry point.  Scalar results are assigned or added
+// as that backend does: rro and pw are added (cg.cpp:133,194); rrn, the
+// field summary, the Jacobi error and the 2-norm are assigned (cg.cpp:253,
+// field_summary.cpp:147-150, jacobi.cpp:116, solver_methods.cpp:116).
 #include <stdlib.h>
 #include <string.h>
 
@@ -136,10 +138,10 @@ void run_field_summary(Chunk* c, Settings* s, double* vol, double* mass,
   double out[4];
   orc_field_summary(c->x, c->y, s->halo_depth, c->volume, c->density,
                     c->energy0, c->u, out);
-  *vol += out[0];
-  *mass += out[1];
-  *ie += out[2];
-  *temp += out[3];
+  *vol = out[0];
+  *mass = out[1];
+  *ie = out[2];
+  *temp = out[3];
 }
 
 void run_cg_init(Chunk* c, Settings* s, double rx, double ry, double* rro) {
@@ -152,7 +154,7 @@ void run_cg_calc_w(Chunk* c, Settings* s, double* pw) {
 }
 
 void run_cg_calc_ur(Chunk* c, Settings* s, double alpha, double* rrn) {
-  *rrn += orc_cg_calc_ur(c->x, c->y, s->halo_depth, alpha, c->u, c->r, c->p,
+  *rrn = orc_cg_calc_ur(c->x, c->y, s->halo_depth, alpha, c->u, c->r, c->p,
                          c->w);
 }
 
@@ -160,15 +162,35 @@ void run_cg_calc_p(Chunk* c, Settings* s, double beta) {
   orc_cg_calc_p(c->x, c->y, s->halo_depth, beta, c->p, c->r);
 }
 
-static void not_built(const char* what) {
-  die(__LINE__, __FILE__, "%s is not part of the oracle\n", what);
+void run_cheby_init(Chunk* c, Settings* s) {
+  orc_cheby_init(c->x, c->y, s->halo_depth, c->theta, c->p, c->r, c->u,
+                 c->u0, c->w, c->kx, c->ky);
 }
-void run_cheby_init(Chunk*, Settings*) { not_built("Chebyshev"); }
-void run_cheby_iterate(Chunk*, Settings*, double, double) { not_built("Chebyshev"); }
-void run_jacobi_init(Chunk*, Settings*, double, double) { not_built("Jacobi"); }
-void run_jacobi_iterate(Chunk*, Settings*, double*) { not_built("Jacobi"); }
-void run_ppcg_init(Chunk*, Settings*) { not_built("PPCG"); }
-void run_ppcg_inner_iteration(Chunk*, Settings*, double, double) { not_built("PPCG"); }
+
+void run_cheby_iterate(Chunk* c, Settings* s, double alpha, double beta) {
+  orc_cheby_iterate(c->x, c->y, s->halo_depth, alpha, beta, c->p, c->r, c->u,
+                    c->u0, c->w, c->kx, c->ky);
+}
+
+void run_jacobi_init(Chunk* c, Settings* s, double rx, double ry) {
+  orc_jacobi_init(c->x, c->y, s->halo_depth, s->coefficient, rx, ry, c->u,
+                  c->u0, c->density, c->energy, c->kx, c->ky);
+}
+
+void run_jacobi_iterate(Chunk* c, Settings* s, double* error) {
+  *error = orc_jacobi_iterate(c->x, c->y, s->halo_depth, c->u, c->u0, c->r,
+                              c->kx, c->ky);
+}
+
+void run_ppcg_init(Chunk* c, Settings* s) {
+  orc_ppcg_init(c->x, c->y, s->halo_depth, c->theta, c->sd, c->r);
+}
+
+void run_ppcg_inner_iteration(Chunk* c, Settings* s, double alpha,
+                              double beta) {
+  orc_ppcg_inner_iteration(c->x, c->y, s->halo_depth, alpha, beta, c->sd,
+                           c->r, c->u, c->kx, c->ky);
+}
 
 void run_copy_u(Chunk* c, Settings* s) {
   orc_copy_u(c->x, c->y, s->halo_depth, c->u, c->u0);
@@ -181,7 +203,7 @@ void run_calculate_residual(Chunk* c, Settings* s) {
 
 void run_calculate_2norm(Chunk* c, Settings* s, FieldBufferType buffer,
                          double* norm) {
-  *norm += orc_calculate_2norm(c->x, c->y, s->halo_depth, buffer);
+  *norm = orc_calculate_2norm(c->x, c->y, s->halo_depth, buffer);
 }
 
 void run_finalise(Chunk* c, Settings* s) {

@@ -5,6 +5,7 @@
  * Reference files (relative to the reference repository):
  *   TeaLeaf/c_kernels/sycl/cg.cpp              cg_init_u/_k/_others, calc_w/ur/p
  *   TeaLeaf/c_kernels/sycl/solver_methods.cpp  copy_u, residual, 2norm, finalise
+ *   TeaLeaf/c_kernels/sycl/jacobi.cpp, cheby.cpp, ppcg.cpp  the other solvers
  *   TeaLeaf/c_kernels/sycl/local_halos.cpp     update_left/right/top/bottom
  *   TeaLeaf/c_kernels/sycl/field_summary.cpp   field_summary_func (+_reduction)
  *   TeaLeaf/c_kernels/sycl/set_chunk_data.cpp  mesh
@@ -26,7 +27,18 @@
  *   - field summary: closed-form volume/mass/ie/temp for grids on which the
  *     tea.in rectangles fall on cell edges;
  *   - reduction tree: inputs whose pairwise-tree sum differs from the
- *     sequential sum by a value worked out by hand, and exact integer sums.
+ *     sequential sum by a value worked out by hand, and exact integer sums;
+ *   - orc_jacobi_*: jacobi_init's coefficients equal cg_init's, the dense
+ *     solution is a fixed point of a sweep, sweeps contract in the max norm,
+ *     r is the previous iterate and the error the L1 norm of the update;
+ *   - orc_cheby_*: with exact eigenvalue bounds the iteration meets the
+ *     Chebyshev error bound |e_k| <= |e_0| / T_k(sigma) at every step, and
+ *     w, r are A u and b - A u of the iterate before the step;
+ *   - orc_ppcg_*: the inner steps keep r = r0 - A z and meet the same bound
+ *     on the residual;
+ *   - all three solvers: the known answers of TeaLeaf/tea.problems, and the
+ *     reference's own host driving this oracle prints the same iteration
+ *     counts, eigenvalue bounds and temperatures as this project's host.
  * No function is left unpinned.
  */
 #include "tea_oracle.h"
@@ -362,6 +374,120 @@ void orc_finalise(int x, int y, int h, const double* u, const double* density,
     for (long kk = h; kk < x - h; ++kk) {
       const long i = kk + jj * x;
       energy[i] = u[i] / density[i];
+    }
+}
+
+/* ---- Jacobi, Chebyshev and PPCG -------------------------------------------
+ * Jacobi (jacobi.cpp), Chebyshev (cheby.cpp) and PPCG inner steps (ppcg.cpp),
+ * composed as the reference's kernel_interface.cpp:246-328 composes them. */
+
+/* jacobi_init (jacobi.cpp:7-54): u0 = u = energy*density one cell in from
+ * the edge of the padded grid; kx, ky from the density as in cg_init_k. */
+void orc_jacobi_init(int x, int y, int h, int coefficient, double rx,
+                     double ry, double* u, double* u0, const double* density,
+                     const double* energy, double* kx, double* ky) {
+#pragma omp parallel for schedule(static)
+  for (long jj = 0; jj < y; ++jj) {
+    for (long kk = 0; kk < x; ++kk) {
+      const long i = kk + jj * x;
+      if (kk > 0 && kk < x - 1 && jj > 0 && jj < y - 1) {
+        u0[i] = energy[i] * density[i];
+        u[i] = u0[i];
+      }
+      if (jj >= h && jj < y - 1 && kk >= h && kk < x - 1) {
+        const int cond = coefficient == ORC_CONDUCTIVITY;
+        const double dc = cond ? density[i] : 1.0 / density[i];
+        const double dl = cond ? density[i - 1] : 1.0 / density[i - 1];
+        const double dd = cond ? density[i - x] : 1.0 / density[i - x];
+        kx[i] = rx * (dl + dc) / (2.0 * dl * dc);
+        ky[i] = ry * (dd + dc) / (2.0 * dd * dc);
+      }
+    }
+  }
+}
+
+/* run_jacobi_iterate (kernel_interface.cpp:287-300): jacobi_copy_u
+ * (jacobi.cpp:120-137, r = u on every cell) then jacobi_iterate
+ * (jacobi.cpp:57-117); returns the tree sum of |u_new - r|. */
+double orc_jacobi_iterate(int x, int y, int h, double* u, const double* u0,
+                          double* r, const double* kx, const double* ky) {
+  const long n = (long)x * y;
+  memcpy(r, u, (size_t)n * sizeof(double));
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, {
+    u[i] = (u0[i] + (kx[i + 1] * r[i + 1] + kx[i] * r[i - 1]) +
+            (ky[i + x] * r[i + x] + ky[i] * r[i - x])) /
+           (1.0 + (kx[i] + kx[i + 1]) + (ky[i] + ky[i + x]));
+    val = fabs(u[i] - r[i]);
+  });
+  return finish(part, x, y);
+}
+
+/* cheby_init (cheby.cpp:7-43): w = A u, r = u0 - w, p = r / theta. */
+void orc_cheby_init(int x, int y, int h, double theta, double* p, double* r,
+                    const double* u, const double* u0, double* w,
+                    const double* kx, const double* ky) {
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) {
+      const long i = kk + jj * x;
+      w[i] = orc_smvp(u, kx, ky, i, x);
+      r[i] = u0[i] - w[i];
+      p[i] = r[i] / theta;
+    }
+}
+
+/* run_cheby_iterate (kernel_interface.cpp:258-271): cheby_iterate
+ * (cheby.cpp:73-110: w = A u, r = u0 - w, p = alpha p + beta r) over the
+ * whole interior, then cheby_calc_u (cheby.cpp:46-70: u += p). */
+void orc_cheby_iterate(int x, int y, int h, double alpha, double beta,
+                       double* p, double* r, double* u, const double* u0,
+                       double* w, const double* kx, const double* ky) {
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) {
+      const long i = kk + jj * x;
+      w[i] = orc_smvp(u, kx, ky, i, x);
+      r[i] = u0[i] - w[i];
+      p[i] = alpha * p[i] + beta * r[i];
+    }
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) {
+      const long i = kk + jj * x;
+      u[i] = u[i] + p[i];
+    }
+}
+
+/* ppcg_init (ppcg.cpp:7-31): sd = r / theta. */
+void orc_ppcg_init(int x, int y, int h, double theta, double* sd,
+                   const double* r) {
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) {
+      const long i = kk + jj * x;
+      sd[i] = r[i] / theta;
+    }
+}
+
+/* run_ppcg_inner_iteration (kernel_interface.cpp:314-328): ppcg_calc_ur
+ * (ppcg.cpp:34-66: r -= A sd, u += sd) over the whole interior, then
+ * ppcg_calc_sd (ppcg.cpp:69-94: sd = alpha sd + beta r). */
+void orc_ppcg_inner_iteration(int x, int y, int h, double alpha, double beta,
+                              double* sd, double* r, double* u,
+                              const double* kx, const double* ky) {
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) {
+      const long i = kk + jj * x;
+      r[i] = r[i] - orc_smvp(sd, kx, ky, i, x);
+      u[i] = u[i] + sd[i];
+    }
+#pragma omp parallel for schedule(static)
+  for (long jj = h; jj < y - h; ++jj)
+    for (long kk = h; kk < x - h; ++kk) {
+      const long i = kk + jj * x;
+      sd[i] = alpha * sd[i] + beta * r[i];
     }
 }
 

@@ -96,6 +96,25 @@ double orc_calculate_2norm(int x, int y, int h, const double* b);
 void orc_finalise(int x, int y, int h, const double* u, const double* density,
                   double* energy);
 
+/* jacobi.cpp, cheby.cpp, ppcg.cpp, composed as kernel_interface.cpp:246-328
+ * composes them.  orc_jacobi_iterate returns the error sum |u_new - u_old|. */
+void orc_jacobi_init(int x, int y, int h, int coefficient, double rx,
+                     double ry, double* u, double* u0, const double* density,
+                     const double* energy, double* kx, double* ky);
+double orc_jacobi_iterate(int x, int y, int h, double* u, const double* u0,
+                          double* r, const double* kx, const double* ky);
+void orc_cheby_init(int x, int y, int h, double theta, double* p, double* r,
+                    const double* u, const double* u0, double* w,
+                    const double* kx, const double* ky);
+void orc_cheby_iterate(int x, int y, int h, double alpha, double beta,
+                       double* p, double* r, double* u, const double* u0,
+                       double* w, const double* kx, const double* ky);
+void orc_ppcg_init(int x, int y, int h, double theta, double* sd,
+                   const double* r);
+void orc_ppcg_inner_iteration(int x, int y, int h, double alpha, double beta,
+                              double* sd, double* r, double* u,
+                              const double* kx, const double* ky);
+
 /* Mixed-precision CG (no reference counterpart; restates the arithmetic of
  * exploringsycl_amd/csrc/mpcg.hip): FP32 p, r, w, kx, ky; FP64 u, u0,
  * dot products and scalars.  Halo updates of p and u use orc_halo_field

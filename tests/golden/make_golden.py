@@ -12,9 +12,12 @@ fortran_bm.json
 refhost_oracle.json
     Output of the reference's own host program (TeaLeaf/*.c and
     TeaLeaf/drivers/*.c compiled unchanged, see oracle/Makefile) linked to the
-    CPU oracle backend, on a few decks: per-step "CG: N iterations" counts and
-    the final "Actual" temperature as printed (%.15e).  This pins the host
-    control flow of this project's driver against the reference's.
+    CPU oracle backend, on a few decks for each solver (CG, Jacobi, Chebyshev,
+    PPCG): per-step "CG: N" / "Jacobi: N" iteration counts, the Chebyshev and
+    PPCG counts ("Cheby: N iterations (E estimated)", "PPCG: N iterations"),
+    the eigenvalue bounds as printed (%.12e) and the final "Actual"
+    temperature as printed (%.15e).  This pins the host control flow of this
+    project's drivers against the reference's.
 """
 
 import json
@@ -61,6 +64,22 @@ def refhost_decks():
         "rect_120x90x3_recip": Deck.with_random_rectangles(
             120, 90, 8, problem_seed(9, 1), end_step=3, coefficient="inverse"),
         "bm2_250x250x10": Deck(250, 250, end_step=10),
+        "jacobi_64x64x1": Deck(64, 64, end_step=1, extra=["use_jacobi"]),
+        "jacobi_rect_40x30x2": Deck.with_random_rectangles(
+            40, 30, 4, problem_seed(9, 2), end_step=2,
+            extra=["use_jacobi", "eps 1e-12"]),
+        "cheby_250x250x2": Deck(250, 250, end_step=2, extra=["use_chebyshev"]),
+        "cheby_errswitch_64x64x1": Deck(64, 64, end_step=1, extra=[
+            "use_chebyshev", "errswitch", "epslim 1e-4"]),
+        "cheby_rect_recip_120x90x3": Deck.with_random_rectangles(
+            120, 90, 8, problem_seed(9, 1), end_step=3, coefficient="inverse",
+            extra=["use_chebyshev", "presteps 40"]),
+        "ppcg_250x250x2": Deck(250, 250, end_step=2, extra=["use_ppcg"]),
+        "ppcg_errswitch_64x64x1": Deck(64, 64, end_step=1, extra=[
+            "use_ppcg", "errswitch"]),
+        "ppcg_rect_recip_120x90x3": Deck.with_random_rectangles(
+            120, 90, 8, problem_seed(9, 1), end_step=3, coefficient="inverse",
+            extra=["use_ppcg", "ppcg_inner_steps 6"]),
     }
 
 
@@ -72,9 +91,18 @@ def run_refhost(deck):
         p = subprocess.run([REFHOST], cwd=wd, capture_output=True, text=True,
                            env=dict(os.environ, OMP_NUM_THREADS="8"), check=True)
     lines = p.stdout.splitlines()
-    iters = [int(l.split()[1]) for l in lines if l.startswith("CG:")]
+    iters = [int(l.split()[1]) for l in lines
+             if l.startswith("CG:") or l.startswith("Jacobi:")]
+    after = []
+    for l in lines:
+        m = re.match(r"(Cheby|PPCG):\s+(\d+) iterations(?: \((\d+) estimated\))?", l)
+        if m:
+            after.append([int(m.group(2)), int(m.group(3)) if m.group(3) else -1])
+    eig = [l.split()[2] for l in lines if l.startswith("Min. eigenvalue")]
+    eig = list(zip(eig, [l.split()[2] for l in lines if l.startswith("Max. eigenvalue")]))
     actual = [l.split()[1] for l in lines if l.startswith("Actual")]
-    return dict(iterations=iters, final_temp_printed=actual[-1] if actual else None)
+    return dict(iterations=iters, after_switch=after, eigenvalues=[list(e) for e in eig],
+                final_temp_printed=actual[-1] if actual else None)
 
 
 def main():

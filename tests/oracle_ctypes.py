@@ -45,6 +45,12 @@ class Oracle:
             "orc_calculate_2norm": ([_I, _I, _I, _P], _D),
             "orc_finalise": ([_I, _I, _I, _P, _P, _P], None),
             "orc_smvp": ([_P, _P, _P, ctypes.c_long, ctypes.c_long], _D),
+            "orc_jacobi_init": ([_I, _I, _I, _I, _D, _D] + [_P] * 6, None),
+            "orc_jacobi_iterate": ([_I, _I, _I] + [_P] * 5, _D),
+            "orc_cheby_init": ([_I, _I, _I, _D] + [_P] * 7, None),
+            "orc_cheby_iterate": ([_I, _I, _I, _D, _D] + [_P] * 7, None),
+            "orc_ppcg_init": ([_I, _I, _I, _D, _P, _P], None),
+            "orc_ppcg_inner_iteration": ([_I, _I, _I, _D, _D] + [_P] * 5, None),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -112,6 +118,31 @@ class Oracle:
 
     def finalise(self, x, y, h, f):
         self.lib.orc_finalise(x, y, h, _p(f["u"]), _p(f["density"]), _p(f["energy"]))
+
+    def jacobi_init(self, x, y, h, coef, rx, ry, f):
+        self.lib.orc_jacobi_init(x, y, h, coef, rx, ry, _p(f["u"]), _p(f["u0"]),
+                                 _p(f["density"]), _p(f["energy"]), _p(f["kx"]),
+                                 _p(f["ky"]))
+
+    def jacobi_iterate(self, x, y, h, f):
+        return self.lib.orc_jacobi_iterate(x, y, h, _p(f["u"]), _p(f["u0"]), _p(f["r"]),
+                                           _p(f["kx"]), _p(f["ky"]))
+
+    def cheby_init(self, x, y, h, theta, f):
+        self.lib.orc_cheby_init(x, y, h, theta, _p(f["p"]), _p(f["r"]), _p(f["u"]),
+                                _p(f["u0"]), _p(f["w"]), _p(f["kx"]), _p(f["ky"]))
+
+    def cheby_iterate(self, x, y, h, alpha, beta, f):
+        self.lib.orc_cheby_iterate(x, y, h, alpha, beta, _p(f["p"]), _p(f["r"]),
+                                   _p(f["u"]), _p(f["u0"]), _p(f["w"]), _p(f["kx"]),
+                                   _p(f["ky"]))
+
+    def ppcg_init(self, x, y, h, theta, f):
+        self.lib.orc_ppcg_init(x, y, h, theta, _p(f["sd"]), _p(f["r"]))
+
+    def ppcg_inner_iteration(self, x, y, h, alpha, beta, f):
+        self.lib.orc_ppcg_inner_iteration(x, y, h, alpha, beta, _p(f["sd"]), _p(f["r"]),
+                                          _p(f["u"]), _p(f["kx"]), _p(f["ky"]))
 
 
 def new_fields(x, y, h=2):

@@ -89,8 +89,16 @@ class _RawDeck:
 def test_same_as_reference_host(name):
     """This project's host reproduces the reference's own host program
     (compiled unchanged against the same oracle backend): identical per-step
-    iteration counts and the same printed final temperature."""
+    iteration counts (CG or Jacobi; Chebyshev and PPCG after the switch, and
+    the Chebyshev estimate), the same printed eigenvalue bounds and the same
+    printed final temperature."""
     want = _refhost()[name]
     rec, _ = _run(_RawDeck(want["deck"]))
     assert [s["iterations"] for s in rec["steps"]] == want["iterations"]
+    after = [[s["solver_iterations"], s["estimated"]] for s in rec["steps"]
+             if s["solver_iterations"] >= 0]
+    assert after == want["after_switch"]
+    eig = [["%.12e" % s["eigmin"], "%.12e" % s["eigmax"]] for s in rec["steps"]
+           if s["solver_iterations"] > 0]
+    assert eig == want["eigenvalues"]
     assert "%.15e" % rec["summaries"][-1]["temp"] == want["final_temp_printed"]
