@@ -62,6 +62,9 @@ def parse_args():
                     help="grid of the time-to-solution comparison (default: --cells)")
     ap.add_argument("--tts-max-iters", type=int, default=12000,
                     help="iteration cap of the time-to-solution solves")
+    ap.add_argument("--solvers", type=int, default=1,
+                    help="time CG, Chebyshev, PPCG and Jacobi on the known-answer "
+                         "deck (0 skips it)")
     return ap.parse_args()
 
 
@@ -184,6 +187,60 @@ def time_to_solution(n, max_iters=12000, timeout=1200):
     return out
 
 
+# Algorithmic bytes per interior cell of each solver's streaming pass.
+SOLVER_KERNELS = {"cg": ("cg_fused_w", BYTES_FUSED_W),
+                  "chebyshev": ("cheby_fused", 72),
+                  "ppcg": ("ppcg_fused", 64),
+                  "jacobi": ("jacobi_fused", 40)}
+
+
+def other_solvers(n, jacobi_iters=2000, timeout=900):
+    """The reference's solvers on its own known-answer problem (tea.problems
+    row n n 1: tea.in states, K = density, eps 1e-15 on the solver's own
+    test): time to solution after one warm-up step, iteration counts, the
+    known-answer check, and the average duration and bandwidth of each
+    solver's streaming pass (HIP events around that kernel only).  Jacobi
+    does not converge within max_iters at this size; it is capped and timed
+    per sweep."""
+    from exploringsycl_amd.deck import Deck
+    from exploringsycl_amd.native import TEALEAF_BIN
+    from exploringsycl_amd.runner import run_deck
+
+    out = {"workload": (f"{n}x{n} cells, tea.in states, coefficient_density, 1 "
+                        "timestep (the tea.problems known-answer deck), eps 1e-15, "
+                        f"Jacobi capped at {jacobi_iters} sweeps, after 1 warm-up step"),
+           "solvers": {}}
+    extras = {"cg": [], "chebyshev": ["use_chebyshev"], "ppcg": ["use_ppcg"],
+              "jacobi": ["use_jacobi", f"max_iters {jacobi_iters}"]}
+    for name, extra in extras.items():
+        kname, kbytes = SOLVER_KERNELS[name]
+        with tempfile.TemporaryDirectory(prefix="tea_solver_") as wd:
+            stats_path = os.path.join(wd, "kernel_stats.json")
+            rec, _ = run_deck(Deck(n, n, end_step=1, extra=extra), binary=TEALEAF_BIN,
+                              workdir=wd, timeout=timeout,
+                              env={"TEA_HIP_STATS": stats_path,
+                                   "TEA_HIP_STATS_ONLY": kname},
+                              extra_args=["--warmup-steps", "1"])
+            with open(stats_path) as fp:
+                k = json.load(fp)[kname]
+        st = rec["steps"][-1]
+        avg_ms = k["total_ms"] / k["launches"]
+        gbs = kbytes * n * n / (avg_ms * 1e-3) / 1e9
+        out["solvers"][name] = {
+            "seconds": round(rec["timed_seconds"], 4),
+            "iterations": st["iterations"],
+            "after_switch": st["solver_iterations"],
+            "known_answer_passed": rec["check_passed"] if rec["check_found"] else None,
+            "temp": rec["summaries"][-1]["temp"],
+            "kernel": {"name": kname, "launches": k["launches"],
+                       "avg_us": round(avg_ms * 1e3, 2),
+                       "algorithmic_bytes_per_cell": kbytes,
+                       "achieved_gbs": round(gbs, 1),
+                       "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)},
+        }
+    return out
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -267,6 +324,8 @@ def main():
         sv = tts["solvers"]
         line["learned_fcg_vs_plain_cg_speedup"] = (
             sv["learned_fcg"]["speedup_vs_plain_cg"] if "learned_fcg" in sv else None)
+    if args.solvers:
+        line["other_solvers"] = other_solvers(n)
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
         line["cpu_baseline"]["cpu_model"] = cpu_model()
@@ -275,6 +334,8 @@ def main():
                              "method": "16-byte nontemporal device copy of 1 GiB, "
                                        "read + write bytes / time"}
     roofline["frac_of_copy_peak"] = round(roofline["achieved"] / peak, 4)
+    for rec in line.get("other_solvers", {}).get("solvers", {}).values():
+        rec["kernel"]["frac_of_copy_peak"] = round(rec["kernel"]["achieved_gbs"] / peak, 4)
     print(json.dumps(line), flush=True)
 
 

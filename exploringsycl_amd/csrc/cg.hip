@@ -604,6 +604,16 @@ struct NormBody {
   }
 };
 
+// The 2-norm over cell pairs, one pair per lane (even rows, more than one
+// group): 8 bytes per cell, the sum tree's first level in-lane.
+__global__ __launch_bounds__(kBlock) void norm_pair_kernel(
+    Geo g, const double* __restrict__ b, double* sub, int nblk) {
+  const PairAt q = pair_at(g, blockIdx.x * kPairBlock + 2 * threadIdx.x);
+  const double2 v = ld2_nt(b, q.ic);
+  double s[1] = {(q.in0 ? v.x * v.x : 0.0) + (q.in1 ? v.y * v.y : 0.0)};
+  wave_subtree_pairs<1>(s, sub, nblk, blockIdx.x);
+}
+
 struct FinaliseBody {
   const double* __restrict__ u;
   const double* __restrict__ density;
@@ -848,7 +858,17 @@ int tea_hip_calculate_2norm(tea_hip_chunk* c, const double* buffer,
     known |= buffer == c->field[fid] &&
              c->field_len[fid] >= (size_t)c->geo.n;
   TEA_CHECK_ARG(known, "buffer is not an x*y field of this chunk");
-  return launch_span1(c, K_CALC_2NORM, NormBody{buffer}, norm);
+  if (!use_pair_blocks(c, 1))
+    return launch_span1(c, K_CALC_2NORM, NormBody{buffer}, norm);
+  {
+    Timed t(c, K_CALC_2NORM);
+    hipLaunchKernelGGL(norm_pair_kernel, pair_grid(c), dim3(kBlock), 0,
+                       c->stream, c->geo, buffer, c->sub, c->nblk);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  int len = c->nspan;
+  TEA_RETURN_IF(launch_span_finish(c, 1, &len));
+  return finish_reduction_n(c, 1, len, norm);
 }
 
 int tea_hip_finalise(tea_hip_chunk* c) {

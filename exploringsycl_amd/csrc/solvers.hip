@@ -194,27 +194,6 @@ struct PpcgCalcSDBody {  // ppcg.cpp:69-94
 // pair never straddles a row).  The stencil operand's left and right
 // neighbours come from the adjacent lanes; lanes 0 and 63 load theirs.
 
-struct PairAt {
-  int e;          // first cell
-  bool valid;     // e < x*y
-  bool in0, in1;  // interior cells
-  int ic;         // e if valid, else a safe even index (x + 2)
-  int ir;         // e if either cell is interior, else x + 2
-};
-
-__device__ inline PairAt pair_at(const Geo& g, int e) {
-  PairAt q;
-  q.e = e;
-  q.valid = e < g.n;
-  const int row = q.valid ? (int)g.dx.div((uint32_t)e) : 0;
-  const int kk = e - row * g.x;
-  q.in0 = q.valid && interior(g, kk, row);
-  q.in1 = q.valid && interior(g, kk + 1, row);
-  q.ic = q.valid ? e : g.x + 2;
-  q.ir = (q.in0 || q.in1) ? e : g.x + 2;
-  return q;
-}
-
 // A stencil operand at a pair: the pair itself, the rows above and below,
 // and the cells left of e and right of e + 1.  Must be called by every lane
 // of the wave (it shuffles).  Lane 0's and lane 63's extra loads are at

@@ -308,6 +308,29 @@ __device__ inline void wave_subtree_pairs(double (&v)[NR], double* sub,
   }
 }
 
+// The cell pair (e, e + 1) of a pair-per-thread lane, e = 512 b + 2 m for
+// lane m of block b (x even: a pair never straddles a row).
+struct PairAt {
+  int e;          // first cell
+  bool valid;     // e < x*y
+  bool in0, in1;  // interior cells
+  int ic;         // e if valid, else a safe even index (x + 2)
+  int ir;         // e if either cell is interior, else x + 2
+};
+
+__device__ inline PairAt pair_at(const Geo& g, int e) {
+  PairAt q;
+  q.e = e;
+  q.valid = e < g.n;
+  const int row = q.valid ? (int)g.dx.div((uint32_t)e) : 0;
+  const int kk = e - row * g.x;
+  q.in0 = q.valid && interior(g, kk, row);
+  q.in1 = q.valid && interior(g, kk + 1, row);
+  q.ic = q.valid ? e : g.x + 2;
+  q.ir = (q.in0 || q.in1) ? e : g.x + 2;
+  return q;
+}
+
 // One block per span: each thread evaluates `body` on its 16 cells (only
 // interior cells; everything else contributes +0.0, as in the reference) and
 // the block stores the span's NR level-2 partials.

@@ -16,6 +16,10 @@ void initialise_application(Chunk** chunks, Settings* settings) {
   State* states = NULL;
   read_config(settings, &states);
   g_states = states;
+  // One chunk per process (one GPU): halo exchange between chunks is not
+  // part of this build, and the backend assigns the scalars it returns.
+  if (settings->num_chunks_per_rank != 1)
+    die(__LINE__, __FILE__, "num_chunks_per_rank must be 1 (one chunk per GPU)\n");
 
   *chunks = (Chunk*)calloc(settings->num_chunks_per_rank, sizeof(Chunk));
   decompose_field(settings, *chunks);
