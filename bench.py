@@ -65,6 +65,8 @@ def parse_args():
     ap.add_argument("--solvers", type=int, default=1,
                     help="time CG, Chebyshev, PPCG and Jacobi on the known-answer "
                          "deck (0 skips it)")
+    ap.add_argument("--solver-max-iters", type=int, default=0,
+                    help="cap on every solve of that comparison (0: none; for profiling)")
     return ap.parse_args()
 
 
@@ -194,14 +196,14 @@ SOLVER_KERNELS = {"cg": ("cg_fused_w", BYTES_FUSED_W),
                   "jacobi": ("jacobi_fused", 40)}
 
 
-def other_solvers(n, jacobi_iters=2000, timeout=900):
+def other_solvers(n, jacobi_iters=2000, max_iters=0, timeout=900):
     """The reference's solvers on its own known-answer problem (tea.problems
     row n n 1: tea.in states, K = density, eps 1e-15 on the solver's own
     test): time to solution after one warm-up step, iteration counts, the
     known-answer check, and the average duration and bandwidth of each
     solver's streaming pass (HIP events around that kernel only).  Jacobi
     does not converge within max_iters at this size; it is capped and timed
-    per sweep."""
+    per sweep.  max_iters > 0 caps every solve (for profiling runs)."""
     from exploringsycl_amd.deck import Deck
     from exploringsycl_amd.native import TEALEAF_BIN
     from exploringsycl_amd.runner import run_deck
@@ -210,8 +212,9 @@ def other_solvers(n, jacobi_iters=2000, timeout=900):
                         "timestep (the tea.problems known-answer deck), eps 1e-15, "
                         f"Jacobi capped at {jacobi_iters} sweeps, after 1 warm-up step"),
            "solvers": {}}
-    extras = {"cg": [], "chebyshev": ["use_chebyshev"], "ppcg": ["use_ppcg"],
-              "jacobi": ["use_jacobi", f"max_iters {jacobi_iters}"]}
+    cap = [f"max_iters {max_iters}"] if max_iters > 0 else []
+    extras = {"cg": cap, "chebyshev": ["use_chebyshev"] + cap, "ppcg": ["use_ppcg"] + cap,
+              "jacobi": ["use_jacobi", f"max_iters {max_iters or jacobi_iters}"]}
     for name, extra in extras.items():
         kname, kbytes = SOLVER_KERNELS[name]
         with tempfile.TemporaryDirectory(prefix="tea_solver_") as wd:
@@ -325,7 +328,7 @@ def main():
         line["learned_fcg_vs_plain_cg_speedup"] = (
             sv["learned_fcg"]["speedup_vs_plain_cg"] if "learned_fcg" in sv else None)
     if args.solvers:
-        line["other_solvers"] = other_solvers(n)
+        line["other_solvers"] = other_solvers(n, max_iters=args.solver_max_iters)
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
         line["cpu_baseline"]["cpu_model"] = cpu_model()
