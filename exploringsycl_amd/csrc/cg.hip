@@ -661,6 +661,7 @@ int sync_p_alt(tea_hip_chunk* c) {
 }  // namespace
 
 int apply_pending(tea_hip_chunk* c) {
+  if (c->ppcg_pend) TEA_RETURN_IF(flush_ppcg(c));
   if (!c->pend_u && !c->pend_p && !c->pend_halo) return TEA_HIP_OK;
   double** f = c->field;
   if (c->pend_u || c->pend_p) {
@@ -850,20 +851,21 @@ int tea_hip_calculate_residual(tea_hip_chunk* c) {
 
 int tea_hip_calculate_2norm(tea_hip_chunk* c, const double* buffer,
                             double* norm) {
-  TEA_ENTER(c);
+  TEA_ENTER_HELD(c);
   TEA_CHECK_ARG(norm != nullptr, "null norm");
+  int fid = -1;
+  for (int f = 0; f < TEA_NUM_FIELDS; ++f)
+    if (buffer == c->field[f] && c->field_len[f] >= (size_t)c->geo.n) fid = f;
+  TEA_CHECK_ARG(fid >= 0, "buffer is not an x*y field of this chunk");
+  // Deferred work can move the field to its second buffer: name it by field.
   TEA_RETURN_IF(apply_pending(c));
-  bool known = false;
-  for (int fid = 0; fid < TEA_NUM_FIELDS; ++fid)
-    known |= buffer == c->field[fid] &&
-             c->field_len[fid] >= (size_t)c->geo.n;
-  TEA_CHECK_ARG(known, "buffer is not an x*y field of this chunk");
+  const double* b = c->field[fid];
   if (!use_pair_blocks(c, 1))
-    return launch_span1(c, K_CALC_2NORM, NormBody{buffer}, norm);
+    return launch_span1(c, K_CALC_2NORM, NormBody{b}, norm);
   {
     Timed t(c, K_CALC_2NORM);
     hipLaunchKernelGGL(norm_pair_kernel, pair_grid(c), dim3(kBlock), 0,
-                       c->stream, c->geo, buffer, c->sub, c->nblk);
+                       c->stream, c->geo, b, c->sub, c->nblk);
   }
   TEA_HIP_TRY(hipGetLastError());
   int len = c->nspan;

@@ -29,7 +29,7 @@ const char* const kKernelNames[K_NUM] = {
     "span_finish",    "jacobi_init",     "jacobi_copy_u", "jacobi_iterate",
     "jacobi_fused",   "cheby_init",      "cheby_iterate", "cheby_calc_u",
     "cheby_fused",    "ppcg_init",       "ppcg_calc_ur",  "ppcg_calc_sd",
-    "ppcg_fused"};
+    "ppcg_fused",     "ppcg_two_step"};
 
 Timed::Timed(tea_hip_chunk* c_, int kid_, hipStream_t s)
     : c(c_), kid(kid_), stream(s ? s : c_->stream) {
@@ -323,6 +323,7 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
     if (c->field[f]) (void)hipFree(c->field[f]);
   if (c->p_alt) (void)hipFree(c->p_alt);
   if (c->spare) (void)hipFree(c->spare);
+  if (c->spare2) (void)hipFree(c->spare2);
   if (c->sub) (void)hipFree(c->sub);
   for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky, c->fcg_z})
     if (b) (void)hipFree(b);

@@ -169,6 +169,7 @@ int tea_hip_set_chunk_data(tea_hip_chunk* c, double x_min, double y_min,
                            double dx, double dy) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_FLUSH_PPCG(c);
   TEA_RETURN_IF(apply_pending(c));
   const Geo g = c->geo;
   double** f = c->field;
@@ -201,6 +202,7 @@ int tea_hip_set_chunk_state(tea_hip_chunk* c, int num_states,
                                     radius && geometry),
                 "null state geometry arrays");
   TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_FLUSH_PPCG(c);
   TEA_RETURN_IF(apply_pending(c));
   const Geo g = c->geo;
   double** f = c->field;
@@ -226,6 +228,7 @@ int tea_hip_set_chunk_state(tea_hip_chunk* c, int num_states,
 int tea_hip_store_energy(tea_hip_chunk* c) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_FLUSH_PPCG(c);
   TEA_RETURN_IF(apply_pending(c));
   {
     Timed t(c, K_STORE_ENERGY);
@@ -282,15 +285,27 @@ int tea_hip_local_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(neighbours && fields_to_exchange, "null arrays");
   TEA_HIP_TRY(hipSetDevice(c->device));
+  bool all_external = true;
+  for (int f = 0; f < 4; ++f) all_external &= neighbours[f] < 0;
+  // The depth-1 SD update between two PPCG inner steps is folded into the
+  // pass that runs both steps (the held-back first step is flushed instead
+  // if anything else comes between them).
+  if (c->ppcg_pend && !c->ppcg_pend_halo && depth == 1 && all_external) {
+    bool only_sd = true;
+    for (int s = 0; s < TEA_HALO_NUM; ++s)
+      only_sd &= (fields_to_exchange[s] != 0) == (s == TEA_HALO_SD);
+    if (only_sd) {
+      c->ppcg_pend_halo = true;
+      return TEA_HIP_OK;
+    }
+  }
   // The depth-1 U/P update that follows cg_calc_p is folded into the next
   // fused cg_calc_w when all faces are external.
   if (c->fused && c->pend_u && c->pend_p && !c->pend_halo && depth == 1) {
     bool only_up = true;
     for (int s = 0; s < TEA_HALO_NUM; ++s)
       only_up &= (fields_to_exchange[s] != 0) == (s == TEA_HALO_U || s == TEA_HALO_P);
-    bool all_ext = true;
-    for (int f = 0; f < 4; ++f) all_ext &= neighbours[f] < 0;
-    if (only_up && all_ext) {
+    if (only_up && all_external) {
       c->pend_halo = true;
       return TEA_HIP_OK;
     }
@@ -304,6 +319,7 @@ int tea_hip_field_summary(tea_hip_chunk* c, double* vol, double* mass,
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(vol && mass && ie && temp, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_FLUSH_PPCG(c);
   TEA_RETURN_IF(apply_pending(c));
   double** f = c->field;
   SummaryBody body{f[TEA_FIELD_VOLUME], f[TEA_FIELD_DENSITY],

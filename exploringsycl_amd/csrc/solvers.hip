@@ -3,7 +3,8 @@
 // TeaLeaf/c_kernels/sycl/kernel_interface.cpp:246-328 composes them).
 //
 // Each iteration entry point is one streaming pass in the default (fused)
-// mode, with one cell pair per lane as in the fused CG kernels:
+// mode, with one cell pair per lane as in the fused CG kernels (and PPCG
+// inner steps two to a pass, see ppcg_two_step_kernel):
 //   Jacobi     reads u (5-point), u0, kx, ky; writes the new u into r's
 //              buffer, the two fields' addresses are then exchanged, so the
 //              old u is "r = u" without a copy; reduces sum |u_new - u|.
@@ -340,7 +341,272 @@ int ensure_spare(tea_hip_chunk* c) {
 // Fused passes run on even row lengths in the default mode.
 bool fused_pairs(const tea_hip_chunk* c) { return c->fused && c->pairs; }
 
+// ---- two PPCG inner steps in one pass ---------------------------------------
+//
+// Steps k and k+1 (ppcg.cpp:34-94) with the depth-1 reflective SD update
+// between them (local_halos.cpp:7-102).  A block owns a tile of kTileY rows
+// x kTileX columns of the interior, one column per thread:
+//   1. sd_k on the tile and two cells around it is staged in LDS;
+//   2. sd_(k+1) = a0 sd_k + b0 (r_k - A sd_k) is formed on the tile and one
+//      cell around it (the tile's r_(k+1) stays in registers);
+//   3. cells of that margin on the halo ring take their mirror's value, as
+//      the halo update gives them;
+//   4. step k+1 for the tile's cells: u in place, r and sd_(k+2) into second
+//      buffers (neighbouring tiles still read the old sd and r);
+//   5. tiles on the edge of the interior write the second buffers' cells
+//      outside it: for sd what the halo update gives, r copied.
+// sd_k and r_k are read once (the margins from L2), u, r and sd written once:
+// 64 bytes per cell for two steps against 128 for two single passes.  Every
+// value uses the single-step kernel's expressions, so the result is bitwise
+// that of two single steps with the halo update between them.
+constexpr int kTileX = kBlock;
+constexpr int kTileY = 4;
+constexpr int kS0W = kTileX + 4, kS0H = kTileY + 4;
+constexpr int kS1W = kTileX + 2, kS1H = kTileY + 2;
+
+__global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
+    Geo g, const double* __restrict__ sd_old, double* __restrict__ sd_new,
+    const double* __restrict__ r, double* __restrict__ r_new,
+    double* __restrict__ u, const double* __restrict__ kx,
+    const double* __restrict__ ky, double a0, double b0, double a1, double b1,
+    int tiles_x, int tiles_y) {
+  // sd_k (rows r0-2.., columns c0-2..) until step 2 is formed, then, in the
+  // same storage, sd_(k+1) (rows r0-1.., columns c0-1..).
+  __shared__ double lds[kS0H * kS0W];
+  double (*S0)[kS0W] = reinterpret_cast<double (*)[kS0W]>(lds);
+  double (*S1)[kS1W] = reinterpret_cast<double (*)[kS1W]>(lds);
+  const int x = g.x, y = g.y, h = g.h;
+  const int xe = x - h, ye = y - h;  // ends of the interior (exclusive)
+  const int txi = blockIdx.x % tiles_x, tyi = blockIdx.x / tiles_x;
+  const int c0 = h + txi * kTileX, r0 = h + tyi * kTileY;
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const bool edge = txi == 0 || tyi == 0 || txi == tiles_x - 1 || tyi == tiles_y - 1;
+
+  // 1. All global reads of the tile's own column (column c0 + t) and of
+  // sd_k for the staging area are issued before the block waits once: r,
+  // kx (through the right boundary face; lane 63 also reads its right
+  // neighbour's) and ky (cells and the row above).
+  const int kk = c0 + t;
+  double rk1[kTileY], kx0[kTileY], kxe[kTileY], kyv[kTileY + 1];
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) rk1[m] = kx0[m] = kxe[m] = kyv[m] = 0.0;
+  kyv[kTileY] = 0.0;
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) {
+    const int jj = r0 + m;
+    if (jj < ye && kk <= xe) kx0[m] = kx[kk + jj * x];
+    if (kk < xe && jj < ye) {
+      const int i = kk + jj * x;
+      rk1[m] = r[i];
+      if (lane == kWave - 1) kxe[m] = kx[i + 1];
+      kyv[m] = ky[i];
+      kyv[m + 1] = ky[i + x];
+    }
+  }
+  // Stage sd_k (cells outside the field are never used: zero).
+  constexpr int kStage = (kS0H * kS0W + kBlock - 1) / kBlock;
+  double st[kStage];
+#pragma unroll
+  for (int n = 0; n < kStage; ++n) {
+    const int q = t + n * kBlock;
+    const int a = q / kS0W, b = q - a * kS0W;
+    const int jj = r0 - 2 + a, kc = c0 - 2 + b;
+    st[n] = (q < kS0H * kS0W && jj >= 0 && jj < y && kc >= 0 && kc < x)
+                ? sd_old[kc + jj * x]
+                : 0.0;
+  }
+#pragma unroll
+  for (int n = 0; n < kStage; ++n) {
+    const int q = t + n * kBlock;
+    if (q < kS0H * kS0W) lds[q] = st[n];
+  }
+  __syncthreads();
+
+  // 2. sd_(k+1) on the tile's own cells (rk1 becomes r_(k+1)) ...
+  double sdk[kTileY], s1own[kTileY];
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) {
+    const int jj = r0 + m;
+    double kx1 = __shfl_down(kx0[m], 1);
+    if (lane == kWave - 1) kx1 = kxe[m];
+    sdk[m] = s1own[m] = 0.0;
+    if (kk < xe && jj < ye) {
+      const double c = S0[m + 2][t + 2];
+      const double rn = rk1[m] - op5(c, S0[m + 2][t + 1], S0[m + 2][t + 3],
+                                     S0[m + 1][t + 2], S0[m + 3][t + 2], kx0[m],
+                                     kx1, kyv[m], kyv[m + 1]);
+      sdk[m] = c;
+      rk1[m] = rn;
+      s1own[m] = a0 * c + b0 * rn;
+    }
+  }
+  // ... and on the one-cell margin: rows -1 and kTileY over columns
+  // -1 .. kTileX, then columns -1 and kTileX over the tile's rows.
+  constexpr int kMargin = 2 * kS1W + 2 * kTileY;
+  constexpr int kMarginPer = (kMargin + kBlock - 1) / kBlock;
+  double s1m[kMarginPer];
+  int s1pos[kMarginPer];
+#pragma unroll
+  for (int n = 0; n < kMarginPer; ++n) {
+    const int q = t + n * kBlock;
+    int a = 0, b = 0;
+    if (q < 2 * kS1W) {
+      a = q < kS1W ? 0 : kS1H - 1;
+      b = q < kS1W ? q : q - kS1W;
+    } else {
+      const int e = q - 2 * kS1W;
+      a = 1 + (e >> 1);
+      b = (e & 1) ? kS1W - 1 : 0;
+    }
+    const int jj = r0 - 1 + a, kc = c0 - 1 + b;
+    double s1 = 0.0;
+    if (q < kMargin && jj >= h && jj < ye && kc >= h && kc < xe) {
+      const int i = kc + jj * x;
+      const double c = S0[a + 1][b + 1];
+      const double rn = r[i] - op5(c, S0[a + 1][b], S0[a + 1][b + 2], S0[a][b + 1],
+                                   S0[a + 2][b + 1], kx[i], kx[i + 1], ky[i],
+                                   ky[i + x]);
+      s1 = a0 * c + b0 * rn;
+    }
+    s1m[n] = s1;
+    s1pos[n] = q < kMargin ? a * kS1W + b : -1;
+  }
+  __syncthreads();  // every read of sd_k is done: S1 replaces it
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) S1[m + 1][t + 1] = s1own[m];
+#pragma unroll
+  for (int n = 0; n < kMarginPer; ++n)
+    if (s1pos[n] >= 0) lds[s1pos[n]] = s1m[n];
+  __syncthreads();
+
+  // 3. Ring cells of the margin take their mirror's value (double mirror at
+  // the corners, as the halo update's left/right-then-bottom/top order gives).
+  if (edge) {
+    for (int q = t; q < kS1H * kS1W; q += kBlock) {
+      const int a = q / kS1W, b = q - a * kS1W;
+      const int jj = r0 - 1 + a, kc = c0 - 1 + b;
+      const bool ring_box = jj >= h - 1 && jj <= ye && kc >= h - 1 && kc <= xe;
+      const bool inner = jj >= h && jj < ye && kc >= h && kc < xe;
+      if (!ring_box || inner) continue;
+      const int mj = jj == h - 1 ? h : (jj == ye ? ye - 1 : jj);
+      const int mk = kc == h - 1 ? h : (kc == xe ? xe - 1 : kc);
+      S1[a][b] = S1[mj - r0 + 1][mk - c0 + 1];
+    }
+    __syncthreads();
+  }
+
+  // 4. Step k+1 on the tile's own cells.
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) {
+    const int jj = r0 + m;
+    double kx1 = __shfl_down(kx0[m], 1);
+    if (lane == kWave - 1) kx1 = kxe[m];
+    if (kk < xe && jj < ye) {
+      const int i = kk + jj * x;
+      const double c = S1[m + 1][t + 1];
+      const double rn = rk1[m] - op5(c, S1[m + 1][t], S1[m + 1][t + 2], S1[m][t + 1],
+                                     S1[m + 2][t + 1], kx0[m], kx1, kyv[m],
+                                     kyv[m + 1]);
+      r_new[i] = rn;
+      u[i] = (u[i] + sdk[m]) + c;
+      sd_new[i] = a1 * c + b1 * rn;
+    }
+  }
+
+  // 5. The second buffers' cells outside the interior next to this tile.
+  // The halo update copies whole columns h-1 and x-h from their inner
+  // neighbours, then whole rows y-h and h-1: a cell takes the value at its
+  // mirror (each coordinate on the ring moved one cell inwards), from
+  // sd_(k+1) where that is interior, else the unchanged old value.  r
+  // outside the interior is copied.
+  if (edge) {
+    const int ka = txi == 0 ? 0 : c0;
+    const int kb = txi == tiles_x - 1 ? x : min(c0 + kTileX, xe);
+    const int ja = tyi == 0 ? 0 : r0;
+    const int jb = tyi == tiles_y - 1 ? y : min(r0 + kTileY, ye);
+    const int w = kb - ka;
+    for (int q = t; q < w * (jb - ja); q += kBlock) {
+      const int jj = ja + q / w, kc = ka + q % w;
+      if (jj >= h && jj < ye && kc >= h && kc < xe) continue;
+      const int mj = jj == h - 1 ? h : (jj == ye ? ye - 1 : jj);
+      const int mk = kc == h - 1 ? h : (kc == xe ? xe - 1 : kc);
+      const bool from_new = mj >= h && mj < ye && mk >= h && mk < xe;
+      sd_new[kc + jj * x] =
+          from_new ? S1[mj - r0 + 1][mk - c0 + 1] : sd_old[mk + mj * x];
+      r_new[kc + jj * x] = r[kc + jj * x];
+    }
+  }
+}
+
+// One inner step: the single pass (even rows) or one launch per reference
+// kernel; the new sd goes to the second buffer in the single pass.
+int ppcg_step(tea_hip_chunk* c, double alpha, double beta) {
+  double** f = c->field;
+  const Geo g = c->geo;
+  if (!fused_pairs(c)) {
+    TEA_RETURN_IF(launch_each_cell(
+        c, K_PPCG_CALC_UR,
+        PpcgCalcURBody{f[TEA_FIELD_SD], f[TEA_FIELD_R], f[TEA_FIELD_U],
+                       f[TEA_FIELD_KX], f[TEA_FIELD_KY], g}));
+    return launch_each_cell(
+        c, K_PPCG_CALC_SD,
+        PpcgCalcSDBody{f[TEA_FIELD_SD], f[TEA_FIELD_R], alpha, beta, g});
+  }
+  TEA_RETURN_IF(ensure_spare(c));
+  {
+    Timed t(c, K_PPCG_FUSED);
+    hipLaunchKernelGGL(ppcg_pair_kernel, pair_grid(c), dim3(kBlock), 0,
+                       c->stream, g, f[TEA_FIELD_SD], c->spare, f[TEA_FIELD_R],
+                       f[TEA_FIELD_U], f[TEA_FIELD_KX], f[TEA_FIELD_KY], alpha,
+                       beta);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  std::swap(f[TEA_FIELD_SD], c->spare);
+  return TEA_HIP_OK;
+}
+
+// Two inner steps with the SD halo update between them, in one pass.
+int ppcg_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
+                   double b1) {
+  double** f = c->field;
+  const Geo g = c->geo;
+  TEA_RETURN_IF(ensure_spare(c));
+  if (!c->spare2 &&
+      hipMalloc(&c->spare2, (size_t)g.n * sizeof(double)) != hipSuccess) {
+    c->spare2 = nullptr;
+    set_error("hipMalloc of the second spare field buffer failed");
+    return TEA_HIP_ENOMEM;
+  }
+  const int tiles_x = (g.x - 2 * g.h + kTileX - 1) / kTileX;
+  const int tiles_y = (g.y - 2 * g.h + kTileY - 1) / kTileY;
+  {
+    Timed t(c, K_PPCG_TWO_STEP);
+    hipLaunchKernelGGL(ppcg_two_step_kernel, dim3(tiles_x * tiles_y),
+                       dim3(kBlock), 0, c->stream, g, f[TEA_FIELD_SD], c->spare,
+                       f[TEA_FIELD_R], c->spare2, f[TEA_FIELD_U], f[TEA_FIELD_KX],
+                       f[TEA_FIELD_KY], a0, b0, a1, b1, tiles_x, tiles_y);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  std::swap(f[TEA_FIELD_SD], c->spare);
+  std::swap(f[TEA_FIELD_R], c->spare2);
+  return TEA_HIP_OK;
+}
+
 }  // namespace
+
+int flush_ppcg(tea_hip_chunk* c) {
+  if (!c->ppcg_pend) return TEA_HIP_OK;
+  const bool halo = c->ppcg_pend_halo;
+  c->ppcg_pend = c->ppcg_pend_halo = false;
+  TEA_RETURN_IF(ppcg_step(c, c->ppcg_alpha, c->ppcg_beta));
+  if (!halo) return TEA_HIP_OK;
+  static const int ext[4] = {-1, -1, -1, -1};
+  int fields[TEA_HALO_NUM] = {0};
+  fields[TEA_HALO_SD] = 1;
+  return launch_halos(c, 1, ext, fields);
+}
+
 }  // namespace tea
 
 using namespace tea;
@@ -447,30 +713,23 @@ int tea_hip_ppcg_init(tea_hip_chunk* c, double theta) {
 
 int tea_hip_ppcg_inner_iteration(tea_hip_chunk* c, double alpha,
                                  double beta) {
-  TEA_ENTER(c);
+  TEA_ENTER_HELD(c);
+  if (c->fused) {
+    // Held back until the next inner step: with the SD halo update between
+    // them (all faces external), both steps run as one pass.
+    if (c->ppcg_pend && c->ppcg_pend_halo) {
+      c->ppcg_pend = c->ppcg_pend_halo = false;
+      return ppcg_two_steps(c, c->ppcg_alpha, c->ppcg_beta, alpha, beta);
+    }
+    TEA_RETURN_IF(apply_pending(c));
+    c->ppcg_pend = true;
+    c->ppcg_pend_halo = false;
+    c->ppcg_alpha = alpha;
+    c->ppcg_beta = beta;
+    return TEA_HIP_OK;
+  }
   TEA_RETURN_IF(apply_pending(c));
-  double** f = c->field;
-  const Geo g = c->geo;
-  if (!fused_pairs(c)) {
-    TEA_RETURN_IF(launch_each_cell(
-        c, K_PPCG_CALC_UR,
-        PpcgCalcURBody{f[TEA_FIELD_SD], f[TEA_FIELD_R], f[TEA_FIELD_U],
-                       f[TEA_FIELD_KX], f[TEA_FIELD_KY], g}));
-    return launch_each_cell(
-        c, K_PPCG_CALC_SD,
-        PpcgCalcSDBody{f[TEA_FIELD_SD], f[TEA_FIELD_R], alpha, beta, g});
-  }
-  TEA_RETURN_IF(ensure_spare(c));
-  {
-    Timed t(c, K_PPCG_FUSED);
-    hipLaunchKernelGGL(ppcg_pair_kernel, pair_grid(c), dim3(kBlock), 0,
-                       c->stream, g, f[TEA_FIELD_SD], c->spare, f[TEA_FIELD_R],
-                       f[TEA_FIELD_U], f[TEA_FIELD_KX], f[TEA_FIELD_KY], alpha,
-                       beta);
-  }
-  TEA_HIP_TRY(hipGetLastError());
-  std::swap(f[TEA_FIELD_SD], c->spare);
-  return TEA_HIP_OK;
+  return ppcg_step(c, alpha, beta);
 }
 
 }  // extern "C"

@@ -94,20 +94,55 @@ def test_cheby(oracle, cells, mode):
 
 @pytest.mark.parametrize("cells", SIZES)
 @pytest.mark.parametrize("mode", MODES)
-def test_ppcg(oracle, cells, mode):
+@pytest.mark.parametrize("steps", [1, 2, 3, 10])
+def test_ppcg(oracle, cells, mode, steps):
+    """ppcg_init, then `steps` inner steps each after a depth-1 SD halo
+    update, as the PPCG driver calls them.  In the fused mode consecutive
+    steps run two to a pass; an odd last step runs alone when the fields are
+    read back."""
     x, y = _dims(cells)
     f = _random_fields(x, y, 24)
     names = ["sd", "r", "u", "kx", "ky"]
+    rng = np.random.default_rng(steps)
+    coefs = list(zip(rng.uniform(-0.5, 1.0, steps), rng.uniform(-0.5, 1.5, steps)))
     with _chunk(x, y, mode) as c:
         _upload(c, f, names)
         c.ppcg_init(2.5)
         oracle.ppcg_init(x, y, H, 2.5, f)
         _assert_fields(c, f, names)
-        for a, b in [(0.2, 0.9), (0.45, -0.3), (0.7, 0.11)]:
+        for a, b in coefs:
             c.local_halos(1, ["sd"])
             oracle.halo_field(x, y, H, 1, f["sd"])
             c.ppcg_inner_iteration(a, b)
             oracle.ppcg_inner_iteration(x, y, H, a, b, f)
+        _assert_fields(c, f, names)
+
+
+@pytest.mark.parametrize("cells", [(250, 250), (601, 599)])
+def test_ppcg_held_step_flushed_by_other_calls(oracle, cells):
+    """A held-back inner step (and its held-back SD halo update) is run
+    before any other entry point: a 2-norm of sd (named by its pre-step
+    address), another halo update, a second step without a halo between."""
+    x, y = _dims(cells)
+    f = _random_fields(x, y, 25)
+    names = ["sd", "r", "u", "kx", "ky"]
+    with _chunk(x, y, "fused") as c:
+        _upload(c, f, names)
+        c.ppcg_inner_iteration(0.3, 0.6)
+        oracle.ppcg_inner_iteration(x, y, H, 0.3, 0.6, f)
+        assert _same(c.calculate_2norm("sd"), oracle.calculate_2norm(x, y, H, f["sd"]))
+        c.ppcg_inner_iteration(0.4, 0.2)
+        oracle.ppcg_inner_iteration(x, y, H, 0.4, 0.2, f)
+        c.local_halos(1, ["sd", "u"])
+        oracle.halo_field(x, y, H, 1, f["sd"])
+        oracle.halo_field(x, y, H, 1, f["u"])
+        c.ppcg_inner_iteration(0.5, 0.1)
+        oracle.ppcg_inner_iteration(x, y, H, 0.5, 0.1, f)
+        c.ppcg_inner_iteration(0.6, 0.3)
+        oracle.ppcg_inner_iteration(x, y, H, 0.6, 0.3, f)
+        c.local_halos(1, ["sd"])
+        oracle.halo_field(x, y, H, 1, f["sd"])
+        assert _same(c.calculate_2norm("r"), oracle.calculate_2norm(x, y, H, f["r"]))
         _assert_fields(c, f, names)
 
 
@@ -125,9 +160,16 @@ def test_field_addresses_follow_the_exchange():
         c.calculate_2norm("u")
         c.calculate_2norm("r")
         sd_before, _ = c.field_info("sd")
-        c.ppcg_inner_iteration(0.5, 0.5)
+        r_before, _ = c.field_info("r")
+        c.ppcg_inner_iteration(0.5, 0.5)  # held back for a possible pair
+        c.calculate_2norm("sd")  # runs it; sd is still named by its old address
         assert c.field_info("sd")[0] != sd_before
-        c.calculate_2norm("sd")
+        c.local_halos(1, ["sd"])
+        c.ppcg_inner_iteration(0.5, 0.5)
+        c.local_halos(1, ["sd"])
+        c.ppcg_inner_iteration(0.5, 0.5)  # with the one before: one pass
+        c.calculate_2norm("r")
+        assert c.field_info("r")[0] != r_before
 
 
 # ---- whole solves ------------------------------------------------------------------
