@@ -352,13 +352,13 @@ __device__ inline double fused_pair_compute(const Geo& g, const FusedWArgs& a,
   const double beta = a.beta;
   const int i = s.e0;
   // Neighbours across the pair boundary, from the adjacent lanes.
-  double pl = __shfl_up(s.P0.y, 1), pr = __shfl_down(s.P0.x, 1);
+  double pl = from_prev_lane(s.P0.y), pr = from_next_lane(s.P0.x);
   double rl = 0.0, rr = 0.0;
   if (APPLY_P) {
-    rl = __shfl_up(s.R0.y, 1);
-    rr = __shfl_down(s.R0.x, 1);
+    rl = from_prev_lane(s.R0.y);
+    rr = from_next_lane(s.R0.x);
   }
-  double kxr = __shfl_down(s.KX.x, 1);
+  double kxr = from_next_lane(s.KX.x);
   if (lane == 0) {
     pl = s.PN;
     if (APPLY_P) rl = s.RN;

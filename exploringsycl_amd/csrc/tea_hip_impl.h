@@ -100,6 +100,34 @@ __device__ inline int xcd_span(int b, int nb) {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// Cross-lane moves on the VALU's data-parallel-primitive (DPP) path instead
+// of LDS permutes.  CTRL is a DPP control: 0x130 wave_shl:1 (lane l takes
+// lane l+1), 0x138 wave_shr:1 (lane l takes lane l-1), 0xB1 and 0x4E
+// quad_perm [1,0,3,2] and [2,3,0,1] (lane l takes lane l^1, l^2), 0x141
+// row_half_mirror (lane i of each 8 takes lane 7-i), 0x140 row_mirror (lane
+// i of each 16 takes lane 15-i).  A lane without a source keeps its value.
+template <int CTRL>
+__device__ inline float dpp(float v) {
+  const int b = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline double dpp(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false));
+}
+template <class T>
+__device__ inline T from_next_lane(T v) { return dpp<0x130>(v); }
+template <class T>
+__device__ inline T from_prev_lane(T v) { return dpp<0x138>(v); }
+
+// The value of lane `l` (a constant), in every lane.
+__device__ inline double lane_value(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
 // Pairwise tree of 64 lanes (lane l <- lane l^m for m = 1..32).  After the
 // step with mask m every lane of an aligned block of 2m lanes holds the same
 // value, the tree sum of that block, so the result equals the reference's
@@ -300,11 +328,18 @@ __device__ inline void wave_subtree_pairs(double (&v)[NR], double* sub,
   const int wv = threadIdx.x / kWave;
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
+    // Tree steps 2, 4, 8 and 16 on the DPP path: after each, every lane of
+    // an aligned block holds the block's sum (the mirrors pair whole blocks
+    // of equal lanes).  Then the four 16-lane sums: (r0 + r1) and (r2 + r3)
+    // are the wave's two groups.
     double t = v[q];
-#pragma unroll
-    for (int m = 1; m < 32; m <<= 1) t = t + __shfl_xor(t, m);
-    const double upper = __shfl_down(t, 32);
-    if (lane == 0) sub[(size_t)q * kWaves * nblk + kWaves * blk + wv] = t + upper;
+    t = t + dpp<0xB1>(t);
+    t = t + dpp<0x4E>(t);
+    t = t + dpp<0x141>(t);
+    t = t + dpp<0x140>(t);
+    const double g0 = lane_value(t, 0) + lane_value(t, 16);
+    const double g1 = lane_value(t, 32) + lane_value(t, 48);
+    if (lane == 0) sub[(size_t)q * kWaves * nblk + kWaves * blk + wv] = g0 + g1;
   }
 }
 
