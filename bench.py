@@ -191,7 +191,7 @@ def time_to_solution(n, max_iters=12000, timeout=1200):
 
 # Algorithmic bytes per interior cell of each solver's streaming pass.
 SOLVER_KERNELS = {"cg": ("cg_fused_w", BYTES_FUSED_W),
-                  "chebyshev": ("cheby_fused", 72),
+                  "chebyshev": ("cheby_two_step", 72),  # two steps per launch
                   "ppcg": ("ppcg_two_step", 64),   # two inner steps per launch
                   "jacobi": ("jacobi_fused", 40)}
 

@@ -661,7 +661,7 @@ int sync_p_alt(tea_hip_chunk* c) {
 }  // namespace
 
 int apply_pending(tea_hip_chunk* c) {
-  if (c->ppcg_pend) TEA_RETURN_IF(flush_ppcg(c));
+  if (c->held) TEA_RETURN_IF(flush_held(c));
   if (!c->pend_u && !c->pend_p && !c->pend_halo) return TEA_HIP_OK;
   double** f = c->field;
   if (c->pend_u || c->pend_p) {

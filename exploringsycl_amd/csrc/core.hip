@@ -29,7 +29,7 @@ const char* const kKernelNames[K_NUM] = {
     "span_finish",    "jacobi_init",     "jacobi_copy_u", "jacobi_iterate",
     "jacobi_fused",   "cheby_init",      "cheby_iterate", "cheby_calc_u",
     "cheby_fused",    "ppcg_init",       "ppcg_calc_ur",  "ppcg_calc_sd",
-    "ppcg_fused",     "ppcg_two_step"};
+    "ppcg_fused",     "ppcg_two_step",   "cheby_two_step"};
 
 Timed::Timed(tea_hip_chunk* c_, int kid_, hipStream_t s)
     : c(c_), kid(kid_), stream(s ? s : c_->stream) {

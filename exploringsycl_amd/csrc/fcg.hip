@@ -731,7 +731,7 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
                        const float* feature_norm, int coarsen, int levels) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(ensure_fcg_buffers(c));
   for (FcgLevel& L : c->fcg_lv) free_level(L);
   if (c->fcg_cross) (void)hipFree(c->fcg_cross);
@@ -828,7 +828,7 @@ int tea_hip_fcg_precondition(tea_hip_chunk* c, int use_net, double* zw) {
   TEA_CHECK_ARG(zw != nullptr, "null zw");
   TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no network attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   int len;
   TEA_RETURN_IF(launch_precondition(c, use_net, 0, &len));
   return finish_reduction_n(c, 1, len, zw);
@@ -841,7 +841,7 @@ int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
   TEA_CHECK_ARG(rr != nullptr && zw != nullptr, "null outputs");
   TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no network attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   int lens[2] = {c->nspan, c->nspan};
   TEA_RETURN_IF(launch_mp_r(c, alpha, &lens[0], false));
   TEA_RETURN_IF(launch_precondition(c, use_net, 1, &lens[1]));
@@ -858,7 +858,7 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
   TEA_CHECK_ARG(c->fcg_z != nullptr && c->mp_r != nullptr, "FCG not initialised");
   TEA_CHECK_ARG(pw && pr, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   // p = z + beta p_old (the mixed-precision kernel with z in place of r),
   // w = A p, and the dots p.w and p.r.
   MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->fcg_z,
@@ -897,7 +897,7 @@ int tea_hip_fcg_read_z(tea_hip_chunk* c, float* host, size_t count) {
   TEA_CHECK_ARG(c != nullptr && c->fcg_z != nullptr, "FCG not initialised");
   TEA_CHECK_ARG(count <= (size_t)c->geo.n, "count exceeds the field length");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   TEA_HIP_TRY(hipMemcpyAsync(host, c->fcg_z, count * sizeof(float), hipMemcpyDeviceToHost,
                              c->stream));
   return sync_chunk(c);

@@ -169,7 +169,7 @@ int tea_hip_set_chunk_data(tea_hip_chunk* c, double x_min, double y_min,
                            double dx, double dy) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(apply_pending(c));
   const Geo g = c->geo;
   double** f = c->field;
@@ -202,7 +202,7 @@ int tea_hip_set_chunk_state(tea_hip_chunk* c, int num_states,
                                     radius && geometry),
                 "null state geometry arrays");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(apply_pending(c));
   const Geo g = c->geo;
   double** f = c->field;
@@ -228,7 +228,7 @@ int tea_hip_set_chunk_state(tea_hip_chunk* c, int num_states,
 int tea_hip_store_energy(tea_hip_chunk* c) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(apply_pending(c));
   {
     Timed t(c, K_STORE_ENERGY);
@@ -287,15 +287,16 @@ int tea_hip_local_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
   TEA_HIP_TRY(hipSetDevice(c->device));
   bool all_external = true;
   for (int f = 0; f < 4; ++f) all_external &= neighbours[f] < 0;
-  // The depth-1 SD update between two PPCG inner steps is folded into the
-  // pass that runs both steps (the held-back first step is flushed instead
-  // if anything else comes between them).
-  if (c->ppcg_pend && !c->ppcg_pend_halo && depth == 1 && all_external) {
-    bool only_sd = true;
+  // The depth-1 SD (U) update between two PPCG inner steps (Chebyshev
+  // steps) is folded into the pass that runs both steps; the held-back
+  // first step is flushed instead if anything else comes between them.
+  if (c->held && !c->held_halo && depth == 1 && all_external) {
+    const int slot = c->held == kHeldPpcg ? TEA_HALO_SD : TEA_HALO_U;
+    bool only_slot = true;
     for (int s = 0; s < TEA_HALO_NUM; ++s)
-      only_sd &= (fields_to_exchange[s] != 0) == (s == TEA_HALO_SD);
-    if (only_sd) {
-      c->ppcg_pend_halo = true;
+      only_slot &= (fields_to_exchange[s] != 0) == (s == slot);
+    if (only_slot) {
+      c->held_halo = true;
       return TEA_HIP_OK;
     }
   }
@@ -319,7 +320,7 @@ int tea_hip_field_summary(tea_hip_chunk* c, double* vol, double* mass,
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(vol && mass && ie && temp, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(apply_pending(c));
   double** f = c->field;
   SummaryBody body{f[TEA_FIELD_VOLUME], f[TEA_FIELD_DENSITY],

@@ -573,7 +573,7 @@ int tea_hip_mpcg_init(tea_hip_chunk* c, int coefficient, double rx, double ry,
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(rr && bb, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(ensure_mp_buffers(c));
   // FP64 initialisation exactly as the plain path: kx, ky, u, w, r, p.
   double rro64 = 0.0;
@@ -611,7 +611,7 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
   TEA_CHECK_ARG(pw != nullptr && pr != nullptr, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   // The two-dot kernel of flexible CG with z = r: p = r + beta p_old,
   // w = A p, p.w and p.r.
   MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->mp_r,
@@ -651,7 +651,7 @@ int tea_hip_mpcg_calc_r(tea_hip_chunk* c, double alpha, double* rr, double* rw) 
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
   TEA_CHECK_ARG(rr != nullptr && rw != nullptr, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   int len;
   TEA_RETURN_IF(launch_mp_r(c, alpha, &len, true));
   double out[2];
@@ -667,7 +667,7 @@ int tea_hip_mpcg_refresh(tea_hip_chunk* c, double alpha, int apply_u,
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
   TEA_CHECK_ARG(rr && rr_true && rw, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   double** f = c->field;
   const Geo g = c->geo;
   if (apply_u) {
@@ -705,7 +705,7 @@ int tea_hip_mp_read(tea_hip_chunk* c, int which, float* host, size_t count) {
                                           : nullptr;
   TEA_CHECK_ARG(src != nullptr, "bad FP32 field id");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_PPCG(c);
+  TEA_FLUSH_HELD(c);
   TEA_HIP_TRY(hipMemcpyAsync(host, src, count * sizeof(float),
                              hipMemcpyDeviceToHost, c->stream));
   return sync_chunk(c);

@@ -3,8 +3,9 @@
 // TeaLeaf/c_kernels/sycl/kernel_interface.cpp:246-328 composes them).
 //
 // Each iteration entry point is one streaming pass in the default (fused)
-// mode, with one cell pair per lane as in the fused CG kernels (and PPCG
-// inner steps two to a pass, see ppcg_two_step_kernel):
+// mode, with one cell pair per lane as in the fused CG kernels; consecutive
+// Chebyshev steps and PPCG inner steps run two to a pass
+// (cheby_two_step_kernel, ppcg_two_step_kernel):
 //   Jacobi     reads u (5-point), u0, kx, ky; writes the new u into r's
 //              buffer, the two fields' addresses are then exchanged, so the
 //              old u is "r = u" without a copy; reduces sum |u_new - u|.
@@ -539,6 +540,243 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
   }
 }
 
+// Two Chebyshev steps (cheby.cpp:46-110) with the depth-1 reflective U
+// update between them, as ppcg_two_step_kernel does for PPCG: u_k staged
+// with two cells around the tile, u_(k+1) = u_k + p_(k+1) with
+// p_(k+1) = a0 p_k + b0 (u0 - A u_k) formed on the tile and one cell around
+// it, the ring cells of that margin mirrored, then step k+1 for the tile's
+// cells.  u and p go to second buffers (neighbouring tiles still read the
+// old ones); w and r, written only at a tile's own cells and never read,
+// stay in place and end as the second step's.  72 bytes per cell for two
+// steps against 144 for two single passes.
+__global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
+    Geo g, const double* __restrict__ u_old, double* __restrict__ u_new,
+    const double* __restrict__ u0, const double* __restrict__ p_old,
+    double* __restrict__ p_new, double* __restrict__ r, double* __restrict__ w,
+    const double* __restrict__ kx, const double* __restrict__ ky, double a0,
+    double b0, double a1, double b1, int tiles_x, int tiles_y) {
+  __shared__ double lds[kS0H * kS0W];
+  double (*S0)[kS0W] = reinterpret_cast<double (*)[kS0W]>(lds);
+  double (*S1)[kS1W] = reinterpret_cast<double (*)[kS1W]>(lds);
+  const int x = g.x, y = g.y, h = g.h;
+  const int xe = x - h, ye = y - h;
+  const int txi = blockIdx.x % tiles_x, tyi = blockIdx.x / tiles_x;
+  const int c0 = h + txi * kTileX, r0 = h + tyi * kTileY;
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const bool edge = txi == 0 || tyi == 0 || txi == tiles_x - 1 || tyi == tiles_y - 1;
+
+  // 1. The tile's own column (u0, p_k, kx through the right boundary face,
+  // ky with the row above) and u_k for the staging area, before one wait.
+  const int kk = c0 + t;
+  double bv[kTileY], pk[kTileY], kx0[kTileY], kxe[kTileY], kyv[kTileY + 1];
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) bv[m] = pk[m] = kx0[m] = kxe[m] = kyv[m] = 0.0;
+  kyv[kTileY] = 0.0;
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) {
+    const int jj = r0 + m;
+    if (jj < ye && kk <= xe) kx0[m] = kx[kk + jj * x];
+    if (kk < xe && jj < ye) {
+      const int i = kk + jj * x;
+      bv[m] = u0[i];
+      pk[m] = p_old[i];
+      if (lane == kWave - 1) kxe[m] = kx[i + 1];
+      kyv[m] = ky[i];
+      kyv[m + 1] = ky[i + x];
+    }
+  }
+  constexpr int kStage = (kS0H * kS0W + kBlock - 1) / kBlock;
+  double st[kStage];
+#pragma unroll
+  for (int n = 0; n < kStage; ++n) {
+    const int q = t + n * kBlock;
+    const int a = q / kS0W, b = q - a * kS0W;
+    const int jj = r0 - 2 + a, kc = c0 - 2 + b;
+    st[n] = (q < kS0H * kS0W && jj >= 0 && jj < y && kc >= 0 && kc < x)
+                ? u_old[kc + jj * x]
+                : 0.0;
+  }
+#pragma unroll
+  for (int n = 0; n < kStage; ++n) {
+    const int q = t + n * kBlock;
+    if (q < kS0H * kS0W) lds[q] = st[n];
+  }
+  __syncthreads();
+
+  // 2. u_(k+1) on the tile's own cells (pk becomes p_(k+1)) ...
+  double u1own[kTileY];
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) {
+    const int jj = r0 + m;
+    double kx1 = from_next_lane(kx0[m]);
+    if (lane == kWave - 1) kx1 = kxe[m];
+    u1own[m] = 0.0;
+    if (kk < xe && jj < ye) {
+      const double c = S0[m + 2][t + 2];
+      const double wv = op5(c, S0[m + 2][t + 1], S0[m + 2][t + 3], S0[m + 1][t + 2],
+                            S0[m + 3][t + 2], kx0[m], kx1, kyv[m], kyv[m + 1]);
+      const double rv = bv[m] - wv;
+      pk[m] = a0 * pk[m] + b0 * rv;
+      u1own[m] = c + pk[m];
+    }
+  }
+  // ... and on the one-cell margin (as in ppcg_two_step_kernel).
+  constexpr int kMargin = 2 * kS1W + 2 * kTileY;
+  constexpr int kMarginPer = (kMargin + kBlock - 1) / kBlock;
+  double u1m[kMarginPer];
+  int u1pos[kMarginPer];
+#pragma unroll
+  for (int n = 0; n < kMarginPer; ++n) {
+    const int q = t + n * kBlock;
+    int a = 0, b = 0;
+    if (q < 2 * kS1W) {
+      a = q < kS1W ? 0 : kS1H - 1;
+      b = q < kS1W ? q : q - kS1W;
+    } else {
+      const int e = q - 2 * kS1W;
+      a = 1 + (e >> 1);
+      b = (e & 1) ? kS1W - 1 : 0;
+    }
+    const int jj = r0 - 1 + a, kc = c0 - 1 + b;
+    double u1 = 0.0;
+    if (q < kMargin && jj >= h && jj < ye && kc >= h && kc < xe) {
+      const int i = kc + jj * x;
+      const double c = S0[a + 1][b + 1];
+      const double wv = op5(c, S0[a + 1][b], S0[a + 1][b + 2], S0[a][b + 1],
+                            S0[a + 2][b + 1], kx[i], kx[i + 1], ky[i], ky[i + x]);
+      const double rv = u0[i] - wv;
+      u1 = c + (a0 * p_old[i] + b0 * rv);
+    }
+    u1m[n] = u1;
+    u1pos[n] = q < kMargin ? a * kS1W + b : -1;
+  }
+  __syncthreads();  // every read of u_k is done: u_(k+1) replaces it
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) S1[m + 1][t + 1] = u1own[m];
+#pragma unroll
+  for (int n = 0; n < kMarginPer; ++n)
+    if (u1pos[n] >= 0) lds[u1pos[n]] = u1m[n];
+  __syncthreads();
+
+  // 3. Ring cells of the margin take their mirror's value.
+  if (edge) {
+    for (int q = t; q < kS1H * kS1W; q += kBlock) {
+      const int a = q / kS1W, b = q - a * kS1W;
+      const int jj = r0 - 1 + a, kc = c0 - 1 + b;
+      const bool ring_box = jj >= h - 1 && jj <= ye && kc >= h - 1 && kc <= xe;
+      const bool inner = jj >= h && jj < ye && kc >= h && kc < xe;
+      if (!ring_box || inner) continue;
+      const int mj = jj == h - 1 ? h : (jj == ye ? ye - 1 : jj);
+      const int mk = kc == h - 1 ? h : (kc == xe ? xe - 1 : kc);
+      S1[a][b] = S1[mj - r0 + 1][mk - c0 + 1];
+    }
+    __syncthreads();
+  }
+
+  // 4. Step k+1 on the tile's own cells.
+#pragma unroll
+  for (int m = 0; m < kTileY; ++m) {
+    const int jj = r0 + m;
+    double kx1 = from_next_lane(kx0[m]);
+    if (lane == kWave - 1) kx1 = kxe[m];
+    if (kk < xe && jj < ye) {
+      const int i = kk + jj * x;
+      const double c = S1[m + 1][t + 1];
+      const double wv = op5(c, S1[m + 1][t], S1[m + 1][t + 2], S1[m][t + 1],
+                            S1[m + 2][t + 1], kx0[m], kx1, kyv[m], kyv[m + 1]);
+      const double rv = bv[m] - wv;
+      const double pv = a1 * pk[m] + b1 * rv;
+      w[i] = wv;
+      r[i] = rv;
+      p_new[i] = pv;
+      u_new[i] = c + pv;
+    }
+  }
+
+  // 5. The second buffers' cells outside the interior next to this tile: u
+  // as the halo update leaves it (see ppcg_two_step_kernel), p copied.
+  if (edge) {
+    const int ka = txi == 0 ? 0 : c0;
+    const int kb = txi == tiles_x - 1 ? x : min(c0 + kTileX, xe);
+    const int ja = tyi == 0 ? 0 : r0;
+    const int jb = tyi == tiles_y - 1 ? y : min(r0 + kTileY, ye);
+    const int wd = kb - ka;
+    for (int q = t; q < wd * (jb - ja); q += kBlock) {
+      const int jj = ja + q / wd, kc = ka + q % wd;
+      if (jj >= h && jj < ye && kc >= h && kc < xe) continue;
+      const int mj = jj == h - 1 ? h : (jj == ye ? ye - 1 : jj);
+      const int mk = kc == h - 1 ? h : (kc == xe ? xe - 1 : kc);
+      const bool from_new = mj >= h && mj < ye && mk >= h && mk < xe;
+      u_new[kc + jj * x] =
+          from_new ? S1[mj - r0 + 1][mk - c0 + 1] : u_old[mk + mj * x];
+      p_new[kc + jj * x] = p_old[kc + jj * x];
+    }
+  }
+}
+
+// The second spare buffer, allocated on first use.
+int ensure_spare2(tea_hip_chunk* c) {
+  if (c->spare2) return TEA_HIP_OK;
+  if (hipMalloc(&c->spare2, (size_t)c->geo.n * sizeof(double)) != hipSuccess) {
+    c->spare2 = nullptr;
+    set_error("hipMalloc of the second spare field buffer failed");
+    return TEA_HIP_ENOMEM;
+  }
+  return TEA_HIP_OK;
+}
+
+// One Chebyshev step: the single pass (even rows) or one launch per
+// reference kernel.
+int cheby_step(tea_hip_chunk* c, double alpha, double beta) {
+  double** f = c->field;
+  const Geo g = c->geo;
+  if (!fused_pairs(c)) {
+    TEA_RETURN_IF(launch_each_cell(
+        c, K_CHEBY_ITERATE,
+        ChebyIterateBody{f[TEA_FIELD_P], f[TEA_FIELD_R], f[TEA_FIELD_U],
+                         f[TEA_FIELD_U0], f[TEA_FIELD_W], f[TEA_FIELD_KX],
+                         f[TEA_FIELD_KY], alpha, beta, g}));
+    return launch_each_cell(c, K_CHEBY_CALC_U,
+                            AddBody{f[TEA_FIELD_U], f[TEA_FIELD_P], g});
+  }
+  TEA_RETURN_IF(ensure_spare(c));
+  {
+    Timed t(c, K_CHEBY_FUSED);
+    hipLaunchKernelGGL(cheby_pair_kernel, pair_grid(c), dim3(kBlock), 0,
+                       c->stream, g, f[TEA_FIELD_U], c->spare, f[TEA_FIELD_U0],
+                       f[TEA_FIELD_P], f[TEA_FIELD_R], f[TEA_FIELD_W],
+                       f[TEA_FIELD_KX], f[TEA_FIELD_KY], alpha, beta);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  std::swap(f[TEA_FIELD_U], c->spare);
+  return TEA_HIP_OK;
+}
+
+// Two Chebyshev steps with the U halo update between them, in one pass.
+int cheby_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
+                    double b1) {
+  double** f = c->field;
+  const Geo g = c->geo;
+  TEA_RETURN_IF(ensure_spare(c));
+  TEA_RETURN_IF(ensure_spare2(c));
+  const int tiles_x = (g.x - 2 * g.h + kTileX - 1) / kTileX;
+  const int tiles_y = (g.y - 2 * g.h + kTileY - 1) / kTileY;
+  {
+    Timed t(c, K_CHEBY_TWO_STEP);
+    hipLaunchKernelGGL(cheby_two_step_kernel, dim3(tiles_x * tiles_y),
+                       dim3(kBlock), 0, c->stream, g, f[TEA_FIELD_U], c->spare,
+                       f[TEA_FIELD_U0], f[TEA_FIELD_P], c->spare2, f[TEA_FIELD_R],
+                       f[TEA_FIELD_W], f[TEA_FIELD_KX], f[TEA_FIELD_KY], a0, b0,
+                       a1, b1, tiles_x, tiles_y);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  std::swap(f[TEA_FIELD_U], c->spare);
+  std::swap(f[TEA_FIELD_P], c->spare2);
+  c->p_alt_synced = false;
+  return TEA_HIP_OK;
+}
+
 // One inner step: the single pass (even rows) or one launch per reference
 // kernel; the new sd goes to the second buffer in the single pass.
 int ppcg_step(tea_hip_chunk* c, double alpha, double beta) {
@@ -572,12 +810,7 @@ int ppcg_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
   double** f = c->field;
   const Geo g = c->geo;
   TEA_RETURN_IF(ensure_spare(c));
-  if (!c->spare2 &&
-      hipMalloc(&c->spare2, (size_t)g.n * sizeof(double)) != hipSuccess) {
-    c->spare2 = nullptr;
-    set_error("hipMalloc of the second spare field buffer failed");
-    return TEA_HIP_ENOMEM;
-  }
+  TEA_RETURN_IF(ensure_spare2(c));
   const int tiles_x = (g.x - 2 * g.h + kTileX - 1) / kTileX;
   const int tiles_y = (g.y - 2 * g.h + kTileY - 1) / kTileY;
   {
@@ -595,15 +828,20 @@ int ppcg_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
 
 }  // namespace
 
-int flush_ppcg(tea_hip_chunk* c) {
-  if (!c->ppcg_pend) return TEA_HIP_OK;
-  const bool halo = c->ppcg_pend_halo;
-  c->ppcg_pend = c->ppcg_pend_halo = false;
-  TEA_RETURN_IF(ppcg_step(c, c->ppcg_alpha, c->ppcg_beta));
+int flush_held(tea_hip_chunk* c) {
+  const int kind = c->held;
+  if (kind == kHeldNone) return TEA_HIP_OK;
+  const bool halo = c->held_halo;
+  c->held = kHeldNone;
+  c->held_halo = false;
+  if (kind == kHeldPpcg)
+    TEA_RETURN_IF(ppcg_step(c, c->held_alpha, c->held_beta));
+  else
+    TEA_RETURN_IF(cheby_step(c, c->held_alpha, c->held_beta));
   if (!halo) return TEA_HIP_OK;
   static const int ext[4] = {-1, -1, -1, -1};
   int fields[TEA_HALO_NUM] = {0};
-  fields[TEA_HALO_SD] = 1;
+  fields[kind == kHeldPpcg ? TEA_HALO_SD : TEA_HALO_U] = 1;
   return launch_halos(c, 1, ext, fields);
 }
 
@@ -677,30 +915,24 @@ int tea_hip_cheby_init(tea_hip_chunk* c, double theta) {
 }
 
 int tea_hip_cheby_iterate(tea_hip_chunk* c, double alpha, double beta) {
-  TEA_ENTER(c);
+  TEA_ENTER_HELD(c);
+  if (c->fused) {
+    // Held back until the next step: with the U halo update between them
+    // (all faces external), both steps run as one pass.
+    if (c->held == kHeldCheby && c->held_halo) {
+      c->held = kHeldNone;
+      c->held_halo = false;
+      return cheby_two_steps(c, c->held_alpha, c->held_beta, alpha, beta);
+    }
+    TEA_RETURN_IF(apply_pending(c));
+    c->held = kHeldCheby;
+    c->held_halo = false;
+    c->held_alpha = alpha;
+    c->held_beta = beta;
+    return TEA_HIP_OK;
+  }
   TEA_RETURN_IF(apply_pending(c));
-  double** f = c->field;
-  const Geo g = c->geo;
-  if (!fused_pairs(c)) {
-    TEA_RETURN_IF(launch_each_cell(
-        c, K_CHEBY_ITERATE,
-        ChebyIterateBody{f[TEA_FIELD_P], f[TEA_FIELD_R], f[TEA_FIELD_U],
-                         f[TEA_FIELD_U0], f[TEA_FIELD_W], f[TEA_FIELD_KX],
-                         f[TEA_FIELD_KY], alpha, beta, g}));
-    return launch_each_cell(c, K_CHEBY_CALC_U,
-                            AddBody{f[TEA_FIELD_U], f[TEA_FIELD_P], g});
-  }
-  TEA_RETURN_IF(ensure_spare(c));
-  {
-    Timed t(c, K_CHEBY_FUSED);
-    hipLaunchKernelGGL(cheby_pair_kernel, pair_grid(c), dim3(kBlock), 0,
-                       c->stream, g, f[TEA_FIELD_U], c->spare, f[TEA_FIELD_U0],
-                       f[TEA_FIELD_P], f[TEA_FIELD_R], f[TEA_FIELD_W],
-                       f[TEA_FIELD_KX], f[TEA_FIELD_KY], alpha, beta);
-  }
-  TEA_HIP_TRY(hipGetLastError());
-  std::swap(f[TEA_FIELD_U], c->spare);
-  return TEA_HIP_OK;
+  return cheby_step(c, alpha, beta);
 }
 
 int tea_hip_ppcg_init(tea_hip_chunk* c, double theta) {
@@ -717,15 +949,16 @@ int tea_hip_ppcg_inner_iteration(tea_hip_chunk* c, double alpha,
   if (c->fused) {
     // Held back until the next inner step: with the SD halo update between
     // them (all faces external), both steps run as one pass.
-    if (c->ppcg_pend && c->ppcg_pend_halo) {
-      c->ppcg_pend = c->ppcg_pend_halo = false;
-      return ppcg_two_steps(c, c->ppcg_alpha, c->ppcg_beta, alpha, beta);
+    if (c->held == kHeldPpcg && c->held_halo) {
+      c->held = kHeldNone;
+      c->held_halo = false;
+      return ppcg_two_steps(c, c->held_alpha, c->held_beta, alpha, beta);
     }
     TEA_RETURN_IF(apply_pending(c));
-    c->ppcg_pend = true;
-    c->ppcg_pend_halo = false;
-    c->ppcg_alpha = alpha;
-    c->ppcg_beta = beta;
+    c->held = kHeldPpcg;
+    c->held_halo = false;
+    c->held_alpha = alpha;
+    c->held_beta = beta;
     return TEA_HIP_OK;
   }
   TEA_RETURN_IF(apply_pending(c));

@@ -450,6 +450,7 @@ enum KernelId {
   K_PPCG_CALC_SD,
   K_PPCG_FUSED,
   K_PPCG_TWO_STEP,
+  K_CHEBY_TWO_STEP,
   K_NUM
 };
 
@@ -546,15 +547,18 @@ struct tea_hip_chunk {
   // call): the fused Chebyshev and PPCG steps write the new u or sd into it
   // and exchange it with the field.
   double* spare = nullptr;
-  // Second spare buffer (x*y, allocated by the first two-step PPCG pass),
-  // which takes the new r there: neighbouring tiles still read the old r.
+  // Second spare buffer (x*y, allocated by the first two-step pass), which
+  // takes the new r (PPCG) or p (Chebyshev) there: neighbouring tiles still
+  // read the old values.
   double* spare2 = nullptr;
 
-  // A PPCG inner step held back (fused mode) so that it can run in one pass
-  // with the next one: its coefficients, and whether the depth-1 SD halo
-  // update that follows it has been requested (and held back too).
-  bool ppcg_pend = false, ppcg_pend_halo = false;
-  double ppcg_alpha = 0.0, ppcg_beta = 0.0;
+  // A PPCG inner step or Chebyshev step held back (fused mode) so that it
+  // can run in one pass with the next one: which (kHeld*), its
+  // coefficients, and whether the depth-1 halo update that follows it (of
+  // SD, or U) has been requested and held back too.
+  int held = 0;
+  bool held_halo = false;
+  double held_alpha = 0.0, held_beta = 0.0;
 
   // Deferred CG work (fused mode).
   int fused = 1;
@@ -642,11 +646,15 @@ inline bool use_pair_blocks(const tea_hip_chunk* c, int kernel_bit) {
   return c->pairs && (c->pair_blocks & kernel_bit) && c->ngroup > 1;
 }
 
-// Applies any deferred CG work (fused mode), after a held-back PPCG step.
+// Kinds of held-back step.
+enum { kHeldNone = 0, kHeldPpcg = 1, kHeldCheby = 2 };
+
+// Applies any deferred CG work (fused mode), after a held-back step.
 int apply_pending(tea_hip_chunk* c);
-// Runs a held-back PPCG inner step, then its held-back SD halo update
+// Runs a held-back PPCG inner step or Chebyshev step, then its held-back
+// halo update
 // (solvers.hip).
-int flush_ppcg(tea_hip_chunk* c);
+int flush_held(tea_hip_chunk* c);
 // Mixed-precision residual update r -= alpha w with the partials of r.r
 // (and, with rw, of r.w as a second array) written to c->partials
 // (mpcg.hip); *len is the number left to finish in each.
@@ -662,15 +670,15 @@ int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
   TEA_HIP_TRY(hipSetDevice((c)->device))
 
 // Entry points other than the PPCG inner step and the halo update first run
-// a held-back PPCG step.
-#define TEA_FLUSH_PPCG(c)                                        \
+// a held-back PPCG or Chebyshev step.
+#define TEA_FLUSH_HELD(c)                                        \
   do {                                                           \
-    if ((c)->ppcg_pend) TEA_RETURN_IF(::tea::flush_ppcg(c));     \
+    if ((c)->held) TEA_RETURN_IF(::tea::flush_held(c));     \
   } while (0)
 
 #define TEA_ENTER(c)   \
   TEA_ENTER_HELD(c);   \
-  TEA_FLUSH_PPCG(c)
+  TEA_FLUSH_HELD(c)
 
 inline dim3 span_grid(const tea_hip_chunk* c) { return dim3(c->nspan); }
 inline dim3 pair_grid(const tea_hip_chunk* c) { return dim3(c->nblk); }

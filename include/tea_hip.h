@@ -214,7 +214,10 @@ int tea_hip_cheby_init(tea_hip_chunk* chunk, double theta);
 
 /* One Chebyshev step on the interior: w = A u, r = u0 - w,
  * p = alpha p + beta r, u = u + p (one pass; the new u goes to a second
- * buffer whose address replaces u's).  Replaces run_cheby_iterate
+ * buffer whose address replaces u's).  In the fused mode the step is held
+ * back until the next call: if that is the depth-1 U halo update and then
+ * the next step, both steps run as one pass (u and p then move to second
+ * buffers); any other entry point runs it first.  Replaces run_cheby_iterate
  * (TeaLeaf/kernel_interface.h:48-49; kernel_interface.cpp:258-271,
  * cheby.cpp:46-110). */
 int tea_hip_cheby_iterate(tea_hip_chunk* chunk, double alpha, double beta);
@@ -225,7 +228,11 @@ int tea_hip_ppcg_init(tea_hip_chunk* chunk, double theta);
 
 /* One inner Chebyshev step of PPCG on the interior: r = r - A sd,
  * u = u + sd, sd = alpha sd + beta r (one pass; the new sd goes to a second
- * buffer whose address replaces sd's).  Replaces run_ppcg_inner_iteration
+ * buffer whose address replaces sd's).  In the fused mode the step is held
+ * back until the next call: if that is the depth-1 SD halo update and then
+ * the next inner step, both steps run as one pass (sd and r then move to
+ * second buffers); any other entry point runs it first.  Replaces
+ * run_ppcg_inner_iteration
  * (TeaLeaf/kernel_interface.h:60-61; kernel_interface.cpp:314-328,
  * ppcg.cpp:34-94). */
 int tea_hip_ppcg_inner_iteration(tea_hip_chunk* chunk, double alpha,
