@@ -346,7 +346,7 @@ bool fused_pairs(const tea_hip_chunk* c) { return c->fused && c->pairs; }
 //
 // Steps k and k+1 (ppcg.cpp:34-94) with the depth-1 reflective SD update
 // between them (local_halos.cpp:7-102).  A block owns a tile of kTileY rows
-// x kTileX columns of the interior, one column per thread:
+// x kTileX columns of the interior; each thread a column of kRows of them:
 //   1. sd_k on the tile and two cells around it is staged in LDS;
 //   2. sd_(k+1) = a0 sd_k + b0 (r_k - A sd_k) is formed on the tile and one
 //      cell around it (the tile's r_(k+1) stays in registers);
@@ -360,8 +360,9 @@ bool fused_pairs(const tea_hip_chunk* c) { return c->fused && c->pairs; }
 // 64 bytes per cell for two steps against 128 for two single passes.  Every
 // value uses the single-step kernel's expressions, so the result is bitwise
 // that of two single steps with the halo update between them.
-constexpr int kTileX = kBlock;
-constexpr int kTileY = 4;
+constexpr int kTileX = 128;
+constexpr int kTileY = 8;
+constexpr int kRows = kTileY * kTileX / kBlock;  // rows per thread
 constexpr int kS0W = kTileX + 4, kS0H = kTileY + 4;
 constexpr int kS1W = kTileX + 2, kS1H = kTileY + 2;
 
@@ -388,14 +389,15 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
   // sd_k for the staging area are issued before the block waits once: r,
   // kx (through the right boundary face; lane 63 also reads its right
   // neighbour's) and ky (cells and the row above).
-  const int kk = c0 + t;
-  double rk1[kTileY], kx0[kTileY], kxe[kTileY], kyv[kTileY + 1];
+  const int tc = t % kTileX, tr = (t / kTileX) * kRows;
+  const int kk = c0 + tc;
+  double rk1[kRows], kx0[kRows], kxe[kRows], kyv[kRows + 1];
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) rk1[m] = kx0[m] = kxe[m] = kyv[m] = 0.0;
-  kyv[kTileY] = 0.0;
+  for (int m = 0; m < kRows; ++m) rk1[m] = kx0[m] = kxe[m] = kyv[m] = 0.0;
+  kyv[kRows] = 0.0;
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) {
-    const int jj = r0 + m;
+  for (int m = 0; m < kRows; ++m) {
+    const int jj = r0 + tr + m;
     if (jj < ye && kk <= xe) kx0[m] = kx[kk + jj * x];
     if (kk < xe && jj < ye) {
       const int i = kk + jj * x;
@@ -425,17 +427,17 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
   __syncthreads();
 
   // 2. sd_(k+1) on the tile's own cells (rk1 becomes r_(k+1)) ...
-  double sdk[kTileY], s1own[kTileY];
+  double sdk[kRows], s1own[kRows];
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) {
-    const int jj = r0 + m;
+  for (int m = 0; m < kRows; ++m) {
+    const int jj = r0 + tr + m;
     double kx1 = __shfl_down(kx0[m], 1);
     if (lane == kWave - 1) kx1 = kxe[m];
     sdk[m] = s1own[m] = 0.0;
     if (kk < xe && jj < ye) {
-      const double c = S0[m + 2][t + 2];
-      const double rn = rk1[m] - op5(c, S0[m + 2][t + 1], S0[m + 2][t + 3],
-                                     S0[m + 1][t + 2], S0[m + 3][t + 2], kx0[m],
+      const double c = S0[tr + m + 2][tc + 2];
+      const double rn = rk1[m] - op5(c, S0[tr + m + 2][tc + 1], S0[tr + m + 2][tc + 3],
+                                     S0[tr + m + 1][tc + 2], S0[tr + m + 3][tc + 2], kx0[m],
                                      kx1, kyv[m], kyv[m + 1]);
       sdk[m] = c;
       rk1[m] = rn;
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
   }
   __syncthreads();  // every read of sd_k is done: S1 replaces it
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) S1[m + 1][t + 1] = s1own[m];
+  for (int m = 0; m < kRows; ++m) S1[tr + m + 1][tc + 1] = s1own[m];
 #pragma unroll
   for (int n = 0; n < kMarginPer; ++n)
     if (s1pos[n] >= 0) lds[s1pos[n]] = s1m[n];
@@ -499,15 +501,15 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
 
   // 4. Step k+1 on the tile's own cells.
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) {
-    const int jj = r0 + m;
+  for (int m = 0; m < kRows; ++m) {
+    const int jj = r0 + tr + m;
     double kx1 = __shfl_down(kx0[m], 1);
     if (lane == kWave - 1) kx1 = kxe[m];
     if (kk < xe && jj < ye) {
       const int i = kk + jj * x;
-      const double c = S1[m + 1][t + 1];
-      const double rn = rk1[m] - op5(c, S1[m + 1][t], S1[m + 1][t + 2], S1[m][t + 1],
-                                     S1[m + 2][t + 1], kx0[m], kx1, kyv[m],
+      const double c = S1[tr + m + 1][tc + 1];
+      const double rn = rk1[m] - op5(c, S1[tr + m + 1][tc], S1[tr + m + 1][tc + 2], S1[tr + m][tc + 1],
+                                     S1[tr + m + 2][tc + 1], kx0[m], kx1, kyv[m],
                                      kyv[m + 1]);
       r_new[i] = rn;
       u[i] = (u[i] + sdk[m]) + c;
@@ -568,14 +570,15 @@ __global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
 
   // 1. The tile's own column (u0, p_k, kx through the right boundary face,
   // ky with the row above) and u_k for the staging area, before one wait.
-  const int kk = c0 + t;
-  double bv[kTileY], pk[kTileY], kx0[kTileY], kxe[kTileY], kyv[kTileY + 1];
+  const int tc = t % kTileX, tr = (t / kTileX) * kRows;
+  const int kk = c0 + tc;
+  double bv[kRows], pk[kRows], kx0[kRows], kxe[kRows], kyv[kRows + 1];
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) bv[m] = pk[m] = kx0[m] = kxe[m] = kyv[m] = 0.0;
-  kyv[kTileY] = 0.0;
+  for (int m = 0; m < kRows; ++m) bv[m] = pk[m] = kx0[m] = kxe[m] = kyv[m] = 0.0;
+  kyv[kRows] = 0.0;
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) {
-    const int jj = r0 + m;
+  for (int m = 0; m < kRows; ++m) {
+    const int jj = r0 + tr + m;
     if (jj < ye && kk <= xe) kx0[m] = kx[kk + jj * x];
     if (kk < xe && jj < ye) {
       const int i = kk + jj * x;
@@ -605,17 +608,17 @@ __global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
   __syncthreads();
 
   // 2. u_(k+1) on the tile's own cells (pk becomes p_(k+1)) ...
-  double u1own[kTileY];
+  double u1own[kRows];
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) {
-    const int jj = r0 + m;
+  for (int m = 0; m < kRows; ++m) {
+    const int jj = r0 + tr + m;
     double kx1 = from_next_lane(kx0[m]);
     if (lane == kWave - 1) kx1 = kxe[m];
     u1own[m] = 0.0;
     if (kk < xe && jj < ye) {
-      const double c = S0[m + 2][t + 2];
-      const double wv = op5(c, S0[m + 2][t + 1], S0[m + 2][t + 3], S0[m + 1][t + 2],
-                            S0[m + 3][t + 2], kx0[m], kx1, kyv[m], kyv[m + 1]);
+      const double c = S0[tr + m + 2][tc + 2];
+      const double wv = op5(c, S0[tr + m + 2][tc + 1], S0[tr + m + 2][tc + 3], S0[tr + m + 1][tc + 2],
+                            S0[tr + m + 3][tc + 2], kx0[m], kx1, kyv[m], kyv[m + 1]);
       const double rv = bv[m] - wv;
       pk[m] = a0 * pk[m] + b0 * rv;
       u1own[m] = c + pk[m];
@@ -653,7 +656,7 @@ __global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
   }
   __syncthreads();  // every read of u_k is done: u_(k+1) replaces it
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) S1[m + 1][t + 1] = u1own[m];
+  for (int m = 0; m < kRows; ++m) S1[tr + m + 1][tc + 1] = u1own[m];
 #pragma unroll
   for (int n = 0; n < kMarginPer; ++n)
     if (u1pos[n] >= 0) lds[u1pos[n]] = u1m[n];
@@ -676,15 +679,15 @@ __global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
 
   // 4. Step k+1 on the tile's own cells.
 #pragma unroll
-  for (int m = 0; m < kTileY; ++m) {
-    const int jj = r0 + m;
+  for (int m = 0; m < kRows; ++m) {
+    const int jj = r0 + tr + m;
     double kx1 = from_next_lane(kx0[m]);
     if (lane == kWave - 1) kx1 = kxe[m];
     if (kk < xe && jj < ye) {
       const int i = kk + jj * x;
-      const double c = S1[m + 1][t + 1];
-      const double wv = op5(c, S1[m + 1][t], S1[m + 1][t + 2], S1[m][t + 1],
-                            S1[m + 2][t + 1], kx0[m], kx1, kyv[m], kyv[m + 1]);
+      const double c = S1[tr + m + 1][tc + 1];
+      const double wv = op5(c, S1[tr + m + 1][tc], S1[tr + m + 1][tc + 2], S1[tr + m][tc + 1],
+                            S1[tr + m + 2][tc + 1], kx0[m], kx1, kyv[m], kyv[m + 1]);
       const double rv = bv[m] - wv;
       const double pv = a1 * pk[m] + b1 * rv;
       w[i] = wv;
