@@ -145,6 +145,15 @@ class DeviceChunk:
     def finalise(self):
         check(self.lib.tea_hip_finalise(self.handle), "finalise")
 
+    def pack_or_unpack(self, depth, face, pack, name, buffer):
+        """Remote-halo strip of field `name` to (pack) or from the float64
+        array `buffer` (kernel_interface.cpp:129-170)."""
+        p, _ = self.field_info(name)
+        assert buffer.dtype == np.float64 and buffer.flags["C_CONTIGUOUS"]
+        check(self.lib.tea_hip_pack_or_unpack(self.handle, depth, face, 1 if pack else 0,
+                                              ctypes.c_void_p(p), _ptr(buffer)),
+              "pack_or_unpack")
+
     # ---- Jacobi, Chebyshev, PPCG (kernel_interface.cpp:246-328) ---------------
     def jacobi_init(self, coefficient, rx, ry):
         check(self.lib.tea_hip_jacobi_init(self.handle, coefficient, rx, ry),

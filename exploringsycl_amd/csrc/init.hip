@@ -141,6 +141,37 @@ __global__ __launch_bounds__(kBlock) void halo_kernel(Geo g, int depth,
     b[(long)(h - 1 - k) * x + kk] = b[(long)(h + k) * x + src_col];
 }
 
+// ---- remote-halo strips (TeaLeaf/c_kernels/sycl/pack_halos.cpp:7-184) -----
+//
+// Item i of a face's strip of n = depth * (y or x) items: the left/right
+// strips run row by row, depth cells each, the bottom/top strips are depth
+// whole rows.  Packing reads the cells next to the interior, unpacking
+// writes the halo cells, exactly at the reference's offsets.
+__global__ __launch_bounds__(kBlock) void strip_kernel(Geo g, int depth,
+                                                      int face, int pack,
+                                                      double* __restrict__ field,
+                                                      double* __restrict__ strip) {
+  const int x = g.x, y = g.y, h = g.h;
+  const long n = (face == TEA_FACE_LEFT || face == TEA_FACE_RIGHT)
+                     ? (long)y * depth : (long)x * depth;
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const long lines = i / depth;
+  long f;
+  if (face == TEA_FACE_LEFT)
+    f = (pack ? h : h - depth) + lines * (x - depth) + i;
+  else if (face == TEA_FACE_RIGHT)
+    f = (pack ? x - h - depth : x - h) + lines * (x - depth) + i;
+  else if (face == TEA_FACE_BOTTOM)
+    f = (long)x * (pack ? h : h - depth) + i;
+  else
+    f = (long)x * (pack ? y - h - depth : y - h) + i;
+  if (pack)
+    strip[i] = field[f];
+  else
+    field[f] = strip[i];
+}
+
 // ---- field summary (TeaLeaf/c_kernels/sycl/field_summary.cpp:6-151) ------
 
 struct SummaryBody {
@@ -313,6 +344,40 @@ int tea_hip_local_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
   }
   TEA_RETURN_IF(apply_pending(c));
   return launch_halos(c, depth, neighbours, fields_to_exchange);
+}
+
+int tea_hip_pack_or_unpack(tea_hip_chunk* c, int depth, int face, int pack,
+                           const double* field, double* host_buffer) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(depth >= 1 && depth <= c->geo.h, "depth must be 1..halo_depth");
+  TEA_CHECK_ARG(face >= TEA_FACE_LEFT && face <= TEA_FACE_TOP, "bad face");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  int fid = -1;
+  for (int f = 0; f < TEA_NUM_FIELDS; ++f)
+    if (field == c->field[f] && c->field_len[f] >= (size_t)c->geo.n) fid = f;
+  TEA_CHECK_ARG(fid >= 0, "field is not an x*y field of this chunk");
+  TEA_RETURN_IF(apply_pending(c));  // may move the field to its second buffer
+  const long n = (face == TEA_FACE_LEFT || face == TEA_FACE_RIGHT)
+                     ? (long)c->geo.y * depth : (long)c->geo.x * depth;
+  TEA_CHECK_ARG(host_buffer != nullptr || n == 0, "null host buffer");
+  TEA_CHECK_ARG((size_t)n <= c->field_len[TEA_FIELD_COMMS_BUFFER],
+                "strip longer than the comms buffer");
+  double* strip = c->field[TEA_FIELD_COMMS_BUFFER];
+  const size_t bytes = (size_t)n * sizeof(double);
+  if (!pack)
+    TEA_HIP_TRY(hipMemcpyAsync(strip, host_buffer, bytes, hipMemcpyHostToDevice,
+                               c->stream));
+  {
+    Timed t(c, K_LOCAL_HALOS);
+    hipLaunchKernelGGL(strip_kernel, flat_grid(n), dim3(kBlock), 0, c->stream,
+                       c->geo, depth, face, pack, c->field[fid], strip);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  if (pack)
+    TEA_HIP_TRY(hipMemcpyAsync(host_buffer, strip, bytes, hipMemcpyDeviceToHost,
+                               c->stream));
+  if (!pack && fid == TEA_FIELD_P) c->p_alt_synced = false;
+  return sync_chunk(c);
 }
 
 int tea_hip_field_summary(tea_hip_chunk* c, double* vol, double* mass,

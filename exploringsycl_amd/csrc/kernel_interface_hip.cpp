@@ -169,11 +169,10 @@ void run_local_halos(Chunk* chunk, Settings* settings, int depth) {
 
 void run_pack_or_unpack(Chunk* chunk, Settings* settings, int depth, int face,
                         bool pack, FieldBufferType field, double* buffer) {
-  (void)chunk; (void)settings; (void)depth; (void)face; (void)pack;
-  (void)field; (void)buffer;
-  die(__LINE__, __FILE__,
-      "run_pack_or_unpack: remote halo exchange is not part of this "
-      "single-GPU backend\n");
+  (void)settings;
+  HIP_CALL(tea_hip_pack_or_unpack(H(chunk), depth, face, pack ? 1 : 0, field,
+                                  buffer));
+  refresh_fields(chunk);
 }
 
 void run_store_energy(Chunk* chunk, Settings* settings) {

@@ -143,6 +143,18 @@ int tea_hip_local_halos(tea_hip_chunk* chunk, int depth,
                         const int neighbours[4],
                         const int fields_to_exchange[6]);
 
+/* Remote-halo strip of one field (one of the handle's fields, by device
+ * address): pack != 0 copies the `depth` rows or columns next to the
+ * interior on `face` (TEA_FACE_*) into host_buffer; pack == 0 copies
+ * host_buffer into that face's halo cells.  The strip has y*depth values
+ * for the left and right faces (row by row) and x*depth for the bottom and
+ * top, in the reference's order, staged through the comms buffer.  Blocks
+ * until the copy is done.  Replaces run_pack_or_unpack
+ * (TeaLeaf/kernel_interface.h:25-27; kernel_interface.cpp:129-170,
+ * pack_halos.cpp:7-184). */
+int tea_hip_pack_or_unpack(tea_hip_chunk* chunk, int depth, int face,
+                           int pack, const double* field, double* host_buffer);
+
 /* Interior sums vol = sum V, mass = sum V*rho, ie = sum V*rho*energy0,
  * temp = sum V*rho*u.  Replaces run_field_summary
  * (TeaLeaf/kernel_interface.h:29-31; field_summary.cpp:6-151). */

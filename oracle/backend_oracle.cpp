@@ -123,9 +123,8 @@ void run_local_halos(Chunk* c, Settings* s, int depth) {
 
 void run_pack_or_unpack(Chunk* c, Settings* s, int depth, int face, bool pack,
                         FieldBufferType field, double* buffer) {
-  (void)c; (void)s; (void)depth; (void)face; (void)pack; (void)field;
-  (void)buffer;
-  die(__LINE__, __FILE__, "run_pack_or_unpack: not part of the oracle\n");
+  orc_pack_or_unpack(c->x, c->y, s->halo_depth, depth, face, pack ? 1 : 0,
+                     field, buffer);
 }
 
 void run_store_energy(Chunk* c, Settings* s) {

@@ -28,6 +28,9 @@
  *     tea.in rectangles fall on cell edges;
  *   - reduction tree: inputs whose pairwise-tree sum differs from the
  *     sequential sum by a value worked out by hand, and exact integer sums;
+ *   - orc_pack_or_unpack: each face's strip equals the 2-D slice of the
+ *     columns (rows) next to the interior, or of the halo, in row-major
+ *     order;
  *   - orc_jacobi_*: jacobi_init's coefficients equal cg_init's, the dense
  *     solution is a fixed point of a sweep, sweeps contract in the max norm,
  *     r is the previous iterate and the error the L1 norm of the update;
@@ -375,6 +378,32 @@ void orc_finalise(int x, int y, int h, const double* u, const double* density,
       const long i = kk + jj * x;
       energy[i] = u[i] / density[i];
     }
+}
+
+/* run_pack_or_unpack (kernel_interface.cpp:129-170, pack_halos.cpp:7-184):
+ * the face strip of `depth` cells next to the interior (pack) or in the halo
+ * (unpack), copied to or from `buffer` in the reference's order.  Faces as
+ * the host's neighbours: 0 left, 1 right, 2 bottom, 3 top. */
+void orc_pack_or_unpack(int x, int y, int h, int depth, int face, int pack,
+                        double* field, double* buffer) {
+  const long n = (face == 0 || face == 1) ? (long)y * depth : (long)x * depth;
+  for (long i = 0; i < n; ++i) {
+    long f;
+    const long lines = i / depth;
+    if (face == 0)      /* pack_left / unpack_left */
+      f = pack ? h + lines * (x - depth) + i : h - depth + lines * (x - depth) + i;
+    else if (face == 1) /* pack_right / unpack_right */
+      f = pack ? x - h - depth + lines * (x - depth) + i
+               : x - h + lines * (x - depth) + i;
+    else if (face == 2) /* pack_bottom / unpack_bottom */
+      f = pack ? (long)x * h + i : (long)x * (h - depth) + i;
+    else                /* pack_top / unpack_top */
+      f = pack ? (long)x * (y - h - depth) + i : (long)x * (y - h) + i;
+    if (pack)
+      buffer[i] = field[f];
+    else
+      field[f] = buffer[i];
+  }
 }
 
 /* ---- Jacobi, Chebyshev and PPCG -------------------------------------------

@@ -1,5 +1,6 @@
 /*
- * CPU oracle for the TeaLeaf conjugate-gradient path.
+ * CPU oracle for the TeaLeaf conjugate-gradient path and the reference's
+ * other solvers (Jacobi, Chebyshev, PPCG).
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the
  * kernel_interface adapter or the host driver) may link or call this code;
@@ -20,8 +21,9 @@
  * value is left.  The build compiles this file with -ffp-contract=off so each
  * multiply and add rounds on its own, as written.
  *
- * Scalar results are RETURNED (assigned); the kernel_interface adapters add
- * them into the driver's accumulators.
+ * Scalar results are RETURNED; the kernel_interface adapter
+ * (backend_oracle.cpp) adds or assigns them as the reference's SYCL
+ * backend does.
  *
  * Parity status: see the header of tea_oracle.c.
  */
@@ -95,6 +97,11 @@ void orc_calculate_residual(int x, int y, int h, const double* u,
 double orc_calculate_2norm(int x, int y, int h, const double* b);
 void orc_finalise(int x, int y, int h, const double* u, const double* density,
                   double* energy);
+
+/* pack_halos.cpp: face 0 left, 1 right, 2 bottom, 3 top; pack != 0 copies
+ * field -> buffer, else buffer -> field. */
+void orc_pack_or_unpack(int x, int y, int h, int depth, int face, int pack,
+                        double* field, double* buffer);
 
 /* jacobi.cpp, cheby.cpp, ppcg.cpp, composed as kernel_interface.cpp:246-328
  * composes them.  orc_jacobi_iterate returns the error sum |u_new - u_old|. */

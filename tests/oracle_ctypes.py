@@ -45,6 +45,7 @@ class Oracle:
             "orc_calculate_2norm": ([_I, _I, _I, _P], _D),
             "orc_finalise": ([_I, _I, _I, _P, _P, _P], None),
             "orc_smvp": ([_P, _P, _P, ctypes.c_long, ctypes.c_long], _D),
+            "orc_pack_or_unpack": ([_I] * 6 + [_P, _P], None),
             "orc_jacobi_init": ([_I, _I, _I, _I, _D, _D] + [_P] * 6, None),
             "orc_jacobi_iterate": ([_I, _I, _I] + [_P] * 5, _D),
             "orc_cheby_init": ([_I, _I, _I, _D] + [_P] * 7, None),
@@ -118,6 +119,10 @@ class Oracle:
 
     def finalise(self, x, y, h, f):
         self.lib.orc_finalise(x, y, h, _p(f["u"]), _p(f["density"]), _p(f["energy"]))
+
+    def pack_or_unpack(self, x, y, h, depth, face, pack, field, buffer):
+        self.lib.orc_pack_or_unpack(x, y, h, depth, face, 1 if pack else 0, _p(field),
+                                    _p(buffer))
 
     def jacobi_init(self, x, y, h, coef, rx, ry, f):
         self.lib.orc_jacobi_init(x, y, h, coef, rx, ry, _p(f["u"]), _p(f["u0"]),

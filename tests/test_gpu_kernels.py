@@ -350,3 +350,26 @@ def test_copy_bandwidth_is_a_plausible_rate():
     assert 1000.0 < gbs.value < 8000.0, gbs.value
     assert lib.tea_hip_copy_bandwidth(0, 17, 1, ctypes.byref(gbs)) != 0
     assert lib.tea_hip_copy_bandwidth(0, 256, 1, None) != 0
+
+
+@pytest.mark.parametrize("cells", SIZES)
+@pytest.mark.parametrize("depth", [1, 2])
+def test_pack_or_unpack(oracle, cells, depth):
+    """Remote-halo strips (kernel_interface.cpp:129-170): every face, both
+    directions, through the host buffer."""
+    xc, yc = cells
+    x, y = xc + 2 * H, yc + 2 * H
+    f = _random_fields(x, y, 31)
+    with _chunk(x, y) as c:
+        _upload(c, f, ["p"])
+        for face in range(4):
+            n = (y if face < 2 else x) * depth
+            got = np.zeros(n)
+            want = np.zeros(n)
+            c.pack_or_unpack(depth, face, True, "p", got)
+            oracle.pack_or_unpack(x, y, H, depth, face, True, f["p"], want)
+            assert np.array_equal(got.view(np.int64), want.view(np.int64)), face
+            strip = np.random.default_rng(face).standard_normal(n)
+            c.pack_or_unpack(depth, face, False, "p", strip)
+            oracle.pack_or_unpack(x, y, H, depth, face, False, f["p"], strip.copy())
+        _assert_fields(c, f, ["p"])

@@ -140,3 +140,29 @@ def test_smvp_single_cell_by_hand(oracle):
     got = oracle.lib.orc_smvp(a.ctypes.data_as(P), k.ctypes.data_as(P),
                               k.ctypes.data_as(P), i, x)
     assert got == want
+
+
+@pytest.mark.parametrize("depth", [1, 2])
+def test_pack_unpack_strips_are_the_face_slices(oracle, depth):
+    """pack_halos.cpp's linear offsets name, for each face, the `depth`
+    columns (rows) next to the interior (pack) or in the halo (unpack), row
+    by row: the 2-D slices below."""
+    x, y = 13, 9
+    rng = np.random.default_rng(depth)
+    field = rng.standard_normal(x * y)
+    g = field.reshape(y, x)
+    want = {0: g[:, H:H + depth], 1: g[:, x - H - depth:x - H],
+            2: g[H:H + depth, :], 3: g[y - H - depth:y - H, :]}
+    for face, sl in want.items():
+        buf = np.zeros(sl.size)
+        oracle.pack_or_unpack(x, y, H, depth, face, True, field, buf)
+        assert np.array_equal(buf, sl.ravel()), face
+    halo = {0: (slice(None), slice(H - depth, H)), 1: (slice(None), slice(x - H, x - H + depth)),
+            2: (slice(H - depth, H), slice(None)), 3: (slice(y - H, y - H + depth), slice(None))}
+    for face, idx in halo.items():
+        f2 = field.copy()
+        buf = rng.standard_normal(g[idx].size)
+        oracle.pack_or_unpack(x, y, H, depth, face, False, f2, buf)
+        expect = g.copy()
+        expect[idx] = buf.reshape(expect[idx].shape)
+        assert np.array_equal(f2.reshape(y, x), expect), face
