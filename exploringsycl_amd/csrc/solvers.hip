@@ -346,7 +346,10 @@ bool fused_pairs(const tea_hip_chunk* c) { return c->fused && c->pairs; }
 //
 // Steps k and k+1 (ppcg.cpp:34-94) with the depth-1 reflective SD update
 // between them (local_halos.cpp:7-102).  A block owns a tile of kTileY rows
-// x kTileX columns of the interior; each thread a column of kRows of them:
+// x kTileX columns of the interior; each thread kRows rows of one column
+// (one cell at 16 x 64 with 1024 threads, which ran 10-15% faster than
+// 8 x 128 with 256 threads and four rows each) and at most one cell of the
+// margin below.  Every global read is issued before the first barrier.
 //   1. sd_k on the tile and two cells around it is staged in LDS;
 //   2. sd_(k+1) = a0 sd_k + b0 (r_k - A sd_k) is formed on the tile and one
 //      cell around it (the tile's r_(k+1) stays in registers);
@@ -360,13 +363,40 @@ bool fused_pairs(const tea_hip_chunk* c) { return c->fused && c->pairs; }
 // 64 bytes per cell for two steps against 128 for two single passes.  Every
 // value uses the single-step kernel's expressions, so the result is bitwise
 // that of two single steps with the halo update between them.
-constexpr int kTileX = 128;
-constexpr int kTileY = 8;
-constexpr int kRows = kTileY * kTileX / kBlock;  // rows per thread
+constexpr int kTileX = 64;
+constexpr int kTileY = 16;
+constexpr int kTSBlock = 1024;                     // threads per tile
+constexpr int kRows = kTileY * kTileX / kTSBlock;  // rows per thread
 constexpr int kS0W = kTileX + 4, kS0H = kTileY + 4;
 constexpr int kS1W = kTileX + 2, kS1H = kTileY + 2;
+constexpr int kMargin = 2 * kS1W + 2 * kTileY;     // step-1 margin cells
+constexpr int kMarginPer = (kMargin + kTSBlock - 1) / kTSBlock;
+static_assert(kTileX % kWave == 0 && kTSBlock % kTileX == 0, "tile shape");
+static_assert(kMarginPer == 1, "one margin cell per thread");
 
-__global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
+// The tile of this block.  The hardware deals blocks to the 8 XCDs in
+// turn (block b to XCD b % 8); XCD k takes the k-th eighth of the tiles in
+// row-major order, so a tile's neighbours, whose rows and boundary cache
+// lines it re-reads, sit in the same L2.
+__device__ inline int tile_of_block(int nt) {
+  const int b = blockIdx.x, k = b & 7, per = nt >> 3, rem = nt & 7;
+  return k * per + min(k, rem) + (b >> 3);
+}
+
+// Position (a, b) in the step-1 area of margin slot q: rows 0 and kS1H-1
+// over all columns, then columns 0 and kS1W-1 over the rows between.
+__device__ inline void margin_pos(int q, int& a, int& b) {
+  if (q < 2 * kS1W) {
+    a = q < kS1W ? 0 : kS1H - 1;
+    b = q < kS1W ? q : q - kS1W;
+  } else {
+    const int e = q - 2 * kS1W;
+    a = 1 + (e >> 1);
+    b = (e & 1) ? kS1W - 1 : 0;
+  }
+}
+
+__global__ __launch_bounds__(kTSBlock) void ppcg_two_step_kernel(
     Geo g, const double* __restrict__ sd_old, double* __restrict__ sd_new,
     const double* __restrict__ r, double* __restrict__ r_new,
     double* __restrict__ u, const double* __restrict__ kx,
@@ -379,21 +409,23 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
   double (*S1)[kS1W] = reinterpret_cast<double (*)[kS1W]>(lds);
   const int x = g.x, y = g.y, h = g.h;
   const int xe = x - h, ye = y - h;  // ends of the interior (exclusive)
-  const int txi = blockIdx.x % tiles_x, tyi = blockIdx.x / tiles_x;
+  const int tile = tile_of_block(tiles_x * tiles_y);
+  const int txi = tile % tiles_x, tyi = tile / tiles_x;
   const int c0 = h + txi * kTileX, r0 = h + tyi * kTileY;
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1);
   const bool edge = txi == 0 || tyi == 0 || txi == tiles_x - 1 || tyi == tiles_y - 1;
 
-  // 1. All global reads of the tile's own column (column c0 + t) and of
-  // sd_k for the staging area are issued before the block waits once: r,
-  // kx (through the right boundary face; lane 63 also reads its right
-  // neighbour's) and ky (cells and the row above).
+  // 1. All global reads are issued before the block waits once: for the
+  // tile's own column (column c0 + t) r, u, kx (through the right boundary
+  // face; lane 63 also reads its right neighbour's) and ky (cells and the
+  // row above); for the margin cells r, kx and ky; sd_k for the staging
+  // area.
   const int tc = t % kTileX, tr = (t / kTileX) * kRows;
   const int kk = c0 + tc;
-  double rk1[kRows], kx0[kRows], kxe[kRows], kyv[kRows + 1];
+  double rk1[kRows], uo[kRows], kx0[kRows], kxe[kRows], kyv[kRows + 1];
 #pragma unroll
-  for (int m = 0; m < kRows; ++m) rk1[m] = kx0[m] = kxe[m] = kyv[m] = 0.0;
+  for (int m = 0; m < kRows; ++m) rk1[m] = uo[m] = kx0[m] = kxe[m] = kyv[m] = 0.0;
   kyv[kRows] = 0.0;
 #pragma unroll
   for (int m = 0; m < kRows; ++m) {
@@ -402,17 +434,39 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
     if (kk < xe && jj < ye) {
       const int i = kk + jj * x;
       rk1[m] = r[i];
+      uo[m] = u[i];
       if (lane == kWave - 1) kxe[m] = kx[i + 1];
       kyv[m] = ky[i];
       kyv[m + 1] = ky[i + x];
     }
   }
+  double mr[kMarginPer], mkx0[kMarginPer], mkx1[kMarginPer], mky0[kMarginPer],
+      mky1[kMarginPer];
+  int mpos[kMarginPer];
+#pragma unroll
+  for (int n = 0; n < kMarginPer; ++n) {
+    const int q = t + n * kTSBlock;
+    int a = 0, b = 0;
+    margin_pos(q, a, b);
+    const int jj = r0 - 1 + a, kc = c0 - 1 + b;
+    mr[n] = mkx0[n] = mkx1[n] = mky0[n] = mky1[n] = 0.0;
+    mpos[n] = -1;
+    if (q < kMargin && jj >= h && jj < ye && kc >= h && kc < xe) {
+      const int i = kc + jj * x;
+      mr[n] = r[i];
+      mkx0[n] = kx[i];
+      mkx1[n] = kx[i + 1];
+      mky0[n] = ky[i];
+      mky1[n] = ky[i + x];
+      mpos[n] = a * kS1W + b;
+    }
+  }
   // Stage sd_k (cells outside the field are never used: zero).
-  constexpr int kStage = (kS0H * kS0W + kBlock - 1) / kBlock;
+  constexpr int kStage = (kS0H * kS0W + kTSBlock - 1) / kTSBlock;
   double st[kStage];
 #pragma unroll
   for (int n = 0; n < kStage; ++n) {
-    const int q = t + n * kBlock;
+    const int q = t + n * kTSBlock;
     const int a = q / kS0W, b = q - a * kS0W;
     const int jj = r0 - 2 + a, kc = c0 - 2 + b;
     st[n] = (q < kS0H * kS0W && jj >= 0 && jj < y && kc >= 0 && kc < x)
@@ -421,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
   }
 #pragma unroll
   for (int n = 0; n < kStage; ++n) {
-    const int q = t + n * kBlock;
+    const int q = t + n * kTSBlock;
     if (q < kS0H * kS0W) lds[q] = st[n];
   }
   __syncthreads();
@@ -445,48 +499,31 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
     }
   }
   // ... and on the one-cell margin: rows -1 and kTileY over columns
-  // -1 .. kTileX, then columns -1 and kTileX over the tile's rows.
-  constexpr int kMargin = 2 * kS1W + 2 * kTileY;
-  constexpr int kMarginPer = (kMargin + kBlock - 1) / kBlock;
+  // -1 .. kTileX, then columns -1 and kTileX over the tile's rows.  Margin
+  // cells outside the interior stay zero here; step 3 mirrors them.
   double s1m[kMarginPer];
-  int s1pos[kMarginPer];
 #pragma unroll
   for (int n = 0; n < kMarginPer; ++n) {
-    const int q = t + n * kBlock;
-    int a = 0, b = 0;
-    if (q < 2 * kS1W) {
-      a = q < kS1W ? 0 : kS1H - 1;
-      b = q < kS1W ? q : q - kS1W;
-    } else {
-      const int e = q - 2 * kS1W;
-      a = 1 + (e >> 1);
-      b = (e & 1) ? kS1W - 1 : 0;
-    }
-    const int jj = r0 - 1 + a, kc = c0 - 1 + b;
-    double s1 = 0.0;
-    if (q < kMargin && jj >= h && jj < ye && kc >= h && kc < xe) {
-      const int i = kc + jj * x;
-      const double c = S0[a + 1][b + 1];
-      const double rn = r[i] - op5(c, S0[a + 1][b], S0[a + 1][b + 2], S0[a][b + 1],
-                                   S0[a + 2][b + 1], kx[i], kx[i + 1], ky[i],
-                                   ky[i + x]);
-      s1 = a0 * c + b0 * rn;
-    }
-    s1m[n] = s1;
-    s1pos[n] = q < kMargin ? a * kS1W + b : -1;
+    const int p = mpos[n] < 0 ? 0 : mpos[n];
+    const int a = p / kS1W, b = p - a * kS1W;
+    const double c = S0[a + 1][b + 1];
+    const double rn = mr[n] - op5(c, S0[a + 1][b], S0[a + 1][b + 2], S0[a][b + 1],
+                                  S0[a + 2][b + 1], mkx0[n], mkx1[n], mky0[n],
+                                  mky1[n]);
+    s1m[n] = a0 * c + b0 * rn;
   }
   __syncthreads();  // every read of sd_k is done: S1 replaces it
 #pragma unroll
   for (int m = 0; m < kRows; ++m) S1[tr + m + 1][tc + 1] = s1own[m];
 #pragma unroll
   for (int n = 0; n < kMarginPer; ++n)
-    if (s1pos[n] >= 0) lds[s1pos[n]] = s1m[n];
+    if (mpos[n] >= 0) lds[mpos[n]] = s1m[n];
   __syncthreads();
 
   // 3. Ring cells of the margin take their mirror's value (double mirror at
   // the corners, as the halo update's left/right-then-bottom/top order gives).
   if (edge) {
-    for (int q = t; q < kS1H * kS1W; q += kBlock) {
+    for (int q = t; q < kS1H * kS1W; q += kTSBlock) {
       const int a = q / kS1W, b = q - a * kS1W;
       const int jj = r0 - 1 + a, kc = c0 - 1 + b;
       const bool ring_box = jj >= h - 1 && jj <= ye && kc >= h - 1 && kc <= xe;
@@ -512,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
                                      S1[tr + m + 2][tc + 1], kx0[m], kx1, kyv[m],
                                      kyv[m + 1]);
       r_new[i] = rn;
-      u[i] = (u[i] + sdk[m]) + c;
+      u[i] = (uo[m] + sdk[m]) + c;
       sd_new[i] = a1 * c + b1 * rn;
     }
   }
@@ -529,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
     const int ja = tyi == 0 ? 0 : r0;
     const int jb = tyi == tiles_y - 1 ? y : min(r0 + kTileY, ye);
     const int w = kb - ka;
-    for (int q = t; q < w * (jb - ja); q += kBlock) {
+    for (int q = t; q < w * (jb - ja); q += kTSBlock) {
       const int jj = ja + q / w, kc = ka + q % w;
       if (jj >= h && jj < ye && kc >= h && kc < xe) continue;
       const int mj = jj == h - 1 ? h : (jj == ye ? ye - 1 : jj);
@@ -551,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void ppcg_two_step_kernel(
 // old ones); w and r, written only at a tile's own cells and never read,
 // stay in place and end as the second step's.  72 bytes per cell for two
 // steps against 144 for two single passes.
-__global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
+__global__ __launch_bounds__(kTSBlock) void cheby_two_step_kernel(
     Geo g, const double* __restrict__ u_old, double* __restrict__ u_new,
     const double* __restrict__ u0, const double* __restrict__ p_old,
     double* __restrict__ p_new, double* __restrict__ r, double* __restrict__ w,
@@ -562,14 +599,16 @@ __global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
   double (*S1)[kS1W] = reinterpret_cast<double (*)[kS1W]>(lds);
   const int x = g.x, y = g.y, h = g.h;
   const int xe = x - h, ye = y - h;
-  const int txi = blockIdx.x % tiles_x, tyi = blockIdx.x / tiles_x;
+  const int tile = tile_of_block(tiles_x * tiles_y);
+  const int txi = tile % tiles_x, tyi = tile / tiles_x;
   const int c0 = h + txi * kTileX, r0 = h + tyi * kTileY;
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1);
   const bool edge = txi == 0 || tyi == 0 || txi == tiles_x - 1 || tyi == tiles_y - 1;
 
   // 1. The tile's own column (u0, p_k, kx through the right boundary face,
-  // ky with the row above) and u_k for the staging area, before one wait.
+  // ky with the row above), the margin cells' u0, p_k, kx and ky, and u_k
+  // for the staging area, before one wait.
   const int tc = t % kTileX, tr = (t / kTileX) * kRows;
   const int kk = c0 + tc;
   double bv[kRows], pk[kRows], kx0[kRows], kxe[kRows], kyv[kRows + 1];
@@ -589,11 +628,33 @@ __global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
       kyv[m + 1] = ky[i + x];
     }
   }
-  constexpr int kStage = (kS0H * kS0W + kBlock - 1) / kBlock;
+  double mb[kMarginPer], mp[kMarginPer], mkx0[kMarginPer], mkx1[kMarginPer],
+      mky0[kMarginPer], mky1[kMarginPer];
+  int mpos[kMarginPer];
+#pragma unroll
+  for (int n = 0; n < kMarginPer; ++n) {
+    const int q = t + n * kTSBlock;
+    int a = 0, b = 0;
+    margin_pos(q, a, b);
+    const int jj = r0 - 1 + a, kc = c0 - 1 + b;
+    mb[n] = mp[n] = mkx0[n] = mkx1[n] = mky0[n] = mky1[n] = 0.0;
+    mpos[n] = -1;
+    if (q < kMargin && jj >= h && jj < ye && kc >= h && kc < xe) {
+      const int i = kc + jj * x;
+      mb[n] = u0[i];
+      mp[n] = p_old[i];
+      mkx0[n] = kx[i];
+      mkx1[n] = kx[i + 1];
+      mky0[n] = ky[i];
+      mky1[n] = ky[i + x];
+      mpos[n] = a * kS1W + b;
+    }
+  }
+  constexpr int kStage = (kS0H * kS0W + kTSBlock - 1) / kTSBlock;
   double st[kStage];
 #pragma unroll
   for (int n = 0; n < kStage; ++n) {
-    const int q = t + n * kBlock;
+    const int q = t + n * kTSBlock;
     const int a = q / kS0W, b = q - a * kS0W;
     const int jj = r0 - 2 + a, kc = c0 - 2 + b;
     st[n] = (q < kS0H * kS0W && jj >= 0 && jj < y && kc >= 0 && kc < x)
@@ -602,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
   }
 #pragma unroll
   for (int n = 0; n < kStage; ++n) {
-    const int q = t + n * kBlock;
+    const int q = t + n * kTSBlock;
     if (q < kS0H * kS0W) lds[q] = st[n];
   }
   __syncthreads();
@@ -625,46 +686,28 @@ __global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
     }
   }
   // ... and on the one-cell margin (as in ppcg_two_step_kernel).
-  constexpr int kMargin = 2 * kS1W + 2 * kTileY;
-  constexpr int kMarginPer = (kMargin + kBlock - 1) / kBlock;
   double u1m[kMarginPer];
-  int u1pos[kMarginPer];
 #pragma unroll
   for (int n = 0; n < kMarginPer; ++n) {
-    const int q = t + n * kBlock;
-    int a = 0, b = 0;
-    if (q < 2 * kS1W) {
-      a = q < kS1W ? 0 : kS1H - 1;
-      b = q < kS1W ? q : q - kS1W;
-    } else {
-      const int e = q - 2 * kS1W;
-      a = 1 + (e >> 1);
-      b = (e & 1) ? kS1W - 1 : 0;
-    }
-    const int jj = r0 - 1 + a, kc = c0 - 1 + b;
-    double u1 = 0.0;
-    if (q < kMargin && jj >= h && jj < ye && kc >= h && kc < xe) {
-      const int i = kc + jj * x;
-      const double c = S0[a + 1][b + 1];
-      const double wv = op5(c, S0[a + 1][b], S0[a + 1][b + 2], S0[a][b + 1],
-                            S0[a + 2][b + 1], kx[i], kx[i + 1], ky[i], ky[i + x]);
-      const double rv = u0[i] - wv;
-      u1 = c + (a0 * p_old[i] + b0 * rv);
-    }
-    u1m[n] = u1;
-    u1pos[n] = q < kMargin ? a * kS1W + b : -1;
+    const int q = mpos[n] < 0 ? 0 : mpos[n];
+    const int a = q / kS1W, b = q - a * kS1W;
+    const double c = S0[a + 1][b + 1];
+    const double wv = op5(c, S0[a + 1][b], S0[a + 1][b + 2], S0[a][b + 1],
+                          S0[a + 2][b + 1], mkx0[n], mkx1[n], mky0[n], mky1[n]);
+    const double rv = mb[n] - wv;
+    u1m[n] = c + (a0 * mp[n] + b0 * rv);
   }
   __syncthreads();  // every read of u_k is done: u_(k+1) replaces it
 #pragma unroll
   for (int m = 0; m < kRows; ++m) S1[tr + m + 1][tc + 1] = u1own[m];
 #pragma unroll
   for (int n = 0; n < kMarginPer; ++n)
-    if (u1pos[n] >= 0) lds[u1pos[n]] = u1m[n];
+    if (mpos[n] >= 0) lds[mpos[n]] = u1m[n];
   __syncthreads();
 
   // 3. Ring cells of the margin take their mirror's value.
   if (edge) {
-    for (int q = t; q < kS1H * kS1W; q += kBlock) {
+    for (int q = t; q < kS1H * kS1W; q += kTSBlock) {
       const int a = q / kS1W, b = q - a * kS1W;
       const int jj = r0 - 1 + a, kc = c0 - 1 + b;
       const bool ring_box = jj >= h - 1 && jj <= ye && kc >= h - 1 && kc <= xe;
@@ -705,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void cheby_two_step_kernel(
     const int ja = tyi == 0 ? 0 : r0;
     const int jb = tyi == tiles_y - 1 ? y : min(r0 + kTileY, ye);
     const int wd = kb - ka;
-    for (int q = t; q < wd * (jb - ja); q += kBlock) {
+    for (int q = t; q < wd * (jb - ja); q += kTSBlock) {
       const int jj = ja + q / wd, kc = ka + q % wd;
       if (jj >= h && jj < ye && kc >= h && kc < xe) continue;
       const int mj = jj == h - 1 ? h : (jj == ye ? ye - 1 : jj);
@@ -768,7 +811,7 @@ int cheby_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
   {
     Timed t(c, K_CHEBY_TWO_STEP);
     hipLaunchKernelGGL(cheby_two_step_kernel, dim3(tiles_x * tiles_y),
-                       dim3(kBlock), 0, c->stream, g, f[TEA_FIELD_U], c->spare,
+                       dim3(kTSBlock), 0, c->stream, g, f[TEA_FIELD_U], c->spare,
                        f[TEA_FIELD_U0], f[TEA_FIELD_P], c->spare2, f[TEA_FIELD_R],
                        f[TEA_FIELD_W], f[TEA_FIELD_KX], f[TEA_FIELD_KY], a0, b0,
                        a1, b1, tiles_x, tiles_y);
@@ -819,7 +862,7 @@ int ppcg_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
   {
     Timed t(c, K_PPCG_TWO_STEP);
     hipLaunchKernelGGL(ppcg_two_step_kernel, dim3(tiles_x * tiles_y),
-                       dim3(kBlock), 0, c->stream, g, f[TEA_FIELD_SD], c->spare,
+                       dim3(kTSBlock), 0, c->stream, g, f[TEA_FIELD_SD], c->spare,
                        f[TEA_FIELD_R], c->spare2, f[TEA_FIELD_U], f[TEA_FIELD_KX],
                        f[TEA_FIELD_KY], a0, b0, a1, b1, tiles_x, tiles_y);
   }

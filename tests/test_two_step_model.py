@@ -3,7 +3,9 @@ cheby_two_step_kernel in exploringsycl_amd/csrc/solvers.hip): the same tile
 shape, staging, margin, ring-mirror and edge-ownership index arithmetic and
 the same expressions, one block at a time.  Checked bitwise against two
 oracle steps with the depth-1 halo update between them, with every field
-access bounds-checked and every output cell written exactly once.  The GPU
+access bounds-checked, every output cell written exactly once, and the
+margin cells outside the interior, which the kernels leave unset, never
+read.  The GPU
 tests check the kernels themselves; this pins the index logic on the CPU."""
 
 import os
@@ -28,6 +30,17 @@ TX, TY = _tile_shape()
 S0W, S0H, S1W, S1H = TX + 4, TY + 4, TX + 2, TY + 2
 
 
+def tile_of_block(b, nt):
+    """Block b's tile: XCD k = b % 8 takes the k-th eighth of the tiles."""
+    k, per, rem = b & 7, nt >> 3, nt & 7
+    return k * per + min(k, rem) + (b >> 3)
+
+
+@pytest.mark.parametrize("nt", [1, 5, 8, 9, 63, 64, 65, 1000, 4096 // 8 * 32 + 3])
+def test_tile_of_block_is_a_permutation(nt):
+    assert sorted(tile_of_block(b, nt) for b in range(nt)) == list(range(nt))
+
+
 def op5(c, l, r, b, t, kx0, kx1, ky0, ky1):
     diag = (1.0 + (kx1 + kx0)) + (ky1 + ky0)
     return (diag * c - (kx1 * r + kx0 * l)) - (ky1 * t + ky0 * b)
@@ -45,7 +58,8 @@ def model_ppcg(x, y, h, sd_old, sd_new, r, r_new, u, kx, ky, a0, b0, a1, b1):
     tiles_y = (y - 2 * h + TY - 1) // TY
     written = np.zeros(n, dtype=int)
     for blk in range(tiles_x * tiles_y):
-        txi, tyi = blk % tiles_x, blk // tiles_x
+        tile = tile_of_block(blk, tiles_x * tiles_y)
+        txi, tyi = tile % tiles_x, tile // tiles_x
         c0, r0 = h + txi * TX, h + tyi * TY
         edge = txi == 0 or tyi == 0 or txi == tiles_x - 1 or tyi == tiles_y - 1
         S0 = np.full((S0H, S0W), np.nan); S1 = np.full((S1H, S1W), np.nan)
@@ -75,13 +89,11 @@ def model_ppcg(x, y, h, sd_old, sd_new, r, r_new, u, kx, ky, a0, b0, a1, b1):
             else:
                 e = q - 2 * S1W; a = 1 + (e >> 1); b = S1W - 1 if (e & 1) else 0
             jj, kc = r0 - 1 + a, c0 - 1 + b
-            s1 = 0.0
             if h <= jj < ye and h <= kc < xe:
                 i = kc + jj * x
                 c = S0[a + 1][b + 1]
                 rn = G(r, i) - op5(c, S0[a + 1][b], S0[a + 1][b + 2], S0[a][b + 1], S0[a + 2][b + 1], G(kx, i), G(kx, i + 1), G(ky, i), G(ky, i + x))
-                s1 = a0 * c + b0 * rn
-            S1[a][b] = s1
+                S1[a][b] = a0 * c + b0 * rn
         if edge:
             upd = {}
             for q in range(S1H * S1W):
@@ -145,7 +157,8 @@ def model_cheby(x, y, h, u_old, u_new, u0, p_old, p_new, r, w, kx, ky, a0, b0, a
     tiles_x = (x - 2 * h + TX - 1) // TX; tiles_y = (y - 2 * h + TY - 1) // TY
     wu = np.zeros(n, dtype=int)
     for blk in range(tiles_x * tiles_y):
-        txi, tyi = blk % tiles_x, blk // tiles_x
+        tile = tile_of_block(blk, tiles_x * tiles_y)
+        txi, tyi = tile % tiles_x, tile // tiles_x
         c0, r0 = h + txi * TX, h + tyi * TY
         edge = txi == 0 or tyi == 0 or txi == tiles_x - 1 or tyi == tiles_y - 1
         S0 = np.full((S0H, S0W), np.nan)
@@ -178,14 +191,12 @@ def model_cheby(x, y, h, u_old, u_new, u0, p_old, p_new, r, w, kx, ky, a0, b0, a
             else:
                 e = q - 2 * S1W; a = 1 + (e >> 1); b = S1W - 1 if (e & 1) else 0
             jj, kc = r0 - 1 + a, c0 - 1 + b
-            u1 = 0.0
             if h <= jj < ye and h <= kc < xe:
                 i = kc + jj * x
                 c = S0[a + 1][b + 1]
                 wv = op5(c, S0[a + 1][b], S0[a + 1][b + 2], S0[a][b + 1], S0[a + 2][b + 1], G(kx, i), G(kx, i + 1), G(ky, i), G(ky, i + x))
                 rv = G(u0, i) - wv
-                u1 = c + (a0 * G(p_old, i) + b0 * rv)
-            S1[a][b] = u1
+                S1[a][b] = c + (a0 * G(p_old, i) + b0 * rv)
         if edge:
             upd = {}
             for q in range(S1H * S1W):
