@@ -402,7 +402,7 @@ __device__ inline double fused_pair_compute(const Geo& g, const FusedWArgs& a,
                     (s.KYt.y * tb + s.KY.y * bb);
   const double psum = (s.in0 ? sa * ca : 0.0) + (s.in1 ? sb * cb : 0.0);
   if (!s.any) return psum;
-  st2_nt(a.w, i, s.in0, s.in1, sa, sb);
+  st2(a.w, i, s.in0, s.in1, sa, sb);
   double ua = 0.0, ub = 0.0;
   if (a.apply_u) {
     ua = s.U.x + a.alpha * s.P0.x;

@@ -207,7 +207,7 @@ __device__ inline void mp_pair_load(const Geo& g, const MpArgs& a, MpPair& s) {
   s.KX = ld2(a.kx, ic);
   s.KY = ld2(a.ky, ir);
   s.KYt = ld2(a.ky, ir + x);
-  if (a.apply_u) s.U = ld2(a.u, ir);
+  if (a.apply_u) s.U = ld2_nt(a.u, ir);
   if (DOT2) s.RD = ld2(a.rdot, ir);
   // Lanes 0 and 63 share each load instruction, at their own addresses
   // (lane 0's kx value is not used).
@@ -289,9 +289,9 @@ __device__ inline void mp_pair_compute(const Geo& g, const MpArgs& a,
   if (a.apply_u) {
     ua = s.U.x + a.alpha * (double)s.P0.x;
     ub = s.U.y + a.alpha * (double)s.P0.y;
-    st2(a.u, i, s.in0, s.in1, ua, ub);
+    st2_nt(a.u, i, s.in0, s.in1, ua, ub);
   }
-  if (a.apply_p) st2(a.p_new, i, s.in0, s.in1, ca, cb);
+  if (a.apply_p) st2_nt(a.p_new, i, s.in0, s.in1, ca, cb);
   // Halo ring of boundary-adjacent cells (rare; per cell).
 #pragma unroll
   for (int c = 0; c < 2; ++c) {

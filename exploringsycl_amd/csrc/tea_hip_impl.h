@@ -273,9 +273,12 @@ __device__ inline void span_reduce_pairs_n(double (&v)[NR][kSlots], double* part
     }
   }
 }
-// Streaming (nontemporal) access for data read or written once per pass;
-// stencil operands (p, r, kx, ky) use ld2 so the neighbouring rows can reuse
-// them from L2.
+// Streaming (nontemporal) access for data read or written once per pass
+// and not read again before the next iteration (u, p_new, and the residual
+// pass's r); stencil operands (p, r, kx, ky) use ld2 so the neighbouring rows
+// can reuse them from L2, and w, which the residual pass reads next, is
+// stored normally so that it can stay in the 256 MB Infinity Cache (storing
+// it streaming made that pass 1-12% slower).
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 __device__ inline double2 ld2_nt(const double* p, int i) {
@@ -289,6 +292,19 @@ __device__ inline void st2_nt(double* p, int i, bool in0, bool in1, double v0,
                               double v1) {
   if (in0 && in1) {
     __builtin_nontemporal_store(dvec2{v0, v1}, reinterpret_cast<dvec2*>(p + i));
+  } else {
+    if (in0) __builtin_nontemporal_store(v0, p + i);
+    if (in1) __builtin_nontemporal_store(v1, p + i + 1);
+  }
+}
+
+// The FP32 pair store of the mixed-precision kernels.
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+
+__device__ inline void st2_nt(float* p, int i, bool in0, bool in1, float v0,
+                              float v1) {
+  if (in0 && in1) {
+    __builtin_nontemporal_store(fvec2{v0, v1}, reinterpret_cast<fvec2*>(p + i));
   } else {
     if (in0) __builtin_nontemporal_store(v0, p + i);
     if (in1) __builtin_nontemporal_store(v1, p + i + 1);

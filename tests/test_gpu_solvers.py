@@ -23,9 +23,11 @@ pytestmark = pytest.mark.gpu
 
 # (x_cells, y_cells): odd rows with one group, even rows with one group (the
 # pair kernels without a reduction), a short and wide chunk, ragged spans,
-# odd rows over many spans, several spans, and the benchmark size.
+# odd rows over many spans, several spans, and the benchmark size; then, for
+# the two-step passes' 16 x 64 tiles, exactly one tile, one tile and one
+# cell pair more each way, and two tiles and one pair each way.
 SIZES = [(1, 1), (2, 2), (60, 3), (250, 250), (601, 599), (1030, 300),
-         (4096, 4096)]
+         (4096, 4096), (64, 16), (66, 18), (130, 34)]
 # Fused (the default), and one launch per reference kernel.
 MODES = ["fused", "unfused"]
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
