@@ -746,7 +746,7 @@ int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
                            0, c->stream, g, args, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
-    TEA_RETURN_IF(launch_span_finish(c, 1, &plen));
+    TEA_RETURN_IF(launch_span_finish(c, 1, &plen, 0, true));
   } else {
     Timed t(c, K_CG_FUSED_W);
     const int sg = c->ngroup == 1 ? 1 : 0;
@@ -798,7 +798,7 @@ int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
                          alpha, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
-    TEA_RETURN_IF(launch_span_finish(c, 1, &rlen));
+    TEA_RETURN_IF(launch_span_finish(c, 1, &rlen, 0, true));
   } else {
     Timed t(c, K_CG_FUSED_R);
     if (c->pairs)
@@ -869,7 +869,7 @@ int tea_hip_calculate_2norm(tea_hip_chunk* c, const double* buffer,
   }
   TEA_HIP_TRY(hipGetLastError());
   int len = c->nspan;
-  TEA_RETURN_IF(launch_span_finish(c, 1, &len));
+  TEA_RETURN_IF(launch_span_finish(c, 1, &len, 0, true));
   return finish_reduction_n(c, 1, len, norm);
 }
 

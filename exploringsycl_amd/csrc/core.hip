@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -69,7 +71,64 @@ int resolve_timing(tea_hip_chunk* c) {
 
 int sync_chunk(tea_hip_chunk* c) {
   TEA_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->poll_len = 0;
   return resolve_timing(c);
+}
+
+int resolve_ready(tea_hip_chunk* c) {
+  size_t k = 0;
+  for (; k < c->inflight.size(); ++k) {
+    const auto& p = c->inflight[k];
+    const hipError_t e = hipEventQuery(p.stop);
+    if (e == hipErrorNotReady) break;
+    TEA_HIP_TRY(e);
+    float ms = 0.0f;
+    TEA_HIP_TRY(hipEventElapsedTime(&ms, p.start, p.stop));
+    c->launches[p.kid] += 1;
+    c->total_ms[p.kid] += ms;
+    c->event_pool.push_back(p.start);
+    c->event_pool.push_back(p.stop);
+  }
+  c->inflight.erase(c->inflight.begin(), c->inflight.begin() + k);
+  return TEA_HIP_OK;
+}
+
+// A quiet NaN with a payload no arithmetic here produces (a NaN made on the
+// device carries the default payload or an input's).  Should an input NaN
+// carry it, the wait below still ends when the stream goes idle.
+constexpr uint64_t kPollSentinel = 0x7ff4c0ffee5e4d1dull;
+
+// The host's view of the partials is coherent, mapped memory: the finisher's
+// stores reach it while the kernel is still retiring, so the host can form
+// the next scalar and launch the next kernel before the stream reports
+// completion.
+static int wait_partials(tea_hip_chunk* c) {
+  if (c->poll_len == 0) return sync_chunk(c);
+  const volatile uint64_t* v =
+      reinterpret_cast<const volatile uint64_t*>(c->host_partials);
+  const int len = c->poll_len, nr = c->poll_nr;
+  const size_t stride = (size_t)c->nspan;
+  c->poll_len = 0;
+  // The stream is queried only after about 2^22 spins (milliseconds, far
+  // longer than a pass over the largest chunk), and then every 2^16: a
+  // query per few hundred spins slowed the kernels it overlapped.
+  int q = 0, i = 0;
+  for (uint64_t spins = 1;; ++spins) {
+    while (q < nr) {
+      while (i < len && v[q * stride + i] != kPollSentinel) ++i;
+      if (i < len) break;
+      ++q;
+      i = 0;
+    }
+    if (q == nr) break;
+    if (spins >= (1ull << 22) && (spins & 0xffff) == 0) {
+      const hipError_t e = hipStreamQuery(c->stream);
+      if (e == hipSuccess) break;
+      if (e != hipErrorNotReady) TEA_HIP_TRY(e);
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return resolve_ready(c);
 }
 
 // Final passes of the reference's reduceArray (TeaLeaf/c_kernels/sycl/
@@ -142,11 +201,21 @@ __global__ __launch_bounds__(kFinishBlock) void span_finish_kernel(
   }
 }
 
-int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot) {
+int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot,
+                       bool poll) {
   TEA_CHECK_ARG(nr >= 1 && nr <= kSubMaxNr, "span_finish: bad reduction count");
   TEA_CHECK_ARG(slot >= 0 && slot + nr <= 4, "span_finish: bad partials slot");
   const dim3 grid((unsigned)((c->nspan + kWave - 1) / kWave));
   double* dst = c->partials + (size_t)slot * c->nspan;
+  c->poll_len = 0;
+  if (poll && c->poll && slot == 0) {
+    const int n = c->nspan == 1 ? 1 : (c->nspan + kWave - 1) / kWave;
+    uint64_t* v = reinterpret_cast<uint64_t*>(c->host_partials);
+    for (int q = 0; q < nr; ++q)
+      for (int i = 0; i < n; ++i) v[(size_t)q * c->nspan + i] = kPollSentinel;
+    c->poll_nr = nr;
+    c->poll_len = n;
+  }
   {
     Timed t(c, K_SPAN_FINISH);
     if (nr == 1)
@@ -174,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void stream_copy_kernel(const double* __res
 }
 
 int finish_reduction_n(tea_hip_chunk* c, int nr, int len, double* out) {
-  TEA_RETURN_IF(sync_chunk(c));
+  TEA_RETURN_IF(wait_partials(c));
   for (int q = 0; q < nr; ++q)
     out[q] = host_tree_passes(c->host_partials + (size_t)q * c->nspan,
                               (size_t)len);
@@ -239,6 +308,7 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
       c->time_kernel[k] = list.find("," + name + ",") != std::string::npos;
     }
   }
+  if (const char* pv = getenv("TEA_HIP_POLL")) c->poll = atoi(pv) != 0;
   if (const char* sp = getenv("TEA_HIP_SPIN"))
     if (atoi(sp) != 0) (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
 

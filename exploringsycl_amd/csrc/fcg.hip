@@ -872,7 +872,7 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
                          c->stream, c->geo, a, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
-    TEA_RETURN_IF(launch_span_finish(c, 2, &len));
+    TEA_RETURN_IF(launch_span_finish(c, 2, &len, 0, true));
   } else {
     Timed t(c, K_FCG_W);
     if (c->pairs)

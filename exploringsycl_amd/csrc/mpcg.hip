@@ -539,7 +539,7 @@ int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw) {
                            c->geo, c->mp_r, c->mp_w, alpha, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
-    return launch_span_finish(c, rw ? 2 : 1, len);
+    return launch_span_finish(c, rw ? 2 : 1, len, 0, rw);
   }
   Timed t(c, K_MP_R);
   const int sg = c->ngroup == 1 ? 1 : 0;
@@ -625,7 +625,7 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
                          c->stream, c->geo, a, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
-    TEA_RETURN_IF(launch_span_finish(c, 2, &len));
+    TEA_RETURN_IF(launch_span_finish(c, 2, &len, 0, true));
   } else {
     Timed t(c, K_MP_W);
     if (c->pairs)

@@ -945,7 +945,7 @@ int tea_hip_jacobi_iterate(tea_hip_chunk* c, double* error) {
   // The new u is in r's buffer; the old u becomes r (the reference's copy).
   std::swap(f[TEA_FIELD_U], f[TEA_FIELD_R]);
   int len = c->nspan;
-  TEA_RETURN_IF(launch_span_finish(c, 1, &len));
+  TEA_RETURN_IF(launch_span_finish(c, 1, &len, 0, true));
   return finish_reduction_n(c, 1, len, error);
 }
 

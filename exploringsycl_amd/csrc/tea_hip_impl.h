@@ -606,6 +606,13 @@ struct tea_hip_chunk {
   hipStream_t fcg_side = nullptr;
   hipEvent_t fcg_fork = nullptr, fcg_join = nullptr;
 
+  // Waiting for a reduction by its values: launch_span_finish(..., poll)
+  // fills the first poll_len partials of arrays 0 .. poll_nr-1 with a
+  // sentinel, and the finish waits until the finisher has overwritten them
+  // all (or the stream is idle).
+  bool poll = true;
+  int poll_nr = 0, poll_len = 0;
+
   // Timing.
   bool timing = false;
   struct Pending {
@@ -637,6 +644,10 @@ int resolve_timing(tea_hip_chunk* c);
 // Synchronises the stream and resolves timing.
 int sync_chunk(tea_hip_chunk* c);
 
+// Resolves the recorded events that have completed, oldest first, without
+// waiting.
+int resolve_ready(tea_hip_chunk* c);
+
 // Waits for the stream, then finishes nr reductions on the host: the
 // reference tree's remaining passes over each array of nspan level-2
 // partials; results go to out[0..nr).
@@ -645,8 +656,12 @@ int finish_reduction(tea_hip_chunk* c, int nr, double* out);
 // Combines the 8 subtree sums of each span in c->sub (nr dot products) into
 // the span's level-2 partials and runs the next tree pass over them, leaving
 // *len partials per dot product in c->partials.
-// The partials go to arrays slot .. slot + nr - 1 of c->partials.
-int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot = 0);
+// The partials go to arrays slot .. slot + nr - 1 of c->partials.  With
+// poll (slot 0), the next finish waits for the values themselves instead of
+// the stream; only for a finish that follows with no other kernel writing
+// the partials in between.
+int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot = 0,
+                       bool poll = false);
 
 // finish_reduction over the first len partials of each array (len is what
 // launch_span_finish leaves).
