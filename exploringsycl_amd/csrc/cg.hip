@@ -661,9 +661,15 @@ int sync_p_alt(tea_hip_chunk* c) {
 }  // namespace
 
 int apply_pending(tea_hip_chunk* c) {
+  TEA_RETURN_IF(apply_cg_pending(c));
   if (c->held) TEA_RETURN_IF(flush_held(c));
+  return TEA_HIP_OK;
+}
+
+int apply_cg_pending(tea_hip_chunk* c) {
   if (!c->pend_u && !c->pend_p && !c->pend_halo) return TEA_HIP_OK;
   double** f = c->field;
+  const bool halo_u = c->pend_u;  // the deferred ring update includes U
   if (c->pend_u || c->pend_p) {
     ApplyPendingBody body{f[TEA_FIELD_U], f[TEA_FIELD_P], f[TEA_FIELD_R],
                           c->pend_alpha, c->pend_beta, c->pend_u ? 1 : 0,
@@ -676,7 +682,8 @@ int apply_pending(tea_hip_chunk* c) {
   if (halo) {
     static const int ext[4] = {-1, -1, -1, -1};
     int fields[TEA_HALO_NUM] = {0};
-    fields[TEA_HALO_U] = fields[TEA_HALO_P] = 1;
+    fields[TEA_HALO_U] = halo_u ? 1 : 0;
+    fields[TEA_HALO_P] = 1;
     TEA_RETURN_IF(launch_halos(c, 1, ext, fields));
     c->p_alt_synced = false;
   }
@@ -697,6 +704,7 @@ int tea_hip_cg_init(tea_hip_chunk* c, int coefficient, double rx, double ry,
                     coefficient == TEA_RECIP_CONDUCTIVITY,
                 "unknown conductivity model");
   TEA_RETURN_IF(apply_pending(c));
+  c->faces_ok_u = c->faces_ok_p = false;
   double** f = c->field;
   const Geo g = c->geo;
   TEA_RETURN_IF(launch_each(

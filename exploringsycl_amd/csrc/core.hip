@@ -93,9 +93,9 @@ int resolve_ready(tea_hip_chunk* c) {
   return TEA_HIP_OK;
 }
 
-// A quiet NaN with a payload no arithmetic here produces (a NaN made on the
-// device carries the default payload or an input's).  Should an input NaN
-// carry it, the wait below still ends when the stream goes idle.
+// A signalling NaN.  Every partial is a sum, and IEEE 754 arithmetic never
+// returns a signalling NaN (a NaN operand comes out quiet), so no partial
+// can equal it; the wait below would still end when the stream goes idle.
 constexpr uint64_t kPollSentinel = 0x7ff4c0ffee5e4d1dull;
 
 // The host's view of the partials is coherent, mapped memory: the finisher's
@@ -436,6 +436,7 @@ int tea_hip_field_write(tea_hip_chunk* c, int field, const double* host,
   TEA_HIP_TRY(hipMemcpyAsync(c->field[field], host, count * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
   if (field == TEA_FIELD_P) c->p_alt_synced = false;
+  c->faces_ok_u = c->faces_ok_p = false;
   return sync_chunk(c);
 }
 

@@ -202,6 +202,7 @@ int tea_hip_set_chunk_data(tea_hip_chunk* c, double x_min, double y_min,
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(apply_pending(c));
+  c->faces_ok_u = c->faces_ok_p = false;
   const Geo g = c->geo;
   double** f = c->field;
   {
@@ -235,6 +236,7 @@ int tea_hip_set_chunk_state(tea_hip_chunk* c, int num_states,
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(apply_pending(c));
+  c->faces_ok_u = c->faces_ok_p = false;
   const Geo g = c->geo;
   double** f = c->field;
   {
@@ -295,6 +297,10 @@ int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
   const int ext_b = neighbours[TEA_FACE_BOTTOM] < 0;
   const int ext_t = neighbours[TEA_FACE_TOP] < 0;
   if (nf == 0 || !(ext_l || ext_r || ext_b || ext_t)) return TEA_HIP_OK;
+  if (ext_l && ext_r && ext_b && ext_t) {
+    c->faces_ok_u |= fields[TEA_HALO_U] != 0;
+    c->faces_ok_p |= fields[TEA_HALO_P] != 0;
+  }
   const long items = 2L * g.y * depth + 2L * g.x * depth;
   {
     Timed t(c, K_LOCAL_HALOS);
@@ -331,18 +337,27 @@ int tea_hip_local_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
       return TEA_HIP_OK;
     }
   }
-  // The depth-1 U/P update that follows cg_calc_p is folded into the next
-  // fused cg_calc_w when all faces are external.
-  if (c->fused && c->pend_u && c->pend_p && !c->pend_halo && depth == 1) {
-    bool only_up = true;
-    for (int s = 0; s < TEA_HALO_NUM; ++s)
-      only_up &= (fields_to_exchange[s] != 0) == (s == TEA_HALO_U || s == TEA_HALO_P);
-    if (only_up && all_external) {
+  // The depth-1 update that follows cg_calc_p is folded into the next fused
+  // cg_calc_w when all faces are external: of U and P with u += alpha p
+  // pending too (CG), of P alone without (the PPCG outer step).
+  if (c->fused && !c->held && c->pend_p && !c->pend_halo && depth == 1 &&
+      all_external) {
+    bool only_up = true, only_p = true;
+    for (int s = 0; s < TEA_HALO_NUM; ++s) {
+      const bool on = fields_to_exchange[s] != 0;
+      only_up &= on == (s == TEA_HALO_U || s == TEA_HALO_P);
+      only_p &= on == (s == TEA_HALO_P);
+    }
+    if (c->pend_u ? (only_up && c->faces_ok_u && c->faces_ok_p)
+                  : (only_p && c->faces_ok_p)) {
       c->pend_halo = true;
       return TEA_HIP_OK;
     }
   }
-  TEA_RETURN_IF(apply_pending(c));
+  // An update of fields other than U and P leaves deferred CG work on u and
+  // p deferred (a held step, which it may read, is run first).
+  if (fields_to_exchange[TEA_HALO_U] || fields_to_exchange[TEA_HALO_P] || c->held)
+    TEA_RETURN_IF(apply_pending(c));
   return launch_halos(c, depth, neighbours, fields_to_exchange);
 }
 
@@ -377,6 +392,7 @@ int tea_hip_pack_or_unpack(tea_hip_chunk* c, int depth, int face, int pack,
     TEA_HIP_TRY(hipMemcpyAsync(host_buffer, strip, bytes, hipMemcpyDeviceToHost,
                                c->stream));
   if (!pack && fid == TEA_FIELD_P) c->p_alt_synced = false;
+  if (!pack) c->faces_ok_u = c->faces_ok_p = false;
   return sync_chunk(c);
 }
 

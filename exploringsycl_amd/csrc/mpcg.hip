@@ -574,6 +574,7 @@ int tea_hip_mpcg_init(tea_hip_chunk* c, int coefficient, double rx, double ry,
   TEA_CHECK_ARG(rr && bb, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_FLUSH_HELD(c);
+  c->faces_ok_u = c->faces_ok_p = false;
   TEA_RETURN_IF(ensure_mp_buffers(c));
   // FP64 initialisation exactly as the plain path: kx, ky, u, w, r, p.
   double rro64 = 0.0;

@@ -360,7 +360,9 @@ bool fused_pairs(const tea_hip_chunk* c) { return c->fused && c->pairs; }
 //   5. tiles on the edge of the interior write the second buffers' cells
 //      outside it: for sd what the halo update gives, r copied.
 // sd_k and r_k are read once (the margins from L2), u, r and sd written once:
-// 64 bytes per cell for two steps against 128 for two single passes.  Every
+// 64 bytes per cell for two steps against 128 for two single passes.  When
+// cg_calc_ur's u += alpha p is still pending (the first pass of a PPCG outer
+// step), the pass also reads p and adds alpha p to u before the steps' sd.  Every
 // value uses the single-step kernel's expressions, so the result is bitwise
 // that of two single steps with the halo update between them.
 constexpr int kTileX = 64;
@@ -400,7 +402,8 @@ __global__ __launch_bounds__(kTSBlock) void ppcg_two_step_kernel(
     Geo g, const double* __restrict__ sd_old, double* __restrict__ sd_new,
     const double* __restrict__ r, double* __restrict__ r_new,
     double* __restrict__ u, const double* __restrict__ kx,
-    const double* __restrict__ ky, double a0, double b0, double a1, double b1,
+    const double* __restrict__ ky, const double* __restrict__ p_pend,
+    double alpha_pend, double a0, double b0, double a1, double b1,
     int tiles_x, int tiles_y) {
   // sd_k (rows r0-2.., columns c0-2..) until step 2 is formed, then, in the
   // same storage, sd_(k+1) (rows r0-1.., columns c0-1..).
@@ -435,6 +438,8 @@ __global__ __launch_bounds__(kTSBlock) void ppcg_two_step_kernel(
       const int i = kk + jj * x;
       rk1[m] = r[i];
       uo[m] = u[i];
+      // u += alpha p of cg_calc_ur, when still pending (cg.cpp:228).
+      if (p_pend != nullptr) uo[m] = uo[m] + alpha_pend * p_pend[i];
       if (lane == kWave - 1) kxe[m] = kx[i + 1];
       kyv[m] = ky[i];
       kyv[m + 1] = ky[i + x];
@@ -852,7 +857,7 @@ int ppcg_step(tea_hip_chunk* c, double alpha, double beta) {
 
 // Two inner steps with the SD halo update between them, in one pass.
 int ppcg_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
-                   double b1) {
+                   double b1, bool pend_u) {
   double** f = c->field;
   const Geo g = c->geo;
   TEA_RETURN_IF(ensure_spare(c));
@@ -864,11 +869,13 @@ int ppcg_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
     hipLaunchKernelGGL(ppcg_two_step_kernel, dim3(tiles_x * tiles_y),
                        dim3(kTSBlock), 0, c->stream, g, f[TEA_FIELD_SD], c->spare,
                        f[TEA_FIELD_R], c->spare2, f[TEA_FIELD_U], f[TEA_FIELD_KX],
-                       f[TEA_FIELD_KY], a0, b0, a1, b1, tiles_x, tiles_y);
+                       f[TEA_FIELD_KY], pend_u ? f[TEA_FIELD_P] : nullptr,
+                       c->pend_alpha, a0, b0, a1, b1, tiles_x, tiles_y);
   }
   TEA_HIP_TRY(hipGetLastError());
   std::swap(f[TEA_FIELD_SD], c->spare);
   std::swap(f[TEA_FIELD_R], c->spare2);
+  if (pend_u) c->pend_u = false;
   return TEA_HIP_OK;
 }
 
@@ -877,6 +884,7 @@ int ppcg_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
 int flush_held(tea_hip_chunk* c) {
   const int kind = c->held;
   if (kind == kHeldNone) return TEA_HIP_OK;
+  TEA_RETURN_IF(apply_cg_pending(c));
   const bool halo = c->held_halo;
   c->held = kHeldNone;
   c->held_halo = false;
@@ -904,6 +912,7 @@ int tea_hip_jacobi_init(tea_hip_chunk* c, int coefficient, double rx,
                     coefficient == TEA_RECIP_CONDUCTIVITY,
                 "unknown coefficient");
   TEA_RETURN_IF(apply_pending(c));
+  c->faces_ok_u = c->faces_ok_p = false;
   double** f = c->field;
   const Geo g = c->geo;
   return launch_each_cell(
@@ -952,6 +961,7 @@ int tea_hip_jacobi_iterate(tea_hip_chunk* c, double* error) {
 int tea_hip_cheby_init(tea_hip_chunk* c, double theta) {
   TEA_ENTER(c);
   TEA_RETURN_IF(apply_pending(c));
+  c->faces_ok_u = c->faces_ok_p = false;
   double** f = c->field;
   return launch_each_cell(
       c, K_CHEBY_INIT,
@@ -983,7 +993,9 @@ int tea_hip_cheby_iterate(tea_hip_chunk* c, double alpha, double beta) {
 
 int tea_hip_ppcg_init(tea_hip_chunk* c, double theta) {
   TEA_ENTER(c);
-  TEA_RETURN_IF(apply_pending(c));
+  // sd = r / theta reads neither u nor p: u += alpha p, left pending by
+  // cg_calc_ur, stays pending for the first inner pass to apply.
+  if (c->pend_p || c->pend_halo) TEA_RETURN_IF(apply_pending(c));
   double** f = c->field;
   return launch_each_cell(
       c, K_PPCG_INIT, DivideBody{f[TEA_FIELD_SD], f[TEA_FIELD_R], theta, c->geo});
@@ -998,9 +1010,15 @@ int tea_hip_ppcg_inner_iteration(tea_hip_chunk* c, double alpha,
     if (c->held == kHeldPpcg && c->held_halo) {
       c->held = kHeldNone;
       c->held_halo = false;
-      return ppcg_two_steps(c, c->held_alpha, c->held_beta, alpha, beta);
+      // u += alpha p left pending by cg_calc_ur precedes both steps; the
+      // pass adds it to u before their sd (the same roundings).
+      const bool pend_u = c->pend_u && !c->pend_p && !c->pend_halo;
+      if (!pend_u) TEA_RETURN_IF(apply_cg_pending(c));
+      return ppcg_two_steps(c, c->held_alpha, c->held_beta, alpha, beta, pend_u);
     }
-    TEA_RETURN_IF(apply_pending(c));
+    // The step is held with u += alpha p still pending (the pass that runs
+    // the step applies it first); any other deferred work is applied now.
+    if (c->held || c->pend_p || c->pend_halo) TEA_RETURN_IF(apply_pending(c));
     c->held = kHeldPpcg;
     c->held_halo = false;
     c->held_alpha = alpha;

@@ -613,6 +613,14 @@ struct tea_hip_chunk {
   bool poll = true;
   int poll_nr = 0, poll_len = 0;
 
+  // A depth-1 reflective update also copies the cells of its face lines
+  // beyond the ring (e.g. column h-1 at rows below h-1) from other cells
+  // outside the interior, which no kernel of the iteration changes.  A
+  // U/P update is deferred into a fused pass, which writes only the ring,
+  // once a real update of all external faces has made those cells what the
+  // update would leave; host writes and initialisations clear the flags.
+  bool faces_ok_u = false, faces_ok_p = false;
+
   // Timing.
   bool timing = false;
   struct Pending {
@@ -680,11 +688,15 @@ inline bool use_pair_blocks(const tea_hip_chunk* c, int kernel_bit) {
 // Kinds of held-back step.
 enum { kHeldNone = 0, kHeldPpcg = 1, kHeldCheby = 2 };
 
-// Applies any deferred CG work (fused mode), after a held-back step.
+// Applies the deferred CG work (fused mode: u += alpha p, p = beta p + r and
+// their halo ring), then runs a held-back step.  Deferred CG work is always
+// older than a held step: a step is held with at most u += alpha p pending,
+// and the CG entry points run a held step before they defer new work.
 int apply_pending(tea_hip_chunk* c);
-// Runs a held-back PPCG inner step or Chebyshev step, then its held-back
-// halo update
-// (solvers.hip).
+// The deferred CG work alone (cg.hip).
+int apply_cg_pending(tea_hip_chunk* c);
+// Runs a held-back PPCG inner step or Chebyshev step (after the older
+// deferred CG work), then its held-back halo update (solvers.hip).
 int flush_held(tea_hip_chunk* c);
 // Mixed-precision residual update r -= alpha w with the partials of r.r
 // (and, with rw, of r.w as a second array) written to c->partials

@@ -237,6 +237,50 @@ def test_fused_large_grid_matches_oracle(oracle):
         _assert_fields(c, f, names)
 
 
+@pytest.mark.parametrize("cells", [(250, 250), (601, 599)])
+def test_cg_iterations_from_unreflected_fields(oracle, cells):
+    """CG iterations on random fields whose halo cells were never reflected.
+    cg_calc_p's U/P halo update is folded into the next fused cg_calc_w only
+    once a real update has made the face lines beyond the ring (copied from
+    cells no iteration changes) what a reflective update leaves: here the
+    first update runs for real and the later ones are folded."""
+    x, y = cells[0] + 2 * H, cells[1] + 2 * H
+    f = _random_fields(x, y, 29)
+    names = ["p", "w", "u", "r", "kx", "ky"]
+    with _chunk(x, y) as c:
+        _upload(c, f, names)
+        for alpha, beta in [(0.6, 0.3), (0.45, 0.8), (0.2, 0.5)]:
+            assert _same(c.cg_calc_w(), oracle.cg_calc_w(x, y, H, f))
+            assert _same(c.cg_calc_ur(alpha), oracle.cg_calc_ur(x, y, H, alpha, f))
+            c.cg_calc_p(beta)
+            oracle.cg_calc_p(x, y, H, beta, f)
+            c.local_halos(1, ["u", "p"])
+            oracle.halo_field(x, y, H, 1, f["u"])
+            oracle.halo_field(x, y, H, 1, f["p"])
+        assert _same(c.cg_calc_w(), oracle.cg_calc_w(x, y, H, f))
+        _assert_fields(c, f, names)
+
+
+@pytest.mark.parametrize("cells", [(250, 250), (4096, 16)])
+def test_reductions_finish_with_the_wait_sentinel_as_input(oracle, cells):
+    """The host waits for a reduction's partials by polling for a sentinel
+    (a signalling NaN) to be overwritten.  A field holding exactly that bit
+    pattern still gives a (quiet) NaN norm, returned without hanging, and the
+    next reduction is exact again."""
+    x, y = cells[0] + 2 * H, cells[1] + 2 * H
+    f = _random_fields(x, y, 23)
+    snan = np.array([0x7FF4C0FFEE5E4D1D], dtype=np.uint64).view(np.float64)[0]
+    bad = f["r"].copy()
+    bad[(y // 2) * x + x // 2] = snan
+    with _chunk(x, y) as c:
+        c.write("r", bad)
+        got = c.calculate_2norm("r")
+        assert np.isnan(got)
+        assert np.array(got).view(np.uint64) != np.uint64(0x7FF4C0FFEE5E4D1D)
+        c.write("r", f["r"])
+        assert _same(c.calculate_2norm("r"), oracle.calculate_2norm(x, y, H, f["r"]))
+
+
 def test_two_norm_rejects_foreign_pointer():
     from exploringsycl_amd.native import TeaHipError
     import ctypes

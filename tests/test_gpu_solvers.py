@@ -182,6 +182,48 @@ def test_ppcg_held_step_flushed_by_other_calls(oracle, cells):
         _assert_fields(c, f, names)
 
 
+@pytest.mark.parametrize("cells", [(250, 250), (601, 599), (66, 18)])
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("steps,interrupt", [(1, False), (2, False), (3, False),
+                                             (10, False), (10, True)])
+def test_ppcg_outer_steps_with_deferred_cg_updates(oracle, cells, mode, steps,
+                                                   interrupt):
+    """PPCG outer steps as the driver makes them (ppcg_driver.c:87-189):
+    w = A p, u += alpha p with r -= alpha w, sd = r / theta, the inner steps
+    each after an SD halo update, the 2-norm of r, p = beta p + r and the P
+    halo update.  In the fused mode u += alpha p is applied by the first
+    two-step pass and p = beta p + r with its ring by the next w = A p; with
+    `interrupt`, a 2-norm of u after the first inner step must see u += alpha
+    p applied before that step."""
+    x, y = _dims(cells)
+    f = _random_fields(x, y, 26)
+    names = ["p", "w", "u", "r", "sd", "kx", "ky"]
+    rng = np.random.default_rng(100 + steps)
+    coefs = list(zip(rng.uniform(-0.5, 1.0, steps), rng.uniform(-0.5, 1.5, steps)))
+    with _chunk(x, y, mode) as c:
+        _upload(c, f, names)
+        for outer, (alpha, beta) in enumerate([(0.7, 0.4), (0.35, 0.9)]):
+            assert _same(c.cg_calc_w(), oracle.cg_calc_w(x, y, H, f))
+            assert _same(c.cg_calc_ur(alpha), oracle.cg_calc_ur(x, y, H, alpha, f))
+            c.ppcg_init(2.5)
+            oracle.ppcg_init(x, y, H, 2.5, f)
+            for k, (a, b) in enumerate(coefs):
+                c.local_halos(1, ["sd"])
+                oracle.halo_field(x, y, H, 1, f["sd"])
+                c.ppcg_inner_iteration(a, b)
+                oracle.ppcg_inner_iteration(x, y, H, a, b, f)
+                if interrupt and k == 0 and outer == 0:
+                    assert _same(c.calculate_2norm("u"),
+                                 oracle.calculate_2norm(x, y, H, f["u"]))
+            assert _same(c.calculate_2norm("r"), oracle.calculate_2norm(x, y, H, f["r"]))
+            c.cg_calc_p(beta)
+            oracle.cg_calc_p(x, y, H, beta, f)
+            c.local_halos(1, ["p"])
+            oracle.halo_field(x, y, H, 1, f["p"])
+        assert _same(c.cg_calc_w(), oracle.cg_calc_w(x, y, H, f))
+        _assert_fields(c, f, names)
+
+
 def test_field_addresses_follow_the_exchange():
     """The fused passes exchange buffers; tea_hip_field reports the current
     ones, and the 2-norm entry point accepts them."""

@@ -45,7 +45,8 @@ def op5(c, l, r, b, t, kx0, kx1, ky0, ky1):
     diag = (1.0 + (kx1 + kx0)) + (ky1 + ky0)
     return (diag * c - (kx1 * r + kx0 * l)) - (ky1 * t + ky0 * b)
 
-def model_ppcg(x, y, h, sd_old, sd_new, r, r_new, u, kx, ky, a0, b0, a1, b1):
+def model_ppcg(x, y, h, sd_old, sd_new, r, r_new, u, kx, ky, a0, b0, a1, b1,
+               p_pend=None, alpha_pend=0.0):
     n = x * y
     def G(arr, i):
         assert 0 <= i < n, ("read OOB", i)
@@ -122,7 +123,10 @@ def model_ppcg(x, y, h, sd_old, sd_new, r, r_new, u, kx, ky, a0, b0, a1, b1):
                     assert not any(np.isnan(v) for v in nb + (c,)), ("unset S1 used", jj, kk)
                     rn = rk1[m] - op5(c, nb[0], nb[1], nb[2], nb[3], kx0[m], kx1[m], kyv[m], kyv[m + 1])
                     P(r_new, i, rn)
-                    P(u, i, (G(u, i) + S0[m + 2][t + 2]) + c)
+                    uo = G(u, i)
+                    if p_pend is not None:
+                        uo = uo + alpha_pend * G(p_pend, i)
+                    P(u, i, (uo + S0[m + 2][t + 2]) + c)
                     P(sd_new, i, a1 * c + b1 * rn); written[i] += 1
         if edge:
             ka = 0 if txi == 0 else c0
@@ -246,23 +250,32 @@ GRIDS = [(3, 2, 2), (10, 7, 2), (TX + 44, 11, 2), (TX, TY, 2), (TX + 1, TY + 1, 
          (2 * TX + 1, 2 * TY + 1, 2), (40, 30, 1), (TX + 43, 13, 3)]
 
 
+@pytest.mark.parametrize("pending", [False, True])
 @pytest.mark.parametrize("xc,yc,h", GRIDS)
-def test_ppcg_two_step_model(oracle, xc, yc, h):
+def test_ppcg_two_step_model(oracle, xc, yc, h, pending):
+    """With `pending`, u += alpha p of cg_calc_ur (interior cells) is applied
+    by the pass before the steps, as the first pass of a PPCG outer step
+    does."""
     rng = np.random.default_rng(5)
     x, y = xc + 2 * h, yc + 2 * h
     n = x * y
-    f = {k: rng.standard_normal(n) for k in ("sd", "r", "u")}
+    f = {k: rng.standard_normal(n) for k in ("sd", "r", "u", "p")}
     f["kx"] = rng.uniform(0, 5, n)
     f["ky"] = rng.uniform(0, 5, n)
     g = {k: v.copy() for k, v in f.items()}
     a0, b0, a1, b1 = 0.3, 0.7, -0.45, 1.2
+    alpha = 0.61
+    if pending:
+        gu = g["u"].reshape(y, x)
+        gu[h:y - h, h:x - h] = gu[h:y - h, h:x - h] + alpha * g["p"].reshape(y, x)[h:y - h, h:x - h]
     oracle.ppcg_inner_iteration(x, y, h, a0, b0, g)
     oracle.halo_field(x, y, h, 1, g["sd"])
     oracle.ppcg_inner_iteration(x, y, h, a1, b1, g)
     sd_new = np.full(n, np.nan)
     r_new = np.full(n, np.nan)
     u = f["u"].copy()
-    model_ppcg(x, y, h, f["sd"], sd_new, f["r"], r_new, u, f["kx"], f["ky"], a0, b0, a1, b1)
+    model_ppcg(x, y, h, f["sd"], sd_new, f["r"], r_new, u, f["kx"], f["ky"], a0, b0, a1, b1,
+               f["p"] if pending else None, alpha)
     for name, got, want in [("sd", sd_new, g["sd"]), ("r", r_new, g["r"]), ("u", u, g["u"])]:
         assert np.array_equal(got.view(np.int64), want.view(np.int64)), name
 
