@@ -109,9 +109,10 @@ static int wait_partials(tea_hip_chunk* c) {
   const int len = c->poll_len, nr = c->poll_nr;
   const size_t stride = (size_t)c->nspan;
   c->poll_len = 0;
-  // The stream is queried only after about 2^22 spins (milliseconds, far
-  // longer than a pass over the largest chunk), and then every 2^16: a
-  // query per few hundred spins slowed the kernels it overlapped.
+  // The stream is queried only after about 2^22 spins (milliseconds, longer
+  // than a pass over an 8192^2 chunk) and then every 2^16 spins, a few
+  // queries per pass even over the largest chunks: a query every few
+  // hundred spins slowed the kernels it overlapped.
   int q = 0, i = 0;
   for (uint64_t spins = 1;; ++spins) {
     while (q < nr) {
