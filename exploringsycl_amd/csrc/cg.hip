@@ -662,8 +662,7 @@ int sync_p_alt(tea_hip_chunk* c) {
 
 int apply_pending(tea_hip_chunk* c) {
   TEA_RETURN_IF(apply_cg_pending(c));
-  if (c->held) TEA_RETURN_IF(flush_held(c));
-  return TEA_HIP_OK;
+  return flush_held(c);
 }
 
 int apply_cg_pending(tea_hip_chunk* c) {

@@ -337,6 +337,24 @@ int tea_hip_local_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
       return TEA_HIP_OK;
     }
   }
+  // A depth-1 SD (U) update with no step held is held back too: the next
+  // PPCG inner step (Chebyshev step) is then held, and the pass that runs
+  // it with the one after reflects the ring of the field it stages.  Any
+  // other entry point runs it first.
+  if (c->fused && c->pairs && !c->held && c->pre_halo < 0 && depth == 1 &&
+      all_external) {
+    int only = -1, count = 0;
+    for (int s = 0; s < TEA_HALO_NUM; ++s)
+      if (fields_to_exchange[s]) {
+        only = s;
+        ++count;
+      }
+    if (count == 1 && (only == TEA_HALO_SD || only == TEA_HALO_U) &&
+        !(only == TEA_HALO_U && (c->pend_u || c->pend_halo))) {
+      c->pre_halo = only;
+      return TEA_HIP_OK;
+    }
+  }
   // The depth-1 update that follows cg_calc_p is folded into the next fused
   // cg_calc_w when all faces are external: of U and P with u += alpha p
   // pending too (CG), of P alone without (the PPCG outer step).
@@ -356,7 +374,8 @@ int tea_hip_local_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
   }
   // An update of fields other than U and P leaves deferred CG work on u and
   // p deferred (a held step, which it may read, is run first).
-  if (fields_to_exchange[TEA_HALO_U] || fields_to_exchange[TEA_HALO_P] || c->held)
+  if (fields_to_exchange[TEA_HALO_U] || fields_to_exchange[TEA_HALO_P] ||
+      c->held || c->pre_halo >= 0)
     TEA_RETURN_IF(apply_pending(c));
   return launch_halos(c, depth, neighbours, fields_to_exchange);
 }

@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kTSBlock) void ppcg_two_step_kernel(
     double* __restrict__ u, const double* __restrict__ kx,
     const double* __restrict__ ky, const double* __restrict__ p_pend,
     double alpha_pend, double a0, double b0, double a1, double b1,
-    int tiles_x, int tiles_y) {
+    int tiles_x, int tiles_y, int pre_halo) {
   // sd_k (rows r0-2.., columns c0-2..) until step 2 is formed, then, in the
   // same storage, sd_(k+1) (rows r0-1.., columns c0-1..).
   __shared__ double lds[kS0H * kS0W];
@@ -484,6 +484,24 @@ __global__ __launch_bounds__(kTSBlock) void ppcg_two_step_kernel(
     if (q < kS0H * kS0W) lds[q] = st[n];
   }
   __syncthreads();
+  // A held-back depth-1 update of the staged field before step k: its ring
+  // cells take their mirror's value (double mirror at the corners).  The
+  // staging area reaches the ring in edge tiles and in tiles next to a
+  // partial last tile.
+  if (pre_halo && (c0 - 2 <= h - 1 || c0 + kTileX + 1 >= xe ||
+                   r0 - 2 <= h - 1 || r0 + kTileY + 1 >= ye)) {
+    for (int q = t; q < kS0H * kS0W; q += kTSBlock) {
+      const int a = q / kS0W, b = q - a * kS0W;
+      const int jj = r0 - 2 + a, kc = c0 - 2 + b;
+      const bool ring_box = jj >= h - 1 && jj <= ye && kc >= h - 1 && kc <= xe;
+      const bool inner = jj >= h && jj < ye && kc >= h && kc < xe;
+      if (!ring_box || inner) continue;
+      const int mj = jj == h - 1 ? h : (jj == ye ? ye - 1 : jj);
+      const int mk = kc == h - 1 ? h : (kc == xe ? xe - 1 : kc);
+      S0[a][b] = S0[mj - r0 + 2][mk - c0 + 2];
+    }
+    __syncthreads();
+  }
 
   // 2. sd_(k+1) on the tile's own cells (rk1 becomes r_(k+1)) ...
   double sdk[kRows], s1own[kRows];
@@ -598,7 +616,7 @@ __global__ __launch_bounds__(kTSBlock) void cheby_two_step_kernel(
     const double* __restrict__ u0, const double* __restrict__ p_old,
     double* __restrict__ p_new, double* __restrict__ r, double* __restrict__ w,
     const double* __restrict__ kx, const double* __restrict__ ky, double a0,
-    double b0, double a1, double b1, int tiles_x, int tiles_y) {
+    double b0, double a1, double b1, int tiles_x, int tiles_y, int pre_halo) {
   __shared__ double lds[kS0H * kS0W];
   double (*S0)[kS0W] = reinterpret_cast<double (*)[kS0W]>(lds);
   double (*S1)[kS1W] = reinterpret_cast<double (*)[kS1W]>(lds);
@@ -672,6 +690,24 @@ __global__ __launch_bounds__(kTSBlock) void cheby_two_step_kernel(
     if (q < kS0H * kS0W) lds[q] = st[n];
   }
   __syncthreads();
+  // A held-back depth-1 update of the staged field before step k: its ring
+  // cells take their mirror's value (double mirror at the corners).  The
+  // staging area reaches the ring in edge tiles and in tiles next to a
+  // partial last tile.
+  if (pre_halo && (c0 - 2 <= h - 1 || c0 + kTileX + 1 >= xe ||
+                   r0 - 2 <= h - 1 || r0 + kTileY + 1 >= ye)) {
+    for (int q = t; q < kS0H * kS0W; q += kTSBlock) {
+      const int a = q / kS0W, b = q - a * kS0W;
+      const int jj = r0 - 2 + a, kc = c0 - 2 + b;
+      const bool ring_box = jj >= h - 1 && jj <= ye && kc >= h - 1 && kc <= xe;
+      const bool inner = jj >= h && jj < ye && kc >= h && kc < xe;
+      if (!ring_box || inner) continue;
+      const int mj = jj == h - 1 ? h : (jj == ye ? ye - 1 : jj);
+      const int mk = kc == h - 1 ? h : (kc == xe ? xe - 1 : kc);
+      S0[a][b] = S0[mj - r0 + 2][mk - c0 + 2];
+    }
+    __syncthreads();
+  }
 
   // 2. u_(k+1) on the tile's own cells (pk becomes p_(k+1)) ...
   double u1own[kRows];
@@ -806,7 +842,7 @@ int cheby_step(tea_hip_chunk* c, double alpha, double beta) {
 
 // Two Chebyshev steps with the U halo update between them, in one pass.
 int cheby_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
-                    double b1) {
+                    double b1, bool pre_halo) {
   double** f = c->field;
   const Geo g = c->geo;
   TEA_RETURN_IF(ensure_spare(c));
@@ -819,12 +855,13 @@ int cheby_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
                        dim3(kTSBlock), 0, c->stream, g, f[TEA_FIELD_U], c->spare,
                        f[TEA_FIELD_U0], f[TEA_FIELD_P], c->spare2, f[TEA_FIELD_R],
                        f[TEA_FIELD_W], f[TEA_FIELD_KX], f[TEA_FIELD_KY], a0, b0,
-                       a1, b1, tiles_x, tiles_y);
+                       a1, b1, tiles_x, tiles_y, pre_halo ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
   std::swap(f[TEA_FIELD_U], c->spare);
   std::swap(f[TEA_FIELD_P], c->spare2);
   c->p_alt_synced = false;
+  if (pre_halo) c->pre_halo = -1;
   return TEA_HIP_OK;
 }
 
@@ -857,7 +894,7 @@ int ppcg_step(tea_hip_chunk* c, double alpha, double beta) {
 
 // Two inner steps with the SD halo update between them, in one pass.
 int ppcg_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
-                   double b1, bool pend_u) {
+                   double b1, bool pend_u, bool pre_halo) {
   double** f = c->field;
   const Geo g = c->geo;
   TEA_RETURN_IF(ensure_spare(c));
@@ -870,21 +907,34 @@ int ppcg_two_steps(tea_hip_chunk* c, double a0, double b0, double a1,
                        dim3(kTSBlock), 0, c->stream, g, f[TEA_FIELD_SD], c->spare,
                        f[TEA_FIELD_R], c->spare2, f[TEA_FIELD_U], f[TEA_FIELD_KX],
                        f[TEA_FIELD_KY], pend_u ? f[TEA_FIELD_P] : nullptr,
-                       c->pend_alpha, a0, b0, a1, b1, tiles_x, tiles_y);
+                       c->pend_alpha, a0, b0, a1, b1, tiles_x, tiles_y,
+                       pre_halo ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
   std::swap(f[TEA_FIELD_SD], c->spare);
   std::swap(f[TEA_FIELD_R], c->spare2);
   if (pend_u) c->pend_u = false;
+  if (pre_halo) c->pre_halo = -1;
   return TEA_HIP_OK;
 }
 
 }  // namespace
 
+int apply_pre_halo(tea_hip_chunk* c) {
+  if (c->pre_halo < 0) return TEA_HIP_OK;
+  static const int ext[4] = {-1, -1, -1, -1};
+  int fields[TEA_HALO_NUM] = {0};
+  fields[c->pre_halo] = 1;
+  c->pre_halo = -1;
+  return launch_halos(c, 1, ext, fields);
+}
+
 int flush_held(tea_hip_chunk* c) {
   const int kind = c->held;
-  if (kind == kHeldNone) return TEA_HIP_OK;
+  if (kind == kHeldNone && c->pre_halo < 0) return TEA_HIP_OK;
   TEA_RETURN_IF(apply_cg_pending(c));
+  TEA_RETURN_IF(apply_pre_halo(c));
+  if (kind == kHeldNone) return TEA_HIP_OK;
   const bool halo = c->held_halo;
   c->held = kHeldNone;
   c->held_halo = false;
@@ -978,9 +1028,18 @@ int tea_hip_cheby_iterate(tea_hip_chunk* c, double alpha, double beta) {
     if (c->held == kHeldCheby && c->held_halo) {
       c->held = kHeldNone;
       c->held_halo = false;
-      return cheby_two_steps(c, c->held_alpha, c->held_beta, alpha, beta);
+      TEA_RETURN_IF(apply_cg_pending(c));
+      const bool pre = c->pre_halo == TEA_HALO_U;
+      if (!pre) TEA_RETURN_IF(apply_pre_halo(c));
+      return cheby_two_steps(c, c->held_alpha, c->held_beta, alpha, beta, pre);
     }
-    TEA_RETURN_IF(apply_pending(c));
+    // A held-back U update before this step stays held: the pass that runs
+    // the step reflects the ring it stages.
+    if (c->pre_halo == TEA_HALO_U && !c->held) {
+      TEA_RETURN_IF(apply_cg_pending(c));
+    } else {
+      TEA_RETURN_IF(apply_pending(c));
+    }
     c->held = kHeldCheby;
     c->held_halo = false;
     c->held_alpha = alpha;
@@ -1014,11 +1073,17 @@ int tea_hip_ppcg_inner_iteration(tea_hip_chunk* c, double alpha,
       // pass adds it to u before their sd (the same roundings).
       const bool pend_u = c->pend_u && !c->pend_p && !c->pend_halo;
       if (!pend_u) TEA_RETURN_IF(apply_cg_pending(c));
-      return ppcg_two_steps(c, c->held_alpha, c->held_beta, alpha, beta, pend_u);
+      const bool pre = c->pre_halo == TEA_HALO_SD;
+      if (!pre) TEA_RETURN_IF(apply_pre_halo(c));
+      return ppcg_two_steps(c, c->held_alpha, c->held_beta, alpha, beta, pend_u,
+                            pre);
     }
-    // The step is held with u += alpha p still pending (the pass that runs
-    // the step applies it first); any other deferred work is applied now.
-    if (c->held || c->pend_p || c->pend_halo) TEA_RETURN_IF(apply_pending(c));
+    // The step is held with u += alpha p and a held-back SD update before
+    // it still pending (the pass that runs the step applies both first);
+    // any other deferred work is applied now.
+    if (c->held || c->pend_p || c->pend_halo ||
+        (c->pre_halo >= 0 && c->pre_halo != TEA_HALO_SD))
+      TEA_RETURN_IF(apply_pending(c));
     c->held = kHeldPpcg;
     c->held_halo = false;
     c->held_alpha = alpha;

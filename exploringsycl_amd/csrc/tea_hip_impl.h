@@ -575,6 +575,11 @@ struct tea_hip_chunk {
   int held = 0;
   bool held_halo = false;
   double held_alpha = 0.0, held_beta = 0.0;
+  // A depth-1 reflective update of SD (PPCG) or U (Chebyshev), all faces
+  // external, requested with no step held and held back so that the next
+  // two-step pass reflects the ring of the field it stages instead (the
+  // halo slot, or -1).
+  int pre_halo = -1;
 
   // Deferred CG work (fused mode).
   int fused = 1;
@@ -695,8 +700,11 @@ enum { kHeldNone = 0, kHeldPpcg = 1, kHeldCheby = 2 };
 int apply_pending(tea_hip_chunk* c);
 // The deferred CG work alone (cg.hip).
 int apply_cg_pending(tea_hip_chunk* c);
-// Runs a held-back PPCG inner step or Chebyshev step (after the older
-// deferred CG work), then its held-back halo update (solvers.hip).
+// A held-back halo update before a step, alone (solvers.hip).
+int apply_pre_halo(tea_hip_chunk* c);
+// Runs what is held back, oldest first: deferred CG work, a held-back halo
+// update before a step (pre_halo), a held-back PPCG inner step or Chebyshev
+// step and its held-back halo update (solvers.hip).
 int flush_held(tea_hip_chunk* c);
 // Mixed-precision residual update r -= alpha w with the partials of r.r
 // (and, with rw, of r.w as a second array) written to c->partials
@@ -713,10 +721,10 @@ int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
   TEA_HIP_TRY(hipSetDevice((c)->device))
 
 // Entry points other than the PPCG inner step and the halo update first run
-// a held-back PPCG or Chebyshev step.
-#define TEA_FLUSH_HELD(c)                                        \
-  do {                                                           \
-    if ((c)->held) TEA_RETURN_IF(::tea::flush_held(c));     \
+// a held-back halo update and PPCG or Chebyshev step.
+#define TEA_FLUSH_HELD(c)                                                  \
+  do {                                                                     \
+    if ((c)->held || (c)->pre_halo >= 0) TEA_RETURN_IF(::tea::flush_held(c)); \
   } while (0)
 
 #define TEA_ENTER(c)   \

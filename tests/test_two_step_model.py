@@ -41,12 +41,28 @@ def test_tile_of_block_is_a_permutation(nt):
     assert sorted(tile_of_block(b, nt) for b in range(nt)) == list(range(nt))
 
 
+def reflect_staged_ring(S0, r0, c0, h, xe, ye):
+    """The kernels' held-back update before step k: ring cells of the staged
+    area take their mirror's value (double mirror at the corners)."""
+    upd = {}
+    for a in range(S0H):
+        for b in range(S0W):
+            jj, kc = r0 - 2 + a, c0 - 2 + b
+            if not (h - 1 <= jj <= ye and h - 1 <= kc <= xe) or (h <= jj < ye and h <= kc < xe):
+                continue
+            mj = h if jj == h - 1 else (ye - 1 if jj == ye else jj)
+            mk = h if kc == h - 1 else (xe - 1 if kc == xe else kc)
+            upd[(a, b)] = S0[mj - r0 + 2][mk - c0 + 2]
+    for (a, b), v in upd.items():
+        S0[a][b] = v
+
+
 def op5(c, l, r, b, t, kx0, kx1, ky0, ky1):
     diag = (1.0 + (kx1 + kx0)) + (ky1 + ky0)
     return (diag * c - (kx1 * r + kx0 * l)) - (ky1 * t + ky0 * b)
 
 def model_ppcg(x, y, h, sd_old, sd_new, r, r_new, u, kx, ky, a0, b0, a1, b1,
-               p_pend=None, alpha_pend=0.0):
+               p_pend=None, alpha_pend=0.0, pre_halo=False):
     n = x * y
     def G(arr, i):
         assert 0 <= i < n, ("read OOB", i)
@@ -68,6 +84,9 @@ def model_ppcg(x, y, h, sd_old, sd_new, r, r_new, u, kx, ky, a0, b0, a1, b1,
             a, b = q // S0W, q % S0W
             jj, kk = r0 - 2 + a, c0 - 2 + b
             S0[a][b] = G(sd_old, kk + jj * x) if (0 <= jj < y and 0 <= kk < x) else 0.0
+        if pre_halo and (c0 - 2 <= h - 1 or c0 + TX + 1 >= xe or
+                         r0 - 2 <= h - 1 or r0 + TY + 1 >= ye):
+            reflect_staged_ring(S0, r0, c0, h, xe, ye)
         regs = {}
         for t in range(TX):
             kk = c0 + t
@@ -151,7 +170,8 @@ def model_ppcg(x, y, h, sd_old, sd_new, r, r_new, u, kx, ky, a0, b0, a1, b1,
     assert (written == 1).all(), ("cells written != once", np.flatnonzero(written != 1)[:10])
 
 
-def model_cheby(x, y, h, u_old, u_new, u0, p_old, p_new, r, w, kx, ky, a0, b0, a1, b1):
+def model_cheby(x, y, h, u_old, u_new, u0, p_old, p_new, r, w, kx, ky, a0, b0, a1, b1,
+                pre_halo=False):
     n = x * y
     def G(arr, i):
         assert 0 <= i < n, ("read OOB", i); return arr[i]
@@ -170,6 +190,9 @@ def model_cheby(x, y, h, u_old, u_new, u0, p_old, p_new, r, w, kx, ky, a0, b0, a
             a, b = q // S0W, q % S0W
             jj, kk = r0 - 2 + a, c0 - 2 + b
             S0[a][b] = G(u_old, kk + jj * x) if (0 <= jj < y and 0 <= kk < x) else 0.0
+        if pre_halo and (c0 - 2 <= h - 1 or c0 + TX + 1 >= xe or
+                         r0 - 2 <= h - 1 or r0 + TY + 1 >= ye):
+            reflect_staged_ring(S0, r0, c0, h, xe, ye)
         S1 = np.full((S1H, S1W), np.nan)
         regs = {}
         for t in range(TX):
@@ -250,12 +273,13 @@ GRIDS = [(3, 2, 2), (10, 7, 2), (TX + 44, 11, 2), (TX, TY, 2), (TX + 1, TY + 1, 
          (2 * TX + 1, 2 * TY + 1, 2), (40, 30, 1), (TX + 43, 13, 3)]
 
 
-@pytest.mark.parametrize("pending", [False, True])
+@pytest.mark.parametrize("pending,pre", [(False, False), (True, False), (True, True)])
 @pytest.mark.parametrize("xc,yc,h", GRIDS)
-def test_ppcg_two_step_model(oracle, xc, yc, h, pending):
+def test_ppcg_two_step_model(oracle, xc, yc, h, pending, pre):
     """With `pending`, u += alpha p of cg_calc_ur (interior cells) is applied
     by the pass before the steps, as the first pass of a PPCG outer step
-    does."""
+    does; with `pre`, the SD update before the first step is held back into
+    the pass, which reflects the ring it stages."""
     rng = np.random.default_rng(5)
     x, y = xc + 2 * h, yc + 2 * h
     n = x * y
@@ -268,6 +292,8 @@ def test_ppcg_two_step_model(oracle, xc, yc, h, pending):
     if pending:
         gu = g["u"].reshape(y, x)
         gu[h:y - h, h:x - h] = gu[h:y - h, h:x - h] + alpha * g["p"].reshape(y, x)[h:y - h, h:x - h]
+    if pre:
+        oracle.halo_field(x, y, h, 1, g["sd"])
     oracle.ppcg_inner_iteration(x, y, h, a0, b0, g)
     oracle.halo_field(x, y, h, 1, g["sd"])
     oracle.ppcg_inner_iteration(x, y, h, a1, b1, g)
@@ -275,13 +301,14 @@ def test_ppcg_two_step_model(oracle, xc, yc, h, pending):
     r_new = np.full(n, np.nan)
     u = f["u"].copy()
     model_ppcg(x, y, h, f["sd"], sd_new, f["r"], r_new, u, f["kx"], f["ky"], a0, b0, a1, b1,
-               f["p"] if pending else None, alpha)
+               f["p"] if pending else None, alpha, pre)
     for name, got, want in [("sd", sd_new, g["sd"]), ("r", r_new, g["r"]), ("u", u, g["u"])]:
         assert np.array_equal(got.view(np.int64), want.view(np.int64)), name
 
 
+@pytest.mark.parametrize("pre", [False, True])
 @pytest.mark.parametrize("xc,yc,h", GRIDS)
-def test_cheby_two_step_model(oracle, xc, yc, h):
+def test_cheby_two_step_model(oracle, xc, yc, h, pre):
     rng = np.random.default_rng(7)
     x, y = xc + 2 * h, yc + 2 * h
     n = x * y
@@ -290,6 +317,8 @@ def test_cheby_two_step_model(oracle, xc, yc, h):
     f["ky"] = rng.uniform(0, 5, n)
     g = {k: v.copy() for k, v in f.items()}
     a0, b0, a1, b1 = 0.3, 0.7, -0.45, 1.2
+    if pre:
+        oracle.halo_field(x, y, h, 1, g["u"])
     oracle.cheby_iterate(x, y, h, a0, b0, g)
     oracle.halo_field(x, y, h, 1, g["u"])
     oracle.cheby_iterate(x, y, h, a1, b1, g)
@@ -297,7 +326,7 @@ def test_cheby_two_step_model(oracle, xc, yc, h):
     p_new = np.full(n, np.nan)
     r, w = f["r"].copy(), f["w"].copy()
     model_cheby(x, y, h, f["u"], u_new, f["u0"], f["p"], p_new, r, w, f["kx"], f["ky"],
-                a0, b0, a1, b1)
+                a0, b0, a1, b1, pre)
     for name, got, want in [("u", u_new, g["u"]), ("p", p_new, g["p"]), ("r", r, g["r"]),
                             ("w", w, g["w"])]:
         assert np.array_equal(got.view(np.int64), want.view(np.int64)), name
