@@ -1,36 +1,32 @@
-// Chunk set-up (TeaLeaf/chunk.c:4-45): padded extents, neighbour list and
-// the host face buffers that a multi-rank exchange would use.
+// The host side of a chunk.  This host runs one chunk per process, covering
+// the whole grid, so a chunk carries only its padded extents, its first dt,
+// its place in the grid and its four neighbours, all of which are the domain
+// boundary.  The face buffers of a multi-rank halo exchange stay null; the
+// backend fills in the extension when the kernels are initialised.
 #include "chunk.h"
 
-#include <stdlib.h>
+#include <algorithm>
 
 void initialise_chunk(Chunk* chunk, Settings* settings, int x, int y) {
-  chunk->x = x + settings->halo_depth * 2;
-  chunk->y = y + settings->halo_depth * 2;
+  *chunk = Chunk{};
+  const int halo_cells = 2 * settings->halo_depth;
+  chunk->x = x + halo_cells;
+  chunk->y = y + halo_cells;
   chunk->dt_init = settings->dt_init;
-  chunk->neighbours = (int*)malloc(sizeof(int) * NUM_FACES);
-  const int lr_len = chunk->y * settings->halo_depth * NUM_FIELDS;
-  const int tb_len = chunk->x * settings->halo_depth * NUM_FIELDS;
-  chunk->left_send = (double*)malloc(sizeof(double) * lr_len);
-  chunk->left_recv = (double*)malloc(sizeof(double) * lr_len);
-  chunk->right_send = (double*)malloc(sizeof(double) * lr_len);
-  chunk->right_recv = (double*)malloc(sizeof(double) * lr_len);
-  chunk->top_send = (double*)malloc(sizeof(double) * tb_len);
-  chunk->top_recv = (double*)malloc(sizeof(double) * tb_len);
-  chunk->bottom_send = (double*)malloc(sizeof(double) * tb_len);
-  chunk->bottom_recv = (double*)malloc(sizeof(double) * tb_len);
-  chunk->ext = (ChunkExtension*)calloc(1, sizeof(ChunkExtension));
+
+  chunk->left = 0;
+  chunk->right = x;
+  chunk->bottom = 0;
+  chunk->top = y;
+  chunk->neighbours = new int[NUM_FACES];
+  std::fill(chunk->neighbours, chunk->neighbours + NUM_FACES, EXTERNAL_FACE);
+
+  chunk->ext = new ChunkExtension{};
 }
 
 void finalise_chunk(Chunk* chunk) {
-  free(chunk->neighbours);
-  free(chunk->ext);
-  free(chunk->left_send);
-  free(chunk->left_recv);
-  free(chunk->right_send);
-  free(chunk->right_recv);
-  free(chunk->top_send);
-  free(chunk->top_recv);
-  free(chunk->bottom_send);
-  free(chunk->bottom_recv);
+  delete[] chunk->neighbours;
+  delete chunk->ext;
+  chunk->neighbours = nullptr;
+  chunk->ext = nullptr;
 }

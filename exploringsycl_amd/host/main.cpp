@@ -59,9 +59,9 @@ static void parse_args(Settings* settings, int argc, char** argv) {
       else if (!strcmp(v, "ppcg")) settings->solver = PPCG_SOLVER;
       else if (!strcmp(v, "jacobi")) settings->solver = JACOBI_SOLVER;
     } else if (!strcmp(a, "-x") && has_val) {
-      settings->grid_x_cells = atoi(argv[++aa]);
+      g_opts.x_cells = atoi(argv[++aa]);
     } else if (!strcmp(a, "-y") && has_val) {
-      settings->grid_y_cells = atoi(argv[++aa]);
+      g_opts.y_cells = atoi(argv[++aa]);
     } else if (!strcmp(a, "--in") && has_val) {
       set_name(settings->tea_in_filename, argv[++aa]);
     } else if (!strcmp(a, "--out") && has_val) {
@@ -88,18 +88,6 @@ static void parse_args(Settings* settings, int argc, char** argv) {
 
 static void write_field(const char* path, Chunk* chunks, Settings* settings);
 
-// Puts the chunk back into its initial state: states, depth-2 halos and
-// energy := energy0 (the tail of initialise_application).
-static void reset_state(Chunk* chunks, Settings* settings) {
-  set_chunk_state_driver(chunks, settings, g_states);
-  reset_fields_to_exchange(settings);
-  settings->fields_to_exchange[FIELD_DENSITY] = true;
-  settings->fields_to_exchange[FIELD_ENERGY0] = true;
-  settings->fields_to_exchange[FIELD_ENERGY1] = true;
-  halo_update_driver(chunks, settings, 2);
-  store_energy_driver(chunks, settings);
-}
-
 int main(int argc, char** argv) {
   initialise_comms(argc, argv);
   Settings* settings = (Settings*)malloc(sizeof(Settings));
@@ -120,7 +108,7 @@ int main(int argc, char** argv) {
     settings->check_result = check;
     g_run = RunRecord();
     g_run.warmup_steps = g_opts.warmup_steps;
-    reset_state(chunks, settings);
+    restore_initial_state(chunks, settings);
     print_and_log(settings, "\n--- warm-up done, timed run follows ---\n");
   }
 
@@ -130,13 +118,12 @@ int main(int argc, char** argv) {
   if (tea_backend_mark) tea_backend_mark("timed_end");
   g_run.timed_seconds = now_seconds() - t0;
 
-  if (settings->rank == MASTER) PRINT_PROFILING_RESULTS(settings->kernel_profile);
   if (!g_opts.json_path.empty()) write_run_json(g_opts.json_path.c_str(), settings);
   if (!g_opts.field_out.empty()) write_field(g_opts.field_out.c_str(), chunks, settings);
 
   kernel_finalise_driver(chunks, settings);
-  for (int cc = 0; cc < settings->num_chunks_per_rank; ++cc) finalise_chunk(&chunks[cc]);
-  free(chunks);
+  finalise_chunk(chunks);
+  delete chunks;
   free(g_states);
   if (settings->tea_out_fp) fclose(settings->tea_out_fp);
   finalise_comms();

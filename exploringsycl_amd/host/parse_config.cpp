@@ -9,8 +9,7 @@
 //    state N; the bounds of states 2..n are moved inwards by 1/100 of a cell
 //    (parse_config.c:222-260) so that a bound on a cell edge cannot fall on
 //    either side by rounding.
-// x_cells / y_cells are only read while they still hold their defaults, so
-// that -x/-y on the command line win.
+// -x/-y on the command line override the deck's x_cells / y_cells.
 #include <ctype.h>
 #include <stdio.h>
 #include <string.h>
@@ -84,12 +83,8 @@ void read_settings(const std::vector<std::string>& lines, Settings* s) {
     if (get_double("ymin", line, &s->grid_y_min)) continue;
     if (get_double("xmax", line, &s->grid_x_max)) continue;
     if (get_double("ymax", line, &s->grid_y_max)) continue;
-    if (s->grid_x_cells == DEF_GRID_X_CELLS &&
-        get_int("x_cells", line, &s->grid_x_cells))
-      continue;
-    if (s->grid_y_cells == DEF_GRID_Y_CELLS &&
-        get_int("y_cells", line, &s->grid_y_cells))
-      continue;
+    if (get_int("x_cells", line, &s->grid_x_cells)) continue;
+    if (get_int("y_cells", line, &s->grid_y_cells)) continue;
     if (get_int("summary_frequency", line, &s->summary_frequency)) continue;
     if (get_int("presteps", line, &s->presteps)) continue;
     if (get_int("ppcg_inner_steps", line, &s->ppcg_inner_steps)) continue;
@@ -159,6 +154,9 @@ void read_settings(const std::vector<std::string>& lines, Settings* s) {
       continue;
     }
   }
+  // -x / -y on the command line take precedence over the deck.
+  if (g_opts.x_cells > 0) s->grid_x_cells = g_opts.x_cells;
+  if (g_opts.y_cells > 0) s->grid_y_cells = g_opts.y_cells;
   s->dx = (s->grid_x_max - s->grid_x_min) / (double)s->grid_x_cells;
   s->dy = (s->grid_y_max - s->grid_y_min) / (double)s->grid_y_cells;
 }

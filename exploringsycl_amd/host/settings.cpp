@@ -1,63 +1,79 @@
-// Default settings (values of TeaLeaf/settings.h:11-45, filled in as
-// TeaLeaf/settings.c:6-61 does; the profiles are zeroed here).
+// The settings a run starts from before tea.in is read.  A deck that omits a
+// key gets these values, so they are the reference's defaults
+// (TeaLeaf/settings.h:11-45); everything not named below starts at zero,
+// false or null.
 #include "settings.h"
 
-#include <string.h>
+#include <stdint.h>
 
-#define MAX_CHAR_LEN 256
+#include <algorithm>
+#include <string>
 
-static char* dup_name(const char* s) {
-  char* p = (char*)calloc(MAX_CHAR_LEN, 1);
-  strncpy(p, s, MAX_CHAR_LEN - 1);
-  return p;
+namespace {
+
+// The file-name and solver-name strings are 256-byte buffers that the
+// command line and the deck overwrite in place.
+char* name_buffer(const std::string& text) {
+  char* buffer = static_cast<char*>(calloc(256, 1));
+  text.copy(buffer, 255);
+  return buffer;
 }
 
+}  // namespace
+
 void set_default_settings(Settings* settings) {
-  memset(settings, 0, sizeof(*settings));
-  settings->test_problem_filename = dup_name(DEF_TEST_PROBLEM_FILENAME);
-  settings->tea_in_filename = dup_name(DEF_TEA_IN_FILENAME);
-  settings->tea_out_filename = dup_name(DEF_TEA_OUT_FILENAME);
-  settings->tea_out_fp = NULL;
-  settings->grid_x_min = DEF_GRID_X_MIN;
-  settings->grid_y_min = DEF_GRID_Y_MIN;
-  settings->grid_x_max = DEF_GRID_X_MAX;
-  settings->grid_y_max = DEF_GRID_Y_MAX;
-  settings->grid_x_cells = DEF_GRID_X_CELLS;
-  settings->grid_y_cells = DEF_GRID_Y_CELLS;
-  settings->dt_init = DEF_DT_INIT;
-  settings->max_iters = DEF_MAX_ITERS;
-  settings->eps = DEF_EPS;
-  settings->end_time = DEF_END_TIME;
-  settings->end_step = DEF_END_STEP;
-  settings->summary_frequency = DEF_SUMMARY_FREQUENCY;
-  settings->solver = DEF_SOLVER;
-  settings->coefficient = DEF_COEFFICIENT;
-  settings->error_switch = DEF_ERROR_SWITCH;
-  settings->presteps = DEF_PRESTEPS;
-  settings->eps_lim = DEF_EPS_LIM;
-  settings->check_result = DEF_CHECK_RESULT;
-  settings->ppcg_inner_steps = DEF_PPCG_INNER_STEPS;
-  settings->preconditioner = DEF_PRECONDITIONER;
-  settings->num_states = DEF_NUM_STATES;
-  settings->num_chunks = DEF_NUM_CHUNKS;
-  settings->num_chunks_per_rank = DEF_NUM_CHUNKS_PER_RANK;
-  settings->num_ranks = DEF_NUM_RANKS;
-  settings->halo_depth = DEF_HALO_DEPTH;
-  settings->is_offload = DEF_IS_OFFLOAD;
-  settings->kernel_language = C;
-  settings->kernel_profile = (struct Profile*)calloc(1, sizeof(struct Profile));
-  settings->application_profile = (struct Profile*)calloc(1, sizeof(struct Profile));
-  settings->wallclock_profile = (struct Profile*)calloc(1, sizeof(struct Profile));
-  settings->fields_to_exchange = (bool*)calloc(NUM_FIELDS, sizeof(bool));
-  settings->solver_name = dup_name("CG");
+  Settings s{};
+
+  // Mesh: 10 x 10 cells on [0, 100] x [0, 100], halos two cells deep.
+  s.grid_x_min = 0.0;
+  s.grid_y_min = 0.0;
+  s.grid_x_max = 100.0;
+  s.grid_y_max = 100.0;
+  s.grid_x_cells = 10;
+  s.grid_y_cells = 10;
+  s.halo_depth = 2;
+
+  // Time: steps of 0.1 up to t = 10; the step count is unbounded unless the
+  // deck sets end_step.
+  s.dt_init = 0.1;
+  s.end_time = 10.0;
+  s.end_step = INT32_MAX;
+  s.summary_frequency = 10;
+
+  // Solver: CG on K = density, stopping at an absolute |r|^2 of 1e-15 or
+  // after 10000 iterations.  presteps, eps_lim and ppcg_inner_steps steer
+  // the switch from CG to Chebyshev or PPCG.
+  s.solver = CG_SOLVER;
+  s.solver_name = name_buffer("CG");
+  s.coefficient = CONDUCTIVITY;
+  s.max_iters = 10000;
+  s.eps = 1.0e-15;
+  s.presteps = 30;
+  s.eps_lim = 1.0e-5;
+  s.ppcg_inner_steps = 10;
+  s.check_result = true;
+  s.kernel_language = C;
+
+  // One rank driving one chunk.
+  s.rank = 0;
+  s.num_ranks = 1;
+  s.num_chunks = 1;
+  s.num_chunks_per_rank = 1;
+
+  s.tea_in_filename = name_buffer("tea.in");
+  s.tea_out_filename = name_buffer("tea.out");
+  s.test_problem_filename = name_buffer("tea.problems");
+  s.fields_to_exchange = static_cast<bool*>(calloc(NUM_FIELDS, sizeof(bool)));
+
+  *settings = s;
 }
 
 void reset_fields_to_exchange(Settings* settings) {
-  for (int ii = 0; ii < NUM_FIELDS; ++ii) settings->fields_to_exchange[ii] = false;
+  std::fill(settings->fields_to_exchange, settings->fields_to_exchange + NUM_FIELDS, false);
 }
 
 bool is_fields_to_exchange(Settings* settings) {
-  for (int ii = 0; ii < NUM_FIELDS; ++ii)
-    if (settings->fields_to_exchange[ii]) return true;
-  return false;
+  return std::any_of(settings->fields_to_exchange,
+                     settings->fields_to_exchange + NUM_FIELDS,
+                     [](bool flagged) { return flagged; });
 }

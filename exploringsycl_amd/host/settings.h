@@ -7,43 +7,9 @@
 #ifndef TEA_SETTINGS_H
 #define TEA_SETTINGS_H
 
-#define __STDC_LIMIT_MACROS
-#include <stdint.h>
-
 #include "shared.h"
 
 #define NUM_FIELDS 6
-
-#define DEF_TEA_IN_FILENAME "tea.in"
-#define DEF_TEA_OUT_FILENAME "tea.out"
-#define DEF_TEST_PROBLEM_FILENAME "tea.problems"
-#define DEF_GRID_X_MIN 0.0
-#define DEF_GRID_Y_MIN 0.0
-#define DEF_GRID_X_MAX 100.0
-#define DEF_GRID_Y_MAX 100.0
-#define DEF_GRID_X_CELLS 10
-#define DEF_GRID_Y_CELLS 10
-#define DEF_DT_INIT 0.1
-#define DEF_MAX_ITERS 10000
-#define DEF_EPS 1.0E-15
-#define DEF_END_TIME 10.0
-#define DEF_END_STEP INT32_MAX
-#define DEF_SUMMARY_FREQUENCY 10
-#define DEF_COEFFICIENT CONDUCTIVITY
-#define DEF_ERROR_SWITCH 0
-#define DEF_PRESTEPS 30
-#define DEF_EPS_LIM 1E-5
-#define DEF_CHECK_RESULT 1
-#define DEF_PPCG_INNER_STEPS 10
-#define DEF_PRECONDITIONER 0
-#define DEF_SOLVER CG_SOLVER
-#define DEF_NUM_STATES 0
-#define DEF_NUM_CHUNKS 1
-#define DEF_NUM_CHUNKS_PER_RANK 1
-#define DEF_NUM_RANKS 1
-#define DEF_HALO_DEPTH 2
-#define DEF_RANK 0
-#define DEF_IS_OFFLOAD false
 
 /* The first four are the reference's; the rest are this project's
  * additions (appended, so the reference's values are unchanged). */

@@ -51,6 +51,8 @@ struct HostOptions {
   int warmup_steps = 0;    /* --warmup-steps W: run W steps, reset, then time */
   bool quiet = false;      /* --quiet: log to tea.out only */
   std::string field_out;   /* --field-out FILE: final interior u, raw float64 */
+  int x_cells = 0;         /* -x CELLS: overrides the deck's x_cells (0: none) */
+  int y_cells = 0;         /* -y CELLS: overrides the deck's y_cells (0: none) */
 };
 
 extern HostOptions g_opts;
