@@ -987,6 +987,31 @@ int tea_nn_train_step(tea_nn* n, int batch, const float* input, const float* tar
   return TEA_HIP_OK;
 }
 
+int tea_nn_adam_step(tea_nn* n, const float* grad, float lr) {
+  NN_CHECK(n);
+  TEA_CHECK_ARG(grad != nullptr, "null gradient");
+  TEA_HIP_TRY(hipSetDevice(n->device));
+  TEA_HIP_TRY(hipMemcpyAsync(n->grad, grad, kNumParams * sizeof(float),
+                             hipMemcpyHostToDevice, n->stream));
+  TEA_RETURN_IF(adam_step(n, lr));
+  TEA_HIP_TRY(hipStreamSynchronize(n->stream));
+  return TEA_HIP_OK;
+}
+
+int tea_nn_get_adam_state(tea_nn* n, float* m, float* v, size_t count, long long* step) {
+  NN_CHECK(n);
+  TEA_CHECK_ARG(m && v && step && count == (size_t)kNumParams,
+                "need two arrays of tea_nn_num_params() values and a step counter");
+  TEA_HIP_TRY(hipSetDevice(n->device));
+  TEA_HIP_TRY(hipMemcpyAsync(m, n->adam_m, count * sizeof(float), hipMemcpyDeviceToHost,
+                             n->stream));
+  TEA_HIP_TRY(hipMemcpyAsync(v, n->adam_v, count * sizeof(float), hipMemcpyDeviceToHost,
+                             n->stream));
+  TEA_HIP_TRY(hipStreamSynchronize(n->stream));
+  *step = n->adam_t;
+  return TEA_HIP_OK;
+}
+
 int tea_nn_load_dataset(tea_nn* n, int count, const float* inputs, const float* targets) {
   NN_CHECK(n);
   TEA_CHECK_ARG(count >= 1 && inputs && targets, "empty dataset");

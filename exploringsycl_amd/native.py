@@ -95,7 +95,7 @@ class TeaHipError(RuntimeError):
 def header_symbols(path=HEADER_PATH):
     """Names of the functions declared in include/tea_hip.h."""
     text = open(path).read()
-    return sorted(set(re.findall(r"\b(tea_hip_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(tea_(?:hip|nn)_\w+)\s*\(", text)))
 
 
 def load(path=LIB_PATH):

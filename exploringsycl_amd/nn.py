@@ -15,23 +15,28 @@ _F = ctypes.POINTER(ctypes.c_float)
 _I = ctypes.POINTER(ctypes.c_int)
 
 
+_vp, _i, _f, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t
+# Argument types of the tea_nn_* functions of include/tea_hip.h.
+SIGNATURES = {
+    "tea_nn_num_params": [],
+    "tea_nn_create": [_i, _i, _i, _i, ctypes.POINTER(_vp)],
+    "tea_nn_destroy": [_vp],
+    "tea_nn_set_params": [_vp, _F, _sz],
+    "tea_nn_get_params": [_vp, _F, _sz],
+    "tea_nn_forward": [_vp, _i, _F, _F],
+    "tea_nn_loss_grad": [_vp, _i, _F, _F, _F, _F],
+    "tea_nn_train_step": [_vp, _i, _F, _F, _f, _F],
+    "tea_nn_adam_step": [_vp, _F, _f],
+    "tea_nn_get_adam_state": [_vp, _F, _F, _sz, ctypes.POINTER(ctypes.c_longlong)],
+    "tea_nn_load_dataset": [_vp, _i, _F, _F],
+    "tea_nn_train_epoch": [_vp, _I, _i, _i, _f, _F],
+}
+
+
 def _bind(lib):
     if getattr(lib, "_nn_bound", False):
         return lib
-    vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
-    sig = {
-        "tea_nn_num_params": [],
-        "tea_nn_create": [i, i, i, i, ctypes.POINTER(vp)],
-        "tea_nn_destroy": [vp],
-        "tea_nn_set_params": [vp, _F, ctypes.c_size_t],
-        "tea_nn_get_params": [vp, _F, ctypes.c_size_t],
-        "tea_nn_forward": [vp, i, _F, _F],
-        "tea_nn_loss_grad": [vp, i, _F, _F, _F, _F],
-        "tea_nn_train_step": [vp, i, _F, _F, f, _F],
-        "tea_nn_load_dataset": [vp, i, _F, _F],
-        "tea_nn_train_epoch": [vp, _I, i, i, f, _F],
-    }
-    for name, args in sig.items():
+    for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
@@ -97,6 +102,20 @@ class Net:
         check(self.lib.tea_nn_train_step(self.handle, x.shape[0], _fp(x), _fp(t), lr,
                                          ctypes.byref(loss)), "train_step")
         return loss.value
+
+    def adam_step(self, grad, lr):
+        """One Adam step on the master parameters with the given gradient."""
+        grad = np.ascontiguousarray(grad, dtype=np.float32)
+        check(self.lib.tea_nn_adam_step(self.handle, _fp(grad), lr), "adam_step")
+
+    def adam_state(self):
+        """(m, v, step): the Adam moments and the number of steps taken."""
+        m = np.empty(self.num_params, dtype=np.float32)
+        v = np.empty(self.num_params, dtype=np.float32)
+        t = ctypes.c_longlong()
+        check(self.lib.tea_nn_get_adam_state(self.handle, _fp(m), _fp(v), m.size,
+                                             ctypes.byref(t)), "get_adam_state")
+        return m, v, t.value
 
     def load_dataset(self, inputs, targets):
         inputs = np.ascontiguousarray(inputs, dtype=np.float32)

@@ -319,6 +319,16 @@ int tea_nn_loss_grad(tea_nn* net, int batch, const float* input,
 /* tea_nn_loss_grad followed by one Adam step (beta 0.9/0.999, eps 1e-8). */
 int tea_nn_train_step(tea_nn* net, int batch, const float* input,
                       const float* target, float lr, float* loss);
+/* One Adam step (Kingma & Ba 2015, Algorithm 1, with bias correction:
+ * beta 0.9/0.999, eps 1e-8 added to the square root of the corrected second
+ * moment) on the master parameters with the host gradient `grad`
+ * (tea_nn_num_params() values), then the BF16 copies are repacked.
+ * tea_nn_train_step is tea_nn_loss_grad followed by this step. */
+int tea_nn_adam_step(tea_nn* net, const float* grad, float lr);
+/* The Adam moments m and v (count = tea_nn_num_params() each) and the
+ * number of steps taken since the last tea_nn_set_params. */
+int tea_nn_get_adam_state(tea_nn* net, float* m, float* v, size_t count,
+                          long long* step);
 /* Copies a training set of `count` samples to device memory. */
 int tea_nn_load_dataset(tea_nn* net, int count, const float* inputs,
                         const float* targets);

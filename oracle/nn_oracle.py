@@ -144,3 +144,24 @@ def loss_grad(params, x, target, bf16_mode=True):
             delta = rnd(delta).astype(np.float64)
     flat = np.concatenate([np.concatenate([g.ravel(), gb.ravel()]) for g, gb in grads])
     return loss, flat
+
+
+ADAM_BETA1, ADAM_BETA2, ADAM_EPS = 0.9, 0.999, 1e-8
+
+
+def adam_step(params, grad, m, v, t, lr, beta1=ADAM_BETA1, beta2=ADAM_BETA2,
+              eps=ADAM_EPS):
+    """One step of Adam (Kingma & Ba 2015, Algorithm 1) in float64.
+
+    t is the 1-based number of this step.  Returns (params, m, v) after it:
+        m <- beta1 m + (1 - beta1) g
+        v <- beta2 v + (1 - beta2) g^2
+        params <- params - lr (m / (1 - beta1^t)) / (sqrt(v / (1 - beta2^t)) + eps)
+    """
+    g = np.asarray(grad, dtype=np.float64)
+    m = beta1 * np.asarray(m, dtype=np.float64) + (1.0 - beta1) * g
+    v = beta2 * np.asarray(v, dtype=np.float64) + (1.0 - beta2) * g * g
+    m_hat = m / (1.0 - beta1 ** t)
+    v_hat = v / (1.0 - beta2 ** t)
+    p = np.asarray(params, dtype=np.float64) - lr * m_hat / (np.sqrt(v_hat) + eps)
+    return p, m, v

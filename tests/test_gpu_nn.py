@@ -91,3 +91,52 @@ def test_training_reduces_loss():
     for _ in range(30):
         last = net.train_epoch(rng.permutation(64), b, 3e-3)
     assert last < 0.2 * first, (first, last)
+
+
+def _adam_close(net, p, m, v, t):
+    gp = net.get_params()
+    gm, gv, gt = net.adam_state()
+    assert gt == t
+    # FP32 on the GPU against the float64 restatement: a few roundings of
+    # each value (relative 1e-6), plus the FP32 spacing of the parameters.
+    assert np.allclose(gm, m, rtol=2e-6, atol=1e-30)
+    assert np.allclose(gv, v, rtol=2e-6, atol=1e-38)
+    assert np.allclose(gp, p, rtol=1e-6, atol=1e-8)
+
+
+def test_adam_step_matches_oracle():
+    """Three Adam steps with fixed gradients (large, small, zero and near
+    eps) against Kingma & Ba's algorithm in float64: master parameters,
+    both moments and the step count."""
+    rng = np.random.default_rng(7)
+    net = _net(1, 4, 32)
+    p = O.init_params(5).astype(np.float64)
+    net.set_params(p.astype(np.float32))
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    n = p.size
+    for t, lr in ((1, 1e-3), (2, 3e-3), (3, 1e-3)):
+        g = rng.standard_normal(n) * 10.0 ** rng.uniform(-9, 1, n)
+        g[:50] = 0.0
+        g[50:100] = 1e-8 * np.sign(g[50:100])
+        g = g.astype(np.float32)
+        net.adam_step(g, lr)
+        p, m, v = O.adam_step(p, g, m, v, t, np.float32(lr))
+        _adam_close(net, p, m, v, t)
+
+
+def test_train_steps_match_oracle_adam():
+    """tea_nn_train_step = the GPU gradient at the current parameters plus
+    one Adam step: three steps against the restatement fed those
+    gradients."""
+    p0, x, t = _case(6, 2, 8, 32)
+    net = _net(2, 8, 32)
+    net.set_params(p0)
+    p = p0.astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for step in (1, 2, 3):
+        _, g = net.loss_grad(x, t)
+        net.train_step(x, t, 1e-3)
+        p, m, v = O.adam_step(p, g, m, v, step, np.float32(1e-3))
+        _adam_close(net, p, m, v, step)

@@ -43,3 +43,40 @@ def test_transpose_is_adjoint():
     lhs = np.sum(O.conv3x3(a, w) * d)
     rhs = np.sum(a * O.conv3x3_transpose(d, w))
     assert np.isclose(lhs, rhs, rtol=1e-12)
+
+
+def test_adam_first_step_is_a_signed_step_of_lr():
+    # With bias correction, step 1 has m_hat = g and v_hat = g^2, so every
+    # parameter moves by lr * g / (|g| + eps): lr * sign(g) for |g| >> eps,
+    # nothing for g = 0, and lr / 2 at |g| = eps.
+    g = np.array([3.0, -0.25, 0.0, 1e-8, -4e-3])
+    p, m, v = O.adam_step(np.zeros(5), g, np.zeros(5), np.zeros(5), 1, 0.01)
+    assert np.allclose(p[[0, 1, 4]], [-0.01, 0.01, 0.01], rtol=1e-5)
+    assert p[2] == 0.0
+    assert np.isclose(p[3], -0.005, rtol=1e-6)
+    assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
+
+
+def test_adam_constant_gradient_moves_by_lr_every_step():
+    # With a constant gradient the bias-corrected moments equal g and g^2 at
+    # every step (m_t = (1 - b1^t) g exactly), so each step is lr * sign(g);
+    # a missing bias correction would shrink the early steps.
+    p, m, v = np.array([1.0, -2.0]), np.zeros(2), np.zeros(2)
+    g = np.array([0.5, -7.0])
+    for t in range(1, 6):
+        before = p
+        p, m, v = O.adam_step(p, g, m, v, t, 1e-3)
+        assert np.allclose(before - p, 1e-3 * np.sign(g), rtol=1e-7)
+        assert np.allclose(m, (1 - 0.9 ** t) * g, rtol=1e-12)
+        assert np.allclose(v, (1 - 0.999 ** t) * g * g, rtol=1e-12)
+
+
+def test_adam_second_step_closed_form():
+    # Step 2 after g1 then g2: m_hat = (b1 g1 + g2)/(1 + b1),
+    # v_hat = (b2 g1^2 + g2^2)/(1 + b2); this pins both betas.
+    g1, g2 = np.array([1.0, 0.2]), np.array([-0.5, 0.6])
+    p, m, v = O.adam_step(np.zeros(2), g1, np.zeros(2), np.zeros(2), 1, 0.1)
+    p2, _, _ = O.adam_step(p, g2, m, v, 2, 0.1)
+    mh = (0.9 * g1 + g2) / 1.9
+    vh = (0.999 * g1 ** 2 + g2 ** 2) / 1.999
+    assert np.allclose(p - p2, 0.1 * mh / (np.sqrt(vh) + 1e-8), rtol=1e-12)
