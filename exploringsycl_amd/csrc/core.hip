@@ -31,7 +31,7 @@ const char* const kKernelNames[K_NUM] = {
     "span_finish",    "jacobi_init",     "jacobi_copy_u", "jacobi_iterate",
     "jacobi_fused",   "cheby_init",      "cheby_iterate", "cheby_calc_u",
     "cheby_fused",    "ppcg_init",       "ppcg_calc_ur",  "ppcg_calc_sd",
-    "ppcg_fused",     "ppcg_two_step",   "cheby_two_step"};
+    "ppcg_fused",     "ppcg_two_step",   "cheby_two_step", "fcg_sweeps"};
 
 Timed::Timed(tea_hip_chunk* c_, int kid_, hipStream_t s)
     : c(c_), kid(kid_), stream(s ? s : c_->stream) {
@@ -399,6 +399,8 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky, c->fcg_z})
     if (b) (void)hipFree(b);
   if (c->fcg_cross) (void)hipFree(c->fcg_cross);
+  if (c->fcg_xsum) (void)hipFree(c->fcg_xsum);
+  if (c->fcg_g) (void)hipFree(c->fcg_g);
   if (c->fcg_side) {
     (void)hipStreamSynchronize(c->fcg_side);
     (void)hipStreamDestroy(c->fcg_side);
@@ -406,9 +408,10 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   if (c->fcg_fork) (void)hipEventDestroy(c->fcg_fork);
   if (c->fcg_join) (void)hipEventDestroy(c->fcg_join);
   for (FcgLevel& L : c->fcg_lv) {
-    for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale})
+    for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale, L.y})
       if (b) (void)hipFree(b);
     if (L.part) (void)hipFree(L.part);
+    if (L.tpart) (void)hipFree(L.tpart);
     if (L.net) tea_nn_destroy(L.net);
   }
   if (c->host_partials) (void)hipHostFree(c->host_partials);

@@ -555,6 +555,261 @@ __global__ __launch_bounds__(256) void gamma2_kernel(const double* __restrict__ 
   }
 }
 
+// ---- the coarse pipeline with Jacobi sweeps as the coarse solve ---------
+// Level 0 coarsens the interior by 8, level 1 by 32.  Every sum below is
+// taken in double in a fixed order, restated by the CPU oracle
+// (orc_fcg_restrict8 / _restrict4 / _sweeps / _gamma), so the whole
+// preconditioner is reproducible bit for bit.
+
+// Block means of r over 8 x 8 fine blocks (level 0) and, with rbar1, of
+// the level-0 means over 4 x 4 blocks (level 1).  Order: each half row of a
+// block (4 cells) as ((r0 + r1) + (r2 + r3)), the 8 half rows of a half
+// block pairwise ((q0 + q1) + (q2 + q3)) + ((q4 + q5) + (q6 + q7)), then the
+// two halves; level 1 likewise over its 4 x 4 level-0 means.
+// Grid (ceil(wc / 32), ceil(hc / 4)): wave w takes level-0 row 4 by + w,
+// lane l column 32 bx + l / 2 and half l % 2 of the block's rows (two
+// 8-byte loads per row when x and h are even).  With dis0 (dis1), also each
+// block's sum of (D^-1/2 rbar)^2 over its cells of the level into rms0
+// (rms1): the network's input scale.
+__global__ __launch_bounds__(256) void restrict8_kernel(
+    Geo g, int hc, int wc, const float* __restrict__ r, float* __restrict__ rbar0,
+    float* __restrict__ rbar1, const float* __restrict__ dis0,
+    const float* __restrict__ dis1, double* __restrict__ rms0, double* __restrict__ rms1) {
+  __shared__ float sm[4][32];
+  __shared__ double sq[4];
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+  const int X = blockIdx.x * 32 + (lane >> 1), Y = blockIdx.y * 4 + wv;
+  const int half = lane & 1;
+  const bool ok = X < wc && Y < hc;
+  double t = 0.0;
+  if (ok) {
+    const int base = (g.h + 8 * Y) * g.x + g.h + 8 * X + 4 * half;
+    double q[8];
+    if (((g.x | g.h) & 1) == 0) {
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const float2 a = ld2(r, base + b * g.x), c = ld2(r, base + b * g.x + 2);
+        q[b] = ((double)a.x + (double)a.y) + ((double)c.x + (double)c.y);
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const float* p = r + base + b * g.x;
+        q[b] = ((double)p[0] + (double)p[1]) + ((double)p[2] + (double)p[3]);
+      }
+    }
+    t = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
+  }
+  const double o = __shfl_xor(t, 1);
+  const float m = ok ? (float)((half ? o + t : t + o) / 64.0) : 0.0f;
+  if (ok && half == 0) rbar0[Y * wc + X] = m;
+  if (rbar1 == nullptr && rms0 == nullptr) return;  // uniform over the grid
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  if (half == 0) sm[wv][lane >> 1] = m;
+  double s2 = 0.0;
+  if (rms0 && ok && half == 0) {
+    const double v = (double)m * (double)dis0[Y * wc + X];
+    s2 = v * v;
+  }
+  for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off);
+  if (lane == 0) sq[wv] = s2;
+  __syncthreads();
+  if (threadIdx.x == 0 && rms0) rms0[bid] = (sq[0] + sq[1]) + (sq[2] + sq[3]);
+  if (rbar1 && threadIdx.x < 8) {
+    const int wc1 = wc >> 2, hc1 = hc >> 2;
+    const int X1 = blockIdx.x * 8 + threadIdx.x, Y1 = blockIdx.y;
+    double v2 = 0.0;
+    if (X1 < wc1 && Y1 < hc1) {
+      double rs[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float* p = &sm[j][4 * threadIdx.x];
+        rs[j] = ((double)p[0] + (double)p[1]) + ((double)p[2] + (double)p[3]);
+      }
+      const float m1 = (float)(((rs[0] + rs[1]) + (rs[2] + rs[3])) / 16.0);
+      rbar1[Y1 * wc1 + X1] = m1;
+      if (rms1) {
+        const double v = (double)m1 * (double)dis1[Y1 * wc1 + X1];
+        v2 = v * v;
+      }
+    }
+    if (rms1) {
+      for (int off = 4; off > 0; off >>= 1) v2 += __shfl_xor(v2, off);
+      if (threadIdx.x == 0) rms1[bid] = v2;
+    }
+  }
+}
+
+constexpr int kSweepTile = 16;
+constexpr int kMaxSweeps = 8;
+constexpr int kSweepR = kSweepTile + 2 * (kMaxSweeps + 1);  // staged side, at most
+
+// `sweeps` Jacobi sweeps on T y = rbar from y = 0 (the iteration of
+// TeaLeaf/c_kernels/sycl/jacobi.cpp:85-91 on the coarse operator: left,
+// bottom, right and top face coefficients kl = kc[i], kb = kc[n + i],
+// kr = kc[i + 1], kt = kc[n + i + wc], zero on the domain edge), in FP64,
+// over one 16 x 16 tile of the level.  The block stages rbar and the face
+// coefficients of the tile and the H = sweeps + 1 cells around it in LDS
+// (zero beyond the grid, where y then stays zero) and takes sweep s over
+// the cells within H - s of the tile, so that the last sweep leaves y
+// exact on the tile and the ring around it.  Then, per tile cell: y
+// rounded to FP32 into y, T y of the rounded y in the operand order of
+// TeaLeaf's SMVP (shared.h:59-63), and the tile's partial sums of y.rbar
+// and y.T y (the block's LDS tree) at part[tile] and part[ntiles + tile],
+// tile = blockIdx.y * gridDim.x + blockIdx.x; with xsum, the sums of T y
+// over the tile's 4 x 4 blocks (the level-1 cells; the cross term).
+__global__ __launch_bounds__(256) void sweeps_kernel(int hc, int wc, int sweeps,
+                                                     const float* __restrict__ kc,
+                                                     const float* __restrict__ rbar,
+                                                     float* __restrict__ y,
+                                                     double* __restrict__ part, int ntiles,
+                                                     double* __restrict__ xsum) {
+  __shared__ float s_rb[kSweepR * kSweepR];
+  __shared__ float s_kl[kSweepR * (kSweepR + 1)];  // left faces, one more column
+  __shared__ float s_kb[(kSweepR + 1) * kSweepR];  // bottom faces, one more row
+  __shared__ double s_y[2][kSweepR * kSweepR];
+  __shared__ double s_a[256], s_b[256], s_t[256];
+  const int H = sweeps + 1, R = kSweepTile + 2 * H;
+  const int X0 = blockIdx.x * kSweepTile - H, Y0 = blockIdx.y * kSweepTile - H;
+  const long n = (long)hc * wc;
+  for (int q = threadIdx.x; q < (R + 1) * (R + 1); q += 256) {
+    const int lx = q % (R + 1), ly = q / (R + 1);
+    const int X = X0 + lx, Y = Y0 + ly;
+    const bool in = X >= 0 && X < wc && Y >= 0 && Y < hc;
+    const long i = (long)Y * wc + X;
+    float kl = 0.0f, kb = 0.0f, rb = 0.0f;
+    if (in) {
+      kl = kc[i];
+      kb = kc[n + i];
+      rb = rbar[i];
+    }
+    if (ly < R) s_kl[ly * (R + 1) + lx] = kl;
+    if (lx < R) s_kb[ly * R + lx] = kb;
+    if (lx < R && ly < R) {
+      s_rb[ly * R + lx] = rb;
+      s_y[0][ly * R + lx] = 0.0;
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int s = 1; s <= sweeps; ++s) {
+    const int lo = s, w = R - 2 * s;
+    const double* yp = s_y[cur];
+    double* yn = s_y[cur ^ 1];
+    for (int q = threadIdx.x; q < w * w; q += 256) {
+      const int lx = lo + q % w, ly = lo + q / w;
+      const int k = ly * R + lx;
+      const double kl = s_kl[ly * (R + 1) + lx], kr = s_kl[ly * (R + 1) + lx + 1];
+      const double kb = s_kb[ly * R + lx], kt = s_kb[(ly + 1) * R + lx];
+      yn[k] = ((double)s_rb[k] + (kr * yp[k + 1] + kl * yp[k - 1]) +
+               (kt * yp[k + R] + kb * yp[k - R])) /
+              ((1.0 + (kl + kr)) + (kb + kt));
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  const double* yv = s_y[cur];
+  const int tx = threadIdx.x & (kSweepTile - 1), ty = threadIdx.x / kSweepTile;
+  const int lx = H + tx, ly = H + ty, k = ly * R + lx;
+  const int X = blockIdx.x * kSweepTile + tx, Y = blockIdx.y * kSweepTile + ty;
+  auto yr = [&](int kk) { return (double)(float)yv[kk]; };
+  const double kl = s_kl[ly * (R + 1) + lx], kr = s_kl[ly * (R + 1) + lx + 1];
+  const double kb = s_kb[ly * R + lx], kt = s_kb[(ly + 1) * R + lx];
+  const double yc = yr(k);
+  const double tyv = (((1.0 + (kr + kl)) + (kt + kb)) * yc - (kr * yr(k + 1) + kl * yr(k - 1))) -
+                     (kt * yr(k + R) + kb * yr(k - R));
+  double a = 0.0, b = 0.0;
+  if (X < wc && Y < hc) {
+    y[(long)Y * wc + X] = (float)yv[k];
+    a = yc * (double)s_rb[k];
+    b = yc * tyv;
+  }
+  s_a[threadIdx.x] = a;
+  s_b[threadIdx.x] = b;
+  s_t[threadIdx.x] = tyv;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s_a[threadIdx.x] += s_a[threadIdx.x + st];
+      s_b[threadIdx.x] += s_b[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+    part[tile] = s_a[0];
+    part[ntiles + tile] = s_b[0];
+  }
+  if (xsum && threadIdx.x < 16) {
+    const int bx = threadIdx.x & 3, by = threadIdx.x >> 2;
+    const int X1 = blockIdx.x * 4 + bx, Y1 = blockIdx.y * 4 + by;
+    if (X1 < (wc >> 2) && Y1 < (hc >> 2)) {
+      double rs[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double* p = &s_t[(4 * by + j) * kSweepTile + 4 * bx];
+        rs[j] = (p[0] + p[1]) + (p[2] + p[3]);
+      }
+      xsum[(long)Y1 * (wc >> 2) + X1] = (rs[0] + rs[1]) + (rs[2] + rs[3]);
+    }
+  }
+}
+
+// The factors g of the coarse directions, into g[0] and g[1].  One level:
+// g = (y.rbar) / (y.T y), 0 unless y.T y > 0 and the ratio is finite.  Two
+// levels: the joint minimiser of the energy norm of e - g0 P0 y0 - g1 P1 y1
+// (the system of gamma2_kernel), whose cross entry c0^2 y0.T0 (Q y1) is the
+// sum over level-1 cells of xsum * y1 (T0 is symmetric).  Each sum in the
+// order of one 256-thread block: thread t adds elements t, t + 256, ... in
+// turn, then the LDS tree.
+__global__ __launch_bounds__(256) void joint_gamma_kernel(
+    int levels, const double* __restrict__ p0, int n0, const double* __restrict__ p1,
+    int n1, const double* __restrict__ xsum, const float* __restrict__ y1, int nx,
+    double c0sq, double c1sq, float* __restrict__ g) {
+  __shared__ double sh[5][256];
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < n0; k += 256) {
+    v[0] += p0[k];
+    v[1] += p0[n0 + k];
+  }
+  if (levels == 2) {
+    for (int k = threadIdx.x; k < n1; k += 256) {
+      v[2] += p1[k];
+      v[3] += p1[n1 + k];
+    }
+    for (int k = threadIdx.x; k < nx; k += 256) v[4] += xsum[k] * (double)y1[k];
+  }
+  for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double a0 = sh[0][0], b0 = sh[1][0];
+  if (levels == 1) {
+    const double q = a0 / b0;
+    g[0] = (b0 > 0.0 && isfinite(q)) ? (float)q : 0.0f;
+    g[1] = 0.0f;
+    return;
+  }
+  const double a1 = c0sq * a0, b11 = c0sq * b0;
+  const double a2 = c1sq * sh[2][0], b22 = c1sq * sh[3][0];
+  const double b12 = c0sq * sh[4][0];
+  const double det = b11 * b22 - b12 * b12;
+  double g1, g2;
+  if (b11 > 0.0 && b22 > 0.0 && det > 1e-12 * b11 * b22) {
+    g1 = (a1 * b22 - a2 * b12) / det;
+    g2 = (a2 * b11 - a1 * b12) / det;
+  } else {
+    g1 = b11 > 0.0 ? a1 / b11 : 0.0;
+    g2 = 0.0;
+  }
+  g[0] = isfinite(g1) ? (float)g1 : 0.0f;
+  g[1] = isfinite(g2) ? (float)g2 : 0.0f;
+}
+
 int ensure_fcg_buffers(tea_hip_chunk* c) {
   if (c->fcg_z) return TEA_HIP_OK;
   if (hipMalloc(&c->fcg_z, (size_t)c->geo.n * sizeof(float)) != hipSuccess) {
@@ -626,6 +881,58 @@ static CoarseTerm coarse_term(const tea_hip_chunk* c, int l) {
   return CoarseTerm{L.tmp, L.scale + 1, shift, L.wc};
 }
 
+// The coarse part of the two-level preconditioner with Jacobi sweeps as
+// the coarse solve: the restriction of both levels in one pass, the sweeps
+// of level 1 on the side stream beside level 0's, and the factors into
+// c->fcg_g.
+static int launch_sweeps_coarse(tea_hip_chunk* c) {
+  const int nl = c->fcg_levels;
+  FcgLevel& L0 = c->fcg_lv[0];
+  FcgLevel& L1 = c->fcg_lv[1];
+  {
+    Timed t(c, K_FCG_RESTRICT);
+    const dim3 grid((unsigned)(L0.wc + 31) / 32, (unsigned)(L0.hc + 3) / 4);
+    hipLaunchKernelGGL(restrict8_kernel, grid, dim3(256), 0, c->stream, c->geo, L0.hc, L0.wc,
+                       c->mp_r, L0.rbar, nl == 2 ? L1.rbar : nullptr, nullptr, nullptr,
+                       nullptr, nullptr);
+    TEA_HIP_TRY(hipGetLastError());
+  }
+  auto sweeps = [&](int l, hipStream_t st) -> int {
+    FcgLevel& L = c->fcg_lv[l];
+    Timed t(c, K_FCG_SWEEPS, st);
+    const dim3 grid((unsigned)(L.wc + kSweepTile - 1) / kSweepTile,
+                    (unsigned)(L.hc + kSweepTile - 1) / kSweepTile);
+    hipLaunchKernelGGL(sweeps_kernel, grid, dim3(256), 0, st, L.hc, L.wc, c->fcg_sweeps,
+                       L.kc, L.rbar, L.y, L.tpart, L.ntiles,
+                       (l == 0 && nl == 2) ? c->fcg_xsum : nullptr);
+    TEA_HIP_TRY(hipGetLastError());
+    return TEA_HIP_OK;
+  };
+  if (nl == 2) {
+    TEA_HIP_TRY(hipEventRecord(c->fcg_fork, c->stream));
+    TEA_HIP_TRY(hipStreamWaitEvent(c->fcg_side, c->fcg_fork, 0));
+    TEA_RETURN_IF(sweeps(1, c->fcg_side));
+    TEA_RETURN_IF(sweeps(0, c->stream));
+    TEA_HIP_TRY(hipEventRecord(c->fcg_join, c->fcg_side));
+    TEA_HIP_TRY(hipStreamWaitEvent(c->stream, c->fcg_join, 0));
+  } else {
+    TEA_RETURN_IF(sweeps(0, c->stream));
+  }
+  Timed t(c, K_FCG_SWEEPS);
+  hipLaunchKernelGGL(joint_gamma_kernel, dim3(1), dim3(256), 0, c->stream, nl, L0.tpart,
+                     L0.ntiles, L1.tpart, L1.ntiles, c->fcg_xsum, L1.y,
+                     nl == 2 ? L1.hc * L1.wc : 0, (double)L0.c * L0.c,
+                     (double)L1.c * L1.c, c->fcg_g);
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
+static int log2_of(int c) {
+  int shift = 0;
+  while ((1 << shift) < c) ++shift;
+  return shift;
+}
+
 // z = M(r) and the partials of z.w into array `slot` of c->partials (not
 // waited for); *len is the number of partials left to finish.
 static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len) {
@@ -634,7 +941,10 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
   const int nl = use_net ? c->fcg_levels : 0;
   double* partials = c->partials + (size_t)slot * c->nspan;
   *len = c->nspan;
-  if (nl == 1) {
+  const bool sweep = c->fcg_sweeps > 0;
+  if (nl > 0 && sweep) {
+    TEA_RETURN_IF(launch_sweeps_coarse(c));
+  } else if (nl == 1) {
     TEA_RETURN_IF(run_restrict(c, 0, c->stream));
     TEA_RETURN_IF(run_level(c, 0, c->stream, true));
   } else if (nl == 2) {
@@ -650,7 +960,7 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
     TEA_HIP_TRY(hipEventRecord(c->fcg_join, c->fcg_side));
     TEA_HIP_TRY(hipStreamWaitEvent(c->stream, c->fcg_join, 0));
   }
-  if (nl == 2) {
+  if (nl == 2 && !sweep) {
     FcgLevel& L1 = c->fcg_lv[0];
     FcgLevel& L2 = c->fcg_lv[1];
     const int n1 = (L1.hc * L1.wc + 255) / 256, n2 = (L2.hc * L2.wc + 255) / 256;
@@ -664,8 +974,12 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
     TEA_HIP_TRY(hipGetLastError());
   }
   const CoarseTerm none{nullptr, nullptr, 0, 0};
-  const CoarseTerm t0 = nl > 0 ? coarse_term(c, 0) : none;
-  const CoarseTerm t1 = nl > 1 ? coarse_term(c, 1) : none;
+  auto term = [&](int l) {
+    const FcgLevel& L = c->fcg_lv[l];
+    return sweep ? CoarseTerm{L.y, c->fcg_g + l, log2_of(L.c), L.wc} : coarse_term(c, l);
+  };
+  const CoarseTerm t0 = nl > 0 ? term(0) : none;
+  const CoarseTerm t1 = nl > 1 ? term(1) : none;
   const int kid = nl == 0 ? K_FCG_JACOBI : K_FCG_PROLONG;
   if (use_pair_blocks(c, 2)) {
     {
@@ -718,14 +1032,81 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
 }
 
 static void free_level(FcgLevel& L) {
-  for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale})
+  for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale, L.y})
     if (b) (void)hipFree(b);
   if (L.part) (void)hipFree(L.part);
+  if (L.tpart) (void)hipFree(L.tpart);
   if (L.net) tea_nn_destroy(L.net);
   L = FcgLevel();
 }
 
+// Frees every coarse level and the buffers shared between them.
+static void release_levels(tea_hip_chunk* c) {
+  for (FcgLevel& L : c->fcg_lv) free_level(L);
+  for (double** b : {&c->fcg_cross, &c->fcg_xsum})
+    if (*b) {
+      (void)hipFree(*b);
+      *b = nullptr;
+    }
+  if (c->fcg_g) (void)hipFree(c->fcg_g);
+  c->fcg_g = nullptr;
+  c->fcg_levels = 0;
+  c->fcg_sweeps = 0;
+}
+
+// The side stream and the fork / join events of a two-level coarse part.
+static int ensure_side_stream(tea_hip_chunk* c) {
+  if (!c->fcg_side) TEA_HIP_TRY(hipStreamCreateWithFlags(&c->fcg_side, hipStreamNonBlocking));
+  if (!c->fcg_fork) TEA_HIP_TRY(hipEventCreateWithFlags(&c->fcg_fork, hipEventDisableTiming));
+  if (!c->fcg_join) TEA_HIP_TRY(hipEventCreateWithFlags(&c->fcg_join, hipEventDisableTiming));
+  return TEA_HIP_OK;
+}
+
 extern "C" {
+
+int tea_hip_fcg_attach_sweeps(tea_hip_chunk* c, int sweeps, int coarsen, int levels) {
+  TEA_CHECK_ARG(c != nullptr, "null chunk handle");
+  TEA_CHECK_ARG(sweeps >= 1 && sweeps <= kMaxSweeps, "sweeps must be 1 to 8");
+  TEA_CHECK_ARG(coarsen == 8, "the coarse sweeps need a coarsening factor of 8");
+  TEA_CHECK_ARG(levels >= 1 && levels <= kFcgMaxLevels, "levels must be 1 or 2");
+  TEA_HIP_TRY(hipSetDevice(c->device));
+  TEA_FLUSH_HELD(c);
+  TEA_RETURN_IF(ensure_fcg_buffers(c));
+  release_levels(c);
+  const int nx = c->geo.x - 2 * c->geo.h, ny = c->geo.y - 2 * c->geo.h;
+  TEA_CHECK_ARG(nx % coarsen == 0 && ny % coarsen == 0,
+                "the interior must be a multiple of the coarsening factor");
+  auto fail = [] {
+    set_error("hipMalloc of the preconditioner's coarse buffers failed");
+    return TEA_HIP_ENOMEM;
+  };
+  // Level l coarsens by 8 * 4^l while the interior divides.
+  for (int l = 0; l < levels; ++l) {
+    const int cl = coarsen << (2 * l);
+    if (nx % cl != 0 || ny % cl != 0) break;
+    FcgLevel& L = c->fcg_lv[l];
+    L.c = cl;
+    L.hc = ny / cl;
+    L.wc = nx / cl;
+    L.ntiles = ((L.wc + kSweepTile - 1) / kSweepTile) * ((L.hc + kSweepTile - 1) / kSweepTile);
+    const size_t cn = (size_t)L.hc * L.wc;
+    if (hipMalloc(&L.kc, 2 * cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.rbar, cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.y, cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.tpart, 2 * (size_t)L.ntiles * sizeof(double)) != hipSuccess)
+      return fail();
+    c->fcg_levels = l + 1;
+  }
+  if (hipMalloc(&c->fcg_g, 2 * sizeof(float)) != hipSuccess) return fail();
+  TEA_HIP_TRY(hipMemsetAsync(c->fcg_g, 0, 2 * sizeof(float), c->stream));
+  if (c->fcg_levels == 2) {
+    const size_t n1 = (size_t)c->fcg_lv[1].hc * c->fcg_lv[1].wc;
+    if (hipMalloc(&c->fcg_xsum, n1 * sizeof(double)) != hipSuccess) return fail();
+    TEA_RETURN_IF(ensure_side_stream(c));
+  }
+  c->fcg_sweeps = sweeps;
+  return TEA_HIP_OK;
+}
 
 int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
                        const float* feature_norm, int coarsen, int levels) {
@@ -733,10 +1114,7 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(ensure_fcg_buffers(c));
-  for (FcgLevel& L : c->fcg_lv) free_level(L);
-  if (c->fcg_cross) (void)hipFree(c->fcg_cross);
-  c->fcg_cross = nullptr;
-  c->fcg_levels = 0;
+  release_levels(c);
   if (!params) return TEA_HIP_OK;  // Jacobi only
   TEA_CHECK_ARG(feature_norm != nullptr, "null feature normalisation");
   TEA_CHECK_ARG(coarsen >= 1 && (coarsen & (coarsen - 1)) == 0,
@@ -782,12 +1160,7 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
       set_error("hipMalloc of the preconditioner's cross partials failed");
       return TEA_HIP_ENOMEM;
     }
-    if (!c->fcg_side)
-      TEA_HIP_TRY(hipStreamCreateWithFlags(&c->fcg_side, hipStreamNonBlocking));
-    if (!c->fcg_fork)
-      TEA_HIP_TRY(hipEventCreateWithFlags(&c->fcg_fork, hipEventDisableTiming));
-    if (!c->fcg_join)
-      TEA_HIP_TRY(hipEventCreateWithFlags(&c->fcg_join, hipEventDisableTiming));
+    TEA_RETURN_IF(ensure_side_stream(c));
   }
   return TEA_HIP_OK;
 }
@@ -806,6 +1179,7 @@ static int fcg_features(tea_hip_chunk* c) {
     const dim3 grid(((unsigned)(L.hc * L.wc) + 255) / 256);
     hipLaunchKernelGGL(coarse_coef_kernel, grid, dim3(256), 0, c->stream, c->geo, L.c,
                        L.hc, L.wc, c->mp_kx, c->mp_ky, L.kc);
+    if (c->fcg_sweeps > 0) continue;  // the sweeps need no network features
     hipLaunchKernelGGL(feature_kernel, grid, dim3(256), 0, c->stream, L.hc, L.wc, L.kc,
                        c->fcg_norm[0], c->fcg_norm[1], c->fcg_norm[2], c->fcg_norm[3],
                        L.feat, L.dis);
@@ -826,7 +1200,7 @@ int tea_hip_fcg_precondition(tea_hip_chunk* c, int use_net, double* zw) {
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(c->fcg_z != nullptr && c->mp_r != nullptr, "FCG not initialised");
   TEA_CHECK_ARG(zw != nullptr, "null zw");
-  TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no network attached");
+  TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no coarse correction attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_FLUSH_HELD(c);
   int len;
@@ -839,7 +1213,7 @@ int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
   TEA_CHECK_ARG(c != nullptr, "null chunk handle");
   TEA_CHECK_ARG(c->fcg_z != nullptr && c->mp_r != nullptr, "FCG not initialised");
   TEA_CHECK_ARG(rr != nullptr && zw != nullptr, "null outputs");
-  TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no network attached");
+  TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no coarse correction attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_FLUSH_HELD(c);
   int lens[2] = {c->nspan, c->nspan};

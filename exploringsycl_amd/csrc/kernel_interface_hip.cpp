@@ -334,6 +334,12 @@ void run_fcg_attach(Chunk* chunk, Settings* settings, const float* params,
   HIP_CALL(tea_hip_fcg_attach(H(chunk), params, count, norm, coarsen, levels));
 }
 
+void run_fcg_attach_sweeps(Chunk* chunk, Settings* settings, int sweeps, int coarsen,
+                           int levels) {
+  (void)settings;
+  HIP_CALL(tea_hip_fcg_attach_sweeps(H(chunk), sweeps, coarsen, levels));
+}
+
 void run_fcg_init(Chunk* chunk, Settings* settings, double rx, double ry,
                   double* rr, double* bb) {
   for (int f = 0; f < NUM_FACES; ++f)

@@ -786,8 +786,10 @@ int loss_and_grad(tea_nn* n, int batch, float* loss) {
 int adam_step(tea_nn* n, float lr) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   n->adam_t += 1;
-  const float c1 = 1.0f - powf(b1, (float)n->adam_t);
-  const float c2 = 1.0f - powf(b2, (float)n->adam_t);
+  // Bias corrections 1 - beta^t of the FP32 betas, formed in double: in FP32,
+  // 1 - powf(0.999f, t) loses about 1e-5 of its value to cancellation.
+  const float c1 = (float)(1.0 - pow((double)b1, (double)n->adam_t));
+  const float c2 = (float)(1.0 - pow((double)b2, (double)n->adam_t));
   hipLaunchKernelGGL(adam_kernel, dim3((kNumParams + 255) / 256), dim3(256), 0,
                      n->stream, n->master, n->grad, n->adam_m, n->adam_v, kNumParams,
                      lr, b1, b2, eps, c1, c2);

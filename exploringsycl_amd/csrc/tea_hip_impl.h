@@ -467,6 +467,7 @@ enum KernelId {
   K_PPCG_FUSED,
   K_PPCG_TWO_STEP,
   K_CHEBY_TWO_STEP,
+  K_FCG_SWEEPS,
   K_NUM
 };
 
@@ -531,6 +532,11 @@ struct FcgLevel {
   float* tmp = nullptr;       // [hc][wc] row pass of the local scale
   double* part = nullptr;     // per-block partial sums (2 * blocks)
   float* scale = nullptr;     // {RMS of D^-1/2 rbar, prolongation factor}
+  // The coarse solve by Jacobi sweeps: its output y and the per-16x16-tile
+  // partials of y.rbar and y.T y (2 * ntiles).
+  float* y = nullptr;
+  double* tpart = nullptr;
+  int ntiles = 0;
 };
 constexpr int kFcgMaxLevels = 2;
 
@@ -605,6 +611,12 @@ struct tea_hip_chunk {
   int fcg_levels = 0;
   FcgLevel fcg_lv[kFcgMaxLevels];
   double* fcg_cross = nullptr;  // partials of the two levels' cross term
+  // Coarse solve by fcg_sweeps Jacobi sweeps instead of the network (0:
+  // network); its level-0 sums of T y over the level-1 blocks, and the
+  // factors g of the levels.
+  int fcg_sweeps = 0;
+  double* fcg_xsum = nullptr;
+  float* fcg_g = nullptr;
   // With two levels, the second level's chain runs on a side stream
   // (default priority) between a fork after the first level's restriction
   // and a join before the joint factors.

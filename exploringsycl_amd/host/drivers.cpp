@@ -290,9 +290,13 @@ void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
   const bool learned = settings->solver == FCG_SOLVER;
   static Chunk* attached = NULL;
   if (attached != c) {
-    if (learned) {
+    if (learned && g_extra.fcg_sweeps > 0) {
+      run_fcg_attach_sweeps(c, settings, g_extra.fcg_sweeps, 8, g_extra.fcg_levels);
+      g_run.coarse_solve = "jacobi sweeps " + std::to_string(g_extra.fcg_sweeps);
+    } else if (learned) {
       if (g_extra.fcg_weights.empty())
-        die(__LINE__, __FILE__, "use_fcg needs fcg_weights=<network file>\n");
+        die(__LINE__, __FILE__,
+            "use_fcg needs fcg_weights=<network file> or fcg_coarse_sweeps=<N>\n");
       NetWeights nw;
       std::string why;
       if (!read_net_weights(g_extra.fcg_weights.c_str(), &nw, &why))
@@ -300,6 +304,7 @@ void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
             g_extra.fcg_weights.c_str(), why.c_str());
       run_fcg_attach(c, settings, nw.params.data(), nw.params.size(), nw.norm,
                      nw.coarsen, g_extra.fcg_levels);
+      g_run.coarse_solve = "network";
     } else {
       run_fcg_attach(c, settings, NULL, 0, NULL, 0, 0);
     }
@@ -370,6 +375,7 @@ void fcg_driver(Chunk* chunks, Settings* settings, double rx, double ry,
     }
     beta = restart ? 0.0 : -zw / pw;
     c->cg_betas[it] = beta;
+    if (restart && !g_run.steps.empty()) g_run.steps.back().fallback_beta = beta;
     rr = rrn;
   }
   if (!converged && pend_u) {

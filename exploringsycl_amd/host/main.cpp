@@ -106,7 +106,9 @@ int main(int argc, char** argv) {
     diffuse(chunks, settings);
     settings->end_step = end_step;
     settings->check_result = check;
+    const std::string coarse_solve = g_run.coarse_solve;
     g_run = RunRecord();
+    g_run.coarse_solve = coarse_solve;
     g_run.warmup_steps = g_opts.warmup_steps;
     restore_initial_state(chunks, settings);
     print_and_log(settings, "\n--- warm-up done, timed run follows ---\n");
@@ -154,15 +156,17 @@ void write_run_json(const char* path, Settings* settings) {
           g_extra.rel_eps);
   fprintf(fp, "  \"warmup_steps\": %d, \"timed_seconds\": %.9f,\n",
           g_run.warmup_steps, g_run.timed_seconds);
+  fprintf(fp, "  \"coarse_solve\": \"%s\",\n", g_run.coarse_solve.c_str());
   fprintf(fp, "  \"steps\": [");
   for (size_t i = 0; i < g_run.steps.size(); ++i) {
     const StepRecord& s = g_run.steps[i];
     fprintf(fp, "%s\n    {\"step\": %d, \"iterations\": %d, \"executed\": %d, "
-                "\"refreshes\": %d, \"fallback_at\": %d, \"rel_residual\": %.17g, "
+                "\"refreshes\": %d, \"fallback_at\": %d, \"fallback_beta\": %.17g, "
+                "\"rel_residual\": %.17g, "
                 "\"error\": %.17g, \"seconds\": %.9f, \"solver_iterations\": %d, "
                 "\"estimated\": %d, \"eigmin\": %.17g, \"eigmax\": %.17g}",
             i ? "," : "", s.step, s.iterations, s.executed, s.refreshes,
-            s.fallback_at, s.rel_residual, s.error, s.seconds,
+            s.fallback_at, s.fallback_beta, s.rel_residual, s.error, s.seconds,
             s.solver_iterations, s.estimated, s.eigmin, s.eigmax);
   }
   fprintf(fp, "\n  ],\n  \"summaries\": [");

@@ -98,6 +98,7 @@ void read_settings(const std::vector<std::string>& lines, Settings* s) {
     if (get_int("refresh_interval", line, &g_extra.refresh_interval)) continue;
     if (get_int("stall_window", line, &g_extra.stall_window)) continue;
     if (get_int("fcg_levels", line, &g_extra.fcg_levels)) continue;
+    if (get_int("fcg_coarse_sweeps", line, &g_extra.fcg_sweeps)) continue;
     if (get_double("stall_factor", line, &g_extra.stall_factor)) continue;
     if (starts_with("fcg_weights", line)) {
       g_extra.fcg_weights = value_after(line, "fcg_weights");

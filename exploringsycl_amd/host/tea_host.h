@@ -17,6 +17,7 @@ struct StepRecord {
   int executed;       /* CG iterations actually run (printed + 1 on convergence) */
   int refreshes;      /* FP64 true-residual refreshes (mixed precision) */
   int fallback_at;    /* FCG: iteration at which Jacobi took over (-1: never) */
+  double fallback_beta = 0.0; /* FCG: the beta of that iteration (the restart) */
   double rel_residual; /* final |b - A u| / |b| where the solver computes it */
   double error;       /* last rr of the solve */
   double seconds;     /* wall time of the whole timestep */
@@ -41,6 +42,7 @@ struct RunRecord {
   bool check_passed = false;
   double timed_seconds = 0.0; /* the steps after warm-up, device-synchronised */
   int warmup_steps = 0;
+  std::string coarse_solve = "none"; /* FCG: "network", "jacobi sweeps N" or "none" */
 };
 
 extern RunRecord g_run;
@@ -63,7 +65,9 @@ struct SolverExtras {
   double rel_eps = 0.0;      /* rel_eps=: stop when |r|/|b| < rel_eps (0: off) */
   int refresh_interval = 50; /* refresh_interval=: FP64 true residual period */
   std::string fcg_weights;   /* fcg_weights=: the preconditioner network file */
-  int fcg_levels = 2;        /* fcg_levels=: coarse levels of the network (1 or 2) */
+  int fcg_levels = 2;        /* fcg_levels=: coarse levels (1 or 2) */
+  int fcg_sweeps = 0;        /* fcg_coarse_sweeps=: N > 0 replaces the network by N
+                                Jacobi sweeps on each coarse operator */
   int stall_window = 400;    /* stall_window=: FCG stagnation window (iterations) */
   double stall_factor = 4.0; /* stall_factor=: least |r| reduction over the window */
 };

@@ -70,6 +70,7 @@ _SIGNATURES = {
     "tea_hip_mpcg_refresh": [_vp, _d, _i, _dp, _dp, _dp],
     "tea_hip_mp_read": [_vp, _i, _fp, ctypes.c_size_t],
     "tea_hip_fcg_attach": [_vp, _fp, ctypes.c_size_t, _fp, _i, _i],
+    "tea_hip_fcg_attach_sweeps": [_vp, _i, _i, _i],
     "tea_hip_fcg_levels": [_vp, _ip],
     "tea_hip_fcg_init": [_vp, _i, _d, _d, _dp, _dp],
     "tea_hip_fcg_precondition": [_vp, _i, _dp],

@@ -359,12 +359,24 @@ int tea_nn_train_epoch(tea_nn* net, const int* order, int count, int batch,
 int tea_hip_fcg_attach(tea_hip_chunk* chunk, const float* params,
                        size_t count, const float* feature_norm, int coarsen,
                        int levels);
-/* The number of coarse levels the network runs on (0: Jacobi only). */
+/* The same two-level preconditioner with a non-learned coarse solve:
+ * y_l = `sweeps` Jacobi sweeps on T_l y = R_l r from y = 0 (the iteration of
+ * TeaLeaf/c_kernels/sycl/jacobi.cpp:85-91 on the coarse operator, FP64),
+ * with the energy-optimal factors g.  sweeps = 6 is what the network is
+ * trained to do; sweeps = 1 is y = D^-1 R r.  coarsen must be 8; level l
+ * coarsens by 8 * 4^l while the interior divides (no width condition).
+ * Every sum is taken in a fixed order: the results do not depend on the
+ * launch and are reproducible bit for bit. */
+int tea_hip_fcg_attach_sweeps(tea_hip_chunk* chunk, int sweeps, int coarsen,
+                              int levels);
+/* The number of coarse levels in use (0: Jacobi only). */
 int tea_hip_fcg_levels(tea_hip_chunk* chunk, int* levels);
-/* tea_hip_mpcg_init plus the network's coarse input features. */
+/* tea_hip_mpcg_init plus the coarse operators (and the network's input
+ * features). */
 int tea_hip_fcg_init(tea_hip_chunk* chunk, int coefficient, double rx,
                      double ry, double* rr, double* bb);
-/* z = M(r) with the network (use_net != 0) or Jacobi; *zw = z.w. */
+/* z = M(r) with the coarse correction (use_net != 0) or Jacobi alone;
+ * *zw = z.w. */
 int tea_hip_fcg_precondition(tea_hip_chunk* chunk, int use_net, double* zw);
 /* tea_hip_mpcg_calc_r then tea_hip_fcg_precondition with one wait:
  * r -= alpha w, *rr = r.r, z = M(r), *zw = z.w. */

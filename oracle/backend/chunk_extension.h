@@ -16,8 +16,10 @@ typedef struct ChunkExtension {
   float* mp_w;
   float* mp_kx;
   float* mp_ky;
-  /* Flexible CG: the preconditioned residual z (Jacobi only here). */
+  /* Flexible CG: the preconditioned residual z, and the coarse levels of
+   * the Jacobi-sweep coarse correction (NULL: Jacobi only). */
   float* fcg_z;
+  struct OracleCoarse* coarse;
 } ChunkExtension;
 
 #endif

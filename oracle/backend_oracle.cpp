@@ -19,6 +19,8 @@
 #define TEA_HAVE_EXT 1
 #endif
 
+static void free_coarse(struct OracleCoarse* q);
+
 static double* alloc_field(size_t n) {
   double* p = (double*)calloc(n ? n : 1, sizeof(double));
   if (!p) die(__LINE__, __FILE__, "Error allocating buffer\n");
@@ -57,6 +59,7 @@ void run_kernel_initialise(Chunk* c, Settings* s) {
   c->ext->mp_r = c->ext->mp_w = c->ext->mp_kx = c->ext->mp_ky = nullptr;
   c->ext->mp_cur = 0;
   c->ext->fcg_z = nullptr;
+  c->ext->coarse = nullptr;
   const int n = s->max_iters > 0 ? s->max_iters : 1;
   c->cg_alphas = new double[n]();
   c->cg_betas = new double[n]();
@@ -75,6 +78,8 @@ void run_kernel_finalise(Chunk* c, Settings* s) {
   float* g[] = {c->ext->mp_p[0], c->ext->mp_p[1], c->ext->mp_r, c->ext->mp_w,
                 c->ext->mp_kx, c->ext->mp_ky, c->ext->fcg_z};
   for (float* p : g) free(p);
+  free_coarse(c->ext->coarse);
+  c->ext->coarse = nullptr;
   delete[] c->cg_alphas;
   delete[] c->cg_betas;
   delete[] c->cheby_alphas;
@@ -265,7 +270,63 @@ void run_mpcg_refresh(Chunk* c, Settings* s, double alpha, int apply_u,
   *rw += out[2];
 }
 
-// ---- flexible CG (Jacobi preconditioner only) ----------------------------
+// ---- flexible CG: Jacobi, or Jacobi plus the two-level coarse correction
+// with Jacobi sweeps as the coarse solve (no network on the CPU) -----------
+
+// Level l coarsens the interior by 8 * 4^l; a level is used while the
+// interior divides.  Sizes and buffers of each level.
+struct OracleCoarse {
+  int levels = 0, sweeps = 0;
+  int c[2] = {0, 0}, hc[2] = {0, 0}, wc[2] = {0, 0}, ntiles[2] = {0, 0};
+  float* kc[2] = {nullptr, nullptr};
+  float* rbar[2] = {nullptr, nullptr};
+  float* y[2] = {nullptr, nullptr};
+  double* part[2] = {nullptr, nullptr};
+  double* cross = nullptr;
+};
+
+static void free_coarse(OracleCoarse* q) {
+  if (!q) return;
+  for (int l = 0; l < 2; ++l) {
+    free(q->kc[l]);
+    free(q->rbar[l]);
+    free(q->y[l]);
+    free(q->part[l]);
+  }
+  free(q->cross);
+  delete q;
+}
+
+void run_fcg_attach_sweeps(Chunk* c, Settings* s, int sweeps, int coarsen,
+                           int levels) {
+  if (sweeps < 1 || coarsen != 8 || levels < 1 || levels > 2)
+    die(__LINE__, __FILE__, "coarse sweeps need sweeps >= 1, coarsen 8, 1 or 2 levels\n");
+  if (!c->ext->fcg_z)
+    c->ext->fcg_z = (float*)calloc((size_t)c->x * c->y, sizeof(float));
+  free_coarse(c->ext->coarse);
+  OracleCoarse* q = new OracleCoarse();
+  q->sweeps = sweeps;
+  const int h = s->halo_depth, nx = c->x - 2 * h, ny = c->y - 2 * h;
+  for (int l = 0; l < levels; ++l) {
+    const int cl = coarsen << (2 * l);
+    if (nx % cl != 0 || ny % cl != 0) break;
+    q->c[l] = cl;
+    q->hc[l] = ny / cl;
+    q->wc[l] = nx / cl;
+    const size_t cn = (size_t)q->hc[l] * q->wc[l];
+    q->ntiles[l] = ((q->wc[l] + 15) / 16) * ((q->hc[l] + 15) / 16);
+    q->kc[l] = (float*)calloc(2 * cn, sizeof(float));
+    q->rbar[l] = (float*)calloc(cn, sizeof(float));
+    q->y[l] = (float*)calloc(cn, sizeof(float));
+    q->part[l] = (double*)calloc(2 * (size_t)q->ntiles[l], sizeof(double));
+    q->levels = l + 1;
+  }
+  if (q->levels == 0)
+    die(__LINE__, __FILE__, "the interior does not divide into 8 x 8 blocks\n");
+  if (q->levels == 2)
+    q->cross = (double*)calloc((size_t)q->hc[1] * q->wc[1], sizeof(double));
+  c->ext->coarse = q;
+}
 
 void run_read_u(Chunk* c, Settings* s, double* host) {
   (void)s;
@@ -279,19 +340,43 @@ void run_fcg_attach(Chunk* c, Settings* s, const float* params, size_t count,
     die(__LINE__, __FILE__, "the oracle backend has no network preconditioner\n");
   if (!c->ext->fcg_z)
     c->ext->fcg_z = (float*)calloc((size_t)c->x * c->y, sizeof(float));
+  free_coarse(c->ext->coarse);
+  c->ext->coarse = nullptr;
 }
 
 void run_fcg_init(Chunk* c, Settings* s, double rx, double ry, double* rr,
                   double* bb) {
   run_mpcg_init(c, s, rx, ry, rr, bb);
+  OracleCoarse* q = c->ext->coarse;
+  if (q)
+    for (int l = 0; l < q->levels; ++l)
+      orc_fcg_coarse_faces(c->x, c->y, s->halo_depth, q->c[l], c->ext->mp_kx,
+                           c->ext->mp_ky, q->kc[l]);
 }
 
 void run_fcg_precondition(Chunk* c, Settings* s, int use_net, double* zw) {
-  if (use_net)
-    die(__LINE__, __FILE__, "the oracle backend has no network preconditioner\n");
   ChunkExtension* e = c->ext;
-  *zw += orc_fcg_jacobi(c->x, c->y, s->halo_depth, e->mp_r, e->mp_kx, e->mp_ky,
-                        e->mp_w, e->fcg_z);
+  OracleCoarse* q = e->coarse;
+  const int h = s->halo_depth;
+  if (use_net && !q)
+    die(__LINE__, __FILE__, "the oracle backend has no network preconditioner\n");
+  if (!use_net) {
+    *zw += orc_fcg_jacobi(c->x, c->y, h, e->mp_r, e->mp_kx, e->mp_ky, e->mp_w,
+                          e->fcg_z);
+    return;
+  }
+  orc_fcg_restrict8(c->x, c->y, h, e->mp_r, q->rbar[0]);
+  if (q->levels == 2) orc_fcg_restrict4(q->hc[1], q->wc[1], q->rbar[0], q->rbar[1]);
+  for (int l = 0; l < q->levels; ++l)
+    orc_fcg_sweeps(q->hc[l], q->wc[l], q->kc[l], q->rbar[l], q->sweeps, q->y[l],
+                   q->part[l], l == 0 && q->levels == 2 ? q->cross : nullptr);
+  float g[2];
+  orc_fcg_gamma(q->levels, q->part[0], q->ntiles[0], q->part[1], q->ntiles[1],
+                q->cross, q->y[1], (long)q->hc[1] * q->wc[1],
+                (double)q->c[0] * q->c[0], (double)q->c[1] * q->c[1], g);
+  *zw += orc_fcg_prolong(c->x, c->y, h, e->mp_r, e->mp_kx, e->mp_ky, e->mp_w,
+                         q->levels, q->y[0], q->wc[0], q->c[0], g[0], q->y[1],
+                         q->wc[1], q->c[1], g[1], e->fcg_z);
 }
 
 void run_fcg_calc_r_precondition(Chunk* c, Settings* s, double alpha,

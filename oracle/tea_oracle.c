@@ -687,3 +687,218 @@ void orc_fcg_calc_w(int x, int y, int h, double alpha, int apply_u,
   GROUP_REDUCE(x, y, h, part, { val = (double)p_new[i] * (double)r[i]; });
   out[1] = finish(part, x, y);
 }
+
+/* ---- two-level coarse correction (no counterpart in the reference) -----
+ * Restates the coarse pipeline of exploringsycl_amd/csrc/fcg.hip with the
+ * non-learned coarse solve: restrict8_kernel (level 0: means of 8 x 8 fine
+ * blocks; level 1: means of 4 x 4 level-0 means), coarse_coef_kernel (the
+ * Galerkin face coefficients), sweeps_kernel (Jacobi sweeps on the coarse
+ * operator, the per-tile partials of y.rbar and y.T y, and the level-1
+ * block sums of T y), joint_gamma_kernel and fcg_z1_kernel.  Every sum is
+ * taken in the device's order, so the results are bitwise the device's.
+ * Coarse cell i of a (hc x wc) level has faces left kc[i], bottom
+ * kc[hc wc + i], right kc[i + 1] and top kc[hc wc + i + wc] (zero on the
+ * domain edge: the reflective boundary). */
+
+void orc_fcg_coarse_faces(int x, int y, int h, int c, const float* kx,
+                          const float* ky, float* kc) {
+  const int hc = (y - 2 * h) / c, wc = (x - 2 * h) / c;
+  const double cc = (double)c * (double)c;
+  for (int Y = 0; Y < hc; ++Y)
+    for (int X = 0; X < wc; ++X) {
+      const long corner = (long)(h + Y * c) * x + h + (long)X * c;
+      double sx = 0.0, sy = 0.0;
+      for (int b = 0; b < c; ++b) {
+        sx += (double)kx[corner + (long)b * x];
+        sy += (double)ky[corner + b];
+      }
+      const long i = (long)Y * wc + X;
+      kc[i] = X == 0 ? 0.0f : (float)(sx / cc);
+      kc[(long)hc * wc + i] = Y == 0 ? 0.0f : (float)(sy / cc);
+    }
+}
+
+void orc_fcg_restrict8(int x, int y, int h, const float* r, float* rbar) {
+  const int hc = (y - 2 * h) / 8, wc = (x - 2 * h) / 8;
+  for (int Y = 0; Y < hc; ++Y)
+    for (int X = 0; X < wc; ++X) {
+      double t[2];
+      for (int half = 0; half < 2; ++half) {
+        double q[8];
+        for (int b = 0; b < 8; ++b) {
+          const float* p = r + (long)(h + 8 * Y + b) * x + h + 8 * X + 4 * half;
+          q[b] = ((double)p[0] + (double)p[1]) + ((double)p[2] + (double)p[3]);
+        }
+        t[half] = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
+      }
+      rbar[(long)Y * wc + X] = (float)((t[0] + t[1]) / 64.0);
+    }
+}
+
+void orc_fcg_restrict4(int hc, int wc, const float* fine, float* coarse) {
+  const long wf = 4L * wc;
+  for (int Y = 0; Y < hc; ++Y)
+    for (int X = 0; X < wc; ++X) {
+      double rs[4];
+      for (int j = 0; j < 4; ++j) {
+        const float* p = fine + (4L * Y + j) * wf + 4L * X;
+        rs[j] = ((double)p[0] + (double)p[1]) + ((double)p[2] + (double)p[3]);
+      }
+      coarse[(long)Y * wc + X] = (float)(((rs[0] + rs[1]) + (rs[2] + rs[3])) / 16.0);
+    }
+}
+
+/* The four face coefficients of coarse cell (X, Y). */
+static void coarse_faces_at(int hc, int wc, const float* kc, int X, int Y,
+                            double* kl, double* kr, double* kb, double* kt) {
+  const long n = (long)hc * wc, i = (long)Y * wc + X;
+  *kl = (double)kc[i];
+  *kb = (double)kc[n + i];
+  *kr = X + 1 < wc ? (double)kc[i + 1] : 0.0;
+  *kt = Y + 1 < hc ? (double)kc[n + i + wc] : 0.0;
+}
+
+#define SWEEP_TILE 16
+
+/* Tree of the 256 slots of one 16 x 16 tile, as a 256-thread block's LDS
+ * tree (stride 128, 64, ..., 1). */
+static double tile_tree(double* s) {
+  for (int st = 128; st > 0; st >>= 1)
+    for (int t = 0; t < st; ++t) s[t] = s[t] + s[t + st];
+  return s[0];
+}
+
+void orc_fcg_sweeps(int hc, int wc, const float* kc, const float* rbar,
+                    int sweeps, float* y_out, double* part, double* cross) {
+  const long n = (long)hc * wc;
+  double* ya = (double*)calloc((size_t)n, sizeof(double));
+  double* yb = (double*)calloc((size_t)n, sizeof(double));
+  double* ty = (double*)calloc((size_t)n, sizeof(double));
+  /* Jacobi sweeps from zero (TeaLeaf/c_kernels/sycl/jacobi.cpp:85-91 on the
+   * coarse operator), in FP64. */
+  for (int s = 0; s < sweeps; ++s) {
+    for (int Y = 0; Y < hc; ++Y)
+      for (int X = 0; X < wc; ++X) {
+        const long i = (long)Y * wc + X;
+        double kl, kr, kb, kt;
+        coarse_faces_at(hc, wc, kc, X, Y, &kl, &kr, &kb, &kt);
+        const double yr = X + 1 < wc ? ya[i + 1] : 0.0;
+        const double yl = X > 0 ? ya[i - 1] : 0.0;
+        const double yt = Y + 1 < hc ? ya[i + wc] : 0.0;
+        const double ybt = Y > 0 ? ya[i - wc] : 0.0;
+        yb[i] = ((double)rbar[i] + (kr * yr + kl * yl) + (kt * yt + kb * ybt)) /
+                ((1.0 + (kl + kr)) + (kb + kt));
+      }
+    double* t = ya;
+    ya = yb;
+    yb = t;
+  }
+  for (long i = 0; i < n; ++i) y_out[i] = (float)ya[i];
+  /* T y with the rounded y, in the operand order of TeaLeaf's SMVP
+   * (shared.h:59-63). */
+  for (int Y = 0; Y < hc; ++Y)
+    for (int X = 0; X < wc; ++X) {
+      const long i = (long)Y * wc + X;
+      double kl, kr, kb, kt;
+      coarse_faces_at(hc, wc, kc, X, Y, &kl, &kr, &kb, &kt);
+      const double yc = (double)y_out[i];
+      const double yr = X + 1 < wc ? (double)y_out[i + 1] : 0.0;
+      const double yl = X > 0 ? (double)y_out[i - 1] : 0.0;
+      const double yt = Y + 1 < hc ? (double)y_out[i + wc] : 0.0;
+      const double ybt = Y > 0 ? (double)y_out[i - wc] : 0.0;
+      ty[i] = (((1.0 + (kr + kl)) + (kt + kb)) * yc - (kr * yr + kl * yl)) -
+              (kt * yt + kb * ybt);
+    }
+  const int ntx = (wc + SWEEP_TILE - 1) / SWEEP_TILE;
+  const int nty = (hc + SWEEP_TILE - 1) / SWEEP_TILE;
+  const int ntiles = ntx * nty;
+  for (int TY = 0; TY < nty; ++TY)
+    for (int TX = 0; TX < ntx; ++TX) {
+      double sa[256], sb[256];
+      for (int t = 0; t < 256; ++t) {
+        const int X = TX * SWEEP_TILE + t % SWEEP_TILE;
+        const int Y = TY * SWEEP_TILE + t / SWEEP_TILE;
+        sa[t] = sb[t] = 0.0;
+        if (X < wc && Y < hc) {
+          const long i = (long)Y * wc + X;
+          const double yc = (double)y_out[i];
+          sa[t] = yc * (double)rbar[i];
+          sb[t] = yc * ty[i];
+        }
+      }
+      part[TY * ntx + TX] = tile_tree(sa);
+      part[ntiles + TY * ntx + TX] = tile_tree(sb);
+    }
+  if (cross) {
+    const int hc2 = hc / 4, wc2 = wc / 4;
+    for (int Y = 0; Y < hc2; ++Y)
+      for (int X = 0; X < wc2; ++X) {
+        double rs[4];
+        for (int j = 0; j < 4; ++j) {
+          const double* p = ty + (4L * Y + j) * wc + 4L * X;
+          rs[j] = (p[0] + p[1]) + (p[2] + p[3]);
+        }
+        cross[(long)Y * wc2 + X] = (rs[0] + rs[1]) + (rs[2] + rs[3]);
+      }
+  }
+  free(ya);
+  free(yb);
+  free(ty);
+}
+
+/* Sum of v[0..n) (times (double)f[k] if f) in the order of one 256-thread
+ * block: thread t adds k = t, t + 256, ... in turn, then the LDS tree. */
+static double block_sum(const double* v, const float* f, long n) {
+  double s[256];
+  for (int t = 0; t < 256; ++t) {
+    double a = 0.0;
+    for (long k = t; k < n; k += 256) a += f ? v[k] * (double)f[k] : v[k];
+    s[t] = a;
+  }
+  return tile_tree(s);
+}
+
+void orc_fcg_gamma(int levels, const double* part0, int n0, const double* part1,
+                   int n1, const double* cross, const float* y1, long ncross,
+                   double c0sq, double c1sq, float g[2]) {
+  const double a0 = block_sum(part0, NULL, n0), b0 = block_sum(part0 + n0, NULL, n0);
+  g[0] = g[1] = 0.0f;
+  if (levels == 1) {
+    const double q = a0 / b0;
+    g[0] = (b0 > 0.0 && isfinite(q)) ? (float)q : 0.0f;
+    return;
+  }
+  const double a1 = c0sq * a0, b11 = c0sq * b0;
+  const double a2 = c1sq * block_sum(part1, NULL, n1);
+  const double b22 = c1sq * block_sum(part1 + n1, NULL, n1);
+  const double b12 = c0sq * block_sum(cross, y1, ncross);
+  const double det = b11 * b22 - b12 * b12;
+  double g1, g2;
+  if (b11 > 0.0 && b22 > 0.0 && det > 1e-12 * b11 * b22) {
+    g1 = (a1 * b22 - a2 * b12) / det;
+    g2 = (a2 * b11 - a1 * b12) / det;
+  } else {
+    g1 = b11 > 0.0 ? a1 / b11 : 0.0;
+    g2 = 0.0;
+  }
+  g[0] = isfinite(g1) ? (float)g1 : 0.0f;
+  g[1] = isfinite(g2) ? (float)g2 : 0.0f;
+}
+
+double orc_fcg_prolong(int x, int y, int h, const float* r, const float* kx,
+                       const float* ky, const float* w, int levels,
+                       const float* y0, int wc0, int c0, float g0,
+                       const float* y1, int wc1, int c1, float g1, float* z) {
+  double* part = alloc_partials(x, y);
+  GROUP_REDUCE(x, y, h, part, {
+    const double d = (1.0 + ((double)kx[i + 1] + (double)kx[i])) +
+                     ((double)ky[i + x] + (double)ky[i]);
+    double v = (double)r[i] / d;
+    if (levels > 0) v = v + (double)g0 * (double)y0[((jj - h) / c0) * wc0 + (kk - h) / c0];
+    if (levels > 1) v = v + (double)g1 * (double)y1[((jj - h) / c1) * wc1 + (kk - h) / c1];
+    const float zv = (float)v;
+    z[i] = zv;
+    val = (double)zv * (double)w[i];
+  });
+  return finish(part, x, y);
+}

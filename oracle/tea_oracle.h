@@ -156,6 +156,34 @@ void orc_fcg_calc_w(int x, int y, int h, double alpha, int apply_u,
                     const float* z, const float* r, double* u, float* w,
                     const float* kx, const float* ky, double out[2]);
 
+/* Two-level coarse correction of the flexible CG with Jacobi sweeps as the
+ * coarse solve (no reference counterpart; restates the coarse pipeline of
+ * exploringsycl_amd/csrc/fcg.hip in the device's summation order).
+ *   orc_fcg_coarse_faces: Galerkin face coefficients of the level that
+ *     coarsens by c (kc: hc*wc left faces, then hc*wc bottom faces).
+ *   orc_fcg_restrict8 / orc_fcg_restrict4: block means of r over 8 x 8 fine
+ *     cells, and of a level's means over 4 x 4 cells (hc, wc: coarse size).
+ *   orc_fcg_sweeps: `sweeps` Jacobi sweeps on T y = rbar from zero; y_out
+ *     (FP32), the per-16x16-tile partials of y.rbar (part[0..ntiles)) and
+ *     y.T y (part[ntiles..2 ntiles)), and, if cross != NULL, the sums of
+ *     T y over 4 x 4 blocks ((hc/4) x (wc/4)).
+ *   orc_fcg_gamma: the energy-optimal factors g of one level or jointly of
+ *     two (cross: level-0 block sums, y1: level-1 y, ncross cells).
+ *   orc_fcg_prolong: z = r / diag(A) + sum_l g_l y_l(block), returns z.w. */
+void orc_fcg_coarse_faces(int x, int y, int h, int c, const float* kx,
+                          const float* ky, float* kc);
+void orc_fcg_restrict8(int x, int y, int h, const float* r, float* rbar);
+void orc_fcg_restrict4(int hc, int wc, const float* fine, float* coarse);
+void orc_fcg_sweeps(int hc, int wc, const float* kc, const float* rbar,
+                    int sweeps, float* y_out, double* part, double* cross);
+void orc_fcg_gamma(int levels, const double* part0, int n0, const double* part1,
+                   int n1, const double* cross, const float* y1, long ncross,
+                   double c0sq, double c1sq, float g[2]);
+double orc_fcg_prolong(int x, int y, int h, const float* r, const float* kx,
+                       const float* ky, const float* w, int levels,
+                       const float* y0, int wc0, int c0, float g0,
+                       const float* y1, int wc1, int c1, float g1, float* z);
+
 /* The 5-point operator applied at one cell (shared.h:59-63). */
 double orc_smvp(const double* a, const double* kx, const double* ky, long i,
                 long x);

@@ -217,3 +217,85 @@ def test_trained_network_gains_over_jacobi(tmp_path, n, seed):
     assert a["iterations"] <= (2 * b["iterations"]) // 3
     ta, tb = learned_rec["summaries"][-1]["temp"], jacobi_rec["summaries"][-1]["temp"]
     assert abs(ta / tb - 1) < 1e-8
+
+
+# ---- the two-level preconditioner with Jacobi sweeps as the coarse solve ----
+
+def _sweep_deck(n, seed, extra, count=24, halo=None):
+    more = [f"halo_depth={halo}"] if halo else []
+    return Deck.with_random_rectangles(
+        n, n, count, problem_seed(5, seed), density="log10", density_range=(-2.0, 2.0),
+        end_step=1, coefficient="inverse", extent=10.0 * n / 4096,
+        extra=["rel_eps=1e-10"] + extra + more)
+
+
+def _same_run(tmp_path, deck):
+    """Runs the deck on the GPU and on the oracle host; asserts the same
+    iteration count, fallback, summaries and final field, bit for bit."""
+    from tests.oracle_ctypes import ORACLE_TEALEAF
+    gpu, _ = run_deck(deck, workdir=str(tmp_path / "gpu"), extra_args=["--field-out", "u.bin"])
+    cpu, _ = run_deck(deck, binary=ORACLE_TEALEAF, workdir=str(tmp_path / "cpu"),
+                      extra_args=["--field-out", "u.bin"])
+    a, b = gpu["steps"][0], cpu["steps"][0]
+    for k in ("iterations", "executed", "refreshes", "fallback_at", "rel_residual"):
+        assert a[k] == b[k], (k, a[k], b[k])
+    assert gpu["summaries"] == cpu["summaries"]
+    ug = np.fromfile(tmp_path / "gpu" / "u.bin")
+    uc = np.fromfile(tmp_path / "cpu" / "u.bin")
+    assert np.array_equal(ug, uc)
+    return gpu
+
+
+@pytest.mark.parametrize("n,sweeps,levels,halo", [
+    (64, 6, 2, None), (96, 6, 2, None), (128, 1, 2, None), (64, 3, 1, None),
+    (200, 6, 2, None), (256, 8, 2, None), (64, 6, 2, 3), (512, 6, 2, None)])
+def test_sweeps_fcg_matches_oracle_bitwise(tmp_path, n, sweeps, levels, halo):
+    """Flexible CG with the non-learned two-level preconditioner (restriction
+    of both levels, Jacobi sweeps on the coarse operators, the joint factors,
+    the prolongation) against the oracle's restatement, the whole solve:
+    every sum is in a fixed order, so iteration counts, summaries and the
+    final field agree bit for bit (odd x with halo depth 3; 200^2 has one
+    level, its 25 x 25 level 0 not a multiple of the 16-cell tile)."""
+    deck = _sweep_deck(n, n + sweeps, ["use_fcg", f"fcg_coarse_sweeps={sweeps}",
+                                        f"fcg_levels={levels}"], halo=halo)
+    rec = _same_run(tmp_path, deck)
+    assert rec["coarse_solve"] == f"jacobi sweeps {sweeps}"
+    assert rec["steps"][0]["rel_residual"] < 1e-10
+
+
+def test_sweeps_fcg_fallback_matches_oracle_bitwise(tmp_path):
+    """A stall rule no method meets: the fallback fires at the end of the
+    first window on the GPU as on the oracle, and both continue as the same
+    Jacobi-PCG to the same field."""
+    deck = _sweep_deck(96, 11, ["use_fcg", "fcg_coarse_sweeps=6", "stall_window=20",
+                                "stall_factor=1000"])
+    rec = _same_run(tmp_path, deck)
+    assert rec["steps"][0]["fallback_at"] == 20
+    assert rec["steps"][0]["fallback_beta"] == 0.0
+
+
+def test_learned_fcg_falls_back_and_converges(tmp_path):
+    """The learned preconditioner under a stall rule no method meets (and
+    under the specification's literal 2x per 20 iterations): the fallback
+    fires, the recursion restarts with beta = 0, and the solve reaches 1e-10
+    at the Jacobi-PCG temperature."""
+    if not os.path.exists(learned.WEIGHTS_PATH):
+        pytest.skip("no trained network file")
+    temps = {}
+    for name, rule in (("strict", ["stall_window=20", "stall_factor=1000"]),
+                       ("literal", ["stall_window=20", "stall_factor=2"])):
+        wd = tmp_path / name
+        wd.mkdir()
+        shutil.copy(learned.WEIGHTS_PATH, wd / "net.bin")
+        rec, _ = run_deck(_sweep_deck(512, 5, ["use_fcg", "fcg_weights=net.bin"] + rule,
+                                      count=64), workdir=str(wd))
+        st = rec["steps"][0]
+        assert st["fallback_at"] > 0 and st["fallback_beta"] == 0.0
+        if name == "strict":
+            assert st["fallback_at"] == 20
+        assert st["rel_residual"] < 1e-10
+        temps[name] = rec["summaries"][-1]["temp"]
+    jp, _ = run_deck(_sweep_deck(512, 5, ["use_jacobi_pcg"], count=64),
+                     workdir=str(tmp_path / "jp"))
+    for t in temps.values():
+        assert abs(t / jp["summaries"][-1]["temp"] - 1) < 1e-8
