@@ -10,9 +10,12 @@ timed (device-synchronised on both sides) by bin/tealeaf itself.
 A second part times the solve the metric names: one seeded 4096 x 4096
 field of 64 rectangles (log10 density uniform in [-2, 2], K = 1/density,
 seed 20269202, not among the training problems), one timestep solved
-to 1e-10 relative residual by plain FP64 CG, mixed-precision CG, Jacobi-PCG
-and flexible CG with the learned preconditioner; each after one warm-up
-step, with its iterations, time and final-field error against FP64 CG.
+to 1e-10 relative residual by plain FP64 CG, mixed-precision CG, Jacobi-PCG,
+flexible CG with the two-level preconditioner and a non-learned coarse solve
+(6 Jacobi sweeps, the network's training target; 1 sweep) and flexible CG
+with the learned preconditioner; each after one warm-up step, with its
+iterations, time and final-field error against FP64 CG.  The same field
+drawn at 8192 x 8192 is then solved by plain FP64 CG and learned FCG.
 
 Printed (one JSON line): effective HBM throughput of the timed solve,
   value = 88 B x N^2 x (CG iterations executed) / t_solve,
@@ -62,6 +65,9 @@ def parse_args():
                     help="grid of the time-to-solution comparison (default: --cells)")
     ap.add_argument("--tts-max-iters", type=int, default=12000,
                     help="iteration cap of the time-to-solution solves")
+    ap.add_argument("--tts-large", type=int, default=8192,
+                    help="grid of the second time-to-solution comparison, plain FP64 CG "
+                         "against learned FCG (0 skips it)")
     ap.add_argument("--solvers", type=int, default=1,
                     help="time CG, Chebyshev, PPCG and Jacobi on the known-answer "
                          "deck (0 skips it)")
@@ -136,9 +142,17 @@ def cpu_baseline(cells, seconds):
     }
 
 
-def time_to_solution(n, max_iters=12000, timeout=1200):
-    """Plain FP64 CG, MPCG, Jacobi-PCG and learned FCG on one unseen field of
-    the 64-rectangle workload at n x n, stopped at 1e-10 relative residual."""
+ALL_TTS_SOLVERS = ("plain_fp64_cg", "mixed_precision_cg", "jacobi_pcg",
+                   "two_level_jacobi_sweeps_6", "two_level_jacobi_sweeps_1",
+                   "learned_fcg")
+
+
+def time_to_solution(n, max_iters=12000, timeout=1200, names=ALL_TTS_SOLVERS):
+    """Plain FP64 CG, MPCG, Jacobi-PCG, the two-level flexible CG with a
+    non-learned coarse solve (6 Jacobi sweeps, the network's training
+    target, and 1 sweep, y = D^-1 R r) and learned FCG on one unseen field
+    of the 64-rectangle workload at n x n, stopped at 1e-10 relative
+    residual (the solvers in `names`)."""
     import numpy as np
     import shutil
     from exploringsycl_amd.deck import Deck, problem_seed
@@ -147,10 +161,13 @@ def time_to_solution(n, max_iters=12000, timeout=1200):
     from exploringsycl_amd.runner import run_deck
 
     seed = problem_seed(5, 3000)
-    solvers = [("plain_fp64_cg", []), ("mixed_precision_cg", ["use_mpcg"]),
-               ("jacobi_pcg", ["use_jacobi_pcg"])]
-    if os.path.exists(WEIGHTS_PATH):
-        solvers.append(("learned_fcg", ["use_fcg", "fcg_weights=net.bin"]))
+    extras = {"plain_fp64_cg": [], "mixed_precision_cg": ["use_mpcg"],
+              "jacobi_pcg": ["use_jacobi_pcg"],
+              "two_level_jacobi_sweeps_6": ["use_fcg", "fcg_coarse_sweeps=6"],
+              "two_level_jacobi_sweeps_1": ["use_fcg", "fcg_coarse_sweeps=1"],
+              "learned_fcg": ["use_fcg", "fcg_weights=net.bin"]}
+    solvers = [(k, extras[k]) for k in names
+               if k != "learned_fcg" or os.path.exists(WEIGHTS_PATH)]
     out = {"workload": (f"{n}x{n} cells, 64 seeded rectangles (log10 density "
                         f"U[-2,2], seed {seed}) + tea.in states, K=1/density, "
                         f"1 timestep, rel. residual 1e-10 (max {max_iters} iterations), "
@@ -326,6 +343,13 @@ def main():
         line["time_to_solution"] = tts
         sv = tts["solvers"]
         line["learned_fcg_vs_plain_cg_speedup"] = (
+            sv["learned_fcg"]["speedup_vs_plain_cg"] if "learned_fcg" in sv else None)
+    if args.tts and args.tts_large:
+        big = time_to_solution(args.tts_large, max_iters=8 * args.tts_max_iters,
+                               names=("plain_fp64_cg", "learned_fcg"))
+        line["time_to_solution_large"] = big
+        sv = big["solvers"]
+        line["learned_fcg_vs_plain_cg_speedup_large"] = (
             sv["learned_fcg"]["speedup_vs_plain_cg"] if "learned_fcg" in sv else None)
     if args.solvers:
         line["other_solvers"] = other_solvers(n, max_iters=args.solver_max_iters)

@@ -124,7 +124,19 @@ static int wait_partials(tea_hip_chunk* c) {
     if (q == nr) break;
     if (spins >= (1ull << 22) && (spins & 0xffff) == 0) {
       const hipError_t e = hipStreamQuery(c->stream);
-      if (e == hipSuccess) break;
+      if (e == hipSuccess) {
+        // The stream is idle, so every partial has been written: a sentinel
+        // left means a finisher wrote fewer partials than were waited for.
+        std::atomic_thread_fence(std::memory_order_acquire);
+        for (int qq = 0; qq < nr; ++qq)
+          for (int ii = 0; ii < len; ++ii)
+            if (v[qq * stride + ii] == kPollSentinel) {
+              set_error("reduction " + std::to_string(qq) + " left partial " +
+                        std::to_string(ii) + " unwritten");
+              return TEA_HIP_EINVAL;
+            }
+        break;
+      }
       if (e != hipErrorNotReady) TEA_HIP_TRY(e);
     }
   }
@@ -398,7 +410,6 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   if (c->sub) (void)hipFree(c->sub);
   for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky, c->fcg_z})
     if (b) (void)hipFree(b);
-  if (c->fcg_cross) (void)hipFree(c->fcg_cross);
   if (c->fcg_xsum) (void)hipFree(c->fcg_xsum);
   if (c->fcg_g) (void)hipFree(c->fcg_g);
   if (c->fcg_side) {
@@ -410,7 +421,7 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (FcgLevel& L : c->fcg_lv) {
     for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale, L.y})
       if (b) (void)hipFree(b);
-    if (L.part) (void)hipFree(L.part);
+    if (L.rms) (void)hipFree(L.rms);
     if (L.tpart) (void)hipFree(L.tpart);
     if (L.net) tea_nn_destroy(L.net);
   }

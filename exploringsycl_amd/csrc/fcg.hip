@@ -56,47 +56,6 @@ struct JacobiBody {
   }
 };
 
-// Coarse block means of r, and per-block partial sums of (D^-1/2 rbar)^2.
-__global__ __launch_bounds__(256) void restrict_kernel(Geo g, int c, int hc, int wc,
-                                                       const float* __restrict__ r,
-                                                       const float* __restrict__ dis,
-                                                       float* __restrict__ rbar,
-                                                       double* __restrict__ part) {
-  __shared__ double sh[256];
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  double v2 = 0.0;
-  if (idx < hc * wc) {
-  const int X = idx % wc, Y = idx / wc;
-  double sum = 0.0;
-  if (((g.x | g.h | c) & 1) == 0) {
-    // Every block row starts on an even index: 8-byte loads.
-    for (int b = 0; b < c; ++b) {
-      const int row = (g.h + Y * c + b) * g.x + g.h + X * c;
-      for (int a = 0; a < c; a += 2) {
-        const float2 v = ld2(r, row + a);
-        sum += (double)v.x + (double)v.y;
-      }
-    }
-  } else {
-    for (int b = 0; b < c; ++b) {
-      const int row = (g.h + Y * c + b) * g.x + g.h + X * c;
-      for (int a = 0; a < c; ++a) sum += (double)r[row + a];
-    }
-  }
-  const float m = (float)(sum / (double)(c * c));
-  rbar[idx] = m;
-  const double ms = (double)m * (double)dis[idx];
-  v2 = ms * ms;
-  }
-  sh[threadIdx.x] = v2;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
 // Local scale of the network input: sigma = RMS of D^-1/2 rbar over the
 // (2 kWin + 1)^2 window around each coarse cell (zero beyond the edge), in
 // two passes (rows into tmp, then columns), each summed in a fixed order,
@@ -311,33 +270,6 @@ __global__ __launch_bounds__(kBlock) void fcg_z1_kernel(
   wave_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
-// Block means of a coarser level from a finer one (factor f) and the
-// per-block partial sums of (D^-1/2 rbar)^2 of the coarser level.
-__global__ __launch_bounds__(256) void restrict_coarse_kernel(
-    int wf, int f, int hc, int wc, const float* __restrict__ fine,
-    const float* __restrict__ dis, float* __restrict__ rbar, double* __restrict__ part) {
-  __shared__ double sh[256];
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  double v2 = 0.0;
-  if (idx < hc * wc) {
-    const int X = idx % wc, Y = idx / wc;
-    double sum = 0.0;
-    for (int b = 0; b < f; ++b)
-      for (int a = 0; a < f; ++a) sum += (double)fine[(Y * f + b) * wf + X * f + a];
-    const float m = (float)(sum / (double)(f * f));
-    rbar[idx] = m;
-    const double ms = (double)m * (double)dis[idx];
-    v2 = ms * ms;
-  }
-  sh[threadIdx.x] = v2;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
 // The coarse operator T = P^T A P / c^2: for each coarse cell, the face
 // coefficients of its left and bottom faces (kc[0][..], kc[1][..]): the sum
 // of the c fine face coefficients along the face over c^2, zero on the
@@ -389,170 +321,6 @@ __global__ __launch_bounds__(256) void feature_kernel(int hc, int wc,
   feat[idx] = (float)((left - m0) / s0);
   feat[hc * wc + idx] = (float)((bottom - m1) / s1);
   dis[idx] = (float)di;
-}
-
-// The network's output back in T's variables, y = D^-1/2 sigma N (into y,
-// not in place: neighbours read N), and the per-block partial sums over
-// coarse cells of y.rbar and y.(T y), with T the coarse operator (identity
-// plus the 5-point operator of the face coefficients kc, zero beyond the
-// domain edge).  Neighbours' y are formed by the same expression.
-__global__ __launch_bounds__(256) void coarse_dots_kernel(int hc, int wc,
-                                                          const float* __restrict__ net,
-                                                          const float* __restrict__ dis,
-                                                          const float* __restrict__ sig,
-                                                          float* __restrict__ y,
-                                                          const float* __restrict__ rbar,
-                                                          const float* __restrict__ kc,
-                                                          double* __restrict__ part) {
-  __shared__ double s0[256], s1[256];
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  double a = 0.0, b = 0.0;
-  if (idx < hc * wc) {
-    auto ys = [&](int j) { return net[j] * (sig[j] * dis[j]); };
-    const int X = idx % wc, Y = idx / wc;
-    const float* kxc = kc;
-    const float* kyc = kc + hc * wc;
-    const float yf = ys(idx);
-    y[idx] = yf;
-    const double yc = yf;
-    const double kl = kxc[idx], kb = kyc[idx];
-    const double kr = X + 1 < wc ? (double)kxc[idx + 1] : 0.0;
-    const double kt = Y + 1 < hc ? (double)kyc[idx + wc] : 0.0;
-    double ty = (1.0 + kl + kr + kb + kt) * yc;
-    if (X > 0) ty -= kl * (double)ys(idx - 1);
-    if (X + 1 < wc) ty -= kr * (double)ys(idx + 1);
-    if (Y > 0) ty -= kb * (double)ys(idx - wc);
-    if (Y + 1 < hc) ty -= kt * (double)ys(idx + wc);
-    a = yc * (double)rbar[idx];
-    b = yc * ty;
-  }
-  s0[threadIdx.x] = a;
-  s1[threadIdx.x] = b;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) {
-      s0[threadIdx.x] += s0[threadIdx.x + st];
-      s1[threadIdx.x] += s1[threadIdx.x + st];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = s0[0];
-    part[gridDim.x + blockIdx.x] = s1[0];
-  }
-}
-
-// The A-optimal factor of the coarse correction P y: (y.rbar) / (y.T y)
-// (the minimiser of the energy norm of e - g P y, since (P y)^T r =
-// c^2 y.rbar and (P y)^T A (P y) = c^2 y.T y); 0 if y.T y is not positive.
-// Written to scale[1].
-__global__ __launch_bounds__(256) void gamma_kernel(const double* __restrict__ part,
-                                                    int nblk, float* __restrict__ scale) {
-  __shared__ double s0[256], s1[256];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += 256) {
-    a += part[i];
-    b += part[nblk + i];
-  }
-  s0[threadIdx.x] = a;
-  s1[threadIdx.x] = b;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) {
-      s0[threadIdx.x] += s0[threadIdx.x + st];
-      s1[threadIdx.x] += s1[threadIdx.x + st];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double g = s0[0] / s1[0];
-    scale[1] = (s1[0] > 0.0 && isfinite(g)) ? (float)g : 0.0f;
-  }
-}
-
-// Partial sums over level-1 cells of y1.(T1 Q y2): Q y2 is the level-2
-// correction on the level-1 grid (constant over blocks of f x f), T1 the
-// level-1 coarse operator.  With P2 = P1 Q, (P1 y1)^T A (P2 y2) =
-// c1^2 y1.(T1 Q y2): the cross term of the two coarse directions.
-__global__ __launch_bounds__(256) void cross_dots_kernel(int hc, int wc, int f, int wc2,
-                                                         const float* __restrict__ y1,
-                                                         const float* __restrict__ y2,
-                                                         const float* __restrict__ kc,
-                                                         double* __restrict__ part) {
-  __shared__ double sh[256];
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  double v = 0.0;
-  if (idx < hc * wc) {
-    const int X = idx % wc, Y = idx / wc;
-    const float* kxc = kc;
-    const float* kyc = kc + hc * wc;
-    auto q = [&](int xx, int yy) { return (double)y2[(yy / f) * wc2 + xx / f]; };
-    const double qc = q(X, Y);
-    const double kl = kxc[idx], kb = kyc[idx];
-    const double kr = X + 1 < wc ? (double)kxc[idx + 1] : 0.0;
-    const double kt = Y + 1 < hc ? (double)kyc[idx + wc] : 0.0;
-    double t = (1.0 + kl + kr + kb + kt) * qc;
-    if (X > 0) t -= kl * q(X - 1, Y);
-    if (X + 1 < wc) t -= kr * q(X + 1, Y);
-    if (Y > 0) t -= kb * q(X, Y - 1);
-    if (Y + 1 < hc) t -= kt * q(X, Y + 1);
-    v = (double)y1[idx] * t;
-  }
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-// Joint factors of the two coarse directions: minimise the energy norm of
-// e - g1 P1 y1 - g2 P2 y2, i.e. solve [B11 B12; B12 B22] g = a with
-// a_l = c_l^2 y_l.rbar_l, B_ll = c_l^2 y_l.T_l y_l, B12 = c1^2 y1.T1 Q y2
-// (the Gram matrix of the two directions in the A inner product, SPD when
-// they are independent).  Falls back to the separate factors a_l / B_ll
-// when it is not safely positive definite.  Writes scale1[1], scale2[1].
-__global__ __launch_bounds__(256) void gamma2_kernel(const double* __restrict__ p1, int n1,
-                                                     const double* __restrict__ p2, int n2,
-                                                     const double* __restrict__ px, int nx,
-                                                     double c1sq, double c2sq,
-                                                     float* __restrict__ scale1,
-                                                     float* __restrict__ scale2) {
-  __shared__ double sh[5][256];
-  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < n1; i += 256) {
-    v[0] += p1[i];
-    v[1] += p1[n1 + i];
-  }
-  for (int i = threadIdx.x; i < n2; i += 256) {
-    v[2] += p2[i];
-    v[3] += p2[n2 + i];
-  }
-  for (int i = threadIdx.x; i < nx; i += 256) v[4] += px[i];
-  for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st)
-      for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double a1 = c1sq * sh[0][0], b11 = c1sq * sh[1][0];
-    const double a2 = c2sq * sh[2][0], b22 = c2sq * sh[3][0];
-    const double b12 = c1sq * sh[4][0];
-    const double det = b11 * b22 - b12 * b12;
-    double g1, g2;
-    if (b11 > 0.0 && b22 > 0.0 && det > 1e-12 * b11 * b22) {
-      g1 = (a1 * b22 - a2 * b12) / det;
-      g2 = (a2 * b11 - a1 * b12) / det;
-    } else {
-      g1 = b11 > 0.0 ? a1 / b11 : 0.0;
-      g2 = 0.0;
-    }
-    scale1[1] = isfinite(g1) ? (float)g1 : 0.0f;
-    scale2[1] = isfinite(g2) ? (float)g2 : 0.0f;
-  }
 }
 
 // ---- the coarse pipeline with Jacobi sweeps as the coarse solve ---------
@@ -755,6 +523,382 @@ __global__ __launch_bounds__(256) void sweeps_kernel(int hc, int wc, int sweeps,
   }
 }
 
+// ---- the learned coarse solve of one level in one launch -----------------
+// net_level_kernel computes, for one 16 x 16 tile of a coarse level, what
+// box_rows_kernel, box_cols_kernel, the six layers of the network
+// (nn.hip: conv_*_kernel, out_layer_kernel) and the coarse dots compute for
+// it, keeping every intermediate in LDS: the staged rbar and D^-1/2 (the
+// tile and 13 cells around it), the local scale sigma and the packed BF16
+// input (7 cells around it: the network's receptive field plus the ring of
+// T y), and the activations of each layer on its shrinking region (28^2,
+// 26^2, ..., 20^2 pixels), then N and y = N sigma D^-1/2 on the tile and its
+// ring.  Each output pixel takes the same MFMA sequence (18 K-steps of 16
+// over (tap, channel), weights as the A operand, the pixel's patch as B) and
+// the same FP32 output layer as the per-layer kernels, so N is bitwise
+// theirs; pixels outside the grid are set to zero after every layer (the
+// zero ring of the per-layer kernels).  The tile's partials of y.rbar and
+// y.T y, the level-1 block sums of T y and y itself are those of
+// sweeps_kernel.
+namespace fused {
+constexpr int kT = 16;                  // output tile side
+constexpr int kRN = 7;                  // network radius 6 + the ring of T y
+constexpr int kS0 = kT + 2 * kRN;       // 30: network input region
+constexpr int kSV = kS0 + 2 * kWin;     // 42: staged rbar, D^-1/2
+constexpr int kSN = kT + 2;             // 18: network output region
+constexpr int kCh = 32;                 // hidden channels
+constexpr int kPx1 = (kS0 - 2) * (kS0 - 2);  // 784: layer-1 output pixels
+constexpr int kPx2 = (kS0 - 4) * (kS0 - 4);  // 676
+// LDS: activations A (layer 1, 3, 5 outputs) and B (input, layer 2, 4
+// outputs), 64 bytes per pixel (32 BF16 channels in four 16-byte chunks,
+// chunk c of pixel p stored in slot c ^ ((p >> 2) & 3) so that the 16 lanes
+// of a 16-byte read, at consecutive pixels, hit distinct banks), then the
+// per-tile arrays kept to the end.
+constexpr int kBytesA = kPx1 * 64;
+constexpr int kBytesB = kPx2 * 64;
+constexpr int kLdsBytes = kBytesA + kBytesB + 4 * kSN * kSN * 4;
+static_assert(kS0 * kS0 * 16 <= kBytesB, "packed input must fit in B");
+static_assert((kSV * kSV * 2 + kSV * kS0 + kS0 * kS0) * 4 <= kBytesA,
+              "local-scale stage must fit in A");
+static_assert(3 * 256 * 8 <= kBytesB, "reductions must fit in B");
+}  // namespace fused
+
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+typedef float f32x16v __attribute__((ext_vector_type(16)));
+
+__device__ inline int act_slot(int p, int c) { return (p << 2) + (c ^ ((p >> 2) & 3)); }
+
+// One 32 -> 32 (or 8 -> 32 for the first layer) convolution over a square
+// region: input side si (chunks per pixel kCin / 8), output side si - 2,
+// ReLU(D + bias), BF16, zero at pixels outside the grid.  Output pixel q of
+// the region lies at grid (gx0 + q % so, gy0 + q / so).
+template <int kCin>
+__device__ inline void fused_conv(const bf16x8* __restrict__ in, bf16x8* __restrict__ out,
+                                  int si, const bf16* __restrict__ wt,
+                                  const float* __restrict__ bias, int gx0, int gy0, int hc,
+                                  int wc, int nwaves) {
+  constexpr int NS = kCin == fused::kCh ? 18 : 5;
+  const int so = si - 2, npx = so * so;
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const int hf = lane >> 5, col = lane & 31;
+  bf16x8 A[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) {
+    const int tap = kCin == fused::kCh ? t >> 1 : 2 * t + hf;
+    const int ci = kCin == fused::kCh ? ((t & 1) << 4) + 8 * hf : 0;
+    A[t] = tap < 9 ? *reinterpret_cast<const bf16x8*>(wt + ((size_t)col * 9 + tap) * kCin + ci)
+                   : bf16x8{};
+  }
+  float bco[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) bco[v] = bias[8 * (v >> 2) + 4 * hf + (v & 3)];
+  const int ntiles = (npx + 31) / 32;
+  for (int tile = wv; tile < ntiles; tile += nwaves) {
+    const int q = min(tile * 32 + col, npx - 1);
+    const int r = q / so, c = q - r * so;
+    f32x16v acc = {};
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+      const int tap = kCin == fused::kCh ? t >> 1 : 2 * t + hf;
+      bf16x8 b = bf16x8{};
+      if (tap < 9) {
+        const int pin = (r + tap / 3) * si + (c + tap % 3);
+        b = kCin == fused::kCh ? in[act_slot(pin, ((t & 1) << 1) + hf)] : in[pin];
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[t], b, acc, 0, 0, 0);
+    }
+    if (tile * 32 + col >= npx) continue;
+    const int gx = gx0 + c, gy = gy0 + r;
+    const bool ing = gx >= 0 && gx < wc && gy >= 0 && gy < hc;
+    bf16x4v* o = reinterpret_cast<bf16x4v*>(out);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      bf16x4v v;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float vv = acc[4 * g + k] + bco[4 * g + k];
+        v[k] = (bf16)(ing && vv > 0.0f ? vv : 0.0f);
+      }
+      o[2 * act_slot(q, g) + hf] = v;
+    }
+  }
+}
+
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void net_level_kernel(
+    int hc, int wc, const float* __restrict__ rbar, const float* __restrict__ dis,
+    const float* __restrict__ feat, const float* __restrict__ kc,
+    const double* __restrict__ rms, int nrms, nn::PackedWeights wts,
+    float* __restrict__ y, double* __restrict__ part, int ntiles,
+    double* __restrict__ xsum) {
+  using namespace fused;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  bf16x8* actA = reinterpret_cast<bf16x8*>(lds);
+  bf16x8* actB = reinterpret_cast<bf16x8*>(lds + kBytesA);
+  float* s_sig = reinterpret_cast<float*>(lds + kBytesA + kBytesB);  // 18 x 18
+  float* s_ds = s_sig + kSN * kSN;
+  float* s_n = s_ds + kSN * kSN;
+  float* s_rb = s_n + kSN * kSN;  // the tile's rbar (16 x 16 of the 18 x 18)
+  // The local-scale stage, in A (dead until layer 1 writes it).
+  float* v_rb = reinterpret_cast<float*>(lds);
+  float* v_ds = v_rb + kSV * kSV;
+  float* v_tmp = v_ds + kSV * kSV;   // rows kSV x columns kS0
+  float* v_sig = v_tmp + kSV * kS0;  // kS0 x kS0
+  double* red = reinterpret_cast<double*>(actB);  // at the end, B is free
+  constexpr int kWaves_ = kThreads / kWave;
+  const int tid = threadIdx.x;
+  const int X0 = blockIdx.x * kT, Y0 = blockIdx.y * kT;
+  const long n = (long)hc * wc;
+  auto in_grid = [&](int X, int Y) { return X >= 0 && X < wc && Y >= 0 && Y < hc; };
+
+  // The RMS of D^-1/2 rbar over the grid from the restriction's partials,
+  // in box_rows_kernel's order (thread t of 256 adds t, t + 256, ...).
+  __shared__ double s_scale[256];
+  __shared__ float s_scale0;
+  if (tid < 256) {
+    double v = 0.0;
+    for (int i = tid; i < nrms; i += 256) v += rms[i];
+    s_scale[tid] = v;
+  }
+  // Stage rbar and D^-1/2 on the tile and 13 cells around it.
+  const int vx0 = X0 - kRN - kWin, vy0 = Y0 - kRN - kWin;
+  for (int q = tid; q < kSV * kSV; q += kThreads) {
+    const int X = vx0 + q % kSV, Y = vy0 + q / kSV;
+    float rb = 0.0f, d = 0.0f;
+    if (in_grid(X, Y)) {
+      rb = rbar[(long)Y * wc + X];
+      d = dis[(long)Y * wc + X];
+    }
+    v_rb[q] = rb;
+    v_ds[q] = d;
+  }
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) s_scale[tid] += s_scale[tid + st];
+    __syncthreads();
+  }
+  if (tid == 0) s_scale0 = (float)sqrt(s_scale[0] / (double)n);
+  // Row pass of the window sums (box_rows_kernel): every staged row, the
+  // input region's columns.
+  for (int q = tid; q < kSV * kS0; q += kThreads) {
+    const int rr = q / kS0, cc = q - rr * kS0 + kWin;
+    const int X = vx0 + cc;
+    double s = 0.0;
+    for (int d = -kWin; d <= kWin; ++d) {
+      if (X + d < 0 || X + d >= wc) continue;
+      const double v = (double)v_rb[rr * kSV + cc + d] * (double)v_ds[rr * kSV + cc + d];
+      s += v * v;
+    }
+    v_tmp[q] = (float)s;
+  }
+  __syncthreads();
+  // Column pass, sigma and the packed input (box_cols_kernel), zero outside
+  // the grid.
+  const float floor_s = kFloor * s_scale0;
+  for (int q = tid; q < kS0 * kS0; q += kThreads) {
+    const int r = q / kS0, c = q - r * kS0;
+    const int X = X0 - kRN + c, Y = Y0 - kRN + r;
+    bf16x8 o = bf16x8{};
+    float sg = 0.0f;
+    if (in_grid(X, Y)) {
+      double s = 0.0;
+      for (int d = -kWin; d <= kWin; ++d) {
+        if (Y + d < 0 || Y + d >= hc) continue;
+        s += (double)v_tmp[(r + kWin + d) * kS0 + c];
+      }
+      sg = fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))), floor_s);
+      const int vi = (r + kWin) * kSV + c + kWin;
+      o[0] = sg > 0.0f ? (bf16)((v_rb[vi] * v_ds[vi]) / sg) : (bf16)0.0f;
+      o[1] = (bf16)feat[(long)Y * wc + X];
+      o[2] = (bf16)feat[n + (long)Y * wc + X];
+    }
+    v_sig[q] = sg;
+    actB[q] = o;
+  }
+  __syncthreads();
+  // Keep what the end needs: sigma and D^-1/2 on the output region, rbar on
+  // the tile.
+  for (int q = tid; q < kSN * kSN; q += kThreads) {
+    const int r = q / kSN, c = q - r * kSN;
+    s_sig[q] = v_sig[(r + kRN - 1) * kS0 + c + kRN - 1];
+    const int vi = (r + kRN - 1 + kWin) * kSV + c + kRN - 1 + kWin;
+    s_ds[q] = v_ds[vi];
+    s_rb[q] = v_rb[vi];
+  }
+  __syncthreads();
+  // Layers 1..5: input (B, side 30) -> A (28) -> B (26) -> A (24) -> B (22)
+  // -> A (20).
+  fused_conv<8>(actB, actA, kS0, wts.wf1, wts.bias, X0 - kRN + 1, Y0 - kRN + 1, hc, wc,
+                kWaves_);
+  __syncthreads();
+#pragma unroll 1
+  for (int l = 1; l < 5; ++l) {
+    const int si = kS0 - 2 * l;
+    const int off = kRN - l - 1;  // output region origin relative to the tile
+    const bf16x8* src = (l & 1) ? actA : actB;
+    bf16x8* dst = (l & 1) ? actB : actA;
+    fused_conv<kCh>(src, dst, si, wts.wfh + (size_t)(l - 1) * kCh * 9 * kCh,
+                    wts.bias + l * kCh, X0 - off, Y0 - off, hc, wc, kWaves_);
+    __syncthreads();
+  }
+  // Output layer on the 18 x 18 region from A (side 20), FP32 in the order
+  // of out_layer_kernel; then y = N sigma D^-1/2 (zero outside the grid).
+  const float b6 = wts.bias[5 * kCh];
+  for (int q = tid; q < kSN * kSN; q += kThreads) {
+    const int r = q / kSN, c = q - r * kSN;
+    float acc = 0.0f;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int pin = (r + tap / 3) * (kSN + 2) + (c + tap % 3);
+#pragma unroll
+      for (int c8 = 0; c8 < kCh; c8 += 8) {
+        const bf16x8 v = actA[act_slot(pin, c8 >> 3)];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += wts.w6[tap * kCh + c8 + k] * (float)v[k];
+      }
+    }
+    const int X = X0 - 1 + c, Y = Y0 - 1 + r;
+    s_n[q] = in_grid(X, Y) ? (acc + b6) * (s_sig[q] * s_ds[q]) : 0.0f;
+  }
+  __syncthreads();
+  // The tile: y, T y of y (SMVP order, as sweeps_kernel), the partials and
+  // the level-1 block sums.
+  double* s_a = red;
+  double* s_b = red + 256;
+  double* s_t = red + 512;
+  if (tid < 256) {
+    const int tx = tid & (kT - 1), ty = tid / kT;
+    const int X = X0 + tx, Y = Y0 + ty;
+    const int k = (ty + 1) * kSN + tx + 1;
+    const long i = (long)Y * wc + X;
+    const bool in = X < wc && Y < hc;
+    double kl = 0.0, kr = 0.0, kb = 0.0, kt = 0.0;
+    if (in) {
+      kl = kc[i];
+      kb = kc[n + i];
+      kr = X + 1 < wc ? (double)kc[i + 1] : 0.0;
+      kt = Y + 1 < hc ? (double)kc[n + i + wc] : 0.0;
+    }
+    const double yc = (double)s_n[k];
+    const double tyv = (((1.0 + (kr + kl)) + (kt + kb)) * yc -
+                        (kr * (double)s_n[k + 1] + kl * (double)s_n[k - 1])) -
+                       (kt * (double)s_n[k + kSN] + kb * (double)s_n[k - kSN]);
+    double a = 0.0, b = 0.0;
+    if (in) {
+      y[i] = s_n[k];
+      a = yc * (double)s_rb[k];
+      b = yc * tyv;
+    }
+    s_a[tid] = a;
+    s_b[tid] = b;
+    s_t[tid] = tyv;
+  }
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      s_a[tid] += s_a[tid + st];
+      s_b[tid] += s_b[tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+    part[tile] = s_a[0];
+    part[ntiles + tile] = s_b[0];
+  }
+  if (xsum && tid < 16) {
+    const int bx = tid & 3, by = tid >> 2;
+    const int X1 = blockIdx.x * 4 + bx, Y1 = blockIdx.y * 4 + by;
+    if (X1 < (wc >> 2) && Y1 < (hc >> 2)) {
+      double rs[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double* p = &s_t[(4 * by + j) * kT + 4 * bx];
+        rs[j] = (p[0] + p[1]) + (p[2] + p[3]);
+      }
+      xsum[(long)Y1 * (wc >> 2) + X1] = (rs[0] + rs[1]) + (rs[2] + rs[3]);
+    }
+  }
+}
+
+// The end of net_level_kernel for the per-layer pipeline (box_rows_kernel,
+// box_cols_kernel and the network's own kernels): from the network output
+// N on the grid, y = N sigma D^-1/2 on the tile and its ring, then the same
+// y, partials and level-1 block sums as net_level_kernel.  One 256-thread
+// block per 16 x 16 tile.
+__global__ __launch_bounds__(256) void net_tile_kernel(
+    int hc, int wc, const float* __restrict__ net, const float* __restrict__ sig,
+    const float* __restrict__ dis, const float* __restrict__ rbar,
+    const float* __restrict__ kc, float* __restrict__ y, double* __restrict__ part,
+    int ntiles, double* __restrict__ xsum) {
+  using namespace fused;
+  __shared__ float s_n[kSN * kSN];
+  __shared__ double s_a[256], s_b[256], s_t[256];
+  const int tid = threadIdx.x;
+  const int X0 = blockIdx.x * kT, Y0 = blockIdx.y * kT;
+  const long n = (long)hc * wc;
+  for (int q = tid; q < kSN * kSN; q += 256) {
+    const int X = X0 - 1 + q % kSN, Y = Y0 - 1 + q / kSN;
+    float v = 0.0f;
+    if (X >= 0 && X < wc && Y >= 0 && Y < hc) {
+      const long j = (long)Y * wc + X;
+      v = net[j] * (sig[j] * dis[j]);
+    }
+    s_n[q] = v;
+  }
+  __syncthreads();
+  const int tx = tid & (kT - 1), ty = tid / kT;
+  const int X = X0 + tx, Y = Y0 + ty;
+  const int k = (ty + 1) * kSN + tx + 1;
+  const long i = (long)Y * wc + X;
+  const bool in = X < wc && Y < hc;
+  double kl = 0.0, kr = 0.0, kb = 0.0, kt = 0.0;
+  if (in) {
+    kl = kc[i];
+    kb = kc[n + i];
+    kr = X + 1 < wc ? (double)kc[i + 1] : 0.0;
+    kt = Y + 1 < hc ? (double)kc[n + i + wc] : 0.0;
+  }
+  const double yc = (double)s_n[k];
+  const double tyv = (((1.0 + (kr + kl)) + (kt + kb)) * yc -
+                      (kr * (double)s_n[k + 1] + kl * (double)s_n[k - 1])) -
+                     (kt * (double)s_n[k + kSN] + kb * (double)s_n[k - kSN]);
+  double a = 0.0, b = 0.0;
+  if (in) {
+    y[i] = s_n[k];
+    a = yc * (double)rbar[i];
+    b = yc * tyv;
+  }
+  s_a[tid] = a;
+  s_b[tid] = b;
+  s_t[tid] = tyv;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      s_a[tid] += s_a[tid + st];
+      s_b[tid] += s_b[tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+    part[tile] = s_a[0];
+    part[ntiles + tile] = s_b[0];
+  }
+  if (xsum && tid < 16) {
+    const int bx = tid & 3, by = tid >> 2;
+    const int X1 = blockIdx.x * 4 + bx, Y1 = blockIdx.y * 4 + by;
+    if (X1 < (wc >> 2) && Y1 < (hc >> 2)) {
+      double rs[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double* p = &s_t[(4 * by + j) * kT + 4 * bx];
+        rs[j] = (p[0] + p[1]) + (p[2] + p[3]);
+      }
+      xsum[(long)Y1 * (wc >> 2) + X1] = (rs[0] + rs[1]) + (rs[2] + rs[3]);
+    }
+  }
+}
+
 // The factors g of the coarse directions, into g[0] and g[1].  One level:
 // g = (y.rbar) / (y.T y), 0 unless y.T y > 0 and the ratio is finite.  Two
 // levels: the joint minimiser of the energy norm of e - g0 P0 y0 - g1 P1 y1
@@ -768,16 +912,28 @@ __global__ __launch_bounds__(256) void joint_gamma_kernel(
     double c0sq, double c1sq, float* __restrict__ g) {
   __shared__ double sh[5][256];
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int k = threadIdx.x; k < n0; k += 256) {
-    v[0] += p0[k];
-    v[1] += p0[n0 + k];
-  }
-  if (levels == 2) {
-    for (int k = threadIdx.x; k < n1; k += 256) {
-      v[2] += p1[k];
-      v[3] += p1[n1 + k];
+  // Each thread's elements are loaded 16 at a time, then added in turn (the
+  // order above), so that the loads overlap.
+  constexpr int kB = 16;
+  auto strided = [&](const double* a, const float* f, int n, double& acc) {
+    for (int k0 = threadIdx.x; k0 < n; k0 += 256 * kB) {
+      double t[kB];
+#pragma unroll
+      for (int j = 0; j < kB; ++j) {
+        const int k = k0 + 256 * j;
+        t[j] = k < n ? (f ? a[k] * (double)f[k] : a[k]) : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < kB; ++j)
+        if (k0 + 256 * j < n) acc += t[j];
     }
-    for (int k = threadIdx.x; k < nx; k += 256) v[4] += xsum[k] * (double)y1[k];
+  };
+  strided(p0, nullptr, n0, v[0]);
+  strided(p0 + n0, nullptr, n0, v[1]);
+  if (levels == 2) {
+    strided(p1, nullptr, n1, v[2]);
+    strided(p1 + n1, nullptr, n1, v[3]);
+    strided(xsum, y1, nx, v[4]);
   }
   for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
   __syncthreads();
@@ -826,99 +982,93 @@ int ensure_fcg_buffers(tea_hip_chunk* c) {
 
 using namespace tea;
 
-// Coarse pipeline of one level: the coarse residual (from r for level 0,
-// from the level above otherwise), its local scale, the network, and the
-// factor g of the coarse direction.
-// The restriction of level l (block means of r, or of the finer level's
-// means), on stream st.
-static int run_restrict(tea_hip_chunk* c, int l, hipStream_t st) {
-  FcgLevel& L = c->fcg_lv[l];
-  const int nblk = (L.hc * L.wc + 255) / 256;
-  Timed t(c, K_FCG_RESTRICT, st);
-  if (l == 0) {
-    hipLaunchKernelGGL(restrict_kernel, dim3(nblk), dim3(256), 0, st, c->geo, L.c, L.hc,
-                       L.wc, c->mp_r, L.dis, L.rbar, L.part);
-  } else {
-    const FcgLevel& F = c->fcg_lv[l - 1];
-    hipLaunchKernelGGL(restrict_coarse_kernel, dim3(nblk), dim3(256), 0, st, F.wc,
-                       L.c / F.c, L.hc, L.wc, F.rbar, L.dis, L.rbar, L.part);
-  }
-  TEA_HIP_TRY(hipGetLastError());
-  return TEA_HIP_OK;
+// Threads of net_level_kernel's blocks.
+constexpr int kNetThreads = 512;
+
+// Whether the learned coarse solve runs as one fused launch per level (the
+// default) or as the per-layer pipeline (TEA_FCG_NET=layers: box_rows,
+// box_cols, the network's own kernels, net_tile_kernel), which gives the
+// same y bit for bit.
+static bool net_per_layer() {
+  static const bool v = [] {
+    const char* e = getenv("TEA_FCG_NET");
+    return e != nullptr && strcmp(e, "layers") == 0;
+  }();
+  return v;
 }
 
-// The rest of level l after its restriction, on stream st: the scales and
-// the network input, the network, the unscaled output (into L.tmp) with its
-// dots, and -- unless the joint factors of two levels follow -- the level's
-// own factor.
-static int run_level(tea_hip_chunk* c, int l, hipStream_t st, bool own_factor) {
+// The learned coarse solve of level l on stream st: y, the tile partials
+// and (level 0 of two) the level-1 block sums of T y.
+static int launch_net_level(tea_hip_chunk* c, int l, hipStream_t st) {
   FcgLevel& L = c->fcg_lv[l];
-  const int cn = L.hc * L.wc;
-  const int nblk = (cn + 255) / 256;
-  {
-    Timed t(c, K_FCG_RESTRICT, st);
+  const dim3 tiles((unsigned)(L.wc + fused::kT - 1) / fused::kT,
+                   (unsigned)(L.hc + fused::kT - 1) / fused::kT);
+  double* xs = (l == 0 && c->fcg_levels == 2) ? c->fcg_xsum : nullptr;
+  Timed t(c, K_FCG_NET, st);
+  if (!net_per_layer()) {
+    hipLaunchKernelGGL((net_level_kernel<kNetThreads>), tiles, dim3(kNetThreads),
+                       fused::kLdsBytes, st, L.hc, L.wc, L.rbar, L.dis, L.feat, L.kc, L.rms,
+                       L.nrms, nn::packed_weights(L.net), L.y, L.tpart, L.ntiles, xs);
+  } else {
+    const int nblk = (L.hc * L.wc + 255) / 256;
     hipLaunchKernelGGL(box_rows_kernel, dim3(nblk + 1), dim3(256), 0, st, L.hc, L.wc,
-                       L.rbar, L.dis, L.tmp, L.part, nblk, L.scale);
+                       L.rbar, L.dis, L.tmp, L.rms, L.nrms, L.scale);
     hipLaunchKernelGGL(box_cols_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc, L.tmp,
                        L.scale, L.sig, L.rbar, L.dis, L.feat, nn::input_tensor(L.net));
-  }
-  {
-    Timed t(c, K_FCG_NET, st);
     TEA_RETURN_IF(nn::forward_on_stream(L.net, st));
-    hipLaunchKernelGGL(coarse_dots_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc,
-                       nn::output(L.net), L.dis, L.sig, L.tmp, L.rbar, L.kc, L.part);
-    if (own_factor)
-      hipLaunchKernelGGL(gamma_kernel, dim3(1), dim3(256), 0, st, L.part, nblk, L.scale);
+    hipLaunchKernelGGL(net_tile_kernel, tiles, dim3(256), 0, st, L.hc, L.wc,
+                       nn::output(L.net), L.sig, L.dis, L.rbar, L.kc, L.y, L.tpart,
+                       L.ntiles, xs);
   }
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
 }
 
-static CoarseTerm coarse_term(const tea_hip_chunk* c, int l) {
-  const FcgLevel& L = c->fcg_lv[l];
-  int shift = 0;
-  while ((1 << shift) < L.c) ++shift;
-  return CoarseTerm{L.tmp, L.scale + 1, shift, L.wc};
+// The non-learned coarse solve of level l on stream st (same outputs).
+static int launch_sweeps_level(tea_hip_chunk* c, int l, hipStream_t st) {
+  FcgLevel& L = c->fcg_lv[l];
+  Timed t(c, K_FCG_SWEEPS, st);
+  const dim3 grid((unsigned)(L.wc + kSweepTile - 1) / kSweepTile,
+                  (unsigned)(L.hc + kSweepTile - 1) / kSweepTile);
+  hipLaunchKernelGGL(sweeps_kernel, grid, dim3(256), 0, st, L.hc, L.wc, c->fcg_sweeps,
+                     L.kc, L.rbar, L.y, L.tpart, L.ntiles,
+                     (l == 0 && c->fcg_levels == 2) ? c->fcg_xsum : nullptr);
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
 }
 
-// The coarse part of the two-level preconditioner with Jacobi sweeps as
-// the coarse solve: the restriction of both levels in one pass, the sweeps
-// of level 1 on the side stream beside level 0's, and the factors into
-// c->fcg_g.
-static int launch_sweeps_coarse(tea_hip_chunk* c) {
+// The coarse part of the two-level preconditioner: the restriction of both
+// levels in one pass (with the RMS partials the network's input scale
+// needs), each level's coarse solve (level 1 on the side stream, beside
+// level 0 on the chunk's), and the factors into c->fcg_g.
+static int launch_coarse(tea_hip_chunk* c) {
   const int nl = c->fcg_levels;
+  const bool net = c->fcg_sweeps == 0;
   FcgLevel& L0 = c->fcg_lv[0];
   FcgLevel& L1 = c->fcg_lv[1];
   {
     Timed t(c, K_FCG_RESTRICT);
     const dim3 grid((unsigned)(L0.wc + 31) / 32, (unsigned)(L0.hc + 3) / 4);
     hipLaunchKernelGGL(restrict8_kernel, grid, dim3(256), 0, c->stream, c->geo, L0.hc, L0.wc,
-                       c->mp_r, L0.rbar, nl == 2 ? L1.rbar : nullptr, nullptr, nullptr,
-                       nullptr, nullptr);
+                       c->mp_r, L0.rbar, nl == 2 ? L1.rbar : nullptr,
+                       net ? L0.dis : nullptr, net && nl == 2 ? L1.dis : nullptr,
+                       net ? L0.rms : nullptr, net && nl == 2 ? L1.rms : nullptr);
     TEA_HIP_TRY(hipGetLastError());
   }
-  auto sweeps = [&](int l, hipStream_t st) -> int {
-    FcgLevel& L = c->fcg_lv[l];
-    Timed t(c, K_FCG_SWEEPS, st);
-    const dim3 grid((unsigned)(L.wc + kSweepTile - 1) / kSweepTile,
-                    (unsigned)(L.hc + kSweepTile - 1) / kSweepTile);
-    hipLaunchKernelGGL(sweeps_kernel, grid, dim3(256), 0, st, L.hc, L.wc, c->fcg_sweeps,
-                       L.kc, L.rbar, L.y, L.tpart, L.ntiles,
-                       (l == 0 && nl == 2) ? c->fcg_xsum : nullptr);
-    TEA_HIP_TRY(hipGetLastError());
-    return TEA_HIP_OK;
+  auto level = [&](int l, hipStream_t st) {
+    return net ? launch_net_level(c, l, st) : launch_sweeps_level(c, l, st);
   };
   if (nl == 2) {
     TEA_HIP_TRY(hipEventRecord(c->fcg_fork, c->stream));
     TEA_HIP_TRY(hipStreamWaitEvent(c->fcg_side, c->fcg_fork, 0));
-    TEA_RETURN_IF(sweeps(1, c->fcg_side));
-    TEA_RETURN_IF(sweeps(0, c->stream));
+    TEA_RETURN_IF(level(1, c->fcg_side));
+    TEA_RETURN_IF(level(0, c->stream));
     TEA_HIP_TRY(hipEventRecord(c->fcg_join, c->fcg_side));
     TEA_HIP_TRY(hipStreamWaitEvent(c->stream, c->fcg_join, 0));
   } else {
-    TEA_RETURN_IF(sweeps(0, c->stream));
+    TEA_RETURN_IF(level(0, c->stream));
   }
-  Timed t(c, K_FCG_SWEEPS);
+  Timed t(c, net ? K_FCG_NET : K_FCG_SWEEPS);
   hipLaunchKernelGGL(joint_gamma_kernel, dim3(1), dim3(256), 0, c->stream, nl, L0.tpart,
                      L0.ntiles, L1.tpart, L1.ntiles, c->fcg_xsum, L1.y,
                      nl == 2 ? L1.hc * L1.wc : 0, (double)L0.c * L0.c,
@@ -941,42 +1091,11 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
   const int nl = use_net ? c->fcg_levels : 0;
   double* partials = c->partials + (size_t)slot * c->nspan;
   *len = c->nspan;
-  const bool sweep = c->fcg_sweeps > 0;
-  if (nl > 0 && sweep) {
-    TEA_RETURN_IF(launch_sweeps_coarse(c));
-  } else if (nl == 1) {
-    TEA_RETURN_IF(run_restrict(c, 0, c->stream));
-    TEA_RETURN_IF(run_level(c, 0, c->stream, true));
-  } else if (nl == 2) {
-    // Level 2 (from level 1's block means) on the side stream while level 1
-    // runs on the chunk's stream; both are small grids, so their launch
-    // latencies overlap.
-    TEA_RETURN_IF(run_restrict(c, 0, c->stream));
-    TEA_HIP_TRY(hipEventRecord(c->fcg_fork, c->stream));
-    TEA_HIP_TRY(hipStreamWaitEvent(c->fcg_side, c->fcg_fork, 0));
-    TEA_RETURN_IF(run_restrict(c, 1, c->fcg_side));
-    TEA_RETURN_IF(run_level(c, 1, c->fcg_side, false));
-    TEA_RETURN_IF(run_level(c, 0, c->stream, false));
-    TEA_HIP_TRY(hipEventRecord(c->fcg_join, c->fcg_side));
-    TEA_HIP_TRY(hipStreamWaitEvent(c->stream, c->fcg_join, 0));
-  }
-  if (nl == 2 && !sweep) {
-    FcgLevel& L1 = c->fcg_lv[0];
-    FcgLevel& L2 = c->fcg_lv[1];
-    const int n1 = (L1.hc * L1.wc + 255) / 256, n2 = (L2.hc * L2.wc + 255) / 256;
-    Timed t(c, K_FCG_NET);
-    hipLaunchKernelGGL(cross_dots_kernel, dim3(n1), dim3(256), 0, c->stream, L1.hc, L1.wc,
-                       L2.c / L1.c, L2.wc, L1.tmp, L2.tmp, L1.kc,
-                       c->fcg_cross);
-    hipLaunchKernelGGL(gamma2_kernel, dim3(1), dim3(256), 0, c->stream, L1.part, n1,
-                       L2.part, n2, c->fcg_cross, n1, (double)L1.c * L1.c,
-                       (double)L2.c * L2.c, L1.scale, L2.scale);
-    TEA_HIP_TRY(hipGetLastError());
-  }
+  if (nl > 0) TEA_RETURN_IF(launch_coarse(c));
   const CoarseTerm none{nullptr, nullptr, 0, 0};
   auto term = [&](int l) {
     const FcgLevel& L = c->fcg_lv[l];
-    return sweep ? CoarseTerm{L.y, c->fcg_g + l, log2_of(L.c), L.wc} : coarse_term(c, l);
+    return CoarseTerm{L.y, c->fcg_g + l, log2_of(L.c), L.wc};
   };
   const CoarseTerm t0 = nl > 0 ? term(0) : none;
   const CoarseTerm t1 = nl > 1 ? term(1) : none;
@@ -1034,20 +1153,29 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
 static void free_level(FcgLevel& L) {
   for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale, L.y})
     if (b) (void)hipFree(b);
-  if (L.part) (void)hipFree(L.part);
+  if (L.rms) (void)hipFree(L.rms);
   if (L.tpart) (void)hipFree(L.tpart);
   if (L.net) tea_nn_destroy(L.net);
   L = FcgLevel();
 }
 
+// net_level_kernel's dynamic LDS exceeds the default limit of a launch:
+// raise it once per process.
+static int ensure_net_lds(int device) {
+  static bool done[64] = {};
+  if (device >= 0 && device < 64 && done[device]) return TEA_HIP_OK;
+  TEA_HIP_TRY(hipFuncSetAttribute(
+      reinterpret_cast<const void*>(&net_level_kernel<kNetThreads>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, fused::kLdsBytes));
+  if (device >= 0 && device < 64) done[device] = true;
+  return TEA_HIP_OK;
+}
+
 // Frees every coarse level and the buffers shared between them.
 static void release_levels(tea_hip_chunk* c) {
   for (FcgLevel& L : c->fcg_lv) free_level(L);
-  for (double** b : {&c->fcg_cross, &c->fcg_xsum})
-    if (*b) {
-      (void)hipFree(*b);
-      *b = nullptr;
-    }
+  if (c->fcg_xsum) (void)hipFree(c->fcg_xsum);
+  c->fcg_xsum = nullptr;
   if (c->fcg_g) (void)hipFree(c->fcg_g);
   c->fcg_g = nullptr;
   c->fcg_levels = 0;
@@ -1129,6 +1257,11 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
   TEA_CHECK_ARG((nx / coarsen) % 32 == 0, "coarse width must be a multiple of 32");
   // Level l coarsens by coarsen * 4^l; a level is added while the grid
   // divides and its width stays a multiple of 32.
+  auto fail = [] {
+    set_error("hipMalloc of the preconditioner's coarse buffers failed");
+    return TEA_HIP_ENOMEM;
+  };
+  TEA_CHECK_ARG(coarsen == 8, "the network's restriction needs a coarsening factor of 8");
   for (int l = 0; l < levels; ++l) {
     const int cl = coarsen << (2 * l);
     if (nx % cl != 0 || ny % cl != 0 || (nx / cl) % 32 != 0) break;
@@ -1137,31 +1270,34 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
     L.hc = ny / cl;
     L.wc = nx / cl;
     const size_t cn = (size_t)L.hc * L.wc;
-    const size_t nblk = (cn + 255) / 256;
     TEA_RETURN_IF(tea_nn_create(c->device, 1, L.hc, L.wc, &L.net));
     TEA_RETURN_IF(tea_nn_set_params(L.net, params, count));
+    // One RMS partial per block of the restriction (restrict8_kernel's grid
+    // over level 0), for either level.
+    const FcgLevel& L0 = c->fcg_lv[0];
+    L.nrms = ((L0.wc + 31) / 32) * ((L0.hc + 3) / 4);
+    L.ntiles = ((L.wc + fused::kT - 1) / fused::kT) * ((L.hc + fused::kT - 1) / fused::kT);
     if (hipMalloc(&L.feat, 2 * cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.kc, 2 * cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.dis, cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.sig, cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.tmp, cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.rbar, cn * sizeof(float)) != hipSuccess ||
-        hipMalloc(&L.part, 2 * nblk * sizeof(double)) != hipSuccess ||
-        hipMalloc(&L.scale, 2 * sizeof(float)) != hipSuccess) {
-      set_error("hipMalloc of the preconditioner's coarse buffers failed");
-      return TEA_HIP_ENOMEM;
-    }
+        hipMalloc(&L.y, cn * sizeof(float)) != hipSuccess ||
+        hipMalloc(&L.rms, (size_t)L.nrms * sizeof(double)) != hipSuccess ||
+        hipMalloc(&L.tpart, 2 * (size_t)L.ntiles * sizeof(double)) != hipSuccess ||
+        hipMalloc(&L.scale, 2 * sizeof(float)) != hipSuccess)
+      return fail();
     c->fcg_levels = l + 1;
   }
+  if (hipMalloc(&c->fcg_g, 2 * sizeof(float)) != hipSuccess) return fail();
+  TEA_HIP_TRY(hipMemsetAsync(c->fcg_g, 0, 2 * sizeof(float), c->stream));
   if (c->fcg_levels == 2) {
-    const size_t n1 = ((size_t)c->fcg_lv[0].hc * c->fcg_lv[0].wc + 255) / 256;
-    if (hipMalloc(&c->fcg_cross, n1 * sizeof(double)) != hipSuccess) {
-      c->fcg_cross = nullptr;
-      set_error("hipMalloc of the preconditioner's cross partials failed");
-      return TEA_HIP_ENOMEM;
-    }
+    const size_t n1 = (size_t)c->fcg_lv[1].hc * c->fcg_lv[1].wc;
+    if (hipMalloc(&c->fcg_xsum, n1 * sizeof(double)) != hipSuccess) return fail();
     TEA_RETURN_IF(ensure_side_stream(c));
   }
+  TEA_RETURN_IF(ensure_net_lds(c->device));
   return TEA_HIP_OK;
 }
 

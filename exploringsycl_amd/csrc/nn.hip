@@ -834,6 +834,9 @@ namespace nn {
 // Used by the flexible-CG preconditioner (fcg.hip).
 int forward_on_stream(tea_nn* n, hipStream_t st) { return forward_on(n, 1, st); }
 bf16* input_tensor(tea_nn* n) { return n->in0; }
+PackedWeights packed_weights(const tea_nn* n) {
+  return PackedWeights{n->wf1, n->wfh, n->w6, n->bias};
+}
 const float* output(tea_nn* n) { return n->out; }
 int sync(tea_nn* n) {
   TEA_HIP_TRY(hipStreamSynchronize(n->stream));

@@ -511,6 +511,17 @@ struct tea_nn;
 namespace tea {
 namespace nn {
 typedef __bf16 bf16;
+// The network's device copies of its parameters, in the layouts of its
+// forward kernels: layer 1 wf1[co][tap][8] (3 channels used) and the hidden
+// layers wfh[layer][co][tap][ci] in BF16, the output layer w6[tap][ci]
+// (BF16 values in FP32) and the biases bias[layer * 32 + co], bias[160].
+struct PackedWeights {
+  const bf16* wf1;
+  const bf16* wfh;
+  const float* w6;
+  const float* bias;
+};
+PackedWeights packed_weights(const tea_nn* n);
 int forward_on_stream(tea_nn* n, hipStream_t st);
 bf16* input_tensor(tea_nn* n);
 const float* output(tea_nn* n);
@@ -530,10 +541,11 @@ struct FcgLevel {
   float* dis = nullptr;       // [hc][wc] D^-1/2 of the coarse operator
   float* sig = nullptr;       // [hc][wc] local input scale of the network
   float* tmp = nullptr;       // [hc][wc] row pass of the local scale
-  double* part = nullptr;     // per-block partial sums (2 * blocks)
-  float* scale = nullptr;     // {RMS of D^-1/2 rbar, prolongation factor}
-  // The coarse solve by Jacobi sweeps: its output y and the per-16x16-tile
-  // partials of y.rbar and y.T y (2 * ntiles).
+  float* scale = nullptr;     // {RMS of D^-1/2 rbar} (per-layer pipeline)
+  double* rms = nullptr;      // per-restriction-block sums of (D^-1/2 rbar)^2
+  int nrms = 0;
+  // The coarse solve's output y (network or Jacobi sweeps) and the
+  // per-16x16-tile partials of y.rbar and y.T y (2 * ntiles).
   float* y = nullptr;
   double* tpart = nullptr;
   int ntiles = 0;
@@ -610,10 +622,9 @@ struct tea_hip_chunk {
   float fcg_norm[4] = {0.0f, 1.0f, 0.0f, 1.0f};
   int fcg_levels = 0;
   FcgLevel fcg_lv[kFcgMaxLevels];
-  double* fcg_cross = nullptr;  // partials of the two levels' cross term
   // Coarse solve by fcg_sweeps Jacobi sweeps instead of the network (0:
-  // network); its level-0 sums of T y over the level-1 blocks, and the
-  // factors g of the levels.
+  // network); the level-0 sums of T y over the level-1 blocks (the cross
+  // term of two levels), and the factors g of the levels.
   int fcg_sweeps = 0;
   double* fcg_xsum = nullptr;
   float* fcg_g = nullptr;

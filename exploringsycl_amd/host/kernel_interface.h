@@ -3,8 +3,12 @@
  * meaning are the reference's, so a backend that implements them links
  * against either this host or the reference's own host code.
  *
- * Scalar out-parameters (rro, pw, rrn, norm, vol, mass, ie, temp) are
- * accumulated into (+=): the drivers zero them and sum over chunks. */
+ * Scalar out-parameters follow the reference SYCL backend: rro and pw are
+ * added to the caller's variable (cg.cpp:133,194); rrn, the 2-norm, the
+ * Jacobi error and the field summary (vol, mass, ie, temp) are assigned
+ * (cg.cpp:253, solver_methods.cpp:116, jacobi.cpp:116,
+ * field_summary.cpp:147-150).  The drivers zero them first, and each process
+ * drives one chunk, so the two agree. */
 #pragma once
 #ifndef TEA_KERNEL_INTERFACE_H
 #define TEA_KERNEL_INTERFACE_H

@@ -21,8 +21,13 @@
  *    They are computed with the reference's reduction tree
  *    (TeaLeaf/c_kernels/sycl/sycl_shared.hpp:35-79) and are bitwise equal to
  *    the FP64 CPU restatement of the reference kernels.
- *  - Calls on one handle are serialised on one HIP stream; calls that return
- *    a scalar synchronise with it.  A handle must not be used from two host
+ *  - Calls on one handle are serialised on one HIP stream.  A call that
+ *    returns a scalar returns once the values it needs have arrived: the
+ *    final reduction kernel writes them into mapped host memory, which the
+ *    call polls, so it may return while that kernel is still retiring.
+ *    Raw device access through tea_hip_field therefore needs tea_hip_flush
+ *    first: work may still be queued or held back (see tea_hip_set_fused
+ *    and tea_hip_local_halos).  A handle must not be used from two host
  *    threads at once.
  */
 #ifndef TEA_HIP_H
@@ -91,7 +96,10 @@ int tea_hip_chunk_destroy(tea_hip_chunk* chunk);
  * TEA_FIELD_P, TEA_FIELD_U, TEA_FIELD_R and TEA_FIELD_SD can change after
  * solver calls (the fused CG, Jacobi, Chebyshev and PPCG iterations write
  * the new p, u or sd into a second buffer and exchange the two); ask again
- * after each call instead of caching them. */
+ * after each call instead of caching them.  The address is returned as
+ * is: deferred updates (see tea_hip_set_fused and tea_hip_local_halos) may
+ * not have reached the field yet; call tea_hip_flush before reading or
+ * writing through it. */
 int tea_hip_field(tea_hip_chunk* chunk, int field, double** dev_ptr,
                   size_t* count);
 
@@ -102,7 +110,8 @@ int tea_hip_field_write(tea_hip_chunk* chunk, int field, const double* host,
 int tea_hip_field_read(tea_hip_chunk* chunk, int field, double* host,
                        size_t count);
 
-/* Waits for all work queued on the handle's stream. */
+/* Waits for all work queued on the handle's stream (deferred work stays
+ * deferred: see tea_hip_flush). */
 int tea_hip_synchronize(tea_hip_chunk* chunk);
 
 /* Text of the last error on the calling thread ("" if none). */
@@ -138,7 +147,14 @@ int tea_hip_store_energy(tea_hip_chunk* chunk);
  * neighbour is negative; one launch covers all fields and faces and gives the
  * same result as the reference's left, right, top, bottom order.  Replaces
  * run_local_halos (TeaLeaf/kernel_interface.h:23;
- * kernel_interface.cpp:100-127, local_halos.cpp:7-102). */
+ * kernel_interface.cpp:100-127, local_halos.cpp:7-102).
+ * In fused mode some updates are held back and folded into the pass that
+ * next reads the field: the depth-1 U/P update after tea_hip_cg_calc_p
+ * (into the next tea_hip_cg_calc_w), the SD update after a held PPCG inner
+ * step or the U update after a held Chebyshev step, and a depth-1 SD or U
+ * update requested with nothing held (the next two-step pass reflects the
+ * ring of the field it stages).  Every other entry point applies them
+ * first, and tea_hip_flush applies them at once. */
 int tea_hip_local_halos(tea_hip_chunk* chunk, int depth,
                         const int neighbours[4],
                         const int fields_to_exchange[6]);
@@ -400,7 +416,8 @@ int tea_hip_fcg_read_z(tea_hip_chunk* chunk, float* host, size_t count);
  * mode 0.  Mode 0: one launch per reference kernel. */
 int tea_hip_set_fused(tea_hip_chunk* chunk, int fused);
 
-/* Applies any deferred CG work now. */
+/* Applies every deferred update now (CG work, held Chebyshev or PPCG
+ * steps, held halo updates) and waits for the stream. */
 int tea_hip_flush(tea_hip_chunk* chunk);
 
 /* Kernel timing with HIP events on the handle's stream (off by default). */

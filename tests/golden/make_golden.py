@@ -18,6 +18,12 @@ refhost_oracle.json
     the eigenvalue bounds as printed (%.12e) and the final "Actual"
     temperature as printed (%.15e).  This pins the host control flow of this
     project's drivers against the reference's.
+
+config2_4096_oracle.json (`make_golden.py config2`, no reference needed)
+    One timestep of the benchmark workload at 4096^2 on this project's host
+    and the CPU oracle: iteration count, the four field summaries as exact
+    doubles and the SHA-256 of the final interior u, which the GPU must
+    reproduce bit for bit.
 """
 
 import json
@@ -105,7 +111,40 @@ def run_refhost(deck):
                 final_temp_printed=actual[-1] if actual else None)
 
 
+ORACLE_HOST = os.path.join(REPO, "oracle", "build", "tealeaf_oracle")
+
+
+def config2_fixture(cells=4096, steps=1):
+    """The bench workload (4096^2, K = 1/density, tea.in + 16 seeded
+    rectangles, eps 1e-15, max 10000 iterations), first timestep, on this
+    project's host driving the CPU oracle: iteration counts, the four field
+    summaries as exact doubles (hex) and the SHA-256 of the final interior u
+    (float64, row-major)."""
+    import hashlib
+    from exploringsycl_amd.deck import config_deck
+    deck = config_deck(2, x_cells=cells, end_step=steps)
+    with tempfile.TemporaryDirectory() as wd:
+        deck.write(os.path.join(wd, "tea.in"))
+        subprocess.run([ORACLE_HOST, "--quiet", "--json", "run.json", "--field-out", "u.bin"],
+                       cwd=wd, check=True, env=dict(os.environ, OMP_NUM_THREADS="8"))
+        rec = json.load(open(os.path.join(wd, "run.json")))
+        with open(os.path.join(wd, "u.bin"), "rb") as f:
+            digest = hashlib.sha256(f.read()).hexdigest()
+    return dict(cells=cells, steps=steps, deck=deck.render(),
+                iterations=[s["iterations"] for s in rec["steps"]],
+                executed=[s["executed"] for s in rec["steps"]],
+                summaries=[{k: (float(v).hex() if k in ("vol", "mass", "ie", "temp") else v)
+                            for k, v in sm.items()} for sm in rec["summaries"]],
+                u_sha256=digest)
+
+
 def main():
+    if sys.argv[1:] == ["config2"]:
+        fx = config2_fixture()
+        with open(os.path.join(HERE, "config2_4096_oracle.json"), "w") as f:
+            json.dump(fx, f, indent=1)
+        print("wrote config2_4096_oracle.json", fx["iterations"])
+        return
     fx = fortran_fixtures()
     with open(os.path.join(HERE, "fortran_bm.json"), "w") as f:
         json.dump(fx, f, indent=1)
