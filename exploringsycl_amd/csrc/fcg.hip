@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "tea_hip_impl.h"
@@ -549,10 +551,10 @@ constexpr int kCh = 32;                 // hidden channels
 constexpr int kPx1 = (kS0 - 2) * (kS0 - 2);  // 784: layer-1 output pixels
 constexpr int kPx2 = (kS0 - 4) * (kS0 - 4);  // 676
 // LDS: activations A (layer 1, 3, 5 outputs) and B (input, layer 2, 4
-// outputs), 64 bytes per pixel (32 BF16 channels in four 16-byte chunks,
-// chunk c of pixel p stored in slot c ^ ((p >> 2) & 3) so that the 16 lanes
-// of a 16-byte read, at consecutive pixels, hit distinct banks), then the
-// per-tile arrays kept to the end.
+// outputs), 64 bytes per pixel (32 BF16 channels in four 16-byte chunks),
+// then the
+// per-tile arrays kept to the end (the chunk order within a pixel is given by
+// act_slot).
 constexpr int kBytesA = kPx1 * 64;
 constexpr int kBytesB = kPx2 * 64;
 constexpr int kLdsBytes = kBytesA + kBytesB + 4 * kSN * kSN * 4;
@@ -565,65 +567,95 @@ static_assert(3 * 256 * 8 <= kBytesB, "reductions must fit in B");
 typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
 typedef float f32x16v __attribute__((ext_vector_type(16)));
 
-__device__ inline int act_slot(int p, int c) { return (p << 2) + (c ^ ((p >> 2) & 3)); }
+// Chunk c of pixel p is stored in slot c ^ (((p >> 1) ^ (p >> 3)) & 3) of the
+// pixel's four: 8 or 16 consecutive pixels' chunk c then fill distinct
+// 16-byte bank groups, so a 16-byte LDS read by consecutive lanes is
+// conflict-free whether the LDS serves 8 or 16 lanes per cycle.
+__device__ inline int act_slot(int p, int c) { return (p << 2) + (c ^ (((p >> 1) ^ (p >> 3)) & 3)); }
+
+// A wave's operands of one layer: its weight fragments (the A operand of
+// each of the NS K-steps: lane l holds output channel l % 32 and the step's
+// 8 K-values of half l / 32) and the biases of the output channels it holds.
+template <int kCin>
+struct LayerFrag {
+  static constexpr int NS = kCin == fused::kCh ? 18 : 5;
+  bf16x8 A[NS];
+  float bco[16];
+  __device__ inline void load(const bf16* __restrict__ wt, const float* __restrict__ bias) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int hf = lane >> 5, col = lane & 31;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+      const int tap = kCin == fused::kCh ? t >> 1 : 2 * t + hf;
+      const int ci = kCin == fused::kCh ? ((t & 1) << 4) + 8 * hf : 0;
+      A[t] = tap < 9 ? *reinterpret_cast<const bf16x8*>(wt + ((size_t)col * 9 + tap) * kCin + ci)
+                     : bf16x8{};
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) bco[v] = bias[8 * (v >> 2) + 4 * hf + (v & 3)];
+  }
+};
 
 // One 32 -> 32 (or 8 -> 32 for the first layer) convolution over a square
 // region: input side si (chunks per pixel kCin / 8), output side si - 2,
 // ReLU(D + bias), BF16, zero at pixels outside the grid.  Output pixel q of
-// the region lies at grid (gx0 + q % so, gy0 + q / so).
-template <int kCin>
+// the region lies at grid (gx0 + q % so, gy0 + q / so).  Each wave takes NT
+// 32-pixel tiles at a time (tiles base, base + nwaves, ...), so NT
+// independent MFMA chains hide the LDS reads of one another.
+template <int kCin, int NT>
 __device__ inline void fused_conv(const bf16x8* __restrict__ in, bf16x8* __restrict__ out,
-                                  int si, const bf16* __restrict__ wt,
-                                  const float* __restrict__ bias, int gx0, int gy0, int hc,
+                                  int si, const LayerFrag<kCin>& w, int gx0, int gy0, int hc,
                                   int wc, int nwaves) {
-  constexpr int NS = kCin == fused::kCh ? 18 : 5;
+  constexpr int NS = LayerFrag<kCin>::NS;
   const int so = si - 2, npx = so * so;
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   const int hf = lane >> 5, col = lane & 31;
-  bf16x8 A[NS];
-#pragma unroll
-  for (int t = 0; t < NS; ++t) {
-    const int tap = kCin == fused::kCh ? t >> 1 : 2 * t + hf;
-    const int ci = kCin == fused::kCh ? ((t & 1) << 4) + 8 * hf : 0;
-    A[t] = tap < 9 ? *reinterpret_cast<const bf16x8*>(wt + ((size_t)col * 9 + tap) * kCin + ci)
-                   : bf16x8{};
-  }
-  float bco[16];
-#pragma unroll
-  for (int v = 0; v < 16; ++v) bco[v] = bias[8 * (v >> 2) + 4 * hf + (v & 3)];
   const int ntiles = (npx + 31) / 32;
-  for (int tile = wv; tile < ntiles; tile += nwaves) {
-    const int q = min(tile * 32 + col, npx - 1);
-    const int r = q / so, c = q - r * so;
-    f32x16v acc = {};
+  for (int base = wv; base < ntiles; base += NT * nwaves) {
+    int r[NT], c[NT], q[NT];
+    f32x16v acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      q[j] = min((base + j * nwaves) * 32 + col, npx - 1);
+      r[j] = q[j] / so;
+      c[j] = q[j] - r[j] * so;
+      acc[j] = f32x16v{};
+    }
 #pragma unroll
     for (int t = 0; t < NS; ++t) {
       const int tap = kCin == fused::kCh ? t >> 1 : 2 * t + hf;
-      bf16x8 b = bf16x8{};
-      if (tap < 9) {
-        const int pin = (r + tap / 3) * si + (c + tap % 3);
-        b = kCin == fused::kCh ? in[act_slot(pin, ((t & 1) << 1) + hf)] : in[pin];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        bf16x8 b = bf16x8{};
+        if (tap < 9) {
+          const int pin = (r[j] + tap / 3) * si + (c[j] + tap % 3);
+          b = kCin == fused::kCh ? in[act_slot(pin, ((t & 1) << 1) + hf)] : in[pin];
+        }
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.A[t], b, acc[j], 0, 0, 0);
       }
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[t], b, acc, 0, 0, 0);
     }
-    if (tile * 32 + col >= npx) continue;
-    const int gx = gx0 + c, gy = gy0 + r;
-    const bool ing = gx >= 0 && gx < wc && gy >= 0 && gy < hc;
-    bf16x4v* o = reinterpret_cast<bf16x4v*>(out);
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      bf16x4v v;
+    for (int j = 0; j < NT; ++j) {
+      const int tile = base + j * nwaves;
+      if (tile >= ntiles || tile * 32 + col >= npx) continue;
+      const int gx = gx0 + c[j], gy = gy0 + r[j];
+      const bool ing = gx >= 0 && gx < wc && gy >= 0 && gy < hc;
+      bf16x4v* o = reinterpret_cast<bf16x4v*>(out);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float vv = acc[4 * g + k] + bco[4 * g + k];
-        v[k] = (bf16)(ing && vv > 0.0f ? vv : 0.0f);
+      for (int g = 0; g < 4; ++g) {
+        bf16x4v v;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float vv = acc[j][4 * g + k] + w.bco[4 * g + k];
+          v[k] = (bf16)(ing && vv > 0.0f ? vv : 0.0f);
+        }
+        o[2 * act_slot(q[j], g) + hf] = v;
       }
-      o[2 * act_slot(q, g) + hf] = v;
     }
   }
 }
 
-template <int kThreads>
+template <int kThreads, int NT>
 __global__ __launch_bounds__(kThreads) void net_level_kernel(
     int hc, int wc, const float* __restrict__ rbar, const float* __restrict__ dis,
     const float* __restrict__ feat, const float* __restrict__ kc,
@@ -655,8 +687,16 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
   __shared__ double s_scale[256];
   __shared__ float s_scale0;
   if (tid < 256) {
+    // Thread t adds partials t, t + 256, ... in turn, loaded 8 at a time.
     double v = 0.0;
-    for (int i = tid; i < nrms; i += 256) v += rms[i];
+    for (int i0 = tid; i0 < nrms; i0 += 256 * 8) {
+      double t8[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) t8[j] = i0 + 256 * j < nrms ? rms[i0 + 256 * j] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (i0 + 256 * j < nrms) v += t8[j];
+    }
     s_scale[tid] = v;
   }
   // Stage rbar and D^-1/2 on the tile and 13 cells around it.
@@ -724,22 +764,27 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
     s_ds[q] = v_ds[vi];
     s_rb[q] = v_rb[vi];
   }
-  __syncthreads();
   // Layers 1..5: input (B, side 30) -> A (28) -> B (26) -> A (24) -> B (22)
-  // -> A (20).
-  fused_conv<8>(actB, actA, kS0, wts.wf1, wts.bias, X0 - kRN + 1, Y0 - kRN + 1, hc, wc,
-                kWaves_);
-  __syncthreads();
+  // -> A (20).  Each layer's weights are loaded before the barrier that
+  // waits for the layer before it, so the loads overlap the wait.
+  {
+    LayerFrag<8> w1;
+    w1.load(wts.wf1, wts.bias);
+    __syncthreads();
+    fused_conv<8, NT>(actB, actA, kS0, w1, X0 - kRN + 1, Y0 - kRN + 1, hc, wc, kWaves_);
+  }
 #pragma unroll 1
   for (int l = 1; l < 5; ++l) {
     const int si = kS0 - 2 * l;
     const int off = kRN - l - 1;  // output region origin relative to the tile
     const bf16x8* src = (l & 1) ? actA : actB;
     bf16x8* dst = (l & 1) ? actB : actA;
-    fused_conv<kCh>(src, dst, si, wts.wfh + (size_t)(l - 1) * kCh * 9 * kCh,
-                    wts.bias + l * kCh, X0 - off, Y0 - off, hc, wc, kWaves_);
+    LayerFrag<kCh> w;
+    w.load(wts.wfh + (size_t)(l - 1) * kCh * 9 * kCh, wts.bias + l * kCh);
     __syncthreads();
+    fused_conv<kCh, NT>(src, dst, si, w, X0 - off, Y0 - off, hc, wc, kWaves_);
   }
+  __syncthreads();
   // Output layer on the 18 x 18 region from A (side 20), FP32 in the order
   // of out_layer_kernel; then y = N sigma D^-1/2 (zero outside the grid).
   const float b6 = wts.bias[5 * kCh];
@@ -904,28 +949,30 @@ __global__ __launch_bounds__(256) void net_tile_kernel(
 // levels: the joint minimiser of the energy norm of e - g0 P0 y0 - g1 P1 y1
 // (the system of gamma2_kernel), whose cross entry c0^2 y0.T0 (Q y1) is the
 // sum over level-1 cells of xsum * y1 (T0 is symmetric).  Each sum in the
-// order of one 256-thread block: thread t adds elements t, t + 256, ... in
-// turn, then the LDS tree.
-__global__ __launch_bounds__(256) void joint_gamma_kernel(
+// order of one kGammaThreads-thread block: thread t adds elements t,
+// t + kGammaThreads, ... in turn, then the LDS tree.
+constexpr int kGammaThreads = 1024;
+__global__ __launch_bounds__(kGammaThreads) void joint_gamma_kernel(
     int levels, const double* __restrict__ p0, int n0, const double* __restrict__ p1,
     int n1, const double* __restrict__ xsum, const float* __restrict__ y1, int nx,
     double c0sq, double c1sq, float* __restrict__ g) {
-  __shared__ double sh[5][256];
+  constexpr int kT = kGammaThreads;
+  __shared__ double sh[5][kT];
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   // Each thread's elements are loaded 16 at a time, then added in turn (the
   // order above), so that the loads overlap.
   constexpr int kB = 16;
   auto strided = [&](const double* a, const float* f, int n, double& acc) {
-    for (int k0 = threadIdx.x; k0 < n; k0 += 256 * kB) {
+    for (int k0 = threadIdx.x; k0 < n; k0 += kT * kB) {
       double t[kB];
 #pragma unroll
       for (int j = 0; j < kB; ++j) {
-        const int k = k0 + 256 * j;
+        const int k = k0 + kT * j;
         t[j] = k < n ? (f ? a[k] * (double)f[k] : a[k]) : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < kB; ++j)
-        if (k0 + 256 * j < n) acc += t[j];
+        if (k0 + kT * j < n) acc += t[j];
     }
   };
   strided(p0, nullptr, n0, v[0]);
@@ -937,7 +984,7 @@ __global__ __launch_bounds__(256) void joint_gamma_kernel(
   }
   for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
+  for (int st = kT / 2; st > 0; st >>= 1) {
     if (threadIdx.x < st)
       for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
     __syncthreads();
@@ -982,17 +1029,43 @@ int ensure_fcg_buffers(tea_hip_chunk* c) {
 
 using namespace tea;
 
-// Threads of net_level_kernel's blocks.
-constexpr int kNetThreads = 512;
+// net_level_kernel's launch shapes: threads per block and 32-pixel tiles
+// in flight per wave (TEA_FCG_NET_SHAPE=threads x tiles, e.g. 512x2).
+struct NetShape {
+  int threads, tiles;
+};
+static const NetShape kNetShapes[] = {{512, 1}, {512, 2}, {512, 4}, {1024, 1}, {1024, 2}};
+static NetShape net_shape() {
+  static const NetShape v = [] {
+    NetShape s{512, 2};
+    const char* e = getenv("TEA_FCG_NET_SHAPE");
+    int t = 0, n = 0;
+    if (e && sscanf(e, "%dx%d", &t, &n) == 2)
+      for (const NetShape& k : kNetShapes)
+        if (k.threads == t && k.tiles == n) s = k;
+    return s;
+  }();
+  return v;
+}
+typedef void (*NetLevelFn)(int, int, const float*, const float*, const float*, const float*,
+                           const double*, int, nn::PackedWeights, float*, double*, int,
+                           double*);
+static NetLevelFn net_level_fn(NetShape s) {
+  if (s.threads == 1024) return s.tiles == 2 ? net_level_kernel<1024, 2> : net_level_kernel<1024, 1>;
+  return s.tiles == 4 ? net_level_kernel<512, 4>
+                      : s.tiles == 2 ? net_level_kernel<512, 2> : net_level_kernel<512, 1>;
+}
 
-// Whether the learned coarse solve runs as one fused launch per level (the
-// default) or as the per-layer pipeline (TEA_FCG_NET=layers: box_rows,
-// box_cols, the network's own kernels, net_tile_kernel), which gives the
-// same y bit for bit.
+// Whether the learned coarse solve runs as the per-layer pipeline (the
+// default: box_rows, box_cols, the network's own kernels, net_tile_kernel)
+// or as one fused launch per level (TEA_FCG_NET=fused), which gives the same
+// y bit for bit.  The fused kernel recomputes each tile's halo at every layer
+// (2.1 times the pixels of the per-layer pipeline with 16 x 16 tiles) and
+// measured slower at 4096^2 (about 91 against 80 us per coarse launch).
 static bool net_per_layer() {
   static const bool v = [] {
     const char* e = getenv("TEA_FCG_NET");
-    return e != nullptr && strcmp(e, "layers") == 0;
+    return e == nullptr || strcmp(e, "fused") != 0;
   }();
   return v;
 }
@@ -1006,9 +1079,10 @@ static int launch_net_level(tea_hip_chunk* c, int l, hipStream_t st) {
   double* xs = (l == 0 && c->fcg_levels == 2) ? c->fcg_xsum : nullptr;
   Timed t(c, K_FCG_NET, st);
   if (!net_per_layer()) {
-    hipLaunchKernelGGL((net_level_kernel<kNetThreads>), tiles, dim3(kNetThreads),
-                       fused::kLdsBytes, st, L.hc, L.wc, L.rbar, L.dis, L.feat, L.kc, L.rms,
-                       L.nrms, nn::packed_weights(L.net), L.y, L.tpart, L.ntiles, xs);
+    const NetShape shape = net_shape();
+    hipLaunchKernelGGL(net_level_fn(shape), tiles, dim3(shape.threads), fused::kLdsBytes, st,
+                       L.hc, L.wc, L.rbar, L.dis, L.feat, L.kc, L.rms, L.nrms,
+                       nn::packed_weights(L.net), L.y, L.tpart, L.ntiles, xs);
   } else {
     const int nblk = (L.hc * L.wc + 255) / 256;
     hipLaunchKernelGGL(box_rows_kernel, dim3(nblk + 1), dim3(256), 0, st, L.hc, L.wc,
@@ -1069,7 +1143,8 @@ static int launch_coarse(tea_hip_chunk* c) {
     TEA_RETURN_IF(level(0, c->stream));
   }
   Timed t(c, net ? K_FCG_NET : K_FCG_SWEEPS);
-  hipLaunchKernelGGL(joint_gamma_kernel, dim3(1), dim3(256), 0, c->stream, nl, L0.tpart,
+  hipLaunchKernelGGL(joint_gamma_kernel, dim3(1), dim3(kGammaThreads), 0, c->stream, nl,
+                     L0.tpart,
                      L0.ntiles, L1.tpart, L1.ntiles, c->fcg_xsum, L1.y,
                      nl == 2 ? L1.hc * L1.wc : 0, (double)L0.c * L0.c,
                      (double)L1.c * L1.c, c->fcg_g);
@@ -1164,9 +1239,10 @@ static void free_level(FcgLevel& L) {
 static int ensure_net_lds(int device) {
   static bool done[64] = {};
   if (device >= 0 && device < 64 && done[device]) return TEA_HIP_OK;
-  TEA_HIP_TRY(hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&net_level_kernel<kNetThreads>),
-      hipFuncAttributeMaxDynamicSharedMemorySize, fused::kLdsBytes));
+  for (const NetShape& k : kNetShapes)
+    TEA_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(net_level_fn(k)),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    fused::kLdsBytes));
   if (device >= 0 && device < 64) done[device] = true;
   return TEA_HIP_OK;
 }

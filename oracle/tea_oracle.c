@@ -846,16 +846,20 @@ void orc_fcg_sweeps(int hc, int wc, const float* kc, const float* rbar,
   free(ty);
 }
 
-/* Sum of v[0..n) (times (double)f[k] if f) in the order of one 256-thread
- * block: thread t adds k = t, t + 256, ... in turn, then the LDS tree. */
+/* Sum of v[0..n) (times (double)f[k] if f) in the order of one 1024-thread
+ * block: thread t adds k = t, t + 1024, ... in turn, then the LDS tree
+ * (stride 512, 256, ..., 1). */
+#define GAMMA_THREADS 1024
 static double block_sum(const double* v, const float* f, long n) {
-  double s[256];
-  for (int t = 0; t < 256; ++t) {
+  double s[GAMMA_THREADS];
+  for (int t = 0; t < GAMMA_THREADS; ++t) {
     double a = 0.0;
-    for (long k = t; k < n; k += 256) a += f ? v[k] * (double)f[k] : v[k];
+    for (long k = t; k < n; k += GAMMA_THREADS) a += f ? v[k] * (double)f[k] : v[k];
     s[t] = a;
   }
-  return tile_tree(s);
+  for (int st = GAMMA_THREADS / 2; st > 0; st >>= 1)
+    for (int t = 0; t < st; ++t) s[t] = s[t] + s[t + st];
+  return s[0];
 }
 
 void orc_fcg_gamma(int levels, const double* part0, int n0, const double* part1,

@@ -304,9 +304,9 @@ def test_learned_fcg_falls_back_and_converges(tmp_path):
 @pytest.mark.skipif(not os.path.exists(learned.WEIGHTS_PATH), reason="no trained network file")
 @pytest.mark.parametrize("n", [512, 1024, 2048])
 def test_fused_network_level_is_bitwise_the_per_layer_pipeline(tmp_path, n):
-    """The learned coarse solve in one launch per level (net_level_kernel)
-    against the per-layer pipeline (box_rows, box_cols, the network's own
-    layer kernels, net_tile_kernel; TEA_FCG_NET=layers): every output pixel
+    """The learned coarse solve in one launch per level (net_level_kernel,
+    TEA_FCG_NET=fused) against the per-layer pipeline (box_rows, box_cols,
+    the network's own layer kernels, net_tile_kernel): every output pixel
     takes the same MFMA sequence, so the whole solve -- iteration count,
     summaries and final field -- is the same bit for bit (512^2 has one
     coarse level, 1024^2 and 2048^2 two)."""
@@ -315,7 +315,7 @@ def test_fused_network_level_is_bitwise_the_per_layer_pipeline(tmp_path, n):
         wd = tmp_path / mode
         wd.mkdir()
         shutil.copy(learned.WEIGHTS_PATH, wd / "net.bin")
-        env = {"TEA_FCG_NET": "layers"} if mode == "layers" else {}
+        env = {"TEA_FCG_NET": mode}
         rec, _ = run_deck(_fcg_deck(n, 7, ["use_fcg", "fcg_weights=net.bin"]),
                           workdir=str(wd), env=env, extra_args=["--field-out", "u.bin"])
         runs[mode] = (rec, np.fromfile(wd / "u.bin"))
