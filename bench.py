@@ -100,45 +100,89 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(cells, seconds):
-    """OpenMP CPU restatement of the reference kernels, unfused dataflow
-    (calc_w, calc_ur, calc_p, U/P halos), on the same deck; times as many CG
-    iterations of the first timestep as fit in `seconds`."""
-    from exploringsycl_amd.deck import config_deck
-    from tests.oracle_ctypes import Oracle, OracleSolver
+def _omp_baseline_lib():
+    """The c_kernels-style OpenMP CG (oracle/ckernels_omp.c) built for this
+    host's vector ISA: the AVX-512 build where /proc/cpuinfo lists avx512f
+    (what -march=native selects there), else the AVX2 build."""
+    import ctypes
+    flags = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = line
+                    break
+    except OSError:
+        pass
+    isa = "v4" if " avx512f" in flags else "v3"
+    lib = ctypes.CDLL(os.path.join(REPO, "oracle", "build", f"libckernels_omp_{isa}.so"))
+    d, i, l = ctypes.c_double, ctypes.c_int, ctypes.c_long
+    pd, pi = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    lib.omp_cg_run.argtypes = [i, i, i, i, d, d, d, i, i, l, d, pd, pd, pi, pi, pd]
+    lib.omp_cg_run.restype = d
+    lib.omp_threads.restype = i
+    return lib, isa
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+
+def _omp_run(lib, deck, coefficient, max_total=0):
+    """Runs deck's timesteps (at most max_total CG iterations in all) on the
+    OpenMP baseline from the deck's initial state (set up by the oracle's
+    state kernels); returns (seconds, per-step executed iterations, temp)."""
+    import ctypes
+    import numpy as np
+    from tests.oracle_ctypes import Oracle, OracleSolver
     o = Oracle()
-    deck = config_deck(2, x_cells=cells)
-    s = OracleSolver(o, cells, cells, deck.states, coefficient=2)
-    x, y, h, f = s.x, s.y, s.h, s.f
-    rro = o.cg_init(x, y, h, 2, s.rx, s.ry, f)
-    for n in ("u", "p"):
-        o.halo_field(x, y, h, 1, f[n])
-    o.copy_u(x, y, h, f)
-    iters = 0
-    t0 = time.perf_counter()
-    while True:
-        pw = o.cg_calc_w(x, y, h, f)
-        alpha = rro / pw
-        rrn = o.cg_calc_ur(x, y, h, alpha, f)
-        o.cg_calc_p(x, y, h, rrn / rro, f)
-        for n in ("u", "p"):
-            o.halo_field(x, y, h, 1, f[n])
-        rro = rrn
-        iters += 1
-        el = time.perf_counter() - t0
-        if el >= seconds and iters >= 3:
-            break
-    gbs = BYTES_PER_CELL_ITER * cells * cells * iters / el / 1e9
+    s = OracleSolver(o, deck.x_cells, deck.y_cells, deck.states, coefficient=coefficient,
+                     dt=deck.dt, extent=deck.extent)
+    dens = np.ascontiguousarray(s.f["density"])
+    ener = np.ascontiguousarray(s.f["energy"])
+    iters = np.zeros(deck.end_step, dtype=np.int32)
+    steps = ctypes.c_int(0)
+    temp = ctypes.c_double(0.0)
+    pd = ctypes.POINTER(ctypes.c_double)
+    secs = lib.omp_cg_run(s.x, s.y, s.h, coefficient, s.rx, s.ry, deck.eps, deck.max_iters,
+                          deck.end_step, max_total, s.dx * s.dy, dens.ctypes.data_as(pd),
+                          ener.ctypes.data_as(pd),
+                          iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                          ctypes.byref(steps), ctypes.byref(temp))
+    return secs, [int(v) for v in iters[:steps.value]], temp.value
+
+
+def cpu_baseline(cells, seconds):
+    """The reference's CPU path in the style of its OpenMP c_kernels
+    (oracle/ckernels_omp.c: unfused calc_w / calc_ur / calc_p, OpenMP
+    reductions over rows, -O3 -march=native -funroll-loops -fopenmp) on all
+    the host threads OpenMP is given: a bounded sample of the same 4096^2
+    deck (as many CG iterations of its first timestep as fit in about
+    `seconds`), and the CPU configuration (512^2, 10 timesteps, K =
+    density) in full."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    from exploringsycl_amd.deck import config_deck
+    lib, isa = _omp_baseline_lib()
+    threads = lib.omp_threads()
+    deck = config_deck(2, x_cells=cells, end_step=1)
+    # Calibrate on 20 iterations, then size the sample to `seconds`.
+    t, it, _ = _omp_run(lib, deck, 2, max_total=20)
+    rate = sum(it) / t
+    n = max(20, int(rate * seconds))
+    t, it, _ = _omp_run(lib, deck, 2, max_total=n)
+    iters = sum(it)
+    gbs = BYTES_PER_CELL_ITER * cells * cells * iters / t / 1e9
+    c1 = config_deck(1)
+    t1, it1, temp1 = _omp_run(lib, c1, 1)
     return {
         "value": round(gbs, 3), "unit": "GB/s",
         "cores": threads, "kind": "port",
-        "sample": (f"first {iters} CG iterations of timestep 1 of the same "
-                   f"{cells}^2 deck, {el:.1f} s, OpenMP FP64 oracle "
-                   f"(unfused reference dataflow), same 88 B/cell/iteration "
-                   f"convention; {iters / el:.2f} iterations/s"),
-        "iterations_per_s": iters / el,
+        "sample": (f"first {iters} CG iterations of timestep 1 of the same {cells}^2 deck, "
+                   f"{t:.1f} s, OpenMP c_kernels-style CG (unfused, FP64, "
+                   f"-march=x86-64-{isa}), same 88 B/cell/iteration convention; "
+                   f"{iters / t:.2f} iterations/s"),
+        "iterations_per_s": iters / t,
+        "config1_512x512x10": {
+            "seconds": round(t1, 4), "iterations": it1, "temp": temp1,
+            "cg_iterations_per_s": round(sum(it1) / t1, 2),
+            "gbs": round(BYTES_PER_CELL_ITER * 512 * 512 * sum(it1) / t1 / 1e9, 3)},
     }
 
 
