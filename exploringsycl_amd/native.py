@@ -56,6 +56,7 @@ _SIGNATURES = {
     "tea_hip_copy_u": [_vp],
     "tea_hip_calculate_residual": [_vp],
     "tea_hip_calculate_2norm": [_vp, _vp, _dp],
+    "tea_hip_calculate_2norm_noalloc": [_i, _vp, _i, _i, _i, _vp, _vp],
     "tea_hip_finalise": [_vp],
     "tea_hip_pack_or_unpack": [_vp, _i, _i, _i, _vp, _dp],
     "tea_hip_jacobi_init": [_vp, _i, _d, _d],

@@ -114,6 +114,22 @@ int tea_hip_field_read(tea_hip_chunk* chunk, int field, double* host,
  * deferred: see tea_hip_flush). */
 int tea_hip_synchronize(tea_hip_chunk* chunk);
 
+/* No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  tea_hip_calculate_2norm for a field the caller owns:
+ * *norm (a DEVICE pointer, written on `stream`, a hipStream_t or NULL for
+ * the null stream) = the sum of b^2 over the interior of the x*y FP64
+ * device array `field` (halo_depth cells left out at each edge), with the
+ * reference's reduction tree, bit for bit as tea_hip_calculate_2norm.
+ * Allocates nothing: its working memory is a fixed device array of 16448
+ * doubles per GPU (held until the process ends), so x*y may be at most
+ * 16448 * 4096 = 67,371,008 cells (8192^2 with a halo of 2); larger grids
+ * are rejected with TEA_HIP_EINVAL.  Calls that overlap on one GPU share
+ * that array and race.  Asynchronous: the result is ready when `stream`
+ * is. */
+int tea_hip_calculate_2norm_noalloc(int device, const double* field, int x, int y,
+                                    int halo_depth, double* norm, void* stream);
+
 /* Text of the last error on the calling thread ("" if none). */
 const char* tea_hip_last_error(void);
 
