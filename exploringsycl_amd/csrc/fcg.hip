@@ -551,12 +551,10 @@ constexpr int kCh = 32;                 // hidden channels
 constexpr int kPx1 = (kS0 - 2) * (kS0 - 2);  // 784: layer-1 output pixels
 constexpr int kPx2 = (kS0 - 4) * (kS0 - 4);  // 676
 // LDS: activations A (layer 1, 3, 5 outputs) and B (input, layer 2, 4
-// outputs), 64 bytes per pixel (32 BF16 channels in four 16-byte chunks),
-// then the
-// per-tile arrays kept to the end (the chunk order within a pixel is given by
-// act_slot).
-constexpr int kBytesA = kPx1 * 64;
-constexpr int kBytesB = kPx2 * 64;
+// outputs), 32 BF16 channels per pixel in four 16-byte chunks at a pitch of
+// five (act_slot), then the per-tile arrays kept to the end.
+constexpr int kBytesA = kPx1 * 80;
+constexpr int kBytesB = kPx2 * 80;
 constexpr int kLdsBytes = kBytesA + kBytesB + 4 * kSN * kSN * 4;
 static_assert(kS0 * kS0 * 16 <= kBytesB, "packed input must fit in B");
 static_assert((kSV * kSV * 2 + kSV * kS0 + kS0 * kS0) * 4 <= kBytesA,
@@ -567,11 +565,12 @@ static_assert(3 * 256 * 8 <= kBytesB, "reductions must fit in B");
 typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
 typedef float f32x16v __attribute__((ext_vector_type(16)));
 
-// Chunk c of pixel p is stored in slot c ^ (((p >> 1) ^ (p >> 3)) & 3) of the
-// pixel's four: 8 or 16 consecutive pixels' chunk c then fill distinct
-// 16-byte bank groups, so a 16-byte LDS read by consecutive lanes is
-// conflict-free whether the LDS serves 8 or 16 lanes per cycle.
-__device__ inline int act_slot(int p, int c) { return (p << 2) + (c ^ (((p >> 1) ^ (p >> 3)) & 3)); }
+// Chunk c of pixel p is 16-byte slot 5 p + c: a pixel pitch of five chunks
+// (80 bytes; 5 is prime to 8 and 16), so the same chunk of any 8 or 16
+// consecutive pixels falls in distinct 16-byte bank groups, wherever the
+// run starts (a tap offset moves it), and a 16-byte LDS read by
+// consecutive lanes is conflict-free.
+__device__ inline int act_slot(int p, int c) { return 5 * p + c; }
 
 // A wave's operands of one layer: its weight fragments (the A operand of
 // each of the NS K-steps: lane l holds output channel l % 32 and the step's
