@@ -272,6 +272,77 @@ __global__ __launch_bounds__(kBlock) void fcg_z1_kernel(
   wave_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
+// The coarse values of a quad's four cells for one level.  With a
+// coarsening factor of 4 or more (shift >= 2) the quad's interior cells lie
+// in at most two coarse cells, those of its first and last interior cells
+// (klo, khi), so two loads serve all four.
+__device__ inline void quad_coarse(const CoarseTerm& t, int kk, int klo, int khi, int row,
+                                   int h, bool any, float (&y)[4]) {
+  const float ylo = t.y[any ? t.index(klo, row, h) : 0];
+  const float yhi = t.y[any ? t.index(khi, row, h) : 0];
+  const int xlo = (klo - h) >> t.shift;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) y[c] = ((kk + c - h) >> t.shift) == xlo ? ylo : yhi;
+}
+
+// fcg_z1_kernel's arithmetic four cells per lane (x a multiple of 4; every
+// level's coarsening factor at least 4).  Lane 63 loads the right
+// neighbour's kx with the quad's own loads.
+template <int NL>
+__global__ __launch_bounds__(kQuadThreads) void fcg_z4_kernel(
+    Geo g, const float* __restrict__ r, const float* __restrict__ kx,
+    const float* __restrict__ ky, const float* __restrict__ w, CoarseTerm t0,
+    CoarseTerm t1, float* __restrict__ z, double* sub, int nblk) {
+  const int blk = blockIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int x = g.x;
+  const int e = blk * kQuadBlock + 4 * threadIdx.x;
+  const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+  const int kk = e - row * x;
+  bool in[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) in[c] = e < g.n && interior(g, kk + c, row);
+  const bool any = in[0] || in[1] || in[2] || in[3];
+  const int ic = e < g.n ? e : x + 4;
+  const int ir = any ? e : x + 4;
+  const float4 R = ld4(r, ir), KX = ld4(kx, ic), KY = ld4(ky, ir), KYt = ld4(ky, ir + x),
+               W = ld4(w, ir);
+  float kxn = 0.0f;
+  if (lane == kWave - 1) kxn = kx[ir + 4];
+  const int klo = kk < g.h ? g.h : kk;
+  const int khi = kk + 3 > x - g.h - 1 ? x - g.h - 1 : kk + 3;
+  float y0[4] = {}, y1[4] = {};
+  float g0 = 0.0f, g1 = 0.0f;
+  if (NL > 0) {
+    g0 = t0.g[0];
+    quad_coarse(t0, kk, klo, khi, row, g.h, any, y0);
+  }
+  if (NL > 1) {
+    g1 = t1.g[0];
+    quad_coarse(t1, kk, klo, khi, row, g.h, any, y1);
+  }
+  float kxr = from_next_lane(KX.x);
+  if (lane == kWave - 1) kxr = kxn;
+  float zv[4];
+  double prod[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float kx0 = q_at(KX, c), kx1 = c == 3 ? kxr : q_at(KX, c + 1);
+    const double d = (1.0 + ((double)kx1 + (double)kx0)) + ((double)q_at(KYt, c) + (double)q_at(KY, c));
+    zv[c] = 0.0f;
+    if (in[c]) {
+      double v = (double)q_at(R, c) / d;
+      if (NL > 0) v = v + (double)g0 * (double)y0[c];
+      if (NL > 1) v = v + (double)g1 * (double)y1[c];
+      zv[c] = (float)v;
+    }
+    prod[c] = in[c] ? (double)zv[c] * (double)q_at(W, c) : 0.0;
+  }
+  double v[1] = {(prod[0] + prod[1]) + (prod[2] + prod[3])};
+  if (any) st4(z, e, in, zv);
+  wave_subtree_quads<1>(v, sub, nblk, blk);
+}
+
 // The coarse operator T = P^T A P / c^2: for each coarse cell, the face
 // coefficients of its left and bottom faces (kc[0][..], kc[1][..]): the sum
 // of the c fine face coefficients along the face over c^2, zero on the
@@ -1174,6 +1245,27 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
   const CoarseTerm t0 = nl > 0 ? term(0) : none;
   const CoarseTerm t1 = nl > 1 ? term(1) : none;
   const int kid = nl == 0 ? K_FCG_JACOBI : K_FCG_PROLONG;
+  bool quads = use_quad_blocks(c, 2);
+  for (int l = 0; l < nl; ++l) quads = quads && c->fcg_lv[l].c >= 4;
+  if (quads) {
+    {
+      Timed t(c, kid);
+      if (nl == 0)
+        hipLaunchKernelGGL((fcg_z4_kernel<0>), quad_grid(c), dim3(kQuadThreads), 0, c->stream, g,
+                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, c->sub,
+                           c->nblk);
+      else if (nl == 1)
+        hipLaunchKernelGGL((fcg_z4_kernel<1>), quad_grid(c), dim3(kQuadThreads), 0, c->stream, g,
+                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, c->sub,
+                           c->nblk);
+      else
+        hipLaunchKernelGGL((fcg_z4_kernel<2>), quad_grid(c), dim3(kQuadThreads), 0, c->stream, g,
+                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, c->sub,
+                           c->nblk);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    return launch_span_finish(c, 1, len, slot);
+  }
   if (use_pair_blocks(c, 2)) {
     {
       Timed t(c, kid);
@@ -1453,8 +1545,12 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
   if (use_pair_blocks(c, 1)) {
     {
       Timed t(c, K_FCG_W);
-      hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
-                         c->stream, c->geo, a, c->sub, c->nblk);
+      if (c->quads)
+        hipLaunchKernelGGL((mp_w4_kernel<true>), quad_grid(c), dim3(kQuadThreads), 0,
+                           c->stream, c->geo, a, c->sub, c->nblk);
+      else
+        hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
+                           c->stream, c->geo, a, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
     TEA_RETURN_IF(launch_span_finish(c, 2, &len, 0, true));

@@ -196,18 +196,19 @@ __device__ inline void mp_pair_load(const Geo& g, const MpArgs& a, MpPair& s) {
   const float* __restrict__ R = a.r;
   const int ic = s.e0 < g.n ? s.e0 : x + 2;
   const int ir = s.any ? s.e0 : x + 2;
+  // Every load is issued unconditionally, r's and u's also where the step
+  // does not use them (the first iteration only): loads under a branch were
+  // each waited for inside it, before the next ones were issued.
   s.P0 = ld2(P, ic);
   s.Pt = ld2(P, ir + x);
   s.Pb = ld2(P, ir - x);
-  if (a.apply_p) {
-    s.R0 = ld2(R, ic);
-    s.Rt = ld2(R, ir + x);
-    s.Rb = ld2(R, ir - x);
-  }
+  s.R0 = ld2(R, ic);
+  s.Rt = ld2(R, ir + x);
+  s.Rb = ld2(R, ir - x);
   s.KX = ld2(a.kx, ic);
   s.KY = ld2(a.ky, ir);
   s.KYt = ld2(a.ky, ir + x);
-  if (a.apply_u) s.U = ld2_nt(a.u, ir);
+  s.U = ld2_nt(a.u, ir);
   if (DOT2) s.RD = ld2(a.rdot, ir);
   // Lanes 0 and 63 share each load instruction, at their own addresses
   // (lane 0's kx value is not used).
@@ -215,7 +216,7 @@ __device__ inline void mp_pair_load(const Geo& g, const MpArgs& a, MpPair& s) {
   if (lane == 0 || lane == kWave - 1) {
     const int ni = lane == 0 ? ir - 1 : ir + 2;
     s.PN = P[ni];
-    if (a.apply_p) s.RN = R[ni];
+    s.RN = R[ni];
     s.KXN = a.kx[ni];
   }
 }
@@ -366,6 +367,188 @@ __global__ __launch_bounds__(kBlock) void mp_w1_kernel(Geo g, MpArgs a,
 
 template __global__ void mp_w1_kernel<true>(Geo, MpArgs, double*, int);
 
+// The reflection of an interior cell's new value v onto the depth-1 halo
+// ring where the cell borders it (kc, rc its column and row).
+template <class T>
+__device__ inline void ring_store(T* f, int i, int kc, int rc, int x, int xl, int xr, int yb,
+                                  int yt, T v) {
+  const bool L = kc == xl, Rg = kc == xr, B = rc == yb, Tp = rc == yt;
+  if (!(L | Rg | B | Tp)) return;
+  if (L) f[i - 1] = v;
+  if (Rg) f[i + 1] = v;
+  if (B) f[i - x] = v;
+  if (Tp) f[i + x] = v;
+  if (L && B) f[i - 1 - x] = v;
+  if (Rg && B) f[i + 1 - x] = v;
+  if (L && Tp) f[i - 1 + x] = v;
+  if (Rg && Tp) f[i + 1 + x] = v;
+}
+
+// Four cells per lane (x a multiple of 4): the arithmetic of mp_w_kernel cell
+// by cell, with 16-byte loads of the FP32 operands and of u.  The
+// neighbours across lanes move on the DPP wave shifts; lane 0 loads p and r
+// at e0 - 1, lane 63 p, r and kx at e0 + 4.  The in-lane dot sums are
+// ((v0 + v1) + (v2 + v3)), tree steps 1 and 2, and the waves store their
+// sums for span_finish_kernel (wave_subtree_quads).
+template <bool DOT2>
+__global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, double* sub, int nblk) {
+  const int blk = blockIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int x = g.x;
+  const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
+  const float* __restrict__ P = a.p_old;
+  const float* __restrict__ R = a.r;
+  const int e = blk * kQuadBlock + 4 * threadIdx.x;
+  const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+  const int kk = e - row * x;
+  bool in[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) in[c] = e < g.n && interior(g, kk + c, row);
+  const bool any = in[0] || in[1] || in[2] || in[3];
+  // A quad's own p, r and kx are its neighbours' left/right operands too, so
+  // they are read whenever it is in range; x + 4 is a safe aligned index
+  // whose rows above and below are in range.
+  const int ic = e < g.n ? e : x + 4;
+  const int ir = any ? e : x + 4;
+  // Every load is issued unconditionally, r's and u's also where the step
+  // does not use them (the first iteration only): loads under a branch were
+  // each waited for inside it, before the next ones were issued.
+  const float4 P0 = ld4(P, ic), Pt = ld4(P, ir + x), Pb = ld4(P, ir - x);
+  const float4 R0 = ld4(R, ic), Rt = ld4(R, ir + x), Rb = ld4(R, ir - x);
+  const float4 KX = ld4(a.kx, ic), KY = ld4(a.ky, ir), KYt = ld4(a.ky, ir + x);
+  // u moves in a lane layout of its own: lane l of a wave updates cells
+  // wb + 2l, +1 (A) and wb + 128 + 2l, +1 (B), wb the wave's first cell, so
+  // each 16-byte load and store covers 1 KB of consecutive cells (32 bytes
+  // per lane in the quad layout took two instructions over every other
+  // 16 bytes).  The p_old values come from the quad lanes by permute.
+  const int ua = blk * kQuadBlock + (threadIdx.x / kWave) * (4 * kWave) + 2 * lane;
+  const int ub = ua + 2 * kWave;
+  const int rA = ua < g.n ? (int)g.dx.div((uint32_t)ua) : 0;
+  const int kA = ua - rA * x;
+  const int rB = ub < g.n ? (int)g.dx.div((uint32_t)ub) : 0;
+  const int kB = ub - rB * x;
+  const bool inA0 = ua < g.n && interior(g, kA, rA);
+  const bool inA1 = ua < g.n && interior(g, kA + 1, rA);
+  const bool inB0 = ub < g.n && interior(g, kB, rB);
+  const bool inB1 = ub < g.n && interior(g, kB + 1, rB);
+  const double2 UA = ld2_nt(a.u, inA0 || inA1 ? ua : x + 2);
+  const double2 UB = ld2_nt(a.u, inB0 || inB1 ? ub : x + 2);
+  float4 RD = {};
+  if (DOT2) RD = ld4(a.rdot, ir);
+  float PN = 0.0f, RN = 0.0f, KXN = 0.0f;
+  if (lane == 0 || lane == kWave - 1) {
+    const int ni = lane == 0 ? ir - 1 : ir + 4;
+    PN = P[ni];
+    RN = R[ni];
+    KXN = a.kx[ni];
+  }
+  float pl = from_prev_lane(P0.w), pr = from_next_lane(P0.x);
+  float rl = 0.0f, rr = 0.0f;
+  if (a.apply_p) {
+    rl = from_prev_lane(R0.w);
+    rr = from_next_lane(R0.x);
+  }
+  float kxr = from_next_lane(KX.x);
+  if (lane == 0) {
+    pl = PN;
+    rl = RN;
+  }
+  if (lane == kWave - 1) {
+    pr = PN;
+    rr = RN;
+    kxr = KXN;
+  }
+  const double beta = a.beta;
+  // p_new of the quad's cells, of its left and right neighbours, and of the
+  // cells below and above (a halo neighbour reflects the cell itself).
+  // Vector values, not arrays: an array indexed by the loop counter (pc[c - 1])
+  // was placed in LDS by the compiler before the loops were unrolled.
+  fvec4 pc, pb, pt;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (a.apply_p) {
+      pc[c] = (float)(beta * (double)q_at(P0, c) + (double)q_at(R0, c));
+      pb[c] = row == yb ? pc[c] : (float)(beta * (double)q_at(Pb, c) + (double)q_at(Rb, c));
+      pt[c] = row == yt ? pc[c] : (float)(beta * (double)q_at(Pt, c) + (double)q_at(Rt, c));
+    } else {
+      pc[c] = q_at(P0, c);
+      pb[c] = q_at(Pb, c);
+      pt[c] = q_at(Pt, c);
+    }
+  }
+  // Left and right operands: the neighbouring cells' p_new, a reflection of
+  // the cell itself next to the halo (p_new's halo is not written yet), and
+  // across lanes the neighbour quads' values.
+  fvec4 lft, rgt;
+  if (a.apply_p) {
+    lft[0] = kk == xl ? pc[0] : (float)(beta * (double)pl + (double)rl);
+    rgt[3] = kk + 3 == xr ? pc[3] : (float)(beta * (double)pr + (double)rr);
+  } else {
+    lft[0] = pl;
+    rgt[3] = pr;
+  }
+  const bool refl = a.apply_p != 0;
+  lft[1] = refl && kk + 1 == xl ? pc[1] : pc[0];
+  lft[2] = refl && kk + 2 == xl ? pc[2] : pc[1];
+  lft[3] = refl && kk + 3 == xl ? pc[3] : pc[2];
+  rgt[0] = refl && kk == xr ? pc[0] : pc[1];
+  rgt[1] = refl && kk + 1 == xr ? pc[1] : pc[2];
+  rgt[2] = refl && kk + 2 == xr ? pc[2] : pc[3];
+  const fvec4 kx0 = {KX.x, KX.y, KX.z, KX.w}, kx1 = {KX.y, KX.z, KX.w, kxr};
+  const fvec4 ky0 = {KY.x, KY.y, KY.z, KY.w}, ky1 = {KYt.x, KYt.y, KYt.z, KYt.w};
+  fvec4 s;
+  double prod[DOT2 ? 2 : 1][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float diag = (1.0f + (kx1[c] + kx0[c])) + (ky1[c] + ky0[c]);
+    s[c] = (diag * pc[c] - (kx1[c] * rgt[c] + kx0[c] * lft[c])) -
+           (ky1[c] * pt[c] + ky0[c] * pb[c]);
+    prod[0][c] = in[c] ? (double)s[c] * (double)pc[c] : 0.0;
+    if (DOT2) prod[DOT2 ? 1 : 0][c] = in[c] ? (double)pc[c] * (double)q_at(RD, c) : 0.0;
+  }
+  double v[DOT2 ? 2 : 1];
+#pragma unroll
+  for (int q = 0; q < (DOT2 ? 2 : 1); ++q)
+    v[q] = (prod[q][0] + prod[q][1]) + (prod[q][2] + prod[q][3]);
+  if (a.apply_u) {
+    // Cells 2l, 2l + 1 of each half-wave are components (x, y) or (z, w) of
+    // quad lane l / 2 (A) or 32 + l / 2 (B).
+    const int sa = lane >> 1, sb = kWave / 2 + (lane >> 1);
+    const bool hi = (lane & 1) != 0;
+    const float ax = __shfl(P0.x, sa), ay = __shfl(P0.y, sa), az = __shfl(P0.z, sa),
+                aw = __shfl(P0.w, sa);
+    const float bx = __shfl(P0.x, sb), by = __shfl(P0.y, sb), bz = __shfl(P0.z, sb),
+                bw = __shfl(P0.w, sb);
+    const double un[4] = {UA.x + a.alpha * (double)(hi ? az : ax),
+                          UA.y + a.alpha * (double)(hi ? aw : ay),
+                          UB.x + a.alpha * (double)(hi ? bz : bx),
+                          UB.y + a.alpha * (double)(hi ? bw : by)};
+    if (inA0 || inA1) st2_nt(a.u, ua, inA0, inA1, un[0], un[1]);
+    if (inB0 || inB1) st2_nt(a.u, ub, inB0, inB1, un[2], un[3]);
+    // Halo ring of boundary-adjacent cells (rare; per cell).
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const bool inc = c == 0 ? inA0 : c == 1 ? inA1 : c == 2 ? inB0 : inB1;
+      const int i = (c < 2 ? ua : ub) + (c & 1);
+      const int kc = (c < 2 ? kA : kB) + (c & 1), rc = c < 2 ? rA : rB;
+      if (inc) ring_store(a.u, i, kc, rc, x, xl, xr, yb, yt, un[c]);
+    }
+  }
+  if (any) {
+    st4(a.w, e, in, s);
+    if (a.apply_p) {
+      st4_nt(a.p_new, e, in, pc);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (in[c]) ring_store(a.p_new, e + c, kk + c, row, x, xl, xr, yb, yt, pc[c]);
+    }
+  }
+  wave_subtree_quads<DOT2 ? 2 : 1>(v, sub, nblk, blk);
+}
+
+template __global__ void mp_w4_kernel<true>(Geo, MpArgs, double*, int);
+
+
 // r -= alpha w and r.r (with RW also r.w), two cells per lane.
 template <bool RW>
 __global__ __launch_bounds__(kBlock) void mp_r2_kernel(
@@ -423,6 +606,37 @@ __global__ __launch_bounds__(kBlock) void mp_r1_kernel(
                     (in1 ? (double)rb * (double)wv.y : 0.0);
   st2(r, e, in0, in1, ra, rb);
   wave_subtree_pairs<RW ? 2 : 1>(v, sub, nblk, blk);
+}
+
+// The same four cells per lane (x a multiple of 4).
+template <bool RW>
+__global__ __launch_bounds__(kQuadThreads) void mp_r4_kernel(
+    Geo g, float* __restrict__ r, const float* __restrict__ w, double alpha,
+    double* sub, int nblk) {
+  const int blk = blockIdx.x;
+  const int e = blk * kQuadBlock + 4 * threadIdx.x;
+  const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+  const int kk = e - row * g.x;
+  bool in[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) in[c] = e < g.n && interior(g, kk + c, row);
+  const bool any = in[0] || in[1] || in[2] || in[3];
+  const int i = any ? e : 0;
+  const float4 rv = ld4(r, i), wv = ld4(w, i);
+  float rn[4];
+  double prod[RW ? 2 : 1][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    rn[c] = (float)((double)q_at(rv, c) - alpha * (double)q_at(wv, c));
+    prod[0][c] = in[c] ? (double)rn[c] * (double)rn[c] : 0.0;
+    if (RW) prod[RW ? 1 : 0][c] = in[c] ? (double)rn[c] * (double)q_at(wv, c) : 0.0;
+  }
+  double v[RW ? 2 : 1];
+#pragma unroll
+  for (int q = 0; q < (RW ? 2 : 1); ++q)
+    v[q] = (prod[q][0] + prod[q][1]) + (prod[q][2] + prod[q][3]);
+  if (any) st4(r, e, in, rn);
+  wave_subtree_quads<RW ? 2 : 1>(v, sub, nblk, blk);
 }
 
 namespace {
@@ -528,6 +742,19 @@ int ensure_mp_buffers(tea_hip_chunk* c) {
 // left to finish in each array.
 int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw) {
   *len = c->nspan;
+  if (use_quad_blocks(c, 2)) {
+    {
+      Timed t(c, K_MP_R);
+      if (rw)
+        hipLaunchKernelGGL(mp_r4_kernel<true>, quad_grid(c), dim3(kQuadThreads), 0, c->stream,
+                           c->geo, c->mp_r, c->mp_w, alpha, c->sub, c->nblk);
+      else
+        hipLaunchKernelGGL(mp_r4_kernel<false>, quad_grid(c), dim3(kQuadThreads), 0, c->stream,
+                           c->geo, c->mp_r, c->mp_w, alpha, c->sub, c->nblk);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    return launch_span_finish(c, rw ? 2 : 1, len, 0, rw);
+  }
   if (use_pair_blocks(c, 2)) {
     {
       Timed t(c, K_MP_R);
@@ -622,8 +849,12 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
   if (use_pair_blocks(c, 1)) {
     {
       Timed t(c, K_MP_W);
-      hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
-                         c->stream, c->geo, a, c->sub, c->nblk);
+      if (c->quads)
+        hipLaunchKernelGGL((mp_w4_kernel<true>), quad_grid(c), dim3(kQuadThreads), 0,
+                           c->stream, c->geo, a, c->sub, c->nblk);
+      else
+        hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
+                           c->stream, c->geo, a, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
     TEA_RETURN_IF(launch_span_finish(c, 2, &len, 0, true));

@@ -359,6 +359,97 @@ __device__ inline void wave_subtree_pairs(double (&v)[NR], double* sub,
   }
 }
 
+// ---- four cells per lane ---------------------------------------------------
+//
+// A 128-thread block over one 512-cell pair block, four cells per lane (x a
+// multiple of 4, so a quad never straddles a row and every quad, and the
+// quads one row up and down, are 16-byte aligned).  The block covers the
+// cells of a pair block so that, as there, the rows above and below a block
+// at x = 4100 lie 8 blocks back and ahead: on the same XCD, whose L2 has
+// them (with 1024-cell blocks they lie 4 blocks away, on another XCD: the
+// prolongation, which reads ky one row up, took 12-22% longer at 4096^2).  The in-lane sum ((v0 + v1) +
+// (v2 + v3)) is tree steps 1 and 2; the four DPP steps of wave_subtree_pairs
+// then finish a 64-cell group in each 16-lane row, so wave w (of 2) holds
+// four groups, g0..g3, and stores the sums pair waves 2w and 2w + 1 would:
+// g0 + g1 and g2 + g3, in sub[q * 4 * nblk + 4 * block + 2 * w] and the slot
+// after it.  span_finish_kernel completes the spans from there, bit for bit
+// as for the pair kernels.
+
+constexpr int kQuadThreads = kPairBlock / 4;  // 128
+constexpr int kQuadBlock = kPairBlock;        // 512 cells
+
+__device__ inline float4 ld4(const float* p, int i) {
+  return *reinterpret_cast<const float4*>(p + i);
+}
+
+__device__ inline float q_at(const float4& v, int c) {
+  return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
+}
+
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+// Stores a quad with one 16-byte store, or its interior members alone.
+__device__ inline void st4(float* p, int i, const bool (&in)[4], const float (&v)[4]) {
+  if (in[0] && in[1] && in[2] && in[3]) {
+    *reinterpret_cast<fvec4*>(p + i) = fvec4{v[0], v[1], v[2], v[3]};
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (in[c]) p[i + c] = v[c];
+  }
+}
+
+__device__ inline void st4(float* p, int i, const bool (&in)[4], fvec4 v) {
+  if (in[0] && in[1] && in[2] && in[3]) {
+    *reinterpret_cast<fvec4*>(p + i) = v;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (in[c]) p[i + c] = v[c];
+  }
+}
+
+// The same with streaming stores.
+__device__ inline void st4_nt(float* p, int i, const bool (&in)[4], fvec4 v) {
+  if (in[0] && in[1] && in[2] && in[3]) {
+    __builtin_nontemporal_store(v, reinterpret_cast<fvec4*>(p + i));
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (in[c]) __builtin_nontemporal_store(v[c], p + i + c);
+  }
+}
+
+__device__ inline void st4_nt(float* p, int i, const bool (&in)[4], const float (&v)[4]) {
+  if (in[0] && in[1] && in[2] && in[3]) {
+    __builtin_nontemporal_store(fvec4{v[0], v[1], v[2], v[3]}, reinterpret_cast<fvec4*>(p + i));
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (in[c]) __builtin_nontemporal_store(v[c], p + i + c);
+  }
+}
+
+template <int NR>
+__device__ inline void wave_subtree_quads(double (&v)[NR], double* sub, int nblk, int blk) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    double t = v[q];
+    t = t + dpp<0xB1>(t);
+    t = t + dpp<0x4E>(t);
+    t = t + dpp<0x141>(t);
+    t = t + dpp<0x140>(t);
+    const double g01 = lane_value(t, 0) + lane_value(t, 16);
+    const double g23 = lane_value(t, 32) + lane_value(t, 48);
+    if (lane == 0) {
+      double* d = sub + (size_t)q * kWaves * nblk + kWaves * blk + 2 * wv;
+      *reinterpret_cast<dvec2*>(d) = dvec2{g01, g23};
+    }
+  }
+}
+
 // The cell pair (e, e + 1) of a pair-per-thread lane, e = 512 b + 2 m for
 // lane m of block b (x even: a pair never straddles a row).
 struct PairAt {
@@ -503,6 +594,8 @@ __global__ void mp_w2_kernel(Geo g, MpArgs a, double* partials, int nspan,
                              int single_group);
 template <bool DOT2>
 __global__ void mp_w1_kernel(Geo g, MpArgs a, double* sub, int nblk);
+template <bool DOT2>
+__global__ void mp_w4_kernel(Geo g, MpArgs a, double* sub, int nblk);
 
 }  // namespace tea
 
@@ -603,6 +696,7 @@ struct tea_hip_chunk {
   int fused = 1;
   int batch = 0;   // load batch of the fused stencil kernel (0: default)
   bool pairs = false;  // two-cells-per-lane kernels (x even)
+  bool quads = false;  // four-cells-per-lane mixed-precision kernels (x % 4 == 0)
   bool pend_u = false, pend_p = false, pend_halo = false;
   double pend_alpha = 0.0, pend_beta = 0.0;
 
@@ -713,6 +807,12 @@ inline bool use_pair_blocks(const tea_hip_chunk* c, int kernel_bit) {
   return c->pairs && (c->pair_blocks & kernel_bit) && c->ngroup > 1;
 }
 
+// The four-cells-per-lane variants of the pair-per-thread kernels (same
+// results bit for bit) apply where those do and x is a multiple of 4.
+inline bool use_quad_blocks(const tea_hip_chunk* c, int kernel_bit) {
+  return c->quads && use_pair_blocks(c, kernel_bit);
+}
+
 // Kinds of held-back step.
 enum { kHeldNone = 0, kHeldPpcg = 1, kHeldCheby = 2 };
 
@@ -756,6 +856,7 @@ int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
 
 inline dim3 span_grid(const tea_hip_chunk* c) { return dim3(c->nspan); }
 inline dim3 pair_grid(const tea_hip_chunk* c) { return dim3(c->nblk); }
+inline dim3 quad_grid(const tea_hip_chunk* c) { return dim3(c->nblk); }
 inline dim3 flat_grid(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
 }  // namespace tea

@@ -297,11 +297,15 @@ def test_halo_slot_names():
     assert HALO_SLOTS == ["density", "energy0", "energy1", "u", "p", "sd"]
 
 
-def test_mixed_precision_kernels_match_oracle(oracle):
+@pytest.mark.parametrize("xc", [123, 124, 126])
+def test_mixed_precision_kernels_match_oracle(oracle, xc):
     """The mixed-precision CG entry points against their CPU restatement,
-    call by call, on a ragged grid (bitwise FP32 arrays and FP64 scalars)."""
+    call by call, on ragged grids (bitwise FP32 arrays and FP64 scalars):
+    x = 127 takes the one-cell-per-lane kernels, x = 128 the four-cell
+    kernels (21 pair blocks: the last four-cell block is half empty), x = 130
+    the two-cell kernels."""
     import ctypes
-    xc, yc = 123, 77
+    yc = 77
     x, y = xc + 2 * H, yc + 2 * H
     f = _random_fields(x, y, 31)
     f["density"][:] = np.random.default_rng(3).uniform(0.1, 100.0, x * y)
