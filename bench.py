@@ -231,6 +231,8 @@ def time_to_solution(n, max_iters=12000, timeout=1200, names=ALL_TTS_SOLVERS):
             rec, _ = run_deck(deck, binary=TEALEAF_BIN, workdir=wd, timeout=timeout,
                               extra_args=["--warmup-steps", "1", "--field-out", "u.bin"])
             st = rec["steps"][0]
+            progress(f"{n}^2 {name}: {st['iterations']} iterations, "
+                     f"{rec['timed_seconds']:.3f} s")
             fields[name] = np.fromfile(os.path.join(wd, "u.bin"))
             out["solvers"][name] = {
                 "seconds": round(rec["timed_seconds"], 4),
@@ -305,6 +307,11 @@ def other_solvers(n, jacobi_iters=2000, max_iters=0, timeout=900):
     return out
 
 
+def progress(msg):
+    """One line on stderr per finished solve (stdout carries only the JSON line)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -337,6 +344,7 @@ def main():
     iters = sum(s["executed"] for s in rec["steps"])
     t = rec["timed_seconds"]
     value = BYTES_PER_CELL_ITER * n * n * iters / t / 1e9
+    progress(f"roofline run: {iters} CG iterations in {t:.3f} s, {value:.1f} GB/s")
 
     k = stats[kname]
     avg_ms = k["total_ms"] / k["launches"]
@@ -397,9 +405,11 @@ def main():
             sv["learned_fcg"]["speedup_vs_plain_cg"] if "learned_fcg" in sv else None)
     if args.solvers:
         line["other_solvers"] = other_solvers(n, max_iters=args.solver_max_iters)
+        progress("the reference's other solvers done")
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
         line["cpu_baseline"]["cpu_model"] = cpu_model()
+        progress("CPU baseline done")
     peak = copy_peak()
     roofline["copy_peak"] = {"value": round(peak, 1), "unit": "GB/s",
                              "method": "16-byte nontemporal device copy of 1 GiB, "
