@@ -143,6 +143,12 @@ struct FusedWArgs {
   const double* ky;
   double alpha, beta;
   int apply_u, ring;
+  // Pair-per-thread kernel only: the new u goes to u_out (u itself for the
+  // iteration the host asked for, a spare buffer when it runs ahead), and
+  // with dev_beta non-null beta is read from the device (the value the run
+  // ahead formed from the device's own partials).
+  double* u_out;
+  const double* dev_beta;
 };
 
 // Cells of one thread are visited in batches of BATCH: all loads of a batch
@@ -349,7 +355,7 @@ __device__ inline double fused_pair_compute(const Geo& g, const FusedWArgs& a,
   const int x = g.x;
   const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
   const double* __restrict__ P = a.p_old;
-  const double beta = a.beta;
+  const double beta = a.dev_beta ? *a.dev_beta : a.beta;
   const int i = s.e0;
   // Neighbours across the pair boundary, from the adjacent lanes.
   double pl = from_prev_lane(s.P0.y), pr = from_next_lane(s.P0.x);
@@ -407,7 +413,7 @@ __device__ inline double fused_pair_compute(const Geo& g, const FusedWArgs& a,
   if (a.apply_u) {
     ua = s.U.x + a.alpha * s.P0.x;
     ub = s.U.y + a.alpha * s.P0.y;
-    st2_nt(a.u, i, s.in0, s.in1, ua, ub);
+    st2_nt(a.u_out, i, s.in0, s.in1, ua, ub);
   }
   if (APPLY_P) st2_nt(a.p_new, i, s.in0, s.in1, ca, cb);
   // Halo ring of boundary-adjacent cells (rare; per cell).
@@ -430,7 +436,8 @@ __device__ inline double fused_pair_compute(const Geo& g, const FusedWArgs& a,
     if (L && T) ring[nr++] = ic - 1 + x;
     if (Rg && T) ring[nr++] = ic + 1 + x;
     for (int q = 0; q < nr; ++q) {
-      if (a.apply_u && a.ring) a.u[ring[q]] = uv;
+      if (a.apply_u && (a.ring || a.u_out != a.u))
+        a.u_out[ring[q]] = a.ring ? uv : a.u[ring[q]];
       if (APPLY_P) a.p_new[ring[q]] = a.ring ? pv : P[ring[q]];
     }
   }
@@ -509,10 +516,13 @@ __global__ __launch_bounds__(kBlock) void fused_r2_kernel(
 }
 
 // fused_r2's arithmetic with one pair per thread; the block stores its
-// 8-leaf subtree of r.r (see fused_w1_kernel).
+// 8-leaf subtree of r.r (see fused_w1_kernel).  The new r goes to r_out (r
+// itself, or a spare buffer when the iteration runs ahead), and with
+// dev_alpha non-null alpha is read from the device.
 __global__ __launch_bounds__(kBlock) void fused_r1_kernel(
-    Geo g, double* __restrict__ r, const double* __restrict__ w, double alpha,
-    double* sub, int nblk) {
+    Geo g, const double* r, double* r_out, const double* __restrict__ w, double alpha,
+    const double* __restrict__ dev_alpha, double* sub, int nblk) {
+  if (dev_alpha) alpha = *dev_alpha;
   const int blk = blockIdx.x;
   const int e = blk * kPairBlock + 2 * threadIdx.x;
   const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
@@ -524,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void fused_r1_kernel(
   const double ra = rv.x - alpha * wv.x;
   const double rb = rv.y - alpha * wv.y;
   double v[1] = {(in0 ? ra * ra : 0.0) + (in1 ? rb * rb : 0.0)};
-  st2_nt(r, e, in0, in1, ra, rb);
+  st2_nt(r_out, e, in0, in1, ra, rb);
   wave_subtree_pairs<1>(v, sub, nblk, blk);
 }
 
@@ -658,7 +668,138 @@ int sync_p_alt(tea_hip_chunk* c) {
   return TEA_HIP_OK;
 }
 
+// ---- run-ahead of the fused CG iteration ----------------------------------
+//
+// The drop-in API returns p.w and r.r to the host, which forms alpha =
+// rro / pw and beta = rrn / rro before the next pass can start
+// (TeaLeaf/drivers/cg_driver.c:87-113): two host round trips per iteration,
+// each costing the wait for the finisher's values, the host's arithmetic and
+// a launch (about 13.5 us of a 272 us iteration at 4096^2).  The device
+// forms the same scalars itself: the last block of each finisher runs the
+// host's final tree passes over the grid's partials and divides
+// (span_finish_kernel's run-ahead tail, core.hip), and the next pass of the
+// iteration is already queued behind it, writing only spare buffers:
+//   after cg_calc_w  (the r-step): r - alpha w into r_alt, and r.r;
+//   after cg_calc_ur (the w-step): u + alpha p into u_alt, p = beta p + r
+//     into p_alt with the U/P halo ring, w = A p into w_alt, p.w; then the
+//     next r-step.
+// The host's next call takes a pass over (exchanging the buffers) when its
+// scalar is bitwise the one the device used (read back with the sums) and,
+// for the w-step, the deferred work is exactly the u and p updates and the
+// U/P ring the pass applied; anything else drops the run-ahead, which wrote
+// nothing but spare buffers, device scalars and its own host slots.  The
+// results are those of the iteration without it, bit for bit.  Each step's
+// sum and scalar go to one of two host slot sets, alternating, so the next
+// step is queued before the host has read the last one.
+
+constexpr int kRaNone = 0, kRaR = 1, kRaWR = 2;
+constexpr int kRaW = 0, kRaRStep = 1;  // slot kinds
+
+int ra_len(const tea_hip_chunk* c) {
+  return c->nspan == 1 ? 1 : (c->nspan + kWave - 1) / kWave;
+}
+
+bool ra_possible(const tea_hip_chunk* c) {
+  return c->ra_enabled && c->fused && c->poll && c->pairs && c->ra_rro_known &&
+         c->ra_fail < 3 && use_pair_blocks(c, 1) && use_pair_blocks(c, 2) &&
+         ra_len(c) <= kWave * kWave;
+}
+
+double* ra_host_slot(tea_hip_chunk* c, int set, int kind) {
+  return c->ra_host + 2 * (2 * set + kind);
+}
+
+double* ra_dev_slot(tea_hip_chunk* c, int set, int kind) {
+  return c->ra_host_dev + 2 * (2 * set + kind);
+}
+
+int ensure_ra(tea_hip_chunk* c) {
+  if (c->ra_host) return TEA_HIP_OK;
+  const size_t bytes = (size_t)c->geo.n * sizeof(double);
+  for (double** b : {&c->r_alt, &c->u_alt, &c->w_alt}) {
+    if (hipMalloc(b, bytes) != hipSuccess) {
+      *b = nullptr;
+      set_error("hipMalloc of a run-ahead buffer failed");
+      return TEA_HIP_ENOMEM;
+    }
+  }
+  // [0..2] scalars, then the finisher's partials, then the blocks-done count.
+  const size_t nd = 3 + (size_t)ra_len(c);
+  if (hipMalloc(&c->ra_dev, (nd + 1) * sizeof(double)) != hipSuccess) {
+    c->ra_dev = nullptr;
+    set_error("hipMalloc of the run-ahead scalars failed");
+    return TEA_HIP_ENOMEM;
+  }
+  c->ra_count = reinterpret_cast<unsigned*>(c->ra_dev + nd);
+  TEA_HIP_TRY(hipMemsetAsync(c->ra_count, 0, sizeof(unsigned), c->stream));
+  double* host = nullptr;
+  if (hipHostMalloc(&host, 8 * sizeof(double),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    set_error("hipHostMalloc of the run-ahead slots failed");
+    return TEA_HIP_ENOMEM;
+  }
+  TEA_HIP_TRY(hipHostGetDevicePointer((void**)&c->ra_host_dev, host, 0));
+  c->ra_host = host;
+  return TEA_HIP_OK;
+}
+
+// The finisher of a run-ahead pass (into the device's partials) with its
+// scalar (into host slot (set, kind)).
+int ra_finish(tea_hip_chunk* c, int set, int kind, double rro, bool from_dev) {
+  if (c->ra_dirty) {
+    // A dropped run-ahead may still be queued and write these slots.
+    TEA_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->ra_dirty = false;
+  }
+  arm_host_values(ra_host_slot(c, set, kind), 2);
+  const RaTail tail{rro, from_dev ? 1 : 0, kind == kRaW ? 0 : 1, c->ra_dev,
+                    ra_dev_slot(c, set, kind), c->ra_count};
+  return launch_span_finish_ra(c, c->ra_dev + 3, tail);
+}
+
+// The r-step behind a w-step whose p.w the slot set's ratio turns into
+// alpha: r_alt = r - alpha w, r.r, beta.
+int ra_launch_r(tea_hip_chunk* c, int set, const double* w, double rro, bool from_dev) {
+  {
+    Timed t(c, K_CG_FUSED_R);
+    hipLaunchKernelGGL(fused_r1_kernel, pair_grid(c), dim3(kBlock), 0, c->stream, c->geo,
+                       c->field[TEA_FIELD_R], c->r_alt, w, 0.0, c->ra_dev, c->sub, c->nblk);
+  }
+  TEA_HIP_TRY(hipGetLastError());
+  return ra_finish(c, set, kRaRStep, rro, from_dev);
+}
+
+// Waits for a slot's sum and scalar.
+int ra_wait(tea_hip_chunk* c, int set, int kind, double* sum, double* ratio) {
+  const double* h = ra_host_slot(c, set, kind);
+  TEA_RETURN_IF(wait_host_values(c, h, 2));
+  *sum = h[0];
+  *ratio = h[1];
+  return TEA_HIP_OK;
+}
+
+void ra_mismatch(tea_hip_chunk* c) {
+  c->ra = kRaNone;
+  c->ra_dirty = true;
+  c->ra_fail += 1;
+}
+
+template <class T>
+void swap_ptr(T*& a, T*& b) {
+  T* t = a;
+  a = b;
+  b = t;
+}
+
 }  // namespace
+
+int ra_discard(tea_hip_chunk* c) {
+  if (c->ra != kRaNone) {
+    c->ra = kRaNone;
+    c->ra_dirty = true;
+  }
+  return TEA_HIP_OK;
+}
 
 int apply_pending(tea_hip_chunk* c) {
   TEA_RETURN_IF(apply_cg_pending(c));
@@ -666,6 +807,7 @@ int apply_pending(tea_hip_chunk* c) {
 }
 
 int apply_cg_pending(tea_hip_chunk* c) {
+  if (c->ra) TEA_RETURN_IF(ra_discard(c));
   if (!c->pend_u && !c->pend_p && !c->pend_halo) return TEA_HIP_OK;
   double** f = c->field;
   const bool halo_u = c->pend_u;  // the deferred ring update includes U
@@ -715,18 +857,46 @@ int tea_hip_cg_init(tea_hip_chunk* c, int coefficient, double rx, double ry,
                             InitKBody{f[TEA_FIELD_W], f[TEA_FIELD_KX],
                                       f[TEA_FIELD_KY], rx, ry, g.x, g.y, g.h}));
   c->p_alt_synced = false;
-  return launch_span1(c, K_CG_INIT_OTHERS,
-                      InitOthersBody{f[TEA_FIELD_KX], f[TEA_FIELD_KY],
-                                     f[TEA_FIELD_P], f[TEA_FIELD_R],
-                                     f[TEA_FIELD_U], f[TEA_FIELD_W], g.x},
-                      rro);
+  TEA_RETURN_IF(launch_span1(c, K_CG_INIT_OTHERS,
+                             InitOthersBody{f[TEA_FIELD_KX], f[TEA_FIELD_KY],
+                                            f[TEA_FIELD_P], f[TEA_FIELD_R],
+                                            f[TEA_FIELD_U], f[TEA_FIELD_W], g.x},
+                             rro));
+  // The host's first rro (TeaLeaf/drivers/cg_driver.c:45-52), for the
+  // run-ahead's first alpha.
+  c->ra_rro = *rro;
+  c->ra_rro_known = true;
+  c->ra_fail = 0;
+  return TEA_HIP_OK;
 }
 
 int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
-  TEA_ENTER(c);
+  TEA_ENTER_RA(c);
   TEA_CHECK_ARG(pw != nullptr, "null pw");
   double** f = c->field;
   const Geo g = c->geo;
+  if (c->ra == kRaWR) {
+    // The w-step in flight applied exactly the deferred work: take it over.
+    if (c->pend_u && c->pend_p && c->pend_halo && !c->held && c->pre_halo < 0 &&
+        c->pend_alpha == c->ra_w_alpha && c->pend_beta == c->ra_beta) {
+      swap_ptr(f[TEA_FIELD_P], c->p_alt);
+      swap_ptr(f[TEA_FIELD_U], c->u_alt);
+      swap_ptr(f[TEA_FIELD_W], c->w_alt);
+      c->pend_u = c->pend_p = c->pend_halo = false;
+      c->ra = kRaR;  // its r-step is queued behind it
+      c->ra_commits += 1;
+      c->ra_fail = 0;
+      double sum = 0.0, alpha = 0.0;
+      TEA_RETURN_IF(ra_wait(c, c->ra_set, kRaW, &sum, &alpha));
+      c->ra_alpha = alpha;
+      *pw = sum;
+      return TEA_HIP_OK;
+    }
+    ra_mismatch(c);
+  } else if (c->ra != kRaNone) {
+    ra_mismatch(c);
+  }
+  TEA_FLUSH_HELD(c);
   if (!c->fused) {
     TEA_RETURN_IF(apply_pending(c));
     return launch_span1(c, K_CG_CALC_W,
@@ -739,10 +909,23 @@ int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
   FusedWArgs args{f[TEA_FIELD_P], c->p_alt,        f[TEA_FIELD_R],
                   f[TEA_FIELD_U], f[TEA_FIELD_W],  f[TEA_FIELD_KX],
                   f[TEA_FIELD_KY], c->pend_alpha,  c->pend_beta,
-                  c->pend_u ? 1 : 0, c->pend_halo ? 1 : 0};
+                  c->pend_u ? 1 : 0, c->pend_halo ? 1 : 0, f[TEA_FIELD_U], nullptr};
   c->pend_u = c->pend_p = c->pend_halo = false;
   int plen = c->nspan;
   if (use_pair_blocks(c, 1)) {
+    const bool ahead = ra_possible(c);
+    if (ahead) {
+      // Start a run-ahead: the spare buffers take the cells outside the
+      // interior (and the ring) that no pass writes.
+      TEA_RETURN_IF(ensure_ra(c));
+      const size_t bytes = (size_t)g.n * sizeof(double);
+      TEA_HIP_TRY(hipMemcpyAsync(c->r_alt, f[TEA_FIELD_R], bytes, hipMemcpyDeviceToDevice,
+                                 c->stream));
+      TEA_HIP_TRY(hipMemcpyAsync(c->u_alt, f[TEA_FIELD_U], bytes, hipMemcpyDeviceToDevice,
+                                 c->stream));
+      TEA_HIP_TRY(hipMemcpyAsync(c->w_alt, f[TEA_FIELD_W], bytes, hipMemcpyDeviceToDevice,
+                                 c->stream));
+    }
     {
       Timed t(c, K_CG_FUSED_W);
       if (apply_p)
@@ -753,6 +936,19 @@ int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
                            0, c->stream, g, args, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
+    if (ahead) {
+      const int set = c->ra_set ^ 1;
+      TEA_RETURN_IF(ra_finish(c, set, kRaW, c->ra_rro, false));
+      TEA_RETURN_IF(ra_launch_r(c, set, f[TEA_FIELD_W], c->ra_rro, false));
+      if (apply_p) swap_ptr(f[TEA_FIELD_P], c->p_alt);
+      c->ra = kRaR;
+      c->ra_set = set;
+      double sum = 0.0, alpha = 0.0;
+      TEA_RETURN_IF(ra_wait(c, set, kRaW, &sum, &alpha));
+      c->ra_alpha = alpha;
+      *pw = sum;
+      return TEA_HIP_OK;
+    }
     TEA_RETURN_IF(launch_span_finish(c, 1, &plen, 0, true));
   } else {
     Timed t(c, K_CG_FUSED_W);
@@ -787,9 +983,54 @@ int tea_hip_cg_calc_w(tea_hip_chunk* c, double* pw) {
 }
 
 int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
-  TEA_ENTER(c);
+  TEA_ENTER_RA(c);
   TEA_CHECK_ARG(rrn != nullptr, "null rrn");
   double** f = c->field;
+  if (c->ra == kRaR) {
+    // The r-step in flight used the device's alpha: take it over when the
+    // host's is the same.
+    if (alpha == c->ra_alpha && !c->pend_u && !c->pend_p && !c->pend_halo && !c->held &&
+        c->pre_halo < 0) {
+      swap_ptr(f[TEA_FIELD_R], c->r_alt);
+      c->pend_u = true;
+      c->pend_alpha = alpha;
+      const int set = c->ra_set;
+      c->ra = kRaNone;
+      if (ra_possible(c)) {
+        // Queue the next w-step (its beta from the device) and r-step
+        // before waiting for this r.r.
+        TEA_RETURN_IF(sync_p_alt(c));
+        const int next = set ^ 1;
+        FusedWArgs a{f[TEA_FIELD_P], c->p_alt,  f[TEA_FIELD_R], f[TEA_FIELD_U],
+                     c->w_alt,       f[TEA_FIELD_KX], f[TEA_FIELD_KY], alpha,
+                     0.0,            1,         1,              c->u_alt,
+                     c->ra_dev + 1};
+        {
+          Timed t(c, K_CG_FUSED_W);
+          hipLaunchKernelGGL(fused_w1_kernel<true>, pair_grid(c), dim3(kBlock), 0, c->stream,
+                             c->geo, a, c->sub, c->nblk);
+        }
+        TEA_HIP_TRY(hipGetLastError());
+        TEA_RETURN_IF(ra_finish(c, next, kRaW, 0.0, true));
+        TEA_RETURN_IF(ra_launch_r(c, next, c->w_alt, 0.0, true));
+        c->ra = kRaWR;
+        c->ra_set = next;
+        c->ra_w_alpha = alpha;
+      }
+      double sum = 0.0, beta = 0.0;
+      TEA_RETURN_IF(ra_wait(c, set, kRaRStep, &sum, &beta));
+      c->ra_beta = beta;
+      c->ra_rro = sum;
+      c->ra_commits += 1;
+      c->ra_fail = 0;
+      *rrn = sum;
+      return TEA_HIP_OK;
+    }
+    ra_mismatch(c);
+  } else if (c->ra != kRaNone) {
+    ra_mismatch(c);
+  }
+  TEA_FLUSH_HELD(c);
   TEA_RETURN_IF(apply_pending(c));
   if (!c->fused)
     return launch_span1(c, K_CG_CALC_UR,
@@ -801,8 +1042,8 @@ int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
     {
       Timed t(c, K_CG_FUSED_R);
       hipLaunchKernelGGL(fused_r1_kernel, pair_grid(c), dim3(kBlock), 0,
-                         c->stream, c->geo, f[TEA_FIELD_R], f[TEA_FIELD_W],
-                         alpha, c->sub, c->nblk);
+                         c->stream, c->geo, f[TEA_FIELD_R], f[TEA_FIELD_R], f[TEA_FIELD_W],
+                         alpha, nullptr, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
     TEA_RETURN_IF(launch_span_finish(c, 1, &rlen, 0, true));
@@ -821,11 +1062,14 @@ int tea_hip_cg_calc_ur(tea_hip_chunk* c, double alpha, double* rrn) {
   TEA_RETURN_IF(finish_reduction_n(c, 1, rlen, rrn));
   c->pend_u = true;
   c->pend_alpha = alpha;
+  c->ra_rro = *rrn;  // the host's next rro (TeaLeaf/drivers/cg_driver.c:113)
   return TEA_HIP_OK;
 }
 
 int tea_hip_cg_calc_p(tea_hip_chunk* c, double beta) {
-  TEA_ENTER(c);
+  TEA_ENTER_RA(c);
+  // Only deferred: a run-ahead w-step stays in flight for cg_calc_w.
+  if (c->held || c->pre_halo >= 0) TEA_FLUSH_HELD(c);
   double** f = c->field;
   if (!c->fused || c->pend_p || c->pend_halo) TEA_RETURN_IF(apply_pending(c));
   if (!c->fused) {

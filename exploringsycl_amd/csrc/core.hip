@@ -31,7 +31,8 @@ const char* const kKernelNames[K_NUM] = {
     "span_finish",    "jacobi_init",     "jacobi_copy_u", "jacobi_iterate",
     "jacobi_fused",   "cheby_init",      "cheby_iterate", "cheby_calc_u",
     "cheby_fused",    "ppcg_init",       "ppcg_calc_ur",  "ppcg_calc_sd",
-    "ppcg_fused",     "ppcg_two_step",   "cheby_two_step", "fcg_sweeps"};
+    "ppcg_fused",     "ppcg_two_step",   "cheby_two_step", "fcg_sweeps",
+    "cg_ra_finish"};
 
 Timed::Timed(tea_hip_chunk* c_, int kid_, hipStream_t s)
     : c(c_), kid(kid_), stream(s ? s : c_->stream) {
@@ -98,21 +99,15 @@ int resolve_ready(tea_hip_chunk* c) {
 // can equal it; the wait below would still end when the stream goes idle.
 constexpr uint64_t kPollSentinel = 0x7ff4c0ffee5e4d1dull;
 
-// The host's view of the partials is coherent, mapped memory: the finisher's
-// stores reach it while the kernel is still retiring, so the host can form
-// the next scalar and launch the next kernel before the stream reports
-// completion.
-static int wait_partials(tea_hip_chunk* c) {
-  if (c->poll_len == 0) return sync_chunk(c);
-  const volatile uint64_t* v =
-      reinterpret_cast<const volatile uint64_t*>(c->host_partials);
-  const int len = c->poll_len, nr = c->poll_nr;
-  const size_t stride = (size_t)c->nspan;
-  c->poll_len = 0;
-  // The stream is queried only after about 2^22 spins (milliseconds, longer
-  // than a pass over an 8192^2 chunk) and then every 2^16 spins, a few
-  // queries per pass even over the largest chunks: a query every few
-  // hundred spins slowed the kernels it overlapped.
+// Spins until none of the nr x len values at v (arrays stride apart) holds
+// the sentinel.  The stream is queried only after about 2^22 spins
+// (milliseconds, longer than a pass over an 8192^2 chunk) and then every
+// 2^16 spins, a few queries per pass even over the largest chunks: a query
+// every few hundred spins slowed the kernels it overlapped.  An idle stream
+// with a sentinel left means a kernel wrote fewer values than were waited
+// for: an error naming the value.
+static int wait_values(tea_hip_chunk* c, const volatile uint64_t* v, size_t stride, int nr,
+                       int len) {
   int q = 0, i = 0;
   for (uint64_t spins = 1;; ++spins) {
     while (q < nr) {
@@ -125,8 +120,6 @@ static int wait_partials(tea_hip_chunk* c) {
     if (spins >= (1ull << 22) && (spins & 0xffff) == 0) {
       const hipError_t e = hipStreamQuery(c->stream);
       if (e == hipSuccess) {
-        // The stream is idle, so every partial has been written: a sentinel
-        // left means a finisher wrote fewer partials than were waited for.
         std::atomic_thread_fence(std::memory_order_acquire);
         for (int qq = 0; qq < nr; ++qq)
           for (int ii = 0; ii < len; ++ii)
@@ -142,6 +135,27 @@ static int wait_partials(tea_hip_chunk* c) {
   }
   std::atomic_thread_fence(std::memory_order_acquire);
   return resolve_ready(c);
+}
+
+// The host's view of the partials is coherent, mapped memory: the finisher's
+// stores reach it while the kernel is still retiring, so the host can form
+// the next scalar and launch the next kernel before the stream reports
+// completion.
+static int wait_partials(tea_hip_chunk* c) {
+  if (c->poll_len == 0) return sync_chunk(c);
+  const int len = c->poll_len, nr = c->poll_nr;
+  c->poll_len = 0;
+  return wait_values(c, reinterpret_cast<const volatile uint64_t*>(c->host_partials),
+                     (size_t)c->nspan, nr, len);
+}
+
+void arm_host_values(double* host, int count) {
+  uint64_t* v = reinterpret_cast<uint64_t*>(host);
+  for (int i = 0; i < count; ++i) v[i] = kPollSentinel;
+}
+
+int wait_host_values(tea_hip_chunk* c, const double* host, int count) {
+  return wait_values(c, reinterpret_cast<const volatile uint64_t*>(host), 0, 1, count);
 }
 
 // Final passes of the reference's reduceArray (TeaLeaf/c_kernels/sycl/
@@ -180,9 +194,27 @@ static double host_tree_passes(double* v, size_t len) {
 // and its partial is the sum.
 constexpr int kFinishBlock = kSubPerSpan * kWave;  // 512 threads = 64 spans
 
-template <int NR>
+// The host's final tree passes (host_tree_passes) over len <= 64^2 values
+// in one wave; buf is LDS of 64 doubles.  Lane 0 returns the sum.
+__device__ inline double wave_tree_passes(const double* part, int len, double* buf) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if (len == 1) return part[0];
+  const int groups = (len + kWave - 1) / kWave;
+  for (int g = 0; g < groups; ++g) {
+    const int i = g * kWave + lane;
+    const double v = wave_tree(i < len ? part[i] : 0.0);
+    if (lane == 0) buf[g] = v;
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  if (groups == 1) return buf[0];
+  return wave_tree(lane < groups ? buf[lane] : 0.0);
+}
+
+template <int NR, bool TAIL = false>
 __global__ __launch_bounds__(kFinishBlock) void span_finish_kernel(
-    const double* __restrict__ sub, int nblk, double* partials, int nspan) {
+    const double* __restrict__ sub, int nblk, double* partials, int nspan,
+    RaTail tail = RaTail{}) {
   __shared__ double lds[NR][kWave];
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1);
@@ -212,6 +244,33 @@ __global__ __launch_bounds__(kFinishBlock) void span_finish_kernel(
     const double g3 = wave_tree(lds[q][lane]);
     if (lane == 0) partials[q * nspan + blockIdx.x] = g3;
   }
+  if (!TAIL) return;
+  // The last block to store its partial (device-scope release before the
+  // count, acquire after it) sums them all.
+  unsigned prev = 0;
+  if (lane == 0) {
+    __threadfence();
+    prev = atomicAdd(tail.count, 1u);
+  }
+  prev = __shfl(prev, 0);
+  if (prev != gridDim.x - 1) return;
+  __threadfence();
+  const int len = nspan == 1 ? 1 : (int)gridDim.x;
+  const double sum = wave_tree_passes(partials, len, &lds[0][0]);
+  if (lane != 0) return;
+  const double r0 = tail.from_dev ? tail.ra[2] : tail.rro;
+  double q;
+  if (tail.mode == 0) {
+    q = r0 / sum;
+    tail.ra[0] = q;
+  } else {
+    q = sum / r0;
+    tail.ra[1] = q;
+    tail.ra[2] = sum;
+  }
+  tail.out[0] = sum;
+  tail.out[1] = q;
+  *tail.count = 0u;
 }
 
 int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot,
@@ -240,6 +299,18 @@ int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot,
   }
   TEA_HIP_TRY(hipGetLastError());
   *len = c->nspan == 1 ? 1 : (c->nspan + kWave - 1) / kWave;
+  return TEA_HIP_OK;
+}
+
+int launch_span_finish_ra(tea_hip_chunk* c, double* dst, const RaTail& tail) {
+  const int grid = (c->nspan + kWave - 1) / kWave;
+  TEA_CHECK_ARG(grid <= kWave * kWave, "span_finish: too many partials for the run-ahead");
+  {
+    Timed t(c, K_CG_RA_FINISH);
+    hipLaunchKernelGGL((span_finish_kernel<1, true>), dim3((unsigned)grid), dim3(kFinishBlock),
+                       0, c->stream, c->sub, c->nblk, dst, c->nspan, tail);
+  }
+  TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
 }
 
@@ -324,6 +395,7 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
     }
   }
   if (const char* pv = getenv("TEA_HIP_POLL")) c->poll = atoi(pv) != 0;
+  if (const char* rv = getenv("TEA_HIP_RUN_AHEAD")) c->ra_enabled = atoi(rv) != 0;
   if (const char* sp = getenv("TEA_HIP_SPIN"))
     if (atoi(sp) != 0) (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
 
@@ -407,6 +479,9 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   for (int f = 0; f < TEA_NUM_FIELDS; ++f)
     if (c->field[f]) (void)hipFree(c->field[f]);
   if (c->p_alt) (void)hipFree(c->p_alt);
+  for (double* b : {c->r_alt, c->u_alt, c->w_alt, c->ra_dev})
+    if (b) (void)hipFree(b);
+  if (c->ra_host) (void)hipHostFree(c->ra_host);
   if (c->spare) (void)hipFree(c->spare);
   if (c->spare2) (void)hipFree(c->spare2);
   if (c->sub) (void)hipFree(c->sub);

@@ -46,14 +46,15 @@ double* field_ptr(tea_hip_chunk* h, int field) {
   return p;
 }
 
-// p, u, r and sd are double-buffered by the fused iterations (CG, Jacobi,
-// Chebyshev, PPCG): republish their addresses after every call that can
-// swap them.
+// p, u, r, w and sd are double-buffered by the fused iterations (CG and
+// its run-ahead, Jacobi, Chebyshev, PPCG): republish their addresses after
+// every call that can swap them.
 void refresh_fields(Chunk* chunk) {
   tea_hip_chunk* h = H(chunk);
   chunk->p = field_ptr(h, TEA_FIELD_P);
   chunk->u = field_ptr(h, TEA_FIELD_U);
   chunk->r = field_ptr(h, TEA_FIELD_R);
+  chunk->w = field_ptr(h, TEA_FIELD_W);
   chunk->sd = field_ptr(h, TEA_FIELD_SD);
 }
 

@@ -559,6 +559,7 @@ enum KernelId {
   K_PPCG_TWO_STEP,
   K_CHEBY_TWO_STEP,
   K_FCG_SWEEPS,
+  K_CG_RA_FINISH,
   K_NUM
 };
 
@@ -728,6 +729,30 @@ struct tea_hip_chunk {
   hipStream_t fcg_side = nullptr;
   hipEvent_t fcg_fork = nullptr, fcg_join = nullptr;
 
+  // Run-ahead of the fused CG iteration (cg.hip): the device forms alpha and
+  // beta from its own partials and runs the next pass of the iteration into
+  // spare buffers before the host asks for it; the host's call commits it
+  // when its scalar is bitwise the device's.
+  int ra = 0;                 // kRaNone, kRaR (r-step in flight), kRaWR
+  bool ra_enabled = true;     // TEA_HIP_RUN_AHEAD=0 turns it off
+  bool ra_dirty = false;      // a discarded run-ahead may still be writing ra_host
+  int ra_fail = 0;            // consecutive mismatches (3 turn it off)
+  int ra_set = 0;             // host slot set of the step in flight
+  long ra_commits = 0;        // run-ahead passes taken over (statistics)
+  double ra_rro = 0.0;        // r.r the host was last given (its rro)
+  bool ra_rro_known = false;
+  double ra_alpha = 0.0;      // alpha of the r-step in flight (the device's)
+  double ra_beta = 0.0;       // beta of the w-step in flight (the device's)
+  double ra_w_alpha = 0.0;    // alpha whose u update the w-step in flight applies
+  double* r_alt = nullptr;    // spare r, u and w (x*y each)
+  double* u_alt = nullptr;
+  double* w_alt = nullptr;
+  double* ra_dev = nullptr;   // device: [0] alpha, [1] beta, [2] r.r, then the
+                              // finisher's partials and its blocks-done count
+  unsigned* ra_count = nullptr;
+  double* ra_host = nullptr;  // pinned: 2 sets x {w-step, r-step} x {sum, ratio}
+  double* ra_host_dev = nullptr;  // device view of ra_host
+
   // Waiting for a reduction by its values: launch_span_finish(..., poll)
   // fills the first poll_len partials of arrays 0 .. poll_nr-1 with a
   // sentinel, and the finish waits until the finisher has overwritten them
@@ -790,12 +815,36 @@ int finish_reduction(tea_hip_chunk* c, int nr, double* out);
 // poll (slot 0), the next finish waits for the values themselves instead of
 // the stream; only for a finish that follows with no other kernel writing
 // the partials in between.
+// The run-ahead's scalar, formed by the last block of the finisher once
+// every block's partial is in: the host's final tree passes (zero-padded
+// groups of 64, each by the reference's pairwise tree) over the grid's
+// partials give the sum; mode 0 (after a w-step) alpha = rro / sum into
+// ra[0]; mode 1 (after an r-step) beta = sum / rro into ra[1] and sum into
+// ra[2].  rro is ra[2] when from_dev, else the value.  The sum and the
+// scalar also go to out[0], out[1] (pinned host memory the host polls).
+// count is the blocks-done counter, zero between launches.
+struct RaTail {
+  double rro;
+  int from_dev, mode;
+  double* ra;
+  double* out;
+  unsigned* count;
+};
+// The finisher of one reduction (array 0 of sub) into dst, device memory
+// (ceil(nspan / 64) partials, 1 with a single span), with the run-ahead's
+// scalar (at most 64^2 partials); not waited for.
+int launch_span_finish_ra(tea_hip_chunk* c, double* dst, const RaTail& tail);
 int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot = 0,
                        bool poll = false);
 
 // finish_reduction over the first len partials of each array (len is what
 // launch_span_finish leaves).
 int finish_reduction_n(tea_hip_chunk* c, int nr, int len, double* out);
+// Values in pinned, mapped host memory that a kernel of the stream writes:
+// arm them with the wait sentinel before the launch, then wait until the
+// kernel has overwritten all count of them.
+void arm_host_values(double* host, int count);
+int wait_host_values(tea_hip_chunk* c, const double* host, int count);
 
 // The same with a length per array.
 int finish_reduction_lens(tea_hip_chunk* c, int nr, const int* lens, double* out);
@@ -838,15 +887,27 @@ int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw);
 int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
                  const int fields[6]);
 
-// Checks the handle and makes its device current (entry points).
-#define TEA_ENTER_HELD(c)                                 \
+// Drops a run-ahead of the fused CG iteration (cg.hip): it wrote only
+// spare buffers and device scalars, so nothing is undone.
+int ra_discard(tea_hip_chunk* c);
+
+// Checks the handle and makes its device current, leaving a run-ahead in
+// flight (the CG entry points that can take it over).
+#define TEA_ENTER_RA(c)                                   \
   TEA_CHECK_ARG((c) != nullptr, "null chunk handle");     \
   TEA_HIP_TRY(hipSetDevice((c)->device))
 
+// Checks the handle and makes its device current (entry points); drops a
+// run-ahead of the CG iteration.
+#define TEA_ENTER_HELD(c)                                 \
+  TEA_ENTER_RA(c);                                        \
+  if ((c)->ra) TEA_RETURN_IF(::tea::ra_discard(c))
+
 // Entry points other than the PPCG inner step and the halo update first run
-// a held-back halo update and PPCG or Chebyshev step.
+// a held-back halo update and PPCG or Chebyshev step (and drop a run-ahead).
 #define TEA_FLUSH_HELD(c)                                                  \
   do {                                                                     \
+    if ((c)->ra) TEA_RETURN_IF(::tea::ra_discard(c));                      \
     if ((c)->held || (c)->pre_halo >= 0) TEA_RETURN_IF(::tea::flush_held(c)); \
   } while (0)
 

@@ -19,7 +19,30 @@
 #define TEA_HAVE_EXT 1
 #endif
 
-static void free_coarse(struct OracleCoarse* q);
+// The two-level coarse correction of flexible CG (below).
+// Level l coarsens the interior by 8 * 4^l; a level is used while the
+// interior divides.  Sizes and buffers of each level.
+struct OracleCoarse {
+  int levels = 0, sweeps = 0;
+  int c[2] = {0, 0}, hc[2] = {0, 0}, wc[2] = {0, 0}, ntiles[2] = {0, 0};
+  float* kc[2] = {nullptr, nullptr};
+  float* rbar[2] = {nullptr, nullptr};
+  float* y[2] = {nullptr, nullptr};
+  double* part[2] = {nullptr, nullptr};
+  double* cross = nullptr;
+};
+
+static void free_coarse(OracleCoarse* q) {
+  if (!q) return;
+  for (int l = 0; l < 2; ++l) {
+    free(q->kc[l]);
+    free(q->rbar[l]);
+    free(q->y[l]);
+    free(q->part[l]);
+  }
+  free(q->cross);
+  delete q;
+}
 
 static double* alloc_field(size_t n) {
   double* p = (double*)calloc(n ? n : 1, sizeof(double));
@@ -273,29 +296,6 @@ void run_mpcg_refresh(Chunk* c, Settings* s, double alpha, int apply_u,
 // ---- flexible CG: Jacobi, or Jacobi plus the two-level coarse correction
 // with Jacobi sweeps as the coarse solve (no network on the CPU) -----------
 
-// Level l coarsens the interior by 8 * 4^l; a level is used while the
-// interior divides.  Sizes and buffers of each level.
-struct OracleCoarse {
-  int levels = 0, sweeps = 0;
-  int c[2] = {0, 0}, hc[2] = {0, 0}, wc[2] = {0, 0}, ntiles[2] = {0, 0};
-  float* kc[2] = {nullptr, nullptr};
-  float* rbar[2] = {nullptr, nullptr};
-  float* y[2] = {nullptr, nullptr};
-  double* part[2] = {nullptr, nullptr};
-  double* cross = nullptr;
-};
-
-static void free_coarse(OracleCoarse* q) {
-  if (!q) return;
-  for (int l = 0; l < 2; ++l) {
-    free(q->kc[l]);
-    free(q->rbar[l]);
-    free(q->y[l]);
-    free(q->part[l]);
-  }
-  free(q->cross);
-  delete q;
-}
 
 void run_fcg_attach_sweeps(Chunk* c, Settings* s, int sweeps, int coarsen,
                            int levels) {
