@@ -1029,28 +1029,37 @@ __global__ __launch_bounds__(kGammaThreads) void joint_gamma_kernel(
   constexpr int kT = kGammaThreads;
   __shared__ double sh[5][kT];
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  // Each thread's elements are loaded 16 at a time, then added in turn (the
-  // order above), so that the loads overlap.
-  constexpr int kB = 16;
-  auto strided = [&](const double* a, const float* f, int n, double& acc) {
-    for (int k0 = threadIdx.x; k0 < n; k0 += kT * kB) {
-      double t[kB];
+  // Each thread's elements are added in turn (the order above).  All five
+  // sums' loads of a round are issued before any is added: a round per sum
+  // (loads waited for sum by sum) took 21 us at 4096^2.  kB elements of
+  // each sum per round.
+  constexpr int kB = 4;
+  const int n2 = levels == 2 ? n1 : 0, nxx = levels == 2 ? nx : 0;
+  const int nmax = max(max(n0, n2), nxx);
+  for (int k0 = threadIdx.x; k0 < nmax; k0 += kT * kB) {
+    double t[5][kB];
 #pragma unroll
-      for (int j = 0; j < kB; ++j) {
-        const int k = k0 + kT * j;
-        t[j] = k < n ? (f ? a[k] * (double)f[k] : a[k]) : 0.0;
-      }
-#pragma unroll
-      for (int j = 0; j < kB; ++j)
-        if (k0 + kT * j < n) acc += t[j];
+    for (int j = 0; j < kB; ++j) {
+      const int k = k0 + kT * j;
+      t[0][j] = k < n0 ? p0[k] : 0.0;
+      t[1][j] = k < n0 ? p0[n0 + k] : 0.0;
+      t[2][j] = k < n2 ? p1[k] : 0.0;
+      t[3][j] = k < n2 ? p1[n1 + k] : 0.0;
+      t[4][j] = k < nxx ? xsum[k] * (double)y1[k] : 0.0;
     }
-  };
-  strided(p0, nullptr, n0, v[0]);
-  strided(p0 + n0, nullptr, n0, v[1]);
-  if (levels == 2) {
-    strided(p1, nullptr, n1, v[2]);
-    strided(p1 + n1, nullptr, n1, v[3]);
-    strided(xsum, y1, nx, v[4]);
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      const int k = k0 + kT * j;
+      if (k < n0) {
+        v[0] += t[0][j];
+        v[1] += t[1][j];
+      }
+      if (k < n2) {
+        v[2] += t[2][j];
+        v[3] += t[3][j];
+      }
+      if (k < nxx) v[4] += t[4][j];
+    }
   }
   for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
   __syncthreads();
@@ -1202,7 +1211,13 @@ static int launch_coarse(tea_hip_chunk* c) {
   auto level = [&](int l, hipStream_t st) {
     return net ? launch_net_level(c, l, st) : launch_sweeps_level(c, l, st);
   };
-  if (nl == 2) {
+  // The network's two level chains overlap on two streams; the sweeps'
+  // single kernels are shorter than the fork and join between the streams
+  // (about 17 us at 4096^2), so they run one after the other.
+  if (nl == 2 && !net) {
+    TEA_RETURN_IF(level(0, c->stream));
+    TEA_RETURN_IF(level(1, c->stream));
+  } else if (nl == 2) {
     TEA_HIP_TRY(hipEventRecord(c->fcg_fork, c->stream));
     TEA_HIP_TRY(hipStreamWaitEvent(c->fcg_side, c->fcg_fork, 0));
     TEA_RETURN_IF(level(1, c->fcg_side));
