@@ -159,3 +159,21 @@ def config_deck(config_id, x_cells=None, end_step=None, problem_id=0):
 
 def float_close(a, b, rel):
     return math.isclose(a, b, rel_tol=rel, abs_tol=0.0)
+
+
+# Decks of the time-to-solution field for one solver: 64 seeded rectangles,
+# log10 density in [-2, 2], K = 1/density, one timestep to 1e-10.
+SOLVER_KEYS = {
+    "cg": [],
+    "mpcg": ["use_mpcg"],
+    "jpcg": ["use_jacobi_pcg"],
+    "fcg": ["use_fcg", "fcg_weights=net.bin"],
+    "fcgs": ["use_fcg", "fcg_coarse_sweeps=6"],
+}
+
+
+def tts_deck(solver, cells=4096, max_iters=12000, seed_id=3000):
+    return Deck.with_random_rectangles(
+        cells, cells, 64, problem_seed(5, seed_id), density="log10",
+        density_range=(-2.0, 2.0), end_step=1, coefficient="inverse",
+        max_iters=max_iters, extra=["rel_eps=1e-10"] + SOLVER_KEYS[solver])

@@ -5,6 +5,10 @@
       --kernel-trace --stats run (DIR/*_kernel_stats.csv, summed over the
       processes of the run).
 
+  python -m exploringsycl_amd.rocprof_summary counters DIR [DIR ...] [--kernel REGEX]
+      mean of every counter per dispatch, per kernel and grid size, from
+      --pmc runs (DIR/*counter_collection.csv).
+
   python -m exploringsycl_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR \
       --kernel REGEX --copy-kernel REGEX --copy-bytes B --algorithmic-bytes A
       HBM bytes per launch of the kernel from a FETCH_SIZE run and a
@@ -73,6 +77,22 @@ def counter_means(d, counter):
     return {k: sum(v) / len(v) for k, v in per.items()}
 
 
+def all_counter_means(d):
+    """{(kernel, grid size): {counter: mean value per dispatch}}."""
+    acc = {}
+    for path in _all(os.path.join(d, "*counter_collection.csv")):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                key = (short(r["Kernel_Name"]), r["Grid_Size"], r["Counter_Name"])
+                acc.setdefault(key, {})
+                disp = (path, r["Dispatch_Id"])
+                acc[key][disp] = acc[key].get(disp, 0.0) + float(r["Counter_Value"])
+    out = {}
+    for (name, grid, counter), per in acc.items():
+        out.setdefault((name, grid), {})[counter] = sum(per.values()) / len(per)
+    return out
+
+
 def _match(table, regex):
     hits = [k for k in table if re.search(regex, k)]
     if len(hits) != 1:
@@ -87,6 +107,9 @@ def main(argv=None):
     s = sub.add_parser("stats")
     s.add_argument("dir")
     s.add_argument("--top", type=int, default=12)
+    cc = sub.add_parser("counters")
+    cc.add_argument("dirs", nargs="+")
+    cc.add_argument("--kernel", default=".")
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -101,6 +124,15 @@ def main(argv=None):
         print(f"{'kernel':64s} {'calls':>7s} {'avg us':>10s} {'total ms':>10s} {'%':>6s}")
         for name, calls, avg, tot, pct in kernel_stats(a.dir)[:a.top]:
             print(f"{short(name)[:64]:64s} {calls:7d} {avg:10.1f} {tot:10.2f} {pct:6.2f}")
+        return 0
+
+    if a.cmd == "counters":
+        for d in a.dirs:
+            print(f"== {d}")
+            for (name, grid), cs in sorted(all_counter_means(d).items()):
+                if re.search(a.kernel, name):
+                    vals = " ".join(f"{k}={v:.4g}" for k, v in sorted(cs.items()))
+                    print(f"  {name[:48]:48s} grid={grid:>9s} {vals}")
         return 0
 
     fetch = counter_means(a.fetch_dir, "FETCH_SIZE")
