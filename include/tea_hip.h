@@ -93,10 +93,11 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
 int tea_hip_chunk_destroy(tea_hip_chunk* chunk);
 
 /* Device address and element count of a field.  The addresses of
- * TEA_FIELD_P, TEA_FIELD_U, TEA_FIELD_R and TEA_FIELD_SD can change after
- * solver calls (the fused CG, Jacobi, Chebyshev and PPCG iterations write
- * the new p, u or sd into a second buffer and exchange the two); ask again
- * after each call instead of caching them.  The address is returned as
+ * TEA_FIELD_P, TEA_FIELD_U, TEA_FIELD_R, TEA_FIELD_W and TEA_FIELD_SD can
+ * change after solver calls (the fused CG, Jacobi, Chebyshev and PPCG
+ * iterations and the CG run-ahead write the new p, u, r, w or sd into a
+ * second buffer and exchange the two); ask again after each call instead
+ * of caching them.  The address is returned as
  * is: deferred updates (see tea_hip_set_fused and tea_hip_local_halos) may
  * not have reached the field yet; call tea_hip_flush before reading or
  * writing through it. */
@@ -202,7 +203,15 @@ int tea_hip_cg_init(tea_hip_chunk* chunk, int coefficient, double rx,
                     double ry, double* rro);
 
 /* w = A p on the interior and *pw = p.w.  Replaces run_cg_calc_w
- * (TeaLeaf/kernel_interface.h:37-38; cg.cpp:137-195). */
+ * (TeaLeaf/kernel_interface.h:37-38; cg.cpp:137-195).
+ * Run-ahead (fused mode, x even, all reductions polled): after this call
+ * the device forms alpha = rro / pw itself (rro: the value the last
+ * tea_hip_cg_init or tea_hip_cg_calc_ur returned) and already runs
+ * r - alpha w into a spare buffer; after tea_hip_cg_calc_ur it forms beta
+ * and runs the next w = A p likewise.  The next call takes that work over
+ * when its alpha (and, for tea_hip_cg_calc_w, the deferred beta, u update
+ * and U/P halo update) is bitwise what the device used; any other call
+ * drops it.  Results are bitwise those without it (TEA_HIP_RUN_AHEAD=0). */
 int tea_hip_cg_calc_w(tea_hip_chunk* chunk, double* pw);
 
 /* u += alpha p, r -= alpha w, *rrn = r.r.  Replaces run_cg_calc_ur
