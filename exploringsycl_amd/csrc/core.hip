@@ -293,8 +293,11 @@ int launch_span_finish(tea_hip_chunk* c, int nr, int* len, int slot,
     if (nr == 1)
       hipLaunchKernelGGL(span_finish_kernel<1>, grid, dim3(kFinishBlock), 0,
                          c->stream, c->sub, c->nblk, dst, c->nspan);
-    else
+    else if (nr == 2)
       hipLaunchKernelGGL(span_finish_kernel<2>, grid, dim3(kFinishBlock), 0,
+                         c->stream, c->sub, c->nblk, dst, c->nspan);
+    else
+      hipLaunchKernelGGL(span_finish_kernel<3>, grid, dim3(kFinishBlock), 0,
                          c->stream, c->sub, c->nblk, dst, c->nspan);
   }
   TEA_HIP_TRY(hipGetLastError());

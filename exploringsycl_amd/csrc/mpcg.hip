@@ -639,6 +639,55 @@ __global__ __launch_bounds__(kQuadThreads) void mp_r4_kernel(
   wave_subtree_quads<RW ? 2 : 1>(v, sub, nblk, blk);
 }
 
+// MpTrueResidualBody's arithmetic (below) with one cell pair per thread (x
+// even): r = u0 - A u in FP64 with the FP64 coefficients, r rounded to FP32
+// into r32, and the three sums (r32.r32, r.r, r32.w) as wave sums for
+// span_finish_kernel.  u's left and right neighbours come from the adjacent
+// lanes by DPP; lane 0 loads u at e - 1, lane 63 u and kx at e + 2.
+__global__ __launch_bounds__(kBlock) void mp_residual1_kernel(
+    Geo g, const double* __restrict__ u, const double* __restrict__ u0,
+    const double* __restrict__ kx, const double* __restrict__ ky, float* __restrict__ r32,
+    const float* __restrict__ w, double* sub, int nblk) {
+  const int blk = blockIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int x = g.x;
+  const int e = blk * kPairBlock + 2 * threadIdx.x;
+  const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
+  const int kk = e - row * x;
+  const bool in0 = e < g.n && interior(g, kk, row);
+  const bool in1 = e < g.n && interior(g, kk + 1, row);
+  const bool any = in0 || in1;
+  const int ic = e < g.n ? e : x + 2;
+  const int ir = any ? e : x + 2;
+  const double2 U = ld2(u, ic), Ut = ld2(u, ir + x), Ub = ld2(u, ir - x);
+  const double2 KX = ld2(kx, ic), KY = ld2(ky, ir), KYt = ld2(ky, ir + x);
+  const double2 B = ld2_nt(u0, ir);
+  const float2 W = ld2(w, ir);
+  double UN = 0.0, KXN = 0.0;
+  if (lane == 0 || lane == kWave - 1) {
+    const int ni = lane == 0 ? ir - 1 : ir + 2;
+    UN = u[ni];
+    KXN = kx[ni];
+  }
+  double ul = from_prev_lane(U.y), ur = from_next_lane(U.x), kxr = from_next_lane(KX.x);
+  if (lane == 0) ul = UN;
+  if (lane == kWave - 1) {
+    ur = UN;
+    kxr = KXN;
+  }
+  const double diag_a = (1.0 + (KX.y + KX.x)) + (KYt.x + KY.x);
+  const double diag_b = (1.0 + (kxr + KX.y)) + (KYt.y + KY.y);
+  const double aua = (diag_a * U.x - (KX.y * U.y + KX.x * ul)) - (KYt.x * Ut.x + KY.x * Ub.x);
+  const double aub = (diag_b * U.y - (kxr * ur + KX.y * U.x)) - (KYt.y * Ut.y + KY.y * Ub.y);
+  const double ra = B.x - aua, rb = B.y - aub;
+  const float fa = (float)ra, fb = (float)rb;
+  double v[3] = {(in0 ? (double)fa * (double)fa : 0.0) + (in1 ? (double)fb * (double)fb : 0.0),
+                 (in0 ? ra * ra : 0.0) + (in1 ? rb * rb : 0.0),
+                 (in0 ? (double)fa * (double)W.x : 0.0) + (in1 ? (double)fb * (double)W.y : 0.0)};
+  if (any) st2(r32, e, in0, in1, fa, fb);
+  wave_subtree_pairs<3>(v, sub, nblk, blk);
+}
+
 namespace {
 
 // FP32 mirrors after the FP64 initialisation: coefficients and p on every
@@ -908,17 +957,30 @@ int tea_hip_mpcg_refresh(tea_hip_chunk* c, double alpha, int apply_u,
     hipLaunchKernelGGL((each_kernel<MpApplyUBody>), flat_grid(g.n),
                        dim3(kBlock), 0, c->stream, g, body);
   }
-  {
-    Timed t(c, K_MP_REFRESH);
-    MpTrueResidualBody body{f[TEA_FIELD_U], f[TEA_FIELD_U0], f[TEA_FIELD_KX],
-                            f[TEA_FIELD_KY], c->mp_r, c->mp_w, g.x};
-    hipLaunchKernelGGL((span_kernel<3, MpTrueResidualBody>), span_grid(c),
-                       dim3(kBlock), 0, c->stream, g, body, c->partials,
-                       c->nspan, c->ngroup == 1 ? 1 : 0);
-  }
-  TEA_HIP_TRY(hipGetLastError());
   double out[3];
-  TEA_RETURN_IF(finish_reduction(c, 3, out));
+  if (use_pair_blocks(c, 2)) {
+    {
+      Timed t(c, K_MP_REFRESH);
+      hipLaunchKernelGGL(mp_residual1_kernel, pair_grid(c), dim3(kBlock), 0, c->stream, g,
+                         f[TEA_FIELD_U], f[TEA_FIELD_U0], f[TEA_FIELD_KX], f[TEA_FIELD_KY],
+                         c->mp_r, c->mp_w, c->sub, c->nblk);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    int len = c->nspan;
+    TEA_RETURN_IF(launch_span_finish(c, 3, &len, 0, true));
+    TEA_RETURN_IF(finish_reduction_n(c, 3, len, out));
+  } else {
+    {
+      Timed t(c, K_MP_REFRESH);
+      MpTrueResidualBody body{f[TEA_FIELD_U], f[TEA_FIELD_U0], f[TEA_FIELD_KX],
+                              f[TEA_FIELD_KY], c->mp_r, c->mp_w, g.x};
+      hipLaunchKernelGGL((span_kernel<3, MpTrueResidualBody>), span_grid(c),
+                         dim3(kBlock), 0, c->stream, g, body, c->partials,
+                         c->nspan, c->ngroup == 1 ? 1 : 0);
+    }
+    TEA_HIP_TRY(hipGetLastError());
+    TEA_RETURN_IF(finish_reduction(c, 3, out));
+  }
   *rr = out[0];
   *rr_true = out[1];
   *rw = out[2];

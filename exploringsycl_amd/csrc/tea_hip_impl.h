@@ -329,7 +329,7 @@ __device__ inline void st2_nt(float* p, int i, bool in0, bool in1, float v0,
 
 constexpr int kPairBlock = 2 * kBlock;              // 512 cells
 constexpr int kSubPerSpan = kSpan / kPairBlock;     // 8
-constexpr int kSubMaxNr = 2;                        // dot products per launch
+constexpr int kSubMaxNr = 3;                        // dot products per launch
 
 // From each lane's in-lane pair sum (tree step 1): steps 2..32 within each
 // 32-lane half (one group each), then the wave's two groups; lane 0 stores
