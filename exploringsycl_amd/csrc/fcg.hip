@@ -78,8 +78,20 @@ __global__ __launch_bounds__(256) void box_rows_kernel(int hc, int wc,
                                                        float* __restrict__ scale) {
   if (blockIdx.x == gridDim.x - 1) {
     __shared__ double sh[256];
+    // Thread t adds partials t, t + 256, ... in turn, loaded 8 at a time
+    // (one load per add waited for each in turn).
     double v = 0.0;
-    for (int i = threadIdx.x; i < npart; i += 256) v += part[i];
+    for (int i0 = threadIdx.x; i0 < npart; i0 += 256 * 8) {
+      double t8[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int i = i0 + 256 * j;
+        t8[j] = i < npart ? part[i] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (i0 + 256 * j < npart) v += t8[j];
+    }
     sh[threadIdx.x] = v;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {

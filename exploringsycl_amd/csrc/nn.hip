@@ -153,6 +153,34 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
 // 16-byte LDS read, which take consecutive pixels, hit distinct banks),
 // and each wave computes one 32-pixel tile from LDS.  Same arithmetic and
 // output as conv_mfma_kernel<CIN, 0>.  Grid (W / 128, H, B).
+// Stages NROWS padded input rows (from row y0) of a 130-pixel segment
+// (from padded column x0) into LDS, CPC 16-byte chunks per pixel at a pitch
+// of kPitch chunks: every thread issues all its loads before its first LDS
+// store (a load-store loop waited for each load in turn: 13 global-memory
+// latencies per thread of the four-row kernel).
+template <int NROWS, int CPC, int kPx, int kPitch>
+__device__ inline void stage_rows(bf16x8* sm, const bf16* __restrict__ inb, int y0, int Wp,
+                                  int x0) {
+  constexpr int kTotal = NROWS * kPx * CPC;
+  constexpr int kIt = (kTotal + 255) / 256;
+  constexpr int CIN = CPC * 8;
+  bf16x8 v[kIt];
+  int slot[kIt];
+#pragma unroll
+  for (int k = 0; k < kIt; ++k) {
+    const int q = threadIdx.x + 256 * k;
+    const int qq = q < kTotal ? q : 0;
+    const int row = qq / (kPx * CPC), rem = qq - row * (kPx * CPC);
+    const int px = rem / CPC, ch = rem - px * CPC;
+    slot[k] = row * (kPx * kPitch) + px * kPitch + ch;
+    v[k] = *reinterpret_cast<const bf16x8*>(inb + ((size_t)(y0 + row) * Wp + (x0 + px)) * CIN +
+                                            ch * 8);
+  }
+#pragma unroll
+  for (int k = 0; k < kIt; ++k)
+    if (threadIdx.x + 256 * k < kTotal) sm[slot[k]] = v[k];
+}
+
 template <int CIN>
 __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(
     const bf16* __restrict__ in, const bf16* __restrict__ wt,
@@ -169,12 +197,7 @@ __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(
   const size_t plane = (size_t)(H + 2) * Wp;
   const bf16* inb = in + (size_t)b * plane * CIN;
   const int x0 = seg * 128;  // padded column of the segment's left ring pixel
-  for (int q = threadIdx.x; q < 3 * kPx * CPC; q += 256) {
-    const int row = q / (kPx * CPC), rem = q - row * (kPx * CPC);
-    const int px = rem / CPC, ch = rem - px * CPC;
-    sm[row][px * kPitch + ch] = *reinterpret_cast<const bf16x8*>(
-        inb + ((size_t)(y + row) * Wp + (x0 + px)) * CIN + ch * 8);
-  }
+  stage_rows<3, CPC, kPx, kPitch>(&sm[0][0], inb, y, Wp, x0);
   bf16x8 A[NS];
 #pragma unroll
   for (int t = 0; t < NS; ++t) {
@@ -242,12 +265,7 @@ __global__ __launch_bounds__(256) void conv_fwd_rows_kernel(
   const size_t plane = (size_t)(H + 2) * Wp;
   const bf16* inb = in + (size_t)b * plane * CIN;
   const int x0 = seg * 128;
-  for (int q = threadIdx.x; q < (R + 2) * kPx * CPC; q += 256) {
-    const int row = q / (kPx * CPC), rem = q - row * (kPx * CPC);
-    const int px = rem / CPC, ch = rem - px * CPC;
-    sm[row][px * kPitch + ch] = *reinterpret_cast<const bf16x8*>(
-        inb + ((size_t)(y0 + row) * Wp + (x0 + px)) * CIN + ch * 8);
-  }
+  stage_rows<R + 2, CPC, kPx, kPitch>(&sm[0][0], inb, y0, Wp, x0);
   bf16x8 A[NS];
 #pragma unroll
   for (int t = 0; t < NS; ++t) {
