@@ -253,6 +253,14 @@ class DeviceChunk:
               "fcg_precondition")
         return v.value
 
+    def fcg_calc_r_precondition(self, alpha, use_net):
+        """r -= alpha w, then z = M(r); returns (r.r, z.w)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.tea_hip_fcg_calc_r_precondition(self.handle, alpha, int(use_net),
+                                                       ctypes.byref(a), ctypes.byref(b)),
+              "fcg_calc_r_precondition")
+        return a.value, b.value
+
     def fcg_calc_w(self, alpha, apply_u, beta):
         a, b = ctypes.c_double(), ctypes.c_double()
         check(self.lib.tea_hip_fcg_calc_w(self.handle, alpha, int(apply_u), beta,

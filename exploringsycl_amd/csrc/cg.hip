@@ -808,6 +808,7 @@ int apply_pending(tea_hip_chunk* c) {
 
 int apply_cg_pending(tea_hip_chunk* c) {
   if (c->ra) TEA_RETURN_IF(ra_discard(c));
+  if (c->hold) TEA_RETURN_IF(apply_mp_hold(c));
   if (!c->pend_u && !c->pend_p && !c->pend_halo) return TEA_HIP_OK;
   double** f = c->field;
   const bool halo_u = c->pend_u;  // the deferred ring update includes U

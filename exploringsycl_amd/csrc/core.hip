@@ -387,6 +387,7 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
   if (const char* pv = getenv("TEA_HIP_PAIRS")) c->pairs = c->pairs && atoi(pv) != 0;
   c->quads = c->pairs && x % 4 == 0;
   if (const char* qv = getenv("TEA_HIP_QUADS")) c->quads = c->quads && atoi(qv) != 0;
+  if (const char* hv = getenv("TEA_HIP_HOLD_U")) c->hold_ok = atoi(hv) != 0;
   c->nblk = (int)((n + kPairBlock - 1) / kPairBlock);
   if (const char* pb = getenv("TEA_HIP_PAIR_BLOCKS")) c->pair_blocks = atoi(pb);
   for (int k = 0; k < K_NUM; ++k) c->time_kernel[k] = true;

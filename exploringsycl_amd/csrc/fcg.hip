@@ -1532,7 +1532,7 @@ int tea_hip_fcg_precondition(tea_hip_chunk* c, int use_net, double* zw) {
   TEA_CHECK_ARG(zw != nullptr, "null zw");
   TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no coarse correction attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_HELD(c);
+  TEA_FLUSH_HELD_KEEP_U(c);
   int len;
   TEA_RETURN_IF(launch_precondition(c, use_net, 0, &len));
   return finish_reduction_n(c, 1, len, zw);
@@ -1545,7 +1545,7 @@ int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
   TEA_CHECK_ARG(rr != nullptr && zw != nullptr, "null outputs");
   TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no coarse correction attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_HELD(c);
+  TEA_FLUSH_HELD_KEEP_U(c);
   int lens[2] = {c->nspan, c->nspan};
   TEA_RETURN_IF(launch_mp_r(c, alpha, &lens[0], false));
   TEA_RETURN_IF(launch_precondition(c, use_net, 1, &lens[1]));
@@ -1562,7 +1562,7 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
   TEA_CHECK_ARG(c->fcg_z != nullptr && c->mp_r != nullptr, "FCG not initialised");
   TEA_CHECK_ARG(pw && pr, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_HELD(c);
+  TEA_FLUSH_HELD_KEEP_U(c);
   // p = z + beta p_old (the mixed-precision kernel with z in place of r),
   // w = A p, and the dots p.w and p.r.
   MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->fcg_z,
@@ -1570,18 +1570,18 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
            alpha, beta, apply_u ? 1 : 0, 1};
   int len = c->nspan;
   if (use_pair_blocks(c, 1)) {
-    {
+    if (c->quads) {
+      TEA_RETURN_IF(launch_mp_w4(c, a, K_FCG_W));
+    } else {
+      if (c->hold) TEA_RETURN_IF(apply_mp_hold(c));
       Timed t(c, K_FCG_W);
-      if (c->quads)
-        hipLaunchKernelGGL((mp_w4_kernel<true>), quad_grid(c), dim3(kQuadThreads), 0,
-                           c->stream, c->geo, a, c->sub, c->nblk);
-      else
-        hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
-                           c->stream, c->geo, a, c->sub, c->nblk);
+      hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, a, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
     TEA_RETURN_IF(launch_span_finish(c, 2, &len, 0, true));
   } else {
+    if (c->hold) TEA_RETURN_IF(apply_mp_hold(c));
     Timed t(c, K_FCG_W);
     if (c->pairs)
       hipLaunchKernelGGL((mp_w2_kernel<true, 4>), span_grid(c), dim3(kBlock), 0,

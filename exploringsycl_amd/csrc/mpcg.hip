@@ -389,8 +389,10 @@ __device__ inline void ring_store(T* f, int i, int kc, int rc, int x, int xl, in
 // neighbours across lanes move on the DPP wave shifts; lane 0 loads p and r
 // at e0 - 1, lane 63 p, r and kx at e0 + 4.  The in-lane dot sums are
 // ((v0 + v1) + (v2 + v3)), tree steps 1 and 2, and the waves store their
-// sums for span_finish_kernel (wave_subtree_quads).
-template <bool DOT2>
+// sums for span_finish_kernel (wave_subtree_quads).  UM selects the u
+// update: none, u += alpha p_old, or u = (u + alpha_h p_h) + alpha p_old (a
+// held update and this step's, each rounded as when applied one by one).
+template <bool DOT2, int UM>
 __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, double* sub, int nblk) {
   const int blk = blockIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
@@ -410,9 +412,10 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
   // whose rows above and below are in range.
   const int ic = e < g.n ? e : x + 4;
   const int ir = any ? e : x + 4;
-  // Every load is issued unconditionally, r's and u's also where the step
-  // does not use them (the first iteration only): loads under a branch were
-  // each waited for inside it, before the next ones were issued.
+  // Every load is issued unconditionally, r's also where the step does not
+  // use it (the first iteration only): loads under a branch were each
+  // waited for inside it, before the next ones were issued.  u (and the
+  // held p) are loaded as UM, a template argument, says.
   const float4 P0 = ld4(P, ic), Pt = ld4(P, ir + x), Pb = ld4(P, ir - x);
   const float4 R0 = ld4(R, ic), Rt = ld4(R, ir + x), Rb = ld4(R, ir - x);
   const float4 KX = ld4(a.kx, ic), KY = ld4(a.ky, ir), KYt = ld4(a.ky, ir + x);
@@ -431,8 +434,18 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
   const bool inA1 = ua < g.n && interior(g, kA + 1, rA);
   const bool inB0 = ub < g.n && interior(g, kB, rB);
   const bool inB1 = ub < g.n && interior(g, kB + 1, rB);
-  const double2 UA = ld2_nt(a.u, inA0 || inA1 ? ua : x + 2);
-  const double2 UB = ld2_nt(a.u, inB0 || inB1 ? ub : x + 2);
+  double2 UA = {}, UB = {};
+  float2 HA = {}, HB = {};
+  if (UM > 0) {
+    UA = ld2_nt(a.u, inA0 || inA1 ? ua : x + 2);
+    UB = ld2_nt(a.u, inB0 || inB1 ? ub : x + 2);
+  }
+  if (UM == 2) {
+    // p_h is the p_new buffer: each of its cells is read here by the wave
+    // that overwrites it below, and by no other.
+    HA = ld2(a.p_h, inA0 || inA1 ? ua : x + 2);
+    HB = ld2(a.p_h, inB0 || inB1 ? ub : x + 2);
+  }
   float4 RD = {};
   if (DOT2) RD = ld4(a.rdot, ir);
   float PN = 0.0f, RN = 0.0f, KXN = 0.0f;
@@ -510,7 +523,7 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
 #pragma unroll
   for (int q = 0; q < (DOT2 ? 2 : 1); ++q)
     v[q] = (prod[q][0] + prod[q][1]) + (prod[q][2] + prod[q][3]);
-  if (a.apply_u) {
+  if (UM > 0) {
     // Cells 2l, 2l + 1 of each half-wave are components (x, y) or (z, w) of
     // quad lane l / 2 (A) or 32 + l / 2 (B).
     const int sa = lane >> 1, sb = kWave / 2 + (lane >> 1);
@@ -519,10 +532,17 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
                 aw = __shfl(P0.w, sa);
     const float bx = __shfl(P0.x, sb), by = __shfl(P0.y, sb), bz = __shfl(P0.z, sb),
                 bw = __shfl(P0.w, sb);
-    const double un[4] = {UA.x + a.alpha * (double)(hi ? az : ax),
-                          UA.y + a.alpha * (double)(hi ? aw : ay),
-                          UB.x + a.alpha * (double)(hi ? bz : bx),
-                          UB.y + a.alpha * (double)(hi ? bw : by)};
+    double u0[4] = {UA.x, UA.y, UB.x, UB.y};
+    if (UM == 2) {
+      u0[0] = u0[0] + a.alpha_h * (double)HA.x;
+      u0[1] = u0[1] + a.alpha_h * (double)HA.y;
+      u0[2] = u0[2] + a.alpha_h * (double)HB.x;
+      u0[3] = u0[3] + a.alpha_h * (double)HB.y;
+    }
+    const double un[4] = {u0[0] + a.alpha * (double)(hi ? az : ax),
+                          u0[1] + a.alpha * (double)(hi ? aw : ay),
+                          u0[2] + a.alpha * (double)(hi ? bz : bx),
+                          u0[3] + a.alpha * (double)(hi ? bw : by)};
     if (inA0 || inA1) st2_nt(a.u, ua, inA0, inA1, un[0], un[1]);
     if (inB0 || inB1) st2_nt(a.u, ub, inB0, inB1, un[2], un[3]);
     // Halo ring of boundary-adjacent cells (rare; per cell).
@@ -546,7 +566,9 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
   wave_subtree_quads<DOT2 ? 2 : 1>(v, sub, nblk, blk);
 }
 
-template __global__ void mp_w4_kernel<true>(Geo, MpArgs, double*, int);
+template __global__ void mp_w4_kernel<true, 0>(Geo, MpArgs, double*, int);
+template __global__ void mp_w4_kernel<true, 1>(Geo, MpArgs, double*, int);
+template __global__ void mp_w4_kernel<true, 2>(Geo, MpArgs, double*, int);
 
 
 // r -= alpha w and r.r (with RW also r.w), two cells per lane.
@@ -786,6 +808,48 @@ int ensure_mp_buffers(tea_hip_chunk* c) {
 
 }  // namespace
 
+int apply_mp_hold(tea_hip_chunk* c) {
+  if (!c->hold) return TEA_HIP_OK;
+  c->hold = false;
+  Timed t(c, K_MP_APPLY_U);
+  MpApplyUBody body{c->field[TEA_FIELD_U], c->mp_p[c->hold_buf], c->hold_alpha, c->geo};
+  hipLaunchKernelGGL((each_kernel<MpApplyUBody>), flat_grid(c->geo.n), dim3(kBlock), 0,
+                     c->stream, c->geo, body);
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
+int launch_mp_w4(tea_hip_chunk* c, MpArgs a, int kid) {
+  const int dst = c->mp_cur ^ 1;  // a.p_new's buffer
+  int um = a.apply_u ? 1 : 0;
+  if (c->hold && !(um == 1 && a.apply_p && c->hold_buf == dst))
+    TEA_RETURN_IF(apply_mp_hold(c));
+  if (c->hold) {
+    um = 2;
+    a.p_h = c->mp_p[dst];
+    a.alpha_h = c->hold_alpha;
+    c->hold = false;
+  } else if (um == 1 && a.apply_p && c->hold_ok) {
+    // Held: the next step's p_new buffer is this step's p_old.
+    um = 0;
+    c->hold = true;
+    c->hold_buf = c->mp_cur;
+    c->hold_alpha = a.alpha;
+  }
+  Timed t(c, kid);
+  if (um == 0)
+    hipLaunchKernelGGL((mp_w4_kernel<true, 0>), quad_grid(c), dim3(kQuadThreads), 0,
+                       c->stream, c->geo, a, c->sub, c->nblk);
+  else if (um == 1)
+    hipLaunchKernelGGL((mp_w4_kernel<true, 1>), quad_grid(c), dim3(kQuadThreads), 0,
+                       c->stream, c->geo, a, c->sub, c->nblk);
+  else
+    hipLaunchKernelGGL((mp_w4_kernel<true, 2>), quad_grid(c), dim3(kQuadThreads), 0,
+                       c->stream, c->geo, a, c->sub, c->nblk);
+  TEA_HIP_TRY(hipGetLastError());
+  return TEA_HIP_OK;
+}
+
 // r -= alpha w and the partials of r.r (and, with rw, of r.w in the second
 // array) into c->partials (not waited for); *len is the number of partials
 // left to finish in each array.
@@ -888,7 +952,7 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
   TEA_CHECK_ARG(pw != nullptr && pr != nullptr, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_HELD(c);
+  TEA_FLUSH_HELD_KEEP_U(c);
   // The two-dot kernel of flexible CG with z = r: p = r + beta p_old,
   // w = A p, p.w and p.r.
   MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->mp_r,
@@ -896,18 +960,18 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
            alpha, beta, apply_u ? 1 : 0, apply_p ? 1 : 0};
   int len = c->nspan;
   if (use_pair_blocks(c, 1)) {
-    {
+    if (c->quads) {
+      TEA_RETURN_IF(launch_mp_w4(c, a, K_MP_W));
+    } else {
+      if (c->hold) TEA_RETURN_IF(apply_mp_hold(c));
       Timed t(c, K_MP_W);
-      if (c->quads)
-        hipLaunchKernelGGL((mp_w4_kernel<true>), quad_grid(c), dim3(kQuadThreads), 0,
-                           c->stream, c->geo, a, c->sub, c->nblk);
-      else
-        hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
-                           c->stream, c->geo, a, c->sub, c->nblk);
+      hipLaunchKernelGGL((mp_w1_kernel<true>), pair_grid(c), dim3(kBlock), 0,
+                         c->stream, c->geo, a, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
     TEA_RETURN_IF(launch_span_finish(c, 2, &len, 0, true));
   } else {
+    if (c->hold) TEA_RETURN_IF(apply_mp_hold(c));
     Timed t(c, K_MP_W);
     if (c->pairs)
       hipLaunchKernelGGL((mp_w2_kernel<true, 4>), span_grid(c), dim3(kBlock), 0,
@@ -932,7 +996,7 @@ int tea_hip_mpcg_calc_r(tea_hip_chunk* c, double alpha, double* rr, double* rw) 
   TEA_CHECK_ARG(c->mp_r != nullptr, "tea_hip_mpcg_init has not been called");
   TEA_CHECK_ARG(rr != nullptr && rw != nullptr, "null outputs");
   TEA_HIP_TRY(hipSetDevice(c->device));
-  TEA_FLUSH_HELD(c);
+  TEA_FLUSH_HELD_KEEP_U(c);
   int len;
   TEA_RETURN_IF(launch_mp_r(c, alpha, &len, true));
   double out[2];

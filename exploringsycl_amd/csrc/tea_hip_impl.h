@@ -582,6 +582,10 @@ struct MpArgs {
   const float* rdot;  // second dot product p.rdot (DOT2 only)
   double alpha, beta;
   int apply_u, apply_p;
+  // A held update u += alpha_h p_h applied first (mp_w4_kernel, UM 2);
+  // p_h is p_new's buffer, read before it is overwritten.
+  const float* p_h;
+  double alpha_h;
 };
 
 template <int kBatch, bool DOT2>
@@ -595,7 +599,9 @@ __global__ void mp_w2_kernel(Geo g, MpArgs a, double* partials, int nspan,
                              int single_group);
 template <bool DOT2>
 __global__ void mp_w1_kernel(Geo g, MpArgs a, double* sub, int nblk);
-template <bool DOT2>
+// UM: 0 leaves u alone, 1 adds alpha p_old, 2 adds alpha_h p_h and then
+// alpha p_old.
+template <bool DOT2, int UM>
 __global__ void mp_w4_kernel(Geo g, MpArgs a, double* sub, int nblk);
 
 }  // namespace tea
@@ -705,6 +711,14 @@ struct tea_hip_chunk {
   // face coefficients; allocated by the first tea_hip_mpcg_init.
   float* mp_p[2] = {nullptr, nullptr};
   int mp_cur = 0;
+  // u += hold_alpha p, p in mp_p[hold_buf], reported done to the host but
+  // held back (TEA_HIP_HOLD_U=0 turns it off): the next four-cells-per-lane
+  // w-step, which overwrites that p buffer, reads it first and applies both
+  // updates in one pass over u; any other entry point applies it first.
+  bool hold_ok = true;
+  bool hold = false;
+  int hold_buf = 0;
+  double hold_alpha = 0.0;
   float* mp_r = nullptr;
   float* mp_w = nullptr;
   float* mp_kx = nullptr;
@@ -882,6 +896,14 @@ int flush_held(tea_hip_chunk* c);
 // (and, with rw, of r.w as a second array) written to c->partials
 // (mpcg.hip); *len is the number left to finish in each.
 int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw);
+// The four-cells-per-lane w-step for the arguments a (p_old in
+// mp_p[mp_cur], p_new in the other buffer), timed as kernel kid, with u +=
+// alpha p_old held back over one step: held when nothing is, applied with
+// the held update when that one's p is this step's p_new buffer; a held
+// update that cannot ride along runs first on its own (mpcg.hip).
+int launch_mp_w4(tea_hip_chunk* c, MpArgs a, int kid);
+// Applies a held-back u += hold_alpha p on its own (mpcg.hip).
+int apply_mp_hold(tea_hip_chunk* c);
 
 // One reflective-halo launch (all flagged fields and external faces).
 int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
@@ -892,10 +914,12 @@ int launch_halos(tea_hip_chunk* c, int depth, const int neighbours[4],
 int ra_discard(tea_hip_chunk* c);
 
 // Checks the handle and makes its device current, leaving a run-ahead in
-// flight (the CG entry points that can take it over).
+// flight (the CG entry points that can take it over); applies a held-back
+// mixed-precision u update.
 #define TEA_ENTER_RA(c)                                   \
   TEA_CHECK_ARG((c) != nullptr, "null chunk handle");     \
-  TEA_HIP_TRY(hipSetDevice((c)->device))
+  TEA_HIP_TRY(hipSetDevice((c)->device));                 \
+  if ((c)->hold) TEA_RETURN_IF(::tea::apply_mp_hold(c))
 
 // Checks the handle and makes its device current (entry points); drops a
 // run-ahead of the CG iteration.
@@ -904,8 +928,17 @@ int ra_discard(tea_hip_chunk* c);
   if ((c)->ra) TEA_RETURN_IF(::tea::ra_discard(c))
 
 // Entry points other than the PPCG inner step and the halo update first run
-// a held-back halo update and PPCG or Chebyshev step (and drop a run-ahead).
+// a held-back halo update and PPCG or Chebyshev step (and drop a run-ahead)
+// and apply a held-back mixed-precision u update.
 #define TEA_FLUSH_HELD(c)                                                  \
+  do {                                                                     \
+    if ((c)->hold) TEA_RETURN_IF(::tea::apply_mp_hold(c));                 \
+    TEA_FLUSH_HELD_KEEP_U(c);                                              \
+  } while (0)
+
+// TEA_FLUSH_HELD leaving a held-back mixed-precision u update held: for the
+// mixed-precision and FCG steps, which touch neither u nor the held p.
+#define TEA_FLUSH_HELD_KEEP_U(c)                                           \
   do {                                                                     \
     if ((c)->ra) TEA_RETURN_IF(::tea::ra_discard(c));                      \
     if ((c)->held || (c)->pre_halo >= 0) TEA_RETURN_IF(::tea::flush_held(c)); \
