@@ -104,11 +104,21 @@ __global__ __launch_bounds__(256) void box_rows_kernel(int hc, int wc,
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= hc * wc) return;
   const int X = idx % wc;
-  double s = 0.0;
+  // All 26 loads first (clamped indices), then the sum in window order;
+  // adding 0 for a cell beyond the edge leaves the sum (of squares, >= +0)
+  // as skipping it does.
+  float rb[2 * kWin + 1], ds[2 * kWin + 1];
+#pragma unroll
   for (int d = -kWin; d <= kWin; ++d) {
-    if (X + d < 0 || X + d >= wc) continue;
-    const double v = (double)rbar[idx + d] * (double)dis[idx + d];
-    s += v * v;
+    const int j = X + d < 0 || X + d >= wc ? idx : idx + d;
+    rb[d + kWin] = rbar[j];
+    ds[d + kWin] = dis[j];
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int d = -kWin; d <= kWin; ++d) {
+    const double v = (double)rb[d + kWin] * (double)ds[d + kWin];
+    s += X + d < 0 || X + d >= wc ? 0.0 : v * v;
   }
   tmp[idx] = (float)s;
 }
@@ -123,21 +133,28 @@ __global__ __launch_bounds__(256) void box_cols_kernel(int hc, int wc,
                                                        const float* __restrict__ dis,
                                                        const float* __restrict__ feat,
                                                        bf16* __restrict__ in0) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
+  // Rows in one contiguous range per XCD (xcd_span): the 13 rows a column
+  // window reads are then mostly in the block's own XCD's L2.
+  const int idx = xcd_span(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (idx >= hc * wc) return;
   const int X = idx % wc, Y = idx / wc;
+  // All 13 loads first (clamped rows), then the sum in window order (the
+  // row sums are >= +0, so adding 0 beyond the edge is skipping).
+  float t[2 * kWin + 1];
+#pragma unroll
+  for (int d = -kWin; d <= kWin; ++d)
+    t[d + kWin] = tmp[Y + d < 0 || Y + d >= hc ? idx : idx + d * wc];
+  const float rb = rbar[idx], ds = dis[idx], f0 = feat[idx], f1 = feat[hc * wc + idx];
   double s = 0.0;
-  for (int d = -kWin; d <= kWin; ++d) {
-    if (Y + d < 0 || Y + d >= hc) continue;
-    s += (double)tmp[idx + d * wc];
-  }
+#pragma unroll
+  for (int d = -kWin; d <= kWin; ++d) s += Y + d < 0 || Y + d >= hc ? 0.0 : (double)t[d + kWin];
   const float sg = fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))),
                          kFloor * scale[0]);
   sig[idx] = sg;
   bf16x8 o = bf16x8{};
-  o[0] = sg > 0.0f ? (bf16)((rbar[idx] * dis[idx]) / sg) : (bf16)0.0f;
-  o[1] = (bf16)feat[idx];
-  o[2] = (bf16)feat[hc * wc + idx];
+  o[0] = sg > 0.0f ? (bf16)((rb * ds) / sg) : (bf16)0.0f;
+  o[1] = (bf16)f0;
+  o[2] = (bf16)f1;
   *reinterpret_cast<bf16x8*>(in0 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * 8) = o;
 }
 
@@ -610,17 +627,17 @@ __global__ __launch_bounds__(256) void sweeps_kernel(int hc, int wc, int sweeps,
 
 // ---- the learned coarse solve of one level in one launch -----------------
 // net_level_kernel computes, for one 16 x 16 tile of a coarse level, what
-// box_rows_kernel, box_cols_kernel, the six layers of the network
-// (nn.hip: conv_*_kernel, out_layer_kernel) and the coarse dots compute for
+// box_rows_kernel, box_cols_kernel, the five hidden layers of the network
+// (nn.hip: conv_*_kernel), out_tile_kernel and the coarse dots compute for
 // it, keeping every intermediate in LDS: the staged rbar and D^-1/2 (the
 // tile and 13 cells around it), the local scale sigma and the packed BF16
 // input (7 cells around it: the network's receptive field plus the ring of
 // T y), and the activations of each layer on its shrinking region (28^2,
 // 26^2, ..., 20^2 pixels), then N and y = N sigma D^-1/2 on the tile and its
 // ring.  Each output pixel takes the same MFMA sequence (18 K-steps of 16
-// over (tap, channel), weights as the A operand, the pixel's patch as B) and
-// the same FP32 output layer as the per-layer kernels, so N is bitwise
-// theirs; pixels outside the grid are set to zero after every layer (the
+// over (tap, channel), weights as the A operand, the pixel's patch as B) as
+// in the per-layer kernels, and the output layer is out_tile_kernel's
+// (fused_out), so N is bitwise theirs; pixels outside the grid are set to zero after every layer (the
 // zero ring of the per-layer kernels).  The tile's partials of y.rbar and
 // y.T y, the level-1 block sums of T y and y itself are those of
 // sweeps_kernel.
@@ -734,6 +751,61 @@ __device__ inline void fused_conv(const bf16x8* __restrict__ in, bf16x8* __restr
         o[2 * act_slot(q[j], g) + hf] = v;
       }
     }
+  }
+}
+
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+// The output layer (32 -> 1) on the matrix cores: for the side-so region of
+// pixels whose 3 x 3 patches lie in `in` (side so + 2, act_slot layout),
+// N - b6 = the sum over (tap, channel) of w6 a, as 9 K-steps (one per tap)
+// of v_mfma_f32_16x16x32_bf16 with w6 in row 0 of the A operand (rows 1-15
+// zero) and the patches of 16 pixels as the columns of B: lane l of a wave
+// supplies chunk l / 16 of tap t's input pixel for pixel l % 16, and lanes
+// 0-15 end with the 16 sums in accumulator 0.  w6 holds BF16 values, so
+// its BF16 operand is exact; the sum is in the MFMA's order.  (On the
+// vector units, 864 FP32 operations per pixel without contraction, this
+// layer was the slowest launch of the chain.)  f(q, sum) takes pixel q's
+// sum.
+struct OutFrag {
+  bf16x8 A[9];
+  // Every lane loads (a load under the lane test was waited for inside it);
+  // lanes other than 0, 16, 32 and 48 then zero theirs.
+  __device__ inline void load(const float* __restrict__ w6) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool row0 = (lane & 15) == 0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const float4 lo = *reinterpret_cast<const float4*>(w6 + t * fused::kCh + 8 * (lane >> 4));
+      const float4 hi =
+          *reinterpret_cast<const float4*>(w6 + t * fused::kCh + 8 * (lane >> 4) + 4);
+      const float f[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+      bf16x8 a;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] = row0 ? (bf16)f[k] : (bf16)0.0f;
+      A[t] = a;
+    }
+  }
+};
+
+template <class F>
+__device__ inline void fused_out(const bf16x8* __restrict__ in, int so, const OutFrag& w,
+                                 int nwaves, F f) {
+  const int npx = so * so, si = so + 2;
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const int px = lane & 15, ch = lane >> 4;
+  const int ntiles = (npx + 15) / 16;
+  for (int tile = wv; tile < ntiles; tile += nwaves) {
+    const int q = min(tile * 16 + px, npx - 1);
+    const int r = q / so, c = q - r * so;
+    f32x4v acc = {};
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int pin = (r + t / 3) * si + (c + t % 3);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.A[t], in[act_slot(pin, ch)], acc, 0, 0,
+                                                     0);
+    }
+    if (ch == 0 && tile * 16 + px < npx) f(q, acc[0]);
   }
 }
 
@@ -866,26 +938,18 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
     __syncthreads();
     fused_conv<kCh, NT>(src, dst, si, w, X0 - off, Y0 - off, hc, wc, kWaves_);
   }
+  // Output layer on the 18 x 18 region from A (side 20) on the matrix cores
+  // (fused_out, as out_tile_kernel); then y = N sigma D^-1/2 (zero outside
+  // the grid).
+  OutFrag wo;
+  wo.load(wts.w6);
   __syncthreads();
-  // Output layer on the 18 x 18 region from A (side 20), FP32 in the order
-  // of out_layer_kernel; then y = N sigma D^-1/2 (zero outside the grid).
   const float b6 = wts.bias[5 * kCh];
-  for (int q = tid; q < kSN * kSN; q += kThreads) {
+  fused_out(actA, kSN, wo, kWaves_, [&](int q, float sum) {
     const int r = q / kSN, c = q - r * kSN;
-    float acc = 0.0f;
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-      const int pin = (r + tap / 3) * (kSN + 2) + (c + tap % 3);
-#pragma unroll
-      for (int c8 = 0; c8 < kCh; c8 += 8) {
-        const bf16x8 v = actA[act_slot(pin, c8 >> 3)];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) acc += wts.w6[tap * kCh + c8 + k] * (float)v[k];
-      }
-    }
     const int X = X0 - 1 + c, Y = Y0 - 1 + r;
-    s_n[q] = in_grid(X, Y) ? (acc + b6) * (s_sig[q] * s_ds[q]) : 0.0f;
-  }
+    s_n[q] = in_grid(X, Y) ? (sum + b6) * (s_sig[q] * s_ds[q]) : 0.0f;
+  });
   __syncthreads();
   // The tile: y, T y of y (SMVP order, as sweeps_kernel), the partials and
   // the level-1 block sums.
@@ -948,42 +1012,90 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
 }
 
 // The end of net_level_kernel for the per-layer pipeline (box_rows_kernel,
-// box_cols_kernel and the network's own kernels): from the network output
-// N on the grid, y = N sigma D^-1/2 on the tile and its ring, then the same
-// y, partials and level-1 block sums as net_level_kernel.  One 256-thread
-// block per 16 x 16 tile.
-__global__ __launch_bounds__(256) void net_tile_kernel(
-    int hc, int wc, const float* __restrict__ net, const float* __restrict__ sig,
+// box_cols_kernel and the network's hidden layers): the output layer on the
+// tile and its ring, from the last hidden layer's activations staged in LDS
+// (the tile and 2 cells around it), by fused_out as in net_level_kernel, so
+// N is bitwise its; then y = N sigma D^-1/2 on the tile and its ring, and
+// the same y, partials and level-1 block sums as net_level_kernel.  One
+// 256-thread block per 16 x 16 tile.  (A separate output-layer launch read
+// each activation about three times from HBM and wrote N for this kernel
+// to read back: 14 + 6.7 us at 512^2.)
+__global__ __launch_bounds__(256) void out_tile_kernel(
+    int hc, int wc, const bf16* __restrict__ a5, const float* __restrict__ w6,
+    const float* __restrict__ bias, const float* __restrict__ sig,
     const float* __restrict__ dis, const float* __restrict__ rbar,
     const float* __restrict__ kc, float* __restrict__ y, double* __restrict__ part,
     int ntiles, double* __restrict__ xsum) {
   using namespace fused;
+  constexpr int kSA = kSN + 2;                // 20: staged activation region
+  constexpr int kChunks = kSA * kSA * 4;      // 16-byte chunks staged
+  constexpr int kIt = (kChunks + 255) / 256;  // per thread
+  constexpr int kNq = (kSN * kSN + 255) / 256;
+  __shared__ bf16x8 s_act[kSA * kSA * 5];
   __shared__ float s_n[kSN * kSN];
+  __shared__ float s_sd[kSN * kSN];
   __shared__ double s_a[256], s_b[256], s_t[256];
   const int tid = threadIdx.x;
   const int X0 = blockIdx.x * kT, Y0 = blockIdx.y * kT;
   const long n = (long)hc * wc;
-  for (int q = tid; q < kSN * kSN; q += 256) {
-    const int X = X0 - 1 + q % kSN, Y = Y0 - 1 + q / kSN;
-    float v = 0.0f;
-    if (X >= 0 && X < wc && Y >= 0 && Y < hc) {
-      const long j = (long)Y * wc + X;
-      v = net[j] * (sig[j] * dis[j]);
-    }
-    s_n[q] = v;
+  // Every global load of the kernel is issued before the first LDS store:
+  // the staged activations, sigma D^-1/2 on the tile and its ring, and the
+  // coarse coefficients and rbar of the thread's tile cell.
+  // Region pixel q is coarse cell (X0 - 2 + q % kSA, Y0 - 2 + q / kSA), at
+  // padded position (+1, +1) in a5 (zero ring; cells beyond it are only
+  // read for ring pixels outside the grid, whose N is not used).
+  OutFrag wo;
+  wo.load(w6);
+  const float b6 = bias[5 * kCh];
+  const int Wp = wc + 2;
+  bf16x8 v[kIt];
+#pragma unroll
+  for (int k = 0; k < kIt; ++k) {
+    const int j = tid + 256 * k;
+    const int q = j >> 2;
+    const int Xp = X0 - 1 + q % kSA, Yp = Y0 - 1 + q / kSA;
+    const bool ok = j < kChunks && Xp >= 0 && Xp <= wc + 1 && Yp >= 0 && Yp <= hc + 1;
+    const size_t at = ok ? ((size_t)Yp * Wp + Xp) * kCh + 8 * (j & 3) : 0;
+    v[k] = *reinterpret_cast<const bf16x8*>(a5 + at);
+    if (!ok) v[k] = bf16x8{};
   }
-  __syncthreads();
+  float sd[kNq];
+#pragma unroll
+  for (int k = 0; k < kNq; ++k) {
+    const int q = tid + 256 * k;
+    const int X = X0 - 1 + q % kSN, Y = Y0 - 1 + q / kSN;
+    const bool ok = q < kSN * kSN && X >= 0 && X < wc && Y >= 0 && Y < hc;
+    const long j = ok ? (long)Y * wc + X : 0;
+    sd[k] = sig[j] * dis[j];  // >= 0; -1 marks a pixel outside the grid
+    if (!ok) sd[k] = -1.0f;
+  }
   const int tx = tid & (kT - 1), ty = tid / kT;
   const int X = X0 + tx, Y = Y0 + ty;
-  const int k = (ty + 1) * kSN + tx + 1;
-  const long i = (long)Y * wc + X;
   const bool in = X < wc && Y < hc;
+  const long i = in ? (long)Y * wc + X : 0;
+  const float fkl = kc[i], fkb = kc[n + i];
+  const float fkr = kc[X + 1 < wc ? i + 1 : i], fkt = kc[Y + 1 < hc ? n + i + wc : n + i];
+  const float frb = rbar[i];
+#pragma unroll
+  for (int k = 0; k < kIt; ++k) {
+    const int j = tid + 256 * k;
+    if (j < kChunks) s_act[act_slot(j >> 2, j & 3)] = v[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kNq; ++k)
+    if (tid + 256 * k < kSN * kSN) s_sd[tid + 256 * k] = sd[k];
+  __syncthreads();
+  fused_out(s_act, kSN, wo, 4, [&](int q, float sum) {
+    s_n[q] = s_sd[q] < 0.0f ? 0.0f : (sum + b6) * s_sd[q];
+  });
+  __syncthreads();
+  const int k = (ty + 1) * kSN + tx + 1;
   double kl = 0.0, kr = 0.0, kb = 0.0, kt = 0.0;
   if (in) {
-    kl = kc[i];
-    kb = kc[n + i];
-    kr = X + 1 < wc ? (double)kc[i + 1] : 0.0;
-    kt = Y + 1 < hc ? (double)kc[n + i + wc] : 0.0;
+    kl = fkl;
+    kb = fkb;
+    kr = X + 1 < wc ? (double)fkr : 0.0;
+    kt = Y + 1 < hc ? (double)fkt : 0.0;
   }
   const double yc = (double)s_n[k];
   const double tyv = (((1.0 + (kr + kl)) + (kt + kb)) * yc -
@@ -992,27 +1104,15 @@ __global__ __launch_bounds__(256) void net_tile_kernel(
   double a = 0.0, b = 0.0;
   if (in) {
     y[i] = s_n[k];
-    a = yc * (double)rbar[i];
+    a = yc * (double)frb;
     b = yc * tyv;
   }
   s_a[tid] = a;
   s_b[tid] = b;
   s_t[tid] = tyv;
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (tid < st) {
-      s_a[tid] += s_a[tid + st];
-      s_b[tid] += s_b[tid + st];
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    const int tile = blockIdx.y * gridDim.x + blockIdx.x;
-    part[tile] = s_a[0];
-    part[ntiles + tile] = s_b[0];
-  }
-  if (xsum && tid < 16) {
-    const int bx = tid & 3, by = tid >> 2;
+  if (xsum && tid >= 64 && tid < 80) {
+    const int bx = tid & 3, by = (tid >> 2) & 3;
     const int X1 = blockIdx.x * 4 + bx, Y1 = blockIdx.y * 4 + by;
     if (X1 < (wc >> 2) && Y1 < (hc >> 2)) {
       double rs[4];
@@ -1022,6 +1122,26 @@ __global__ __launch_bounds__(256) void net_tile_kernel(
         rs[j] = (p[0] + p[1]) + (p[2] + p[3]);
       }
       xsum[(long)Y1 * (wc >> 2) + X1] = (rs[0] + rs[1]) + (rs[2] + rs[3]);
+    }
+  }
+  // The tile sums: the LDS tree's steps 128 and 64, then steps 32..1 by
+  // shuffles in wave 0 (element t still adds element t + st at each step).
+  if (tid < 128) {
+    s_a[tid] += s_a[tid + 128];
+    s_b[tid] += s_b[tid + 128];
+  }
+  __syncthreads();
+  if (tid < 64) {
+    double va = s_a[tid] + s_a[tid + 64], vb = s_b[tid] + s_b[tid + 64];
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) {
+      va = va + __shfl_down(va, st);
+      vb = vb + __shfl_down(vb, st);
+    }
+    if (tid == 0) {
+      const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+      part[tile] = va;
+      part[ntiles + tile] = vb;
     }
   }
 }
@@ -1148,7 +1268,7 @@ static NetLevelFn net_level_fn(NetShape s) {
 }
 
 // Whether the learned coarse solve runs as the per-layer pipeline (the
-// default: box_rows, box_cols, the network's own kernels, net_tile_kernel)
+// default: box_rows, box_cols, the network's hidden layers, out_tile_kernel)
 // or as one fused launch per level (TEA_FCG_NET=fused), which gives the same
 // y bit for bit.  The fused kernel recomputes each tile's halo at every layer
 // (2.1 times the pixels of the per-layer pipeline with 16 x 16 tiles) and
@@ -1180,10 +1300,11 @@ static int launch_net_level(tea_hip_chunk* c, int l, hipStream_t st) {
                        L.rbar, L.dis, L.tmp, L.rms, L.nrms, L.scale);
     hipLaunchKernelGGL(box_cols_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc, L.tmp,
                        L.scale, L.sig, L.rbar, L.dis, L.feat, nn::input_tensor(L.net));
-    TEA_RETURN_IF(nn::forward_on_stream(L.net, st));
-    hipLaunchKernelGGL(net_tile_kernel, tiles, dim3(256), 0, st, L.hc, L.wc,
-                       nn::output(L.net), L.sig, L.dis, L.rbar, L.kc, L.y, L.tpart,
-                       L.ntiles, xs);
+    TEA_RETURN_IF(nn::forward_hidden_on_stream(L.net, st));
+    const nn::PackedWeights wts = nn::packed_weights(L.net);
+    hipLaunchKernelGGL(out_tile_kernel, tiles, dim3(256), 0, st, L.hc, L.wc,
+                       nn::last_hidden(L.net), wts.w6, wts.bias, L.sig, L.dis, L.rbar, L.kc,
+                       L.y, L.tpart, L.ntiles, xs);
   }
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;

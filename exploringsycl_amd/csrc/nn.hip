@@ -146,13 +146,6 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(
   }
 }
 
-// Forward layer for rows of 128 pixels or more: a block takes a 128-pixel
-// segment of row y, stages input rows y-1..y+1 over the segment plus its
-// 1-pixel ring in LDS once (130 pixels x CIN channels x 3 rows, in 16-byte
-// chunks; a pixel's chunks are padded to kPitch so that the 16 lanes of a
-// 16-byte LDS read, which take consecutive pixels, hit distinct banks),
-// and each wave computes one 32-pixel tile from LDS.  Same arithmetic and
-// output as conv_mfma_kernel<CIN, 0>.  Grid (W / 128, H, B).
 // Stages NROWS padded input rows (from row y0) of a 130-pixel segment
 // (from padded column x0) into LDS, CPC 16-byte chunks per pixel at a pitch
 // of kPitch chunks: every thread issues all its loads before its first LDS
@@ -181,6 +174,28 @@ __device__ inline void stage_rows(bf16x8* sm, const bf16* __restrict__ inb, int 
     if (threadIdx.x + 256 * k < kTotal) sm[slot[k]] = v[k];
 }
 
+// The (segment, row group, sample) of this block in a (W / 128, H / R, B)
+// grid, numbered so that each XCD takes one contiguous range of row groups:
+// the hardware deals blocks round-robin to the 8 XCDs, which put the row
+// groups above and below a block on other XCDs, so the input rows two
+// blocks share were fetched into two L2s (1.27x the activations per hidden
+// layer at 512^2 with four rows per block).
+__device__ inline void xcd_tile(int& seg, int& grp, int& b) {
+  const int nb = gridDim.x * gridDim.y * gridDim.z;
+  const int L = xcd_span(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+  seg = L % gridDim.x;
+  const int rest = L / gridDim.x;
+  grp = rest % gridDim.y;
+  b = rest / gridDim.y;
+}
+
+// Forward layer for rows of 128 pixels or more: a block takes a 128-pixel
+// segment of row y, stages input rows y-1..y+1 over the segment plus its
+// 1-pixel ring in LDS once (130 pixels x CIN channels x 3 rows, in 16-byte
+// chunks; a pixel's chunks are padded to kPitch so that the 16 lanes of a
+// 16-byte LDS read, which take consecutive pixels, hit distinct banks),
+// and each wave computes one 32-pixel tile from LDS.  Same arithmetic and
+// output as conv_mfma_kernel<CIN, 0>.  Grid (W / 128, H, B).
 template <int CIN>
 __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(
     const bf16* __restrict__ in, const bf16* __restrict__ wt,
@@ -190,14 +205,16 @@ __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(
   constexpr int kPitch = CPC == 1 ? 1 : CPC + 1;  // padded chunks per pixel
   constexpr int kPx = 130;
   __shared__ bf16x8 sm[3][kPx * kPitch];
-  const int seg = blockIdx.x, y = blockIdx.y, b = blockIdx.z;
+  int seg, y, b;
+  xcd_tile(seg, y, b);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int hf = lane >> 5, col = lane & 31;
   const int Wp = W + 2;
   const size_t plane = (size_t)(H + 2) * Wp;
   const bf16* inb = in + (size_t)b * plane * CIN;
   const int x0 = seg * 128;  // padded column of the segment's left ring pixel
-  stage_rows<3, CPC, kPx, kPitch>(&sm[0][0], inb, y, Wp, x0);
+  // Weights and biases first: loaded after the staged rows' LDS stores, they
+  // were a second global-memory latency before the first MFMA.
   bf16x8 A[NS];
 #pragma unroll
   for (int t = 0; t < NS; ++t) {
@@ -211,6 +228,7 @@ __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(
   float bco[16];
 #pragma unroll
   for (int v = 0; v < 16; ++v) bco[v] = bias[8 * (v >> 2) + 4 * hf + (v & 3)];
+  stage_rows<3, CPC, kPx, kPitch>(&sm[0][0], inb, y, Wp, x0);
   __syncthreads();
   const int xl = wv * 32 + col;  // pixel within the segment
   bf16x8 Bv[NS];
@@ -258,14 +276,16 @@ __global__ __launch_bounds__(256) void conv_fwd_rows_kernel(
   constexpr int kPitch = CPC == 1 ? 1 : CPC + 1;
   constexpr int kPx = 130;
   __shared__ bf16x8 sm[R + 2][kPx * kPitch];
-  const int seg = blockIdx.x, y0 = blockIdx.y * R, b = blockIdx.z;
+  int seg, grp, b;
+  xcd_tile(seg, grp, b);
+  const int y0 = grp * R;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int hf = lane >> 5, col = lane & 31;
   const int Wp = W + 2;
   const size_t plane = (size_t)(H + 2) * Wp;
   const bf16* inb = in + (size_t)b * plane * CIN;
   const int x0 = seg * 128;
-  stage_rows<R + 2, CPC, kPx, kPitch>(&sm[0][0], inb, y0, Wp, x0);
+  // Weights and biases first (as conv_fwd_lds_kernel).
   bf16x8 A[NS];
 #pragma unroll
   for (int t = 0; t < NS; ++t) {
@@ -279,6 +299,7 @@ __global__ __launch_bounds__(256) void conv_fwd_rows_kernel(
   float bco[16];
 #pragma unroll
   for (int v = 0; v < 16; ++v) bco[v] = bias[8 * (v >> 2) + 4 * hf + (v & 3)];
+  stage_rows<R + 2, CPC, kPx, kPitch>(&sm[0][0], inb, y0, Wp, x0);
   __syncthreads();
   const int xl = wv * 32 + col;
 #pragma unroll 1
@@ -322,7 +343,9 @@ __global__ __launch_bounds__(256) void out_layer_kernel(
     const float* __restrict__ bias, float* __restrict__ out, int B, int H,
     int W) {
   const float b6 = bias[5 * kC];
-  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  // Rows in one contiguous range per XCD (xcd_span): the rows above and
+  // below a block's pixels are then in its XCD's L2.
+  const long idx = (long)xcd_span(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   const long total = (long)B * H * W;
   if (idx >= total) return;
   const int x = (int)(idx % W);
@@ -690,8 +713,9 @@ dim3 rows_grid(const tea_nn* n, int batch) { return dim3(n->H, batch); }
 unsigned flat_blocks(long count) { return (unsigned)((count + 255) / 256); }
 
 // Forward pass over `batch` samples whose packed input is in n->in0, on
-// stream `st` (the network's own, or a solver's).
-int forward_on(tea_nn* n, int batch, hipStream_t st) {
+// stream `st` (the network's own, or a solver's); without the output layer
+// unless with_output (its caller then runs it on the last activations).
+int forward_on(tea_nn* n, int batch, hipStream_t st, bool with_output = true) {
   const int H = n->H, W = n->W;
   // Several rows per block only while the grid keeps at least 512 blocks:
   // on small grids the one-row kernel's parallelism matters more than the
@@ -730,8 +754,9 @@ int forward_on(tea_nn* n, int batch, hipStream_t st) {
                          st, n->act[l - 1], n->wfh + (size_t)(l - 1) * kWh,
                          n->bias + l * kC, nullptr, n->act[l], H, W, batch);
   }
-  hipLaunchKernelGGL(out_layer_kernel, dim3(flat_blocks((long)batch * H * W)), dim3(256),
-                     0, st, n->act[4], n->w6, n->bias, n->out, batch, H, W);
+  if (with_output)
+    hipLaunchKernelGGL(out_layer_kernel, dim3(flat_blocks((long)batch * H * W)), dim3(256),
+                       0, st, n->act[4], n->w6, n->bias, n->out, batch, H, W);
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
 }
@@ -851,6 +876,8 @@ namespace tea {
 namespace nn {
 // Used by the flexible-CG preconditioner (fcg.hip).
 int forward_on_stream(tea_nn* n, hipStream_t st) { return forward_on(n, 1, st); }
+int forward_hidden_on_stream(tea_nn* n, hipStream_t st) { return forward_on(n, 1, st, false); }
+const bf16* last_hidden(tea_nn* n) { return n->act[4]; }
 bf16* input_tensor(tea_nn* n) { return n->in0; }
 PackedWeights packed_weights(const tea_nn* n) {
   return PackedWeights{n->wf1, n->wfh, n->w6, n->bias};

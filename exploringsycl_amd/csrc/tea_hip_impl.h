@@ -623,6 +623,10 @@ struct PackedWeights {
 };
 PackedWeights packed_weights(const tea_nn* n);
 int forward_on_stream(tea_nn* n, hipStream_t st);
+// The hidden layers alone; the last one's activations (padded (H + 2) x
+// (W + 2) pixels of 32 BF16 channels, zero ring) at last_hidden.
+int forward_hidden_on_stream(tea_nn* n, hipStream_t st);
+const bf16* last_hidden(tea_nn* n);
 bf16* input_tensor(tea_nn* n);
 const float* output(tea_nn* n);
 }  // namespace nn
