@@ -1378,7 +1378,18 @@ static int log2_of(int c) {
 
 // z = M(r) and the partials of z.w into array `slot` of c->partials (not
 // waited for); *len is the number of partials left to finish.
-static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len) {
+// Whether z = M(r) takes the four-cells-per-lane kernel.
+static bool precondition_quads(const tea_hip_chunk* c, int use_net) {
+  bool quads = use_quad_blocks(c, 2);
+  const int nl = use_net ? c->fcg_levels : 0;
+  for (int l = 0; l < nl; ++l) quads = quads && c->fcg_lv[l].c >= 4;
+  return quads;
+}
+
+// With sub_q >= 0 (four-cells-per-lane kernel only) the z.w subtree sums go
+// to array sub_q of c->sub and are left for the caller to finish.
+static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len,
+                               int sub_q = -1) {
   const Geo g = c->geo;
   const int sg = c->ngroup == 1 ? 1 : 0;
   const int nl = use_net ? c->fcg_levels : 0;
@@ -1393,25 +1404,25 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
   const CoarseTerm t0 = nl > 0 ? term(0) : none;
   const CoarseTerm t1 = nl > 1 ? term(1) : none;
   const int kid = nl == 0 ? K_FCG_JACOBI : K_FCG_PROLONG;
-  bool quads = use_quad_blocks(c, 2);
-  for (int l = 0; l < nl; ++l) quads = quads && c->fcg_lv[l].c >= 4;
-  if (quads) {
+  if (precondition_quads(c, use_net)) {
+    double* sub = c->sub + (size_t)(sub_q > 0 ? sub_q : 0) * kWaves * c->nblk;
     {
       Timed t(c, kid);
       if (nl == 0)
         hipLaunchKernelGGL((fcg_z4_kernel<0>), quad_grid(c), dim3(kQuadThreads), 0, c->stream, g,
-                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, c->sub,
+                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, sub,
                            c->nblk);
       else if (nl == 1)
         hipLaunchKernelGGL((fcg_z4_kernel<1>), quad_grid(c), dim3(kQuadThreads), 0, c->stream, g,
-                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, c->sub,
+                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, sub,
                            c->nblk);
       else
         hipLaunchKernelGGL((fcg_z4_kernel<2>), quad_grid(c), dim3(kQuadThreads), 0, c->stream, g,
-                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, c->sub,
+                           c->mp_r, c->mp_kx, c->mp_ky, c->mp_w, t0, t1, c->fcg_z, sub,
                            c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
+    if (sub_q >= 0) return TEA_HIP_OK;
     return launch_span_finish(c, 1, len, slot);
   }
   if (use_pair_blocks(c, 2)) {
@@ -1667,11 +1678,21 @@ int tea_hip_fcg_calc_r_precondition(tea_hip_chunk* c, double alpha, int use_net,
   TEA_CHECK_ARG(!use_net || c->fcg_levels > 0, "no coarse correction attached");
   TEA_HIP_TRY(hipSetDevice(c->device));
   TEA_FLUSH_HELD_KEEP_U(c);
-  int lens[2] = {c->nspan, c->nspan};
-  TEA_RETURN_IF(launch_mp_r(c, alpha, &lens[0], false));
-  TEA_RETURN_IF(launch_precondition(c, use_net, 1, &lens[1]));
   double out[2];
-  TEA_RETURN_IF(finish_reduction_lens(c, 2, lens, out));
+  if (use_quad_blocks(c, 2) && precondition_quads(c, use_net)) {
+    // Both kernels leave their subtree sums in c->sub (r.r in array 0, z.w
+    // in array 1; nothing between them uses it) for one finisher launch.
+    int len;
+    TEA_RETURN_IF(launch_mp_r(c, alpha, &len, false, false));
+    TEA_RETURN_IF(launch_precondition(c, use_net, 0, &len, 1));
+    TEA_RETURN_IF(launch_span_finish(c, 2, &len, 0, true));
+    TEA_RETURN_IF(finish_reduction_n(c, 2, len, out));
+  } else {
+    int lens[2] = {c->nspan, c->nspan};
+    TEA_RETURN_IF(launch_mp_r(c, alpha, &lens[0], false));
+    TEA_RETURN_IF(launch_precondition(c, use_net, 1, &lens[1]));
+    TEA_RETURN_IF(finish_reduction_lens(c, 2, lens, out));
+  }
   *rr = out[0];
   *zw = out[1];
   return TEA_HIP_OK;

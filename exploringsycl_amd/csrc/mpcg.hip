@@ -853,8 +853,9 @@ int launch_mp_w4(tea_hip_chunk* c, MpArgs a, int kid) {
 // r -= alpha w and the partials of r.r (and, with rw, of r.w in the second
 // array) into c->partials (not waited for); *len is the number of partials
 // left to finish in each array.
-int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw) {
+int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw, bool finish) {
   *len = c->nspan;
+  TEA_CHECK_ARG(finish || use_quad_blocks(c, 2), "launch_mp_r: unfinished sums need quads");
   if (use_quad_blocks(c, 2)) {
     {
       Timed t(c, K_MP_R);
@@ -866,6 +867,7 @@ int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw) {
                            c->geo, c->mp_r, c->mp_w, alpha, c->sub, c->nblk);
     }
     TEA_HIP_TRY(hipGetLastError());
+    if (!finish) return TEA_HIP_OK;
     return launch_span_finish(c, rw ? 2 : 1, len, 0, rw);
   }
   if (use_pair_blocks(c, 2)) {

@@ -898,8 +898,10 @@ int apply_pre_halo(tea_hip_chunk* c);
 int flush_held(tea_hip_chunk* c);
 // Mixed-precision residual update r -= alpha w with the partials of r.r
 // (and, with rw, of r.w as a second array) written to c->partials
-// (mpcg.hip); *len is the number left to finish in each.
-int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw);
+// (mpcg.hip); *len is the number left to finish in each.  Without finish
+// (four-cells-per-lane kernel only) the subtree sums stay in c->sub for the
+// caller's finisher.
+int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw, bool finish = true);
 // The four-cells-per-lane w-step for the arguments a (p_old in
 // mp_p[mp_cur], p_new in the other buffer), timed as kernel kid, with u +=
 // alpha p_old held back over one step: held when nothing is, applied with
