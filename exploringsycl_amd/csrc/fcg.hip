@@ -1163,21 +1163,30 @@ __global__ __launch_bounds__(kGammaThreads) void joint_gamma_kernel(
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   // Each thread's elements are added in turn (the order above).  All five
   // sums' loads of a round are issued before any is added: a round per sum
-  // (loads waited for sum by sum) took 21 us at 4096^2.  kB elements of
-  // each sum per round.
-  constexpr int kB = 4;
+  // (loads waited for sum by sum) took 21 us at 4096^2.  A round takes kB
+  // elements of each tile sum and kBx of the cross sum, whose arrays are 16
+  // times longer (level-1 cells against level-0 tiles): one round at
+  // 4096^2 (four rounds of four took 10 us).
+  constexpr int kB = 4, kBx = 16;
   const int n2 = levels == 2 ? n1 : 0, nxx = levels == 2 ? nx : 0;
-  const int nmax = max(max(n0, n2), nxx);
-  for (int k0 = threadIdx.x; k0 < nmax; k0 += kT * kB) {
-    double t[5][kB];
+  const int nt = max(n0, n2);
+  for (int k0 = threadIdx.x, kx0 = threadIdx.x; k0 < nt || kx0 < nxx;
+       k0 += kT * kB, kx0 += kT * kBx) {
+    double t[4][kB], tx[kBx];
 #pragma unroll
     for (int j = 0; j < kB; ++j) {
       const int k = k0 + kT * j;
-      t[0][j] = k < n0 ? p0[k] : 0.0;
-      t[1][j] = k < n0 ? p0[n0 + k] : 0.0;
-      t[2][j] = k < n2 ? p1[k] : 0.0;
-      t[3][j] = k < n2 ? p1[n1 + k] : 0.0;
-      t[4][j] = k < nxx ? xsum[k] * (double)y1[k] : 0.0;
+      const int a = k < n0 ? k : 0, b = k < n2 ? k : 0;
+      t[0][j] = p0[a];
+      t[1][j] = p0[n0 + a];
+      t[2][j] = n2 > 0 ? p1[b] : 0.0;
+      t[3][j] = n2 > 0 ? p1[n1 + b] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kBx; ++j) {
+      const int k = kx0 + kT * j;
+      const int a = k < nxx ? k : 0;
+      tx[j] = nxx > 0 ? xsum[a] * (double)y1[a] : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < kB; ++j) {
@@ -1190,18 +1199,31 @@ __global__ __launch_bounds__(kGammaThreads) void joint_gamma_kernel(
         v[2] += t[2][j];
         v[3] += t[3][j];
       }
-      if (k < nxx) v[4] += t[4][j];
     }
+#pragma unroll
+    for (int j = 0; j < kBx; ++j)
+      if (kx0 + kT * j < nxx) v[4] += tx[j];
   }
   for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
   __syncthreads();
-  for (int st = kT / 2; st > 0; st >>= 1) {
+  // The LDS tree (element t adds element t + st, st = kT / 2 .. 1): the
+  // steps down to 64 through LDS, the last six by shuffles in wave 0.
+  for (int st = kT / 2; st >= kWave; st >>= 1) {
     if (threadIdx.x < st)
       for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
     __syncthreads();
   }
+  if (threadIdx.x >= kWave) return;
+  double tot[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    double x = sh[q][threadIdx.x];
+#pragma unroll
+    for (int st = kWave / 2; st > 0; st >>= 1) x = x + __shfl_down(x, st);
+    tot[q] = x;
+  }
   if (threadIdx.x != 0) return;
-  const double a0 = sh[0][0], b0 = sh[1][0];
+  const double a0 = tot[0], b0 = tot[1];
   if (levels == 1) {
     const double q = a0 / b0;
     g[0] = (b0 > 0.0 && isfinite(q)) ? (float)q : 0.0f;
@@ -1209,8 +1231,8 @@ __global__ __launch_bounds__(kGammaThreads) void joint_gamma_kernel(
     return;
   }
   const double a1 = c0sq * a0, b11 = c0sq * b0;
-  const double a2 = c1sq * sh[2][0], b22 = c1sq * sh[3][0];
-  const double b12 = c0sq * sh[4][0];
+  const double a2 = c1sq * tot[2], b22 = c1sq * tot[3];
+  const double b12 = c0sq * tot[4];
   const double det = b11 * b22 - b12 * b12;
   double g1, g2;
   if (b11 > 0.0 && b22 > 0.0 && det > 1e-12 * b11 * b22) {
