@@ -489,7 +489,8 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   if (c->spare) (void)hipFree(c->spare);
   if (c->spare2) (void)hipFree(c->spare2);
   if (c->sub) (void)hipFree(c->sub);
-  for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky, c->fcg_z})
+  for (float* b : {c->mp_p[0], c->mp_p[1], c->mp_p[2], c->mp_r, c->mp_w, c->mp_kx, c->mp_ky,
+                   c->fcg_z})
     if (b) (void)hipFree(b);
   if (c->fcg_xsum) (void)hipFree(c->fcg_xsum);
   if (c->fcg_g) (void)hipFree(c->fcg_g);

@@ -1729,7 +1729,7 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
   TEA_FLUSH_HELD_KEEP_U(c);
   // p = z + beta p_old (the mixed-precision kernel with z in place of r),
   // w = A p, and the dots p.w and p.r.
-  MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->fcg_z,
+  MpArgs a{c->mp_p[c->mp_cur], c->mp_p[mp_next(c->mp_cur)], c->fcg_z,
            c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky, c->mp_r,
            alpha, beta, apply_u ? 1 : 0, 1};
   int len = c->nspan;
@@ -1757,7 +1757,7 @@ int tea_hip_fcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u, double beta,
                          c->ngroup == 1 ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
-  c->mp_cur ^= 1;
+  c->mp_cur = mp_next(c->mp_cur);
   double out[2];
   TEA_RETURN_IF(finish_reduction_n(c, 2, len, out));
   *pw = out[0];

@@ -390,8 +390,9 @@ __device__ inline void ring_store(T* f, int i, int kc, int rc, int x, int xl, in
 // at e0 - 1, lane 63 p, r and kx at e0 + 4.  The in-lane dot sums are
 // ((v0 + v1) + (v2 + v3)), tree steps 1 and 2, and the waves store their
 // sums for span_finish_kernel (wave_subtree_quads).  UM selects the u
-// update: none, u += alpha p_old, or u = (u + alpha_h p_h) + alpha p_old (a
-// held update and this step's, each rounded as when applied one by one).
+// update: none, u += alpha p_old, or u = ((u + alpha_h p_h) + alpha_h2
+// p_h2) + alpha p_old (two held updates and this step's, each rounded as
+// when made one by one).
 template <bool DOT2, int UM>
 __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, double* sub, int nblk) {
   const int blk = blockIdx.x;
@@ -435,16 +436,18 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
   const bool inB0 = ub < g.n && interior(g, kB, rB);
   const bool inB1 = ub < g.n && interior(g, kB + 1, rB);
   double2 UA = {}, UB = {};
-  float2 HA = {}, HB = {};
+  float2 HA = {}, HB = {}, GA = {}, GB = {};
   if (UM > 0) {
     UA = ld2_nt(a.u, inA0 || inA1 ? ua : x + 2);
     UB = ld2_nt(a.u, inB0 || inB1 ? ub : x + 2);
   }
-  if (UM == 2) {
+  if (UM == 3) {
     // p_h is the p_new buffer: each of its cells is read here by the wave
     // that overwrites it below, and by no other.
     HA = ld2(a.p_h, inA0 || inA1 ? ua : x + 2);
     HB = ld2(a.p_h, inB0 || inB1 ? ub : x + 2);
+    GA = ld2(a.p_h2, inA0 || inA1 ? ua : x + 2);
+    GB = ld2(a.p_h2, inB0 || inB1 ? ub : x + 2);
   }
   float4 RD = {};
   if (DOT2) RD = ld4(a.rdot, ir);
@@ -533,11 +536,15 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
     const float bx = __shfl(P0.x, sb), by = __shfl(P0.y, sb), bz = __shfl(P0.z, sb),
                 bw = __shfl(P0.w, sb);
     double u0[4] = {UA.x, UA.y, UB.x, UB.y};
-    if (UM == 2) {
+    if (UM == 3) {
       u0[0] = u0[0] + a.alpha_h * (double)HA.x;
       u0[1] = u0[1] + a.alpha_h * (double)HA.y;
       u0[2] = u0[2] + a.alpha_h * (double)HB.x;
       u0[3] = u0[3] + a.alpha_h * (double)HB.y;
+      u0[0] = u0[0] + a.alpha_h2 * (double)GA.x;
+      u0[1] = u0[1] + a.alpha_h2 * (double)GA.y;
+      u0[2] = u0[2] + a.alpha_h2 * (double)GB.x;
+      u0[3] = u0[3] + a.alpha_h2 * (double)GB.y;
     }
     const double un[4] = {u0[0] + a.alpha * (double)(hi ? az : ax),
                           u0[1] + a.alpha * (double)(hi ? aw : ay),
@@ -568,7 +575,7 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
 
 template __global__ void mp_w4_kernel<true, 0>(Geo, MpArgs, double*, int);
 template __global__ void mp_w4_kernel<true, 1>(Geo, MpArgs, double*, int);
-template __global__ void mp_w4_kernel<true, 2>(Geo, MpArgs, double*, int);
+template __global__ void mp_w4_kernel<true, 3>(Geo, MpArgs, double*, int);
 
 
 // r -= alpha w and r.r (with RW also r.w), two cells per lane.
@@ -729,7 +736,7 @@ __global__ __launch_bounds__(kBlock) void mp_convert_all_kernel(
     int n, const double* __restrict__ kx, const double* __restrict__ ky,
     const double* __restrict__ p, const double* __restrict__ r,
     float* __restrict__ kx32, float* __restrict__ ky32,
-    float* __restrict__ p32a, float* __restrict__ p32b,
+    float* __restrict__ p32a, float* __restrict__ p32b, float* __restrict__ p32c,
     float* __restrict__ r32) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -738,6 +745,7 @@ __global__ __launch_bounds__(kBlock) void mp_convert_all_kernel(
   const float pv = (float)p[i];
   p32a[i] = pv;
   p32b[i] = pv;
+  p32c[i] = pv;
   r32[i] = (float)r[i];
 }
 
@@ -793,7 +801,7 @@ struct MpTrueResidualBody {
 int ensure_mp_buffers(tea_hip_chunk* c) {
   if (c->mp_r) return TEA_HIP_OK;
   const size_t bytes = (size_t)c->geo.n * sizeof(float);
-  float** bufs[] = {&c->mp_p[0], &c->mp_p[1], &c->mp_r, &c->mp_w, &c->mp_kx,
+  float** bufs[] = {&c->mp_p[0], &c->mp_p[1], &c->mp_p[2], &c->mp_r, &c->mp_w, &c->mp_kx,
                     &c->mp_ky};
   for (float** b : bufs) {
     if (hipMalloc(b, bytes) != hipSuccess) {
@@ -809,32 +817,41 @@ int ensure_mp_buffers(tea_hip_chunk* c) {
 }  // namespace
 
 int apply_mp_hold(tea_hip_chunk* c) {
-  if (!c->hold) return TEA_HIP_OK;
-  c->hold = false;
-  Timed t(c, K_MP_APPLY_U);
-  MpApplyUBody body{c->field[TEA_FIELD_U], c->mp_p[c->hold_buf], c->hold_alpha, c->geo};
-  hipLaunchKernelGGL((each_kernel<MpApplyUBody>), flat_grid(c->geo.n), dim3(kBlock), 0,
-                     c->stream, c->geo, body);
-  TEA_HIP_TRY(hipGetLastError());
+  const int n = c->hold;
+  c->hold = 0;
+  for (int i = 0; i < n; ++i) {
+    Timed t(c, K_MP_APPLY_U);
+    MpApplyUBody body{c->field[TEA_FIELD_U], c->mp_p[c->hold_buf[i]], c->hold_alpha[i],
+                      c->geo};
+    hipLaunchKernelGGL((each_kernel<MpApplyUBody>), flat_grid(c->geo.n), dim3(kBlock), 0,
+                       c->stream, c->geo, body);
+    TEA_HIP_TRY(hipGetLastError());
+  }
   return TEA_HIP_OK;
 }
 
 int launch_mp_w4(tea_hip_chunk* c, MpArgs a, int kid) {
-  const int dst = c->mp_cur ^ 1;  // a.p_new's buffer
+  const int cur = c->mp_cur, dst = mp_next(cur), prev = mp_next(dst);
   int um = a.apply_u ? 1 : 0;
-  if (c->hold && !(um == 1 && a.apply_p && c->hold_buf == dst))
-    TEA_RETURN_IF(apply_mp_hold(c));
-  if (c->hold) {
-    um = 2;
+  // Held updates are those of the two steps before this one: their p are
+  // in p_new's buffer (the older) and in the buffer before p_old's.
+  const bool ride = um == 1 && a.apply_p &&
+                    (c->hold == 0 || (c->hold == 1 && c->hold_buf[0] == prev) ||
+                     (c->hold == 2 && c->hold_buf[0] == dst && c->hold_buf[1] == prev));
+  if (c->hold && !ride) TEA_RETURN_IF(apply_mp_hold(c));
+  if (c->hold == 2) {
+    um = 3;
     a.p_h = c->mp_p[dst];
-    a.alpha_h = c->hold_alpha;
-    c->hold = false;
+    a.alpha_h = c->hold_alpha[0];
+    a.p_h2 = c->mp_p[prev];
+    a.alpha_h2 = c->hold_alpha[1];
+    c->hold = 0;
   } else if (um == 1 && a.apply_p && c->hold_ok) {
-    // Held: the next step's p_new buffer is this step's p_old.
+    // Held: p_old's buffer is not written until two steps on.
     um = 0;
-    c->hold = true;
-    c->hold_buf = c->mp_cur;
-    c->hold_alpha = a.alpha;
+    c->hold_buf[c->hold] = cur;
+    c->hold_alpha[c->hold] = a.alpha;
+    ++c->hold;
   }
   Timed t(c, kid);
   if (um == 0)
@@ -844,7 +861,7 @@ int launch_mp_w4(tea_hip_chunk* c, MpArgs a, int kid) {
     hipLaunchKernelGGL((mp_w4_kernel<true, 1>), quad_grid(c), dim3(kQuadThreads), 0,
                        c->stream, c->geo, a, c->sub, c->nblk);
   else
-    hipLaunchKernelGGL((mp_w4_kernel<true, 2>), quad_grid(c), dim3(kQuadThreads), 0,
+    hipLaunchKernelGGL((mp_w4_kernel<true, 3>), quad_grid(c), dim3(kQuadThreads), 0,
                        c->stream, c->geo, a, c->sub, c->nblk);
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
@@ -933,7 +950,7 @@ int tea_hip_mpcg_init(tea_hip_chunk* c, int coefficient, double rx, double ry,
     hipLaunchKernelGGL(mp_convert_all_kernel, flat_grid(g.n), dim3(kBlock), 0,
                        c->stream, g.n, f[TEA_FIELD_KX], f[TEA_FIELD_KY],
                        f[TEA_FIELD_P], f[TEA_FIELD_R], c->mp_kx, c->mp_ky,
-                       c->mp_p[0], c->mp_p[1], c->mp_r);
+                       c->mp_p[0], c->mp_p[1], c->mp_p[2], c->mp_r);
     MpConvertBody body{f[TEA_FIELD_R], f[TEA_FIELD_U0]};
     hipLaunchKernelGGL((span_kernel<2, MpConvertBody>), span_grid(c),
                        dim3(kBlock), 0, c->stream, g, body, c->partials,
@@ -957,7 +974,7 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
   TEA_FLUSH_HELD_KEEP_U(c);
   // The two-dot kernel of flexible CG with z = r: p = r + beta p_old,
   // w = A p, p.w and p.r.
-  MpArgs a{c->mp_p[c->mp_cur], c->mp_p[c->mp_cur ^ 1], c->mp_r,
+  MpArgs a{c->mp_p[c->mp_cur], c->mp_p[mp_next(c->mp_cur)], c->mp_r,
            c->field[TEA_FIELD_U], c->mp_w, c->mp_kx, c->mp_ky, c->mp_r,
            alpha, beta, apply_u ? 1 : 0, apply_p ? 1 : 0};
   int len = c->nspan;
@@ -985,7 +1002,7 @@ int tea_hip_mpcg_calc_w(tea_hip_chunk* c, double alpha, int apply_u,
                          c->ngroup == 1 ? 1 : 0);
   }
   TEA_HIP_TRY(hipGetLastError());
-  if (apply_p) c->mp_cur ^= 1;
+  if (apply_p) c->mp_cur = mp_next(c->mp_cur);
   double out[2];
   TEA_RETURN_IF(finish_reduction_n(c, 2, len, out));
   *pw = out[0];

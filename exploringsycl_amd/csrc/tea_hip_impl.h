@@ -582,10 +582,12 @@ struct MpArgs {
   const float* rdot;  // second dot product p.rdot (DOT2 only)
   double alpha, beta;
   int apply_u, apply_p;
-  // A held update u += alpha_h p_h applied first (mp_w4_kernel, UM 2);
-  // p_h is p_new's buffer, read before it is overwritten.
+  // Held updates applied first (mp_w4_kernel, UM 3): u += alpha_h p_h,
+  // then u += alpha_h2 p_h2; p_h is p_new's buffer, read before it is
+  // overwritten.
   const float* p_h;
-  double alpha_h;
+  const float* p_h2;
+  double alpha_h, alpha_h2;
 };
 
 template <int kBatch, bool DOT2>
@@ -599,8 +601,8 @@ __global__ void mp_w2_kernel(Geo g, MpArgs a, double* partials, int nspan,
                              int single_group);
 template <bool DOT2>
 __global__ void mp_w1_kernel(Geo g, MpArgs a, double* sub, int nblk);
-// UM: 0 leaves u alone, 1 adds alpha p_old, 2 adds alpha_h p_h and then
-// alpha p_old.
+// UM: 0 leaves u alone, 1 adds alpha p_old, 3 adds alpha_h p_h, alpha_h2
+// p_h2 and then alpha p_old.
 template <bool DOT2, int UM>
 __global__ void mp_w4_kernel(Geo g, MpArgs a, double* sub, int nblk);
 
@@ -634,6 +636,12 @@ const float* output(tea_nn* n);
 
 // ---------------------------------------------------------------------------
 // The chunk handle.
+
+// The mixed-precision p buffers: a ring of three, so that a w-step can
+// hold the u updates of two steps back (their p are the two buffers it
+// does not write).
+constexpr int kMpRing = 3;
+inline int mp_next(int i) { return i + 1 == kMpRing ? 0 : i + 1; }
 
 // One coarse level of the learned preconditioner.
 struct FcgLevel {
@@ -711,18 +719,20 @@ struct tea_hip_chunk {
   bool pend_u = false, pend_p = false, pend_halo = false;
   double pend_alpha = 0.0, pend_beta = 0.0;
 
-  // Mixed-precision CG: FP32 p (two buffers, mp_cur is current), r, w and
+  // Mixed-precision CG: FP32 p (a ring of three buffers, mp_cur is
+  // current; each w-step writes the next), r, w and
   // face coefficients; allocated by the first tea_hip_mpcg_init.
-  float* mp_p[2] = {nullptr, nullptr};
+  float* mp_p[kMpRing] = {nullptr, nullptr, nullptr};
   int mp_cur = 0;
-  // u += hold_alpha p, p in mp_p[hold_buf], reported done to the host but
-  // held back (TEA_HIP_HOLD_U=0 turns it off): the next four-cells-per-lane
-  // w-step, which overwrites that p buffer, reads it first and applies both
-  // updates in one pass over u; any other entry point applies it first.
+  // Up to two updates u += hold_alpha[i] p, p in mp_p[hold_buf[i]] (oldest
+  // first), reported done to the host but held back (TEA_HIP_HOLD_U=0 turns
+  // this off): the four-cells-per-lane w-step that overwrites the oldest's
+  // p buffer reads both held p first and makes the held updates and its own
+  // in one pass over u; any other entry point makes them first.
   bool hold_ok = true;
-  bool hold = false;
-  int hold_buf = 0;
-  double hold_alpha = 0.0;
+  int hold = 0;
+  int hold_buf[2] = {0, 0};
+  double hold_alpha[2] = {0.0, 0.0};
   float* mp_r = nullptr;
   float* mp_w = nullptr;
   float* mp_kx = nullptr;
@@ -903,12 +913,13 @@ int flush_held(tea_hip_chunk* c);
 // caller's finisher.
 int launch_mp_r(tea_hip_chunk* c, double alpha, int* len, bool rw, bool finish = true);
 // The four-cells-per-lane w-step for the arguments a (p_old in
-// mp_p[mp_cur], p_new in the other buffer), timed as kernel kid, with u +=
-// alpha p_old held back over one step: held when nothing is, applied with
-// the held update when that one's p is this step's p_new buffer; a held
-// update that cannot ride along runs first on its own (mpcg.hip).
+// mp_p[mp_cur], p_new in mp_p[mp_next(mp_cur)]), timed as kernel kid, with
+// u += alpha p_old held back over two steps: held while fewer than two
+// are, made with the two held updates when the oldest one's p is this
+// step's p_new buffer; held updates that cannot ride along run first on
+// their own (mpcg.hip).
 int launch_mp_w4(tea_hip_chunk* c, MpArgs a, int kid);
-// Applies a held-back u += hold_alpha p on its own (mpcg.hip).
+// Makes the held-back u updates on their own, oldest first (mpcg.hip).
 int apply_mp_hold(tea_hip_chunk* c);
 
 // One reflective-halo launch (all flagged fields and external faces).

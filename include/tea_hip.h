@@ -313,12 +313,11 @@ int tea_hip_mpcg_init(tea_hip_chunk* chunk, int coefficient, double rx,
 
 /* If apply_u: u += alpha p_old (FP64).  If apply_p: p = beta p_old + r.
  * Then w = A p (FP32), *pw = p.w and *pr = p.r.  With four cells per lane
- * (x a multiple of 4) the u update of every other call is held back and
- * made by the next call in the same pass over u, rounded as if made one by
- * one (TEA_HIP_HOLD_U=0 makes each at once); every other entry point
- * except tea_hip_mpcg_calc_r and the FCG steps makes a held update first,
- * and tea_hip_flush does before u is read through tea_hip_field's
- * address. */
+ * (x a multiple of 4) two calls in turn hold their u update back and the
+ * third makes all three in one pass over u, rounded as if made one by one
+ * (TEA_HIP_HOLD_U=0 makes each at once); every other entry point except
+ * tea_hip_mpcg_calc_r and the FCG steps makes held updates first, and
+ * tea_hip_flush does before u is read through tea_hip_field's address. */
 int tea_hip_mpcg_calc_w(tea_hip_chunk* chunk, double alpha, int apply_u,
                         double beta, int apply_p, double* pw, double* pr);
 

@@ -1,6 +1,6 @@
 """The mixed-precision u update held over two w-steps: with four cells per
-lane the library holds u += alpha p back at every other mpcg_calc_w /
-fcg_calc_w and makes it with the next one in the same pass over u.  These
+lane two mpcg_calc_w / fcg_calc_w calls in turn hold their u += alpha p
+back and the third makes all three in one pass over u.  These
 tests run the drivers' call sequences with the hold on and off
 (TEA_HIP_HOLD_U=0) and require the same scalars and fields bit for bit, also
 when other calls come between the steps (reads of u, refreshes after an odd
@@ -80,16 +80,17 @@ def _mpcg(x, y, seed, hold, iters, refresh_at=(), stray=None):
 @pytest.mark.parametrize("cells", [(124, 60), (252, 131), (121, 50)])
 def test_held_u_is_the_step_by_step_update_bit_for_bit(cells):
     """25 steps with refreshes after steps 6 and 12 and at the end; x = 128
-    and 256 take the four-cell kernels (and the hold), x = 125 not.  Steps
-    with a u update alternate between holding it and making two: the
-    refresh after step 6 finds none held, those after 12 and 24 one each."""
+    and 256 take the four-cell kernels (and the hold), x = 125 not.  Of the
+    steps with a u update, two hold theirs and the third makes all three:
+    the refresh after step 6 finds none held, those after 12 and 24 two
+    each, each made by a launch of its own."""
     x, y = cells[0] + 2 * H, cells[1] + 2 * H
     s0, u0, n0 = _mpcg(x, y, 3, False, 25, refresh_at=(6, 12))
     s1, u1, n1 = _mpcg(x, y, 3, True, 25, refresh_at=(6, 12))
     assert _same(s0, s1)
     assert _same(u0, u1)
     assert n0 == 3
-    assert n1 == (n0 + 2 if x % 4 == 0 else n0), (n0, n1)
+    assert n1 == (n0 + 4 if x % 4 == 0 else n0), (n0, n1)
 
 
 def test_held_u_is_made_before_other_calls_read_u():
