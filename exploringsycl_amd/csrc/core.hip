@@ -501,7 +501,7 @@ int tea_hip_chunk_destroy(tea_hip_chunk* c) {
   if (c->fcg_fork) (void)hipEventDestroy(c->fcg_fork);
   if (c->fcg_join) (void)hipEventDestroy(c->fcg_join);
   for (FcgLevel& L : c->fcg_lv) {
-    for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale, L.y})
+    for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.y})
       if (b) (void)hipFree(b);
     if (L.rms) (void)hipFree(L.rms);
     if (L.tpart) (void)hipFree(L.tpart);

@@ -66,93 +66,102 @@ struct JacobiBody {
 constexpr int kWin = 6;         // the network's receptive-field radius
 constexpr float kFloor = 1e-3f;
 
-// Row pass of the window sums; the grid's extra last block computes the
-// RMS scale[0] (of D^-1/2 rbar over the coarse grid, from the restriction's
-// per-block partials) at the same time, since the row pass does not need it.
-__global__ __launch_bounds__(256) void box_rows_kernel(int hc, int wc,
-                                                       const float* __restrict__ rbar,
-                                                       const float* __restrict__ dis,
-                                                       float* __restrict__ tmp,
-                                                       const double* __restrict__ part,
-                                                       int npart,
-                                                       float* __restrict__ scale) {
-  if (blockIdx.x == gridDim.x - 1) {
-    __shared__ double sh[256];
-    // Thread t adds partials t, t + 256, ... in turn, loaded 8 at a time
-    // (one load per add waited for each in turn).
-    double v = 0.0;
-    for (int i0 = threadIdx.x; i0 < npart; i0 += 256 * 8) {
-      double t8[8];
+// sigma and the network's packed input for one 16 x 16 tile per 256-thread
+// block: the RMS floor from the restriction's per-block partials (thread t
+// adds partials t, t + 256, ... in turn, then the LDS tree), rbar and
+// D^-1/2 staged on the tile and kWin cells around it, the row pass of the
+// window sums (FP64, rounded to FP32) on the staged rows, the column pass
+// (FP64 over the 13 row sums), sigma = max(RMS of the window, kFloor times
+// the grid's RMS), and the input: channel 0 = D^-1/2 rbar / sigma (0 where
+// sigma is 0), channels 1, 2 = the features.  Cells beyond the grid's edge
+// add nothing (adding +0 to a sum of squares, which is >= +0, is skipping
+// them).  Each block redoes the RMS and the row sums of the 12 rows beyond
+// its tile: cheaper than a second launch (a row-pass and a column-pass
+// launch took 4.1 + 4.9 us at 512^2).
+constexpr int kBoxT = 16;
+constexpr int kBoxS = kBoxT + 2 * kWin;  // 28: staged region side
+__global__ __launch_bounds__(256) void box_tile_kernel(
+    int hc, int wc, const float* __restrict__ rbar, const float* __restrict__ dis,
+    const double* __restrict__ part, int npart, const float* __restrict__ feat,
+    float* __restrict__ sig, bf16* __restrict__ in0) {
+  constexpr int kN = kBoxS * kBoxS;
+  constexpr int kIt = (kN + 255) / 256;
+  __shared__ float s_rb[kN], s_ds[kN];
+  __shared__ float s_tmp[kBoxS * kBoxT];
+  __shared__ double s_sc[256];
+  __shared__ float s_floor;
+  const int tid = threadIdx.x;
+  const int X0 = blockIdx.x * kBoxT, Y0 = blockIdx.y * kBoxT;
+  const long n = (long)hc * wc;
+  // Every global load first: the RMS partials (8 per thread and round),
+  // the staged region, the tile cell's features.
+  double v = 0.0;
+  for (int i0 = tid; i0 < npart; i0 += 256 * 8) {
+    double t8[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int i = i0 + 256 * j;
-        t8[j] = i < npart ? part[i] : 0.0;
-      }
+    for (int j = 0; j < 8; ++j) t8[j] = part[i0 + 256 * j < npart ? i0 + 256 * j : 0];
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (i0 + 256 * j < npart) v += t8[j];
-    }
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) scale[0] = (float)sqrt(sh[0] / (double)((long)hc * wc));
-    return;
+    for (int j = 0; j < 8; ++j)
+      if (i0 + 256 * j < npart) v += t8[j];
   }
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= hc * wc) return;
-  const int X = idx % wc;
-  // All 26 loads first (clamped indices), then the sum in window order;
-  // adding 0 for a cell beyond the edge leaves the sum (of squares, >= +0)
-  // as skipping it does.
-  float rb[2 * kWin + 1], ds[2 * kWin + 1];
+  float rb[kIt], ds[kIt];
 #pragma unroll
-  for (int d = -kWin; d <= kWin; ++d) {
-    const int j = X + d < 0 || X + d >= wc ? idx : idx + d;
-    rb[d + kWin] = rbar[j];
-    ds[d + kWin] = dis[j];
+  for (int k = 0; k < kIt; ++k) {
+    const int q = tid + 256 * k;
+    const int X = X0 - kWin + q % kBoxS, Y = Y0 - kWin + q / kBoxS;
+    const bool ok = q < kN && X >= 0 && X < wc && Y >= 0 && Y < hc;
+    const long j = ok ? (long)Y * wc + X : 0;
+    rb[k] = rbar[j];
+    ds[k] = dis[j];
+    if (!ok) rb[k] = ds[k] = 0.0f;
   }
+  const int tx = tid % kBoxT, ty = tid / kBoxT;
+  const int X = X0 + tx, Y = Y0 + ty;
+  const bool in = X < wc && Y < hc;
+  const long i = in ? (long)Y * wc + X : 0;
+  const float f0 = feat[i], f1 = feat[n + i];
+  s_sc[tid] = v;
+#pragma unroll
+  for (int k = 0; k < kIt; ++k)
+    if (tid + 256 * k < kN) {
+      s_rb[tid + 256 * k] = rb[k];
+      s_ds[tid + 256 * k] = ds[k];
+    }
+  __syncthreads();
+  // Row pass: staged row r, tile column c (region column c + kWin).
+  for (int q = tid; q < kBoxS * kBoxT; q += 256) {
+    const int r = q / kBoxT, c = q - r * kBoxT + kWin;
+    const int Xc = X0 - kWin + c;
+    double s = 0.0;
+#pragma unroll
+    for (int d = -kWin; d <= kWin; ++d) {
+      const double w = (double)s_rb[r * kBoxS + c + d] * (double)s_ds[r * kBoxS + c + d];
+      s += Xc + d < 0 || Xc + d >= wc ? 0.0 : w * w;
+    }
+    s_tmp[q] = (float)s;
+  }
+  // The LDS tree of the RMS sum (element t adds element t + st): steps 128
+  // and 64 through LDS, 32 .. 1 by shuffles in wave 0.
+  if (tid < 128) s_sc[tid] += s_sc[tid + 128];
+  __syncthreads();
+  if (tid < 64) {
+    double x = s_sc[tid] + s_sc[tid + 64];
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) x = x + __shfl_down(x, st);
+    if (tid == 0) s_floor = kFloor * (float)sqrt(x / (double)n);
+  }
+  __syncthreads();
+  if (!in) return;
+  const float floor_s = s_floor;
   double s = 0.0;
-#pragma unroll
-  for (int d = -kWin; d <= kWin; ++d) {
-    const double v = (double)rb[d + kWin] * (double)ds[d + kWin];
-    s += X + d < 0 || X + d >= wc ? 0.0 : v * v;
-  }
-  tmp[idx] = (float)s;
-}
-
-// Column pass: sigma (floored), then the cell's network input: channel 0 =
-// D^-1/2 rbar / sigma (0 where sigma is 0), channels 1, 2 = features.
-__global__ __launch_bounds__(256) void box_cols_kernel(int hc, int wc,
-                                                       const float* __restrict__ tmp,
-                                                       const float* __restrict__ scale,
-                                                       float* __restrict__ sig,
-                                                       const float* __restrict__ rbar,
-                                                       const float* __restrict__ dis,
-                                                       const float* __restrict__ feat,
-                                                       bf16* __restrict__ in0) {
-  // Rows in one contiguous range per XCD (xcd_span): the 13 rows a column
-  // window reads are then mostly in the block's own XCD's L2.
-  const int idx = xcd_span(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  if (idx >= hc * wc) return;
-  const int X = idx % wc, Y = idx / wc;
-  // All 13 loads first (clamped rows), then the sum in window order (the
-  // row sums are >= +0, so adding 0 beyond the edge is skipping).
-  float t[2 * kWin + 1];
 #pragma unroll
   for (int d = -kWin; d <= kWin; ++d)
-    t[d + kWin] = tmp[Y + d < 0 || Y + d >= hc ? idx : idx + d * wc];
-  const float rb = rbar[idx], ds = dis[idx], f0 = feat[idx], f1 = feat[hc * wc + idx];
-  double s = 0.0;
-#pragma unroll
-  for (int d = -kWin; d <= kWin; ++d) s += Y + d < 0 || Y + d >= hc ? 0.0 : (double)t[d + kWin];
-  const float sg = fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))),
-                         kFloor * scale[0]);
-  sig[idx] = sg;
+    s += Y + d < 0 || Y + d >= hc ? 0.0 : (double)s_tmp[(ty + kWin + d) * kBoxT + tx];
+  const float sg = fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))), floor_s);
+  sig[i] = sg;
+  const int vi = (ty + kWin) * kBoxS + tx + kWin;
   bf16x8 o = bf16x8{};
-  o[0] = sg > 0.0f ? (bf16)((rb * ds) / sg) : (bf16)0.0f;
+  o[0] = sg > 0.0f ? (bf16)((s_rb[vi] * s_ds[vi]) / sg) : (bf16)0.0f;
   o[1] = (bf16)f0;
   o[2] = (bf16)f1;
   *reinterpret_cast<bf16x8*>(in0 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * 8) = o;
@@ -627,7 +636,7 @@ __global__ __launch_bounds__(256) void sweeps_kernel(int hc, int wc, int sweeps,
 
 // ---- the learned coarse solve of one level in one launch -----------------
 // net_level_kernel computes, for one 16 x 16 tile of a coarse level, what
-// box_rows_kernel, box_cols_kernel, the five hidden layers of the network
+// box_tile_kernel, the five hidden layers of the network
 // (nn.hip: conv_*_kernel), out_tile_kernel and the coarse dots compute for
 // it, keeping every intermediate in LDS: the staged rbar and D^-1/2 (the
 // tile and 13 cells around it), the local scale sigma and the packed BF16
@@ -837,7 +846,7 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
   auto in_grid = [&](int X, int Y) { return X >= 0 && X < wc && Y >= 0 && Y < hc; };
 
   // The RMS of D^-1/2 rbar over the grid from the restriction's partials,
-  // in box_rows_kernel's order (thread t of 256 adds t, t + 256, ...).
+  // in box_tile_kernel's order (thread t of 256 adds t, t + 256, ...).
   __shared__ double s_scale[256];
   __shared__ float s_scale0;
   if (tid < 256) {
@@ -871,7 +880,7 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
     __syncthreads();
   }
   if (tid == 0) s_scale0 = (float)sqrt(s_scale[0] / (double)n);
-  // Row pass of the window sums (box_rows_kernel): every staged row, the
+  // Row pass of the window sums (box_tile_kernel's): every staged row, the
   // input region's columns.
   for (int q = tid; q < kSV * kS0; q += kThreads) {
     const int rr = q / kS0, cc = q - rr * kS0 + kWin;
@@ -885,7 +894,7 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
     v_tmp[q] = (float)s;
   }
   __syncthreads();
-  // Column pass, sigma and the packed input (box_cols_kernel), zero outside
+  // Column pass, sigma and the packed input (box_tile_kernel's), zero outside
   // the grid.
   const float floor_s = kFloor * s_scale0;
   for (int q = tid; q < kS0 * kS0; q += kThreads) {
@@ -1011,8 +1020,8 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
   }
 }
 
-// The end of net_level_kernel for the per-layer pipeline (box_rows_kernel,
-// box_cols_kernel and the network's hidden layers): the output layer on the
+// The end of net_level_kernel for the per-layer pipeline (box_tile_kernel
+// and the network's hidden layers): the output layer on the
 // tile and its ring, from the last hidden layer's activations staged in LDS
 // (the tile and 2 cells around it), by fused_out as in net_level_kernel, so
 // N is bitwise its; then y = N sigma D^-1/2 on the tile and its ring, and
@@ -1290,7 +1299,7 @@ static NetLevelFn net_level_fn(NetShape s) {
 }
 
 // Whether the learned coarse solve runs as the per-layer pipeline (the
-// default: box_rows, box_cols, the network's hidden layers, out_tile_kernel)
+// default: box_tile_kernel, the network's hidden layers, out_tile_kernel)
 // or as one fused launch per level (TEA_FCG_NET=fused), which gives the same
 // y bit for bit.  The fused kernel recomputes each tile's halo at every layer
 // (2.1 times the pixels of the per-layer pipeline with 16 x 16 tiles) and
@@ -1317,11 +1326,10 @@ static int launch_net_level(tea_hip_chunk* c, int l, hipStream_t st) {
                        L.hc, L.wc, L.rbar, L.dis, L.feat, L.kc, L.rms, L.nrms,
                        nn::packed_weights(L.net), L.y, L.tpart, L.ntiles, xs);
   } else {
-    const int nblk = (L.hc * L.wc + 255) / 256;
-    hipLaunchKernelGGL(box_rows_kernel, dim3(nblk + 1), dim3(256), 0, st, L.hc, L.wc,
-                       L.rbar, L.dis, L.tmp, L.rms, L.nrms, L.scale);
-    hipLaunchKernelGGL(box_cols_kernel, dim3(nblk), dim3(256), 0, st, L.hc, L.wc, L.tmp,
-                       L.scale, L.sig, L.rbar, L.dis, L.feat, nn::input_tensor(L.net));
+    const dim3 boxes((unsigned)(L.wc + kBoxT - 1) / kBoxT,
+                     (unsigned)(L.hc + kBoxT - 1) / kBoxT);
+    hipLaunchKernelGGL(box_tile_kernel, boxes, dim3(256), 0, st, L.hc, L.wc, L.rbar, L.dis,
+                       L.rms, L.nrms, L.feat, L.sig, nn::input_tensor(L.net));
     TEA_RETURN_IF(nn::forward_hidden_on_stream(L.net, st));
     const nn::PackedWeights wts = nn::packed_weights(L.net);
     hipLaunchKernelGGL(out_tile_kernel, tiles, dim3(256), 0, st, L.hc, L.wc,
@@ -1498,7 +1506,7 @@ static int launch_precondition(tea_hip_chunk* c, int use_net, int slot, int* len
 }
 
 static void free_level(FcgLevel& L) {
-  for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.tmp, L.scale, L.y})
+  for (float* b : {L.feat, L.rbar, L.kc, L.dis, L.sig, L.y})
     if (b) (void)hipFree(b);
   if (L.rms) (void)hipFree(L.rms);
   if (L.tpart) (void)hipFree(L.tpart);
@@ -1629,12 +1637,10 @@ int tea_hip_fcg_attach(tea_hip_chunk* c, const float* params, size_t count,
         hipMalloc(&L.kc, 2 * cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.dis, cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.sig, cn * sizeof(float)) != hipSuccess ||
-        hipMalloc(&L.tmp, cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.rbar, cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.y, cn * sizeof(float)) != hipSuccess ||
         hipMalloc(&L.rms, (size_t)L.nrms * sizeof(double)) != hipSuccess ||
-        hipMalloc(&L.tpart, 2 * (size_t)L.ntiles * sizeof(double)) != hipSuccess ||
-        hipMalloc(&L.scale, 2 * sizeof(float)) != hipSuccess)
+        hipMalloc(&L.tpart, 2 * (size_t)L.ntiles * sizeof(double)) != hipSuccess)
       return fail();
     c->fcg_levels = l + 1;
   }

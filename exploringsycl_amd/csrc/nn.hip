@@ -174,6 +174,47 @@ __device__ inline void stage_rows(bf16x8* sm, const bf16* __restrict__ inb, int 
     if (threadIdx.x + 256 * k < kTotal) sm[slot[k]] = v[k];
 }
 
+// Stores one 32-pixel row of a layer's MFMA output (bias, ReLU, BF16) at
+// px, the row's pixel of lane col: lane (hf, col) holds channels 8 g + 4 hf
+// + 0..3, g = 0..3, of pixel col.  Lanes col and col + 32 swap half of
+// theirs, so that each holds 16 consecutive channels (32 bytes at 16 hf),
+// stored with two 16-byte stores (four 8-byte stores at a 64-byte stride
+// each wrote the first layer's output at 1.5 TB/s).
+__device__ inline void store_out_row(bf16* px, const f32x16& acc, const float (&bco)[16],
+                                     int hf) {
+  union Pack {
+    bf16x4 v;
+    unsigned u[2];
+  };
+  Pack o[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float vv = acc[4 * g + q] + bco[4 * g + q];
+      o[g].v[q] = (bf16)(vv > 0.0f ? vv : 0.0f);
+    }
+  // hf 0 keeps channels 0-3 and 8-11 and takes 4-7 and 12-15 (the
+  // partner's g 0, 1); hf 1 keeps 20-23 and 28-31 and takes 16-19 and
+  // 24-27 (the partner's g 2, 3).
+  Pack in[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int w = 0; w < 2; ++w) in[j].u[w] = __shfl_xor(hf ? o[j].u[w] : o[2 + j].u[w], 32);
+  union Pack8 {
+    bf16x8 v;
+    bf16x4 h[2];
+  };
+  Pack8 lo, hi;
+  lo.h[0] = hf ? in[0].v : o[0].v;
+  lo.h[1] = hf ? o[2].v : in[0].v;
+  hi.h[0] = hf ? in[1].v : o[1].v;
+  hi.h[1] = hf ? o[3].v : in[1].v;
+  *reinterpret_cast<bf16x8*>(px + 16 * hf) = lo.v;
+  *reinterpret_cast<bf16x8*>(px + 16 * hf + 8) = hi.v;
+}
+
 // The (segment, row group, sample) of this block in a (W / 128, H / R, B)
 // grid, numbered so that each XCD takes one contiguous range of row groups:
 // the hardware deals blocks round-robin to the 8 XCDs, which put the row
@@ -248,17 +289,7 @@ __global__ __launch_bounds__(256) void conv_fwd_lds_kernel(
   for (int t = 0; t < NS; ++t)
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[t], Bv[t], acc, 0, 0, 0);
   const size_t pix = (size_t)b * plane + (size_t)(y + 1) * Wp + (x0 + xl + 1);
-#pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const int co0 = 8 * g + 4 * hf;
-    bf16x4 o;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float vv = acc[4 * g + q] + bco[4 * g + q];
-      o[q] = (bf16)(vv > 0.0f ? vv : 0.0f);
-    }
-    *reinterpret_cast<bf16x4*>(out + pix * kC + co0) = o;
-  }
+  store_out_row(out + pix * kC, acc, bco, hf);
 }
 
 // conv_fwd_lds_kernel over R output rows per block: the block stages R + 2
@@ -321,17 +352,7 @@ __global__ __launch_bounds__(256) void conv_fwd_rows_kernel(
     for (int t = 0; t < NS; ++t)
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[t], Bv[t], acc, 0, 0, 0);
     const size_t pix = (size_t)b * plane + (size_t)(y0 + r + 1) * Wp + (x0 + xl + 1);
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int co0 = 8 * g + 4 * hf;
-      bf16x4 o;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float vv = acc[4 * g + q] + bco[4 * g + q];
-        o[q] = (bf16)(vv > 0.0f ? vv : 0.0f);
-      }
-      *reinterpret_cast<bf16x4*>(out + pix * kC + co0) = o;
-    }
+    store_out_row(out + pix * kC, acc, bco, hf);
   }
 }
 

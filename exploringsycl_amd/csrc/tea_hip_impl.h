@@ -652,8 +652,6 @@ struct FcgLevel {
   float* kc = nullptr;        // [2][hc][wc] coarse left / bottom face coefficients
   float* dis = nullptr;       // [hc][wc] D^-1/2 of the coarse operator
   float* sig = nullptr;       // [hc][wc] local input scale of the network
-  float* tmp = nullptr;       // [hc][wc] row pass of the local scale
-  float* scale = nullptr;     // {RMS of D^-1/2 rbar} (per-layer pipeline)
   double* rms = nullptr;      // per-restriction-block sums of (D^-1/2 rbar)^2
   int nrms = 0;
   // The coarse solve's output y (network or Jacobi sweeps) and the
