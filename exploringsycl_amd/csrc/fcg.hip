@@ -66,107 +66,6 @@ struct JacobiBody {
 constexpr int kWin = 6;         // the network's receptive-field radius
 constexpr float kFloor = 1e-3f;
 
-// sigma and the network's packed input for one 16 x 16 tile per 256-thread
-// block: the RMS floor from the restriction's per-block partials (thread t
-// adds partials t, t + 256, ... in turn, then the LDS tree), rbar and
-// D^-1/2 staged on the tile and kWin cells around it, the row pass of the
-// window sums (FP64, rounded to FP32) on the staged rows, the column pass
-// (FP64 over the 13 row sums), sigma = max(RMS of the window, kFloor times
-// the grid's RMS), and the input: channel 0 = D^-1/2 rbar / sigma (0 where
-// sigma is 0), channels 1, 2 = the features.  Cells beyond the grid's edge
-// add nothing (adding +0 to a sum of squares, which is >= +0, is skipping
-// them).  Each block redoes the RMS and the row sums of the 12 rows beyond
-// its tile: cheaper than a second launch (a row-pass and a column-pass
-// launch took 4.1 + 4.9 us at 512^2).
-constexpr int kBoxT = 16;
-constexpr int kBoxS = kBoxT + 2 * kWin;  // 28: staged region side
-__global__ __launch_bounds__(256) void box_tile_kernel(
-    int hc, int wc, const float* __restrict__ rbar, const float* __restrict__ dis,
-    const double* __restrict__ part, int npart, const float* __restrict__ feat,
-    float* __restrict__ sig, bf16* __restrict__ in0) {
-  constexpr int kN = kBoxS * kBoxS;
-  constexpr int kIt = (kN + 255) / 256;
-  __shared__ float s_rb[kN], s_ds[kN];
-  __shared__ float s_tmp[kBoxS * kBoxT];
-  __shared__ double s_sc[256];
-  __shared__ float s_floor;
-  const int tid = threadIdx.x;
-  const int X0 = blockIdx.x * kBoxT, Y0 = blockIdx.y * kBoxT;
-  const long n = (long)hc * wc;
-  // Every global load first: the RMS partials (8 per thread and round),
-  // the staged region, the tile cell's features.
-  double v = 0.0;
-  for (int i0 = tid; i0 < npart; i0 += 256 * 8) {
-    double t8[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) t8[j] = part[i0 + 256 * j < npart ? i0 + 256 * j : 0];
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (i0 + 256 * j < npart) v += t8[j];
-  }
-  float rb[kIt], ds[kIt];
-#pragma unroll
-  for (int k = 0; k < kIt; ++k) {
-    const int q = tid + 256 * k;
-    const int X = X0 - kWin + q % kBoxS, Y = Y0 - kWin + q / kBoxS;
-    const bool ok = q < kN && X >= 0 && X < wc && Y >= 0 && Y < hc;
-    const long j = ok ? (long)Y * wc + X : 0;
-    rb[k] = rbar[j];
-    ds[k] = dis[j];
-    if (!ok) rb[k] = ds[k] = 0.0f;
-  }
-  const int tx = tid % kBoxT, ty = tid / kBoxT;
-  const int X = X0 + tx, Y = Y0 + ty;
-  const bool in = X < wc && Y < hc;
-  const long i = in ? (long)Y * wc + X : 0;
-  const float f0 = feat[i], f1 = feat[n + i];
-  s_sc[tid] = v;
-#pragma unroll
-  for (int k = 0; k < kIt; ++k)
-    if (tid + 256 * k < kN) {
-      s_rb[tid + 256 * k] = rb[k];
-      s_ds[tid + 256 * k] = ds[k];
-    }
-  __syncthreads();
-  // Row pass: staged row r, tile column c (region column c + kWin).
-  for (int q = tid; q < kBoxS * kBoxT; q += 256) {
-    const int r = q / kBoxT, c = q - r * kBoxT + kWin;
-    const int Xc = X0 - kWin + c;
-    double s = 0.0;
-#pragma unroll
-    for (int d = -kWin; d <= kWin; ++d) {
-      const double w = (double)s_rb[r * kBoxS + c + d] * (double)s_ds[r * kBoxS + c + d];
-      s += Xc + d < 0 || Xc + d >= wc ? 0.0 : w * w;
-    }
-    s_tmp[q] = (float)s;
-  }
-  // The LDS tree of the RMS sum (element t adds element t + st): steps 128
-  // and 64 through LDS, 32 .. 1 by shuffles in wave 0.
-  if (tid < 128) s_sc[tid] += s_sc[tid + 128];
-  __syncthreads();
-  if (tid < 64) {
-    double x = s_sc[tid] + s_sc[tid + 64];
-#pragma unroll
-    for (int st = 32; st > 0; st >>= 1) x = x + __shfl_down(x, st);
-    if (tid == 0) s_floor = kFloor * (float)sqrt(x / (double)n);
-  }
-  __syncthreads();
-  if (!in) return;
-  const float floor_s = s_floor;
-  double s = 0.0;
-#pragma unroll
-  for (int d = -kWin; d <= kWin; ++d)
-    s += Y + d < 0 || Y + d >= hc ? 0.0 : (double)s_tmp[(ty + kWin + d) * kBoxT + tx];
-  const float sg = fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))), floor_s);
-  sig[i] = sg;
-  const int vi = (ty + kWin) * kBoxS + tx + kWin;
-  bf16x8 o = bf16x8{};
-  o[0] = sg > 0.0f ? (bf16)((s_rb[vi] * s_ds[vi]) / sg) : (bf16)0.0f;
-  o[1] = (bf16)f0;
-  o[2] = (bf16)f1;
-  *reinterpret_cast<bf16x8*>(in0 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * 8) = o;
-}
-
 // One coarse term of the learned preconditioner at a fine cell: g[0] times
 // y at the coarse cell ((kk-h) >> shift, (jj-h) >> shift).
 struct CoarseTerm {
@@ -636,9 +535,9 @@ __global__ __launch_bounds__(256) void sweeps_kernel(int hc, int wc, int sweeps,
 
 // ---- the learned coarse solve of one level in one launch -----------------
 // net_level_kernel computes, for one 16 x 16 tile of a coarse level, what
-// box_tile_kernel, the five hidden layers of the network
-// (nn.hip: conv_*_kernel), out_tile_kernel and the coarse dots compute for
-// it, keeping every intermediate in LDS: the staged rbar and D^-1/2 (the
+// box_conv_kernel (with the first layer), the other four hidden layers of
+// the network (nn.hip: conv_*_kernel), out_tile_kernel and the coarse dots
+// compute for it, keeping every intermediate in LDS: the staged rbar and D^-1/2 (the
 // tile and 13 cells around it), the local scale sigma and the packed BF16
 // input (7 cells around it: the network's receptive field plus the ring of
 // T y), and the activations of each layer on its shrinking region (28^2,
@@ -846,7 +745,7 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
   auto in_grid = [&](int X, int Y) { return X >= 0 && X < wc && Y >= 0 && Y < hc; };
 
   // The RMS of D^-1/2 rbar over the grid from the restriction's partials,
-  // in box_tile_kernel's order (thread t of 256 adds t, t + 256, ...).
+  // in box_conv_kernel's order (thread t of 256 adds t, t + 256, ...).
   __shared__ double s_scale[256];
   __shared__ float s_scale0;
   if (tid < 256) {
@@ -880,7 +779,7 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
     __syncthreads();
   }
   if (tid == 0) s_scale0 = (float)sqrt(s_scale[0] / (double)n);
-  // Row pass of the window sums (box_tile_kernel's): every staged row, the
+  // Row pass of the window sums (box_conv_kernel's): every staged row, the
   // input region's columns.
   for (int q = tid; q < kSV * kS0; q += kThreads) {
     const int rr = q / kS0, cc = q - rr * kS0 + kWin;
@@ -894,7 +793,7 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
     v_tmp[q] = (float)s;
   }
   __syncthreads();
-  // Column pass, sigma and the packed input (box_tile_kernel's), zero outside
+  // Column pass, sigma and the packed input (box_conv_kernel's), zero outside
   // the grid.
   const float floor_s = kFloor * s_scale0;
   for (int q = tid; q < kS0 * kS0; q += kThreads) {
@@ -1020,8 +919,147 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
   }
 }
 
-// The end of net_level_kernel for the per-layer pipeline (box_tile_kernel
-// and the network's hidden layers): the output layer on the
+// The start of the per-layer pipeline, for one 16 x 16 tile per 256-thread
+// block: sigma and the network's packed input on the tile and its ring, and
+// the first layer on the tile from that input in LDS.
+// - The RMS floor from the restriction's per-block partials (thread t adds
+//   partials t, t + 256, ... in turn, then the LDS tree).
+// - rbar and D^-1/2 staged on the tile and 7 cells around it; the row pass
+//   of the window sums (FP64, rounded to FP32) on the staged rows over the
+//   tile's and its ring's columns; the column pass (FP64 over the 13 row
+//   sums); sigma = max(RMS of the window, kFloor times the grid's RMS); the
+//   input: channel 0 = D^-1/2 rbar / sigma (0 where sigma is 0), channels
+//   1, 2 = the features, zero outside the grid.  Cells beyond the grid's
+//   edge add nothing (adding +0 to a sum of squares, which is >= +0, is
+//   skipping them).
+// - Layer 1 (8 -> 32 channels) on the tile: eight 32-pixel tiles, the MFMA
+//   sequence of the network's own first-layer kernels (LayerFrag<8>), so
+//   its activations are bitwise theirs; stored at the layer's padded
+//   positions.
+// Each block redoes the RMS and the row sums of the rows around its tile:
+// cheaper than the separate launches (row pass, column pass and the first
+// layer: 4.1 + 4.9 + 8.9 us at 512^2).
+constexpr int kBoxT = 16;
+constexpr int kBoxC = kBoxT + 2;          // 18: sigma and input region
+constexpr int kBoxS = kBoxC + 2 * kWin;   // 30: staged region
+__global__ __launch_bounds__(256) void box_conv_kernel(
+    int hc, int wc, const float* __restrict__ rbar, const float* __restrict__ dis,
+    const double* __restrict__ part, int npart, const float* __restrict__ feat,
+    float* __restrict__ sig, nn::PackedWeights wts, bf16* __restrict__ a1) {
+  constexpr int kN = kBoxS * kBoxS;
+  constexpr int kIt = (kN + 255) / 256;
+  __shared__ float s_rb[kN], s_ds[kN];
+  __shared__ float s_tmp[kBoxS * kBoxC];
+  __shared__ bf16x8 s_in[kBoxC * kBoxC];
+  __shared__ double s_sc[256];
+  __shared__ float s_floor;
+  const int tid = threadIdx.x;
+  const int X0 = blockIdx.x * kBoxT, Y0 = blockIdx.y * kBoxT;
+  const long n = (long)hc * wc;
+  auto in_grid = [&](int X, int Y) { return X >= 0 && X < wc && Y >= 0 && Y < hc; };
+  // Every global load first: the first layer's weights, the RMS partials (8
+  // per thread and round), the staged region.
+  LayerFrag<8> w1;
+  w1.load(wts.wf1, wts.bias);
+  double v = 0.0;
+  for (int i0 = tid; i0 < npart; i0 += 256 * 8) {
+    double t8[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t8[j] = part[i0 + 256 * j < npart ? i0 + 256 * j : 0];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (i0 + 256 * j < npart) v += t8[j];
+  }
+  // Staged cell q: (X0 - 1 - kWin + q % kBoxS, Y0 - 1 - kWin + q / kBoxS).
+  float rb[kIt], ds[kIt];
+#pragma unroll
+  for (int k = 0; k < kIt; ++k) {
+    const int q = tid + 256 * k;
+    const int X = X0 - 1 - kWin + q % kBoxS, Y = Y0 - 1 - kWin + q / kBoxS;
+    const bool ok = q < kN && in_grid(X, Y);
+    const long j = ok ? (long)Y * wc + X : 0;
+    rb[k] = rbar[j];
+    ds[k] = dis[j];
+    if (!ok) rb[k] = ds[k] = 0.0f;
+  }
+  s_sc[tid] = v;
+#pragma unroll
+  for (int k = 0; k < kIt; ++k)
+    if (tid + 256 * k < kN) {
+      s_rb[tid + 256 * k] = rb[k];
+      s_ds[tid + 256 * k] = ds[k];
+    }
+  __syncthreads();
+  // Row pass: staged row r, region column c (staged column c + kWin).
+  for (int q = tid; q < kBoxS * kBoxC; q += 256) {
+    const int r = q / kBoxC, c = q - r * kBoxC + kWin;
+    const int Xc = X0 - 1 - kWin + c;
+    double s = 0.0;
+#pragma unroll
+    for (int d = -kWin; d <= kWin; ++d) {
+      const double w = (double)s_rb[r * kBoxS + c + d] * (double)s_ds[r * kBoxS + c + d];
+      s += Xc + d < 0 || Xc + d >= wc ? 0.0 : w * w;
+    }
+    s_tmp[q] = (float)s;
+  }
+  // The LDS tree of the RMS sum (element t adds element t + st): steps 128
+  // and 64 through LDS, 32 .. 1 by shuffles in wave 0.
+  if (tid < 128) s_sc[tid] += s_sc[tid + 128];
+  __syncthreads();
+  if (tid < 64) {
+    double x = s_sc[tid] + s_sc[tid + 64];
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) x = x + __shfl_down(x, st);
+    if (tid == 0) s_floor = kFloor * (float)sqrt(x / (double)n);
+  }
+  __syncthreads();
+  // Column pass, sigma and the input on the tile and its ring.
+  const float floor_s = s_floor;
+  for (int q = tid; q < kBoxC * kBoxC; q += 256) {
+    const int r = q / kBoxC, c = q - r * kBoxC;
+    const int X = X0 - 1 + c, Y = Y0 - 1 + r;
+    bf16x8 o = bf16x8{};
+    if (in_grid(X, Y)) {
+      double s = 0.0;
+#pragma unroll
+      for (int d = -kWin; d <= kWin; ++d)
+        s += Y + d < 0 || Y + d >= hc ? 0.0 : (double)s_tmp[(r + kWin + d) * kBoxC + c];
+      const float sg =
+          fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))), floor_s);
+      const long i = (long)Y * wc + X;
+      if (r >= 1 && r <= kBoxT && c >= 1 && c <= kBoxT) sig[i] = sg;
+      const int vi = (r + kWin) * kBoxS + c + kWin;
+      o[0] = sg > 0.0f ? (bf16)((s_rb[vi] * s_ds[vi]) / sg) : (bf16)0.0f;
+      o[1] = (bf16)feat[i];
+      o[2] = (bf16)feat[n + i];
+    }
+    s_in[q] = o;
+  }
+  __syncthreads();
+  // Layer 1: wave w takes 32-pixel tiles w and w + 4 (tile rows 2j, 2j + 1).
+  const int lane = tid & (kWave - 1), wv = tid / kWave;
+  const int hf = lane >> 5, col = lane & 31;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int j = wv + 4 * k;
+    const int q = 32 * j + col;
+    const int r = q / kBoxT, c = q - r * kBoxT;
+    f32x16v acc = {};
+#pragma unroll
+    for (int t = 0; t < LayerFrag<8>::NS; ++t) {
+      const int tap = 2 * t + hf;
+      const bf16x8 b =
+          tap < 9 ? s_in[(r + tap / 3) * kBoxC + (c + tap % 3)] : bf16x8{};
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1.A[t], b, acc, 0, 0, 0);
+    }
+    const int X = X0 + c, Y = Y0 + r;
+    bf16* px = a1 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * fused::kCh;
+    nn::store_out_row(px, acc, w1.bco, hf, X < wc && Y < hc);
+  }
+}
+
+// The end of net_level_kernel for the per-layer pipeline (box_conv_kernel
+// and the network's other hidden layers): the output layer on the
 // tile and its ring, from the last hidden layer's activations staged in LDS
 // (the tile and 2 cells around it), by fused_out as in net_level_kernel, so
 // N is bitwise its; then y = N sigma D^-1/2 on the tile and its ring, and
@@ -1299,7 +1337,7 @@ static NetLevelFn net_level_fn(NetShape s) {
 }
 
 // Whether the learned coarse solve runs as the per-layer pipeline (the
-// default: box_tile_kernel, the network's hidden layers, out_tile_kernel)
+// default: box_conv_kernel, the network's other hidden layers, out_tile_kernel)
 // or as one fused launch per level (TEA_FCG_NET=fused), which gives the same
 // y bit for bit.  The fused kernel recomputes each tile's halo at every layer
 // (2.1 times the pixels of the per-layer pipeline with 16 x 16 tiles) and
@@ -1328,9 +1366,10 @@ static int launch_net_level(tea_hip_chunk* c, int l, hipStream_t st) {
   } else {
     const dim3 boxes((unsigned)(L.wc + kBoxT - 1) / kBoxT,
                      (unsigned)(L.hc + kBoxT - 1) / kBoxT);
-    hipLaunchKernelGGL(box_tile_kernel, boxes, dim3(256), 0, st, L.hc, L.wc, L.rbar, L.dis,
-                       L.rms, L.nrms, L.feat, L.sig, nn::input_tensor(L.net));
-    TEA_RETURN_IF(nn::forward_hidden_on_stream(L.net, st));
+    hipLaunchKernelGGL(box_conv_kernel, boxes, dim3(256), 0, st, L.hc, L.wc, L.rbar, L.dis,
+                       L.rms, L.nrms, L.feat, L.sig, nn::packed_weights(L.net),
+                       nn::first_hidden(L.net));
+    TEA_RETURN_IF(nn::forward_hidden_on_stream(L.net, st, true));
     const nn::PackedWeights wts = nn::packed_weights(L.net);
     hipLaunchKernelGGL(out_tile_kernel, tiles, dim3(256), 0, st, L.hc, L.wc,
                        nn::last_hidden(L.net), wts.w6, wts.bias, L.sig, L.dis, L.rbar, L.kc,

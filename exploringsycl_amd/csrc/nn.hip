@@ -28,11 +28,6 @@
 namespace tea {
 namespace nn {
 
-typedef __bf16 bf16;
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
 constexpr int kC = 32;   // hidden channels
 constexpr int kC0 = 8;   // input channels in memory (3 used)
 constexpr int kIn = 3;   // real input channels
@@ -172,47 +167,6 @@ __device__ inline void stage_rows(bf16x8* sm, const bf16* __restrict__ inb, int 
 #pragma unroll
   for (int k = 0; k < kIt; ++k)
     if (threadIdx.x + 256 * k < kTotal) sm[slot[k]] = v[k];
-}
-
-// Stores one 32-pixel row of a layer's MFMA output (bias, ReLU, BF16) at
-// px, the row's pixel of lane col: lane (hf, col) holds channels 8 g + 4 hf
-// + 0..3, g = 0..3, of pixel col.  Lanes col and col + 32 swap half of
-// theirs, so that each holds 16 consecutive channels (32 bytes at 16 hf),
-// stored with two 16-byte stores (four 8-byte stores at a 64-byte stride
-// each wrote the first layer's output at 1.5 TB/s).
-__device__ inline void store_out_row(bf16* px, const f32x16& acc, const float (&bco)[16],
-                                     int hf) {
-  union Pack {
-    bf16x4 v;
-    unsigned u[2];
-  };
-  Pack o[4];
-#pragma unroll
-  for (int g = 0; g < 4; ++g)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float vv = acc[4 * g + q] + bco[4 * g + q];
-      o[g].v[q] = (bf16)(vv > 0.0f ? vv : 0.0f);
-    }
-  // hf 0 keeps channels 0-3 and 8-11 and takes 4-7 and 12-15 (the
-  // partner's g 0, 1); hf 1 keeps 20-23 and 28-31 and takes 16-19 and
-  // 24-27 (the partner's g 2, 3).
-  Pack in[2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j)
-#pragma unroll
-    for (int w = 0; w < 2; ++w) in[j].u[w] = __shfl_xor(hf ? o[j].u[w] : o[2 + j].u[w], 32);
-  union Pack8 {
-    bf16x8 v;
-    bf16x4 h[2];
-  };
-  Pack8 lo, hi;
-  lo.h[0] = hf ? in[0].v : o[0].v;
-  lo.h[1] = hf ? o[2].v : in[0].v;
-  hi.h[0] = hf ? in[1].v : o[1].v;
-  hi.h[1] = hf ? o[3].v : in[1].v;
-  *reinterpret_cast<bf16x8*>(px + 16 * hf) = lo.v;
-  *reinterpret_cast<bf16x8*>(px + 16 * hf + 8) = hi.v;
 }
 
 // The (segment, row group, sample) of this block in a (W / 128, H / R, B)
@@ -736,7 +690,8 @@ unsigned flat_blocks(long count) { return (unsigned)((count + 255) / 256); }
 // Forward pass over `batch` samples whose packed input is in n->in0, on
 // stream `st` (the network's own, or a solver's); without the output layer
 // unless with_output (its caller then runs it on the last activations).
-int forward_on(tea_nn* n, int batch, hipStream_t st, bool with_output = true) {
+int forward_on(tea_nn* n, int batch, hipStream_t st, bool with_output = true,
+               bool skip_first = false) {
   const int H = n->H, W = n->W;
   // Several rows per block only while the grid keeps at least 512 blocks:
   // on small grids the one-row kernel's parallelism matters more than the
@@ -745,7 +700,9 @@ int forward_on(tea_nn* n, int batch, hipStream_t st, bool with_output = true) {
   if (W % 128 == 0 && (R == 2 || R == 4) && H % R == 0 &&
       (long)(W / 128) * (H / R) * batch >= 512) {
     const dim3 grid(W / 128, H / R, batch);
-    if (R == 4)
+    if (skip_first)
+      ;
+    else if (R == 4)
       hipLaunchKernelGGL((conv_fwd_rows_kernel<kC0, 4>), grid, dim3(256), 0, st, n->in0,
                          n->wf1, n->bias, n->act[0], H, W);
     else
@@ -762,14 +719,16 @@ int forward_on(tea_nn* n, int batch, hipStream_t st, bool with_output = true) {
     }
   } else if (W % 128 == 0) {
     const dim3 grid(W / 128, H, batch);
-    hipLaunchKernelGGL((conv_fwd_lds_kernel<kC0>), grid, dim3(256), 0, st, n->in0, n->wf1,
-                       n->bias, n->act[0], H, W);
+    if (!skip_first)
+      hipLaunchKernelGGL((conv_fwd_lds_kernel<kC0>), grid, dim3(256), 0, st, n->in0, n->wf1,
+                         n->bias, n->act[0], H, W);
     for (int l = 1; l < 5; ++l)
       hipLaunchKernelGGL((conv_fwd_lds_kernel<kC>), grid, dim3(256), 0, st, n->act[l - 1],
                          n->wfh + (size_t)(l - 1) * kWh, n->bias + l * kC, n->act[l], H, W);
   } else {
-    hipLaunchKernelGGL((conv_mfma_kernel<kC0, 0>), rows_grid(n, batch), dim3(256), 0,
-                       st, n->in0, n->wf1, n->bias, nullptr, n->act[0], H, W, batch);
+    if (!skip_first)
+      hipLaunchKernelGGL((conv_mfma_kernel<kC0, 0>), rows_grid(n, batch), dim3(256), 0,
+                         st, n->in0, n->wf1, n->bias, nullptr, n->act[0], H, W, batch);
     for (int l = 1; l < 5; ++l)
       hipLaunchKernelGGL((conv_mfma_kernel<kC, 0>), rows_grid(n, batch), dim3(256), 0,
                          st, n->act[l - 1], n->wfh + (size_t)(l - 1) * kWh,
@@ -897,8 +856,11 @@ namespace tea {
 namespace nn {
 // Used by the flexible-CG preconditioner (fcg.hip).
 int forward_on_stream(tea_nn* n, hipStream_t st) { return forward_on(n, 1, st); }
-int forward_hidden_on_stream(tea_nn* n, hipStream_t st) { return forward_on(n, 1, st, false); }
+int forward_hidden_on_stream(tea_nn* n, hipStream_t st, bool from_second) {
+  return forward_on(n, 1, st, false, from_second);
+}
 const bf16* last_hidden(tea_nn* n) { return n->act[4]; }
+bf16* first_hidden(tea_nn* n) { return n->act[0]; }
 bf16* input_tensor(tea_nn* n) { return n->in0; }
 PackedWeights packed_weights(const tea_nn* n) {
   return PackedWeights{n->wf1, n->wfh, n->w6, n->bias};

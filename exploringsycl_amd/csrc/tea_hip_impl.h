@@ -613,6 +613,53 @@ struct tea_nn;
 namespace tea {
 namespace nn {
 typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Stores one 32-pixel row of a layer's MFMA output (bias, ReLU, BF16) at
+// px, the row's pixel of lane col: lane (hf, col) holds channels 8 g + 4 hf
+// + 0..3, g = 0..3, of pixel col.  Lanes col and col + 32 swap half of
+// theirs, so that each holds 16 consecutive channels (32 bytes at 16 hf),
+// stored with two 16-byte stores (four 8-byte stores at a 64-byte stride
+// each wrote the first layer's output at 1.5 TB/s).  Every lane takes part
+// in the exchange; only lanes with ok store.
+__device__ inline void store_out_row(bf16* px, const f32x16& acc, const float (&bco)[16],
+                                     int hf, bool ok = true) {
+  union Pack {
+    bf16x4 v;
+    unsigned u[2];
+  };
+  Pack o[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float vv = acc[4 * g + q] + bco[4 * g + q];
+      o[g].v[q] = (bf16)(vv > 0.0f ? vv : 0.0f);
+    }
+  // hf 0 keeps channels 0-3 and 8-11 and takes 4-7 and 12-15 (the
+  // partner's g 0, 1); hf 1 keeps 20-23 and 28-31 and takes 16-19 and
+  // 24-27 (the partner's g 2, 3).
+  Pack in[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int w = 0; w < 2; ++w) in[j].u[w] = __shfl_xor(hf ? o[j].u[w] : o[2 + j].u[w], 32);
+  union Pack8 {
+    bf16x8 v;
+    bf16x4 h[2];
+  };
+  Pack8 lo, hi;
+  lo.h[0] = hf ? in[0].v : o[0].v;
+  lo.h[1] = hf ? o[2].v : in[0].v;
+  hi.h[0] = hf ? in[1].v : o[1].v;
+  hi.h[1] = hf ? o[3].v : in[1].v;
+  if (!ok) return;
+  *reinterpret_cast<bf16x8*>(px + 16 * hf) = lo.v;
+  *reinterpret_cast<bf16x8*>(px + 16 * hf + 8) = hi.v;
+}
+
 // The network's device copies of its parameters, in the layouts of its
 // forward kernels: layer 1 wf1[co][tap][8] (3 channels used) and the hidden
 // layers wfh[layer][co][tap][ci] in BF16, the output layer w6[tap][ci]
@@ -626,9 +673,12 @@ struct PackedWeights {
 PackedWeights packed_weights(const tea_nn* n);
 int forward_on_stream(tea_nn* n, hipStream_t st);
 // The hidden layers alone; the last one's activations (padded (H + 2) x
-// (W + 2) pixels of 32 BF16 channels, zero ring) at last_hidden.
-int forward_hidden_on_stream(tea_nn* n, hipStream_t st);
+// (W + 2) pixels of 32 BF16 channels, zero ring) at last_hidden.  With
+// from_second, layers 2-5 only: the caller has written the first layer's
+// activations at first_hidden (same layout).
+int forward_hidden_on_stream(tea_nn* n, hipStream_t st, bool from_second = false);
 const bf16* last_hidden(tea_nn* n);
+bf16* first_hidden(tea_nn* n);
 bf16* input_tensor(tea_nn* n);
 const float* output(tea_nn* n);
 }  // namespace nn
