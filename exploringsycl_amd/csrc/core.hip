@@ -220,14 +220,18 @@ __global__ __launch_bounds__(kFinishBlock) void span_finish_kernel(
   const int lane = t & (kWave - 1);
   const int blk = blockIdx.x * kFinishBlock + t;
   const int span = blk / kSubPerSpan;
+  // Every sum's loads first (a round per sum was waited for sum by sum).
+  const bool in = blk < nblk && span < nspan;
+  double w[NR][kWaves];
 #pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    const double* __restrict__ sq = sub + (size_t)q * kWaves * nblk;
-    double w[kWaves];
+  for (int q = 0; q < NR; ++q)
 #pragma unroll
     for (int k = 0; k < kWaves; ++k)
-      w[k] = (blk < nblk && span < nspan) ? sq[kWaves * blk + k] : 0.0;
-    double v = (w[0] + w[1]) + (w[2] + w[3]);
+      w[q][k] = sub[(size_t)q * kWaves * nblk + (in ? kWaves * blk + k : 0)];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    if (!in) w[q][0] = w[q][1] = w[q][2] = w[q][3] = 0.0;
+    double v = (w[q][0] + w[q][1]) + (w[q][2] + w[q][3]);
     v = v + __shfl_xor(v, 1);
     v = v + __shfl_xor(v, 2);
     v = v + __shfl_xor(v, 4);

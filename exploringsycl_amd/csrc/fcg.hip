@@ -958,7 +958,9 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
   const long n = (long)hc * wc;
   auto in_grid = [&](int X, int Y) { return X >= 0 && X < wc && Y >= 0 && Y < hc; };
   // Every global load first: the first layer's weights, the RMS partials (8
-  // per thread and round), the staged region.
+  // per thread and round), the staged region, the features of the thread's
+  // input pixels.
+  constexpr int kNq = (kBoxC * kBoxC + 255) / 256;
   LayerFrag<8> w1;
   w1.load(wts.wf1, wts.bias);
   double v = 0.0;
@@ -981,6 +983,15 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
     rb[k] = rbar[j];
     ds[k] = dis[j];
     if (!ok) rb[k] = ds[k] = 0.0f;
+  }
+  float f0[kNq], f1[kNq];
+#pragma unroll
+  for (int k = 0; k < kNq; ++k) {
+    const int q = tid + 256 * k;
+    const int X = X0 - 1 + q % kBoxC, Y = Y0 - 1 + q / kBoxC;
+    const long j = q < kBoxC * kBoxC && in_grid(X, Y) ? (long)Y * wc + X : 0;
+    f0[k] = feat[j];
+    f1[k] = feat[n + j];
   }
   s_sc[tid] = v;
 #pragma unroll
@@ -1015,7 +1026,10 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
   __syncthreads();
   // Column pass, sigma and the input on the tile and its ring.
   const float floor_s = s_floor;
-  for (int q = tid; q < kBoxC * kBoxC; q += 256) {
+#pragma unroll
+  for (int k = 0; k < kNq; ++k) {
+    const int q = tid + 256 * k;
+    if (q >= kBoxC * kBoxC) break;
     const int r = q / kBoxC, c = q - r * kBoxC;
     const int X = X0 - 1 + c, Y = Y0 - 1 + r;
     bf16x8 o = bf16x8{};
@@ -1030,8 +1044,8 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
       if (r >= 1 && r <= kBoxT && c >= 1 && c <= kBoxT) sig[i] = sg;
       const int vi = (r + kWin) * kBoxS + c + kWin;
       o[0] = sg > 0.0f ? (bf16)((s_rb[vi] * s_ds[vi]) / sg) : (bf16)0.0f;
-      o[1] = (bf16)feat[i];
-      o[2] = (bf16)feat[n + i];
+      o[1] = (bf16)f0[k];
+      o[2] = (bf16)f1[k];
     }
     s_in[q] = o;
   }
@@ -1214,9 +1228,17 @@ __global__ __launch_bounds__(kGammaThreads) void joint_gamma_kernel(
   // elements of each tile sum and kBx of the cross sum, whose arrays are 16
   // times longer (level-1 cells against level-0 tiles): one round at
   // 4096^2 (four rounds of four took 10 us).
+  // Every load is unconditional, from a clamped index (p0's first element
+  // stands in for a level that is not there): a load under a test on the
+  // level count was branched around and waited for inside the branch, one
+  // memory latency per element (10 us at 4096^2).
   constexpr int kB = 4, kBx = 16;
   const int n2 = levels == 2 ? n1 : 0, nxx = levels == 2 ? nx : 0;
   const int nt = max(n0, n2);
+  const double* q1 = n2 > 0 ? p1 : p0;
+  const int q1b = n2 > 0 ? n1 : 0;
+  const double* xs = nxx > 0 ? xsum : p0;
+  const float* ys = nxx > 0 ? y1 : reinterpret_cast<const float*>(p0);
   for (int k0 = threadIdx.x, kx0 = threadIdx.x; k0 < nt || kx0 < nxx;
        k0 += kT * kB, kx0 += kT * kBx) {
     double t[4][kB], tx[kBx];
@@ -1226,14 +1248,14 @@ __global__ __launch_bounds__(kGammaThreads) void joint_gamma_kernel(
       const int a = k < n0 ? k : 0, b = k < n2 ? k : 0;
       t[0][j] = p0[a];
       t[1][j] = p0[n0 + a];
-      t[2][j] = n2 > 0 ? p1[b] : 0.0;
-      t[3][j] = n2 > 0 ? p1[n1 + b] : 0.0;
+      t[2][j] = q1[b];
+      t[3][j] = q1[q1b + b];
     }
 #pragma unroll
     for (int j = 0; j < kBx; ++j) {
       const int k = kx0 + kT * j;
       const int a = k < nxx ? k : 0;
-      tx[j] = nxx > 0 ? xsum[a] * (double)y1[a] : 0.0;
+      tx[j] = xs[a] * (double)ys[a];
     }
 #pragma unroll
     for (int j = 0; j < kB; ++j) {
