@@ -349,6 +349,7 @@ __global__ __launch_bounds__(256) void feature_kernel(int hc, int wc,
 // 8-byte loads per row when x and h are even).  With dis0 (dis1), also each
 // block's sum of (D^-1/2 rbar)^2 over its cells of the level into rms0
 // (rms1): the network's input scale.
+template <bool kPairs>
 __global__ __launch_bounds__(256) void restrict8_kernel(
     Geo g, int hc, int wc, const float* __restrict__ r, float* __restrict__ rbar0,
     float* __restrict__ rbar1, const float* __restrict__ dis0,
@@ -359,16 +360,27 @@ __global__ __launch_bounds__(256) void restrict8_kernel(
   const int X = blockIdx.x * 32 + (lane >> 1), Y = blockIdx.y * 4 + wv;
   const int half = lane & 1;
   const bool ok = X < wc && Y < hc;
+  // D^-1/2 of the cell for the RMS partial, loaded with the rest (a pointer
+  // that is always valid stands in when there is none).
+  const float* d0p = dis0 ? dis0 : r;
+  const float d0 = d0p[dis0 && ok ? Y * wc + X : 0];
   double t = 0.0;
   if (ok) {
     const int base = (g.h + 8 * Y) * g.x + g.h + 8 * X + 4 * half;
     double q[8];
-    if (((g.x | g.h) & 1) == 0) {
+    // kPairs: x and h even, two 8-byte loads per row.  A template argument,
+    // not a test: the two forms' last row was merged into one load after the
+    // branch, waited for on its own.
+    if (kPairs) {
+      float2 a[8], c[8];
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
-        const float2 a = ld2(r, base + b * g.x), c = ld2(r, base + b * g.x + 2);
-        q[b] = ((double)a.x + (double)a.y) + ((double)c.x + (double)c.y);
+        a[b] = ld2(r, base + b * g.x);
+        c[b] = ld2(r, base + b * g.x + 2);
       }
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+        q[b] = ((double)a[b].x + (double)a[b].y) + ((double)c[b].x + (double)c[b].y);
     } else {
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
@@ -386,7 +398,7 @@ __global__ __launch_bounds__(256) void restrict8_kernel(
   if (half == 0) sm[wv][lane >> 1] = m;
   double s2 = 0.0;
   if (rms0 && ok && half == 0) {
-    const double v = (double)m * (double)dis0[Y * wc + X];
+    const double v = (double)m * (double)d0;
     s2 = v * v;
   }
   for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off);
@@ -1426,10 +1438,12 @@ static int launch_coarse(tea_hip_chunk* c) {
   {
     Timed t(c, K_FCG_RESTRICT);
     const dim3 grid((unsigned)(L0.wc + 31) / 32, (unsigned)(L0.hc + 3) / 4);
-    hipLaunchKernelGGL(restrict8_kernel, grid, dim3(256), 0, c->stream, c->geo, L0.hc, L0.wc,
-                       c->mp_r, L0.rbar, nl == 2 ? L1.rbar : nullptr,
-                       net ? L0.dis : nullptr, net && nl == 2 ? L1.dis : nullptr,
-                       net ? L0.rms : nullptr, net && nl == 2 ? L1.rms : nullptr);
+    const bool pairs = ((c->geo.x | c->geo.h) & 1) == 0;
+    hipLaunchKernelGGL(pairs ? restrict8_kernel<true> : restrict8_kernel<false>, grid,
+                       dim3(256), 0, c->stream, c->geo, L0.hc, L0.wc, c->mp_r, L0.rbar,
+                       nl == 2 ? L1.rbar : nullptr, net ? L0.dis : nullptr,
+                       net && nl == 2 ? L1.dis : nullptr, net ? L0.rms : nullptr,
+                       net && nl == 2 ? L1.rms : nullptr);
     TEA_HIP_TRY(hipGetLastError());
   }
   auto level = [&](int l, hipStream_t st) {
