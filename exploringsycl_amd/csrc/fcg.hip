@@ -547,10 +547,10 @@ __global__ __launch_bounds__(256) void sweeps_kernel(int hc, int wc, int sweeps,
 
 // ---- the learned coarse solve of one level in one launch -----------------
 // net_level_kernel computes, for one 16 x 16 tile of a coarse level, what
-// box_conv_kernel (with the first layer), the other four hidden layers of
-// the network (nn.hip: conv_*_kernel), out_tile_kernel and the coarse dots
-// compute for it, keeping every intermediate in LDS: the staged rbar and D^-1/2 (the
-// tile and 13 cells around it), the local scale sigma and the packed BF16
+// box_conv_kernel (with the first two layers), the other three hidden
+// layers of the network (nn.hip: conv_*_kernel), out_tile_kernel and the
+// coarse dots compute for it, keeping every intermediate in LDS: the staged
+// rbar and D^-1/2 (the tile and 13 cells around it), the local scale sigma and the packed BF16
 // input (7 cells around it: the network's receptive field plus the ring of
 // T y), and the activations of each layer on its shrinking region (28^2,
 // 26^2, ..., 20^2 pixels), then N and y = N sigma D^-1/2 on the tile and its
@@ -932,36 +932,49 @@ __global__ __launch_bounds__(kThreads) void net_level_kernel(
 }
 
 // The start of the per-layer pipeline, for one 16 x 16 tile per 256-thread
-// block: sigma and the network's packed input on the tile and its ring, and
-// the first layer on the tile from that input in LDS.
+// block: sigma and the network's packed input on the tile and 2 cells
+// around it, the first layer on the tile and its ring, and the second layer
+// on the tile, with the input and the first layer's activations in LDS.
 // - The RMS floor from the restriction's per-block partials (thread t adds
 //   partials t, t + 256, ... in turn, then the LDS tree).
-// - rbar and D^-1/2 staged on the tile and 7 cells around it; the row pass
+// - rbar and D^-1/2 staged on the tile and 8 cells around it; the row pass
 //   of the window sums (FP64, rounded to FP32) on the staged rows over the
-//   tile's and its ring's columns; the column pass (FP64 over the 13 row
-//   sums); sigma = max(RMS of the window, kFloor times the grid's RMS); the
-//   input: channel 0 = D^-1/2 rbar / sigma (0 where sigma is 0), channels
-//   1, 2 = the features, zero outside the grid.  Cells beyond the grid's
-//   edge add nothing (adding +0 to a sum of squares, which is >= +0, is
-//   skipping them).
-// - Layer 1 (8 -> 32 channels) on the tile: eight 32-pixel tiles, the MFMA
-//   sequence of the network's own first-layer kernels (LayerFrag<8>), so
-//   its activations are bitwise theirs; stored at the layer's padded
-//   positions.
-// Each block redoes the RMS and the row sums of the rows around its tile:
-// cheaper than the separate launches (row pass, column pass and the first
-// layer: 4.1 + 4.9 + 8.9 us at 512^2).
+//   input region's columns; the column pass (FP64 over the 13 row sums);
+//   sigma = max(RMS of the window, kFloor times the grid's RMS); the input:
+//   channel 0 = D^-1/2 rbar / sigma (0 where sigma is 0), channels 1, 2 =
+//   the features, zero outside the grid.  Cells beyond the grid's edge add
+//   nothing (adding +0 to a sum of squares, which is >= +0, is skipping
+//   them).
+// - Layer 1 (8 -> 32 channels) on the 18 x 18 region (fused_conv, zero
+//   outside the grid) and layer 2 (32 -> 32) on the tile, each pixel in the
+//   network's own layer kernels' MFMA sequence, so the second layer's
+//   activations are bitwise theirs; stored at its padded positions.
+// Each block redoes the RMS, the row sums of the rows around its tile and
+// layer 1 on the tile's ring: cheaper than the separate launches (the
+// window passes, layer 1 and layer 2 as four launches took 4.1 + 4.9 + 8.9
+// + 12.1 us at 512^2, as one local-scale-and-layer-1 launch and layer 2
+// 15.3 + 12.1 us).
 constexpr int kBoxT = 16;
-constexpr int kBoxC = kBoxT + 2;          // 18: sigma and input region
-constexpr int kBoxS = kBoxC + 2 * kWin;   // 30: staged region
+constexpr int kBoxA = kBoxT + 2;          // 18: layer-1 region
+constexpr int kBoxC = kBoxT + 4;          // 20: sigma and input region
+constexpr int kBoxS = kBoxC + 2 * kWin;   // 32: staged region
+constexpr int kBoxStage = (2 * kBoxS * kBoxS + kBoxS * kBoxC) * 4;  // bytes
+constexpr int kBoxA1 = kBoxA * kBoxA * 5 * 16;                       // bytes
 __global__ __launch_bounds__(256) void box_conv_kernel(
     int hc, int wc, const float* __restrict__ rbar, const float* __restrict__ dis,
     const double* __restrict__ part, int npart, const float* __restrict__ feat,
-    float* __restrict__ sig, nn::PackedWeights wts, bf16* __restrict__ a1) {
+    float* __restrict__ sig, nn::PackedWeights wts, bf16* __restrict__ a2) {
   constexpr int kN = kBoxS * kBoxS;
   constexpr int kIt = (kN + 255) / 256;
-  __shared__ float s_rb[kN], s_ds[kN];
-  __shared__ float s_tmp[kBoxS * kBoxC];
+  constexpr int kNq = (kBoxC * kBoxC + 255) / 256;
+  // The local-scale stage is dead once the input is formed: layer 1's
+  // activations (act_slot layout) take its place.
+  __shared__ __attribute__((aligned(16)))
+  unsigned char s_u[kBoxStage > kBoxA1 ? kBoxStage : kBoxA1];
+  float* s_rb = reinterpret_cast<float*>(s_u);
+  float* s_ds = s_rb + kN;
+  float* s_tmp = s_ds + kN;
+  bf16x8* s_a1 = reinterpret_cast<bf16x8*>(s_u);
   __shared__ bf16x8 s_in[kBoxC * kBoxC];
   __shared__ double s_sc[256];
   __shared__ float s_floor;
@@ -972,7 +985,6 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
   // Every global load first: the first layer's weights, the RMS partials (8
   // per thread and round), the staged region, the features of the thread's
   // input pixels.
-  constexpr int kNq = (kBoxC * kBoxC + 255) / 256;
   LayerFrag<8> w1;
   w1.load(wts.wf1, wts.bias);
   double v = 0.0;
@@ -984,12 +996,12 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
     for (int j = 0; j < 8; ++j)
       if (i0 + 256 * j < npart) v += t8[j];
   }
-  // Staged cell q: (X0 - 1 - kWin + q % kBoxS, Y0 - 1 - kWin + q / kBoxS).
+  // Staged cell q: (X0 - 2 - kWin + q % kBoxS, Y0 - 2 - kWin + q / kBoxS).
   float rb[kIt], ds[kIt];
 #pragma unroll
   for (int k = 0; k < kIt; ++k) {
     const int q = tid + 256 * k;
-    const int X = X0 - 1 - kWin + q % kBoxS, Y = Y0 - 1 - kWin + q / kBoxS;
+    const int X = X0 - 2 - kWin + q % kBoxS, Y = Y0 - 2 - kWin + q / kBoxS;
     const bool ok = q < kN && in_grid(X, Y);
     const long j = ok ? (long)Y * wc + X : 0;
     rb[k] = rbar[j];
@@ -1000,7 +1012,7 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
 #pragma unroll
   for (int k = 0; k < kNq; ++k) {
     const int q = tid + 256 * k;
-    const int X = X0 - 1 + q % kBoxC, Y = Y0 - 1 + q / kBoxC;
+    const int X = X0 - 2 + q % kBoxC, Y = Y0 - 2 + q / kBoxC;
     const long j = q < kBoxC * kBoxC && in_grid(X, Y) ? (long)Y * wc + X : 0;
     f0[k] = feat[j];
     f1[k] = feat[n + j];
@@ -1016,7 +1028,7 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
   // Row pass: staged row r, region column c (staged column c + kWin).
   for (int q = tid; q < kBoxS * kBoxC; q += 256) {
     const int r = q / kBoxC, c = q - r * kBoxC + kWin;
-    const int Xc = X0 - 1 - kWin + c;
+    const int Xc = X0 - 2 - kWin + c;
     double s = 0.0;
 #pragma unroll
     for (int d = -kWin; d <= kWin; ++d) {
@@ -1036,14 +1048,14 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
     if (tid == 0) s_floor = kFloor * (float)sqrt(x / (double)n);
   }
   __syncthreads();
-  // Column pass, sigma and the input on the tile and its ring.
+  // Column pass, sigma and the input on the tile and 2 cells around it.
   const float floor_s = s_floor;
 #pragma unroll
   for (int k = 0; k < kNq; ++k) {
     const int q = tid + 256 * k;
     if (q >= kBoxC * kBoxC) break;
     const int r = q / kBoxC, c = q - r * kBoxC;
-    const int X = X0 - 1 + c, Y = Y0 - 1 + r;
+    const int X = X0 - 2 + c, Y = Y0 - 2 + r;
     bf16x8 o = bf16x8{};
     if (in_grid(X, Y)) {
       double s = 0.0;
@@ -1053,7 +1065,7 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
       const float sg =
           fmaxf((float)sqrt(s / (double)((2 * kWin + 1) * (2 * kWin + 1))), floor_s);
       const long i = (long)Y * wc + X;
-      if (r >= 1 && r <= kBoxT && c >= 1 && c <= kBoxT) sig[i] = sg;
+      if (r >= 2 && r < 2 + kBoxT && c >= 2 && c < 2 + kBoxT) sig[i] = sg;
       const int vi = (r + kWin) * kBoxS + c + kWin;
       o[0] = sg > 0.0f ? (bf16)((s_rb[vi] * s_ds[vi]) / sg) : (bf16)0.0f;
       o[1] = (bf16)f0[k];
@@ -1061,8 +1073,15 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
     }
     s_in[q] = o;
   }
+  // Layer 2's weights, loaded before the barrier that waits for the input.
+  LayerFrag<fused::kCh> w2;
+  w2.load(wts.wfh, wts.bias + fused::kCh);
   __syncthreads();
-  // Layer 1: wave w takes 32-pixel tiles w and w + 4 (tile rows 2j, 2j + 1).
+  // Layer 1 on the 18 x 18 region (the stage's LDS is dead now).
+  fused_conv<8, 1>(s_in, s_a1, kBoxC, w1, X0 - 1, Y0 - 1, hc, wc, kWaves);
+  __syncthreads();
+  // Layer 2 on the tile: wave w takes 32-pixel tiles w and w + 4 (tile rows
+  // 2j, 2j + 1).
   const int lane = tid & (kWave - 1), wv = tid / kWave;
   const int hf = lane >> 5, col = lane & 31;
 #pragma unroll
@@ -1072,20 +1091,20 @@ __global__ __launch_bounds__(256) void box_conv_kernel(
     const int r = q / kBoxT, c = q - r * kBoxT;
     f32x16v acc = {};
 #pragma unroll
-    for (int t = 0; t < LayerFrag<8>::NS; ++t) {
-      const int tap = 2 * t + hf;
-      const bf16x8 b =
-          tap < 9 ? s_in[(r + tap / 3) * kBoxC + (c + tap % 3)] : bf16x8{};
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1.A[t], b, acc, 0, 0, 0);
+    for (int t = 0; t < LayerFrag<fused::kCh>::NS; ++t) {
+      const int tap = t >> 1;
+      const int pin = (r + tap / 3) * kBoxA + (c + tap % 3);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+          w2.A[t], s_a1[act_slot(pin, ((t & 1) << 1) + hf)], acc, 0, 0, 0);
     }
     const int X = X0 + c, Y = Y0 + r;
-    bf16* px = a1 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * fused::kCh;
-    nn::store_out_row(px, acc, w1.bco, hf, X < wc && Y < hc);
+    bf16* px = a2 + ((size_t)(Y + 1) * (wc + 2) + (X + 1)) * fused::kCh;
+    nn::store_out_row(px, acc, w2.bco, hf, X < wc && Y < hc);
   }
 }
 
 // The end of net_level_kernel for the per-layer pipeline (box_conv_kernel
-// and the network's other hidden layers): the output layer on the
+// and the network's last three hidden layers): the output layer on the
 // tile and its ring, from the last hidden layer's activations staged in LDS
 // (the tile and 2 cells around it), by fused_out as in net_level_kernel, so
 // N is bitwise its; then y = N sigma D^-1/2 on the tile and its ring, and
@@ -1371,11 +1390,11 @@ static NetLevelFn net_level_fn(NetShape s) {
 }
 
 // Whether the learned coarse solve runs as the per-layer pipeline (the
-// default: box_conv_kernel, the network's other hidden layers, out_tile_kernel)
-// or as one fused launch per level (TEA_FCG_NET=fused), which gives the same
-// y bit for bit.  The fused kernel recomputes each tile's halo at every layer
-// (2.1 times the pixels of the per-layer pipeline with 16 x 16 tiles) and
-// measured slower at 4096^2 (about 91 against 80 us per coarse launch).
+// default: box_conv_kernel, the network's last three hidden layers,
+// out_tile_kernel) or as one fused launch per level (TEA_FCG_NET=fused),
+// which gives the same y bit for bit.  The fused kernel recomputes each
+// tile's halo at every layer (2.4 times the MFMA work of the per-layer
+// pipeline with 16 x 16 tiles) and measured slower at 4096^2.
 static bool net_per_layer() {
   static const bool v = [] {
     const char* e = getenv("TEA_FCG_NET");
@@ -1402,8 +1421,8 @@ static int launch_net_level(tea_hip_chunk* c, int l, hipStream_t st) {
                      (unsigned)(L.hc + kBoxT - 1) / kBoxT);
     hipLaunchKernelGGL(box_conv_kernel, boxes, dim3(256), 0, st, L.hc, L.wc, L.rbar, L.dis,
                        L.rms, L.nrms, L.feat, L.sig, nn::packed_weights(L.net),
-                       nn::first_hidden(L.net));
-    TEA_RETURN_IF(nn::forward_hidden_on_stream(L.net, st, true));
+                       nn::hidden(L.net, 1));
+    TEA_RETURN_IF(nn::forward_hidden_on_stream(L.net, st, 2));
     const nn::PackedWeights wts = nn::packed_weights(L.net);
     hipLaunchKernelGGL(out_tile_kernel, tiles, dim3(256), 0, st, L.hc, L.wc,
                        nn::last_hidden(L.net), wts.w6, wts.bias, L.sig, L.dis, L.rbar, L.kc,

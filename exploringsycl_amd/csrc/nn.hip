@@ -689,9 +689,11 @@ unsigned flat_blocks(long count) { return (unsigned)((count + 255) / 256); }
 
 // Forward pass over `batch` samples whose packed input is in n->in0, on
 // stream `st` (the network's own, or a solver's); without the output layer
-// unless with_output (its caller then runs it on the last activations).
+// unless with_output (its caller then runs it on the last activations); from
+// hidden layer `first` on (0-based: the caller has written the activations
+// of the layers before it).
 int forward_on(tea_nn* n, int batch, hipStream_t st, bool with_output = true,
-               bool skip_first = false) {
+               int first = 0) {
   const int H = n->H, W = n->W;
   // Several rows per block only while the grid keeps at least 512 blocks:
   // on small grids the one-row kernel's parallelism matters more than the
@@ -700,7 +702,7 @@ int forward_on(tea_nn* n, int batch, hipStream_t st, bool with_output = true,
   if (W % 128 == 0 && (R == 2 || R == 4) && H % R == 0 &&
       (long)(W / 128) * (H / R) * batch >= 512) {
     const dim3 grid(W / 128, H / R, batch);
-    if (skip_first)
+    if (first > 0)
       ;
     else if (R == 4)
       hipLaunchKernelGGL((conv_fwd_rows_kernel<kC0, 4>), grid, dim3(256), 0, st, n->in0,
@@ -708,7 +710,7 @@ int forward_on(tea_nn* n, int batch, hipStream_t st, bool with_output = true,
     else
       hipLaunchKernelGGL((conv_fwd_rows_kernel<kC0, 2>), grid, dim3(256), 0, st, n->in0,
                          n->wf1, n->bias, n->act[0], H, W);
-    for (int l = 1; l < 5; ++l) {
+    for (int l = first > 1 ? first : 1; l < 5; ++l) {
       const bf16* w = n->wfh + (size_t)(l - 1) * kWh;
       if (R == 4)
         hipLaunchKernelGGL((conv_fwd_rows_kernel<kC, 4>), grid, dim3(256), 0, st,
@@ -719,17 +721,17 @@ int forward_on(tea_nn* n, int batch, hipStream_t st, bool with_output = true,
     }
   } else if (W % 128 == 0) {
     const dim3 grid(W / 128, H, batch);
-    if (!skip_first)
+    if (first == 0)
       hipLaunchKernelGGL((conv_fwd_lds_kernel<kC0>), grid, dim3(256), 0, st, n->in0, n->wf1,
                          n->bias, n->act[0], H, W);
-    for (int l = 1; l < 5; ++l)
+    for (int l = first > 1 ? first : 1; l < 5; ++l)
       hipLaunchKernelGGL((conv_fwd_lds_kernel<kC>), grid, dim3(256), 0, st, n->act[l - 1],
                          n->wfh + (size_t)(l - 1) * kWh, n->bias + l * kC, n->act[l], H, W);
   } else {
-    if (!skip_first)
+    if (first == 0)
       hipLaunchKernelGGL((conv_mfma_kernel<kC0, 0>), rows_grid(n, batch), dim3(256), 0,
                          st, n->in0, n->wf1, n->bias, nullptr, n->act[0], H, W, batch);
-    for (int l = 1; l < 5; ++l)
+    for (int l = first > 1 ? first : 1; l < 5; ++l)
       hipLaunchKernelGGL((conv_mfma_kernel<kC, 0>), rows_grid(n, batch), dim3(256), 0,
                          st, n->act[l - 1], n->wfh + (size_t)(l - 1) * kWh,
                          n->bias + l * kC, nullptr, n->act[l], H, W, batch);
@@ -856,11 +858,11 @@ namespace tea {
 namespace nn {
 // Used by the flexible-CG preconditioner (fcg.hip).
 int forward_on_stream(tea_nn* n, hipStream_t st) { return forward_on(n, 1, st); }
-int forward_hidden_on_stream(tea_nn* n, hipStream_t st, bool from_second) {
-  return forward_on(n, 1, st, false, from_second);
+int forward_hidden_on_stream(tea_nn* n, hipStream_t st, int first) {
+  return forward_on(n, 1, st, false, first);
 }
 const bf16* last_hidden(tea_nn* n) { return n->act[4]; }
-bf16* first_hidden(tea_nn* n) { return n->act[0]; }
+bf16* hidden(tea_nn* n, int l) { return n->act[l]; }
 bf16* input_tensor(tea_nn* n) { return n->in0; }
 PackedWeights packed_weights(const tea_nn* n) {
   return PackedWeights{n->wf1, n->wfh, n->w6, n->bias};

@@ -672,13 +672,13 @@ struct PackedWeights {
 };
 PackedWeights packed_weights(const tea_nn* n);
 int forward_on_stream(tea_nn* n, hipStream_t st);
-// The hidden layers alone; the last one's activations (padded (H + 2) x
-// (W + 2) pixels of 32 BF16 channels, zero ring) at last_hidden.  With
-// from_second, layers 2-5 only: the caller has written the first layer's
-// activations at first_hidden (same layout).
-int forward_hidden_on_stream(tea_nn* n, hipStream_t st, bool from_second = false);
+// The hidden layers alone, from layer `first` on (0-based; the caller has
+// written layer first - 1's activations at hidden(n, first - 1)); the last
+// layer's activations (padded (H + 2) x (W + 2) pixels of 32 BF16 channels,
+// zero ring) at last_hidden.
+int forward_hidden_on_stream(tea_nn* n, hipStream_t st, int first = 0);
 const bf16* last_hidden(tea_nn* n);
-bf16* first_hidden(tea_nn* n);
+bf16* hidden(tea_nn* n, int l);
 bf16* input_tensor(tea_nn* n);
 const float* output(tea_nn* n);
 }  // namespace nn
