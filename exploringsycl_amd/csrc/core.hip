@@ -404,6 +404,7 @@ int tea_hip_chunk_create(int device, int x, int y, int halo_depth,
   }
   if (const char* pv = getenv("TEA_HIP_POLL")) c->poll = atoi(pv) != 0;
   if (const char* rv = getenv("TEA_HIP_RUN_AHEAD")) c->ra_enabled = atoi(rv) != 0;
+  if (const char* sv = getenv("TEA_HIP_STACK")) c->stack = atoi(sv);
   if (const char* sp = getenv("TEA_HIP_SPIN"))
     if (atoi(sp) != 0) (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
 

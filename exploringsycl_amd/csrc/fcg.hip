@@ -277,7 +277,7 @@ __global__ __launch_bounds__(kQuadThreads) void fcg_z4_kernel(
   }
   double v[1] = {(prod[0] + prod[1]) + (prod[2] + prod[3])};
   if (any) st4(z, e, in, zv);
-  wave_subtree_quads<1>(v, sub, nblk, blk);
+  wave_subtree_quads<1>(v, sub, nblk, blk, threadIdx.x / kWave);
 }
 
 // The coarse operator T = P^T A P / c^2: for each coarse cell, the face

@@ -392,16 +392,25 @@ __device__ inline void ring_store(T* f, int i, int kc, int rc, int x, int xl, in
 // sums for span_finish_kernel (wave_subtree_quads).  UM selects the u
 // update: none, u += alpha p_old, or u = ((u + alpha_h p_h) + alpha_h2
 // p_h2) + alpha p_old (two held updates and this step's, each rounded as
-// when made one by one).
-template <bool DOT2, int UM>
-__global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, double* sub, int nblk) {
-  const int blk = blockIdx.x;
+// when made one by one).  A workgroup of KS x 128 threads takes KS blocks
+// one row apart (block c + S t, t = KS j .. KS j + KS - 1, S = x / 512;
+// blocks past the grid return at once), which share their rows above and
+// below in the CU's L1.
+template <bool DOT2, int UM, int KS>
+__global__ __launch_bounds__(kQuadThreads * KS) void mp_w4_kernel(Geo g, MpArgs a, double* sub,
+                                                                  int nblk, int S) {
+  const int tid = KS == 1 ? (int)threadIdx.x : (int)threadIdx.x % kQuadThreads;
+  int blk = blockIdx.x;
+  if (KS > 1) {
+    blk = (int)(blockIdx.x % S) + S * ((int)(blockIdx.x / S) * KS + (int)threadIdx.x / kQuadThreads);
+    if (blk >= nblk) return;
+  }
   const int lane = threadIdx.x & (kWave - 1);
   const int x = g.x;
   const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
   const float* __restrict__ P = a.p_old;
   const float* __restrict__ R = a.r;
-  const int e = blk * kQuadBlock + 4 * threadIdx.x;
+  const int e = blk * kQuadBlock + 4 * tid;
   const int row = e < g.n ? (int)g.dx.div((uint32_t)e) : 0;
   const int kk = e - row * x;
   bool in[4];
@@ -425,7 +434,7 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
   // each 16-byte load and store covers 1 KB of consecutive cells (32 bytes
   // per lane in the quad layout took two instructions over every other
   // 16 bytes).  The p_old values come from the quad lanes by permute.
-  const int ua = blk * kQuadBlock + (threadIdx.x / kWave) * (4 * kWave) + 2 * lane;
+  const int ua = blk * kQuadBlock + (tid / kWave) * (4 * kWave) + 2 * lane;
   const int ub = ua + 2 * kWave;
   const int rA = ua < g.n ? (int)g.dx.div((uint32_t)ua) : 0;
   const int kA = ua - rA * x;
@@ -570,12 +579,8 @@ __global__ __launch_bounds__(kQuadThreads) void mp_w4_kernel(Geo g, MpArgs a, do
         if (in[c]) ring_store(a.p_new, e + c, kk + c, row, x, xl, xr, yb, yt, pc[c]);
     }
   }
-  wave_subtree_quads<DOT2 ? 2 : 1>(v, sub, nblk, blk);
+  wave_subtree_quads<DOT2 ? 2 : 1>(v, sub, nblk, blk, tid / kWave);
 }
-
-template __global__ void mp_w4_kernel<true, 0>(Geo, MpArgs, double*, int);
-template __global__ void mp_w4_kernel<true, 1>(Geo, MpArgs, double*, int);
-template __global__ void mp_w4_kernel<true, 3>(Geo, MpArgs, double*, int);
 
 
 // r -= alpha w and r.r (with RW also r.w), two cells per lane.
@@ -665,7 +670,7 @@ __global__ __launch_bounds__(kQuadThreads) void mp_r4_kernel(
   for (int q = 0; q < (RW ? 2 : 1); ++q)
     v[q] = (prod[q][0] + prod[q][1]) + (prod[q][2] + prod[q][3]);
   if (any) st4(r, e, in, rn);
-  wave_subtree_quads<RW ? 2 : 1>(v, sub, nblk, blk);
+  wave_subtree_quads<RW ? 2 : 1>(v, sub, nblk, blk, threadIdx.x / kWave);
 }
 
 // MpTrueResidualBody's arithmetic (below) with one cell pair per thread (x
@@ -854,15 +859,32 @@ int launch_mp_w4(tea_hip_chunk* c, MpArgs a, int kid) {
     ++c->hold;
   }
   Timed t(c, kid);
+  // c->stack blocks one row apart per workgroup, as launch_fused_w1 (cg.hip)
+  // does for the FP64 pass.
+  const int S = c->geo.x / kQuadBlock;
+  const int ks = (c->stack == 2 || c->stack == 4) && S >= 1 ? c->stack : 1;
+  const int tn = S >= 1 ? (c->nblk + S - 1) / S : 0;
+  const dim3 grid = ks == 1 ? quad_grid(c) : dim3((unsigned)(S * ((tn + ks - 1) / ks)));
+  const dim3 block(kQuadThreads * ks);
+#define TEA_MP_W4(UMV)                                                                    \
+  do {                                                                                    \
+    if (ks == 1)                                                                          \
+      hipLaunchKernelGGL((mp_w4_kernel<true, UMV, 1>), grid, block, 0, c->stream, c->geo, \
+                         a, c->sub, c->nblk, S);                                          \
+    else if (ks == 2)                                                                     \
+      hipLaunchKernelGGL((mp_w4_kernel<true, UMV, 2>), grid, block, 0, c->stream, c->geo, \
+                         a, c->sub, c->nblk, S);                                          \
+    else                                                                                  \
+      hipLaunchKernelGGL((mp_w4_kernel<true, UMV, 4>), grid, block, 0, c->stream, c->geo, \
+                         a, c->sub, c->nblk, S);                                          \
+  } while (0)
   if (um == 0)
-    hipLaunchKernelGGL((mp_w4_kernel<true, 0>), quad_grid(c), dim3(kQuadThreads), 0,
-                       c->stream, c->geo, a, c->sub, c->nblk);
+    TEA_MP_W4(0);
   else if (um == 1)
-    hipLaunchKernelGGL((mp_w4_kernel<true, 1>), quad_grid(c), dim3(kQuadThreads), 0,
-                       c->stream, c->geo, a, c->sub, c->nblk);
+    TEA_MP_W4(1);
   else
-    hipLaunchKernelGGL((mp_w4_kernel<true, 3>), quad_grid(c), dim3(kQuadThreads), 0,
-                       c->stream, c->geo, a, c->sub, c->nblk);
+    TEA_MP_W4(3);
+#undef TEA_MP_W4
   TEA_HIP_TRY(hipGetLastError());
   return TEA_HIP_OK;
 }

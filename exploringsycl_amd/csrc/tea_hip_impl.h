@@ -339,9 +339,9 @@ constexpr int kSubMaxNr = 3;                        // dot products per launch
 // barrier.
 template <int NR>
 __device__ inline void wave_subtree_pairs(double (&v)[NR], double* sub,
-                                          int nblk, int blk) {
+                                          int nblk, int blk, int wv = -1) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
+  if (wv < 0) wv = threadIdx.x / kWave;
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     // Tree steps 2, 4, 8 and 16 on the DPP path: after each, every lane of
@@ -431,9 +431,9 @@ __device__ inline void st4_nt(float* p, int i, const bool (&in)[4], const float 
 }
 
 template <int NR>
-__device__ inline void wave_subtree_quads(double (&v)[NR], double* sub, int nblk, int blk) {
+__device__ inline void wave_subtree_quads(double (&v)[NR], double* sub, int nblk, int blk,
+                                          int wv) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     double t = v[q];
@@ -603,8 +603,8 @@ template <bool DOT2>
 __global__ void mp_w1_kernel(Geo g, MpArgs a, double* sub, int nblk);
 // UM: 0 leaves u alone, 1 adds alpha p_old, 3 adds alpha_h p_h, alpha_h2
 // p_h2 and then alpha p_old.
-template <bool DOT2, int UM>
-__global__ void mp_w4_kernel(Geo g, MpArgs a, double* sub, int nblk);
+template <bool DOT2, int UM, int KS>
+__global__ void mp_w4_kernel(Geo g, MpArgs a, double* sub, int nblk, int S);
 
 }  // namespace tea
 
@@ -811,6 +811,10 @@ struct tea_hip_chunk {
   // when its scalar is bitwise the device's.
   int ra = 0;                 // kRaNone, kRaR (r-step in flight), kRaWR
   bool ra_enabled = true;     // TEA_HIP_RUN_AHEAD=0 turns it off
+  // Blocks one row apart per workgroup of the four- and two-cells-per-lane
+  // stencil passes (mp_w4_kernel, fused_w1_kernel): 1, 2 (the default) or 4
+  // (TEA_HIP_STACK).
+  int stack = 2;
   bool ra_dirty = false;      // a discarded run-ahead may still be writing ra_host
   int ra_fail = 0;            // consecutive mismatches (3 turn it off)
   int ra_set = 0;             // host slot set of the step in flight
