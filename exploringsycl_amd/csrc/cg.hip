@@ -476,12 +476,9 @@ __global__ __launch_bounds__(kBlock) void fused_w2_kernel(Geo g, FusedWArgs a,
 template <bool APPLY_P, int KS>
 __global__ __launch_bounds__(kBlock * KS) void fused_w1_kernel(Geo g, FusedWArgs a,
                                                                double* sub, int nblk, int S) {
-  const int tid = KS == 1 ? (int)threadIdx.x : (int)threadIdx.x % kBlock;
-  int blk = blockIdx.x;
-  if (KS > 1) {
-    blk = (int)(blockIdx.x % S) + S * ((int)(blockIdx.x / S) * KS + (int)threadIdx.x / kBlock);
-    if (blk >= nblk) return;
-  }
+  int tid;
+  const int blk = stacked_block<KS, kBlock>(S, &tid);
+  if (blk >= nblk) return;
   const int lane = threadIdx.x & (kWave - 1);
   FusedPair s;
   fused_pair_index(g, blk * kPairBlock + 2 * tid, s);
@@ -490,22 +487,15 @@ __global__ __launch_bounds__(kBlock * KS) void fused_w1_kernel(Geo g, FusedWArgs
   wave_subtree_pairs<1>(v, sub, nblk, blk, tid / kWave);
 }
 
-// Launches fused_w1_kernel with c->stack (KS) blocks per workgroup.  With
-// S = x / 512 blocks per row, block c + S t holds the cells one row above
-// block c + S (t - 1) (shifted by x - 512 S cells), so workgroup (c, j)
-// takes blocks c + S t for t = KS j .. KS j + KS - 1: the rows above and
-// below one of its blocks are mostly another's own rows, loaded on the same
-// CU, and come from its L1 rather than as L2 requests.  Each block's cells,
-// lanes and wave sums are unchanged, so the sums are too, bit for bit.  At
-// 4096^2 two blocks per workgroup take the FP64 iteration from 269.2 to
-// 267.6 us and the mixed-precision w-step from 83.2 to 79.0 us (four: 270.6
-// us).
+// Launches fused_w1_kernel with stack_factor(c) blocks one row apart per
+// workgroup (stacked_block, tea_hip_impl.h): their rows above and below come
+// mostly from the CU's L1.  At 4096^2 two blocks per workgroup take the FP64
+// iteration from 269.2 to 267.6 us and the mixed-precision w-step from 83.2
+// to 79.0 us (four: 270.6 us).
 template <bool APPLY_P>
 void launch_fused_w1(tea_hip_chunk* c, const FusedWArgs& a) {
-  const int S = c->geo.x / kPairBlock;
-  const int ks = (c->stack == 2 || c->stack == 4) && S >= 1 ? c->stack : 1;
-  const int tn = S >= 1 ? (c->nblk + S - 1) / S : 0;
-  const dim3 grid = ks == 1 ? pair_grid(c) : dim3((unsigned)(S * ((tn + ks - 1) / ks)));
+  const int S = stack_rows(c), ks = stack_factor(c);
+  const dim3 grid = stacked_grid(c, ks);
   if (ks == 1)
     hipLaunchKernelGGL((fused_w1_kernel<APPLY_P, 1>), grid, dim3(kBlock), 0, c->stream,
                        c->geo, a, c->sub, c->nblk, S);

@@ -399,12 +399,9 @@ __device__ inline void ring_store(T* f, int i, int kc, int rc, int x, int xl, in
 template <bool DOT2, int UM, int KS>
 __global__ __launch_bounds__(kQuadThreads * KS) void mp_w4_kernel(Geo g, MpArgs a, double* sub,
                                                                   int nblk, int S) {
-  const int tid = KS == 1 ? (int)threadIdx.x : (int)threadIdx.x % kQuadThreads;
-  int blk = blockIdx.x;
-  if (KS > 1) {
-    blk = (int)(blockIdx.x % S) + S * ((int)(blockIdx.x / S) * KS + (int)threadIdx.x / kQuadThreads);
-    if (blk >= nblk) return;
-  }
+  int tid;
+  const int blk = stacked_block<KS, kQuadThreads>(S, &tid);
+  if (blk >= nblk) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int x = g.x;
   const int xl = g.h, xr = g.x - g.h - 1, yb = g.h, yt = g.y - g.h - 1;
@@ -859,13 +856,9 @@ int launch_mp_w4(tea_hip_chunk* c, MpArgs a, int kid) {
     ++c->hold;
   }
   Timed t(c, kid);
-  // c->stack blocks one row apart per workgroup, as launch_fused_w1 (cg.hip)
-  // does for the FP64 pass.
-  const int S = c->geo.x / kQuadBlock;
-  const int ks = (c->stack == 2 || c->stack == 4) && S >= 1 ? c->stack : 1;
-  const int tn = S >= 1 ? (c->nblk + S - 1) / S : 0;
-  const dim3 grid = ks == 1 ? quad_grid(c) : dim3((unsigned)(S * ((tn + ks - 1) / ks)));
-  const dim3 block(kQuadThreads * ks);
+  // stack_factor(c) blocks one row apart per workgroup (stacked_block).
+  const int S = stack_rows(c), ks = stack_factor(c);
+  const dim3 grid = stacked_grid(c, ks), block(kQuadThreads * ks);
 #define TEA_MP_W4(UMV)                                                                    \
   do {                                                                                    \
     if (ks == 1)                                                                          \

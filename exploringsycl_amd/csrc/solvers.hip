@@ -255,12 +255,16 @@ __device__ inline void st2_all(double* p, const PairAt& q, double v0,
   if (q.valid) st2_nt(p, q.e, true, true, v0, v1);
 }
 
-__global__ __launch_bounds__(kBlock) void jacobi_pair_kernel(
+template <int KS>
+__global__ __launch_bounds__(kBlock * KS) void jacobi_pair_kernel(
     Geo g, const double* __restrict__ u_old, double* __restrict__ u_new,
     const double* __restrict__ u0, const double* __restrict__ kx,
-    const double* __restrict__ ky, double* sub, int nblk) {
+    const double* __restrict__ ky, double* sub, int nblk, int S) {
+  int tid;
+  const int blk = stacked_block<KS, kBlock>(S, &tid);
+  if (blk >= nblk) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const PairAt q = pair_at(g, blockIdx.x * kPairBlock + 2 * threadIdx.x);
+  const PairAt q = pair_at(g, blk * kPairBlock + 2 * tid);
   const Operand o = load_operand(u_old, q, g.x, lane);
   const Faces f = load_faces(kx, ky, q, g.x, lane);
   const double2 s = ld2_nt(u0, q.ir);
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void jacobi_pair_kernel(
   st2_all(u_new, q, n0, n1);
   double v[1] = {(q.in0 ? fabs(n0 - o.c.x) : 0.0) +
                  (q.in1 ? fabs(n1 - o.c.y) : 0.0)};
-  wave_subtree_pairs<1>(v, sub, nblk, blockIdx.x);
+  wave_subtree_pairs<1>(v, sub, nblk, blk, tid / kWave);
 }
 
 __global__ __launch_bounds__(kBlock) void cheby_pair_kernel(
@@ -995,10 +999,21 @@ int tea_hip_jacobi_iterate(tea_hip_chunk* c, double* error) {
   }
   {
     Timed t(c, K_JACOBI_FUSED);
-    hipLaunchKernelGGL(jacobi_pair_kernel, pair_grid(c), dim3(kBlock), 0,
-                       c->stream, g, f[TEA_FIELD_U], f[TEA_FIELD_R],
-                       f[TEA_FIELD_U0], f[TEA_FIELD_KX], f[TEA_FIELD_KY],
-                       c->sub, c->nblk);
+    // Workgroups of stack_factor(c) blocks one row apart (tea_hip_impl.h).
+    const int ks = stack_factor(c), S = stack_rows(c);
+    const dim3 grid = stacked_grid(c, ks), block(kBlock * ks);
+    if (ks == 1)
+      hipLaunchKernelGGL(jacobi_pair_kernel<1>, grid, block, 0, c->stream, g, f[TEA_FIELD_U],
+                         f[TEA_FIELD_R], f[TEA_FIELD_U0], f[TEA_FIELD_KX], f[TEA_FIELD_KY],
+                         c->sub, c->nblk, S);
+    else if (ks == 2)
+      hipLaunchKernelGGL(jacobi_pair_kernel<2>, grid, block, 0, c->stream, g, f[TEA_FIELD_U],
+                         f[TEA_FIELD_R], f[TEA_FIELD_U0], f[TEA_FIELD_KX], f[TEA_FIELD_KY],
+                         c->sub, c->nblk, S);
+    else
+      hipLaunchKernelGGL(jacobi_pair_kernel<4>, grid, block, 0, c->stream, g, f[TEA_FIELD_U],
+                         f[TEA_FIELD_R], f[TEA_FIELD_U0], f[TEA_FIELD_KX], f[TEA_FIELD_KY],
+                         c->sub, c->nblk, S);
   }
   TEA_HIP_TRY(hipGetLastError());
   // The new u is in r's buffer; the old u becomes r (the reference's copy).

@@ -811,9 +811,10 @@ struct tea_hip_chunk {
   // when its scalar is bitwise the device's.
   int ra = 0;                 // kRaNone, kRaR (r-step in flight), kRaWR
   bool ra_enabled = true;     // TEA_HIP_RUN_AHEAD=0 turns it off
-  // Blocks one row apart per workgroup of the four- and two-cells-per-lane
-  // stencil passes (mp_w4_kernel, fused_w1_kernel): 1, 2 (the default) or 4
-  // (TEA_HIP_STACK).
+  // Blocks one row apart per workgroup of the stencil passes mp_w4_kernel,
+  // fused_w1_kernel and jacobi_pair_kernel: 1, 2 (the default) or 4
+  // (TEA_HIP_STACK).  (fcg_z4_kernel, whose only row above is ky's, took
+  // 57.4 us with two against 56.0 us with one.)
   int stack = 2;
   bool ra_dirty = false;      // a discarded run-ahead may still be writing ra_host
   int ra_fail = 0;            // consecutive mismatches (3 turn it off)
@@ -1021,5 +1022,31 @@ inline dim3 span_grid(const tea_hip_chunk* c) { return dim3(c->nspan); }
 inline dim3 pair_grid(const tea_hip_chunk* c) { return dim3(c->nblk); }
 inline dim3 quad_grid(const tea_hip_chunk* c) { return dim3(c->nblk); }
 inline dim3 flat_grid(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+// Stacked workgroups of the 512-cell-block stencil kernels: with S = x / 512
+// blocks per row, block c + S t holds the cells one row above block c + S
+// (t - 1), so a workgroup of KS blocks takes c + S t for t = KS j .. KS j +
+// KS - 1 and their rows above and below are mostly each other's own rows,
+// loaded on the same CU (L1 hits, not L2 requests).  Each block's cells,
+// lanes and wave sums are those of one block per workgroup.  stack_factor is
+// c->stack where it applies (1, 2 or 4; 1 for rows shorter than a block).
+inline int stack_factor(const tea_hip_chunk* c) {
+  const int S = c->geo.x / kPairBlock;
+  return (c->stack == 2 || c->stack == 4) && S >= 1 ? c->stack : 1;
+}
+inline int stack_rows(const tea_hip_chunk* c) { return c->geo.x / kPairBlock; }
+inline dim3 stacked_grid(const tea_hip_chunk* c, int ks) {
+  if (ks == 1) return dim3(c->nblk);
+  const int S = stack_rows(c), tn = (c->nblk + S - 1) / S;
+  return dim3((unsigned)(S * ((tn + ks - 1) / ks)));
+}
+// The block and thread index of the calling thread in a workgroup of KS
+// blocks of T threads (blk >= nblk for the unused blocks of the last row).
+template <int KS, int T>
+__device__ inline int stacked_block(int S, int* tid) {
+  *tid = KS == 1 ? (int)threadIdx.x : (int)threadIdx.x % T;
+  if (KS == 1) return (int)blockIdx.x;
+  return (int)(blockIdx.x % S) + S * ((int)(blockIdx.x / S) * KS + (int)threadIdx.x / T);
+}
 
 }  // namespace tea

@@ -1,6 +1,8 @@
 """Stacked blocks in the stencil passes: with TEA_HIP_STACK=K a workgroup of
-the four-cells-per-lane mixed-precision w-step (mp_w4_kernel) and of the
-pair-per-thread FP64 CG pass (fused_w1_kernel) takes K blocks one row apart
+the four-cells-per-lane mixed-precision w-step (mp_w4_kernel, also FCG's),
+and of the pair-per-thread FP64 CG pass
+(fused_w1_kernel) and Jacobi sweep (jacobi_pair_kernel, checked against the
+oracle in test_gpu_solvers.py) takes K blocks one row apart
 (block c + S t, S = x / 512), so their rows above and below come from the
 CU's L1.  Every block's cells, lanes and wave sums are unchanged, so the
 iterations must give the same scalars and fields bit for bit with K = 1, 2
@@ -122,6 +124,36 @@ def test_stacked_mixed_precision_w_step(cells, k):
 def test_stacked_fcg_w_step():
     x, y = 600 + 2 * H, 90 + 2 * H
     for a, b in zip(_fcg(x, y, 1), _fcg(x, y, 2)):
+        assert _same(a, b)
+
+
+def _fcg_two_level(x, y, k, iters=8):
+    """FCG with the two-level preconditioner (6 Jacobi sweeps on both coarse
+    levels) between the w-steps."""
+    with _chunk(x, y, k) as c:
+        _fields(c, x, y)
+        from exploringsycl_amd.chunk import check
+        check(c.lib.tea_hip_fcg_attach_sweeps(c.handle, 6, 8, 2), "fcg_attach_sweeps")
+        assert c.fcg_levels() == 2
+        rr, _ = c.fcg_init(2, 0.8, 1.3)
+        zw = c.fcg_precondition(True)
+        s = [rr, zw]
+        alpha = beta = 0.0
+        pu = False
+        for _ in range(iters):
+            pw, pr = c.fcg_calc_w(alpha, pu, beta)
+            alpha = pr / pw
+            rrn, zw = c.fcg_calc_r_precondition(alpha, True)
+            pu = True
+            s += [pw, pr, rrn, zw]
+            beta = -zw / pw
+        s += list(c.mpcg_refresh(alpha, True))
+        return [np.array(s), c.read("u"), c.read_z().astype(np.float64)]
+
+
+def test_stacked_two_level_fcg():
+    x, y = 1024 + 2 * H, 64 + 2 * H
+    for a, b in zip(_fcg_two_level(x, y, 1), _fcg_two_level(x, y, 2)):
         assert _same(a, b)
 
 
